@@ -1,0 +1,92 @@
+"""ctypes binding of libblml_hip.so (include/blml_hip.h).  Loading fails loudly: there is no CPU fallback."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libblml_hip.so')
+
+MAXD, MAXK = 5, 8
+F32, F64 = 0, 1
+FAC_LINE, FAC_GROUP = 0, 1
+STOP_REASONS = ('tol', 'stall', 'll_tol', 'max_sweeps')
+
+
+class Factor(ctypes.Structure):
+    _fields_ = [('kind', ctypes.c_int32), ('naxes', ctypes.c_int32), ('axes', ctypes.c_int32 * MAXD),
+                ('circ', ctypes.c_int32 * MAXD), ('tlen', ctypes.c_int32 * MAXD),
+                ('freq_off', ctypes.c_int64 * MAXD), ('data_off', ctypes.c_int64), ('spec_off', ctypes.c_int64)]
+
+
+class Fit(ctypes.Structure):
+    _fields_ = [('d', ctypes.c_int32), ('K', ctypes.c_int32), ('complex_g', ctypes.c_int32),
+                ('shared', ctypes.c_int32), ('equal_s', ctypes.c_int32), ('B', ctypes.c_int32),
+                ('shape', ctypes.c_int32 * MAXD), ('fac_begin', ctypes.c_int32 * MAXK),
+                ('fac_count', ctypes.c_int32 * MAXK), ('init_K', ctypes.c_int32), ('has_signs', ctypes.c_int32),
+                ('n', ctypes.c_double), ('s', ctypes.c_double * MAXK), ('counts_off', ctypes.c_int64),
+                ('draws_off', ctypes.c_int64), ('init_off', ctypes.c_int64), ('signs_off', ctypes.c_int64),
+                ('out_off', ctypes.c_int64), ('dens_off', ctypes.c_int64), ('hist_off', ctypes.c_int64),
+                ('ws_off', ctypes.c_int64), ('ws_len', ctypes.c_int64), ('scratch_len', ctypes.c_int64)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [('tol', ctypes.c_double), ('max_sweeps', ctypes.c_int64), ('anderson', ctypes.c_int32),
+                ('stall', ctypes.c_int32), ('phase_noise', ctypes.c_double), ('ll_tol', ctypes.c_double),
+                ('ll_window', ctypes.c_int32), ('solver', ctypes.c_int32), ('lbfgs_m', ctypes.c_int32),
+                ('hist_cap', ctypes.c_int32)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [('loglik', ctypes.c_double), ('residual', ctypes.c_double), ('n_sweeps', ctypes.c_int32),
+                ('stop_reason', ctypes.c_int32), ('converged', ctypes.c_int32), ('n_anderson', ctypes.c_int32),
+                ('n_damped', ctypes.c_int32), ('n_gradient', ctypes.c_int32), ('start_sweeps', ctypes.c_int32),
+                ('n_hist', ctypes.c_int32)]
+
+
+RESULT_DTYPE = np.dtype([(name, np.float64 if t is ctypes.c_double else np.int32) for name, t in Result._fields_])
+assert RESULT_DTYPE.itemsize == ctypes.sizeof(Result)
+
+EXPORTS = ('blml_plan_create', 'blml_plan_destroy', 'blml_matvec', 'blml_fit', 'blml_certificate',
+           'blml_plan_info', 'blml_last_error')
+
+_lib = None
+
+
+def load(path=None):
+    """The library handle; raises (ImportError / OSError) when it is not built -- never falls back to the CPU."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(f"libblml_hip.so not built ({p}); run __graft_entry__.build() "
+                          "or python band-limited-ml_amd/build.py")
+    lib = ctypes.CDLL(p)
+    vp, i32, i64, dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+    lib.blml_plan_create.argtypes = [ctypes.POINTER(Fit), i32, ctypes.POINTER(Factor), i32, ctypes.c_void_p, i64,
+                                     ctypes.c_void_p, i64, i32, ctypes.POINTER(Params), ctypes.POINTER(vp)]
+    lib.blml_plan_destroy.argtypes = [vp]
+    lib.blml_matvec.argtypes = [vp, dp, dp, vp]
+    lib.blml_fit.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, vp]
+    lib.blml_certificate.argtypes = [vp, dp, dp, i32, ctypes.c_double, dp, dp, vp]
+    lib.blml_plan_info.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i32),
+                                   ctypes.POINTER(i32)]
+    lib.blml_last_error.restype = ctypes.c_char_p
+    for name in EXPORTS[:-1]:
+        getattr(lib, name).restype = ctypes.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc):
+    """Maps a status code to the reference's exception types (ValueError / RuntimeError / MemoryError)."""
+    if rc == 0:
+        return
+    msg = load().blml_last_error().decode(errors='replace')
+    if rc == 1:
+        raise ValueError(msg)
+    if rc == 3:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)

@@ -1,0 +1,401 @@
+"""BLMLBinned on the MI355X engine: the reference estimator's constructor, fit signature, attributes and errors
+(blml_binned.py:104-518), with every fit computed by libblml_hip.so (one persistent launch per batch of fits).
+
+    m = BLMLBinned(solver='lbfgs', dtype=np.float32).fit(counts, modes, complex_g=True)
+    m.density_, m.Se_, m.e_hat_, m.n_, m.loglik_, m.certificate()
+    fits = fit_many([(counts, modes, dict(complex_g=True)), ...], solver='lbfgs')      # one launch for all
+
+Host work is validation, the model spec, the kernel tables (units.py) and the start's random draws (numpy
+default_rng(seed) in the reference's order, blml_binned.py:429-436); the solve runs on the device.  Device memory
+is allocated through torch (plumbing only); the engine receives raw pointers."""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from .modes import parse_modes
+from .units import circular_dims, mode_factors, next_fast_len5
+
+__all__ = ['BLMLBinned', 'fit_many', 'FitBatch']
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("BLMLBinned(engine='hip') needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+class _FitSpec:
+    """One validated fit: inputs and its factor tables (blml_binned.py:119-154, 383-413)."""
+
+    def __init__(self, counts, modes, h, origin, complex_g, signs, init, seed):
+        if hasattr(counts, 'detach'):
+            counts = counts.detach().cpu().numpy()
+        counts = np.asarray(counts.get() if hasattr(counts, 'get') else counts)
+        if counts.ndim < 1 or np.any(counts < 0) or not np.all(np.isfinite(counts)):
+            raise ValueError("counts: a non-negative finite array on the lattice, one axis per dimension")
+        d = counts.ndim
+        if d > _lib.MAXD:
+            raise ValueError(f"counts: at most {_lib.MAXD} dimensions on this engine")
+        shape = counts.shape
+        specs, _ = parse_modes(modes, d)
+        if len(specs) > _lib.MAXK:
+            raise ValueError(f"at most {_lib.MAXK} modes on this engine")
+        self.h = np.broadcast_to(np.asarray(h, float), (d,)).copy()
+        self.origin = np.broadcast_to(np.asarray(origin, float), (d,)).copy()
+        circ = circular_dims(specs, shape, self.h)
+        self.pad = tuple(m if c else next_fast_len5(2 * m - 1) for m, c in zip(shape, circ))
+        self.counts, self.shape, self.d, self.specs, self.modes = counts.astype(np.float64), shape, d, specs, modes
+        self.K = len(specs)
+        self.B = int(np.prod(shape))
+        self.n = float(self.counts.sum(dtype=np.float64))
+        if not self.n > 0:
+            raise ValueError("counts: no observations")
+        self.complex_g = bool(complex_g)
+        self.psd = self.complex_g or self.K > 1
+        if signs is not None and self.psd:
+            raise ValueError("complex_g / rank > 1 estimate the phases; do not pass signs as well")
+        self.signs = None
+        if signs is not None:
+            if callable(signs):
+                sg = np.sign(signs(self.centres())).reshape(shape)
+            else:
+                sg = np.sign(np.asarray(signs, float))
+            if np.any(sg[self.counts > 0] == 0):
+                raise ValueError("signs: nonzero on every occupied bin")
+            self.signs = np.ascontiguousarray(sg, dtype=np.float64)
+        # one factor chain per distinct kernel (modes with the same kernel share it, blml_binned.py:142-154)
+        key = lambda sp: (tuple(sp['fc']), tuple(sp['kinds']), tuple(sp['periods']))
+        chains, self.mode_chain, s = {}, [], []
+        for sp in specs:
+            k = key(sp)
+            if k not in chains:
+                chains[k] = (len(chains),) + mode_factors(sp, shape, self.h, circ)
+            self.mode_chain.append(chains[k][0])
+            s.append(chains[k][1])
+        self.chains = [v[2] for v in sorted(chains.values(), key=lambda v: v[0])]
+        self.s = np.array(s)
+        self.shared = len(chains) == 1
+        self.equal_s = bool(np.allclose(self.s, self.s[0]))
+        # warm start (blml_binned.py:403-408) and the start's draws in the reference's order
+        self.init = None
+        kk = 0
+        if init is not None and getattr(init, 'e_hat_', None) is not None and tuple(init.shape_) == tuple(shape):
+            E0 = np.asarray(init.e_hat_)
+            kk = min(self.K, E0.shape[0])
+            e = np.zeros((kk, self.B, 2))
+            e[:, :, 0] = np.real(E0[:kk]).reshape(kk, -1)
+            e[:, :, 1] = np.imag(E0[:kk]).reshape(kk, -1) if np.iscomplexobj(E0) else 0.0
+            self.init = e
+        self.init_K = kk
+        rng = np.random.default_rng(seed)
+        draws = []
+        if self.init is None:
+            if self.psd:
+                if self.complex_g:
+                    draws.append(rng.standard_normal(shape).ravel())
+                for _ in range(1, self.K):
+                    draws.append(rng.standard_normal(shape).ravel())
+                    if self.complex_g:
+                        draws.append(rng.standard_normal(shape).ravel())
+        else:
+            for _ in range(kk, self.K):
+                draws.append(rng.standard_normal(shape).ravel())
+                if self.complex_g:
+                    draws.append(rng.standard_normal(shape).ravel())
+        self.draws = np.concatenate(draws) if draws else np.zeros(1)
+
+    def centres(self):
+        g = np.meshgrid(*[self.origin[j] + self.h[j] * np.arange(m) for j, m in enumerate(self.shape)],
+                        indexing='ij')
+        return np.stack([x.ravel() for x in g], 1)
+
+    @property
+    def nc(self):
+        return 2 if self.complex_g else 1
+
+
+class FitBatch:
+    """A plan for a batch of fits and its device buffers.  build() uploads everything; run() is one persistent
+    launch (asynchronous on the current stream); results() copies the outputs back."""
+
+    def __init__(self, fitspecs, params, dtype, hist_cap=4096):
+        self.fits = list(fitspecs)
+        self.params = params
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.float32, np.float64):
+            raise ValueError("dtype: np.float64 (the plain FP64 method) or np.float32 (FP32 transforms, FP64 "
+                             "reductions)")
+        self.code = _lib.F32 if self.dtype == np.float32 else _lib.F64
+        self.hist_cap = int(hist_cap)
+        self.plan = None
+        self._build_tables()
+
+    def _build_tables(self):
+        nf = len(self.fits)
+        Fits = (_lib.Fit * nf)()
+        fac_list, fdata, freqs = [], [], []
+        nd = nfq = 0
+        off = dict(counts=0, draws=0, init=0, signs=0, out=0, dens=0, hist=0)
+        for i, fs in enumerate(self.fits):
+            F = Fits[i]
+            F.d, F.K, F.complex_g, F.shared, F.equal_s, F.B = fs.d, fs.K, int(fs.complex_g), int(fs.shared), \
+                int(fs.equal_s), fs.B
+            for j in range(_lib.MAXD):
+                F.shape[j] = fs.shape[j] if j < fs.d else 1
+            chain_begin = []
+            for chain in fs.chains:
+                chain_begin.append(len(fac_list))
+                for fa in chain:
+                    C = _lib.Factor()
+                    C.kind = _lib.FAC_LINE if fa['kind'] == 'line' else _lib.FAC_GROUP
+                    C.naxes = len(fa['axes'])
+                    for a in range(C.naxes):
+                        C.axes[a], C.circ[a], C.tlen[a] = fa['axes'][a], fa['circ'][a], fa['tlen'][a]
+                        if fa['freqs'][a] is not None:
+                            C.freq_off[a] = nfq
+                            freqs.append(np.asarray(fa['freqs'][a], np.int32))
+                            nfq += len(fa['freqs'][a])
+                    C.data_off = nd
+                    fdata.append(np.asarray(fa['data'], np.float64))
+                    nd += len(fa['data'])
+                    fac_list.append(C)
+            for k in range(fs.K):
+                ch = fs.mode_chain[k]
+                F.fac_begin[k] = chain_begin[ch]
+                F.fac_count[k] = len(fs.chains[ch])
+                F.s[k] = fs.s[k]
+            F.init_K = fs.init_K
+            F.has_signs = int(fs.signs is not None)
+            F.n = fs.n
+            F.counts_off = off['counts']; off['counts'] += fs.B
+            F.draws_off = off['draws']; off['draws'] += len(fs.draws)
+            F.init_off = off['init']; off['init'] += 0 if fs.init is None else fs.init.size
+            F.signs_off = off['signs']; off['signs'] += 0 if fs.signs is None else fs.B
+            F.out_off = off['out']; off['out'] += fs.K * fs.B * fs.nc
+            F.dens_off = off['dens']; off['dens'] += fs.B
+            F.hist_off = off['hist']; off['hist'] += self.hist_cap
+        self._Fits = Fits
+        Facs = (_lib.Factor * max(1, len(fac_list)))(*fac_list)
+        self._Facs, self._nfacs = Facs, len(fac_list)
+        self._fdata = np.ascontiguousarray(np.concatenate(fdata) if fdata else np.zeros(1))
+        self._freqs = np.ascontiguousarray(np.concatenate(freqs) if freqs else np.zeros(1, np.int32))
+        self.sizes = off
+        self.counts = np.concatenate([fs.counts.ravel() for fs in self.fits])
+        self.draws = np.concatenate([fs.draws for fs in self.fits])
+        inits = [fs.init.ravel() for fs in self.fits if fs.init is not None]
+        self.inits = np.concatenate(inits) if inits else None
+        signs = [fs.signs.ravel() for fs in self.fits if fs.signs is not None]
+        self.signs = np.concatenate(signs) if signs else None
+
+    def build(self):
+        """Creates the plan (spectra on the device, workspace) and uploads the inputs."""
+        torch = _torch()
+        lib = _lib.load()
+        P = self.params
+        prm = _lib.Params()
+        prm.tol = float(P['tol'])
+        prm.max_sweeps = -1 if P['max_sweeps'] is None else int(P['max_sweeps'])
+        prm.anderson = int(P['anderson'] or 0)
+        prm.stall = int(P['stall'])
+        prm.phase_noise = float(P['phase_noise'])
+        prm.ll_tol = float('nan') if P['ll_tol'] is None else float(P['ll_tol'])
+        prm.ll_window = int(P['ll_window'])
+        prm.solver = 1 if P['solver'] == 'lbfgs' else 0
+        prm.lbfgs_m = int(P['lbfgs_m'])
+        prm.hist_cap = self.hist_cap
+        self._prm = prm
+        plan = ctypes.c_void_p()
+        _lib.check(lib.blml_plan_create(self._Fits, len(self.fits), self._Facs, self._nfacs,
+                                        self._fdata.ctypes.data, len(self._fdata), self._freqs.ctypes.data,
+                                        len(self._freqs), self.code, ctypes.byref(prm), ctypes.byref(plan)))
+        self.plan = plan
+        dev = 'cuda'
+        tdt = torch.float32 if self.code == _lib.F32 else torch.float64
+        f64 = torch.float64
+        self.d_counts = torch.from_numpy(self.counts).to(dev)
+        self.d_draws = torch.from_numpy(self.draws).to(dev)
+        self.d_init = None if self.inits is None else torch.from_numpy(self.inits).to(dev)
+        self.d_signs = None if self.signs is None else torch.from_numpy(self.signs).to(dev)
+        nout = max(1, self.sizes['out'])
+        self.d_e = torch.zeros(nout, dtype=tdt, device=dev)
+        self.d_se = torch.zeros(nout, dtype=tdt, device=dev)
+        self.d_dens = torch.zeros(max(1, self.sizes['dens']), dtype=f64, device=dev)
+        self.d_res = torch.zeros(len(self.fits) * _lib.RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        self.d_mh = torch.zeros(max(1, self.sizes['hist']), dtype=f64, device=dev)
+        self.d_rh = torch.zeros(max(1, self.sizes['hist']), dtype=f64, device=dev)
+        return self
+
+    def info(self):
+        lib = _lib.load()
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        c, d = ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(lib.blml_plan_info(self.plan, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
+        return dict(out_elems=a.value, ws_bytes=b.value, lds_bytes=c.value, grid=d.value)
+
+    def stream(self):
+        return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+    def run(self):
+        """One launch that fits every fit of the batch (asynchronous)."""
+        lib = _lib.load()
+        ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+        _lib.check(lib.blml_fit(self.plan, ptr(self.d_counts), ptr(self.d_draws), ptr(self.d_init),
+                                ptr(self.d_signs), ptr(self.d_e), ptr(self.d_se), ptr(self.d_dens), ptr(self.d_res),
+                                ptr(self.d_mh), ptr(self.d_rh), self.stream()))
+        return self
+
+    def matvec(self, E):
+        """U = _Sall(E) for every fit: E a device tensor laid out like the outputs (plan dtype)."""
+        torch = _torch()
+        U = torch.empty_like(E)
+        _lib.check(_lib.load().blml_matvec(self.plan, ctypes.c_void_p(E.data_ptr()), ctypes.c_void_p(U.data_ptr()),
+                                           self.stream()))
+        return U
+
+    def certificate(self, max_iter=20000, tol=1e-12):
+        torch = _torch()
+        lam = torch.zeros(len(self.fits), dtype=torch.float64, device='cuda')
+        its = torch.zeros(len(self.fits), dtype=torch.int32, device='cuda')
+        _lib.check(_lib.load().blml_certificate(self.plan, ctypes.c_void_p(self.d_e.data_ptr()),
+                                                ctypes.c_void_p(self.d_counts.data_ptr()), int(max_iter), float(tol),
+                                                ctypes.c_void_p(lam.data_ptr()), ctypes.c_void_p(its.data_ptr()),
+                                                self.stream()))
+        return lam.cpu().numpy(), its.cpu().numpy()
+
+    def results(self):
+        """Per-fit host outputs: list of dicts with e_hat, Se, density and the scalar attributes."""
+        torch = _torch()
+        torch.cuda.synchronize()
+        e = self.d_e.cpu().numpy()
+        se = self.d_se.cpu().numpy()
+        dens = self.d_dens.cpu().numpy()
+        res = np.frombuffer(self.d_res.cpu().numpy().tobytes(), dtype=_lib.RESULT_DTYPE)
+        mh = self.d_mh.cpu().numpy()
+        rh = self.d_rh.cpu().numpy()
+        out = []
+        for i, fs in enumerate(self.fits):
+            F = self._Fits[i]
+            V = fs.K * fs.B * fs.nc
+            ev = e[F.out_off:F.out_off + V].reshape(fs.K, fs.B, fs.nc)
+            sv = se[F.out_off:F.out_off + V].reshape(fs.K, fs.B, fs.nc)
+            if fs.complex_g:
+                cdt = np.complex64 if self.code == _lib.F32 else np.complex128
+                ev = (ev[..., 0] + 1j * ev[..., 1]).astype(cdt)
+                sv = (sv[..., 0] + 1j * sv[..., 1]).astype(cdt)
+            else:
+                ev, sv = ev[..., 0], sv[..., 0]
+            r = res[i]
+            nh = min(int(r['n_hist']), self.hist_cap)
+            out.append(dict(e_hat=ev.reshape((fs.K,) + fs.shape), Se=sv.reshape((fs.K,) + fs.shape),
+                            density=dens[F.dens_off:F.dens_off + fs.B].reshape(fs.shape),
+                            loglik=float(r['loglik']), residual=float(r['residual']), n_sweeps=int(r['n_sweeps']),
+                            stop_reason=_lib.STOP_REASONS[int(r['stop_reason'])], converged=bool(r['converged']),
+                            n_anderson=int(r['n_anderson']), n_damped=int(r['n_damped']),
+                            n_gradient=int(r['n_gradient']), start_sweeps=int(r['start_sweeps']),
+                            merit_history=mh[F.hist_off:F.hist_off + nh].tolist(),
+                            rho_history=rh[F.hist_off:F.hist_off + nh].tolist()))
+        return out
+
+    def close(self):
+        if self.plan is not None:
+            _lib.load().blml_plan_destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BLMLBinned:
+    """BLML on counts on a full regular lattice, fitted on the MI355X (blml_binned.py:104-518).  engine: 'hip'
+    (alias 'gpu'); dtype np.float64 keeps the plain FP64 method, np.float32 runs FP32 transforms with FP64
+    reductions.  workers and verbose are accepted for compatibility (the device has no host threads to set)."""
+
+    def __init__(self, tol=1e-5, max_sweeps=None, anderson=5, engine='hip', dtype=np.float64, phase_noise=0.1,
+                 seed=0, workers=-1, verbose=False, stall=1000, ll_tol=None, ll_window=100, solver='cd', lbfgs_m=20):
+        if solver not in ('cd', 'lbfgs'):
+            raise ValueError("solver: 'cd' (coordinate / Jacobi updates with Anderson mixing) or 'lbfgs'")
+        if engine not in ('hip', 'gpu'):
+            raise ValueError("engine: 'hip' (alias 'gpu'): this estimator runs on the MI355X engine only")
+        self.solver, self.lbfgs_m = solver, int(lbfgs_m)
+        self.tol, self.max_sweeps, self.anderson = tol, max_sweeps, anderson
+        self.engine, self.dtype, self.phase_noise, self.seed = engine, np.dtype(dtype), phase_noise, seed
+        self.workers, self.verbose, self.stall = workers, verbose, stall
+        self.ll_tol, self.ll_window = ll_tol, ll_window
+
+    def _params(self):
+        return dict(tol=self.tol, max_sweeps=self.max_sweeps, anderson=self.anderson, stall=self.stall,
+                    phase_noise=self.phase_noise, ll_tol=self.ll_tol, ll_window=self.ll_window, solver=self.solver,
+                    lbfgs_m=self.lbfgs_m)
+
+    def fit(self, counts, modes, h=1.0, origin=0.0, complex_g=False, signs=None, init=None):
+        fs = _FitSpec(counts, modes, h, origin, complex_g, signs, init, self.seed)
+        batch = FitBatch([fs], self._params(), self.dtype).build().run()
+        self._adopt(fs, batch, batch.results()[0], 0)
+        return self
+
+    def _adopt(self, fs, batch, r, index):
+        self._batch, self._index = batch, index
+        self.h_, self.origin_, self.pad_, self.shape_, self.d_ = fs.h, fs.origin, fs.pad, fs.shape, fs.d
+        self.specs_, self.modes_, self.K_, self.shared_ = fs.specs, fs.modes, fs.K, fs.shared
+        self.s_ = fs.s
+        self.rdt_ = np.float32 if self.dtype == np.float32 else np.float64
+        self.cdt_ = np.complex64 if self.dtype == np.float32 else np.complex128
+        self.w_ = fs.counts.astype(self.rdt_)
+        self.n_ = fs.n
+        self.c_ = self.n_ * self.w_
+        self.occ_ = fs.counts > 0
+        self.complex_g_, self.psd_ = fs.complex_g, fs.psd
+        self.signs_ = fs.signs
+        self.e_hat_, self.Se_, self.density_ = r['e_hat'], r['Se'], r['density']
+        self.loglik_ = r['loglik']
+        self.n_samples_ = self.n_
+        self.n_sweeps_, self.residual_, self.converged_ = r['n_sweeps'], r['residual'], r['converged']
+        self.stop_reason_ = r['stop_reason']
+        self.merit_history_, self.rho_history_ = r['merit_history'], r['rho_history']
+        self.n_anderson_, self.n_damped_, self.n_gradient_ = r['n_anderson'], r['n_damped'], r['n_gradient']
+        self.start_sweeps_ = r['start_sweeps']
+        return self
+
+    def centres(self):
+        g = np.meshgrid(*[self.origin_[j] + self.h_[j] * np.arange(m) for j, m in enumerate(self.shape_)],
+                        indexing='ij')
+        return np.stack([x.ravel() for x in g], 1)
+
+    def certificate(self, k=1, tol=1e-6):
+        """lambda_max(D^1/2 S D^1/2) / n (shared kernel only, blml_binned.py:501-518), by power iteration on the
+        device; the iteration stops when the FP64 Rayleigh quotient changes by less than tol * 1e-6 (relative)."""
+        if not self.shared_:
+            raise ValueError("certificate: shared kernel only (per-mode kernels: no convex lift)")
+        if k != 1:
+            raise ValueError("certificate: k = 1 (the largest eigenvalue) on this engine")
+        lam, its = self._batch.certificate(max_iter=100000, tol=tol * 1e-6)
+        self.certificate_iters_ = int(its[self._index])
+        return float(lam[self._index])
+
+
+def fit_many(jobs, tol=1e-5, max_sweeps=None, anderson=5, dtype=np.float64, phase_noise=0.1, seed=0, stall=1000,
+             ll_tol=None, ll_window=100, solver='cd', lbfgs_m=20, hist_cap=4096):
+    """Fits many independent lattices in one persistent launch.  jobs: iterable of (counts, modes, fit kwargs)
+    with the keywords of BLMLBinned.fit.  Returns one fitted BLMLBinned per job (same attributes)."""
+    est = BLMLBinned(tol=tol, max_sweeps=max_sweeps, anderson=anderson, dtype=dtype, phase_noise=phase_noise,
+                     seed=seed, stall=stall, ll_tol=ll_tol, ll_window=ll_window, solver=solver, lbfgs_m=lbfgs_m)
+    specs = []
+    for counts, modes, kw in jobs:
+        kw = dict(kw)
+        specs.append(_FitSpec(counts, modes, kw.pop('h', 1.0), kw.pop('origin', 0.0), kw.pop('complex_g', False),
+                              kw.pop('signs', None), kw.pop('init', None), seed))
+        if kw:
+            raise TypeError(f"unknown fit keywords {sorted(kw)}")
+    batch = FitBatch(specs, est._params(), dtype, hist_cap=hist_cap).build().run()
+    out = []
+    for i, (fs, r) in enumerate(zip(specs, batch.results())):
+        m = BLMLBinned(tol=tol, max_sweeps=max_sweeps, anderson=anderson, dtype=dtype, phase_noise=phase_noise,
+                       seed=seed, stall=stall, ll_tol=ll_tol, ll_window=ll_window, solver=solver, lbfgs_m=lbfgs_m)
+        out.append(m._adopt(fs, batch, r, i))
+    return out

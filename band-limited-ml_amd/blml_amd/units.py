@@ -1,0 +1,187 @@
+"""Kernel units as lattice operators: the host side of plan creation.
+
+Each mode's kernel is a product of units (blml_trivial.py:330-335).  On the fit's lattice every unit becomes one
+factor of the engine (include/blml_hip.h, blml_factor_t):
+  * a one-column unit on a padded axis: its even row K(o h), o = 0..m-1 (sinc blml_trivial.py:368-370, slits
+    :442-444, dirichlet :509-515), convolved by a power-of-two FFT of length >= 2m - 1;
+  * a periodic unit on circular axes (dirichlet :501-515, pball :642-665): its spectrum is a harmonic indicator,
+    given as the band's frequencies per axis and (count of harmonics) / prod P on their product grid;
+  * a multi-column unit on padded axes (ball :582-591, or pball whose period is not the box): its table on the
+    power-of-two lattice, masked to |off| <= m - 1 as the reference's _offsets (blml_binned.py:63-72).
+All values are computed here in FP64 from the kernel formulas; the device turns rows and tables into spectra."""
+import numpy as np
+
+from .modes import split_kind, units_of
+
+PERIODIC = ('dirichlet', 'pball')
+
+
+def pow2_at_least(n):
+    p = 2
+    while p < n:
+        p <<= 1
+    return p
+
+
+def next_fast_len5(n):
+    """Smallest 2^a 3^b 5^c >= n: the reference's pad per padded dim (blml_binned.py:134, 1533-1546)."""
+    best = pow2_at_least(n)
+    a = 1
+    while a <= best:
+        b = a
+        while b <= best:
+            c = b
+            while c < n:
+                c *= 5
+            best = min(best, c)
+            b *= 3
+        a *= 2
+    return best
+
+
+def dirichlet_N(fc, P):
+    return max(0, int(np.floor(fc * P / 2.0 + 1e-9)))
+
+
+def dirichlet_Mi(fc, P, fill):
+    if fill is None or fill >= 1.0:
+        return 0
+    return max(2 * (int(np.ceil((1.0 - fill) * fc * P / 2.0 - 1e-9)) - 1) + 1, 0)
+
+
+def pball_harmonics(fc, periods):
+    """(H, D) integer harmonics with sum (k_i / P_i)^2 <= (fc / 2)^2, and 1 / prod P (blml_trivial.py:632-639)."""
+    P = np.asarray(periods, float)
+    R = float(fc) / 2.0
+    N = [max(0, int(np.floor(R * p + 1e-9))) for p in P]
+    axes = [np.arange(-v, v + 1) for v in N]
+    grid = np.stack([g.ravel() for g in np.meshgrid(*axes, indexing='ij')], 1)
+    keep = ((grid / P) ** 2).sum(1) <= R * R * (1 + 1e-9) + 1e-12
+    return grid[keep], 1.0 / float(np.prod(P))
+
+
+def one_column_values(kind, fc, period, t):
+    """K of a one-column unit at coordinate offsets t."""
+    name, tag = split_kind(kind)
+    t = np.asarray(t, float)
+    if name == 'sinc':
+        return fc * np.sinc(fc * t)
+    if name == 'slits':
+        fi = (1.0 - float(tag)) * fc
+        return fc * np.sinc(fc * t) - fi * np.sinc(fi * t)
+    if name == 'dirichlet':
+        fill = float(tag) if tag else None
+        P = float(period)
+        M = 2 * dirichlet_N(fc, P) + 1
+        Mi = dirichlet_Mi(fc, P, fill)
+        u = np.mod(np.abs(t), P)
+        u = np.minimum(u, P - u)
+        with np.errstate(divide='ignore', invalid='ignore'):
+            v = (np.sin(np.pi * M * u / P) - np.sin(np.pi * Mi * u / P)) / np.sin(np.pi * u / P)
+        return (1.0 / P) * np.where(u < 1e-10, float(M - Mi), v)
+    raise ValueError(f"not a one-column kind: {kind!r}")
+
+
+def ball_profile(s, D):
+    """FT of the ball |nu| <= 1/2 in D dims (blml_trivial.py:135-143)."""
+    from scipy.special import gamma, jv
+    s = np.asarray(s, float)
+    out = np.full(s.shape, np.pi ** (D / 2) * 0.5 ** D / gamma(D / 2 + 1))
+    nz = s > 0
+    out[nz] = 0.5 ** (D / 2) * jv(D / 2, np.pi * s[nz]) / s[nz] ** (D / 2)
+    return out
+
+
+def multi_values(name, fcs, periods, diffs):
+    """K of a multi-column unit (ball / pball) at per-column offsets."""
+    if name == 'ball':
+        s = np.sqrt(sum((f * dk) ** 2 for f, dk in zip(fcs, diffs)))
+        return float(np.prod(fcs)) * ball_profile(s, len(fcs))
+    H, sc = pball_harmonics(fcs[0], periods)
+    acc = 0.0 * sum(diffs)
+    for k in H:
+        ph = 0.0
+        for i, P in enumerate(periods):
+            if k[i]:
+                ph = ph + (2 * np.pi * k[i] / P) * diffs[i]
+        acc = acc + np.cos(ph)
+    return sc * acc
+
+
+def unit_at_zero(kind, dims, spec):
+    name = split_kind(kind)[0]
+    fc = np.asarray(spec['fc'], float)
+    if name in ('ball', 'pball'):
+        per = [spec['periods'][j] for j in dims]
+        return float(multi_values(name, fc[dims], per, [np.zeros(1)] * len(dims))[0])
+    return float(one_column_values(kind, float(fc[dims[0]]), spec['periods'][dims[0]], np.zeros(1))[0])
+
+
+def circular_dims(specs, shape, h):
+    """Per dim: circular (pad = m) iff every mode is periodic there with period m h (blml_binned.py:130-135)."""
+    out = []
+    for j, m in enumerate(shape):
+        out.append(all(split_kind(sp['kinds'][j])[0] in PERIODIC and sp['periods'][j] is not None and
+                       abs(m * h[j] - sp['periods'][j]) <= 1e-9 * sp['periods'][j] for sp in specs))
+    return out
+
+
+def mode_factors(spec, shape, h, circ):
+    """(s = K(0), [factor dicts]) of one mode.  A factor dict: kind ('line' | 'group'), axes, circ per axis, tlen
+    per axis, freqs per circular axis (int arrays), data (float64 array)."""
+    d = len(shape)
+    s = 1.0
+    facs = []
+    for kind, dims in units_of(spec['kinds']):
+        name = split_kind(kind)[0]
+        s = s * unit_at_zero(kind, dims, spec)
+        cflags = [bool(circ[j]) for j in dims]
+        if any(cflags) and not all(cflags):
+            raise ValueError(f"unit {kind!r} over dims {dims}: some of its dims are circular (period = box) and some "
+                             "are not; the engine needs all or none")
+        fc = np.asarray(spec['fc'], float)
+        if len(dims) == 1 and name not in ('ball', 'pball'):
+            j = dims[0]
+            m = shape[j]
+            if cflags[0]:
+                P = float(spec['periods'][j])
+                N = dirichlet_N(fc[j], P)
+                Mi = dirichlet_Mi(fc[j], P, float(split_kind(kind)[1]) if split_kind(kind)[1] else None)
+                lo = (Mi + 1) // 2 if Mi else 0          # |k| >= lo kept (Mi = 2 Ni + 1 removed)
+                ks = np.array([k for k in range(-N, N + 1) if abs(k) >= lo], dtype=np.int64)
+                F, cnt = np.unique(np.mod(ks, m), return_counts=True)
+                facs.append(dict(kind='line', axes=[j], circ=[1], tlen=[len(F)], freqs=[F.astype(np.int32)],
+                                 data=cnt.astype(float) / P))
+            else:
+                L = pow2_at_least(2 * m - 1)
+                row = one_column_values(kind, fc[j], spec['periods'][j], np.arange(m) * h[j])
+                facs.append(dict(kind='line', axes=[j], circ=[0], tlen=[L], freqs=[None], data=row.astype(float)))
+            continue
+        per = [spec['periods'][j] for j in dims]
+        if all(cflags):
+            if name != 'pball':
+                raise ValueError(f"unit {kind!r}: circular dims need a periodic kind")
+            H, sc = pball_harmonics(fc[dims[0]], per)
+            red = np.stack([np.mod(H[:, a], shape[j]) for a, j in enumerate(dims)], 1)
+            Fs = [np.unique(red[:, a]) for a in range(len(dims))]
+            idx = [np.searchsorted(Fs[a], red[:, a]) for a in range(len(dims))]
+            dense = np.zeros([len(F) for F in Fs])
+            np.add.at(dense, tuple(idx), 1.0)
+            facs.append(dict(kind='group', axes=list(dims), circ=[1] * len(dims), tlen=[len(F) for F in Fs],
+                             freqs=[F.astype(np.int32) for F in Fs], data=(dense * sc).ravel()))
+        else:
+            Ls = [pow2_at_least(2 * shape[j] - 1) for j in dims]
+            offs = []
+            for a, j in enumerate(dims):
+                L = Ls[a]
+                i = np.arange(L)
+                off = np.where(i < (L + 1) // 2, i, i - L).astype(float)
+                offs.append((off, np.abs(off) <= shape[j] - 1))
+            grids = np.meshgrid(*[o[0] * h[j] for o, j in zip(offs, dims)], indexing='ij')
+            masks = np.meshgrid(*[o[1] for o in offs], indexing='ij')
+            T = multi_values(name, fc[dims], per, grids)
+            for mk in masks:
+                T = T * mk
+            facs.append(dict(kind='group', axes=list(dims), circ=[0] * len(dims), tlen=Ls, freqs=[None] * len(dims),
+                             data=np.ascontiguousarray(T, dtype=float).ravel()))
+    return s, facs

@@ -1,0 +1,746 @@
+// libblml_hip.so: kernels and C-ABI of the binned-fit engine (declarations and contracts in include/blml_hip.h).
+//
+// Kernels
+//   spec_kernel      one workgroup per factor: the spectrum of every kernel unit (plan creation; replaces
+//                    _ModeKernel._table / _ffts, blml_binned.py:79-90)
+//   matvec_kernel    one workgroup per fit: U = [S_k E_k]_k (_Sall, blml_binned.py:156-168)
+//   fit_kernel       persistent: each workgroup takes fits from a queue (longest first) and runs the whole fit
+//                    (blml_binned.py:383-445) with the solvers of solver.hpp
+//   cert_kernel      one workgroup per fit: power iteration for certificate() (blml_binned.py:501-518)
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "blml_hip.h"
+#include "solver.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHECK(x)                                                                              \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) return fail(BLML_ERUNTIME, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kStaticLds = (int)sizeof(SolverLds) + 64;
+constexpr int kMaxLds = 160 * 1024;
+
+// ------------------------------------------------------------------------------------------------ kernels
+template <typename T>
+struct DevPlan {
+    const blml_fit_t *fits;
+    const blml_factor_t *facs;
+    const double *fdata;
+    const int *freqs;
+    T *spec;
+    const cpx<T> *tw;
+    int Lmax;
+    T *ws;
+    int nfits;
+    blml_params_t params;
+};
+
+template <typename T>
+__global__ void twiddle_kernel(cpx<T> *tw, int Lmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < Lmax / 2) {
+        double sn, cs;
+        sincospi(-2.0 * (double)i / (double)Lmax, &sn, &cs);
+        tw[i] = {(T)cs, (T)sn};
+    }
+}
+
+// Spectrum of one factor.  LINE padded: Kf[q] = (t0 + 2 sum_{o=1}^{m-1} t_o cos(2 pi q o / L)) / L, the DFT of the
+// even, zero-padded row, in FP64 with exact integer phase reduction.  Circular factors: the given values.
+// GROUP padded: the forward FFT of the table over its power-of-two lattice (real, the table being even), times
+// prod 1/L.
+template <typename T>
+__global__ void __launch_bounds__(BLML_NT) spec_kernel(DevPlan<T> p, int nfacs, const int *fac_m, cpx<T> *tmp,
+                                                       const long *tmp_off) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int fi = blockIdx.x;
+    if (fi >= nfacs) return;
+    const blml_factor_t fa = p.facs[fi];
+    T *spec = p.spec + fa.spec_off;
+    const double *data = p.fdata + fa.data_off;
+    if (fa.kind == BLML_FAC_LINE && !fa.circ[0]) {
+        const int m = fac_m[fi * BLML_MAXD], L = fa.tlen[0];
+        for (int q = threadIdx.x; q < L; q += BLML_NT) {
+            double acc = data[0];
+            long ph = 0;
+            for (int o = 1; o < m; ++o) {
+                ph += q;
+                if (ph >= L) ph -= L;
+                acc += 2.0 * data[o] * cospi(2.0 * (double)ph / (double)L);
+            }
+            spec[q] = (T)(acc / (double)L);
+        }
+        return;
+    }
+    long total = 1;
+    for (int a = 0; a < fa.naxes; ++a) total *= fa.tlen[a];
+    bool allcirc = true;
+    for (int a = 0; a < fa.naxes; ++a) allcirc = allcirc && fa.circ[a];
+    if (allcirc) {
+        for (long i = threadIdx.x; i < total; i += BLML_NT) spec[i] = (T)data[i];
+        return;
+    }
+    // padded group: FFT of the table along each of its axes (the table's own lattice: ext = tlen)
+    cpx<T> *X = tmp + tmp_off[fi], *Y = X + total;
+    for (long i = threadIdx.x; i < total; i += BLML_NT) X[i] = {(T)data[i], (T)0};
+    __syncthreads();
+    int ext[BLML_MAXD];
+    for (int a = 0; a < fa.naxes; ++a) ext[a] = fa.tlen[a];
+    cpx<T> *cur = X, *oth = Y;
+    for (int a = fa.naxes - 1; a >= 0; --a) {
+        Pass<T> ps;
+        ps.op = OP_FFT_FULL;
+        ps.L = fa.tlen[a];
+        ps.logL = ilog2(ps.L);
+        ps.m = ps.L;
+        ps.nF = 0;
+        ps.F = nullptr;
+        ps.spec = nullptr;
+        ps.n_in = ps.n_out = ps.L;
+        line_geom<T>(ext, fa.naxes, a, ps.outer, ps.inner);
+        run_pass<T>(ps, cur, oth, (cpx<T> *)lds_raw, p.tw, p.Lmax);
+        cpx<T> *t = cur; cur = oth; oth = t;
+    }
+    double inv = 1.0;
+    for (int a = 0; a < fa.naxes; ++a) inv /= (double)fa.tlen[a];
+    for (long i = threadIdx.x; i < total; i += BLML_NT) spec[i] = (T)((double)cur[i].x * inv);
+}
+
+template <typename T>
+__device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, MatvecCtx<T> &mv, void *lds) {
+    mv.d = f.d;
+    for (int j = 0; j < BLML_MAXD; ++j) mv.shape[j] = j < f.d ? f.shape[j] : 1;
+    mv.facs = p.facs;
+    mv.spec = p.spec;
+    mv.freqs = p.freqs;
+    mv.tw = p.tw;
+    mv.Lmax = p.Lmax;
+    T *ws = p.ws + f.ws_off;
+    mv.X = (cpx<T> *)ws;
+    mv.Y = mv.X + f.scratch_len;
+    mv.lds = (cpx<T> *)lds;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLML_NT) matvec_kernel(DevPlan<T> p, const T *E, T *U) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int fi = blockIdx.x;
+    if (fi >= p.nfits) return;
+    const blml_fit_t f = p.fits[fi];
+    MatvecCtx<T> mv;
+    make_matvec_ctx<T>(p, f, mv, lds_raw);
+    const int nc = f.complex_g ? 2 : 1;
+    for (int k = 0; k < f.K; ++k) {
+        const long off = f.out_off + (long)k * f.B * nc;
+        matvec_mode<T>(mv, f.fac_begin[k], f.fac_count[k], E + off, U + off, f.B, nc);
+    }
+}
+
+template <typename T>
+__device__ int n_slots(const blml_params_t &P, const blml_fit_t &f) {
+    const int cd = 13 + 2 * (P.anderson + 1);
+    const int lb = 9 + 2 * (P.lbfgs_m + 1);
+    const bool psd = f.complex_g || f.K > 1;
+    return (P.solver == 1 && psd) ? (cd > lb ? cd : lb) : cd;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *order, int *queue,
+                                                      const double *counts, const double *draws,
+                                                      const double *init, const double *signs, T *e_out, T *se_out,
+                                                      double *density, blml_result_t *results, double *merit_hist,
+                                                      double *rho_hist) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ SolverLds sl;
+    __shared__ int s_fit;
+    for (;;) {
+        if (threadIdx.x == 0) s_fit = atomicAdd(queue, 1);
+        __syncthreads();
+        const int qi = s_fit;
+        __syncthreads();
+        if (qi >= p.nfits) break;
+        const int fi = order[qi];
+        const blml_fit_t f = p.fits[fi];
+        FitSolver<T> S;
+        make_matvec_ctx<T>(p, f, S.mv, lds_raw);
+        S.f = &p.fits[fi];
+        S.P = &p.params;
+        S.sl = &sl;
+        S.B = f.B;
+        S.n = f.n;
+        for (int k = 0; k < BLML_MAXK; ++k) S.s[k] = f.s[k];
+        S.equal_s = f.equal_s != 0;
+        S.w = counts + f.counts_off;
+        S.sg = f.has_signs ? signs + f.signs_off : nullptr;
+        const int ncf = f.complex_g ? 2 : 1;
+        S.Vmax = (long)f.K * f.B * ncf;
+        S.pool = p.ws + f.ws_off + 4 * f.scratch_len;
+        S.gstep = 0.0;
+        S.merit_hist = merit_hist ? merit_hist + f.hist_off : nullptr;
+        S.rho_hist = rho_hist ? rho_hist + f.hist_off : nullptr;
+        S.hist_cap = p.params.hist_cap;
+        S.nhist_m = S.nhist_r = 0;
+        S.ring_n = 0;
+        const bool psd_full = f.complex_g || f.K > 1;
+        const double pn = p.params.phase_noise;
+        const double *dr = draws + f.draws_off;
+        int start_sweeps = 0;
+        T *E = S.vec(0);
+        if (f.init_K > 0) {
+            // warm start (blml_binned.py:403-408): given modes, zero off the occupied bins, noise in the others
+            const double *E0 = init + f.init_off;
+            const int kk = f.init_K < f.K ? f.init_K : f.K;
+            for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
+                const bool o = S.w[b] > 0;
+                for (int k = 0; k < f.K; ++k) {
+                    const long i = ((long)k * f.B + b) * ncf;
+                    double re = 0.0, im = 0.0;
+                    if (k < kk && o) {
+                        re = E0[((long)k * f.B + b) * 2];
+                        im = f.complex_g ? E0[((long)k * f.B + b) * 2 + 1] : 0.0;
+                    }
+                    E[i] = (T)re;
+                    if (ncf == 2) E[i + 1] = (T)im;
+                }
+            }
+            __syncthreads();
+            long dp = 0;
+            for (int k = kk; k < f.K; ++k) {
+                for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
+                    const double a0 = ncf == 2 ? sqrt((double)E[2 * b] * E[2 * b] + (double)E[2 * b + 1] * E[2 * b + 1])
+                                               : fabs((double)E[b]);
+                    const long i = ((long)k * f.B + b) * ncf;
+                    if (ncf == 2) {
+                        E[i] = (T)(pn * a0 * (dr[dp + b] / sqrt(2.0)));
+                        E[i + 1] = (T)(pn * a0 * (dr[dp + f.B + b] / sqrt(2.0)));
+                    } else {
+                        E[i] = (T)(pn * a0 * dr[dp + b]);
+                    }
+                }
+                dp += (long)f.B * ncf;
+            }
+            __syncthreads();
+        } else {
+            // cold start: e0 = sqrt(c / s0) on occupied bins (times the signs)
+            for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
+                const double sgb = S.sg ? S.sg[b] : 1.0;
+                E[b] = S.w[b] > 0 ? (T)(sqrt(f.n * S.w[b] / f.s[0]) * sgb) : (T)0;
+            }
+            __syncthreads();
+            if (psd_full) {
+                // the real rank-1 solution under mode 0 (blml_binned.py:421-425)
+                S.K = 1; S.nc = 1; S.psd = false; S.rec = false;
+                const FitOut so = S.solve_cd();
+                start_sweeps = so.n_sweeps;
+                // phases and the other modes (blml_binned.py:429-436), assembled in slot 2 then moved to slot 0
+                T *E2 = S.vec(2);
+                const T *er = S.vec(0);
+                long dp = 0;
+                for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
+                    if (f.complex_g) {
+                        double sn, cs;
+                        sincos(pn * dr[b], &sn, &cs);
+                        const double r = er[b];
+                        E2[2 * b] = (T)(r * cs);
+                        E2[2 * b + 1] = (T)(r * sn);
+                    } else {
+                        E2[b] = er[b];
+                    }
+                }
+                if (f.complex_g) dp = f.B;
+                __syncthreads();
+                for (int k = 1; k < f.K; ++k) {
+                    for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
+                        const double a0 = ncf == 2
+                            ? sqrt((double)E2[2 * b] * E2[2 * b] + (double)E2[2 * b + 1] * E2[2 * b + 1])
+                            : fabs((double)E2[b]);
+                        const long i = ((long)k * f.B + b) * ncf;
+                        if (ncf == 2) {
+                            E2[i] = (T)(pn * a0 * (dr[dp + b] / sqrt(2.0)));
+                            E2[i + 1] = (T)(pn * a0 * (dr[dp + f.B + b] / sqrt(2.0)));
+                        } else {
+                            E2[i] = (T)(pn * a0 * dr[dp + b]);
+                        }
+                    }
+                    dp += (long)f.B * ncf;
+                    __syncthreads();
+                }
+                S.K = f.K; S.nc = ncf; S.psd = true;
+                S.copy(S.vec(0), E2);
+            }
+        }
+        S.K = f.K;
+        S.nc = ncf;
+        S.psd = psd_full;
+        S.rec = true;
+        S.nhist_m = S.nhist_r = 0;
+        const FitOut o = (p.params.solver == 1 && psd_full) ? S.solve_lbfgs() : S.solve_cd();
+        // outputs (blml_binned.py:438-445)
+        const T *Ef = S.vec(0), *Uf = S.vec(1);
+        const long V = (long)f.K * f.B * ncf;
+        for (long i = threadIdx.x; i < V; i += BLML_NT) {
+            e_out[f.out_off + i] = Ef[i];
+            se_out[f.out_off + i] = Uf[i];
+        }
+        double ll = 0.0;
+        const double n2 = f.n * f.n;
+        for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
+            const double dens = S.Fb(Uf, b) / n2;
+            density[f.dens_off + b] = dens;
+            if (S.w[b] > 0) ll += S.w[b] * log(dens);
+        }
+        ll = block_sum1(ll, sl.red);
+        if (threadIdx.x == 0) {
+            blml_result_t r;
+            r.loglik = ll / f.n;
+            r.residual = o.residual;
+            r.n_sweeps = o.n_sweeps;
+            r.stop_reason = o.stop;
+            r.converged = o.converged;
+            r.n_anderson = o.n_and;
+            r.n_damped = o.n_damp;
+            r.n_gradient = o.n_grad;
+            r.start_sweeps = start_sweeps;
+            r.n_hist = S.nhist_m;
+            results[fi] = r;
+        }
+        __syncthreads();
+    }
+}
+
+// certificate(): lambda_max(D^1/2 S_0 D^1/2) / n, D_b = |e_b|^2 / w_b on occupied bins, by power iteration with an
+// FP64 Rayleigh quotient.  Stops when the quotient changes by less than tol (relative) or after max_iter steps.
+template <typename T>
+__global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<T> p, const T *e_hat, const double *counts,
+                                                       int max_iter, double tol, double *lam_out, int *iters) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[BLML_NWAVE * BLML_NRED];
+    const int fi = blockIdx.x;
+    if (fi >= p.nfits) return;
+    const blml_fit_t f = p.fits[fi];
+    if (!f.shared) {
+        if (threadIdx.x == 0) { lam_out[fi] = NAN; iters[fi] = 0; }
+        return;
+    }
+    MatvecCtx<T> mv;
+    make_matvec_ctx<T>(p, f, mv, lds_raw);
+    const int nc = f.complex_g ? 2 : 1;
+    const int B = f.B;
+    T *pool = p.ws + f.ws_off + 4 * f.scratch_len;
+    double *dh = (double *)pool;                      // B doubles
+    T *x = (T *)(dh + B), *y = x + B;
+    const double *w = counts + f.counts_off;
+    const T *E = e_hat + f.out_off;
+    for (int b = threadIdx.x; b < B; b += BLML_NT) {
+        double D = 0.0;
+        if (w[b] > 0) {
+            for (int k = 0; k < f.K; ++k) {
+                const long i = ((long)k * B + b) * nc;
+                D += (double)E[i] * E[i] + (nc == 2 ? (double)E[i + 1] * E[i + 1] : 0.0);
+            }
+            D /= w[b];
+        }
+        dh[b] = sqrt(D);
+    }
+    __syncthreads();
+    double nrm = 0.0;
+    for (int b = threadIdx.x; b < B; b += BLML_NT) nrm += dh[b] * dh[b];
+    nrm = sqrt(block_sum1(nrm, red));
+    if (!(nrm > 0)) {
+        if (threadIdx.x == 0) { lam_out[fi] = 0.0; iters[fi] = 0; }
+        return;
+    }
+    for (int b = threadIdx.x; b < B; b += BLML_NT) x[b] = (T)(dh[b] / nrm);
+    __syncthreads();
+    double lam = 0.0, lam_prev = -INFINITY;
+    int it = 0;
+    for (it = 1; it <= max_iter; ++it) {
+        for (int b = threadIdx.x; b < B; b += BLML_NT) y[b] = (T)(dh[b] * (double)x[b]);
+        __syncthreads();
+        matvec_mode<T>(mv, f.fac_begin[0], f.fac_count[0], y, y, B, 1);
+        double a[2] = {0.0, 0.0};
+        for (int b = threadIdx.x; b < B; b += BLML_NT) {
+            const double yb = dh[b] * (double)y[b];
+            a[0] += (double)x[b] * yb;
+            a[1] += yb * yb;
+        }
+        block_sum<2>(a, red);
+        double xx = 0.0;
+        for (int b = threadIdx.x; b < B; b += BLML_NT) xx += (double)x[b] * (double)x[b];
+        xx = block_sum1(xx, red);
+        lam = a[0] / xx;
+        const double ny = sqrt(a[1]);
+        if (!(ny > 0)) break;
+        for (int b = threadIdx.x; b < B; b += BLML_NT) x[b] = (T)(dh[b] * (double)y[b] / ny);
+        __syncthreads();
+        if (fabs(lam - lam_prev) <= tol * fabs(lam)) break;
+        lam_prev = lam;
+    }
+    if (threadIdx.x == 0) {
+        lam_out[fi] = lam / f.n;
+        iters[fi] = it > max_iter ? max_iter : it;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ host plan
+struct Plan {
+    int dtype;
+    int nfits, nfacs;
+    blml_params_t params;
+    std::vector<blml_fit_t> hfits;
+    std::vector<blml_factor_t> hfacs;
+    blml_fit_t *d_fits = nullptr;
+    blml_factor_t *d_facs = nullptr;
+    double *d_fdata = nullptr;
+    int *d_freqs = nullptr;
+    void *d_spec = nullptr;
+    void *d_tw = nullptr;
+    void *d_ws = nullptr;
+    int *d_order = nullptr;
+    int *d_queue = nullptr;
+    int Lmax = 2;
+    int lds_bytes = 0;
+    int grid = 1;
+    int64_t out_elems = 0, dens_elems = 0;
+    int64_t ws_elems = 0;
+    size_t esize = 4;
+};
+
+int ceil_pow2(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+template <typename T>
+DevPlan<T> dev_plan(const Plan &pl) {
+    DevPlan<T> p;
+    p.fits = pl.d_fits;
+    p.facs = pl.d_facs;
+    p.fdata = pl.d_fdata;
+    p.freqs = pl.d_freqs;
+    p.spec = (T *)pl.d_spec;
+    p.tw = (const cpx<T> *)pl.d_tw;
+    p.Lmax = pl.Lmax;
+    p.ws = (T *)pl.d_ws;
+    p.nfits = pl.nfits;
+    p.params = pl.params;
+    return p;
+}
+
+// validates the tables and fills in spec_off, ws_off, ws_len, scratch_len; returns the LDS each launch needs
+int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs, int64_t n_freqs,
+           std::vector<int> &fac_m, std::vector<long> &tmp_off, long &tmp_total, int64_t &spec_total) {
+    const blml_params_t &P = pl.params;
+    if (!(P.anderson >= 0 && P.anderson <= BLML_MAXAND))
+        return fail(BLML_EVALUE, "anderson: 0.." + std::to_string(BLML_MAXAND) + " on this engine");
+    if (!(P.lbfgs_m >= 1 && P.lbfgs_m <= BLML_MAXHIST))
+        return fail(BLML_EVALUE, "lbfgs_m: 1.." + std::to_string(BLML_MAXHIST) + " on this engine");
+    if (!(P.ll_window >= 1 && P.ll_window < BLML_RING))
+        return fail(BLML_EVALUE, "ll_window: 1.." + std::to_string(BLML_RING - 1) + " on this engine");
+    if (P.hist_cap < 0) return fail(BLML_EVALUE, "hist_cap must be >= 0");
+    fac_m.assign((size_t)pl.nfacs * BLML_MAXD, 0);
+    std::vector<int> fac_owner(pl.nfacs, -1);
+    int lds_elems = 0;
+    int Lmax = 2;
+    spec_total = 0;
+    // factors: sizes and spectra offsets (owner fit gives the axis lengths)
+    for (int fi = 0; fi < pl.nfits; ++fi) {
+        const blml_fit_t &f = pl.hfits[fi];
+        if (f.d < 1 || f.d > BLML_MAXD) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": 1 <= d <= 5");
+        if (f.K < 1 || f.K > BLML_MAXK) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": 1 <= K <= 8");
+        long B = 1;
+        for (int j = 0; j < f.d; ++j) {
+            if (f.shape[j] < 1) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": empty lattice axis");
+            B *= f.shape[j];
+        }
+        if (B != f.B || B > (1L << 30)) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": B mismatch");
+        for (int k = 0; k < f.K; ++k) {
+            if (f.fac_begin[k] < 0 || f.fac_count[k] < 0 || f.fac_begin[k] + f.fac_count[k] > pl.nfacs)
+                return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": factor range");
+            for (int q = 0; q < f.fac_count[k]; ++q) {
+                const int id = f.fac_begin[k] + q;
+                if (fac_owner[id] >= 0 && fac_owner[id] != fi)
+                    return fail(BLML_EVALUE, "factor shared between fits");
+                fac_owner[id] = fi;
+            }
+        }
+    }
+    tmp_off.assign(pl.nfacs, 0);
+    tmp_total = 0;
+    for (int id = 0; id < pl.nfacs; ++id) {
+        blml_factor_t &fa = pl.hfacs[id];
+        const int fi = fac_owner[id];
+        if (fi < 0) return fail(BLML_EVALUE, "factor " + std::to_string(id) + " belongs to no fit");
+        const blml_fit_t &f = pl.hfits[fi];
+        if (fa.naxes < 1 || fa.naxes > f.d) return fail(BLML_EVALUE, "factor axes");
+        if (fa.kind == BLML_FAC_LINE && fa.naxes != 1) return fail(BLML_EVALUE, "LINE factor with several axes");
+        long total = 1;
+        int ncirc = 0;
+        for (int a = 0; a < fa.naxes; ++a) {
+            const int ax = fa.axes[a];
+            if (ax < 0 || ax >= f.d || (a > 0 && ax <= fa.axes[a - 1])) return fail(BLML_EVALUE, "factor axes");
+            const int m = f.shape[ax];
+            fac_m[(size_t)id * BLML_MAXD + a] = m;
+            const int tl = fa.tlen[a];
+            if (fa.circ[a]) {
+                ++ncirc;
+                if (tl < 1 || tl > m) return fail(BLML_EVALUE, "circular factor: 1 <= frequencies <= m");
+                if (fa.freq_off[a] < 0 || fa.freq_off[a] + tl > n_freqs) return fail(BLML_EVALUE, "freq range");
+                for (int q = 0; q < tl; ++q) {
+                    const int v = freqs[fa.freq_off[a] + q];
+                    if (v < 0 || v >= m) return fail(BLML_EVALUE, "frequency outside [0, m)");
+                }
+                const int op = fa.kind == BLML_FAC_LINE ? OP_CONV_CIRC : OP_DFT_FWD;
+                lds_elems = std::max(lds_elems, pass_lds_elems(op, m, 0, tl));
+                if (fa.kind == BLML_FAC_GROUP) lds_elems = std::max(lds_elems, pass_lds_elems(OP_DFT_INV, m, 0, tl));
+            } else {
+                if (tl < 2 * m - 1 || tl != ceil_pow2(tl) || tl < 2)
+                    return fail(BLML_EVALUE, "padded factor: power-of-two length >= 2m - 1");
+                Lmax = std::max(Lmax, tl);
+                lds_elems = std::max(lds_elems, pass_lds_elems(OP_CONV_PAD, m, tl, 0));
+            }
+            total *= tl;
+        }
+        if (ncirc != 0 && ncirc != fa.naxes)
+            return fail(BLML_EVALUE, "a multi-column unit with both circular and padded axes is not supported");
+        const bool line_pad = fa.kind == BLML_FAC_LINE && !fa.circ[0];
+        const long need = line_pad ? fac_m[(size_t)id * BLML_MAXD] : total;
+        if (fa.data_off < 0 || fa.data_off + need > n_fdata) return fail(BLML_EVALUE, "factor data range");
+        fa.spec_off = spec_total;
+        spec_total += total;
+        if (fa.kind == BLML_FAC_GROUP && ncirc == 0) {
+            tmp_off[id] = tmp_total;
+            tmp_total += 2 * total;
+        }
+    }
+    // fits: scratch (largest intermediate of any factor chain) and workspace
+    int64_t ws = 0;
+    for (int fi = 0; fi < pl.nfits; ++fi) {
+        blml_fit_t &f = pl.hfits[fi];
+        long scratch = f.B;
+        for (int k = 0; k < f.K; ++k) {
+            for (int q = 0; q < f.fac_count[k]; ++q) {
+                const blml_factor_t &fa = pl.hfacs[f.fac_begin[k] + q];
+                if (fa.kind != BLML_FAC_GROUP) continue;
+                long ext[BLML_MAXD];
+                for (int j = 0; j < f.d; ++j) ext[j] = f.shape[j];
+                for (int a = fa.naxes - 1; a >= 0; --a) {
+                    ext[fa.axes[a]] = fa.tlen[a];
+                    long t = 1;
+                    for (int j = 0; j < f.d; ++j) t *= ext[j];
+                    scratch = std::max(scratch, t);
+                }
+            }
+        }
+        f.scratch_len = scratch;
+        const int ncf = f.complex_g ? 2 : 1;
+        const long Vmax = (long)f.K * f.B * ncf;
+        const int cd = 13 + 2 * (P.anderson + 1), lb = 9 + 2 * (P.lbfgs_m + 1);
+        const bool psd = f.complex_g || f.K > 1;
+        const int nsl = (P.solver == 1 && psd) ? std::max(cd, lb) : cd;
+        long len = 4 * scratch + (long)nsl * Vmax;
+        // the certificate reuses the pool: B doubles + 2 B values
+        len = std::max(len, 4 * scratch + 4L * f.B + 2L * f.B);
+        len = (len + 63) / 64 * 64;
+        f.ws_off = ws;
+        f.ws_len = len;
+        ws += len;
+        pl.out_elems = std::max<int64_t>(pl.out_elems, f.out_off + Vmax);
+        pl.dens_elems = std::max<int64_t>(pl.dens_elems, f.dens_off + f.B);
+    }
+    pl.ws_elems = ws;
+    pl.Lmax = Lmax;
+    lds_elems = std::max(lds_elems, 16);
+    const int lds = lds_elems * (int)(2 * pl.esize);
+    if (lds + kStaticLds > kMaxLds)
+        return fail(BLML_EVALUE, "a transform line needs " + std::to_string(lds) + " bytes of LDS (limit " +
+                                     std::to_string(kMaxLds - kStaticLds) + "): lattice axis too long");
+    pl.lds_bytes = lds;
+    return BLML_OK;
+}
+
+template <typename T>
+int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs, int64_t n_freqs) {
+    std::vector<int> fac_m;
+    std::vector<long> tmp_off;
+    long tmp_total = 0;
+    int64_t spec_total = 0;
+    int rc = layout(pl, fdata, n_fdata, freqs, n_freqs, fac_m, tmp_off, tmp_total, spec_total);
+    if (rc) return rc;
+    HIPCHECK(hipMalloc(&pl.d_fits, sizeof(blml_fit_t) * std::max(1, pl.nfits)));
+    HIPCHECK(hipMalloc(&pl.d_facs, sizeof(blml_factor_t) * std::max(1, pl.nfacs)));
+    HIPCHECK(hipMalloc(&pl.d_fdata, sizeof(double) * std::max<int64_t>(1, n_fdata)));
+    HIPCHECK(hipMalloc(&pl.d_freqs, sizeof(int) * std::max<int64_t>(1, n_freqs)));
+    HIPCHECK(hipMalloc(&pl.d_spec, sizeof(T) * std::max<int64_t>(1, spec_total)));
+    HIPCHECK(hipMalloc(&pl.d_tw, sizeof(cpx<T>) * (pl.Lmax / 2)));
+    HIPCHECK(hipMalloc(&pl.d_ws, sizeof(T) * std::max<int64_t>(1, pl.ws_elems)));
+    HIPCHECK(hipMalloc(&pl.d_order, sizeof(int) * std::max(1, pl.nfits)));
+    HIPCHECK(hipMalloc(&pl.d_queue, sizeof(int)));
+    HIPCHECK(hipMemcpy(pl.d_fits, pl.hfits.data(), sizeof(blml_fit_t) * pl.nfits, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(pl.d_facs, pl.hfacs.data(), sizeof(blml_factor_t) * pl.nfacs, hipMemcpyHostToDevice));
+    if (n_fdata) HIPCHECK(hipMemcpy(pl.d_fdata, fdata, sizeof(double) * n_fdata, hipMemcpyHostToDevice));
+    if (n_freqs) HIPCHECK(hipMemcpy(pl.d_freqs, freqs, sizeof(int) * n_freqs, hipMemcpyHostToDevice));
+    // longest fits first (the persistent queue's order)
+    std::vector<int> order(pl.nfits);
+    for (int i = 0; i < pl.nfits; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const blml_fit_t &fa = pl.hfits[a], &fb = pl.hfits[b];
+        return (double)fa.B * fa.K * (1 + fa.complex_g) > (double)fb.B * fb.K * (1 + fb.complex_g);
+    });
+    HIPCHECK(hipMemcpy(pl.d_order, order.data(), sizeof(int) * pl.nfits, hipMemcpyHostToDevice));
+    twiddle_kernel<T><<<(pl.Lmax / 2 + 255) / 256, 256>>>((cpx<T> *)pl.d_tw, pl.Lmax);
+    HIPCHECK(hipGetLastError());
+    int *d_fac_m = nullptr;
+    long *d_tmp_off = nullptr;
+    cpx<T> *d_tmp = nullptr;
+    HIPCHECK(hipMalloc(&d_fac_m, sizeof(int) * std::max<size_t>(1, fac_m.size())));
+    HIPCHECK(hipMalloc(&d_tmp_off, sizeof(long) * std::max(1, pl.nfacs)));
+    HIPCHECK(hipMalloc(&d_tmp, sizeof(cpx<T>) * std::max<long>(1, tmp_total)));
+    HIPCHECK(hipMemcpy(d_fac_m, fac_m.data(), sizeof(int) * fac_m.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(d_tmp_off, tmp_off.data(), sizeof(long) * pl.nfacs, hipMemcpyHostToDevice));
+    if (pl.nfacs > 0) {
+        spec_kernel<T><<<pl.nfacs, BLML_NT, pl.lds_bytes>>>(dev_plan<T>(pl), pl.nfacs, d_fac_m, d_tmp, d_tmp_off);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipFree(d_fac_m));
+    HIPCHECK(hipFree(d_tmp_off));
+    HIPCHECK(hipFree(d_tmp));
+    // persistent grid: as many workgroups as fit on the device at once (capped by the number of fits)
+    int dev = 0, ncu = 0, occ = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fit_kernel<T>, BLML_NT, pl.lds_bytes));
+    if (occ < 1) occ = 1;
+    pl.grid = std::max(1, std::min(pl.nfits, ncu * occ));
+    return BLML_OK;
+}
+
+void free_plan(Plan *pl) {
+    if (!pl) return;
+    void *ptrs[] = {pl->d_fits, pl->d_facs, pl->d_fdata, pl->d_freqs, pl->d_spec,
+                    pl->d_tw,   pl->d_ws,   pl->d_order, pl->d_queue};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete pl;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------ C-ABI
+extern "C" {
+
+const char *blml_last_error(void) { return g_err.c_str(); }
+
+int blml_plan_create(const blml_fit_t *fits, int32_t nfits, const blml_factor_t *facs, int32_t nfacs,
+                     const double *fdata, int64_t n_fdata, const int32_t *freqs, int64_t n_freqs, int32_t dtype,
+                     const blml_params_t *params, void **plan) {
+    if (!plan || !params || (nfits > 0 && !fits) || (nfacs > 0 && !facs)) return fail(BLML_EVALUE, "null argument");
+    if (nfits < 1) return fail(BLML_EVALUE, "a plan needs at least one fit");
+    if (dtype != BLML_F32 && dtype != BLML_F64) return fail(BLML_EVALUE, "dtype: BLML_F32 or BLML_F64");
+    Plan *pl = new Plan();
+    pl->dtype = dtype;
+    pl->nfits = nfits;
+    pl->nfacs = nfacs;
+    pl->params = *params;
+    pl->hfits.assign(fits, fits + nfits);
+    pl->hfacs.assign(facs, facs + nfacs);
+    pl->esize = dtype == BLML_F32 ? 4 : 8;
+    const int rc = dtype == BLML_F32 ? create_t<float>(*pl, fdata, n_fdata, freqs, n_freqs)
+                                     : create_t<double>(*pl, fdata, n_fdata, freqs, n_freqs);
+    if (rc) {
+        free_plan(pl);
+        return rc;
+    }
+    *plan = pl;
+    return BLML_OK;
+}
+
+int blml_plan_destroy(void *plan) {
+    free_plan((Plan *)plan);
+    return BLML_OK;
+}
+
+int blml_plan_info(void *plan, int64_t *out_elems, int64_t *ws_bytes, int32_t *lds_bytes, int32_t *grid) {
+    Plan *pl = (Plan *)plan;
+    if (!pl) return fail(BLML_EVALUE, "null plan");
+    if (out_elems) *out_elems = pl->out_elems;
+    if (ws_bytes) *ws_bytes = pl->ws_elems * (int64_t)pl->esize;
+    if (lds_bytes) *lds_bytes = pl->lds_bytes;
+    if (grid) *grid = pl->grid;
+    return BLML_OK;
+}
+
+int blml_matvec(void *plan, const void *E, void *U, void *stream) {
+    Plan *pl = (Plan *)plan;
+    if (!pl || !E || !U) return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (pl->dtype == BLML_F32)
+        matvec_kernel<float><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<float>(*pl), (const float *)E,
+                                                                          (float *)U);
+    else
+        matvec_kernel<double><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<double>(*pl), (const double *)E,
+                                                                           (double *)U);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+int blml_fit(void *plan, const double *counts, const double *draws, const double *init, const double *signs,
+             void *e_out, void *se_out, double *density, blml_result_t *results, double *merit_hist, double *rho_hist,
+             void *stream) {
+    Plan *pl = (Plan *)plan;
+    if (!pl || !counts || !draws || !e_out || !se_out || !density || !results) return fail(BLML_EVALUE, "null argument");
+    for (const blml_fit_t &f : pl->hfits) {
+        if (f.init_K > 0 && !init) return fail(BLML_EVALUE, "a warm-started fit needs init");
+        if (f.has_signs && !signs) return fail(BLML_EVALUE, "a fit with signs needs signs");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHECK(hipMemsetAsync(pl->d_queue, 0, sizeof(int), st));
+    if (pl->dtype == BLML_F32)
+        fit_kernel<float><<<pl->grid, BLML_NT, pl->lds_bytes, st>>>(
+            dev_plan<float>(*pl), pl->d_order, pl->d_queue, counts, draws, init, signs, (float *)e_out,
+            (float *)se_out, density, results, merit_hist, rho_hist);
+    else
+        fit_kernel<double><<<pl->grid, BLML_NT, pl->lds_bytes, st>>>(
+            dev_plan<double>(*pl), pl->d_order, pl->d_queue, counts, draws, init, signs, (double *)e_out,
+            (double *)se_out, density, results, merit_hist, rho_hist);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+int blml_certificate(void *plan, const void *e_hat, const double *counts, int32_t max_iter, double tol,
+                     double *lam, int32_t *iters, void *stream) {
+    Plan *pl = (Plan *)plan;
+    if (!pl || !e_hat || !counts || !lam || !iters) return fail(BLML_EVALUE, "null argument");
+    if (max_iter < 1) return fail(BLML_EVALUE, "max_iter must be >= 1");
+    hipStream_t st = (hipStream_t)stream;
+    if (pl->dtype == BLML_F32)
+        cert_kernel<float><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<float>(*pl), (const float *)e_hat,
+                                                                        counts, max_iter, tol, lam, iters);
+    else
+        cert_kernel<double><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<double>(*pl), (const double *)e_hat,
+                                                                         counts, max_iter, tol, lam, iters);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+}  // extern "C"

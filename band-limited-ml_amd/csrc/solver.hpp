@@ -1,0 +1,681 @@
+// One binned fit, solved start to finish by one workgroup: the start (blml_binned.py:415-436), Jacobi coordinate
+// descent with Anderson mixing, damping and gradient fallback (blml_binned.py:223-276, 357-380, 1463-1478), or
+// L-BFGS with Armijo backtracking and the canonical form (blml_binned.py:278-355), and the outputs
+// (blml_binned.py:438-445).  Vectors are stored in T (FP32 or FP64); every reduction (log-likelihood, Q, dot
+// products, curvature s.y / y.y, residual max) accumulates in FP64.  The scalar control flow is executed
+// identically by every thread: it only depends on reduction results that all threads hold bit-identically.
+#pragma once
+#include <math.h>
+#include <limits.h>
+#include "transform.hpp"
+
+#define BLML_MAXHIST 64      // L-BFGS memory limit
+#define BLML_MAXAND 5        // Anderson history limit
+#define BLML_RING 1024       // merit window ring (ll_window + 1 <= BLML_RING)
+#define BLML_HARD_CAP 1000000L  // sweeps / iterations of one solve when max_sweeps is None (every launch ends)
+
+struct SolverLds {
+    double red[BLML_NWAVE * BLML_NRED];
+    double ring[BLML_RING];
+    double lb_r[BLML_MAXHIST + 1], lb_a[BLML_MAXHIST + 1], lb_sy[BLML_MAXHIST + 1], lb_yy[BLML_MAXHIST + 1];
+    double and_g[BLML_MAXAND];
+};
+
+struct FitOut {
+    int n_sweeps, stop, converged, n_and, n_damp, n_grad;
+    double residual;
+};
+
+__device__ inline double cd_root(double b, double c, double s) {
+    const double disc = sqrt(b * b + 4.0 * s * c);
+    return b >= 0 ? 2.0 * c / fmax(b + disc, 1e-300) : (-b + disc) / (2.0 * s);
+}
+
+// The root lam > max s of lam * sum_k R2_k / (lam - s_k)^2 = c (blml_binned.py:1556-1573), 16 safeguarded
+// Newton steps on phi^-1/2.
+__device__ inline double secular_root(const double *R2, const double *s, int K, double c) {
+    double smax = s[0], tot = 0.0;
+    for (int k = 0; k < K; ++k) { smax = fmax(smax, s[k]); tot += R2[k]; }
+    const double cpos = fmax(c, 1e-30);
+    double hi = (tot + sqrt(tot * tot + 4 * cpos * smax * tot)) / (2 * cpos);
+    double lo = 0.0, mu = hi;
+    for (int it = 0; it < 16; ++it) {
+        double A = 0.0, Bv = 0.0;
+        for (int k = 0; k < K; ++k) {
+            const double q = 1.0 / (mu - (s[k] - smax));
+            const double W = R2[k] * (q * q);
+            A += W;
+            Bv += W * q;
+        }
+        const double lam = smax + mu, phi = lam * A;
+        const bool big = phi > c;
+        lo = big ? mu : lo;
+        hi = big ? hi : mu;
+        const double nt = mu + 2 * phi * (1 - sqrt(phi / cpos)) / (A - 2 * lam * Bv);
+        mu = (nt >= lo && nt <= hi) ? nt : 0.5 * (lo + hi);
+    }
+    return smax + mu;
+}
+
+template <typename T>
+struct FitSolver {
+    MatvecCtx<T> mv;
+    const blml_fit_t *f;
+    const blml_params_t *P;
+    SolverLds *sl;
+    int B, K, nc;          // active: the real start runs K = 1, nc = 1
+    bool psd;
+    double n;
+    double s[BLML_MAXK];
+    bool equal_s;
+    const double *w;
+    const double *sg;      // signs or null
+    T *pool;
+    long Vmax;
+    double gstep;          // the gradient fallback's step, kept between calls (0: unset)
+    double *merit_hist, *rho_hist;
+    int hist_cap, nhist_m, nhist_r;
+    int ring_n;            // entries pushed to the merit ring
+    bool rec;              // record merit / rho histories (the final solve only, as merit_history_ is)
+
+    __device__ T *vec(int i) const { return pool + (long)i * Vmax; }
+    __device__ long V() const { return (long)K * B * nc; }
+
+    // ------------------------------------------------------------------ basic vector operations
+    __device__ void Sall(const T *E, T *U) {
+        for (int k = 0; k < K; ++k)
+            matvec_mode<T>(mv, f->fac_begin[k], f->fac_count[k], E + (long)k * B * nc, U + (long)k * B * nc, B,
+                           nc);
+    }
+    __device__ double dot(const T *a, const T *b) {
+        double acc = 0.0;
+        const long n_ = V();
+        for (long i = threadIdx.x; i < n_; i += BLML_NT) acc += (double)a[i] * (double)b[i];
+        return block_sum1(acc, sl->red);
+    }
+    __device__ void copy(T *dst, const T *src) {
+        const long n_ = V();
+        for (long i = threadIdx.x; i < n_; i += BLML_NT) dst[i] = src[i];
+        __syncthreads();
+    }
+    __device__ void scale(T *a, double sc) {
+        const long n_ = V();
+        for (long i = threadIdx.x; i < n_; i += BLML_NT) a[i] = (T)((double)a[i] * sc);
+        __syncthreads();
+    }
+    // out = a + t * b
+    __device__ void axpy(T *out, const T *a, double t, const T *b) {
+        const long n_ = V();
+        for (long i = threadIdx.x; i < n_; i += BLML_NT) out[i] = (T)((double)a[i] + t * (double)b[i]);
+        __syncthreads();
+    }
+    __device__ bool occ(int b) const { return w[b] > 0; }
+    __device__ double cb(int b) const { return n * w[b]; }
+    __device__ double absq(const T *X, int k, int b) const {
+        const long i = ((long)k * B + b) * nc;
+        return nc == 2 ? (double)X[i] * X[i] + (double)X[i + 1] * X[i + 1] : (double)X[i] * X[i];
+    }
+    __device__ double Fb(const T *U, int b) const {
+        double F = 0.0;
+        for (int k = 0; k < K; ++k) F += absq(U, k, b);
+        return F;
+    }
+
+    // ------------------------------------------------------------------ merit, update, rescale, residual
+    __device__ double merit(const T *E, const T *U) {
+        if (psd) {
+            double a[2] = {0.0, 0.0};
+            for (int b = threadIdx.x; b < B; b += BLML_NT) {
+                a[0] += cb(b) * log(fmax(Fb(U, b), 1e-30));
+            }
+            const long n_ = V();
+            for (long i = threadIdx.x; i < n_; i += BLML_NT) a[1] += (double)E[i] * (double)U[i];
+            block_sum<2>(a, sl->red);
+            const double Q = a[1];
+            if (!(Q > 0)) return -INFINITY;
+            return a[0] / n - n * log(Q);
+        }
+        double a[3] = {0.0, 0.0, 0.0};
+        for (int b = threadIdx.x; b < B; b += BLML_NT) {
+            const double e = E[b], u = U[b];
+            const double sgb = sg ? sg[b] : 1.0;
+            a[0] += e * u;
+            if (occ(b)) {
+                a[1] += cb(b) * log(fabs(e));
+                if (sgb * e <= 0) a[2] += 1.0;
+            }
+        }
+        block_sum<3>(a, sl->red);
+        if (a[2] > 0) return -INFINITY;
+        return -(0.5 * a[0] - a[1]) / n;
+    }
+
+    __device__ void update(const T *E, const T *U, T *out) {
+        if (!psd) {
+            const double s0 = s[0];
+            for (int b = threadIdx.x; b < B; b += BLML_NT) {
+                const double sgb = sg ? sg[b] : 1.0;
+                const double R = (double)U[b] - s0 * (double)E[b];
+                const double t = occ(b) ? cd_root(sgb * R, cb(b), s0) : 0.0;
+                out[b] = (T)(sgb * t);
+            }
+            __syncthreads();
+            return;
+        }
+        for (int b = threadIdx.x; b < B; b += BLML_NT) {
+            double Rr[BLML_MAXK], Ri[BLML_MAXK], R2[BLML_MAXK];
+            double a2 = 0.0;
+            for (int k = 0; k < K; ++k) {
+                const long i = ((long)k * B + b) * nc;
+                Rr[k] = (double)U[i] - s[k] * (double)E[i];
+                Ri[k] = nc == 2 ? (double)U[i + 1] - s[k] * (double)E[i + 1] : 0.0;
+                R2[k] = Rr[k] * Rr[k] + Ri[k] * Ri[k];
+                a2 += R2[k];
+            }
+            const double c = cb(b);
+            const double a = sqrt(a2);
+            double fr[BLML_MAXK], fi[BLML_MAXK];
+            if (equal_s) {
+                const double s0 = s[0];
+                const double t = 2 * c / (a + sqrt(a2 + 4 * s0 * c));
+                const double g = t / (a > 0 ? a : 1.0);
+                for (int k = 0; k < K; ++k) { fr[k] = Rr[k] * g; fi[k] = Ri[k] * g; }
+                if (a == 0 && c > 0) { fr[0] = sqrt(c / s0); fi[0] = 0.0; }
+            } else {
+                const bool lone = (a2 == 0) && (c > 0);
+                const double lam = secular_root(R2, s, K, c);
+                for (int k = 0; k < K; ++k) {
+                    if (c > 0 && !lone) {
+                        const double dn = lam - s[k];
+                        fr[k] = Rr[k] / dn; fi[k] = Ri[k] / dn;
+                    } else {
+                        fr[k] = 0.0; fi[k] = 0.0;
+                    }
+                }
+                if (lone) { fr[0] = sqrt(c / s[0]); fi[0] = 0.0; }
+            }
+            for (int k = 0; k < K; ++k) {
+                const long i = ((long)k * B + b) * nc;
+                out[i] = (T)fr[k];
+                if (nc == 2) out[i + 1] = (T)fi[k];
+            }
+        }
+        __syncthreads();
+    }
+
+    __device__ void rescale(T *E, T *U) {
+        const double Q = dot(E, U);
+        const double sc = Q > 0 ? n / sqrt(Q) : 1.0;
+        const long n_ = V();
+        for (long i = threadIdx.x; i < n_; i += BLML_NT) {
+            E[i] = (T)((double)E[i] * sc);
+            U[i] = (T)((double)U[i] * sc);
+        }
+        __syncthreads();
+    }
+
+    // max over occupied bins of ||u_b (|e_b|^2 / c_b) - e_b|| / ||e_b|| of the rescaled pair (sc E, sc U)
+    __device__ double resid(const T *E, const T *U, double sc) {
+        double r = 0.0;
+        for (int b = threadIdx.x; b < B; b += BLML_NT) {
+            if (!occ(b)) continue;
+            double nrm2 = 0.0;
+            for (int k = 0; k < K; ++k) nrm2 += absq(E, k, b);
+            nrm2 *= sc * sc;
+            const double fac = nrm2 / cb(b);
+            double acc = 0.0;
+            for (int k = 0; k < K; ++k) {
+                const long i = ((long)k * B + b) * nc;
+                for (int q = 0; q < nc; ++q) {
+                    const double v = sc * (double)U[i + q] * fac - sc * (double)E[i + q];
+                    acc += v * v;
+                }
+            }
+            r = fmax(r, sqrt(acc) / sqrt(fmax(nrm2, 1e-300)));
+        }
+        return block_max(r, sl->red);
+    }
+
+    __device__ double Qof(const T *E, const T *U) { return dot(E, U); }
+
+    // L = sum w log F - n log Q and (want_grad) g = 2 (S(w U / F) - n U / Q) on occupied bins.  Vt, SVt: scratch.
+    __device__ double lgrad(const T *E, const T *U, T *g, T *Vt, T *SVt, bool want_grad) {
+        double a[2] = {0.0, 0.0};
+        for (int b = threadIdx.x; b < B; b += BLML_NT) {
+            const bool o = occ(b);
+            const double F = Fb(U, b);
+            a[0] += w[b] * log(o ? F : 1.0);
+            const double fac = o ? w[b] / F : 0.0;
+            for (int k = 0; k < K; ++k) {
+                const long i = ((long)k * B + b) * nc;
+                for (int q = 0; q < nc; ++q) {
+                    a[1] += (double)E[i + q] * (double)U[i + q];
+                    Vt[i + q] = (T)(fac * (double)U[i + q]);
+                }
+            }
+        }
+        block_sum<2>(a, sl->red);
+        const double Q = a[1];
+        if (!(Q > 0)) return -INFINITY;
+        const double L = a[0] - n * log(Q);
+        if (!want_grad) return L;
+        Sall(Vt, SVt);
+        const double nq = n / Q;
+        for (int b = threadIdx.x; b < B; b += BLML_NT) {
+            const bool o = occ(b);
+            for (int k = 0; k < K; ++k) {
+                const long i = ((long)k * B + b) * nc;
+                for (int q = 0; q < nc; ++q) {
+                    const T G = (T)((double)SVt[i + q] - nq * (double)U[i + q]);
+                    g[i + q] = o ? (T)(2.0 * (double)G) : (T)0;
+                }
+            }
+        }
+        __syncthreads();
+        return L;
+    }
+
+    // ------------------------------------------------------------------ histories
+    __device__ void push_merit(double v) {
+        if (threadIdx.x == 0) {
+            sl->ring[ring_n % BLML_RING] = v;
+            if (rec && merit_hist && nhist_m < hist_cap) merit_hist[nhist_m] = v;
+        }
+        ++ring_n;
+        if (rec) ++nhist_m;
+        __syncthreads();
+    }
+    __device__ double ring_back(int k) const { return sl->ring[(ring_n - 1 - k) % BLML_RING]; }
+    __device__ void push_rho(double v) {
+        if (threadIdx.x == 0 && rec && rho_hist && nhist_r < hist_cap) rho_hist[nhist_r] = v;
+        if (rec) ++nhist_r;
+    }
+
+    // ------------------------------------------------------------------ Anderson type II
+    // hX / hF: slot arrays (ring of cap = mA + 1, ordered from head); returns false on a singular history.
+    __device__ bool anderson(const int *slotX, const int *slotF, int cnt, const T *E, const T *f, T *EA) {
+        const int m = cnt - 1;
+        double acc[20];
+        for (int i = 0; i < 20; ++i) acc[i] = 0.0;
+        const long n_ = V();
+        for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+            double dF[BLML_MAXAND];
+            for (int i = 0; i < m; ++i) dF[i] = (double)(T)(vec(slotF[i + 1])[e] - vec(slotF[i])[e]);
+            const double fe = f[e];
+            int p = 0;
+            for (int i = 0; i < m; ++i)
+                for (int j = i; j < m; ++j) acc[p++] += dF[i] * dF[j];
+            for (int i = 0; i < m; ++i) acc[p++] += dF[i] * fe;
+        }
+        block_sum<20>(acc, sl->red);
+        double G[BLML_MAXAND][BLML_MAXAND + 1];
+        int p = 0;
+        double tr = 0.0;
+        for (int i = 0; i < m; ++i)
+            for (int j = i; j < m; ++j) { G[i][j] = acc[p]; G[j][i] = acc[p]; ++p; }
+        for (int i = 0; i < m; ++i) { G[i][m] = acc[p++]; tr += G[i][i]; }
+        for (int i = 0; i < m; ++i) G[i][i] += 1e-12 * tr;
+        // Gaussian elimination with partial pivoting (every thread solves the same small system)
+        for (int col = 0; col < m; ++col) {
+            int piv = col;
+            for (int r = col + 1; r < m; ++r)
+                if (fabs(G[r][col]) > fabs(G[piv][col])) piv = r;
+            if (G[piv][col] == 0.0) return false;
+            if (piv != col)
+                for (int q = 0; q <= m; ++q) { const double t = G[col][q]; G[col][q] = G[piv][q]; G[piv][q] = t; }
+            for (int r = col + 1; r < m; ++r) {
+                const double fct = G[r][col] / G[col][col];
+                for (int q = col; q <= m; ++q) G[r][q] -= fct * G[col][q];
+            }
+        }
+        double gam[BLML_MAXAND];
+        for (int i = m - 1; i >= 0; --i) {
+            double v = G[i][m];
+            for (int q = i + 1; q < m; ++q) v -= G[i][q] * gam[q];
+            gam[i] = v / G[i][i];
+        }
+        for (int i = 0; i < m; ++i)
+            if (!isfinite(gam[i])) return false;
+        for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+            double mix = 0.0;
+            for (int i = 0; i < m; ++i) {
+                const double dX = (double)vec(slotX[i + 1])[e] - (double)vec(slotX[i])[e];
+                const double dF = (double)vec(slotF[i + 1])[e] - (double)vec(slotF[i])[e];
+                mix += gam[i] * (dX + dF);
+            }
+            EA[e] = (T)((double)E[e] + (double)f[e] - mix);
+        }
+        __syncthreads();
+        return true;
+    }
+
+    // ------------------------------------------------------------------ gradient fallback (psd)
+    // From (E, U, cur): up to 5 Armijo-backtracked steepest-ascent steps on L; result in (Eo, Uo), merit returned.
+    __device__ double grad_steps(const T *E0, const T *U0, double cur, T *Eo, T *Uo, T *Vb, T *Gb, T *SGb,
+                                 T *En, T *Un) {
+        copy(Eo, E0);
+        copy(Uo, U0);
+        for (int st = 0; st < 5; ++st) {
+            double a2[2] = {0.0, 0.0};
+            for (int b = threadIdx.x; b < B; b += BLML_NT) {
+                const bool o = occ(b);
+                const double F = Fb(Uo, b);
+                const double fac = o ? w[b] / (o ? F : 1.0) : 0.0;
+                for (int k = 0; k < K; ++k) {
+                    const long i = ((long)k * B + b) * nc;
+                    for (int q = 0; q < nc; ++q) {
+                        a2[0] += (double)Eo[i + q] * (double)Uo[i + q];
+                        Vb[i + q] = (T)(fac * (double)Uo[i + q]);
+                    }
+                }
+            }
+            block_sum<2>(a2, sl->red);
+            const double Q = a2[0];
+            Sall(Vb, Gb);
+            const double nq = n / Q;
+            for (int b = threadIdx.x; b < B; b += BLML_NT) {
+                const bool o = occ(b);
+                for (int k = 0; k < K; ++k) {
+                    const long i = ((long)k * B + b) * nc;
+                    for (int q = 0; q < nc; ++q) {
+                        const T G = (T)((double)Gb[i + q] - nq * (double)Uo[i + q]);
+                        Gb[i + q] = o ? G : (T)0;
+                    }
+                }
+            }
+            __syncthreads();
+            Sall(Gb, SGb);
+            const double g2 = dot(Gb, Gb);
+            if (!(g2 > 0)) break;
+            double a = gstep;
+            if (!(a != 0.0)) a = 1e-3 * sqrt(dot(Eo, Eo) / g2);
+            bool ok = false;
+            double mn = cur;
+            for (int t = 0; t < 40; ++t) {
+                axpy(En, Eo, a, Gb);
+                axpy(Un, Uo, a, SGb);
+                mn = merit(En, Un);
+                if (mn > cur) { ok = true; break; }
+                a *= 0.5;
+            }
+            if (!ok) break;
+            copy(Eo, En);
+            copy(Uo, Un);
+            cur = mn;
+            gstep = 2 * a;
+        }
+        return cur;
+    }
+
+    // ------------------------------------------------------------------ Jacobi CD + Anderson (blml_binned.py:223-276)
+    // E in slot 0; the solution ends in slots 0 (E) and 1 (U).
+    __device__ FitOut solve_cd() {
+        // slots: 0 E, 1 U, 2 EJ, 3 UJ, 4 EA, 5 UA, 6 EP, 7 UP, 8 V, 9 G, 10 SG, 11 En, 12 Un, 13.. history
+        int iE = 0, iU = 1, iEJ = 2, iUJ = 3, iEA = 4, iUA = 5;
+        const int iEP = 6, iUP = 7;
+        const int mA = P->anderson;
+        const int hcap = mA + 1;
+        int hX[BLML_MAXAND + 1], hF[BLML_MAXAND + 1];
+        int hhead = 0, hcnt = 0;
+        FitOut o{};
+        Sall(vec(iE), vec(iU));
+        rescale(vec(iE), vec(iU));
+        double cur = merit(vec(iE), vec(iU));
+        long it = 0;
+        double res = resid(vec(iE), vec(iU), 1.0);
+        push_rho(res);
+        const long cap = P->max_sweeps < 0 ? BLML_HARD_CAP : P->max_sweeps;
+        double best_res = res, best_m = cur;
+        long best_it = 0;
+        ring_n = 0;
+        push_merit(cur);
+        int stop = BLML_STOP_TOL;
+        const bool use_ll = !isnan(P->ll_tol);
+        const int lw = P->ll_window;
+        while (res >= P->tol && it < cap) {
+            if (it - best_it >= P->stall) { stop = BLML_STOP_STALL; break; }
+            if (use_ll && ring_n > lw && (ring_back(0) - ring_back(lw)) / n < P->ll_tol) {
+                stop = BLML_STOP_LLTOL; break;
+            }
+            ++it;
+            update(vec(iE), vec(iU), vec(iEJ));
+            Sall(vec(iEJ), vec(iUJ));
+            double mJ = merit(vec(iEJ), vec(iUJ));
+            if (mA) {
+                // f = EJ - E; hX.append(E); hF.append(f); keep the last mA + 1
+                int slot;
+                if (hcnt < hcap) {
+                    slot = (hhead + hcnt) % hcap;
+                    ++hcnt;
+                } else {
+                    slot = hhead;
+                    hhead = (hhead + 1) % hcap;
+                }
+                T *xs = vec(13 + slot), *fs = vec(13 + hcap + slot);
+                const T *Ec = vec(iE), *EJc = vec(iEJ);
+                const long n_ = V();
+                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                    xs[e] = Ec[e];
+                    fs[e] = (T)((double)EJc[e] - (double)Ec[e]);
+                }
+                __syncthreads();
+                if (hcnt >= 3) {
+                    int sx[BLML_MAXAND + 1], sf[BLML_MAXAND + 1];
+                    for (int i = 0; i < hcnt; ++i) {
+                        sx[i] = 13 + (hhead + i) % hcap;
+                        sf[i] = 13 + hcap + (hhead + i) % hcap;
+                    }
+                    if (anderson(sx, sf, hcnt, vec(iE), fs, vec(iEA))) {
+                        Sall(vec(iEA), vec(iUA));
+                        const double mAc = merit(vec(iEA), vec(iUA));
+                        if (mAc > mJ) {
+                            int t = iEJ; iEJ = iEA; iEA = t;
+                            t = iUJ; iUJ = iUA; iUA = t;
+                            mJ = mAc;
+                            ++o.n_and;
+                        }
+                    } else {
+                        hcnt = 0; hhead = 0;
+                    }
+                }
+            }
+            if (mJ < cur - 1e-12 * fabs(cur)) {
+                hcnt = 0; hhead = 0;
+                ++o.n_damp;
+                update(vec(iE), vec(iU), vec(iEP));
+                Sall(vec(iEP), vec(iUP));
+                double a = 0.5;
+                bool found = false;
+                while (a > 1e-8) {
+                    const long n_ = V();
+                    T *EJ = vec(iEJ), *UJ = vec(iUJ);
+                    const T *Ec = vec(iE), *Uc = vec(iU), *EPc = vec(iEP), *UPc = vec(iUP);
+                    for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                        const double de = (T)((double)EPc[e] - (double)Ec[e]);
+                        const double du = (T)((double)UPc[e] - (double)Uc[e]);
+                        EJ[e] = (T)((double)Ec[e] + (double)(T)(a * de));
+                        UJ[e] = (T)((double)Uc[e] + (double)(T)(a * du));
+                    }
+                    __syncthreads();
+                    mJ = merit(vec(iEJ), vec(iUJ));
+                    if (mJ >= cur) { found = true; break; }
+                    a *= 0.5;
+                }
+                if (!found) {
+                    if (psd) {
+                        mJ = grad_steps(vec(iE), vec(iU), cur, vec(iEJ), vec(iUJ), vec(8), vec(9), vec(10),
+                                        vec(11), vec(12));
+                        ++o.n_grad;
+                    } else {
+                        copy(vec(iEJ), vec(iE));
+                        copy(vec(iUJ), vec(iU));
+                        mJ = cur;
+                    }
+                }
+            }
+            rescale(vec(iEJ), vec(iUJ));
+            { int t = iE; iE = iEJ; iEJ = t; t = iU; iU = iUJ; iUJ = t; }
+            cur = psd ? mJ : merit(vec(iE), vec(iU));
+            push_merit(cur);
+            res = resid(vec(iE), vec(iU), 1.0);
+            push_rho(res);
+            if (res < 0.9 * best_res || cur > best_m + 1e-13 * fabs(best_m)) {
+                best_res = fmin(res, best_res);
+                best_it = it;
+                best_m = fmax(cur, best_m);
+            }
+        }
+        if (res >= P->tol && stop == BLML_STOP_TOL) stop = BLML_STOP_MAXSWEEPS;
+        if (iE != 0) { copy(vec(0), vec(iE)); copy(vec(1), vec(iU)); }
+        o.n_sweeps = (int)it;
+        o.stop = stop;
+        o.residual = res;
+        o.converged = res < P->tol;
+        return o;
+    }
+
+    // ------------------------------------------------------------------ L-BFGS (blml_binned.py:290-355)
+    __device__ FitOut solve_lbfgs() {
+        // slots: 0 E, 1 U, 2 g, 3 q, 4 En, 5 Un, 6 gn, 7 Vt, 8 SVt, 9.. S (m + 1), 10 + m.. Y (m + 1).  The pair
+        // memory is a ring of m + 1 physical slots holding at most m pairs, so a candidate pair never overwrites a
+        // kept one before its curvature test.
+        int iE = 0, iU = 1, ig = 2, iEn = 4, iUn = 5, ign = 6;
+        const int iq = 3, iVt = 7, iSVt = 8;
+        const int mh = P->lbfgs_m, ph = mh + 1;
+        FitOut o{};
+        Sall(vec(iE), vec(iU));
+        rescale(vec(iE), vec(iU));
+        double L = lgrad(vec(iE), vec(iU), vec(ig), vec(iVt), vec(iSVt), true);
+        int head = 0, cnt = 0;
+        long it = 0;
+        double Q = Qof(vec(iE), vec(iU));
+        double res = resid(vec(iE), vec(iU), Q > 0 ? n / sqrt(Q) : 1.0);
+        push_rho(res);
+        ring_n = 0;
+        push_merit(L / n);
+        const long cap = P->max_sweeps < 0 ? BLML_HARD_CAP : P->max_sweeps;
+        int stop = BLML_STOP_TOL;
+        bool reset = false;
+        const double ftol = isnan(P->ll_tol) ? 1e-11 : P->ll_tol;
+        const int fwin = 10;
+        while (res >= P->tol && it < cap) {
+            if (ring_n > fwin && ring_back(0) - ring_back(fwin) < ftol) { stop = BLML_STOP_LLTOL; break; }
+            ++it;
+            T *q = vec(iq);
+            copy(q, vec(ig));
+            for (int j = cnt - 1; j >= 0; --j) {
+                const int sl_ = (head + j) % ph;
+                const double a = sl->lb_r[sl_] * dot(vec(9 + sl_), q);
+                if (threadIdx.x == 0) sl->lb_a[j] = a;
+                axpy(q, q, -a, vec(9 + ph + sl_));
+            }
+            if (cnt > 0) {
+                const int nw = (head + cnt - 1) % ph;
+                scale(q, sl->lb_sy[nw] / sl->lb_yy[nw]);
+            } else {
+                const double ee = dot(vec(iE), vec(iE)), gg = dot(vec(ig), vec(ig));
+                scale(q, 1e-3 * sqrt(ee / fmax(gg, 1e-300)));
+            }
+            for (int j = 0; j < cnt; ++j) {
+                const int sl_ = (head + j) % ph;
+                const double bb = sl->lb_r[sl_] * dot(vec(9 + ph + sl_), q);
+                axpy(q, q, sl->lb_a[j] - bb, vec(9 + sl_));
+            }
+            double slope = dot(vec(ig), q);
+            if (!(slope > 0)) {
+                cnt = 0; head = 0;
+                const double ee = dot(vec(iE), vec(iE)), gg = dot(vec(ig), vec(ig));
+                const double sc = 1e-3 * sqrt(ee / fmax(gg, 1e-300));
+                const long n_ = V();
+                const T *g = vec(ig);
+                for (long e = threadIdx.x; e < n_; e += BLML_NT) q[e] = (T)((double)g[e] * sc);
+                __syncthreads();
+                slope = dot(vec(ig), q);
+            }
+            double t = 1.0, Ln = -INFINITY;
+            bool ok = false;
+            for (int tr = 0; tr < 40; ++tr) {
+                axpy(vec(iEn), vec(iE), t, q);
+                Sall(vec(iEn), vec(iUn));
+                Ln = lgrad(vec(iEn), vec(iUn), vec(ign), vec(iVt), vec(iSVt), true);
+                if (Ln >= L + 1e-4 * t * slope) { ok = true; break; }
+                t *= 0.5;
+            }
+            if (!ok) {
+                if (reset || cnt == 0) { stop = BLML_STOP_STALL; break; }
+                cnt = 0; head = 0;
+                reset = true;
+                continue;
+            }
+            reset = false;
+            // s = En - E, y = g - gn, into the next slot; curvature test
+            const int slot = (head + cnt) % ph;
+            {
+                T *S = vec(9 + slot), *Yv = vec(9 + ph + slot);
+                const T *En = vec(iEn), *Ec = vec(iE), *g = vec(ig), *gn = vec(ign);
+                double a3[3] = {0.0, 0.0, 0.0};
+                const long n_ = V();
+                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                    const T sv = (T)((double)En[e] - (double)Ec[e]);
+                    const T yv = (T)((double)g[e] - (double)gn[e]);
+                    S[e] = sv;
+                    Yv[e] = yv;
+                    a3[0] += (double)sv * (double)yv;
+                    a3[1] += (double)sv * (double)sv;
+                    a3[2] += (double)yv * (double)yv;
+                }
+                block_sum<3>(a3, sl->red);
+                if (a3[0] > 1e-12 * sqrt(a3[1] * a3[2])) {
+                    if (threadIdx.x == 0) {
+                        sl->lb_r[slot] = 1.0 / a3[0];
+                        sl->lb_sy[slot] = a3[0];
+                        sl->lb_yy[slot] = a3[2];
+                    }
+                    if (cnt < mh) ++cnt;
+                    else head = (head + 1) % ph;
+                }
+                __syncthreads();
+            }
+            { int tt = iE; iE = iEn; iEn = tt; tt = iU; iU = iUn; iUn = tt; tt = ig; ig = ign; ign = tt; }
+            L = Ln;
+            Q = Qof(vec(iE), vec(iU));
+            res = resid(vec(iE), vec(iU), Q > 0 ? n / sqrt(Q) : 1.0);
+            push_rho(res);
+            push_merit(L / n);
+        }
+        if (res >= P->tol && stop == BLML_STOP_TOL) stop = BLML_STOP_MAXSWEEPS;
+        // canonical form: e_k = (Q / n) (w / F) u_k on occupied bins, kept unless it loses likelihood
+        {
+            double a1[1] = {0.0};
+            const long n_ = V();
+            const T *Ec = vec(iE), *Uc = vec(iU);
+            for (long e = threadIdx.x; e < n_; e += BLML_NT) a1[0] += (double)Ec[e] * (double)Uc[e];
+            block_sum<1>(a1, sl->red);
+            const double Qc = a1[0];
+            T *Ecn = vec(iEn);
+            for (int b = threadIdx.x; b < B; b += BLML_NT) {
+                const bool ob = occ(b);
+                const double F = Fb(Uc, b);
+                const double fac = ob ? (Qc / n) * w[b] / F : 0.0;
+                for (int k = 0; k < K; ++k) {
+                    const long i = ((long)k * B + b) * nc;
+                    for (int qq = 0; qq < nc; ++qq) Ecn[i + qq] = (T)(fac * (double)Uc[i + qq]);
+                }
+            }
+            __syncthreads();
+            Sall(vec(iEn), vec(iUn));
+            const double Lc = lgrad(vec(iEn), vec(iUn), vec(ign), vec(iVt), vec(iSVt), false);
+            if (Lc >= L - 1e-9 * n) {
+                int tt = iE; iE = iEn; iEn = tt; tt = iU; iU = iUn; iUn = tt;
+            }
+        }
+        rescale(vec(iE), vec(iU));
+        res = resid(vec(iE), vec(iU), 1.0);
+        if (iE != 0) { copy(vec(0), vec(iE)); copy(vec(1), vec(iU)); }
+        o.n_sweeps = (int)it;
+        o.stop = stop;
+        o.residual = res;
+        o.converged = res < P->tol || stop == BLML_STOP_LLTOL;
+        return o;
+    }
+};

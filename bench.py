@@ -1,0 +1,199 @@
+"""Benchmark of the binned BLML fit on one MI355X.
+
+Workload (the headline configuration): 2048 independent 2-D binned fits of uneven lattices, one persistent launch
+per step.  Sides round(exp(U[ln 16, ln 128])) per axis; half the fits are a periodic-ball signal mode
+(fc ~ U[0.15, 0.45], period = the lattice) plus a periodic-ball background (fc = 3 / sqrt(m1 m2)), the other half
+a two-slit signal per column (fc ~ U[0.2, 0.5], fill ~ U[0.3, 0.7]) plus a dirichlet x dirichlet background with
+the lattice as period (fc = 3 / m per column: one harmonic each way); half the fits complex_g.  Fields: CN(0, 1)
+coefficients on the signal band's harmonics, density |g|^2 normalised with a 5% uniform background, Poisson
+counts with totals log-uniform in [1e3, 1e6].  numpy default_rng(3).  L-BFGS (lbfgs_m 20), FP32 transforms with
+FP64 reductions; the plain FP64 method is run on the same GPU as the reference point of the metric.
+
+A step = one launch that fits all 2048 fits from their cold start.  value = fits / s of the FP32 engine.
+Prints one JSON line (see the contract in the repository's design notes)."""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, 'band-limited-ml_amd')):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK = 8.0e12
+N_FITS = 2048
+SOLVER = dict(solver='lbfgs', lbfgs_m=20, tol=1e-5, ll_tol=1e-9, max_sweeps=3000)
+
+
+# ------------------------------------------------------------------------------------------------ workload
+def _band_field(rng, shape, harmonics):
+    """g(x) = sum_k a_k exp(2 pi i k.x / m), a_k ~ CN(0, 1), on the lattice."""
+    ks = np.asarray(harmonics, float)
+    a = (rng.standard_normal(len(ks)) + 1j * rng.standard_normal(len(ks))) / np.sqrt(2)
+    g = np.zeros(shape, complex)
+    e = [np.exp(2j * np.pi * np.outer(ks[:, j], np.arange(m)) / m) for j, m in enumerate(shape)]
+    for i in range(len(ks)):
+        g += a[i] * np.multiply.outer(e[0][i], e[1][i])
+    return g
+
+
+def _pball_harmonics(fc, P):
+    R = fc / 2.0
+    N = [int(np.floor(R * p + 1e-9)) for p in P]
+    K1, K2 = np.meshgrid(np.arange(-N[0], N[0] + 1), np.arange(-N[1], N[1] + 1), indexing='ij')
+    keep = (K1 / P[0]) ** 2 + (K2 / P[1]) ** 2 <= R * R * (1 + 1e-9) + 1e-12
+    return np.stack([K1[keep], K2[keep]], 1)
+
+
+def _slits_band(fc, fill, m):
+    """Integer harmonics k of an m-periodic column inside the two-slit band (1 - fill) fc / 2 <= |k| / m <= fc / 2
+    (the nearest pair +-k when no integer falls inside)."""
+    k = np.arange(-(m // 2), m // 2 + 1)
+    nu = np.abs(k) / m
+    band = k[(nu <= fc / 2) & (nu >= (1 - fill) * fc / 2)]
+    if len(band) == 0:                      # a lattice too coarse for the band: its nearest harmonic pair
+        kc = max(1, int(round(m * fc * (2 - fill) / 4)))
+        band = np.array([-kc, kc])
+    return band
+
+
+def uneven_fits(n_fits=N_FITS, seed=3):
+    """[(counts, make_modes groups, fit kwargs, description)] of the batched uneven-lattice workload."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_fits):
+        shape = tuple(int(round(math.exp(rng.uniform(math.log(16), math.log(128))))) for _ in range(2))
+        pball = rng.uniform() < 0.5
+        cplx = rng.uniform() < 0.5
+        total = math.exp(rng.uniform(math.log(1e3), math.log(1e6)))
+        P = [float(shape[0]), float(shape[1])]
+        if pball:
+            fc = float(rng.uniform(0.15, 0.45))
+            H = _pball_harmonics(fc, P)
+            fbg = 3.0 / math.sqrt(shape[0] * shape[1])
+            groups = [dict(fc=[[fc, fc], [fbg, fbg]], kinds=['pball', 'pball'], periods=P)]
+        else:
+            fc = [float(rng.uniform(0.2, 0.5)) for _ in range(2)]
+            fill = [float(rng.uniform(0.3, 0.7)) for _ in range(2)]
+            bands = [_slits_band(fc[j], fill[j], shape[j]) for j in range(2)]
+            H = np.stack(np.meshgrid(*bands, indexing='ij'), -1).reshape(-1, 2)
+            groups = [dict(fc=fc, kinds=[f'slits:{fill[0]!r}', f'slits:{fill[1]!r}']),
+                      dict(fc=[3.0 / shape[0], 3.0 / shape[1]], kinds=['dirichlet', 'dirichlet'], periods=P)]
+        g = _band_field(rng, shape, H)
+        p = np.abs(g) ** 2
+        p = p / p.sum()
+        p = 0.95 * p + 0.05 / p.size
+        counts = rng.poisson(total * p).astype(float)
+        if counts.sum() == 0:
+            counts.flat[0] = 1.0
+        out.append((counts, groups, dict(complex_g=bool(cplx)), dict(kind='pball' if pball else 'slits', shape=shape)))
+    return out
+
+
+def modes_of(groups, lib):
+    return [md for kw in groups for md in lib.make_modes(**kw)]
+
+
+# ------------------------------------------------------------------------------------------------ bench
+def cpu_baseline(fits, seconds=20.0):
+    """The FP64 oracle (numpy + scipy.fft, workers=-1) on the first fits of the workload until ~seconds of work."""
+    from oracle import blml_oracle as O
+    cores = len(os.sched_getaffinity(0))
+    done, t0 = 0, time.perf_counter()
+    bins = 0
+    for counts, groups, kw, _ in fits:
+        O.BLMLBinnedOracle(**SOLVER).fit(counts, modes_of(groups, O), **kw)
+        done += 1
+        bins += counts.size
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    return dict(value=done / dt, unit='fits/s', cores=cores, kind='port',
+                sample=f'first {done} of the {len(fits)} fits ({bins} bins), FP64 numpy/scipy.fft workers=-1, '
+                       f'{dt:.1f} s')
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--fits', type=int, default=N_FITS)
+    ap.add_argument('--no-fp64', action='store_true', help='skip the plain FP64 reference run')
+    ap.add_argument('--cpu-seconds', type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (independent fits '
+              'shard across GPUs with no exchange; see the design notes)', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import blml_amd
+    from blml_amd.binned import _FitSpec, FitBatch
+    fits = uneven_fits(args.fits)
+    print(f'bench: {len(fits)} fits, {sum(c.size for c, *_ in fits)} bins', file=sys.stderr, flush=True)
+    specs = [_FitSpec(c, modes_of(g, blml_amd), 1.0, 0.0, kw['complex_g'], None, None, 0) for c, g, kw, _ in fits]
+    params = dict(tol=SOLVER['tol'], max_sweeps=SOLVER['max_sweeps'], anderson=5, stall=1000, phase_noise=0.1,
+                  ll_tol=SOLVER['ll_tol'], ll_window=100, solver=SOLVER['solver'], lbfgs_m=SOLVER['lbfgs_m'])
+
+    def timed(dtype, steps, warmup):
+        b = FitBatch(specs, params, dtype, hist_cap=0).build()
+        for _ in range(warmup):
+            b.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        return b, dt
+
+    b32, dt32 = timed(np.float32, args.steps, args.warmup)
+    r32 = b32.results()
+    print(f'bench: fp32 {dt32 / args.steps * 1e3:.1f} ms/step', file=sys.stderr, flush=True)
+    ll32 = np.array([r['loglik'] for r in r32])
+    extra = {}
+    if not args.no_fp64:
+        b64, dt64 = timed(np.float64, 1, 0)
+        r64 = b64.results()
+        ll64 = np.array([r['loglik'] for r in r64])
+        rel = np.abs(ll32 - ll64) / np.abs(ll64)
+        extra = dict(fp64_fits_per_s=len(fits) / dt64, speedup_vs_fp64=dt64 / (dt32 / args.steps),
+                     frac_within_1e6=float(np.mean(rel <= 1e-6)), median_rel_ll=float(np.median(rel)))
+        b64.close()
+        print(f'bench: fp64 {dt64 * 1e3:.1f} ms/step; within 1e-6: {extra["frac_within_1e6"]:.3f}',
+              file=sys.stderr, flush=True)
+    # roofline of the batched Gram mat-vec (all fits, all modes), timed with HIP events on the launch stream
+    E = b32.d_e.clone()
+    Uo = b32.matvec(E)
+    st = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    ev0.record(st)
+    for _ in range(reps):
+        b32.matvec(E)
+    ev1.record(st)
+    torch.cuda.synchronize()
+    t_mv = ev0.elapsed_time(ev1) / 1e3 / reps
+    alg = sum(2 * fs.K * fs.nc * fs.B * 4 for fs in specs)
+    roof = dict(bound='hbm', achieved=alg / t_mv / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
+                frac=alg / t_mv / HBM_PEAK, traffic=None, kernel='matvec_kernel', bytes_per_launch=alg,
+                us_per_launch=t_mv * 1e6)
+    cpu = cpu_baseline(fits, args.cpu_seconds)
+    ms = dt32 / args.steps * 1e3
+    line = dict(metric='fits/sec at loglik tol 1e-6 (warm-start FP32 vs plain FP64) + mat-vec HBM GB/s',
+                value=len(fits) / (dt32 / args.steps), unit='fits/s', n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=ms, higher_is_better=True, scaling='weak', vs_baseline=None,
+                dtype='f32 transforms, f64 reductions', data='synthetic',
+                config=dict(workload=f'{len(fits)} uneven 2-D binned fits (pball / slits + background, '
+                                     'L-BFGS, cold start)', fits=len(fits), solver=SOLVER),
+                roofline=roof, cpu_baseline=cpu, **extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,138 @@
+/* C-ABI of the MI355X engine for the binned BLML fit (libblml_hip.so).
+ *
+ * The reference has no native interface for this path: BLMLBinned reaches its device through a Python seam,
+ * `_backend(engine) -> (xp, fft)` (blml_binned.py:40-46), and calls cupyx rfftn/irfftn (blml_binned.py:156-165),
+ * CuPy elementwise/reduction ops (blml_binned.py:171-220, 278-288) and cupyx eigsh (blml_binned.py:510-518).
+ * Each entry point below replaces one group of those calls; the Python host mirror (blml_amd.binned.BLMLBinned)
+ * binds them with ctypes.  All array arguments are device pointers unless marked (host); no torch types appear.
+ *
+ * Status codes map to the reference's exception types at the Python layer:
+ *   BLML_OK 0, BLML_EVALUE 1 -> ValueError, BLML_ERUNTIME 2 -> RuntimeError, BLML_EMEMORY 3 -> MemoryError.
+ * blml_last_error() returns the message of the last failing call on the calling thread.
+ */
+#ifndef BLML_HIP_H
+#define BLML_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BLML_MAXD 5   /* lattice dimensions */
+#define BLML_MAXK 8   /* modes per fit */
+
+enum { BLML_OK = 0, BLML_EVALUE = 1, BLML_ERUNTIME = 2, BLML_EMEMORY = 3 };
+enum { BLML_F32 = 0, BLML_F64 = 1 };
+enum { BLML_FAC_LINE = 0, BLML_FAC_GROUP = 1 };
+enum { BLML_STOP_TOL = 0, BLML_STOP_STALL = 1, BLML_STOP_LLTOL = 2, BLML_STOP_MAXSWEEPS = 3 };
+
+/* One factor (kernel unit) of a mode's product kernel, as an operator on the lattice.
+ * LINE: one axis; padded (circ=0): linear convolution with the even row data[0..m-1] = K(off*h), off = 0..m-1,
+ *       done as a zero-padded power-of-two FFT of length tlen >= 2m-1 (any such length gives the same operator
+ *       as the reference's 5-smooth pad, blml_binned.py:63-90);
+ *       circular (circ=1): the periodic kernel's spectrum restricted to its band: data[0..tlen-1] are the
+ *       spectrum values (already divided by the period) at the frequencies freqs[freq_off[0] + q].
+ * GROUP: several axes (a multi-column unit: ball, pball).  All circular: data holds the dense spectrum over the
+ *       product of the per-axis frequency lists (row-major in increasing axis order).  All padded: data holds the
+ *       kernel table over the power-of-two lattice prod tlen (wrapped offsets, zero outside |off| <= m-1); the
+ *       library transforms it to the spectrum at plan creation.                                                */
+typedef struct {
+    int32_t kind;
+    int32_t naxes;
+    int32_t axes[BLML_MAXD];
+    int32_t circ[BLML_MAXD];
+    int32_t tlen[BLML_MAXD];
+    int64_t freq_off[BLML_MAXD];
+    int64_t data_off;
+    int64_t spec_off;          /* set by the library */
+} blml_factor_t;
+
+/* One fit of a batch (the arguments of BLMLBinned.fit, blml_binned.py:383). */
+typedef struct {
+    int32_t d, K, complex_g, shared;
+    int32_t equal_s;                 /* np.allclose(s_, s_[0]) (blml_binned.py:197) */
+    int32_t B;
+    int32_t shape[BLML_MAXD];
+    int32_t fac_begin[BLML_MAXK];
+    int32_t fac_count[BLML_MAXK];
+    int32_t init_K;                  /* 0: cold start (blml_binned.py:415-427); >0: warm start with that many modes */
+    int32_t has_signs;
+    double n;                        /* total count */
+    double s[BLML_MAXK];             /* K_k(0) */
+    int64_t counts_off;              /* element offsets into the batch arrays */
+    int64_t draws_off;
+    int64_t init_off;
+    int64_t signs_off;
+    int64_t out_off;                 /* e_hat / Se: K*B*(1+complex_g) elements */
+    int64_t dens_off;                /* density: B elements */
+    int64_t hist_off;                /* merit / rho history: hist_cap elements */
+    int64_t ws_off;                  /* set by the library */
+    int64_t ws_len;                  /* set by the library */
+    int64_t scratch_len;             /* complex elements of each transform scratch buffer (set by the library) */
+} blml_fit_t;
+
+/* Solver knobs (BLMLBinned.__init__, blml_binned.py:107-116); one set per batch. */
+typedef struct {
+    double tol;
+    int64_t max_sweeps;              /* < 0: None */
+    int32_t anderson;                /* history, <= 5 */
+    int32_t stall;
+    double phase_noise;
+    double ll_tol;                   /* NaN: None */
+    int32_t ll_window;
+    int32_t solver;                  /* 0 'cd', 1 'lbfgs' */
+    int32_t lbfgs_m;                 /* <= 64 */
+    int32_t hist_cap;                /* merit / rho history entries kept per fit */
+} blml_params_t;
+
+/* Per-fit outputs (the scalar attributes of a fitted BLMLBinned, blml_binned.py:273-275, 351-354, 438-445). */
+typedef struct {
+    double loglik;
+    double residual;
+    int32_t n_sweeps;
+    int32_t stop_reason;
+    int32_t converged;
+    int32_t n_anderson;
+    int32_t n_damped;
+    int32_t n_gradient;
+    int32_t start_sweeps;
+    int32_t n_hist;
+} blml_result_t;
+
+/* Plan: uploads the fit and factor tables (host arrays), computes every spectrum on the device, allocates the
+ * workspace.  Replaces _setup/_build_kernels/_ModeKernel (blml_binned.py:49-154).  dtype: BLML_F32 (FP32
+ * transforms, FP64 reductions) or BLML_F64 (the plain FP64 method).                                          */
+int blml_plan_create(const blml_fit_t *fits, int32_t nfits, const blml_factor_t *facs, int32_t nfacs,
+                     const double *fdata, int64_t n_fdata, const int32_t *freqs, int64_t n_freqs,
+                     int32_t dtype, const blml_params_t *params, void **plan);
+int blml_plan_destroy(void *plan);
+
+/* _Sall over every fit of the plan: U = [S_k E_k]_k (blml_binned.py:156-168).  E and U hold every fit's
+ * K*B*(1+complex_g) values of the plan dtype at the fit's out_off.                                           */
+int blml_matvec(void *plan, const void *E, void *U, void *stream);
+
+/* The whole fit of every fit of the plan in one persistent launch: start (real CD under mode 0 and the phase
+ * draws, or the warm start), CD + Anderson or L-BFGS, canonical form, outputs (blml_binned.py:223-445).
+ * counts, draws, init, signs: double arrays indexed by the fits' offsets (init / signs may be NULL when no fit
+ * uses them).  e_out, se_out: plan dtype.  results: device array of nfits blml_result_t.  merit_hist, rho_hist:
+ * device doubles, hist_cap per fit (may be NULL).                                                             */
+int blml_fit(void *plan, const double *counts, const double *draws, const double *init, const double *signs,
+             void *e_out, void *se_out, double *density, blml_result_t *results, double *merit_hist,
+             double *rho_hist, void *stream);
+
+/* certificate(): lambda_max(D^1/2 S_0 D^1/2) / n per fit by power iteration on the device with an FP64 Rayleigh
+ * quotient (blml_binned.py:501-518).  e_hat: plan dtype (as written by blml_fit).  lam: nfits doubles (NaN for
+ * a fit without a shared kernel).  iters: nfits int32 (iterations used).                                      */
+int blml_certificate(void *plan, const void *e_hat, const double *counts, int32_t max_iter, double tol,
+                     double *lam, int32_t *iters, void *stream);
+
+/* Sizes the plan chose (for the caller's buffers and for reporting). */
+int blml_plan_info(void *plan, int64_t *out_elems, int64_t *ws_bytes, int32_t *lds_bytes, int32_t *grid);
+
+const char *blml_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BLML_HIP_H */

@@ -1,0 +1,219 @@
+"""Parity of the HIP engine (through the C-ABI) with the FP64 oracle and the reference's golden vectors.
+Tolerances: FP64 engine -- mat-vec 1e-12 relative to the row sums of |S|, loglik 1e-9 relative (converged fits),
+density 1e-6 max-relative; FP32 engine -- mat-vec 2e-6 relative to max |S e|, loglik 1e-6 relative to the FP64
+oracle."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import blml_oracle as O
+from cases import golden_cases
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), 'golden', 'binned_fits.npz')
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _modes(mm, lib):
+    return [md for kw in mm for md in lib.make_modes(**kw)]
+
+
+MATVEC_CASES = [
+    ('1d sinc', (40,), [dict(fc=[0.35], kinds=['sinc'])], 1.0, False),
+    ('1d sinc 4096', (4096,), [dict(fc=[0.05], kinds=['sinc'])], 1.0, False),
+    ('1d dirichlet circular complex', (32,), [dict(fc=[0.3], kinds=['dirichlet'], periods=[32.0])], 1.0, True),
+    ('1d dirichlet fill circular', (32,), [dict(fc=[0.6], kinds=['dirichlet:0.5'], periods=[32.0])], 1.0, False),
+    ('2d sinc x dirichlet complex', (12, 16), [dict(fc=[0.4, 0.3], kinds=['sinc', 'dirichlet'],
+                                                    periods=[None, 16.0])], 1.0, True),
+    ('2d ball h=0.5', (11, 13), [dict(fc=[0.8, 0.8], kinds=['ball', 'ball'])], 0.5, False),
+    ('2d pball 31x31 + bg complex', (31, 31), [dict(fc=[[0.33, 0.33], [3 / 31, 3 / 31]], kinds=['pball', 'pball'],
+                                                    periods=[31.0, 31.0])], 1.0, True),
+    ('2d pball padded', (9, 10), [dict(fc=[0.4, 0.4], kinds=['pball', 'pball'], periods=[12.0, 15.0])], 1.0, False),
+    ('2d slits + dirichlet bg 128x97 complex', (128, 97), [dict(fc=[0.35, 0.3], kinds=['slits:0.5', 'slits:0.4']),
+                                                           dict(fc=[3 / 128, 3 / 97], kinds=['dirichlet', 'dirichlet'],
+                                                                periods=[128.0, 97.0])], 1.0, True),
+    ('2d pball 127x17 complex', (127, 17), [dict(fc=[0.45, 0.45], kinds=['pball', 'pball'],
+                                                 periods=[127.0, 17.0])], 1.0, True),
+    ('3d sinc', (6, 7, 5), [dict(fc=[0.5, 0.4, 0.3], kinds=['sinc'] * 3)], 1.0, False),
+    ('5d sinc 8^5', (8,) * 5, [dict(fc=[0.3] * 5, kinds=['sinc'] * 5)], 1.0, False),
+]
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32], ids=['f64', 'f32'])
+def test_matvec_matches_oracle(dtype):
+    import torch
+    import blml_amd
+    from blml_amd.binned import _FitSpec, FitBatch
+    rng = np.random.default_rng(0)
+    specs, oracles, Es = [], [], []
+    for name, shape, mm, h, cplx in MATVEC_CASES:
+        counts = rng.poisson(3.0, shape).astype(float) + 1
+        fs = _FitSpec(counts, _modes(mm, blml_amd), h, 0.0, cplx, None, None, 0)
+        specs.append(fs)
+        om = O.BLMLBinnedOracle()
+        om._setup(counts, _modes(mm, O), h, 0.0)
+        oracles.append(om)
+        E = rng.standard_normal((fs.K,) + shape)
+        if cplx:
+            E = E + 1j * rng.standard_normal((fs.K,) + shape)
+        Es.append(E)
+    params = dict(tol=1e-5, max_sweeps=None, anderson=5, stall=1000, phase_noise=0.1, ll_tol=None, ll_window=100,
+                  solver='cd', lbfgs_m=20)
+    _note(f'matvec batch of {len(specs)} fits, dtype {np.dtype(dtype).name}')
+    batch = FitBatch(specs, params, dtype).build()
+    flat = np.zeros(batch.sizes['out'])
+    for fs, F, E in zip(specs, batch._Fits, Es):
+        V = fs.K * fs.B * fs.nc
+        v = np.stack([E.real, E.imag], -1) if fs.complex_g else E[..., None]
+        flat[F.out_off:F.out_off + V] = v.reshape(-1)
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    U = batch.matvec(torch.from_numpy(flat).to('cuda', tdt)).double().cpu().numpy()
+    for (name, shape, mm, h, cplx), fs, F, E, om in zip(MATVEC_CASES, specs, batch._Fits, Es, oracles):
+        V = fs.K * fs.B * fs.nc
+        u = U[F.out_off:F.out_off + V].reshape((fs.K, fs.B, fs.nc))
+        u = u[..., 0] + 1j * u[..., 1] if cplx else u[..., 0]
+        for k in range(fs.K):
+            want = om._S(k, E[k]).ravel()
+            err = np.abs(u[k] - want).max()
+            if dtype == np.float64:
+                G = O.dense_gram(om.specs_[k], shape, om.h_) if fs.B <= 4096 else None
+                scale = np.abs(G).sum(1).max() if G is not None else np.abs(want).max() * 10
+                assert err < 1e-12 * max(1.0, scale) * max(1.0, np.abs(E).max()), (name, k, err)
+            else:
+                assert err < 2e-6 * np.abs(want).max() * max(1.0, np.log2(fs.B)), (name, k, err)
+    batch.close()
+
+
+def _hist(rng, shape, n, f):
+    g = np.meshgrid(*[np.arange(mm, dtype=float) for mm in shape], indexing='ij')
+    p = f(*g); p = p / p.sum()
+    return rng.multinomial(n, p.ravel()).reshape(shape).astype(float)
+
+
+REAL_FITS = [
+    ('1d sinc 40', (40,), dict(fc=[0.35], kinds=['sinc']), 1.0),
+    ('1d sinc 512', (512,), dict(fc=[0.05], kinds=['sinc']), 1.0),
+    ('1d dirichlet 32', (32,), dict(fc=[0.3], kinds=['dirichlet'], periods=[32.0]), 1.0),
+    ('2d sinc x sinc', (16, 18), dict(fc=[0.4, 0.3], kinds=['sinc', 'sinc']), 1.0),
+    ('2d ball h=0.5', (14, 14), dict(fc=[0.8, 0.8], kinds=['ball', 'ball']), 0.5),
+    ('2d pball', (15, 17), dict(fc=[0.45, 0.45], kinds=['pball', 'pball'], periods=[15.0, 17.0]), 1.0),
+]
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32], ids=['f64', 'f32'])
+def test_real_rank1_fits_match_oracle(dtype):
+    import blml_amd
+    rng = np.random.default_rng(5)
+    jobs, refs = [], []
+    for name, shape, mm, h in REAL_FITS:
+        f = lambda *x: np.exp(-0.5 * sum(((xi - s / 2.3) / (s / 5)) ** 2 for xi, s in zip(x, shape))) + 0.05
+        N = _hist(rng, shape, 3000, f)
+        jobs.append((N, blml_amd.make_modes(**mm), dict(h=h)))
+        refs.append(O.BLMLBinnedOracle(tol=1e-9 if dtype == np.float64 else 1e-5,
+                                       ll_tol=None if dtype == np.float64 else 1e-13).fit(N, O.make_modes(**mm), h=h))
+    _note(f'real rank-1 fits, dtype {np.dtype(dtype).name}')
+    kw = dict(tol=1e-9) if dtype == np.float64 else dict(tol=1e-5, ll_tol=1e-13)
+    fits = blml_amd.fit_many(jobs, dtype=dtype, **kw)
+    for (name, *_), m, r in zip(REAL_FITS, fits, refs):
+        rel = abs(m.loglik_ - r.loglik_) / abs(r.loglik_)
+        occ = r.occ_
+        drel = np.max(np.abs(m.density_ - r.density_)[occ] / r.density_[occ])
+        if dtype == np.float64:
+            assert rel < 1e-9 and drel < 1e-6, (name, rel, drel, m.stop_reason_, r.stop_reason_)
+        else:
+            assert rel < 1e-6, (name, rel, drel, m.stop_reason_, m.n_sweeps_)
+
+
+def test_golden_fits_fp64():
+    """The FP64 engine against the reference's own outputs (tests/golden/make_golden.py)."""
+    import blml_amd
+    z = np.load(GOLD, allow_pickle=False)
+    meta = json.loads(str(z['meta']))
+    for i, (name, counts, mm, est_kw, fit_kw) in enumerate(golden_cases()):
+        _note(f'golden fit {i}: {name}')
+        m = blml_amd.BLMLBinned(dtype=np.float64, **est_kw).fit(counts, _modes(mm, blml_amd), **fit_kw)
+        ref_ll = float(z[f'c{i}_loglik'])
+        rel = abs(m.loglik_ - ref_ll) / abs(ref_ll)
+        d_ref = z[f'c{i}_density']; occ = counts > 0
+        drel = np.max(np.abs(m.density_ - d_ref)[occ] / d_ref[occ])
+        _note(f'   loglik rel {rel:.2e} density max-rel {drel:.2e} sweeps {m.n_sweeps_} '
+              f'(ref {int(z[f"c{i}_n_sweeps"])}) stop {m.stop_reason_} (ref {meta[i]["stop_reason"]})')
+        assert list(m.pad_) == meta[i]['pad'] and m.shared_ == meta[i]['shared']
+        if meta[i]['stop_reason'] == 'max_sweeps':
+            assert rel < 1e-7, (name, rel)
+        else:
+            # stopped by max|rho| < tol: the field is pinned; stopped by ll_tol (L-BFGS): only the likelihood is,
+            # and the density moves along near-flat directions by up to ~1e-4
+            assert rel < 1e-9, (name, rel)
+            assert drel < (1e-5 if meta[i]['stop_reason'] == 'tol' else 1e-4), (name, drel)
+
+
+def test_golden_fits_fp32_within_1e6():
+    """The FP32 engine (FP64 reductions) against the reference's FP64 outputs: loglik within 1e-6 relative."""
+    import blml_amd
+    z = np.load(GOLD, allow_pickle=False)
+    for i, (name, counts, mm, est_kw, fit_kw) in enumerate(golden_cases()):
+        kw = dict(est_kw)
+        _note(f'golden fit {i} fp32: {name}')
+        m = blml_amd.BLMLBinned(dtype=np.float32, **kw).fit(counts, _modes(mm, blml_amd), **fit_kw)
+        ref_ll = float(z[f'c{i}_loglik'])
+        rel = abs(m.loglik_ - ref_ll) / abs(ref_ll)
+        _note(f'   loglik rel {rel:.2e} sweeps {m.n_sweeps_} stop {m.stop_reason_}')
+        assert rel < 1e-6, (name, rel)
+
+
+@pytest.mark.parametrize('cplx', [False, True])
+def test_certificate_matches_dense(cplx):
+    import blml_amd
+    rng = np.random.default_rng(2)
+    c2 = lambda x, y: np.exp(-0.5 * (((x - 11) / 2.5) ** 2 + ((y - 13) / 3) ** 2)) + 0.01
+    N = _hist(rng, (16, 16), 4000, c2)
+    _note(f'certificate complex={cplx}')
+    m = blml_amd.BLMLBinned(tol=1e-8, dtype=np.float64).fit(N, blml_amd.make_modes([0.4, 0.4], ['ball', 'ball']),
+                                                            complex_g=cplx)
+    occ = N.ravel() > 0
+    om = O.BLMLBinnedOracle()
+    om._setup(N, O.make_modes([0.4, 0.4], ['ball', 'ball']), 1.0, 0.0)
+    S = O.dense_gram(om.specs_[0], N.shape, om.h_)[np.ix_(occ, occ)]
+    E = np.asarray(m.e_hat_).reshape(m.K_, -1)[:, occ]
+    D = (np.abs(E) ** 2).sum(0) / N.ravel()[occ]
+    lam = np.linalg.eigvalsh(np.sqrt(D)[:, None] * S * np.sqrt(D)[None])[-1] / m.n_
+    got = m.certificate()
+    _note(f'   certificate {got:.9f} dense {lam:.9f} iters {m.certificate_iters_}')
+    assert abs(got - lam) < 1e-6 * lam
+    with pytest.raises(ValueError):
+        blml_amd.BLMLBinned(solver='lbfgs', max_sweeps=3).fit(
+            np.ones((31, 31)), blml_amd.make_modes([[0.33, 0.33], [3 / 31, 3 / 31]], ['pball', 'pball'],
+                                                   periods=[31.0, 31.0]), complex_g=True).certificate()
+
+
+def test_batch_equals_single_fits():
+    """fit_many (one persistent launch, longest-first queue) gives each fit exactly what a fit of its own gives."""
+    import blml_amd
+    rng = np.random.default_rng(9)
+    jobs = []
+    for i in range(12):
+        shape = tuple(int(v) for v in rng.integers(10, 40, 2))
+        N = rng.poisson(4.0, shape).astype(float)
+        if i % 2:
+            mm = [dict(fc=[0.4, 0.4], kinds=['pball', 'pball'], periods=[float(shape[0]), float(shape[1])]),
+                  dict(fc=[3.0 / shape[0], 3.0 / shape[0]], kinds=['pball', 'pball'],
+                       periods=[float(shape[0]), float(shape[1])])]
+        else:
+            mm = [dict(fc=[0.35, 0.3], kinds=['slits:0.5', 'slits:0.5']),
+                  dict(fc=[3.0 / shape[0], 3.0 / shape[1]], kinds=['dirichlet', 'dirichlet'],
+                       periods=[float(shape[0]), float(shape[1])])]
+        jobs.append((N, _modes(mm, blml_amd), dict(complex_g=bool(i % 3))))
+    _note('batch of 12 vs single fits')
+    many = blml_amd.fit_many(jobs, solver='lbfgs', max_sweeps=200, dtype=np.float64)
+    for (N, modes, kw), m in zip(jobs, many):
+        one = blml_amd.BLMLBinned(solver='lbfgs', max_sweeps=200, dtype=np.float64).fit(N, modes, **kw)
+        assert one.loglik_ == m.loglik_ and np.array_equal(one.density_, m.density_)

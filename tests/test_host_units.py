@@ -1,0 +1,163 @@
+"""Host side of plan creation, checked without a GPU: the factor tables blml_amd.units builds, applied with the
+engine's conventions (a numpy model of the device passes: zero-padded power-of-two convolution for padded axes,
+band-restricted DFT for circular axes, dense spectrum for multi-column units), reproduce the oracle's dense Gram;
+K(0) matches the oracle's table; the pad rule and the library's exported symbols."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import blml_oracle as O
+from blml_amd import units as U
+from blml_amd import modes as M
+from blml_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _apply_axis_line(X, axis, fa):
+    X = np.moveaxis(X, axis, -1)
+    m = X.shape[-1]
+    if fa['circ'][0]:
+        F = fa['freqs'][0]
+        W = np.exp(-2j * np.pi * np.outer(F, np.arange(m)) / m)
+        Y = (X @ W.T) * fa['data']
+        out = Y @ np.conj(W)
+    else:
+        L = fa['tlen'][0]
+        j = np.arange(L)
+        row = np.zeros(L)
+        row[:m] = fa['data']
+        row[L - m + 1:] = fa['data'][1:][::-1]
+        Kf = np.real(np.fft.fft(row))
+        out = np.fft.ifft(np.fft.fft(X, n=L, axis=-1) * Kf, axis=-1)[..., :m]
+        assert np.allclose(row[(L - j) % L], row)                       # even row
+    return np.moveaxis(out, -1, axis)
+
+
+def _apply_group(X, fa, shape):
+    axes = fa['axes']
+    if all(fa['circ']):
+        Y = X
+        for a, ax in reversed(list(enumerate(axes))):
+            F = fa['freqs'][a]; m = shape[ax]
+            W = np.exp(-2j * np.pi * np.outer(F, np.arange(m)) / m)
+            Y = np.moveaxis(np.moveaxis(Y, ax, -1) @ W.T, -1, ax)
+        spec = fa['data'].reshape(fa['tlen'])
+        sh = [1] * X.ndim
+        for a, ax in enumerate(axes):
+            sh[ax] = fa['tlen'][a]
+        Y = Y * spec.reshape(sh)
+        for a, ax in enumerate(axes):
+            F = fa['freqs'][a]; m = shape[ax]
+            W = np.exp(-2j * np.pi * np.outer(F, np.arange(m)) / m)
+            Y = np.moveaxis(np.moveaxis(Y, ax, -1) @ np.conj(W), -1, ax)
+        return Y
+    T = fa['data'].reshape(fa['tlen'])
+    Kf = np.real(np.fft.fftn(T))
+    Ls = fa['tlen']
+    Y = np.fft.fftn(X, s=Ls, axes=axes)
+    sh = [1] * X.ndim
+    for a, ax in enumerate(axes):
+        sh[ax] = Ls[a]
+    Y = np.fft.ifftn(Y * Kf.reshape(sh), axes=axes)
+    sl = tuple(slice(0, shape[j]) for j in range(X.ndim))
+    return Y[sl]
+
+
+def apply_mode(facs, e, shape):
+    X = e.astype(complex)
+    for fa in facs:
+        X = _apply_axis_line(X, fa['axes'][0], fa) if fa['kind'] == 'line' else _apply_group(X, fa, shape)
+    return X
+
+
+CASES = [
+    ('1d sinc', (40,), [dict(fc=[0.35], kinds=['sinc'])], 1.0),
+    ('1d dirichlet circular', (32,), [dict(fc=[0.3], kinds=['dirichlet'], periods=[32.0])], 1.0),
+    ('1d dirichlet fill circular', (32,), [dict(fc=[0.6], kinds=['dirichlet:0.5'], periods=[32.0])], 1.0),
+    ('1d dirichlet wraps', (8,), [dict(fc=[1.3], kinds=['dirichlet'], periods=[8.0])], 1.0),
+    ('1d dirichlet long period', (32,), [dict(fc=[0.3], kinds=['dirichlet'], periods=[50.0])], 1.0),
+    ('2d sinc x dirichlet', (12, 16), [dict(fc=[0.4, 0.3], kinds=['sinc', 'dirichlet'], periods=[None, 16.0])], 1.0),
+    ('2d ball h=0.5', (11, 13), [dict(fc=[0.8, 0.8], kinds=['ball', 'ball'])], 0.5),
+    ('2d pball circular', (17, 23), [dict(fc=[0.4, 0.4], kinds=['pball', 'pball'], periods=[17.0, 23.0])], 1.0),
+    ('2d pball padded', (9, 10), [dict(fc=[0.4, 0.4], kinds=['pball', 'pball'], periods=[12.0, 15.0])], 1.0),
+    ('2d slits + dirichlet bg', (14, 19), [dict(fc=[0.35, 0.3], kinds=['slits:0.5', 'slits:0.4']),
+                                           dict(fc=[0.15, 0.15], kinds=['dirichlet', 'dirichlet'],
+                                                periods=[14.0, 19.0])], 1.0),
+    ('3d sinc', (6, 7, 5), [dict(fc=[0.5, 0.4, 0.3], kinds=['sinc'] * 3)], 1.0),
+    ('3d pball x sinc', (7, 9, 5), [dict(fc=[0.5, 0.5, 0.3], kinds=['pball', 'pball', 'sinc'],
+                                         periods=[7.0, 9.0, None])], 1.0),
+]
+
+
+@pytest.mark.parametrize('name,shape,mm,h', CASES, ids=[c[0] for c in CASES])
+def test_factors_reproduce_dense_gram(name, shape, mm, h):
+    modes = [md for kw in mm for md in M.make_modes(**kw)]
+    specs, _ = M.parse_modes(modes, len(shape))
+    ospecs = O.parse_modes([md for kw in mm for md in O.make_modes(**kw)], len(shape))
+    hv = np.full(len(shape), h)
+    circ = U.circular_dims(specs, shape, hv)
+    om = O.BLMLBinnedOracle()
+    om._setup(np.ones(shape), [md for kw in mm for md in O.make_modes(**kw)], h, 0.0)
+    assert tuple(m if c else U.next_fast_len5(2 * m - 1) for m, c in zip(shape, circ)) == om.pad_
+    rng = np.random.default_rng(7)
+    for k, (sp, osp) in enumerate(zip(specs, ospecs)):
+        s, facs = U.mode_factors(sp, shape, hv, circ)
+        assert s == om.s_[k]                                          # K(0) bit-identical to the oracle's T[0]
+        G = O.dense_gram(osp, shape, hv)
+        e = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        got = apply_mode(facs, e, shape).ravel()
+        want = G @ e.ravel()
+        assert np.abs(got - want).max() < 1e-12 * max(1.0, np.abs(G).sum(1).max()), name
+
+
+def test_parse_modes_errors_mirror_reference():
+    with pytest.raises(TypeError):
+        M.parse_modes({0: dict(kind='sinc', fc=0.3)}, 1)
+    with pytest.raises(ValueError):
+        M.parse_modes([{0: dict(kind='sinc', fc=-0.3)}], 1)
+    with pytest.raises(ValueError):
+        M.parse_modes([{0: dict(kind='slits', fc=0.3)}], 1)              # fill missing
+    with pytest.raises(ValueError):
+        M.parse_modes([{0: dict(kind='ball', fc=0.3)}], 1)               # multi-column kind on an int key
+    with pytest.raises(ValueError):
+        M.parse_modes([{0: dict(kind='sinc', fc=0.3)}], 2)               # column 1 missing
+    with pytest.raises(ValueError):
+        M.parse_modes([{0: dict(kind='dirichlet', fc=0.3)}], 1)          # period missing
+    with pytest.raises(ValueError):
+        M.parse_modes([{0: dict(kind='sinc', fc=0.3, bogus=1)}], 1)
+
+
+def test_make_modes_matches_oracle_tokens():
+    for kw in [dict(fc=[0.3, 0.2], kinds=['pball', 'pball'], periods=[31.0, 31.0]),
+               dict(fc=[[0.3, 0.2], [0.1, 0.1]], kinds=['slits:0.5', 'dirichlet:0.7'], periods=[None, 20.0]),
+               dict(fc=[0.4, 0.4, 0.2], kinds=['ball', 'ball', 'sinc'])]:
+        a, _ = M.parse_modes(M.make_modes(**kw), len(kw['kinds']))
+        b = O.parse_modes(O.make_modes(**kw), len(kw['kinds']))
+        for x, y in zip(a, b):
+            assert x['kinds'] == y['kinds'] and np.array_equal(x['fc'], y['fc']) and x['periods'] == y['periods']
+
+
+def _declared_functions():
+    src = open(os.path.join(ROOT, 'include', 'blml_hip.h')).read()
+    return sorted(set(re.findall(r'^\s*(?:int|const char \*)\s*(blml_\w+)\s*\(', src, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    path = _lib.LIB_PATH
+    if not os.path.exists(path):
+        pytest.skip('libblml_hip.so not built')
+    lib = ctypes.CDLL(path)
+    declared = _declared_functions()
+    assert set(declared) == set(_lib.EXPORTS)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_struct_layouts_match_header():
+    # sizes of the C structs as the header lays them out (natural alignment, x86-64 / amdgcn agree)
+    assert ctypes.sizeof(_lib.Factor) == 4 * 2 + 4 * 5 * 3 + 4 + 8 * 5 + 8 + 8
+    assert ctypes.sizeof(_lib.Result) == 8 * 2 + 4 * 8
