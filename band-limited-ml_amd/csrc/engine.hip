@@ -48,10 +48,25 @@ struct DevPlan {
     T *spec;
     const cpx<T> *tw;
     int Lmax;
-    T *ws;
+    unsigned char *ws;     // per-fit workspaces, byte offsets fit.ws_off
     int nfits;
     blml_params_t params;
 };
+
+// A fit's workspace (bytes from ws_off): [X, Y: scratch_len complex values of the transform type]
+// [pool: n_slots * Vmax FP64 state vectors][FP32 plans: X64, Y64, the finishing pass's FP64 scratch].
+__host__ __device__ inline long align256(long v) { return (v + 255) / 256 * 256; }
+__host__ __device__ inline int n_slots(const blml_params_t &P, const blml_fit_t &f) {
+    const int cd = 13 + 2 * (P.anderson + 1);
+    const int lb = 9 + 2 * (P.lbfgs_m + 1);
+    const bool psd = f.complex_g || f.K > 1;
+    return (P.solver == 1 && psd) ? (cd > lb ? cd : lb) : cd;
+}
+__host__ __device__ inline long pool_bytes_off(const blml_fit_t &f, int esize) { return align256(4L * f.scratch_len * esize); }
+__host__ __device__ inline long vmax_of(const blml_fit_t &f) { return (long)f.K * f.B * (f.complex_g ? 2 : 1); }
+__host__ __device__ inline long fin_bytes_off(const blml_params_t &P, const blml_fit_t &f, int esize) {
+    return align256(pool_bytes_off(f, esize) + 8L * n_slots(P, f) * vmax_of(f));
+}
 
 template <typename T>
 __global__ void twiddle_kernel(cpx<T> *tw, int Lmax) {
@@ -133,8 +148,7 @@ __device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, Matvec
     mv.freqs = p.freqs;
     mv.tw = p.tw;
     mv.Lmax = p.Lmax;
-    T *ws = p.ws + f.ws_off;
-    mv.X = (cpx<T> *)ws;
+    mv.X = (cpx<T> *)(p.ws + f.ws_off);
     mv.Y = mv.X + f.scratch_len;
     mv.lds = (cpx<T> *)lds;
 }
@@ -150,17 +164,10 @@ __global__ void __launch_bounds__(BLML_NT) matvec_kernel(DevPlan<T> p, const T *
     const int nc = f.complex_g ? 2 : 1;
     for (int k = 0; k < f.K; ++k) {
         const long off = f.out_off + (long)k * f.B * nc;
-        matvec_mode<T>(mv, f.fac_begin[k], f.fac_count[k], E + off, U + off, f.B, nc);
+        matvec_mode<T, T, T>(mv, f.fac_begin[k], f.fac_count[k], E + off, U + off, f.B, nc);
     }
 }
 
-template <typename T>
-__device__ int n_slots(const blml_params_t &P, const blml_fit_t &f) {
-    const int cd = 13 + 2 * (P.anderson + 1);
-    const int lb = 9 + 2 * (P.lbfgs_m + 1);
-    const bool psd = f.complex_g || f.K > 1;
-    return (P.solver == 1 && psd) ? (cd > lb ? cd : lb) : cd;
-}
 
 template <typename T>
 __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *order, int *queue,
@@ -192,7 +199,8 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
         S.sg = f.has_signs ? signs + f.signs_off : nullptr;
         const int ncf = f.complex_g ? 2 : 1;
         S.Vmax = (long)f.K * f.B * ncf;
-        S.pool = p.ws + f.ws_off + 4 * f.scratch_len;
+        S.pool = (double *)(p.ws + f.ws_off + pool_bytes_off(f, (int)sizeof(T)));
+        S.inc = sizeof(T) == 4;
         S.gstep = 0.0;
         S.merit_hist = merit_hist ? merit_hist + f.hist_off : nullptr;
         S.rho_hist = rho_hist ? rho_hist + f.hist_off : nullptr;
@@ -203,7 +211,7 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
         const double pn = p.params.phase_noise;
         const double *dr = draws + f.draws_off;
         int start_sweeps = 0;
-        T *E = S.vec(0);
+        double *E = S.vec(0);
         if (f.init_K > 0) {
             // warm start (blml_binned.py:403-408): given modes, zero off the occupied bins, noise in the others
             const double *E0 = init + f.init_off;
@@ -217,8 +225,8 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
                         re = E0[((long)k * f.B + b) * 2];
                         im = f.complex_g ? E0[((long)k * f.B + b) * 2 + 1] : 0.0;
                     }
-                    E[i] = (T)re;
-                    if (ncf == 2) E[i + 1] = (T)im;
+                    E[i] = re;
+                    if (ncf == 2) E[i + 1] = im;
                 }
             }
             __syncthreads();
@@ -229,10 +237,10 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
                                                : fabs((double)E[b]);
                     const long i = ((long)k * f.B + b) * ncf;
                     if (ncf == 2) {
-                        E[i] = (T)(pn * a0 * (dr[dp + b] / sqrt(2.0)));
-                        E[i + 1] = (T)(pn * a0 * (dr[dp + f.B + b] / sqrt(2.0)));
+                        E[i] = (pn * a0 * (dr[dp + b] / sqrt(2.0)));
+                        E[i + 1] = (pn * a0 * (dr[dp + f.B + b] / sqrt(2.0)));
                     } else {
-                        E[i] = (T)(pn * a0 * dr[dp + b]);
+                        E[i] = (pn * a0 * dr[dp + b]);
                     }
                 }
                 dp += (long)f.B * ncf;
@@ -242,7 +250,7 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
             // cold start: e0 = sqrt(c / s0) on occupied bins (times the signs)
             for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
                 const double sgb = S.sg ? S.sg[b] : 1.0;
-                E[b] = S.w[b] > 0 ? (T)(sqrt(f.n * S.w[b] / f.s[0]) * sgb) : (T)0;
+                E[b] = S.w[b] > 0 ? sqrt(f.n * S.w[b] / f.s[0]) * sgb : 0.0;
             }
             __syncthreads();
             if (psd_full) {
@@ -251,16 +259,16 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
                 const FitOut so = S.solve_cd();
                 start_sweeps = so.n_sweeps;
                 // phases and the other modes (blml_binned.py:429-436), assembled in slot 2 then moved to slot 0
-                T *E2 = S.vec(2);
-                const T *er = S.vec(0);
+                double *E2 = S.vec(2);
+                const double *er = S.vec(0);
                 long dp = 0;
                 for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
                     if (f.complex_g) {
                         double sn, cs;
                         sincos(pn * dr[b], &sn, &cs);
                         const double r = er[b];
-                        E2[2 * b] = (T)(r * cs);
-                        E2[2 * b + 1] = (T)(r * sn);
+                        E2[2 * b] = (r * cs);
+                        E2[2 * b + 1] = (r * sn);
                     } else {
                         E2[b] = er[b];
                     }
@@ -274,10 +282,10 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
                             : fabs((double)E2[b]);
                         const long i = ((long)k * f.B + b) * ncf;
                         if (ncf == 2) {
-                            E2[i] = (T)(pn * a0 * (dr[dp + b] / sqrt(2.0)));
-                            E2[i + 1] = (T)(pn * a0 * (dr[dp + f.B + b] / sqrt(2.0)));
+                            E2[i] = (pn * a0 * (dr[dp + b] / sqrt(2.0)));
+                            E2[i + 1] = (pn * a0 * (dr[dp + f.B + b] / sqrt(2.0)));
                         } else {
-                            E2[i] = (T)(pn * a0 * dr[dp + b]);
+                            E2[i] = (pn * a0 * dr[dp + b]);
                         }
                     }
                     dp += (long)f.B * ncf;
@@ -294,11 +302,11 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
         S.nhist_m = S.nhist_r = 0;
         const FitOut o = (p.params.solver == 1 && psd_full) ? S.solve_lbfgs() : S.solve_cd();
         // outputs (blml_binned.py:438-445)
-        const T *Ef = S.vec(0), *Uf = S.vec(1);
+        const double *Ef = S.vec(0), *Uf = S.vec(1);
         const long V = (long)f.K * f.B * ncf;
         for (long i = threadIdx.x; i < V; i += BLML_NT) {
-            e_out[f.out_off + i] = Ef[i];
-            se_out[f.out_off + i] = Uf[i];
+            e_out[f.out_off + i] = (T)Ef[i];
+            se_out[f.out_off + i] = (T)Uf[i];
         }
         double ll = 0.0;
         const double n2 = f.n * f.n;
@@ -344,9 +352,8 @@ __global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<T> p, const T *e_
     make_matvec_ctx<T>(p, f, mv, lds_raw);
     const int nc = f.complex_g ? 2 : 1;
     const int B = f.B;
-    T *pool = p.ws + f.ws_off + 4 * f.scratch_len;
-    double *dh = (double *)pool;                      // B doubles
-    T *x = (T *)(dh + B), *y = x + B;
+    double *dh = (double *)(p.ws + f.ws_off + pool_bytes_off(f, (int)sizeof(T)));   // B doubles
+    double *x = dh + B, *y = x + B;
     const double *w = counts + f.counts_off;
     const T *E = e_hat + f.out_off;
     for (int b = threadIdx.x; b < B; b += BLML_NT) {
@@ -368,14 +375,14 @@ __global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<T> p, const T *e_
         if (threadIdx.x == 0) { lam_out[fi] = 0.0; iters[fi] = 0; }
         return;
     }
-    for (int b = threadIdx.x; b < B; b += BLML_NT) x[b] = (T)(dh[b] / nrm);
+    for (int b = threadIdx.x; b < B; b += BLML_NT) x[b] = dh[b] / nrm;
     __syncthreads();
     double lam = 0.0, lam_prev = -INFINITY;
     int it = 0;
     for (it = 1; it <= max_iter; ++it) {
-        for (int b = threadIdx.x; b < B; b += BLML_NT) y[b] = (T)(dh[b] * (double)x[b]);
+        for (int b = threadIdx.x; b < B; b += BLML_NT) y[b] = dh[b] * x[b];
         __syncthreads();
-        matvec_mode<T>(mv, f.fac_begin[0], f.fac_count[0], y, y, B, 1);
+        matvec_mode<T, double, double>(mv, f.fac_begin[0], f.fac_count[0], y, y, B, 1);
         double a[2] = {0.0, 0.0};
         for (int b = threadIdx.x; b < B; b += BLML_NT) {
             const double yb = dh[b] * (double)y[b];
@@ -389,7 +396,7 @@ __global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<T> p, const T *e_
         lam = a[0] / xx;
         const double ny = sqrt(a[1]);
         if (!(ny > 0)) break;
-        for (int b = threadIdx.x; b < B; b += BLML_NT) x[b] = (T)(dh[b] * (double)y[b] / ny);
+        for (int b = threadIdx.x; b < B; b += BLML_NT) x[b] = dh[b] * y[b] / ny;
         __syncthreads();
         if (fabs(lam - lam_prev) <= tol * fabs(lam)) break;
         lam_prev = lam;
@@ -398,6 +405,55 @@ __global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<T> p, const T *e_
         lam_out[fi] = lam / f.n;
         iters[fi] = it > max_iter ? max_iter : it;
     }
+}
+
+// FP32 plans only: Se, density and loglik of every fit re-evaluated with an FP64 mat-vec of the FP32-converged
+// coefficients (the iterations run in FP32; the outputs' likelihood sum is FP64 end to end).  p carries the FP64
+// spectra and twiddles; its workspace is the FP32 plan's, reinterpreted (sized for it by layout()).
+__global__ void __launch_bounds__(BLML_NT) finish64_kernel(DevPlan<double> p, float *se_out,
+                                                          const double *counts, double *density,
+                                                          blml_result_t *results) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[BLML_NWAVE * BLML_NRED];
+    const int fi = blockIdx.x;
+    if (fi >= p.nfits) return;
+    const blml_fit_t f = p.fits[fi];
+    MatvecCtx<double> mv;
+    mv.d = f.d;
+    for (int j = 0; j < BLML_MAXD; ++j) mv.shape[j] = j < f.d ? f.shape[j] : 1;
+    mv.facs = p.facs;
+    mv.spec = p.spec;
+    mv.freqs = p.freqs;
+    mv.tw = p.tw;
+    mv.Lmax = p.Lmax;
+    mv.X = (cpx<double> *)(p.ws + f.ws_off + fin_bytes_off(p.params, f, 4));
+    mv.Y = mv.X + f.scratch_len;
+    mv.lds = (cpx<double> *)lds_raw;
+    const int nc = f.complex_g ? 2 : 1;
+    const long V = (long)f.K * f.B * nc;
+    // the fit's FP64 state: E in pool slot 0; U64 overwrites slot 1
+    const double *E64 = (const double *)(p.ws + f.ws_off + pool_bytes_off(f, 4));
+    double *U64 = (double *)E64 + vmax_of(f);
+    for (int k = 0; k < f.K; ++k) {
+        const long off = (long)k * f.B * nc;
+        matvec_mode<double, double, double>(mv, f.fac_begin[k], f.fac_count[k], E64 + off, U64 + off, f.B, nc);
+    }
+    for (long i = threadIdx.x; i < V; i += BLML_NT) se_out[f.out_off + i] = (float)U64[i];
+    const double *w = counts + f.counts_off;
+    const double n2 = f.n * f.n;
+    double ll = 0.0;
+    for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
+        double F = 0.0;
+        for (int k = 0; k < f.K; ++k) {
+            const long i = ((long)k * f.B + b) * nc;
+            F += U64[i] * U64[i] + (nc == 2 ? U64[i + 1] * U64[i + 1] : 0.0);
+        }
+        const double dens = F / n2;
+        density[f.dens_off + b] = dens;
+        if (w[b] > 0) ll += w[b] * log(dens);
+    }
+    ll = block_sum1(ll, red);
+    if (threadIdx.x == 0) results[fi].loglik = ll / f.n;
 }
 
 // ------------------------------------------------------------------------------------------------ host plan
@@ -413,6 +469,9 @@ struct Plan {
     int *d_freqs = nullptr;
     void *d_spec = nullptr;
     void *d_tw = nullptr;
+    void *d_spec64 = nullptr;   // FP32 plans: FP64 spectra and twiddles for the finishing pass
+    void *d_tw64 = nullptr;
+    int lds64_bytes = 0;
     void *d_ws = nullptr;
     int *d_order = nullptr;
     int *d_queue = nullptr;
@@ -440,7 +499,23 @@ DevPlan<T> dev_plan(const Plan &pl) {
     p.spec = (T *)pl.d_spec;
     p.tw = (const cpx<T> *)pl.d_tw;
     p.Lmax = pl.Lmax;
-    p.ws = (T *)pl.d_ws;
+    p.ws = (unsigned char *)pl.d_ws;
+    p.nfits = pl.nfits;
+    p.params = pl.params;
+    return p;
+}
+
+// the FP64 view of an FP32 plan used by the finishing pass
+DevPlan<double> dev_plan64(const Plan &pl) {
+    DevPlan<double> p;
+    p.fits = pl.d_fits;
+    p.facs = pl.d_facs;
+    p.fdata = pl.d_fdata;
+    p.freqs = pl.d_freqs;
+    p.spec = (double *)pl.d_spec64;
+    p.tw = (const cpx<double> *)pl.d_tw64;
+    p.Lmax = pl.Lmax;
+    p.ws = (unsigned char *)pl.d_ws;
     p.nfits = pl.nfits;
     p.params = pl.params;
     return p;
@@ -552,15 +627,12 @@ int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs,
             }
         }
         f.scratch_len = scratch;
-        const int ncf = f.complex_g ? 2 : 1;
-        const long Vmax = (long)f.K * f.B * ncf;
-        const int cd = 13 + 2 * (P.anderson + 1), lb = 9 + 2 * (P.lbfgs_m + 1);
-        const bool psd = f.complex_g || f.K > 1;
-        const int nsl = (P.solver == 1 && psd) ? std::max(cd, lb) : cd;
-        long len = 4 * scratch + (long)nsl * Vmax;
-        // the certificate reuses the pool: B doubles + 2 B values
-        len = std::max(len, 4 * scratch + 4L * f.B + 2L * f.B);
-        len = (len + 63) / 64 * 64;
+        const long Vmax = vmax_of(f);
+        // bytes: transform scratch, FP64 state pool (also the certificate's 3 B doubles: n_slots >= 3), and for
+        // FP32 plans the finishing pass's FP64 scratch
+        long len = fin_bytes_off(P, f, (int)pl.esize);
+        if (pl.esize == 4) len += 32L * scratch;
+        len = align256(len);
         f.ws_off = ws;
         f.ws_len = len;
         ws += len;
@@ -575,6 +647,10 @@ int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs,
         return fail(BLML_EVALUE, "a transform line needs " + std::to_string(lds) + " bytes of LDS (limit " +
                                      std::to_string(kMaxLds - kStaticLds) + "): lattice axis too long");
     pl.lds_bytes = lds;
+    pl.lds64_bytes = lds_elems * 16;
+    if (pl.esize == 4 && pl.lds64_bytes + kStaticLds > kMaxLds)
+        return fail(BLML_EVALUE, "the FP64 finishing pass of a line needs " + std::to_string(pl.lds64_bytes) +
+                                     " bytes of LDS: lattice axis too long");
     return BLML_OK;
 }
 
@@ -592,7 +668,7 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
     HIPCHECK(hipMalloc(&pl.d_freqs, sizeof(int) * std::max<int64_t>(1, n_freqs)));
     HIPCHECK(hipMalloc(&pl.d_spec, sizeof(T) * std::max<int64_t>(1, spec_total)));
     HIPCHECK(hipMalloc(&pl.d_tw, sizeof(cpx<T>) * (pl.Lmax / 2)));
-    HIPCHECK(hipMalloc(&pl.d_ws, sizeof(T) * std::max<int64_t>(1, pl.ws_elems)));
+    HIPCHECK(hipMalloc(&pl.d_ws, std::max<int64_t>(256, pl.ws_elems)));
     HIPCHECK(hipMalloc(&pl.d_order, sizeof(int) * std::max(1, pl.nfits)));
     HIPCHECK(hipMalloc(&pl.d_queue, sizeof(int)));
     HIPCHECK(hipMemcpy(pl.d_fits, pl.hfits.data(), sizeof(blml_fit_t) * pl.nfits, hipMemcpyHostToDevice));
@@ -621,6 +697,21 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
         spec_kernel<T><<<pl.nfacs, BLML_NT, pl.lds_bytes>>>(dev_plan<T>(pl), pl.nfacs, d_fac_m, d_tmp, d_tmp_off);
         HIPCHECK(hipGetLastError());
     }
+    if (sizeof(T) == 4) {
+        HIPCHECK(hipMalloc(&pl.d_spec64, sizeof(double) * std::max<int64_t>(1, spec_total)));
+        HIPCHECK(hipMalloc(&pl.d_tw64, sizeof(cpx<double>) * (pl.Lmax / 2)));
+        twiddle_kernel<double><<<(pl.Lmax / 2 + 255) / 256, 256>>>((cpx<double> *)pl.d_tw64, pl.Lmax);
+        HIPCHECK(hipGetLastError());
+        cpx<double> *d_tmp64 = nullptr;
+        HIPCHECK(hipMalloc(&d_tmp64, sizeof(cpx<double>) * std::max<long>(1, tmp_total)));
+        if (pl.nfacs > 0) {
+            spec_kernel<double><<<pl.nfacs, BLML_NT, pl.lds64_bytes>>>(dev_plan64(pl), pl.nfacs, d_fac_m, d_tmp64,
+                                                                       d_tmp_off);
+            HIPCHECK(hipGetLastError());
+        }
+        HIPCHECK(hipDeviceSynchronize());
+        HIPCHECK(hipFree(d_tmp64));
+    }
     HIPCHECK(hipDeviceSynchronize());
     HIPCHECK(hipFree(d_fac_m));
     HIPCHECK(hipFree(d_tmp_off));
@@ -637,8 +728,8 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
 
 void free_plan(Plan *pl) {
     if (!pl) return;
-    void *ptrs[] = {pl->d_fits, pl->d_facs, pl->d_fdata, pl->d_freqs, pl->d_spec,
-                    pl->d_tw,   pl->d_ws,   pl->d_order, pl->d_queue};
+    void *ptrs[] = {pl->d_fits, pl->d_facs, pl->d_fdata, pl->d_freqs, pl->d_spec, pl->d_tw,
+                    pl->d_spec64, pl->d_tw64, pl->d_ws, pl->d_order, pl->d_queue};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete pl;
@@ -684,7 +775,7 @@ int blml_plan_info(void *plan, int64_t *out_elems, int64_t *ws_bytes, int32_t *l
     Plan *pl = (Plan *)plan;
     if (!pl) return fail(BLML_EVALUE, "null plan");
     if (out_elems) *out_elems = pl->out_elems;
-    if (ws_bytes) *ws_bytes = pl->ws_elems * (int64_t)pl->esize;
+    if (ws_bytes) *ws_bytes = pl->ws_elems;
     if (lds_bytes) *lds_bytes = pl->lds_bytes;
     if (grid) *grid = pl->grid;
     return BLML_OK;
@@ -715,14 +806,18 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
     }
     hipStream_t st = (hipStream_t)stream;
     HIPCHECK(hipMemsetAsync(pl->d_queue, 0, sizeof(int), st));
-    if (pl->dtype == BLML_F32)
+    if (pl->dtype == BLML_F32) {
         fit_kernel<float><<<pl->grid, BLML_NT, pl->lds_bytes, st>>>(
             dev_plan<float>(*pl), pl->d_order, pl->d_queue, counts, draws, init, signs, (float *)e_out,
             (float *)se_out, density, results, merit_hist, rho_hist);
-    else
+        HIPCHECK(hipGetLastError());
+        finish64_kernel<<<pl->nfits, BLML_NT, pl->lds64_bytes, st>>>(dev_plan64(*pl), (float *)se_out, counts,
+                                                                     density, results);
+    } else {
         fit_kernel<double><<<pl->grid, BLML_NT, pl->lds_bytes, st>>>(
             dev_plan<double>(*pl), pl->d_order, pl->d_queue, counts, draws, init, signs, (double *)e_out,
             (double *)se_out, density, results, merit_hist, rho_hist);
+    }
     HIPCHECK(hipGetLastError());
     return BLML_OK;
 }

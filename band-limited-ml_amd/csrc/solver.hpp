@@ -1,12 +1,20 @@
 // One binned fit, solved start to finish by one workgroup: the start (blml_binned.py:415-436), Jacobi coordinate
 // descent with Anderson mixing, damping and gradient fallback (blml_binned.py:223-276, 357-380, 1463-1478), or
-// L-BFGS with Armijo backtracking and the canonical form (blml_binned.py:278-355), and the outputs
-// (blml_binned.py:438-445).  Vectors are stored in T (FP32 or FP64); every reduction (log-likelihood, Q, dot
-// products, curvature s.y / y.y, residual max) accumulates in FP64.  The scalar control flow is executed
-// identically by every thread: it only depends on reduction results that all threads hold bit-identically.
+// L-BFGS with Armijo backtracking and the canonical form (blml_binned.py:278-355).
+//
+// State vectors (E, U, gradients, histories) are FP64; the transforms of the mat-vec run in T (FP32 or FP64) and
+// every reduction accumulates in FP64.  Two mat-vec modes:
+//   exact (FP64 plans): U_new = S E_new recomputed from scratch wherever the reference does, so the iteration is the
+//         reference's step for step (up to FFT rounding);
+//   incremental (FP32 plans): U_new = U + S (E_new - E), and across line-search trials U + t S q with one S q per
+//         iteration.  The FP32 rounding then scales with the increment, not with U, so the likelihoods the solver
+//         compares stay consistent to FP64 accuracy (a fresh FP32 S E carries an absolute error ~1e-7 |U|, which puts
+//         noise of ~n 1e-7 into L = sum w log F - n log Q and defeats the Armijo and merit tests).
+// The scalar control flow is executed identically by every thread: it depends only on reduction results that all
+// threads hold bit-identically.
 #pragma once
-#include <math.h>
 #include <limits.h>
+#include <math.h>
 #include "transform.hpp"
 
 #define BLML_MAXHIST 64      // L-BFGS memory limit
@@ -18,7 +26,6 @@ struct SolverLds {
     double red[BLML_NWAVE * BLML_NRED];
     double ring[BLML_RING];
     double lb_r[BLML_MAXHIST + 1], lb_a[BLML_MAXHIST + 1], lb_sy[BLML_MAXHIST + 1], lb_yy[BLML_MAXHIST + 1];
-    double and_g[BLML_MAXAND];
 };
 
 struct FitOut {
@@ -65,12 +72,13 @@ struct FitSolver {
     SolverLds *sl;
     int B, K, nc;          // active: the real start runs K = 1, nc = 1
     bool psd;
+    bool inc;              // incremental mat-vecs (FP32 transforms)
     double n;
     double s[BLML_MAXK];
     bool equal_s;
     const double *w;
     const double *sg;      // signs or null
-    T *pool;
+    double *pool;
     long Vmax;
     double gstep;          // the gradient fallback's step, kept between calls (0: unset)
     double *merit_hist, *rho_hist;
@@ -78,58 +86,68 @@ struct FitSolver {
     int ring_n;            // entries pushed to the merit ring
     bool rec;              // record merit / rho histories (the final solve only, as merit_history_ is)
 
-    __device__ T *vec(int i) const { return pool + (long)i * Vmax; }
+    __device__ double *vec(int i) const { return pool + (long)i * Vmax; }
     __device__ long V() const { return (long)K * B * nc; }
 
-    // ------------------------------------------------------------------ basic vector operations
-    __device__ void Sall(const T *E, T *U) {
-        for (int k = 0; k < K; ++k)
-            matvec_mode<T>(mv, f->fac_begin[k], f->fac_count[k], E + (long)k * B * nc, U + (long)k * B * nc, B,
-                           nc);
+    // ------------------------------------------------------------------ mat-vecs
+    __device__ void Sall(const double *E, double *U) {
+        for (int k = 0; k < K; ++k) {
+            const long o = (long)k * B * nc;
+            matvec_mode<T, double, double>(mv, f->fac_begin[k], f->fac_count[k], E + o, U + o, B, nc);
+        }
     }
-    __device__ double dot(const T *a, const T *b) {
+    // U_new = S E_new: recomputed (exact mode) or U + S (E_new - E) (incremental mode)
+    __device__ void Snew(const double *En, double *Un, const double *E, const double *U) {
+        if (!inc) { Sall(En, Un); return; }
+        for (int k = 0; k < K; ++k) {
+            const long o = (long)k * B * nc;
+            matvec_mode<T, double, double>(mv, f->fac_begin[k], f->fac_count[k], En + o, Un + o, B, nc, E + o,
+                                           U + o);
+        }
+    }
+
+    // ------------------------------------------------------------------ basic vector operations
+    __device__ double dot(const double *a, const double *b) {
         double acc = 0.0;
         const long n_ = V();
-        for (long i = threadIdx.x; i < n_; i += BLML_NT) acc += (double)a[i] * (double)b[i];
+        for (long i = threadIdx.x; i < n_; i += BLML_NT) acc += a[i] * b[i];
         return block_sum1(acc, sl->red);
     }
-    __device__ void copy(T *dst, const T *src) {
+    __device__ void copy(double *dst, const double *src) {
         const long n_ = V();
         for (long i = threadIdx.x; i < n_; i += BLML_NT) dst[i] = src[i];
         __syncthreads();
     }
-    __device__ void scale(T *a, double sc) {
+    __device__ void scale(double *a, double sc) {
         const long n_ = V();
-        for (long i = threadIdx.x; i < n_; i += BLML_NT) a[i] = (T)((double)a[i] * sc);
+        for (long i = threadIdx.x; i < n_; i += BLML_NT) a[i] = a[i] * sc;
         __syncthreads();
     }
     // out = a + t * b
-    __device__ void axpy(T *out, const T *a, double t, const T *b) {
+    __device__ void axpy(double *out, const double *a, double t, const double *b) {
         const long n_ = V();
-        for (long i = threadIdx.x; i < n_; i += BLML_NT) out[i] = (T)((double)a[i] + t * (double)b[i]);
+        for (long i = threadIdx.x; i < n_; i += BLML_NT) out[i] = a[i] + t * b[i];
         __syncthreads();
     }
     __device__ bool occ(int b) const { return w[b] > 0; }
     __device__ double cb(int b) const { return n * w[b]; }
-    __device__ double absq(const T *X, int k, int b) const {
+    __device__ double absq(const double *X, int k, int b) const {
         const long i = ((long)k * B + b) * nc;
-        return nc == 2 ? (double)X[i] * X[i] + (double)X[i + 1] * X[i + 1] : (double)X[i] * X[i];
+        return nc == 2 ? X[i] * X[i] + X[i + 1] * X[i + 1] : X[i] * X[i];
     }
-    __device__ double Fb(const T *U, int b) const {
+    __device__ double Fb(const double *U, int b) const {
         double F = 0.0;
         for (int k = 0; k < K; ++k) F += absq(U, k, b);
         return F;
     }
 
     // ------------------------------------------------------------------ merit, update, rescale, residual
-    __device__ double merit(const T *E, const T *U) {
+    __device__ double merit(const double *E, const double *U) {
         if (psd) {
             double a[2] = {0.0, 0.0};
-            for (int b = threadIdx.x; b < B; b += BLML_NT) {
-                a[0] += cb(b) * log(fmax(Fb(U, b), 1e-30));
-            }
+            for (int b = threadIdx.x; b < B; b += BLML_NT) a[0] += cb(b) * log(fmax(Fb(U, b), 1e-30));
             const long n_ = V();
-            for (long i = threadIdx.x; i < n_; i += BLML_NT) a[1] += (double)E[i] * (double)U[i];
+            for (long i = threadIdx.x; i < n_; i += BLML_NT) a[1] += E[i] * U[i];
             block_sum<2>(a, sl->red);
             const double Q = a[1];
             if (!(Q > 0)) return -INFINITY;
@@ -150,14 +168,14 @@ struct FitSolver {
         return -(0.5 * a[0] - a[1]) / n;
     }
 
-    __device__ void update(const T *E, const T *U, T *out) {
+    __device__ void update(const double *E, const double *U, double *out) {
         if (!psd) {
             const double s0 = s[0];
             for (int b = threadIdx.x; b < B; b += BLML_NT) {
                 const double sgb = sg ? sg[b] : 1.0;
-                const double R = (double)U[b] - s0 * (double)E[b];
+                const double R = U[b] - s0 * E[b];
                 const double t = occ(b) ? cd_root(sgb * R, cb(b), s0) : 0.0;
-                out[b] = (T)(sgb * t);
+                out[b] = sgb * t;
             }
             __syncthreads();
             return;
@@ -167,55 +185,54 @@ struct FitSolver {
             double a2 = 0.0;
             for (int k = 0; k < K; ++k) {
                 const long i = ((long)k * B + b) * nc;
-                Rr[k] = (double)U[i] - s[k] * (double)E[i];
-                Ri[k] = nc == 2 ? (double)U[i + 1] - s[k] * (double)E[i + 1] : 0.0;
+                Rr[k] = U[i] - s[k] * E[i];
+                Ri[k] = nc == 2 ? U[i + 1] - s[k] * E[i + 1] : 0.0;
                 R2[k] = Rr[k] * Rr[k] + Ri[k] * Ri[k];
                 a2 += R2[k];
             }
             const double c = cb(b);
             const double a = sqrt(a2);
-            double fr[BLML_MAXK], fi[BLML_MAXK];
             if (equal_s) {
                 const double s0 = s[0];
                 const double t = 2 * c / (a + sqrt(a2 + 4 * s0 * c));
                 const double g = t / (a > 0 ? a : 1.0);
-                for (int k = 0; k < K; ++k) { fr[k] = Rr[k] * g; fi[k] = Ri[k] * g; }
-                if (a == 0 && c > 0) { fr[0] = sqrt(c / s0); fi[0] = 0.0; }
+                for (int k = 0; k < K; ++k) { Rr[k] *= g; Ri[k] *= g; }
+                if (a == 0 && c > 0) { Rr[0] = sqrt(c / s0); Ri[0] = 0.0; }
             } else {
                 const bool lone = (a2 == 0) && (c > 0);
                 const double lam = secular_root(R2, s, K, c);
                 for (int k = 0; k < K; ++k) {
                     if (c > 0 && !lone) {
                         const double dn = lam - s[k];
-                        fr[k] = Rr[k] / dn; fi[k] = Ri[k] / dn;
+                        Rr[k] /= dn; Ri[k] /= dn;
                     } else {
-                        fr[k] = 0.0; fi[k] = 0.0;
+                        Rr[k] = 0.0; Ri[k] = 0.0;
                     }
                 }
-                if (lone) { fr[0] = sqrt(c / s[0]); fi[0] = 0.0; }
+                if (lone) { Rr[0] = sqrt(c / s[0]); Ri[0] = 0.0; }
             }
             for (int k = 0; k < K; ++k) {
                 const long i = ((long)k * B + b) * nc;
-                out[i] = (T)fr[k];
-                if (nc == 2) out[i + 1] = (T)fi[k];
+                out[i] = Rr[k];
+                if (nc == 2) out[i + 1] = Ri[k];
             }
         }
         __syncthreads();
     }
 
-    __device__ void rescale(T *E, T *U) {
+    __device__ void rescale(double *E, double *U) {
         const double Q = dot(E, U);
         const double sc = Q > 0 ? n / sqrt(Q) : 1.0;
         const long n_ = V();
         for (long i = threadIdx.x; i < n_; i += BLML_NT) {
-            E[i] = (T)((double)E[i] * sc);
-            U[i] = (T)((double)U[i] * sc);
+            E[i] *= sc;
+            U[i] *= sc;
         }
         __syncthreads();
     }
 
     // max over occupied bins of ||u_b (|e_b|^2 / c_b) - e_b|| / ||e_b|| of the rescaled pair (sc E, sc U)
-    __device__ double resid(const T *E, const T *U, double sc) {
+    __device__ double resid(const double *E, const double *U, double sc) {
         double r = 0.0;
         for (int b = threadIdx.x; b < B; b += BLML_NT) {
             if (!occ(b)) continue;
@@ -227,7 +244,7 @@ struct FitSolver {
             for (int k = 0; k < K; ++k) {
                 const long i = ((long)k * B + b) * nc;
                 for (int q = 0; q < nc; ++q) {
-                    const double v = sc * (double)U[i + q] * fac - sc * (double)E[i + q];
+                    const double v = sc * U[i + q] * fac - sc * E[i + q];
                     acc += v * v;
                 }
             }
@@ -236,10 +253,9 @@ struct FitSolver {
         return block_max(r, sl->red);
     }
 
-    __device__ double Qof(const T *E, const T *U) { return dot(E, U); }
-
-    // L = sum w log F - n log Q and (want_grad) g = 2 (S(w U / F) - n U / Q) on occupied bins.  Vt, SVt: scratch.
-    __device__ double lgrad(const T *E, const T *U, T *g, T *Vt, T *SVt, bool want_grad) {
+    // L = sum w log F - n log Q (blml_binned.py:278-284); writes Vt = (w / F) U on occupied bins when Vt is given
+    // (the gradient's mat-vec input) and Q to *Qout
+    __device__ double lval(const double *E, const double *U, double *Vt, double *Qout) {
         double a[2] = {0.0, 0.0};
         for (int b = threadIdx.x; b < B; b += BLML_NT) {
             const bool o = occ(b);
@@ -249,30 +265,28 @@ struct FitSolver {
             for (int k = 0; k < K; ++k) {
                 const long i = ((long)k * B + b) * nc;
                 for (int q = 0; q < nc; ++q) {
-                    a[1] += (double)E[i + q] * (double)U[i + q];
-                    Vt[i + q] = (T)(fac * (double)U[i + q]);
+                    a[1] += E[i + q] * U[i + q];
+                    if (Vt) Vt[i + q] = fac * U[i + q];
                 }
             }
         }
         block_sum<2>(a, sl->red);
-        const double Q = a[1];
-        if (!(Q > 0)) return -INFINITY;
-        const double L = a[0] - n * log(Q);
-        if (!want_grad) return L;
+        *Qout = a[1];
+        if (!(a[1] > 0)) return -INFINITY;
+        return a[0] - n * log(a[1]);
+    }
+    // g = 2 (S Vt - n U / Q) on occupied bins (blml_binned.py:285-288); SVt: scratch
+    __device__ void lgrad_from(const double *U, double Q, const double *Vt, double *SVt, double *g) {
         Sall(Vt, SVt);
         const double nq = n / Q;
         for (int b = threadIdx.x; b < B; b += BLML_NT) {
             const bool o = occ(b);
             for (int k = 0; k < K; ++k) {
                 const long i = ((long)k * B + b) * nc;
-                for (int q = 0; q < nc; ++q) {
-                    const T G = (T)((double)SVt[i + q] - nq * (double)U[i + q]);
-                    g[i + q] = o ? (T)(2.0 * (double)G) : (T)0;
-                }
+                for (int q = 0; q < nc; ++q) g[i + q] = o ? 2.0 * (SVt[i + q] - nq * U[i + q]) : 0.0;
             }
         }
         __syncthreads();
-        return L;
     }
 
     // ------------------------------------------------------------------ histories
@@ -292,15 +306,16 @@ struct FitSolver {
     }
 
     // ------------------------------------------------------------------ Anderson type II
-    // hX / hF: slot arrays (ring of cap = mA + 1, ordered from head); returns false on a singular history.
-    __device__ bool anderson(const int *slotX, const int *slotF, int cnt, const T *E, const T *f, T *EA) {
+    // slotX / slotF: the history slots in order (oldest first); returns false on a singular history.
+    __device__ bool anderson(const int *slotX, const int *slotF, int cnt, const double *E, const double *f,
+                             double *EA) {
         const int m = cnt - 1;
         double acc[20];
         for (int i = 0; i < 20; ++i) acc[i] = 0.0;
         const long n_ = V();
         for (long e = threadIdx.x; e < n_; e += BLML_NT) {
             double dF[BLML_MAXAND];
-            for (int i = 0; i < m; ++i) dF[i] = (double)(T)(vec(slotF[i + 1])[e] - vec(slotF[i])[e]);
+            for (int i = 0; i < m; ++i) dF[i] = vec(slotF[i + 1])[e] - vec(slotF[i])[e];
             const double fe = f[e];
             int p = 0;
             for (int i = 0; i < m; ++i)
@@ -339,11 +354,11 @@ struct FitSolver {
         for (long e = threadIdx.x; e < n_; e += BLML_NT) {
             double mix = 0.0;
             for (int i = 0; i < m; ++i) {
-                const double dX = (double)vec(slotX[i + 1])[e] - (double)vec(slotX[i])[e];
-                const double dF = (double)vec(slotF[i + 1])[e] - (double)vec(slotF[i])[e];
+                const double dX = vec(slotX[i + 1])[e] - vec(slotX[i])[e];
+                const double dF = vec(slotF[i + 1])[e] - vec(slotF[i])[e];
                 mix += gam[i] * (dX + dF);
             }
-            EA[e] = (T)((double)E[e] + (double)f[e] - mix);
+            EA[e] = E[e] + f[e] - mix;
         }
         __syncthreads();
         return true;
@@ -351,36 +366,20 @@ struct FitSolver {
 
     // ------------------------------------------------------------------ gradient fallback (psd)
     // From (E, U, cur): up to 5 Armijo-backtracked steepest-ascent steps on L; result in (Eo, Uo), merit returned.
-    __device__ double grad_steps(const T *E0, const T *U0, double cur, T *Eo, T *Uo, T *Vb, T *Gb, T *SGb,
-                                 T *En, T *Un) {
+    __device__ double grad_steps(const double *E0, const double *U0, double cur, double *Eo, double *Uo, double *Vb,
+                                 double *Gb, double *SGb, double *En, double *Un) {
         copy(Eo, E0);
         copy(Uo, U0);
         for (int st = 0; st < 5; ++st) {
-            double a2[2] = {0.0, 0.0};
-            for (int b = threadIdx.x; b < B; b += BLML_NT) {
-                const bool o = occ(b);
-                const double F = Fb(Uo, b);
-                const double fac = o ? w[b] / (o ? F : 1.0) : 0.0;
-                for (int k = 0; k < K; ++k) {
-                    const long i = ((long)k * B + b) * nc;
-                    for (int q = 0; q < nc; ++q) {
-                        a2[0] += (double)Eo[i + q] * (double)Uo[i + q];
-                        Vb[i + q] = (T)(fac * (double)Uo[i + q]);
-                    }
-                }
-            }
-            block_sum<2>(a2, sl->red);
-            const double Q = a2[0];
+            double Q = 0.0;
+            lval(Eo, Uo, Vb, &Q);
             Sall(Vb, Gb);
             const double nq = n / Q;
             for (int b = threadIdx.x; b < B; b += BLML_NT) {
                 const bool o = occ(b);
                 for (int k = 0; k < K; ++k) {
                     const long i = ((long)k * B + b) * nc;
-                    for (int q = 0; q < nc; ++q) {
-                        const T G = (T)((double)Gb[i + q] - nq * (double)Uo[i + q]);
-                        Gb[i + q] = o ? G : (T)0;
-                    }
+                    for (int q = 0; q < nc; ++q) Gb[i + q] = o ? Gb[i + q] - nq * Uo[i + q] : 0.0;
                 }
             }
             __syncthreads();
@@ -415,7 +414,6 @@ struct FitSolver {
         const int iEP = 6, iUP = 7;
         const int mA = P->anderson;
         const int hcap = mA + 1;
-        int hX[BLML_MAXAND + 1], hF[BLML_MAXAND + 1];
         int hhead = 0, hcnt = 0;
         FitOut o{};
         Sall(vec(iE), vec(iU));
@@ -439,7 +437,7 @@ struct FitSolver {
             }
             ++it;
             update(vec(iE), vec(iU), vec(iEJ));
-            Sall(vec(iEJ), vec(iUJ));
+            Snew(vec(iEJ), vec(iUJ), vec(iE), vec(iU));
             double mJ = merit(vec(iEJ), vec(iUJ));
             if (mA) {
                 // f = EJ - E; hX.append(E); hF.append(f); keep the last mA + 1
@@ -451,12 +449,12 @@ struct FitSolver {
                     slot = hhead;
                     hhead = (hhead + 1) % hcap;
                 }
-                T *xs = vec(13 + slot), *fs = vec(13 + hcap + slot);
-                const T *Ec = vec(iE), *EJc = vec(iEJ);
+                double *xs = vec(13 + slot), *fs = vec(13 + hcap + slot);
+                const double *Ec = vec(iE), *EJc = vec(iEJ);
                 const long n_ = V();
                 for (long e = threadIdx.x; e < n_; e += BLML_NT) {
                     xs[e] = Ec[e];
-                    fs[e] = (T)((double)EJc[e] - (double)Ec[e]);
+                    fs[e] = EJc[e] - Ec[e];
                 }
                 __syncthreads();
                 if (hcnt >= 3) {
@@ -466,7 +464,7 @@ struct FitSolver {
                         sf[i] = 13 + hcap + (hhead + i) % hcap;
                     }
                     if (anderson(sx, sf, hcnt, vec(iE), fs, vec(iEA))) {
-                        Sall(vec(iEA), vec(iUA));
+                        Snew(vec(iEA), vec(iUA), vec(iE), vec(iU));
                         const double mAc = merit(vec(iEA), vec(iUA));
                         if (mAc > mJ) {
                             int t = iEJ; iEJ = iEA; iEA = t;
@@ -483,18 +481,16 @@ struct FitSolver {
                 hcnt = 0; hhead = 0;
                 ++o.n_damp;
                 update(vec(iE), vec(iU), vec(iEP));
-                Sall(vec(iEP), vec(iUP));
+                Snew(vec(iEP), vec(iUP), vec(iE), vec(iU));
                 double a = 0.5;
                 bool found = false;
                 while (a > 1e-8) {
                     const long n_ = V();
-                    T *EJ = vec(iEJ), *UJ = vec(iUJ);
-                    const T *Ec = vec(iE), *Uc = vec(iU), *EPc = vec(iEP), *UPc = vec(iUP);
+                    double *EJ = vec(iEJ), *UJ = vec(iUJ);
+                    const double *Ec = vec(iE), *Uc = vec(iU), *EPc = vec(iEP), *UPc = vec(iUP);
                     for (long e = threadIdx.x; e < n_; e += BLML_NT) {
-                        const double de = (T)((double)EPc[e] - (double)Ec[e]);
-                        const double du = (T)((double)UPc[e] - (double)Uc[e]);
-                        EJ[e] = (T)((double)Ec[e] + (double)(T)(a * de));
-                        UJ[e] = (T)((double)Uc[e] + (double)(T)(a * du));
+                        EJ[e] = Ec[e] + a * (EPc[e] - Ec[e]);
+                        UJ[e] = Uc[e] + a * (UPc[e] - Uc[e]);
                     }
                     __syncthreads();
                     mJ = merit(vec(iEJ), vec(iUJ));
@@ -535,20 +531,23 @@ struct FitSolver {
     }
 
     // ------------------------------------------------------------------ L-BFGS (blml_binned.py:290-355)
+    // Each trial of the line search evaluates L only; the gradient is formed once, at the accepted point (the
+    // reference forms it at every trial and uses only the accepted one: the same iterates).
     __device__ FitOut solve_lbfgs() {
-        // slots: 0 E, 1 U, 2 g, 3 q, 4 En, 5 Un, 6 gn, 7 Vt, 8 SVt, 9.. S (m + 1), 10 + m.. Y (m + 1).  The pair
-        // memory is a ring of m + 1 physical slots holding at most m pairs, so a candidate pair never overwrites a
-        // kept one before its curvature test.
+        // slots: 0 E, 1 U, 2 g, 3 q, 4 En, 5 Un, 6 gn, 7 Vt, 8 SVt (also S q), 9.. S (m + 1), 10 + m.. Y (m + 1).
+        // The pair memory is a ring of m + 1 physical slots holding at most m pairs, so a candidate pair never
+        // overwrites a kept one before its curvature test.
         int iE = 0, iU = 1, ig = 2, iEn = 4, iUn = 5, ign = 6;
         const int iq = 3, iVt = 7, iSVt = 8;
         const int mh = P->lbfgs_m, ph = mh + 1;
         FitOut o{};
         Sall(vec(iE), vec(iU));
         rescale(vec(iE), vec(iU));
-        double L = lgrad(vec(iE), vec(iU), vec(ig), vec(iVt), vec(iSVt), true);
+        double Q = 0.0;
+        double L = lval(vec(iE), vec(iU), vec(iVt), &Q);
+        lgrad_from(vec(iU), Q, vec(iVt), vec(iSVt), vec(ig));
         int head = 0, cnt = 0;
         long it = 0;
-        double Q = Qof(vec(iE), vec(iU));
         double res = resid(vec(iE), vec(iU), Q > 0 ? n / sqrt(Q) : 1.0);
         push_rho(res);
         ring_n = 0;
@@ -561,7 +560,7 @@ struct FitSolver {
         while (res >= P->tol && it < cap) {
             if (ring_n > fwin && ring_back(0) - ring_back(fwin) < ftol) { stop = BLML_STOP_LLTOL; break; }
             ++it;
-            T *q = vec(iq);
+            double *q = vec(iq);
             copy(q, vec(ig));
             for (int j = cnt - 1; j >= 0; --j) {
                 const int sl_ = (head + j) % ph;
@@ -587,17 +586,25 @@ struct FitSolver {
                 const double ee = dot(vec(iE), vec(iE)), gg = dot(vec(ig), vec(ig));
                 const double sc = 1e-3 * sqrt(ee / fmax(gg, 1e-300));
                 const long n_ = V();
-                const T *g = vec(ig);
-                for (long e = threadIdx.x; e < n_; e += BLML_NT) q[e] = (T)((double)g[e] * sc);
+                const double *g = vec(ig);
+                for (long e = threadIdx.x; e < n_; e += BLML_NT) q[e] = g[e] * sc;
                 __syncthreads();
                 slope = dot(vec(ig), q);
             }
-            double t = 1.0, Ln = -INFINITY;
+            if (inc) Sall(q, vec(iSVt));             // S q: every trial's U is U + t S q
+            double t = 1.0, Ln = -INFINITY, Qn = 0.0;
             bool ok = false;
             for (int tr = 0; tr < 40; ++tr) {
-                axpy(vec(iEn), vec(iE), t, q);
-                Sall(vec(iEn), vec(iUn));
-                Ln = lgrad(vec(iEn), vec(iUn), vec(ign), vec(iVt), vec(iSVt), true);
+                const long n_ = V();
+                double *En = vec(iEn), *Un = vec(iUn);
+                const double *Ec = vec(iE), *Uc = vec(iU), *Sq = vec(iSVt);
+                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                    En[e] = Ec[e] + t * q[e];
+                    if (inc) Un[e] = Uc[e] + t * Sq[e];
+                }
+                __syncthreads();
+                if (!inc) Sall(vec(iEn), vec(iUn));
+                Ln = lval(vec(iEn), vec(iUn), nullptr, &Qn);
                 if (Ln >= L + 1e-4 * t * slope) { ok = true; break; }
                 t *= 0.5;
             }
@@ -608,21 +615,23 @@ struct FitSolver {
                 continue;
             }
             reset = false;
+            lval(vec(iEn), vec(iUn), vec(iVt), &Qn);
+            lgrad_from(vec(iUn), Qn, vec(iVt), vec(iSVt), vec(ign));
             // s = En - E, y = g - gn, into the next slot; curvature test
             const int slot = (head + cnt) % ph;
             {
-                T *S = vec(9 + slot), *Yv = vec(9 + ph + slot);
-                const T *En = vec(iEn), *Ec = vec(iE), *g = vec(ig), *gn = vec(ign);
+                double *Sv = vec(9 + slot), *Yv = vec(9 + ph + slot);
+                const double *En = vec(iEn), *Ec = vec(iE), *g = vec(ig), *gn = vec(ign);
                 double a3[3] = {0.0, 0.0, 0.0};
                 const long n_ = V();
                 for (long e = threadIdx.x; e < n_; e += BLML_NT) {
-                    const T sv = (T)((double)En[e] - (double)Ec[e]);
-                    const T yv = (T)((double)g[e] - (double)gn[e]);
-                    S[e] = sv;
+                    const double sv = En[e] - Ec[e];
+                    const double yv = g[e] - gn[e];
+                    Sv[e] = sv;
                     Yv[e] = yv;
-                    a3[0] += (double)sv * (double)yv;
-                    a3[1] += (double)sv * (double)sv;
-                    a3[2] += (double)yv * (double)yv;
+                    a3[0] += sv * yv;
+                    a3[1] += sv * sv;
+                    a3[2] += yv * yv;
                 }
                 block_sum<3>(a3, sl->red);
                 if (a3[0] > 1e-12 * sqrt(a3[1] * a3[2])) {
@@ -638,7 +647,7 @@ struct FitSolver {
             }
             { int tt = iE; iE = iEn; iEn = tt; tt = iU; iU = iUn; iUn = tt; tt = ig; ig = ign; ign = tt; }
             L = Ln;
-            Q = Qof(vec(iE), vec(iU));
+            Q = Qn;
             res = resid(vec(iE), vec(iU), Q > 0 ? n / sqrt(Q) : 1.0);
             push_rho(res);
             push_merit(L / n);
@@ -646,25 +655,22 @@ struct FitSolver {
         if (res >= P->tol && stop == BLML_STOP_TOL) stop = BLML_STOP_MAXSWEEPS;
         // canonical form: e_k = (Q / n) (w / F) u_k on occupied bins, kept unless it loses likelihood
         {
-            double a1[1] = {0.0};
-            const long n_ = V();
-            const T *Ec = vec(iE), *Uc = vec(iU);
-            for (long e = threadIdx.x; e < n_; e += BLML_NT) a1[0] += (double)Ec[e] * (double)Uc[e];
-            block_sum<1>(a1, sl->red);
-            const double Qc = a1[0];
-            T *Ecn = vec(iEn);
+            const double Qc = dot(vec(iE), vec(iU));
+            double *Ecn = vec(iEn);
+            const double *Uc = vec(iU);
             for (int b = threadIdx.x; b < B; b += BLML_NT) {
                 const bool ob = occ(b);
                 const double F = Fb(Uc, b);
                 const double fac = ob ? (Qc / n) * w[b] / F : 0.0;
                 for (int k = 0; k < K; ++k) {
                     const long i = ((long)k * B + b) * nc;
-                    for (int qq = 0; qq < nc; ++qq) Ecn[i + qq] = (T)(fac * (double)Uc[i + qq]);
+                    for (int qq = 0; qq < nc; ++qq) Ecn[i + qq] = fac * Uc[i + qq];
                 }
             }
             __syncthreads();
-            Sall(vec(iEn), vec(iUn));
-            const double Lc = lgrad(vec(iEn), vec(iUn), vec(ign), vec(iVt), vec(iSVt), false);
+            Snew(vec(iEn), vec(iUn), vec(iE), vec(iU));
+            double Qd = 0.0;
+            const double Lc = lval(vec(iEn), vec(iUn), nullptr, &Qd);
             if (Lc >= L - 1e-9 * n) {
                 int tt = iE; iE = iEn; iEn = tt; tt = iU; iU = iUn; iUn = tt;
             }

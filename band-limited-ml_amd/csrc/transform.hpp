@@ -341,21 +341,30 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa) {
     // 2 * naxes passes: the result is back in c.X
 }
 
-// U_k = S_k E_k for one mode.  E, U: B * nc values of type T (nc = 1 real, 2 interleaved complex).
-template <typename T>
-__device__ void matvec_mode(const MatvecCtx<T> &c, const int fac_begin, const int fac_count, const T *E, T *U,
-                            int B, int nc) {
-    for (int b = threadIdx.x; b < B; b += BLML_NT)
-        c.X[b] = nc == 2 ? cpx<T>{E[2 * b], E[2 * b + 1]} : cpx<T>{E[b], (T)0};
+// U_k = S_k E_k for one mode.  E, U: B * nc values (nc = 1 real, 2 interleaved complex) of any real type; the
+// transforms run in T.  With bases: U = Ubase + S_k (E - Ebase) -- the incremental form, whose rounding error in T is
+// relative to the increment, not to U (E and Ebase / U and Ubase may not alias each other's outputs).
+template <typename T, typename Ti, typename To>
+__device__ void matvec_mode(const MatvecCtx<T> &c, const int fac_begin, const int fac_count, const Ti *E, To *U,
+                            int B, int nc, const Ti *Ebase = nullptr, const To *Ubase = nullptr) {
+    for (int b = threadIdx.x; b < B; b += BLML_NT) {
+        double re = (double)E[nc * b], im = nc == 2 ? (double)E[2 * b + 1] : 0.0;
+        if (Ebase) {
+            re -= (double)Ebase[nc * b];
+            if (nc == 2) im -= (double)Ebase[2 * b + 1];
+        }
+        c.X[b] = {(T)re, (T)im};
+    }
     __syncthreads();
     for (int f = 0; f < fac_count; ++f) apply_factor<T>(c, c.facs[fac_begin + f]);
     for (int b = threadIdx.x; b < B; b += BLML_NT) {
-        if (nc == 2) {
-            U[2 * b] = c.X[b].x;
-            U[2 * b + 1] = c.X[b].y;
-        } else {
-            U[b] = c.X[b].x;
+        double re = (double)c.X[b].x, im = (double)c.X[b].y;
+        if (Ubase) {
+            re += (double)Ubase[nc * b];
+            if (nc == 2) im += (double)Ubase[2 * b + 1];
         }
+        U[nc * b] = (To)re;
+        if (nc == 2) U[2 * b + 1] = (To)im;
     }
     __syncthreads();
 }

@@ -67,8 +67,8 @@ typedef struct {
     int64_t out_off;                 /* e_hat / Se: K*B*(1+complex_g) elements */
     int64_t dens_off;                /* density: B elements */
     int64_t hist_off;                /* merit / rho history: hist_cap elements */
-    int64_t ws_off;                  /* set by the library */
-    int64_t ws_len;                  /* set by the library */
+    int64_t ws_off;                  /* bytes into the plan workspace (set by the library) */
+    int64_t ws_len;                  /* bytes (set by the library) */
     int64_t scratch_len;             /* complex elements of each transform scratch buffer (set by the library) */
 } blml_fit_t;
 
