@@ -41,10 +41,12 @@ class Result(ctypes.Structure):
     _fields_ = [('loglik', ctypes.c_double), ('residual', ctypes.c_double), ('n_sweeps', ctypes.c_int32),
                 ('stop_reason', ctypes.c_int32), ('converged', ctypes.c_int32), ('n_anderson', ctypes.c_int32),
                 ('n_damped', ctypes.c_int32), ('n_gradient', ctypes.c_int32), ('start_sweeps', ctypes.c_int32),
-                ('n_hist', ctypes.c_int32)]
+                ('n_hist', ctypes.c_int32), ('n_matvec', ctypes.c_int32), ('reserved', ctypes.c_int32),
+                ('seconds', ctypes.c_double * 5)]
 
 
-RESULT_DTYPE = np.dtype([(name, np.float64 if t is ctypes.c_double else np.int32) for name, t in Result._fields_])
+RESULT_DTYPE = np.dtype([(name, (np.float64, 5) if name == 'seconds' else
+                          (np.float64 if t is ctypes.c_double else np.int32)) for name, t in Result._fields_])
 assert RESULT_DTYPE.itemsize == ctypes.sizeof(Result)
 
 EXPORTS = ('blml_plan_create', 'blml_plan_destroy', 'blml_matvec', 'blml_fit', 'blml_certificate',

@@ -295,6 +295,8 @@ class FitBatch:
                             stop_reason=_lib.STOP_REASONS[int(r['stop_reason'])], converged=bool(r['converged']),
                             n_anderson=int(r['n_anderson']), n_damped=int(r['n_damped']),
                             n_gradient=int(r['n_gradient']), start_sweeps=int(r['start_sweeps']),
+                            n_matvec=int(r['n_matvec']), device_seconds=dict(zip(
+                                ('fit', 'start', 'matvec', 'direction', 'line_search'), map(float, r['seconds']))),
                             merit_history=mh[F.hist_off:F.hist_off + nh].tolist(),
                             rho_history=rh[F.hist_off:F.hist_off + nh].tolist()))
         return out

@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
                                                       const double *counts, const double *draws,
                                                       const double *init, const double *signs, T *e_out, T *se_out,
                                                       double *density, blml_result_t *results, double *merit_hist,
-                                                      double *rho_hist) {
+                                                      double *rho_hist, double tick_hz) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ SolverLds sl;
     __shared__ int s_fit;
@@ -202,6 +202,10 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
         S.pool = (double *)(p.ws + f.ws_off + pool_bytes_off(f, (int)sizeof(T)));
         S.inc = sizeof(T) == 4;
         S.gstep = 0.0;
+        S.t_mv = S.t_dir = S.t_ls = 0;
+        S.n_mv = 0;
+        const long long tfit0 = wall_clock64();
+        long long tstart = 0;
         S.merit_hist = merit_hist ? merit_hist + f.hist_off : nullptr;
         S.rho_hist = rho_hist ? rho_hist + f.hist_off : nullptr;
         S.hist_cap = p.params.hist_cap;
@@ -258,6 +262,7 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
                 S.K = 1; S.nc = 1; S.psd = false; S.rec = false;
                 const FitOut so = S.solve_cd();
                 start_sweeps = so.n_sweeps;
+                tstart = wall_clock64() - tfit0;
                 // phases and the other modes (blml_binned.py:429-436), assembled in slot 2 then moved to slot 0
                 double *E2 = S.vec(2);
                 const double *er = S.vec(0);
@@ -328,6 +333,13 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
             r.n_gradient = o.n_grad;
             r.start_sweeps = start_sweeps;
             r.n_hist = S.nhist_m;
+            r.n_matvec = S.n_mv;
+            r.reserved = 0;
+            r.seconds[0] = (double)(wall_clock64() - tfit0) / tick_hz;
+            r.seconds[1] = (double)tstart / tick_hz;
+            r.seconds[2] = (double)S.t_mv / tick_hz;
+            r.seconds[3] = (double)S.t_dir / tick_hz;
+            r.seconds[4] = (double)S.t_ls / tick_hz;
             results[fi] = r;
         }
         __syncthreads();
@@ -472,6 +484,7 @@ struct Plan {
     void *d_spec64 = nullptr;   // FP32 plans: FP64 spectra and twiddles for the finishing pass
     void *d_tw64 = nullptr;
     int lds64_bytes = 0;
+    double tick_hz = 1e8;      // wall_clock64() rate
     void *d_ws = nullptr;
     int *d_order = nullptr;
     int *d_queue = nullptr;
@@ -721,6 +734,9 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fit_kernel<T>, BLML_NT, pl.lds_bytes));
+    int khz = 0;
+    HIPCHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    if (khz > 0) pl.tick_hz = 1e3 * khz;
     if (occ < 1) occ = 1;
     pl.grid = std::max(1, std::min(pl.nfits, ncu * occ));
     return BLML_OK;
@@ -809,14 +825,14 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
     if (pl->dtype == BLML_F32) {
         fit_kernel<float><<<pl->grid, BLML_NT, pl->lds_bytes, st>>>(
             dev_plan<float>(*pl), pl->d_order, pl->d_queue, counts, draws, init, signs, (float *)e_out,
-            (float *)se_out, density, results, merit_hist, rho_hist);
+            (float *)se_out, density, results, merit_hist, rho_hist, pl->tick_hz);
         HIPCHECK(hipGetLastError());
         finish64_kernel<<<pl->nfits, BLML_NT, pl->lds64_bytes, st>>>(dev_plan64(*pl), (float *)se_out, counts,
                                                                      density, results);
     } else {
         fit_kernel<double><<<pl->grid, BLML_NT, pl->lds_bytes, st>>>(
             dev_plan<double>(*pl), pl->d_order, pl->d_queue, counts, draws, init, signs, (double *)e_out,
-            (double *)se_out, density, results, merit_hist, rho_hist);
+            (double *)se_out, density, results, merit_hist, rho_hist, pl->tick_hz);
     }
     HIPCHECK(hipGetLastError());
     return BLML_OK;

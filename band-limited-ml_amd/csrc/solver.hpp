@@ -85,25 +85,33 @@ struct FitSolver {
     int hist_cap, nhist_m, nhist_r;
     int ring_n;            // entries pushed to the merit ring
     bool rec;              // record merit / rho histories (the final solve only, as merit_history_ is)
+    long long t_mv, t_dir, t_ls;   // wall-clock ticks in mat-vecs, L-BFGS direction, line search
+    int n_mv;
 
     __device__ double *vec(int i) const { return pool + (long)i * Vmax; }
     __device__ long V() const { return (long)K * B * nc; }
 
     // ------------------------------------------------------------------ mat-vecs
     __device__ void Sall(const double *E, double *U) {
+        const long long t0 = wall_clock64();
         for (int k = 0; k < K; ++k) {
             const long o = (long)k * B * nc;
             matvec_mode<T, double, double>(mv, f->fac_begin[k], f->fac_count[k], E + o, U + o, B, nc);
         }
+        t_mv += wall_clock64() - t0;
+        n_mv += K;
     }
     // U_new = S E_new: recomputed (exact mode) or U + S (E_new - E) (incremental mode)
     __device__ void Snew(const double *En, double *Un, const double *E, const double *U) {
         if (!inc) { Sall(En, Un); return; }
+        const long long t0 = wall_clock64();
         for (int k = 0; k < K; ++k) {
             const long o = (long)k * B * nc;
             matvec_mode<T, double, double>(mv, f->fac_begin[k], f->fac_count[k], En + o, Un + o, B, nc, E + o,
                                            U + o);
         }
+        t_mv += wall_clock64() - t0;
+        n_mv += K;
     }
 
     // ------------------------------------------------------------------ basic vector operations
@@ -560,6 +568,7 @@ struct FitSolver {
         while (res >= P->tol && it < cap) {
             if (ring_n > fwin && ring_back(0) - ring_back(fwin) < ftol) { stop = BLML_STOP_LLTOL; break; }
             ++it;
+            const long long td0 = wall_clock64();
             double *q = vec(iq);
             copy(q, vec(ig));
             for (int j = cnt - 1; j >= 0; --j) {
@@ -591,6 +600,8 @@ struct FitSolver {
                 __syncthreads();
                 slope = dot(vec(ig), q);
             }
+            const long long td1 = wall_clock64();
+            t_dir += td1 - td0;
             if (inc) Sall(q, vec(iSVt));             // S q: every trial's U is U + t S q
             double t = 1.0, Ln = -INFINITY, Qn = 0.0;
             bool ok = false;
@@ -608,6 +619,7 @@ struct FitSolver {
                 if (Ln >= L + 1e-4 * t * slope) { ok = true; break; }
                 t *= 0.5;
             }
+            t_ls += wall_clock64() - td1;
             if (!ok) {
                 if (reset || cnt == 0) { stop = BLML_STOP_STALL; break; }
                 cnt = 0; head = 0;

@@ -98,6 +98,9 @@ typedef struct {
     int32_t n_gradient;
     int32_t start_sweeps;
     int32_t n_hist;
+    int32_t n_matvec;                /* single-mode Gram mat-vecs the fit performed */
+    int32_t reserved;
+    double seconds[5];               /* device time: whole fit, start solve, mat-vecs, L-BFGS direction, line search */
 } blml_result_t;
 
 /* Plan: uploads the fit and factor tables (host arrays), computes every spectrum on the device, allocates the

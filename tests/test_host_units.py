@@ -157,7 +157,27 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
-def test_struct_layouts_match_header():
-    # sizes of the C structs as the header lays them out (natural alignment, x86-64 / amdgcn agree)
-    assert ctypes.sizeof(_lib.Factor) == 4 * 2 + 4 * 5 * 3 + 4 + 8 * 5 + 8 + 8
-    assert ctypes.sizeof(_lib.Result) == 8 * 2 + 4 * 8
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors of the C structs agree with the header's layout (sizes and field offsets, compiled by gcc)."""
+    import shutil
+    import subprocess
+    if shutil.which('gcc') is None:
+        pytest.skip('gcc not available')
+    fields = {'blml_factor_t': _lib.Factor, 'blml_fit_t': _lib.Fit, 'blml_params_t': _lib.Params,
+              'blml_result_t': _lib.Result}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "blml_hip.h"', 'int main(void) {']
+    for cname, py in fields.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / 'layout.c'
+    src.write_text('\n'.join(lines))
+    exe = tmp_path / 'layout'
+    subprocess.run(['gcc', '-I', os.path.join(ROOT, 'include'), str(src), '-o', str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split('\n')
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in fields.items():
+        assert got[(cname, 'size')] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
