@@ -3,7 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define BLML_NT 256                 // threads per workgroup (4 wavefronts of 64)
+#define BLML_NT 512                 // threads per workgroup (8 wavefronts of 64)
 #define BLML_NWAVE (BLML_NT / 64)
 #define BLML_NRED 24                // simultaneous FP64 sums one reduction can carry
 

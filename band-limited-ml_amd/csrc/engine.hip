@@ -71,7 +71,7 @@ __host__ __device__ inline long fin_bytes_off(const blml_params_t &P, const blml
 template <typename T>
 __global__ void twiddle_kernel(cpx<T> *tw, int Lmax) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < Lmax / 2) {
+    if (i < Lmax) {
         double sn, cs;
         sincospi(-2.0 * (double)i / (double)Lmax, &sn, &cs);
         tw[i] = {(T)cs, (T)sn};
@@ -124,7 +124,6 @@ __global__ void __launch_bounds__(BLML_NT) spec_kernel(DevPlan<T> p, int nfacs, 
         Pass<T> ps;
         ps.op = OP_FFT_FULL;
         ps.L = fa.tlen[a];
-        ps.logL = ilog2(ps.L);
         ps.m = ps.L;
         ps.nF = 0;
         ps.F = nullptr;
@@ -305,7 +304,11 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
         S.psd = psd_full;
         S.rec = true;
         S.nhist_m = S.nhist_r = 0;
-        const FitOut o = (p.params.solver == 1 && psd_full) ? S.solve_lbfgs() : S.solve_cd();
+        FitOut o;
+        if (p.params.solver == 1 && psd_full)
+            o = (S.inc && p.params.lbfgs_m <= BLML_CMAX) ? S.solve_lbfgs_compact() : S.solve_lbfgs();
+        else
+            o = S.solve_cd();
         // outputs (blml_binned.py:438-445)
         const double *Ef = S.vec(0), *Uf = S.vec(1);
         const long V = (long)f.K * f.B * ncf;
@@ -597,14 +600,12 @@ int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs,
                     const int v = freqs[fa.freq_off[a] + q];
                     if (v < 0 || v >= m) return fail(BLML_EVALUE, "frequency outside [0, m)");
                 }
-                const int op = fa.kind == BLML_FAC_LINE ? OP_CONV_CIRC : OP_DFT_FWD;
-                lds_elems = std::max(lds_elems, pass_lds_elems(op, m, 0, tl));
-                if (fa.kind == BLML_FAC_GROUP) lds_elems = std::max(lds_elems, pass_lds_elems(OP_DFT_INV, m, 0, tl));
             } else {
                 if (tl < 2 * m - 1 || tl != ceil_pow2(tl) || tl < 2)
                     return fail(BLML_EVALUE, "padded factor: power-of-two length >= 2m - 1");
+                if (tl > 8192)
+                    return fail(BLML_EVALUE, "padded lattice axis longer than 4096 bins: not supported by this engine");
                 Lmax = std::max(Lmax, tl);
-                lds_elems = std::max(lds_elems, pass_lds_elems(OP_CONV_PAD, m, tl, 0));
             }
             total *= tl;
         }
@@ -618,6 +619,10 @@ int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs,
         if (fa.kind == BLML_FAC_GROUP && ncirc == 0) {
             tmp_off[id] = tmp_total;
             tmp_total += 2 * total;
+            // the spectrum's forward FFTs over the table's own lattice (spec_kernel)
+            for (int a = 0; a < fa.naxes; ++a)
+                lds_elems = std::max(lds_elems, pass_lds_elems(OP_FFT_FULL, fa.tlen[a], fa.tlen[a], 0,
+                                                               total / fa.tlen[a]));
         }
     }
     // fits: scratch (largest intermediate of any factor chain) and workspace
@@ -625,17 +630,41 @@ int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs,
     for (int fi = 0; fi < pl.nfits; ++fi) {
         blml_fit_t &f = pl.hfits[fi];
         long scratch = f.B;
+        // replay every pass of the fit's mat-vecs: scratch = the largest intermediate, LDS = the largest round
         for (int k = 0; k < f.K; ++k) {
             for (int q = 0; q < f.fac_count[k]; ++q) {
                 const blml_factor_t &fa = pl.hfacs[f.fac_begin[k] + q];
-                if (fa.kind != BLML_FAC_GROUP) continue;
+                if (fa.kind == BLML_FAC_LINE) {
+                    const int m = f.shape[fa.axes[0]];
+                    const long nl = f.B / m;
+                    lds_elems = std::max(lds_elems, fa.circ[0] ? pass_lds_elems(OP_CONV_CIRC, m, 0, fa.tlen[0], nl)
+                                                               : pass_lds_elems(OP_CONV_PAD, m, fa.tlen[0], 0, nl));
+                    continue;
+                }
                 long ext[BLML_MAXD];
                 for (int j = 0; j < f.d; ++j) ext[j] = f.shape[j];
+                auto lines_of = [&](int axis) {
+                    long t = 1;
+                    for (int j = 0; j < f.d; ++j)
+                        if (j != axis) t *= ext[j];
+                    return t;
+                };
                 for (int a = fa.naxes - 1; a >= 0; --a) {
-                    ext[fa.axes[a]] = fa.tlen[a];
+                    const int ax = fa.axes[a], m = f.shape[ax];
+                    lds_elems = std::max(lds_elems, fa.circ[a]
+                        ? pass_lds_elems(OP_DFT_FWD, m, 0, fa.tlen[a], lines_of(ax))
+                        : pass_lds_elems(OP_FFT_FWD, m, fa.tlen[a], 0, lines_of(ax)));
+                    ext[ax] = fa.tlen[a];
                     long t = 1;
                     for (int j = 0; j < f.d; ++j) t *= ext[j];
                     scratch = std::max(scratch, t);
+                }
+                for (int a = 0; a < fa.naxes; ++a) {
+                    const int ax = fa.axes[a], m = f.shape[ax];
+                    lds_elems = std::max(lds_elems, fa.circ[a]
+                        ? pass_lds_elems(OP_DFT_INV, m, 0, fa.tlen[a], lines_of(ax))
+                        : pass_lds_elems(OP_FFT_INV, m, fa.tlen[a], 0, lines_of(ax)));
+                    ext[ax] = m;
                 }
             }
         }
@@ -680,7 +709,7 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
     HIPCHECK(hipMalloc(&pl.d_fdata, sizeof(double) * std::max<int64_t>(1, n_fdata)));
     HIPCHECK(hipMalloc(&pl.d_freqs, sizeof(int) * std::max<int64_t>(1, n_freqs)));
     HIPCHECK(hipMalloc(&pl.d_spec, sizeof(T) * std::max<int64_t>(1, spec_total)));
-    HIPCHECK(hipMalloc(&pl.d_tw, sizeof(cpx<T>) * (pl.Lmax / 2)));
+    HIPCHECK(hipMalloc(&pl.d_tw, sizeof(cpx<T>) * pl.Lmax));
     HIPCHECK(hipMalloc(&pl.d_ws, std::max<int64_t>(256, pl.ws_elems)));
     HIPCHECK(hipMalloc(&pl.d_order, sizeof(int) * std::max(1, pl.nfits)));
     HIPCHECK(hipMalloc(&pl.d_queue, sizeof(int)));
@@ -696,7 +725,7 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
         return (double)fa.B * fa.K * (1 + fa.complex_g) > (double)fb.B * fb.K * (1 + fb.complex_g);
     });
     HIPCHECK(hipMemcpy(pl.d_order, order.data(), sizeof(int) * pl.nfits, hipMemcpyHostToDevice));
-    twiddle_kernel<T><<<(pl.Lmax / 2 + 255) / 256, 256>>>((cpx<T> *)pl.d_tw, pl.Lmax);
+    twiddle_kernel<T><<<(pl.Lmax + 255) / 256, 256>>>((cpx<T> *)pl.d_tw, pl.Lmax);
     HIPCHECK(hipGetLastError());
     int *d_fac_m = nullptr;
     long *d_tmp_off = nullptr;
@@ -712,8 +741,8 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
     }
     if (sizeof(T) == 4) {
         HIPCHECK(hipMalloc(&pl.d_spec64, sizeof(double) * std::max<int64_t>(1, spec_total)));
-        HIPCHECK(hipMalloc(&pl.d_tw64, sizeof(cpx<double>) * (pl.Lmax / 2)));
-        twiddle_kernel<double><<<(pl.Lmax / 2 + 255) / 256, 256>>>((cpx<double> *)pl.d_tw64, pl.Lmax);
+        HIPCHECK(hipMalloc(&pl.d_tw64, sizeof(cpx<double>) * pl.Lmax));
+        twiddle_kernel<double><<<(pl.Lmax + 255) / 256, 256>>>((cpx<double> *)pl.d_tw64, pl.Lmax);
         HIPCHECK(hipGetLastError());
         cpx<double> *d_tmp64 = nullptr;
         HIPCHECK(hipMalloc(&d_tmp64, sizeof(cpx<double>) * std::max<long>(1, tmp_total)));

@@ -19,13 +19,19 @@
 
 #define BLML_MAXHIST 64      // L-BFGS memory limit
 #define BLML_MAXAND 5        // Anderson history limit
-#define BLML_RING 1024       // merit window ring (ll_window + 1 <= BLML_RING)
+#define BLML_RING 512        // merit window ring (ll_window + 1 <= BLML_RING)
+#define BLML_CMAX 24         // L-BFGS memory of the compact form (incremental mode); larger memories use two-loop
+#define BLML_CCHUNK 8        // history pairs per sub-pass of the compact form's dot products
 #define BLML_HARD_CAP 1000000L  // sweeps / iterations of one solve when max_sweeps is None (every launch ends)
 
 struct SolverLds {
     double red[BLML_NWAVE * BLML_NRED];
     double ring[BLML_RING];
     double lb_r[BLML_MAXHIST + 1], lb_a[BLML_MAXHIST + 1], lb_sy[BLML_MAXHIST + 1], lb_yy[BLML_MAXHIST + 1];
+    // compact L-BFGS: R = triu(S^T Y), Y^T Y (logical order, oldest first), S^T g, Y^T g, and the new dots
+    double cR[BLML_CMAX * BLML_CMAX], cYY[BLML_CMAX * BLML_CMAX];
+    double ca[BLML_CMAX + 1], cb[BLML_CMAX + 1], csg[BLML_CMAX + 1], cyg[BLML_CMAX + 1];
+    double cp1[BLML_CMAX], cp2[BLML_CMAX];
 };
 
 struct FitOut {
@@ -666,6 +672,253 @@ struct FitSolver {
         }
         if (res >= P->tol && stop == BLML_STOP_TOL) stop = BLML_STOP_MAXSWEEPS;
         // canonical form: e_k = (Q / n) (w / F) u_k on occupied bins, kept unless it loses likelihood
+        {
+            const double Qc = dot(vec(iE), vec(iU));
+            double *Ecn = vec(iEn);
+            const double *Uc = vec(iU);
+            for (int b = threadIdx.x; b < B; b += BLML_NT) {
+                const bool ob = occ(b);
+                const double F = Fb(Uc, b);
+                const double fac = ob ? (Qc / n) * w[b] / F : 0.0;
+                for (int k = 0; k < K; ++k) {
+                    const long i = ((long)k * B + b) * nc;
+                    for (int qq = 0; qq < nc; ++qq) Ecn[i + qq] = fac * Uc[i + qq];
+                }
+            }
+            __syncthreads();
+            Snew(vec(iEn), vec(iUn), vec(iE), vec(iU));
+            double Qd = 0.0;
+            const double Lc = lval(vec(iEn), vec(iUn), nullptr, &Qd);
+            if (Lc >= L - 1e-9 * n) {
+                int tt = iE; iE = iEn; iEn = tt; tt = iU; iU = iUn; iUn = tt;
+            }
+        }
+        rescale(vec(iE), vec(iU));
+        res = resid(vec(iE), vec(iU), 1.0);
+        if (iE != 0) { copy(vec(0), vec(iE)); copy(vec(1), vec(iU)); }
+        o.n_sweeps = (int)it;
+        o.stop = stop;
+        o.residual = res;
+        o.converged = res < P->tol || stop == BLML_STOP_LLTOL;
+        return o;
+    }
+
+    // ------------------------------------------------------------------ L-BFGS, compact form (incremental mode)
+    // The same iteration as solve_lbfgs (the same BFGS matrix from the same accepted pairs, blml_binned.py:303-335),
+    // with the direction H g from the compact representation H = gamma I + [S  gamma Y] M [S^T; gamma Y^T]
+    // (Byrd, Nocedal & Schnabel 1994): with a = S^T g, b = Y^T g, R = triu(S^T Y), D = diag(s_i . y_i),
+    //   u = R^-1 a,  p1 = R^-T ((D + gamma Y^T Y) u - gamma b),  p2 = -u,  H g = gamma g + S p1 + gamma Y p2.
+    // One pass over the history per iteration produces S^T g_new and Y^T g_new, from which the new column of R
+    // (s_i . y = s_i . g - s_i . g_new) and of Y^T Y follow; a second pass forms the direction.
+    __device__ void dots_history(int head, int ph, int cnt, const double *gn) {
+        // csg[i] = s_i . gn, cyg[i] = y_i . gn for the cnt pairs (logical order), in chunks of BLML_CCHUNK pairs
+        const long n_ = V();
+        for (int c0 = 0; c0 < cnt; c0 += BLML_CCHUNK) {
+            double acc[2 * BLML_CCHUNK];
+#pragma unroll
+            for (int i = 0; i < 2 * BLML_CCHUNK; ++i) acc[i] = 0.0;
+            const int nc_ = cnt - c0 < BLML_CCHUNK ? cnt - c0 : BLML_CCHUNK;
+            const double *Sp[BLML_CCHUNK], *Yp[BLML_CCHUNK];
+#pragma unroll
+            for (int i = 0; i < BLML_CCHUNK; ++i) {
+                const int sl_ = (head + c0 + (i < nc_ ? i : 0)) % ph;
+                Sp[i] = vec(9 + sl_);
+                Yp[i] = vec(9 + ph + sl_);
+            }
+            for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                const double ge = gn[e];
+#pragma unroll
+                for (int i = 0; i < BLML_CCHUNK; ++i) {
+                    if (i < nc_) {
+                        acc[i] += Sp[i][e] * ge;
+                        acc[BLML_CCHUNK + i] += Yp[i][e] * ge;
+                    }
+                }
+            }
+            block_sum<2 * BLML_CCHUNK>(acc, sl->red);
+            if (threadIdx.x == 0)
+                for (int i = 0; i < nc_; ++i) { sl->csg[c0 + i] = acc[i]; sl->cyg[c0 + i] = acc[BLML_CCHUNK + i]; }
+        }
+        __syncthreads();
+    }
+
+    __device__ FitOut solve_lbfgs_compact() {
+        int iE = 0, iU = 1, ig = 2, iEn = 4, iUn = 5, ign = 6;
+        const int iq = 3, iVt = 7, iSVt = 8;
+        const int mh = P->lbfgs_m, ph = mh + 1;
+        FitOut o{};
+        Sall(vec(iE), vec(iU));
+        rescale(vec(iE), vec(iU));
+        double Q = 0.0;
+        double L = lval(vec(iE), vec(iU), vec(iVt), &Q);
+        lgrad_from(vec(iU), Q, vec(iVt), vec(iSVt), vec(ig));
+        double gg = dot(vec(ig), vec(ig)), ee = dot(vec(iE), vec(iE));
+        int head = 0, cnt = 0;
+        double gam = 1.0;
+        long it = 0;
+        double res = resid(vec(iE), vec(iU), Q > 0 ? n / sqrt(Q) : 1.0);
+        push_rho(res);
+        ring_n = 0;
+        push_merit(L / n);
+        const long cap = P->max_sweeps < 0 ? BLML_HARD_CAP : P->max_sweeps;
+        int stop = BLML_STOP_TOL;
+        bool reset = false;
+        const double ftol = isnan(P->ll_tol) ? 1e-11 : P->ll_tol;
+        const int fwin = 10;
+        const long n_ = V();
+        while (res >= P->tol && it < cap) {
+            if (ring_n > fwin && ring_back(0) - ring_back(fwin) < ftol) { stop = BLML_STOP_LLTOL; break; }
+            ++it;
+            const long long td0 = wall_clock64();
+            double *q = vec(iq);
+            double slope = 0.0;
+            if (cnt > 0) {
+                // small systems, solved redundantly by every thread from LDS
+                double u[BLML_CMAX], w[BLML_CMAX];
+                for (int i = cnt - 1; i >= 0; --i) {            // R u = a (upper triangular)
+                    double v = sl->ca[i];
+                    for (int j = i + 1; j < cnt; ++j) v -= sl->cR[i * BLML_CMAX + j] * u[j];
+                    u[i] = v / sl->cR[i * BLML_CMAX + i];
+                }
+                for (int i = 0; i < cnt; ++i) {                  // w = (D + gamma Y^T Y) u - gamma b
+                    double v = sl->cR[i * BLML_CMAX + i] * u[i] - gam * sl->cb[i];
+                    for (int j = 0; j < cnt; ++j) v += gam * sl->cYY[i * BLML_CMAX + j] * u[j];
+                    w[i] = v;
+                }
+                for (int i = 0; i < cnt; ++i) {                  // R^T p1 = w (lower triangular)
+                    double v = w[i];
+                    for (int j = 0; j < i; ++j) v -= sl->cR[j * BLML_CMAX + i] * w[j];
+                    w[i] = v / sl->cR[i * BLML_CMAX + i];
+                }
+                if (threadIdx.x == 0)
+                    for (int i = 0; i < cnt; ++i) { sl->cp1[i] = w[i]; sl->cp2[i] = -gam * u[i]; }
+                __syncthreads();
+                double acc = 0.0;
+                const double *g = vec(ig);
+                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                    double v = gam * g[e];
+                    for (int i = 0; i < cnt; ++i) {
+                        const int sl_ = (head + i) % ph;
+                        v += sl->cp1[i] * vec(9 + sl_)[e] + sl->cp2[i] * vec(9 + ph + sl_)[e];
+                    }
+                    q[e] = v;
+                    acc += g[e] * v;
+                }
+                slope = block_sum1(acc, sl->red);
+            }
+            if (!(slope > 0)) {
+                cnt = 0; head = 0;
+                const double sc = 1e-3 * sqrt(ee / fmax(gg, 1e-300));
+                const double *g = vec(ig);
+                double acc = 0.0;
+                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                    q[e] = g[e] * sc;
+                    acc += g[e] * q[e];
+                }
+                slope = block_sum1(acc, sl->red);
+            }
+            const long long td1 = wall_clock64();
+            t_dir += td1 - td0;
+            Sall(q, vec(iSVt));                              // S q: every trial's U is U + t S q
+            double t = 1.0, Ln = -INFINITY, Qn = 0.0;
+            bool ok = false;
+            for (int tr = 0; tr < 40; ++tr) {
+                double *En = vec(iEn), *Un = vec(iUn);
+                const double *Ec = vec(iE), *Uc = vec(iU), *Sq = vec(iSVt);
+                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                    En[e] = Ec[e] + t * q[e];
+                    Un[e] = Uc[e] + t * Sq[e];
+                }
+                __syncthreads();
+                Ln = lval(vec(iEn), vec(iUn), nullptr, &Qn);
+                if (Ln >= L + 1e-4 * t * slope) { ok = true; break; }
+                t *= 0.5;
+            }
+            t_ls += wall_clock64() - td1;
+            if (!ok) {
+                if (reset || cnt == 0) { stop = BLML_STOP_STALL; break; }
+                cnt = 0; head = 0;
+                reset = true;
+                continue;
+            }
+            reset = false;
+            lval(vec(iEn), vec(iUn), vec(iVt), &Qn);
+            lgrad_from(vec(iUn), Qn, vec(iVt), vec(iSVt), vec(ign));
+            // the kept pairs' dots with the new gradient
+            dots_history(head, ph, cnt, vec(ign));
+            // the candidate pair s = En - E, y = g - gn and its scalars
+            const int slot = (head + cnt) % ph;
+            double a7[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            {
+                double *Sv = vec(9 + slot), *Yv = vec(9 + ph + slot);
+                const double *En = vec(iEn), *Ec = vec(iE), *g = vec(ig), *gn = vec(ign);
+                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                    const double sv = En[e] - Ec[e];
+                    const double yv = g[e] - gn[e];
+                    Sv[e] = sv;
+                    Yv[e] = yv;
+                    a7[0] += sv * yv;
+                    a7[1] += sv * sv;
+                    a7[2] += yv * yv;
+                    a7[3] += sv * gn[e];
+                    a7[4] += yv * gn[e];
+                    a7[5] += gn[e] * gn[e];
+                    a7[6] += En[e] * En[e];
+                }
+                block_sum<7>(a7, sl->red);
+            }
+            // update the small matrices (thread 0) and a, b for the new gradient
+            const bool accept = a7[0] > 1e-12 * sqrt(a7[1] * a7[2]);
+            if (threadIdx.x == 0) {
+                // R's new column and Y^T Y's new row / column for the kept pairs, from the previous a, b
+                double rc[BLML_CMAX + 1], yc[BLML_CMAX + 1];
+                for (int i = 0; i < cnt; ++i) {
+                    rc[i] = sl->ca[i] - sl->csg[i];
+                    yc[i] = sl->cb[i] - sl->cyg[i];
+                }
+                for (int i = 0; i < cnt; ++i) { sl->ca[i] = sl->csg[i]; sl->cb[i] = sl->cyg[i]; }
+                if (accept) {
+                    int c = cnt;
+                    if (cnt == mh) {                            // drop the oldest pair
+                        for (int i = 1; i < cnt; ++i)
+                            for (int j = 1; j < cnt; ++j) {
+                                sl->cR[(i - 1) * BLML_CMAX + (j - 1)] = sl->cR[i * BLML_CMAX + j];
+                                sl->cYY[(i - 1) * BLML_CMAX + (j - 1)] = sl->cYY[i * BLML_CMAX + j];
+                            }
+                        for (int i = 1; i < cnt; ++i) {
+                            sl->ca[i - 1] = sl->ca[i]; sl->cb[i - 1] = sl->cb[i];
+                            rc[i - 1] = rc[i]; yc[i - 1] = yc[i];
+                        }
+                        c = cnt - 1;
+                    }
+                    for (int i = 0; i < c; ++i) {
+                        sl->cR[i * BLML_CMAX + c] = rc[i];
+                        sl->cR[c * BLML_CMAX + i] = 0.0;
+                        sl->cYY[i * BLML_CMAX + c] = yc[i];
+                        sl->cYY[c * BLML_CMAX + i] = yc[i];
+                    }
+                    sl->cR[c * BLML_CMAX + c] = a7[0];
+                    sl->cYY[c * BLML_CMAX + c] = a7[2];
+                    sl->ca[c] = a7[3];
+                    sl->cb[c] = a7[4];
+                }
+            }
+            if (accept) {
+                if (cnt < mh) ++cnt;
+                else head = (head + 1) % ph;
+                gam = a7[0] / a7[2];
+            }
+            __syncthreads();
+            gg = a7[5];
+            ee = a7[6];
+            { int tt = iE; iE = iEn; iEn = tt; tt = iU; iU = iUn; iUn = tt; tt = ig; ig = ign; ign = tt; }
+            L = Ln;
+            Q = Qn;
+            res = resid(vec(iE), vec(iU), Q > 0 ? n / sqrt(Q) : 1.0);
+            push_rho(res);
+            push_merit(L / n);
+        }
+        if (res >= P->tol && stop == BLML_STOP_TOL) stop = BLML_STOP_MAXSWEEPS;
         {
             const double Qc = dot(vec(iE), vec(iU));
             double *Ecn = vec(iEn);

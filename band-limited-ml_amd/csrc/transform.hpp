@@ -2,10 +2,12 @@
 // irfftn(rfftn(e, s=pad) * Kf, s=pad)[:m], blml_binned.py:156-165).
 //
 // A mode's kernel is a product of units (blml_trivial.py:330-335); each unit is an operator along its own lattice
-// axes, so S_k is the composition of the units' operators.  Every operator is applied line by line: a line (all
-// bins along one axis with the other indices fixed) is staged in LDS, transformed, and written back.
-//   padded axis   : linear convolution with an even row, as a zero-padded power-of-two FFT (in-place radix-2,
-//                   DIT forward from bit-reversed input, DIF inverse to bit-reversed output), spectrum in between;
+// axes, so S_k is the composition of the units' operators.  Every operator is applied line by line: the lines (all
+// bins along one axis with the other indices fixed) are staged in LDS in rounds of several lines, transformed, and
+// written back.
+//   padded axis   : linear convolution with an even row, as a zero-padded power-of-two FFT (Stockham autosort,
+//                   radices 16/8/4/2 with the radix-R butterflies in registers; natural order in and out), the
+//                   spectrum multiplied in between;
 //   circular axis : the periodic kernel's spectrum is a harmonic indicator, so the DFT is evaluated only at the
 //                   band's frequencies (a band-restricted DFT) and inverted from them.
 // Multi-axis units (ball, pball) transform axis by axis into a scratch buffer, multiply by their dense spectrum,
@@ -25,202 +27,295 @@ enum {
     OP_FFT_FULL = 6,   // L -> L forward (spectra of padded tables at plan creation)
 };
 
+#define BLML_DFT_TEAM 16   // threads per line of the band-restricted DFT passes
+
 struct PassShape {
     int team;   // threads per line
-    int buf;    // complex LDS elements per team
-    int roots;  // complex LDS elements of the shared root table (circular ops)
+    int lines;  // lines per round
+    int buf;    // complex LDS elements per line
+    int roots;  // complex LDS elements of the shared table: roots of unity (circular ops) or the two-level
+                // twiddle table lo[64], hi[L / 64] (FFT ops)
 };
 
-// Threads per line and LDS per line of an op.  The host uses the same rule to size the dynamic LDS.
-__host__ __device__ inline PassShape pass_shape(int op, int m, int L, int nF) {
-    PassShape s{1, 0, 0};
-    if (op == OP_CONV_PAD || op == OP_FFT_FWD || op == OP_FFT_INV || op == OP_FFT_FULL) {
-        int t = L / 2;
+__host__ __device__ inline bool is_fft_op(int op) {
+    return op == OP_CONV_PAD || op == OP_FFT_FWD || op == OP_FFT_INV || op == OP_FFT_FULL;
+}
+
+// Threads and LDS per line of an op, for a pass over nlines lines.  The host uses the same rule to size the
+// dynamic LDS.
+//   FFT ops: L / 16 threads per line (each holds 16 values: one radix-16 butterfly, or 16 / R radix-R ones), so up
+//            to BLML_NT / (L / 16) lines per round (L <= 16 BLML_NT); a line shorter than 16 is one thread's.
+//   DFT ops: 16 threads per line, up to BLML_NT / 16 lines per round.
+__host__ __device__ inline PassShape pass_shape(int op, int m, int L, int nF, long nlines) {
+    PassShape s{1, 1, 0, 0};
+    int cap;
+    if (is_fft_op(op)) {
+        int t = L / 16;
         if (t < 1) t = 1;
-        if (t > BLML_NT) t = BLML_NT;
         s.team = t;
+        cap = BLML_NT / t;
         s.buf = L;
+        s.roots = 64 + (L >= 64 ? L / 64 : 1);
     } else {
-        s.team = (m + nF <= 512) ? 64 : BLML_NT;
-        s.buf = (op == OP_CONV_CIRC) ? (m + nF) : (op == OP_DFT_FWD ? m : nF);
+        s.team = BLML_DFT_TEAM;
+        cap = BLML_NT / BLML_DFT_TEAM;
+        s.buf = m + nF;
         s.roots = m;
     }
+    s.lines = nlines < cap ? (int)(nlines > 0 ? nlines : 1) : cap;
     return s;
 }
 
-__host__ __device__ inline int pass_lds_elems(int op, int m, int L, int nF) {
-    PassShape s = pass_shape(op, m, L, nF);
-    return s.roots + (BLML_NT / s.team) * s.buf;
+__host__ __device__ inline int pass_lds_elems(int op, int m, int L, int nF, long nlines) {
+    PassShape s = pass_shape(op, m, L, nF, nlines);
+    return s.roots + s.lines * s.buf;
 }
 
-__device__ __forceinline__ int bitrev(int i, int logL) {
-    return logL == 0 ? 0 : (int)(__brev((unsigned)i) >> (32 - logL));
+// cos / sin of 2 pi k / 16
+__device__ __forceinline__ double c16cos(int k) {
+    const double c1 = 0.92387953251128674, c2 = 0.70710678118654752, c3 = 0.38268343236508977;
+    switch (k & 15) {
+        case 0: return 1.0;   case 1: return c1;   case 2: return c2;   case 3: return c3;
+        case 4: return 0.0;   case 5: return -c3;  case 6: return -c2;  case 7: return -c1;
+        case 8: return -1.0;  case 9: return -c1;  case 10: return -c2; case 11: return -c3;
+        case 12: return 0.0;  case 13: return c3;  case 14: return c2;  default: return c1;
+    }
 }
+__device__ __forceinline__ double c16sin(int k) { return c16cos(k - 4); }
 
-// Forward FFT in place on x[0..L): bit-reversed input, natural output.  tw[k] = exp(-2 pi i k / Lmax).
-template <typename T>
-__device__ void fft_dit_fwd(cpx<T> *x, int L, const cpx<T> *tw, int Lmax, int tid, int team) {
-    for (int h = 1; h < L; h <<= 1) {
-        const int step = Lmax / (2 * h);
-        for (int j = tid; j < L / 2; j += team) {
+__host__ __device__ constexpr int rev_bits(int r, int bits) {
+    int o = 0;
+    for (int i = 0; i < bits; ++i) o |= ((r >> i) & 1) << (bits - 1 - i);
+    return o;
+}
+template <int R>
+struct Log2 { static constexpr int v = R <= 1 ? 0 : 1 + Log2<R / 2>::v; };
+template <>
+struct Log2<1> { static constexpr int v = 0; };
+
+// In-register R-point DFT (R = 2..16), natural order in and out: y_k = sum_r v_r exp(dir 2 pi i r k / R).
+template <typename T, int R>
+__device__ __forceinline__ void dft_reg(cpx<T> *v, int dir) {
+    constexpr int LB = Log2<R>::v;
+    cpx<T> u[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) u[rev_bits(r, LB)] = v[r];
+#pragma unroll
+    for (int h = 1; h < R; h <<= 1) {
+#pragma unroll
+        for (int j = 0; j < R / 2; ++j) {
             const int pos = j & (h - 1);
             const int i0 = ((j - pos) << 1) + pos, i1 = i0 + h;
-            const cpx<T> w = tw[pos * step];
-            const cpx<T> a = x[i0], t = cmul(w, x[i1]);
-            x[i0] = cadd(a, t);
-            x[i1] = csub(a, t);
+            const int k = pos * (16 / (2 * h));
+            const cpx<T> wv = {(T)c16cos(k), (T)(dir * c16sin(k))};
+            const cpx<T> t = cmul(wv, u[i1]);
+            u[i1] = csub(u[i0], t);
+            u[i0] = cadd(u[i0], t);
         }
-        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = u[r];
+}
+
+// exp(dir 2 pi i k / L) from the two-level table in LDS: lo[j] = exp(-2 pi i j / L) (j < 64) and
+// hi[i] = exp(-2 pi i 64 i / L) (i < L / 64), so exp(-2 pi i k / L) = hi[k >> 6] lo[k & 63].
+template <typename T>
+__device__ __forceinline__ cpx<T> twiddle(const cpx<T> *tl, int L, int k, int dir) {
+    k &= L - 1;
+    cpx<T> w = tl[k & 63];
+    if (L > 64) w = cmul(w, tl[64 + (k >> 6)]);
+    return dir < 0 ? w : cpx<T>{w.x, -w.y};
+}
+
+// Fills the two-level table of length-L twiddles from the full-circle table tw[k] = exp(-2 pi i k / Lmax).
+template <typename T>
+__device__ __forceinline__ void load_twiddles(cpx<T> *tl, int L, const cpx<T> *tw, int Lmax) {
+    const int st = Lmax / L;
+    const int nhi = L >= 64 ? L / 64 : 1;
+    for (int j = threadIdx.x; j < 64 + nhi; j += BLML_NT) {
+        if (j < 64) tl[j] = j < L ? tw[(long)j * st] : cpx<T>{(T)1, (T)0};
+        else tl[j] = tw[(long)(j - 64) * 64 * st];
     }
 }
 
-// Inverse FFT (unscaled) in place: natural input, bit-reversed output.
-template <typename T>
-__device__ void fft_dif_inv(cpx<T> *x, int L, const cpx<T> *tw, int Lmax, int tid, int team) {
-    for (int h = L / 2; h >= 1; h >>= 1) {
-        const int step = Lmax / (2 * h);
-        for (int j = tid; j < L / 2; j += team) {
-            const int pos = j & (h - 1);
-            const int i0 = ((j - pos) << 1) + pos, i1 = i0 + h;
-            const cpx<T> w = tw[pos * step];
-            const cpx<T> a = x[i0], b = x[i1];
-            x[i0] = cadd(a, b);
-            x[i1] = cmulc(csub(a, b), w);   // * conj(w): the inverse transform's twiddle
+// One Stockham stage of radix R on a line x[0..L) in LDS (natural order in, natural order out).  The line's
+// butterflies j = j0 + b * nthr, b < NB = 16 / R, belong to this thread (nthr = L / 16 threads per line).
+// mul: multiply the inputs by spec (real) as they are read (the first inverse stage of a convolution).
+template <typename T, int R, int NB>
+__device__ __noinline__ void stockham_stage(cpx<T> *x, int L, int Ns, int j0, int nthr, const cpx<T> *tl,
+                                            int dir, const T *mul) {
+    cpx<T> v[R * NB];
+    const int stride = L / R;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = j0 + b * nthr;
+        const int k = j & (Ns - 1);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            cpx<T> a = x[j + r * stride];
+            if (mul) {
+                const T s = mul[j + r * stride];
+                a = {a.x * s, a.y * s};
+            }
+            if (r > 0 && Ns > 1) a = cmul(a, twiddle(tl, L, k * r * (L / (Ns * R)), dir));
+            v[b * R + r] = a;
         }
-        __syncthreads();
+        dft_reg<T, R>(v + b * R, dir);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = j0 + b * nthr;
+        const int k = j & (Ns - 1);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[base + r * Ns] = v[b * R + r];
+    }
+    __syncthreads();
+}
+
+// Whole FFT of a line of length L (<= 16 BLML_NT) in LDS, all threads participating (lines of a round share the
+// schedule).  dir = -1 forward, +1 inverse (unscaled).  mul (optional): spectrum applied on the first stage's reads.
+template <typename T>
+__device__ void fft_line(cpx<T> *x, int L, int j0, int team, const cpx<T> *tl, int dir, const T *mul) {
+    if (L < 16) {                       // one thread per line, one stage of radix L
+        switch (L) {
+            case 2: stockham_stage<T, 2, 1>(x, L, 1, 0, 1, tl, dir, mul); break;
+            case 4: stockham_stage<T, 4, 1>(x, L, 1, 0, 1, tl, dir, mul); break;
+            case 8: stockham_stage<T, 8, 1>(x, L, 1, 0, 1, tl, dir, mul); break;
+            default: __syncthreads(); __syncthreads(); break;   // L = 1: identity (same barriers as a stage)
+        }
+        return;
+    }
+    int Ns = 1;
+    bool first = true;
+    while (Ns < L) {
+        const int rem = L / Ns;
+        const T *m_ = first ? mul : nullptr;
+        if (rem >= 16) stockham_stage<T, 16, 1>(x, L, Ns, j0, team, tl, dir, m_), Ns *= 16;
+        else if (rem == 8) stockham_stage<T, 8, 2>(x, L, Ns, j0, team, tl, dir, m_), Ns *= 8;
+        else if (rem == 4) stockham_stage<T, 4, 4>(x, L, Ns, j0, team, tl, dir, m_), Ns *= 4;
+        else stockham_stage<T, 2, 8>(x, L, Ns, j0, team, tl, dir, m_), Ns *= 2;
+        first = false;
     }
 }
 
 template <typename T>
 struct Pass {
     int op;
-    int m, L, logL, nF;
+    int m, L, nF;
     int n_in, n_out;      // line length in / out
     int outer, inner;     // lines = outer * inner; element (o, i, r) at (o * n * inner + i * inner + r)
     const int *F;         // circular: frequencies (0 <= F < m)
     const T *spec;        // CONV ops: spectrum values (real)
 };
 
-// Applies one pass to every line.  src and dst may alias when n_in == n_out (each line is staged in LDS before it
-// is written back, and lines are disjoint).  All threads execute the same sequence of barriers.
+// Moves a round of lines between global memory and LDS (line l of the round at lbuf[l * stride_l + i]).
+// load: positions [0, width) of each LDS line, from global positions < nvalid (zero beyond; zero lines past the
+// end); store: positions [0, width) to global.  Contiguous lines (inner == 1) are walked element-fastest, strided
+// lines line-fastest, so consecutive threads touch consecutive addresses either way.
 template <typename T>
-__device__ void run_pass(const Pass<T> &p, const cpx<T> *src, cpx<T> *dst, cpx<T> *lds, const cpx<T> *tw,
+__device__ __forceinline__ void round_io(bool load, const Pass<T> &p, long base, int nl, int n, int nvalid,
+                                         int width, int off, cpx<T> *lbuf, int stride_l, const cpx<T> *src,
+                                         cpx<T> *dst) {
+    const long nlines = (long)p.outer * p.inner;
+    const int total = nl * width;
+    for (int idx = threadIdx.x; idx < total; idx += BLML_NT) {
+        int l, i;
+        if (p.inner == 1) { l = idx / width; i = idx - l * width; }
+        else { l = idx % nl; i = idx / nl; }
+        const long li = base + l;
+        cpx<T> *cell = lbuf + l * stride_l + off + i;
+        if (li >= nlines) {
+            if (load) *cell = {(T)0, (T)0};
+            continue;
+        }
+        const long o = li / p.inner, r = li - o * p.inner;
+        const size_t g = (size_t)o * n * p.inner + (size_t)i * p.inner + r;
+        if (load) *cell = i < nvalid ? src[g] : cpx<T>{(T)0, (T)0};
+        else dst[g] = *cell;
+    }
+}
+
+// Applies one pass to every line.  src and dst may alias when n_in == n_out (a round's lines are staged in LDS
+// before any is written back, and lines are disjoint).  All threads execute the same sequence of barriers.
+template <typename T>
+__device__ __noinline__ void run_pass(const Pass<T> &p, const cpx<T> *src, cpx<T> *dst, cpx<T> *lds, const cpx<T> *tw,
                          int Lmax) {
-    const PassShape sh = pass_shape(p.op, p.m, p.L, p.nF);
-    const int team = sh.team, nteams = BLML_NT / team;
-    const int tm = threadIdx.x / team, tid = threadIdx.x % team;
+    const long nlines = (long)p.outer * p.inner;
+    const PassShape sh = pass_shape(p.op, p.m, p.L, p.nF, nlines);
+    const int team = sh.team, nl = sh.lines;
+    const int tm = threadIdx.x / team, tid = threadIdx.x - tm * team;
+    const bool has_line = tm < nl;
     cpx<T> *roots = lds;
-    cpx<T> *buf = lds + sh.roots + tm * sh.buf;
-    const bool circ = sh.roots > 0;
-    if (circ) {
+    cpx<T> *lines = lds + sh.roots;
+    cpx<T> *buf = lines + (has_line ? tm : 0) * sh.buf;
+    if (is_fft_op(p.op)) {
+        load_twiddles<T>(roots, p.L, tw, Lmax);
+    } else {
         for (int j = threadIdx.x; j < p.m; j += BLML_NT) {
             double sn, cs;
             sincospi(-2.0 * (double)j / (double)p.m, &sn, &cs);
             roots[j] = {(T)cs, (T)sn};
         }
-        __syncthreads();
     }
-    const long nlines = (long)p.outer * p.inner;
-    for (long base = 0; base < nlines; base += nteams) {
-        const long li = base + tm;
-        const bool act = li < nlines;
-        long o = 0, r = 0;
-        if (act) {
-            o = li / p.inner;
-            r = li - o * p.inner;
-        }
-        const cpx<T> *sl = src + (size_t)o * p.n_in * p.inner + r;
-        cpx<T> *dl = dst + (size_t)o * p.n_out * p.inner + r;
-        const long st = p.inner;
-        switch (p.op) {
-            case OP_CONV_PAD:
-            case OP_FFT_FWD:
-            case OP_FFT_FULL: {
-                const int nin = (p.op == OP_FFT_FULL) ? p.L : p.m;
-                for (int i = tid; i < p.L; i += team) buf[i] = {(T)0, (T)0};
-                __syncthreads();
-                if (act)
-                    for (int i = tid; i < nin; i += team) buf[bitrev(i, p.logL)] = sl[i * st];
-                __syncthreads();
-                fft_dit_fwd(buf, p.L, tw, Lmax, tid, team);
-                if (p.op == OP_CONV_PAD) {
-                    for (int q = tid; q < p.L; q += team) {
-                        const T s = p.spec[q];
-                        buf[q] = {buf[q].x * s, buf[q].y * s};
-                    }
-                    __syncthreads();
-                    fft_dif_inv(buf, p.L, tw, Lmax, tid, team);
-                    if (act)
-                        for (int i = tid; i < p.m; i += team) dl[i * st] = buf[bitrev(i, p.logL)];
-                } else {
-                    if (act)
-                        for (int q = tid; q < p.L; q += team) dl[q * st] = buf[q];
-                }
-                __syncthreads();
-                break;
+    for (long base = 0; base < nlines; base += nl) {
+        if (is_fft_op(p.op)) {
+            const int nin = (p.op == OP_FFT_FULL || p.op == OP_FFT_INV) ? p.L : p.m;
+            round_io<T>(true, p, base, nl, p.n_in, nin, p.L, 0, lines, sh.buf, src, nullptr);
+            __syncthreads();
+            const int j0 = has_line ? tid : 0;
+            if (p.op == OP_FFT_INV) {
+                fft_line<T>(buf, p.L, j0, team, roots, +1, nullptr);
+            } else {
+                fft_line<T>(buf, p.L, j0, team, roots, -1, nullptr);
+                if (p.op == OP_CONV_PAD) fft_line<T>(buf, p.L, j0, team, roots, +1, p.spec);
             }
-            case OP_FFT_INV: {
-                if (act)
-                    for (int q = tid; q < p.L; q += team) buf[q] = sl[q * st];
-                __syncthreads();
-                fft_dif_inv(buf, p.L, tw, Lmax, tid, team);
-                if (act)
-                    for (int i = tid; i < p.m; i += team) dl[i * st] = buf[bitrev(i, p.logL)];
-                __syncthreads();
-                break;
-            }
-            case OP_CONV_CIRC:
-            case OP_DFT_FWD: {
-                cpx<T> *xl = buf;
-                cpx<T> *Xf = buf + p.m;
-                if (act)
-                    for (int i = tid; i < p.m; i += team) xl[i] = sl[i * st];
-                __syncthreads();
+            const int nout = (p.op == OP_FFT_FWD || p.op == OP_FFT_FULL) ? p.L : p.m;
+            round_io<T>(false, p, base, nl, p.n_out, nout, nout, 0, lines, sh.buf, nullptr, dst);
+            __syncthreads();
+        } else {
+            // band-restricted DFT on line buffers [0, m) (values) and [m, m + nF) (band)
+            cpx<T> *xl = buf, *Xf = buf + p.m;
+            if (p.op == OP_DFT_INV)
+                round_io<T>(true, p, base, nl, p.n_in, p.nF, p.nF, p.m, lines, sh.buf, src, nullptr);
+            else
+                round_io<T>(true, p, base, nl, p.n_in, p.m, p.m, 0, lines, sh.buf, src, nullptr);
+            __syncthreads();
+            if (p.op != OP_DFT_INV) {
                 for (int q = tid; q < p.nF; q += team) {
-                    const int f = p.F[q];
+                    const int fq = p.F[q];
                     int idx = 0;
                     cpx<T> acc = {(T)0, (T)0};
                     for (int j = 0; j < p.m; ++j) {
                         acc = cadd(acc, cmul(xl[j], roots[idx]));
-                        idx += f;
+                        idx += fq;
                         if (idx >= p.m) idx -= p.m;
                     }
                     if (p.op == OP_CONV_CIRC) {
                         const T s = p.spec[q];
-                        Xf[q] = {acc.x * s, acc.y * s};
-                    } else if (act) {
-                        dl[q * st] = acc;
+                        acc = {acc.x * s, acc.y * s};
                     }
+                    if (has_line) Xf[q] = acc;
                 }
                 __syncthreads();
-                if (p.op == OP_CONV_CIRC) {
-                    for (int i = tid; i < p.m; i += team) {
-                        cpx<T> acc = {(T)0, (T)0};
-                        for (int q = 0; q < p.nF; ++q) {
-                            const int idx = (int)(((long)p.F[q] * i) % p.m);
-                            acc = cadd(acc, cmulc(Xf[q], roots[idx]));
-                        }
-                        if (act) dl[i * st] = acc;
-                    }
-                    __syncthreads();
-                }
-                break;
             }
-            case OP_DFT_INV: {
-                if (act)
-                    for (int q = tid; q < p.nF; q += team) buf[q] = sl[q * st];
-                __syncthreads();
+            if (p.op == OP_DFT_FWD) {
+                round_io<T>(false, p, base, nl, p.n_out, p.nF, p.nF, p.m, lines, sh.buf, nullptr, dst);
+            } else {
+                // inverse from the band: out[i] = sum_q Xf[q] exp(+2 pi i F_q i / m), into [0, m)
                 for (int i = tid; i < p.m; i += team) {
                     cpx<T> acc = {(T)0, (T)0};
                     for (int q = 0; q < p.nF; ++q) {
-                        const int idx = (int)(((long)p.F[q] * i) % p.m);
-                        acc = cadd(acc, cmulc(buf[q], roots[idx]));
+                        const int idx = (int)(((unsigned)p.F[q] * (unsigned)i) % (unsigned)p.m);
+                        acc = cadd(acc, cmulc(Xf[q], roots[idx]));
                     }
-                    if (act) dl[i * st] = acc;
+                    if (has_line) xl[i] = acc;
                 }
                 __syncthreads();
-                break;
+                round_io<T>(false, p, base, nl, p.n_out, p.m, p.m, 0, lines, sh.buf, nullptr, dst);
             }
+            __syncthreads();
         }
     }
 }
@@ -233,7 +328,7 @@ struct MatvecCtx {
     const blml_factor_t *facs;   // the plan's factor table
     const T *spec;             // the plan's spectra
     const int *freqs;          // the plan's frequency lists
-    const cpx<T> *tw;          // twiddles of the longest FFT
+    const cpx<T> *tw;          // full-circle twiddles of the longest FFT
     int Lmax;
     cpx<T> *X, *Y;             // scratch (scratch_len complex each)
     cpx<T> *lds;
@@ -267,12 +362,6 @@ __device__ inline void line_geom(const int *ext, int d, int axis, int &outer, in
     for (int j = axis + 1; j < d; ++j) inner *= ext[j];
 }
 
-__device__ inline int ilog2(int L) {
-    int l = 0;
-    while ((1 << l) < L) ++l;
-    return l;
-}
-
 // Applies one factor to the complex lattice array in c.X (in place; group factors use c.Y as well).
 template <typename T>
 __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa) {
@@ -288,11 +377,9 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa) {
             p.nF = fa.tlen[0];
             p.F = c.freqs + fa.freq_off[0];
             p.L = 0;
-            p.logL = 0;
         } else {
             p.op = OP_CONV_PAD;
             p.L = fa.tlen[0];
-            p.logL = ilog2(p.L);
             p.nF = 0;
             p.F = nullptr;
         }
@@ -309,10 +396,10 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa) {
         p.n_in = ext[a];
         line_geom<T>(ext, c.d, a, p.outer, p.inner);
         if (fa.circ[ai]) {
-            p.op = OP_DFT_FWD; p.nF = fa.tlen[ai]; p.F = c.freqs + fa.freq_off[ai]; p.L = 0; p.logL = 0;
+            p.op = OP_DFT_FWD; p.nF = fa.tlen[ai]; p.F = c.freqs + fa.freq_off[ai]; p.L = 0;
             p.n_out = p.nF;
         } else {
-            p.op = OP_FFT_FWD; p.L = fa.tlen[ai]; p.logL = ilog2(p.L); p.nF = 0; p.F = nullptr;
+            p.op = OP_FFT_FWD; p.L = fa.tlen[ai]; p.nF = 0; p.F = nullptr;
             p.n_out = p.L;
         }
         p.spec = nullptr;
@@ -329,9 +416,9 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa) {
         p.n_out = p.m;
         line_geom<T>(ext, c.d, a, p.outer, p.inner);
         if (fa.circ[ai]) {
-            p.op = OP_DFT_INV; p.nF = fa.tlen[ai]; p.F = c.freqs + fa.freq_off[ai]; p.L = 0; p.logL = 0;
+            p.op = OP_DFT_INV; p.nF = fa.tlen[ai]; p.F = c.freqs + fa.freq_off[ai]; p.L = 0;
         } else {
-            p.op = OP_FFT_INV; p.L = fa.tlen[ai]; p.logL = ilog2(p.L); p.nF = 0; p.F = nullptr;
+            p.op = OP_FFT_INV; p.L = fa.tlen[ai]; p.nF = 0; p.F = nullptr;
         }
         p.spec = nullptr;
         run_pass<T>(p, cur, oth, c.lds, c.tw, c.Lmax);
