@@ -12,6 +12,13 @@ struct cpx {
     T x, y;
 };
 
+// The launch's dynamic LDS.  Functions that are not inlined reach LDS only through this declaration (and offsets
+// into it), never through pointer arguments: a pointer argument is generic, which turns every LDS access into a
+// FLAT instruction, while a pointer derived from a __shared__ object in the same function compiles to ds_ ops.
+extern __shared__ __align__(16) unsigned char g_dyn_lds[];
+template <typename T>
+__device__ __forceinline__ cpx<T> *dyn_lds() { return reinterpret_cast<cpx<T> *>(g_dyn_lds); }
+
 template <typename T>
 __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};

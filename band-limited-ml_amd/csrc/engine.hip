@@ -35,8 +35,6 @@ int fail(int code, const std::string &msg) {
         if (e_ != hipSuccess) return fail(BLML_ERUNTIME, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kStaticLds = (int)sizeof(SolverLds) + 64;
-constexpr int kMaxLds = 160 * 1024;
 
 // ------------------------------------------------------------------------------------------------ kernels
 template <typename T>
@@ -85,7 +83,6 @@ __global__ void twiddle_kernel(cpx<T> *tw, int Lmax) {
 template <typename T>
 __global__ void __launch_bounds__(BLML_NT) spec_kernel(DevPlan<T> p, int nfacs, const int *fac_m, cpx<T> *tmp,
                                                        const long *tmp_off) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     const int fi = blockIdx.x;
     if (fi >= nfacs) return;
     const blml_factor_t fa = p.facs[fi];
@@ -129,8 +126,9 @@ __global__ void __launch_bounds__(BLML_NT) spec_kernel(DevPlan<T> p, int nfacs, 
         ps.F = nullptr;
         ps.spec = nullptr;
         ps.n_in = ps.n_out = ps.L;
+        ps.pairs = false;
         line_geom<T>(ext, fa.naxes, a, ps.outer, ps.inner);
-        run_pass<T>(ps, cur, oth, (cpx<T> *)lds_raw, p.tw, p.Lmax);
+        run_pass<T>(ps, cur, oth, p.tw, p.Lmax);
         cpx<T> *t = cur; cur = oth; oth = t;
     }
     double inv = 1.0;
@@ -139,7 +137,7 @@ __global__ void __launch_bounds__(BLML_NT) spec_kernel(DevPlan<T> p, int nfacs, 
 }
 
 template <typename T>
-__device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, MatvecCtx<T> &mv, void *lds) {
+__device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, MatvecCtx<T> &mv) {
     mv.d = f.d;
     for (int j = 0; j < BLML_MAXD; ++j) mv.shape[j] = j < f.d ? f.shape[j] : 1;
     mv.facs = p.facs;
@@ -149,17 +147,15 @@ __device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, Matvec
     mv.Lmax = p.Lmax;
     mv.X = (cpx<T> *)(p.ws + f.ws_off);
     mv.Y = mv.X + f.scratch_len;
-    mv.lds = (cpx<T> *)lds;
 }
 
 template <typename T>
 __global__ void __launch_bounds__(BLML_NT) matvec_kernel(DevPlan<T> p, const T *E, T *U) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     const int fi = blockIdx.x;
     if (fi >= p.nfits) return;
     const blml_fit_t f = p.fits[fi];
     MatvecCtx<T> mv;
-    make_matvec_ctx<T>(p, f, mv, lds_raw);
+    make_matvec_ctx<T>(p, f, mv);
     const int nc = f.complex_g ? 2 : 1;
     for (int k = 0; k < f.K; ++k) {
         const long off = f.out_off + (long)k * f.B * nc;
@@ -174,8 +170,6 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
                                                       const double *init, const double *signs, T *e_out, T *se_out,
                                                       double *density, blml_result_t *results, double *merit_hist,
                                                       double *rho_hist, double tick_hz) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ SolverLds sl;
     __shared__ int s_fit;
     for (;;) {
         if (threadIdx.x == 0) s_fit = atomicAdd(queue, 1);
@@ -186,10 +180,9 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
         const int fi = order[qi];
         const blml_fit_t f = p.fits[fi];
         FitSolver<T> S;
-        make_matvec_ctx<T>(p, f, S.mv, lds_raw);
+        make_matvec_ctx<T>(p, f, S.mv);
         S.f = &p.fits[fi];
         S.P = &p.params;
-        S.sl = &sl;
         S.B = f.B;
         S.n = f.n;
         for (int k = 0; k < BLML_MAXK; ++k) S.s[k] = f.s[k];
@@ -323,7 +316,7 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
             density[f.dens_off + b] = dens;
             if (S.w[b] > 0) ll += S.w[b] * log(dens);
         }
-        ll = block_sum1(ll, sl.red);
+        ll = block_sum1(ll, g_sl.red);
         if (threadIdx.x == 0) {
             blml_result_t r;
             r.loglik = ll / f.n;
@@ -354,7 +347,6 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
 template <typename T>
 __global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<T> p, const T *e_hat, const double *counts,
                                                        int max_iter, double tol, double *lam_out, int *iters) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[BLML_NWAVE * BLML_NRED];
     const int fi = blockIdx.x;
     if (fi >= p.nfits) return;
@@ -364,7 +356,7 @@ __global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<T> p, const T *e_
         return;
     }
     MatvecCtx<T> mv;
-    make_matvec_ctx<T>(p, f, mv, lds_raw);
+    make_matvec_ctx<T>(p, f, mv);
     const int nc = f.complex_g ? 2 : 1;
     const int B = f.B;
     double *dh = (double *)(p.ws + f.ws_off + pool_bytes_off(f, (int)sizeof(T)));   // B doubles
@@ -428,7 +420,6 @@ __global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<T> p, const T *e_
 __global__ void __launch_bounds__(BLML_NT) finish64_kernel(DevPlan<double> p, float *se_out,
                                                           const double *counts, double *density,
                                                           blml_result_t *results) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[BLML_NWAVE * BLML_NRED];
     const int fi = blockIdx.x;
     if (fi >= p.nfits) return;
@@ -443,7 +434,6 @@ __global__ void __launch_bounds__(BLML_NT) finish64_kernel(DevPlan<double> p, fl
     mv.Lmax = p.Lmax;
     mv.X = (cpx<double> *)(p.ws + f.ws_off + fin_bytes_off(p.params, f, 4));
     mv.Y = mv.X + f.scratch_len;
-    mv.lds = (cpx<double> *)lds_raw;
     const int nc = f.complex_g ? 2 : 1;
     const long V = (long)f.K * f.B * nc;
     // the fit's FP64 state: E in pool slot 0; U64 overwrites slot 1
@@ -668,6 +658,8 @@ int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs,
                 }
             }
         }
+        if (scratch >= (1L << 31) || (long)f.B * 2 >= (1L << 31))
+            return fail(BLML_EVALUE, "a fit's transform intermediate exceeds 2^31 values");
         f.scratch_len = scratch;
         const long Vmax = vmax_of(f);
         // bytes: transform scratch, FP64 state pool (also the certificate's 3 B doubles: n_slots >= 3), and for
@@ -684,15 +676,16 @@ int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs,
     pl.ws_elems = ws;
     pl.Lmax = Lmax;
     lds_elems = std::max(lds_elems, 16);
-    const int lds = lds_elems * (int)(2 * pl.esize);
-    if (lds + kStaticLds > kMaxLds)
-        return fail(BLML_EVALUE, "a transform line needs " + std::to_string(lds) + " bytes of LDS (limit " +
-                                     std::to_string(kMaxLds - kStaticLds) + "): lattice axis too long");
-    pl.lds_bytes = lds;
+    pl.lds_bytes = lds_elems * (int)(2 * pl.esize);
     pl.lds64_bytes = lds_elems * 16;
-    if (pl.esize == 4 && pl.lds64_bytes + kStaticLds > kMaxLds)
-        return fail(BLML_EVALUE, "the FP64 finishing pass of a line needs " + std::to_string(pl.lds64_bytes) +
-                                     " bytes of LDS: lattice axis too long");
+    return BLML_OK;
+}
+
+// raises `st` to the static LDS (bytes of __shared__ arrays) of kernel `kern`
+int static_lds(const void *kern, int &st) {
+    hipFuncAttributes a;
+    HIPCHECK(hipFuncGetAttributes(&a, kern));
+    st = std::max(st, (int)a.sharedSizeBytes);
     return BLML_OK;
 }
 
@@ -704,6 +697,24 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
     int64_t spec_total = 0;
     int rc = layout(pl, fdata, n_fdata, freqs, n_freqs, fac_m, tmp_off, tmp_total, spec_total);
     if (rc) return rc;
+    // a launch's LDS = its kernel's static arrays + the dynamic line buffers; both must fit one workgroup's LDS
+    int dev = 0, max_lds = 0, st = 0, st64 = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    rc = static_lds((const void *)fit_kernel<T>, st);
+    if (!rc) rc = static_lds((const void *)matvec_kernel<T>, st);
+    if (!rc) rc = static_lds((const void *)cert_kernel<T>, st);
+    if (!rc) rc = static_lds((const void *)spec_kernel<T>, st);
+    if (!rc && sizeof(T) == 4) rc = static_lds((const void *)finish64_kernel, st64);
+    if (!rc && sizeof(T) == 4) rc = static_lds((const void *)spec_kernel<double>, st64);
+    if (rc) return rc;
+    if (pl.lds_bytes + st > max_lds)
+        return fail(BLML_EVALUE, "a transform line needs " + std::to_string(pl.lds_bytes) + " bytes of LDS (limit " +
+                                     std::to_string(max_lds - st) + "): lattice axis too long");
+    if (sizeof(T) == 4 && pl.lds64_bytes + st64 > max_lds)
+        return fail(BLML_EVALUE, "the FP64 finishing pass of a line needs " + std::to_string(pl.lds64_bytes) +
+                                     " bytes of LDS (limit " + std::to_string(max_lds - st64) +
+                                     "): lattice axis too long");
     HIPCHECK(hipMalloc(&pl.d_fits, sizeof(blml_fit_t) * std::max(1, pl.nfits)));
     HIPCHECK(hipMalloc(&pl.d_facs, sizeof(blml_factor_t) * std::max(1, pl.nfacs)));
     HIPCHECK(hipMalloc(&pl.d_fdata, sizeof(double) * std::max<int64_t>(1, n_fdata)));
@@ -759,8 +770,7 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
     HIPCHECK(hipFree(d_tmp_off));
     HIPCHECK(hipFree(d_tmp));
     // persistent grid: as many workgroups as fit on the device at once (capped by the number of fits)
-    int dev = 0, ncu = 0, occ = 0;
-    HIPCHECK(hipGetDevice(&dev));
+    int ncu = 0, occ = 0;
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fit_kernel<T>, BLML_NT, pl.lds_bytes));
     int khz = 0;

@@ -34,6 +34,10 @@ struct SolverLds {
     double cp1[BLML_CMAX], cp2[BLML_CMAX];
 };
 
+// The solver's LDS (reduction scratch, merit ring, L-BFGS scalars): a file-scope __shared__ object, so every access
+// compiles to ds_ instructions (see dyn_lds).
+__shared__ SolverLds g_sl;
+
 struct FitOut {
     int n_sweeps, stop, converged, n_and, n_damp, n_grad;
     double residual;
@@ -75,7 +79,6 @@ struct FitSolver {
     MatvecCtx<T> mv;
     const blml_fit_t *f;
     const blml_params_t *P;
-    SolverLds *sl;
     int B, K, nc;          // active: the real start runs K = 1, nc = 1
     bool psd;
     bool inc;              // incremental mat-vecs (FP32 transforms)
@@ -125,7 +128,7 @@ struct FitSolver {
         double acc = 0.0;
         const long n_ = V();
         for (long i = threadIdx.x; i < n_; i += BLML_NT) acc += a[i] * b[i];
-        return block_sum1(acc, sl->red);
+        return block_sum1(acc, g_sl.red);
     }
     __device__ void copy(double *dst, const double *src) {
         const long n_ = V();
@@ -162,7 +165,7 @@ struct FitSolver {
             for (int b = threadIdx.x; b < B; b += BLML_NT) a[0] += cb(b) * log(fmax(Fb(U, b), 1e-30));
             const long n_ = V();
             for (long i = threadIdx.x; i < n_; i += BLML_NT) a[1] += E[i] * U[i];
-            block_sum<2>(a, sl->red);
+            block_sum<2>(a, g_sl.red);
             const double Q = a[1];
             if (!(Q > 0)) return -INFINITY;
             return a[0] / n - n * log(Q);
@@ -177,7 +180,7 @@ struct FitSolver {
                 if (sgb * e <= 0) a[2] += 1.0;
             }
         }
-        block_sum<3>(a, sl->red);
+        block_sum<3>(a, g_sl.red);
         if (a[2] > 0) return -INFINITY;
         return -(0.5 * a[0] - a[1]) / n;
     }
@@ -264,7 +267,7 @@ struct FitSolver {
             }
             r = fmax(r, sqrt(acc) / sqrt(fmax(nrm2, 1e-300)));
         }
-        return block_max(r, sl->red);
+        return block_max(r, g_sl.red);
     }
 
     // L = sum w log F - n log Q (blml_binned.py:278-284); writes Vt = (w / F) U on occupied bins when Vt is given
@@ -284,7 +287,7 @@ struct FitSolver {
                 }
             }
         }
-        block_sum<2>(a, sl->red);
+        block_sum<2>(a, g_sl.red);
         *Qout = a[1];
         if (!(a[1] > 0)) return -INFINITY;
         return a[0] - n * log(a[1]);
@@ -306,14 +309,14 @@ struct FitSolver {
     // ------------------------------------------------------------------ histories
     __device__ void push_merit(double v) {
         if (threadIdx.x == 0) {
-            sl->ring[ring_n % BLML_RING] = v;
+            g_sl.ring[ring_n % BLML_RING] = v;
             if (rec && merit_hist && nhist_m < hist_cap) merit_hist[nhist_m] = v;
         }
         ++ring_n;
         if (rec) ++nhist_m;
         __syncthreads();
     }
-    __device__ double ring_back(int k) const { return sl->ring[(ring_n - 1 - k) % BLML_RING]; }
+    __device__ double ring_back(int k) const { return g_sl.ring[(ring_n - 1 - k) % BLML_RING]; }
     __device__ void push_rho(double v) {
         if (threadIdx.x == 0 && rec && rho_hist && nhist_r < hist_cap) rho_hist[nhist_r] = v;
         if (rec) ++nhist_r;
@@ -336,7 +339,7 @@ struct FitSolver {
                 for (int j = i; j < m; ++j) acc[p++] += dF[i] * dF[j];
             for (int i = 0; i < m; ++i) acc[p++] += dF[i] * fe;
         }
-        block_sum<20>(acc, sl->red);
+        block_sum<20>(acc, g_sl.red);
         double G[BLML_MAXAND][BLML_MAXAND + 1];
         int p = 0;
         double tr = 0.0;
@@ -579,21 +582,21 @@ struct FitSolver {
             copy(q, vec(ig));
             for (int j = cnt - 1; j >= 0; --j) {
                 const int sl_ = (head + j) % ph;
-                const double a = sl->lb_r[sl_] * dot(vec(9 + sl_), q);
-                if (threadIdx.x == 0) sl->lb_a[j] = a;
+                const double a = g_sl.lb_r[sl_] * dot(vec(9 + sl_), q);
+                if (threadIdx.x == 0) g_sl.lb_a[j] = a;
                 axpy(q, q, -a, vec(9 + ph + sl_));
             }
             if (cnt > 0) {
                 const int nw = (head + cnt - 1) % ph;
-                scale(q, sl->lb_sy[nw] / sl->lb_yy[nw]);
+                scale(q, g_sl.lb_sy[nw] / g_sl.lb_yy[nw]);
             } else {
                 const double ee = dot(vec(iE), vec(iE)), gg = dot(vec(ig), vec(ig));
                 scale(q, 1e-3 * sqrt(ee / fmax(gg, 1e-300)));
             }
             for (int j = 0; j < cnt; ++j) {
                 const int sl_ = (head + j) % ph;
-                const double bb = sl->lb_r[sl_] * dot(vec(9 + ph + sl_), q);
-                axpy(q, q, sl->lb_a[j] - bb, vec(9 + sl_));
+                const double bb = g_sl.lb_r[sl_] * dot(vec(9 + ph + sl_), q);
+                axpy(q, q, g_sl.lb_a[j] - bb, vec(9 + sl_));
             }
             double slope = dot(vec(ig), q);
             if (!(slope > 0)) {
@@ -651,12 +654,12 @@ struct FitSolver {
                     a3[1] += sv * sv;
                     a3[2] += yv * yv;
                 }
-                block_sum<3>(a3, sl->red);
+                block_sum<3>(a3, g_sl.red);
                 if (a3[0] > 1e-12 * sqrt(a3[1] * a3[2])) {
                     if (threadIdx.x == 0) {
-                        sl->lb_r[slot] = 1.0 / a3[0];
-                        sl->lb_sy[slot] = a3[0];
-                        sl->lb_yy[slot] = a3[2];
+                        g_sl.lb_r[slot] = 1.0 / a3[0];
+                        g_sl.lb_sy[slot] = a3[0];
+                        g_sl.lb_yy[slot] = a3[2];
                     }
                     if (cnt < mh) ++cnt;
                     else head = (head + 1) % ph;
@@ -735,9 +738,9 @@ struct FitSolver {
                     }
                 }
             }
-            block_sum<2 * BLML_CCHUNK>(acc, sl->red);
+            block_sum<2 * BLML_CCHUNK>(acc, g_sl.red);
             if (threadIdx.x == 0)
-                for (int i = 0; i < nc_; ++i) { sl->csg[c0 + i] = acc[i]; sl->cyg[c0 + i] = acc[BLML_CCHUNK + i]; }
+                for (int i = 0; i < nc_; ++i) { g_sl.csg[c0 + i] = acc[i]; g_sl.cyg[c0 + i] = acc[BLML_CCHUNK + i]; }
         }
         __syncthreads();
     }
@@ -776,22 +779,22 @@ struct FitSolver {
                 // small systems, solved redundantly by every thread from LDS
                 double u[BLML_CMAX], w[BLML_CMAX];
                 for (int i = cnt - 1; i >= 0; --i) {            // R u = a (upper triangular)
-                    double v = sl->ca[i];
-                    for (int j = i + 1; j < cnt; ++j) v -= sl->cR[i * BLML_CMAX + j] * u[j];
-                    u[i] = v / sl->cR[i * BLML_CMAX + i];
+                    double v = g_sl.ca[i];
+                    for (int j = i + 1; j < cnt; ++j) v -= g_sl.cR[i * BLML_CMAX + j] * u[j];
+                    u[i] = v / g_sl.cR[i * BLML_CMAX + i];
                 }
                 for (int i = 0; i < cnt; ++i) {                  // w = (D + gamma Y^T Y) u - gamma b
-                    double v = sl->cR[i * BLML_CMAX + i] * u[i] - gam * sl->cb[i];
-                    for (int j = 0; j < cnt; ++j) v += gam * sl->cYY[i * BLML_CMAX + j] * u[j];
+                    double v = g_sl.cR[i * BLML_CMAX + i] * u[i] - gam * g_sl.cb[i];
+                    for (int j = 0; j < cnt; ++j) v += gam * g_sl.cYY[i * BLML_CMAX + j] * u[j];
                     w[i] = v;
                 }
                 for (int i = 0; i < cnt; ++i) {                  // R^T p1 = w (lower triangular)
                     double v = w[i];
-                    for (int j = 0; j < i; ++j) v -= sl->cR[j * BLML_CMAX + i] * w[j];
-                    w[i] = v / sl->cR[i * BLML_CMAX + i];
+                    for (int j = 0; j < i; ++j) v -= g_sl.cR[j * BLML_CMAX + i] * w[j];
+                    w[i] = v / g_sl.cR[i * BLML_CMAX + i];
                 }
                 if (threadIdx.x == 0)
-                    for (int i = 0; i < cnt; ++i) { sl->cp1[i] = w[i]; sl->cp2[i] = -gam * u[i]; }
+                    for (int i = 0; i < cnt; ++i) { g_sl.cp1[i] = w[i]; g_sl.cp2[i] = -gam * u[i]; }
                 __syncthreads();
                 double acc = 0.0;
                 const double *g = vec(ig);
@@ -799,12 +802,12 @@ struct FitSolver {
                     double v = gam * g[e];
                     for (int i = 0; i < cnt; ++i) {
                         const int sl_ = (head + i) % ph;
-                        v += sl->cp1[i] * vec(9 + sl_)[e] + sl->cp2[i] * vec(9 + ph + sl_)[e];
+                        v += g_sl.cp1[i] * vec(9 + sl_)[e] + g_sl.cp2[i] * vec(9 + ph + sl_)[e];
                     }
                     q[e] = v;
                     acc += g[e] * v;
                 }
-                slope = block_sum1(acc, sl->red);
+                slope = block_sum1(acc, g_sl.red);
             }
             if (!(slope > 0)) {
                 cnt = 0; head = 0;
@@ -815,7 +818,7 @@ struct FitSolver {
                     q[e] = g[e] * sc;
                     acc += g[e] * q[e];
                 }
-                slope = block_sum1(acc, sl->red);
+                slope = block_sum1(acc, g_sl.red);
             }
             const long long td1 = wall_clock64();
             t_dir += td1 - td0;
@@ -865,7 +868,7 @@ struct FitSolver {
                     a7[5] += gn[e] * gn[e];
                     a7[6] += En[e] * En[e];
                 }
-                block_sum<7>(a7, sl->red);
+                block_sum<7>(a7, g_sl.red);
             }
             // update the small matrices (thread 0) and a, b for the new gradient
             const bool accept = a7[0] > 1e-12 * sqrt(a7[1] * a7[2]);
@@ -873,34 +876,34 @@ struct FitSolver {
                 // R's new column and Y^T Y's new row / column for the kept pairs, from the previous a, b
                 double rc[BLML_CMAX + 1], yc[BLML_CMAX + 1];
                 for (int i = 0; i < cnt; ++i) {
-                    rc[i] = sl->ca[i] - sl->csg[i];
-                    yc[i] = sl->cb[i] - sl->cyg[i];
+                    rc[i] = g_sl.ca[i] - g_sl.csg[i];
+                    yc[i] = g_sl.cb[i] - g_sl.cyg[i];
                 }
-                for (int i = 0; i < cnt; ++i) { sl->ca[i] = sl->csg[i]; sl->cb[i] = sl->cyg[i]; }
+                for (int i = 0; i < cnt; ++i) { g_sl.ca[i] = g_sl.csg[i]; g_sl.cb[i] = g_sl.cyg[i]; }
                 if (accept) {
                     int c = cnt;
                     if (cnt == mh) {                            // drop the oldest pair
                         for (int i = 1; i < cnt; ++i)
                             for (int j = 1; j < cnt; ++j) {
-                                sl->cR[(i - 1) * BLML_CMAX + (j - 1)] = sl->cR[i * BLML_CMAX + j];
-                                sl->cYY[(i - 1) * BLML_CMAX + (j - 1)] = sl->cYY[i * BLML_CMAX + j];
+                                g_sl.cR[(i - 1) * BLML_CMAX + (j - 1)] = g_sl.cR[i * BLML_CMAX + j];
+                                g_sl.cYY[(i - 1) * BLML_CMAX + (j - 1)] = g_sl.cYY[i * BLML_CMAX + j];
                             }
                         for (int i = 1; i < cnt; ++i) {
-                            sl->ca[i - 1] = sl->ca[i]; sl->cb[i - 1] = sl->cb[i];
+                            g_sl.ca[i - 1] = g_sl.ca[i]; g_sl.cb[i - 1] = g_sl.cb[i];
                             rc[i - 1] = rc[i]; yc[i - 1] = yc[i];
                         }
                         c = cnt - 1;
                     }
                     for (int i = 0; i < c; ++i) {
-                        sl->cR[i * BLML_CMAX + c] = rc[i];
-                        sl->cR[c * BLML_CMAX + i] = 0.0;
-                        sl->cYY[i * BLML_CMAX + c] = yc[i];
-                        sl->cYY[c * BLML_CMAX + i] = yc[i];
+                        g_sl.cR[i * BLML_CMAX + c] = rc[i];
+                        g_sl.cR[c * BLML_CMAX + i] = 0.0;
+                        g_sl.cYY[i * BLML_CMAX + c] = yc[i];
+                        g_sl.cYY[c * BLML_CMAX + i] = yc[i];
                     }
-                    sl->cR[c * BLML_CMAX + c] = a7[0];
-                    sl->cYY[c * BLML_CMAX + c] = a7[2];
-                    sl->ca[c] = a7[3];
-                    sl->cb[c] = a7[4];
+                    g_sl.cR[c * BLML_CMAX + c] = a7[0];
+                    g_sl.cYY[c * BLML_CMAX + c] = a7[2];
+                    g_sl.ca[c] = a7[3];
+                    g_sl.cb[c] = a7[4];
                 }
             }
             if (accept) {

@@ -29,6 +29,11 @@ enum {
 
 #define BLML_DFT_TEAM 16   // threads per line of the band-restricted DFT passes
 
+// LDS layout of an FFT line: element i at i ^ ((i >> 4) & 15), a permutation within each block of 16.  A radix-16
+// stage writes with stride 16 (one LDS bank for the whole wavefront in plain order); the swizzle spreads those
+// writes over 16 offsets, while consecutive reads stay within a block (conflict-free), at no extra LDS.
+__host__ __device__ __forceinline__ int pidx(int i) { return i ^ ((i >> 4) & 15); }
+
 struct PassShape {
     int team;   // threads per line
     int lines;  // lines per round
@@ -55,7 +60,7 @@ __host__ __device__ inline PassShape pass_shape(int op, int m, int L, int nF, lo
         s.team = t;
         cap = BLML_NT / t;
         s.buf = L;
-        s.roots = 64 + (L >= 64 ? L / 64 : 1);
+        s.roots = L <= 1024 ? (L > 1 ? L : 2) : 64 + L / 64;
     } else {
         s.team = BLML_DFT_TEAM;
         cap = BLML_NT / BLML_DFT_TEAM;
@@ -93,6 +98,21 @@ struct Log2 { static constexpr int v = R <= 1 ? 0 : 1 + Log2<R / 2>::v; };
 template <>
 struct Log2<1> { static constexpr int v = 0; };
 
+// q = a / d, r = a % d for 0 <= a and 1 <= d, from a float reciprocal with one correction step when a < 2^24
+// (exact there; integer division is a long instruction sequence on the GPU), by integer division beyond.
+__device__ __forceinline__ int fdivmod(int a, int d, float inv_d, int &r) {
+    if (a >= (1 << 24)) {
+        const int q = a / d;
+        r = a - q * d;
+        return q;
+    }
+    int q = (int)((float)a * inv_d);
+    r = a - q * d;
+    if (r < 0) { --q; r += d; }
+    else if (r >= d) { ++q; r -= d; }
+    return q;
+}
+
 // In-register R-point DFT (R = 2..16), natural order in and out: y_k = sum_r v_r exp(dir 2 pi i r k / R).
 template <typename T, int R>
 __device__ __forceinline__ void dft_reg(cpx<T> *v, int dir) {
@@ -117,23 +137,28 @@ __device__ __forceinline__ void dft_reg(cpx<T> *v, int dir) {
     for (int r = 0; r < R; ++r) v[r] = u[r];
 }
 
-// exp(dir 2 pi i k / L) from the two-level table in LDS: lo[j] = exp(-2 pi i j / L) (j < 64) and
-// hi[i] = exp(-2 pi i 64 i / L) (i < L / 64), so exp(-2 pi i k / L) = hi[k >> 6] lo[k & 63].
+// exp(dir 2 pi i k / L) from the twiddle table in LDS: for L <= 1024 the table itself, tl[k] = exp(-2 pi i k / L);
+// beyond, two levels lo[j] = exp(-2 pi i j / L) (j < 64), hi[i] = exp(-2 pi i 64 i / L) (i < L / 64), so
+// exp(-2 pi i k / L) = hi[k >> 6] lo[k & 63].
 template <typename T>
 __device__ __forceinline__ cpx<T> twiddle(const cpx<T> *tl, int L, int k, int dir) {
     k &= L - 1;
-    cpx<T> w = tl[k & 63];
-    if (L > 64) w = cmul(w, tl[64 + (k >> 6)]);
+    cpx<T> w;
+    if (L <= 1024) w = tl[k];
+    else w = cmul(tl[k & 63], tl[64 + (k >> 6)]);
     return dir < 0 ? w : cpx<T>{w.x, -w.y};
 }
 
-// Fills the two-level table of length-L twiddles from the full-circle table tw[k] = exp(-2 pi i k / Lmax).
+// Fills the table of length-L twiddles from the full-circle table tw[k] = exp(-2 pi i k / Lmax).
 template <typename T>
 __device__ __forceinline__ void load_twiddles(cpx<T> *tl, int L, const cpx<T> *tw, int Lmax) {
     const int st = Lmax / L;
-    const int nhi = L >= 64 ? L / 64 : 1;
-    for (int j = threadIdx.x; j < 64 + nhi; j += BLML_NT) {
-        if (j < 64) tl[j] = j < L ? tw[(long)j * st] : cpx<T>{(T)1, (T)0};
+    if (L <= 1024) {
+        for (int j = threadIdx.x; j < L; j += BLML_NT) tl[j] = tw[(long)j * st];
+        return;
+    }
+    for (int j = threadIdx.x; j < 64 + L / 64; j += BLML_NT) {
+        if (j < 64) tl[j] = tw[(long)j * st];
         else tl[j] = tw[(long)(j - 64) * 64 * st];
     }
 }
@@ -142,8 +167,10 @@ __device__ __forceinline__ void load_twiddles(cpx<T> *tl, int L, const cpx<T> *t
 // butterflies j = j0 + b * nthr, b < NB = 16 / R, belong to this thread (nthr = L / 16 threads per line).
 // mul: multiply the inputs by spec (real) as they are read (the first inverse stage of a convolution).
 template <typename T, int R, int NB>
-__device__ __noinline__ void stockham_stage(cpx<T> *x, int L, int Ns, int j0, int nthr, const cpx<T> *tl,
-                                            int dir, const T *mul) {
+__device__ __noinline__ void stockham_stage(int xoff, int L, int Ns, int j0, int nthr, int toff, int dir,
+                                            const T *mul) {
+    cpx<T> *x = dyn_lds<T>() + xoff;             // the line (LDS)
+    const cpx<T> *tl = dyn_lds<T>() + toff;      // its twiddle table (LDS)
     cpx<T> v[R * NB];
     const int stride = L / R;
 #pragma unroll
@@ -152,7 +179,7 @@ __device__ __noinline__ void stockham_stage(cpx<T> *x, int L, int Ns, int j0, in
         const int k = j & (Ns - 1);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            cpx<T> a = x[j + r * stride];
+            cpx<T> a = x[pidx(j + r * stride)];
             if (mul) {
                 const T s = mul[j + r * stride];
                 a = {a.x * s, a.y * s};
@@ -169,7 +196,7 @@ __device__ __noinline__ void stockham_stage(cpx<T> *x, int L, int Ns, int j0, in
         const int k = j & (Ns - 1);
         const int base = (j - k) * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[base + r * Ns] = v[b * R + r];
+        for (int r = 0; r < R; ++r) x[pidx(base + r * Ns)] = v[b * R + r];
     }
     __syncthreads();
 }
@@ -177,12 +204,12 @@ __device__ __noinline__ void stockham_stage(cpx<T> *x, int L, int Ns, int j0, in
 // Whole FFT of a line of length L (<= 16 BLML_NT) in LDS, all threads participating (lines of a round share the
 // schedule).  dir = -1 forward, +1 inverse (unscaled).  mul (optional): spectrum applied on the first stage's reads.
 template <typename T>
-__device__ void fft_line(cpx<T> *x, int L, int j0, int team, const cpx<T> *tl, int dir, const T *mul) {
+__device__ void fft_line(int xoff, int L, int j0, int team, int toff, int dir, const T *mul) {
     if (L < 16) {                       // one thread per line, one stage of radix L
         switch (L) {
-            case 2: stockham_stage<T, 2, 1>(x, L, 1, 0, 1, tl, dir, mul); break;
-            case 4: stockham_stage<T, 4, 1>(x, L, 1, 0, 1, tl, dir, mul); break;
-            case 8: stockham_stage<T, 8, 1>(x, L, 1, 0, 1, tl, dir, mul); break;
+            case 2: stockham_stage<T, 2, 1>(xoff, L, 1, 0, 1, toff, dir, mul); break;
+            case 4: stockham_stage<T, 4, 1>(xoff, L, 1, 0, 1, toff, dir, mul); break;
+            case 8: stockham_stage<T, 8, 1>(xoff, L, 1, 0, 1, toff, dir, mul); break;
             default: __syncthreads(); __syncthreads(); break;   // L = 1: identity (same barriers as a stage)
         }
         return;
@@ -192,10 +219,10 @@ __device__ void fft_line(cpx<T> *x, int L, int j0, int team, const cpx<T> *tl, i
     while (Ns < L) {
         const int rem = L / Ns;
         const T *m_ = first ? mul : nullptr;
-        if (rem >= 16) stockham_stage<T, 16, 1>(x, L, Ns, j0, team, tl, dir, m_), Ns *= 16;
-        else if (rem == 8) stockham_stage<T, 8, 2>(x, L, Ns, j0, team, tl, dir, m_), Ns *= 8;
-        else if (rem == 4) stockham_stage<T, 4, 4>(x, L, Ns, j0, team, tl, dir, m_), Ns *= 4;
-        else stockham_stage<T, 2, 8>(x, L, Ns, j0, team, tl, dir, m_), Ns *= 2;
+        if (rem >= 16) stockham_stage<T, 16, 1>(xoff, L, Ns, j0, team, toff, dir, m_), Ns *= 16;
+        else if (rem == 8) stockham_stage<T, 8, 2>(xoff, L, Ns, j0, team, toff, dir, m_), Ns *= 8;
+        else if (rem == 4) stockham_stage<T, 4, 4>(xoff, L, Ns, j0, team, toff, dir, m_), Ns *= 4;
+        else stockham_stage<T, 2, 8>(xoff, L, Ns, j0, team, toff, dir, m_), Ns *= 2;
         first = false;
     }
 }
@@ -208,41 +235,87 @@ struct Pass {
     int outer, inner;     // lines = outer * inner; element (o, i, r) at (o * n * inner + i * inner + r)
     const int *F;         // circular: frequencies (0 <= F < m)
     const T *spec;        // CONV ops: spectrum values (real)
+    bool pairs;           // CONV ops on real data: two lines per transform, as real and imaginary part (exact: the
+                          // operator is real, so S (x + i y) = S x + i S y)
 };
 
-// Moves a round of lines between global memory and LDS (line l of the round at lbuf[l * stride_l + i]).
+// Moves a round of lines between global memory and LDS (line l of the round at lbuf[l * stride_l + off + i]).
 // load: positions [0, width) of each LDS line, from global positions < nvalid (zero beyond; zero lines past the
-// end); store: positions [0, width) to global.  Contiguous lines (inner == 1) are walked element-fastest, strided
-// lines line-fastest, so consecutive threads touch consecutive addresses either way.
+// end); store: positions [0, width) to global.  lofs[l]: the global offset of the round's line l (or -1 past the
+// end).  Contiguous lines (inner == 1) are walked element-fastest, strided lines line-fastest, so consecutive
+// threads touch consecutive addresses either way.
 template <typename T>
-__device__ __forceinline__ void round_io(bool load, const Pass<T> &p, long base, int nl, int n, int nvalid,
-                                         int width, int off, cpx<T> *lbuf, int stride_l, const cpx<T> *src,
-                                         cpx<T> *dst) {
-    const long nlines = (long)p.outer * p.inner;
+__device__ __forceinline__ void round_io(bool load, int inner, int nl, int nvalid, int width, int off, cpx<T> *lbuf,
+                                         int stride_l, const int *lofs, const int *lofs2, const cpx<T> *src,
+                                         cpx<T> *dst, bool padded = false) {
+    constexpr int UB = 8;                 // elements per thread in flight: the loads of a batch issue together
     const int total = nl * width;
-    for (int idx = threadIdx.x; idx < total; idx += BLML_NT) {
-        int l, i;
-        if (p.inner == 1) { l = idx / width; i = idx - l * width; }
-        else { l = idx % nl; i = idx / nl; }
-        const long li = base + l;
-        cpx<T> *cell = lbuf + l * stride_l + off + i;
-        if (li >= nlines) {
-            if (load) *cell = {(T)0, (T)0};
-            continue;
+    const int dv = inner == 1 ? width : nl;
+    const float inv = 1.0f / (float)dv;
+    for (int b0 = threadIdx.x; b0 < total; b0 += UB * BLML_NT) {
+        cpx<T> v[UB];
+        int cell[UB];
+        long g[UB], g2[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int idx = b0 + u * BLML_NT;
+            cell[u] = -1;
+            g[u] = -1;
+            g2[u] = -1;
+            v[u] = {(T)0, (T)0};
+            if (idx < total) {
+                int l, i, rr;
+                if (inner == 1) { l = fdivmod(idx, dv, inv, rr); i = rr; }
+                else { i = fdivmod(idx, dv, inv, rr); l = rr; }
+                const int lo = lofs[l];
+                cell[u] = l * stride_l + off + (padded ? pidx(i) : i);
+                if (lo >= 0 && (!load || i < nvalid)) {
+                    g[u] = lo + (long)i * inner;
+                    if (lofs2 && lofs2[l] >= 0) g2[u] = (long)lofs2[l] + (long)i * inner;
+                }
+            }
         }
-        const long o = li / p.inner, r = li - o * p.inner;
-        const size_t g = (size_t)o * n * p.inner + (size_t)i * p.inner + r;
-        if (load) *cell = i < nvalid ? src[g] : cpx<T>{(T)0, (T)0};
-        else dst[g] = *cell;
+        if (load) {
+            if (lofs2) {
+#pragma unroll
+                for (int u = 0; u < UB; ++u) {
+                    if (g[u] >= 0) v[u].x = src[g[u]].x;
+                    if (g2[u] >= 0) v[u].y = src[g2[u]].x;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < UB; ++u)
+                    if (g[u] >= 0) v[u] = src[g[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < UB; ++u)
+                if (cell[u] >= 0) lbuf[cell[u]] = v[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < UB; ++u)
+                if (g[u] >= 0) v[u] = lbuf[cell[u]];
+            if (lofs2) {
+#pragma unroll
+                for (int u = 0; u < UB; ++u) {
+                    if (g[u] >= 0) dst[g[u]] = {v[u].x, (T)0};
+                    if (g2[u] >= 0) dst[g2[u]] = {v[u].y, (T)0};
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < UB; ++u)
+                    if (g[u] >= 0) dst[g[u]] = v[u];
+            }
+        }
     }
 }
 
 // Applies one pass to every line.  src and dst may alias when n_in == n_out (a round's lines are staged in LDS
 // before any is written back, and lines are disjoint).  All threads execute the same sequence of barriers.
 template <typename T>
-__device__ __noinline__ void run_pass(const Pass<T> &p, const cpx<T> *src, cpx<T> *dst, cpx<T> *lds, const cpx<T> *tw,
-                         int Lmax) {
-    const long nlines = (long)p.outer * p.inner;
+__device__ __noinline__ void run_pass(const Pass<T> &p, const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw, int Lmax) {
+    cpx<T> *lds = dyn_lds<T>();
+    const long nlines0 = (long)p.outer * p.inner;
+    const long nlines = p.pairs ? (nlines0 + 1) / 2 : nlines0;    // transforms (line pairs when pairing)
     const PassShape sh = pass_shape(p.op, p.m, p.L, p.nF, nlines);
     const int team = sh.team, nl = sh.lines;
     const int tm = threadIdx.x / team, tid = threadIdx.x - tm * team;
@@ -259,28 +332,51 @@ __device__ __noinline__ void run_pass(const Pass<T> &p, const cpx<T> *src, cpx<T
             roots[j] = {(T)cs, (T)sn};
         }
     }
+    // line offsets into the scratch (int: the plan keeps every intermediate below 2^31 values)
+    __shared__ int lofs_in[BLML_NT], lofs_out[BLML_NT], lofs_in2[BLML_NT], lofs_out2[BLML_NT];
+    const int *li2 = p.pairs ? lofs_in2 : nullptr;
+    const int *lo2 = p.pairs ? lofs_out2 : nullptr;
+    const float inv_inner = 1.0f / (float)p.inner;
     for (long base = 0; base < nlines; base += nl) {
+        for (int l = threadIdx.x; l < nl; l += BLML_NT) {
+            const long t = base + l;
+            const long la = p.pairs ? 2 * t : t, lb = p.pairs ? 2 * t + 1 : -1;
+            lofs_in[l] = lofs_out[l] = lofs_in2[l] = lofs_out2[l] = -1;
+            if (t < nlines) {
+                int r;
+                int o = fdivmod((int)la, p.inner, inv_inner, r);
+                lofs_in[l] = (int)((long)o * p.n_in * p.inner + r);
+                lofs_out[l] = (int)((long)o * p.n_out * p.inner + r);
+                if (lb >= 0 && lb < nlines0) {
+                    o = fdivmod((int)lb, p.inner, inv_inner, r);
+                    lofs_in2[l] = (int)((long)o * p.n_in * p.inner + r);
+                    lofs_out2[l] = (int)((long)o * p.n_out * p.inner + r);
+                }
+            }
+        }
+        __syncthreads();
         if (is_fft_op(p.op)) {
             const int nin = (p.op == OP_FFT_FULL || p.op == OP_FFT_INV) ? p.L : p.m;
-            round_io<T>(true, p, base, nl, p.n_in, nin, p.L, 0, lines, sh.buf, src, nullptr);
+            round_io<T>(true, p.inner, nl, nin, p.L, 0, lines, sh.buf, lofs_in, li2, src, nullptr, true);
             __syncthreads();
             const int j0 = has_line ? tid : 0;
+            const int boff = (int)(buf - lds), roff = 0;
             if (p.op == OP_FFT_INV) {
-                fft_line<T>(buf, p.L, j0, team, roots, +1, nullptr);
+                fft_line<T>(boff, p.L, j0, team, roff, +1, nullptr);
             } else {
-                fft_line<T>(buf, p.L, j0, team, roots, -1, nullptr);
-                if (p.op == OP_CONV_PAD) fft_line<T>(buf, p.L, j0, team, roots, +1, p.spec);
+                fft_line<T>(boff, p.L, j0, team, roff, -1, nullptr);
+                if (p.op == OP_CONV_PAD) fft_line<T>(boff, p.L, j0, team, roff, +1, p.spec);
             }
             const int nout = (p.op == OP_FFT_FWD || p.op == OP_FFT_FULL) ? p.L : p.m;
-            round_io<T>(false, p, base, nl, p.n_out, nout, nout, 0, lines, sh.buf, nullptr, dst);
+            round_io<T>(false, p.inner, nl, nout, nout, 0, lines, sh.buf, lofs_out, lo2, nullptr, dst, true);
             __syncthreads();
         } else {
             // band-restricted DFT on line buffers [0, m) (values) and [m, m + nF) (band)
             cpx<T> *xl = buf, *Xf = buf + p.m;
             if (p.op == OP_DFT_INV)
-                round_io<T>(true, p, base, nl, p.n_in, p.nF, p.nF, p.m, lines, sh.buf, src, nullptr);
+                round_io<T>(true, p.inner, nl, p.nF, p.nF, p.m, lines, sh.buf, lofs_in, nullptr, src, nullptr);
             else
-                round_io<T>(true, p, base, nl, p.n_in, p.m, p.m, 0, lines, sh.buf, src, nullptr);
+                round_io<T>(true, p.inner, nl, p.m, p.m, 0, lines, sh.buf, lofs_in, li2, src, nullptr);
             __syncthreads();
             if (p.op != OP_DFT_INV) {
                 for (int q = tid; q < p.nF; q += team) {
@@ -301,19 +397,21 @@ __device__ __noinline__ void run_pass(const Pass<T> &p, const cpx<T> *src, cpx<T
                 __syncthreads();
             }
             if (p.op == OP_DFT_FWD) {
-                round_io<T>(false, p, base, nl, p.n_out, p.nF, p.nF, p.m, lines, sh.buf, nullptr, dst);
+                round_io<T>(false, p.inner, nl, p.nF, p.nF, p.m, lines, sh.buf, lofs_out, nullptr, nullptr, dst);
             } else {
                 // inverse from the band: out[i] = sum_q Xf[q] exp(+2 pi i F_q i / m), into [0, m)
+                const float inv_m = 1.0f / (float)p.m;
                 for (int i = tid; i < p.m; i += team) {
                     cpx<T> acc = {(T)0, (T)0};
                     for (int q = 0; q < p.nF; ++q) {
-                        const int idx = (int)(((unsigned)p.F[q] * (unsigned)i) % (unsigned)p.m);
+                        int idx;
+                        fdivmod(p.F[q] * i, p.m, inv_m, idx);
                         acc = cadd(acc, cmulc(Xf[q], roots[idx]));
                     }
                     if (has_line) xl[i] = acc;
                 }
                 __syncthreads();
-                round_io<T>(false, p, base, nl, p.n_out, p.m, p.m, 0, lines, sh.buf, nullptr, dst);
+                round_io<T>(false, p.inner, nl, p.m, p.m, 0, lines, sh.buf, lofs_out, lo2, nullptr, dst);
             }
             __syncthreads();
         }
@@ -331,23 +429,29 @@ struct MatvecCtx {
     const cpx<T> *tw;          // full-circle twiddles of the longest FFT
     int Lmax;
     cpx<T> *X, *Y;             // scratch (scratch_len complex each)
-    cpx<T> *lds;
 };
 
 template <typename T>
 __device__ void mul_group_spec(const MatvecCtx<T> &c, const blml_factor_t &fa, cpx<T> *buf, const int *ext,
                                const T *spec) {
-    long total = 1;
+    int total = 1;
     for (int j = 0; j < c.d; ++j) total *= ext[j];
-    for (long idx = threadIdx.x; idx < total; idx += BLML_NT) {
-        long rem = idx;
-        long g = 0;
-        int coord[BLML_MAXD];
-        for (int j = c.d - 1; j >= 0; --j) {
-            coord[j] = (int)(rem % ext[j]);
-            rem /= ext[j];
+    const bool all = fa.naxes == c.d;      // the group spans every axis: the spectrum index is the element index
+    float inv[BLML_MAXD];
+    for (int j = 0; j < c.d; ++j) inv[j] = 1.0f / (float)ext[j];
+    for (int idx = threadIdx.x; idx < total; idx += BLML_NT) {
+        int g = idx;
+        if (!all) {
+            int rem = idx;
+            int coord[BLML_MAXD];
+            for (int j = c.d - 1; j >= 0; --j) {
+                int r;
+                rem = fdivmod(rem, ext[j], inv[j], r);
+                coord[j] = r;
+            }
+            g = 0;
+            for (int a = 0; a < fa.naxes; ++a) g = g * fa.tlen[a] + coord[fa.axes[a]];
         }
-        for (int a = 0; a < fa.naxes; ++a) g = g * fa.tlen[a] + coord[fa.axes[a]];
         const T s = spec[g];
         buf[idx] = {buf[idx].x * s, buf[idx].y * s};
     }
@@ -364,10 +468,11 @@ __device__ inline void line_geom(const int *ext, int d, int axis, int &outer, in
 
 // Applies one factor to the complex lattice array in c.X (in place; group factors use c.Y as well).
 template <typename T>
-__device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa) {
+__device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa, bool real) {
     if (fa.kind == BLML_FAC_LINE) {
         const int a = fa.axes[0];
         Pass<T> p;
+        p.pairs = real;
         p.m = c.shape[a];
         p.n_in = p.n_out = p.m;
         line_geom<T>(c.shape, c.d, a, p.outer, p.inner);
@@ -383,7 +488,7 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa) {
             p.nF = 0;
             p.F = nullptr;
         }
-        run_pass<T>(p, c.X, c.X, c.lds, c.tw, c.Lmax);
+        run_pass<T>(p, c.X, c.X, c.tw, c.Lmax);
         return;
     }
     int ext[BLML_MAXD];
@@ -403,7 +508,8 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa) {
             p.n_out = p.L;
         }
         p.spec = nullptr;
-        run_pass<T>(p, cur, oth, c.lds, c.tw, c.Lmax);
+        p.pairs = false;
+        run_pass<T>(p, cur, oth, c.tw, c.Lmax);
         ext[a] = p.n_out;
         cpx<T> *t = cur; cur = oth; oth = t;
     }
@@ -421,7 +527,8 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa) {
             p.op = OP_FFT_INV; p.L = fa.tlen[ai]; p.nF = 0; p.F = nullptr;
         }
         p.spec = nullptr;
-        run_pass<T>(p, cur, oth, c.lds, c.tw, c.Lmax);
+        p.pairs = false;
+        run_pass<T>(p, cur, oth, c.tw, c.Lmax);
         ext[a] = p.m;
         cpx<T> *t = cur; cur = oth; oth = t;
     }
@@ -434,24 +541,54 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa) {
 template <typename T, typename Ti, typename To>
 __device__ void matvec_mode(const MatvecCtx<T> &c, const int fac_begin, const int fac_count, const Ti *E, To *U,
                             int B, int nc, const Ti *Ebase = nullptr, const To *Ubase = nullptr) {
-    for (int b = threadIdx.x; b < B; b += BLML_NT) {
-        double re = (double)E[nc * b], im = nc == 2 ? (double)E[2 * b + 1] : 0.0;
-        if (Ebase) {
-            re -= (double)Ebase[nc * b];
-            if (nc == 2) im -= (double)Ebase[2 * b + 1];
+    constexpr int UB = 4;                 // bins per thread in flight
+    for (int b0 = threadIdx.x; b0 < B; b0 += UB * BLML_NT) {
+        double re[UB], im[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int b = b0 + u * BLML_NT;
+            re[u] = 0.0; im[u] = 0.0;
+            if (b < B) {
+                re[u] = (double)E[nc * b];
+                if (nc == 2) im[u] = (double)E[2 * b + 1];
+                if (Ebase) {
+                    re[u] -= (double)Ebase[nc * b];
+                    if (nc == 2) im[u] -= (double)Ebase[2 * b + 1];
+                }
+            }
         }
-        c.X[b] = {(T)re, (T)im};
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int b = b0 + u * BLML_NT;
+            if (b < B) c.X[b] = {(T)re[u], (T)im[u]};
+        }
     }
     __syncthreads();
-    for (int f = 0; f < fac_count; ++f) apply_factor<T>(c, c.facs[fac_begin + f]);
-    for (int b = threadIdx.x; b < B; b += BLML_NT) {
-        double re = (double)c.X[b].x, im = (double)c.X[b].y;
-        if (Ubase) {
-            re += (double)Ubase[nc * b];
-            if (nc == 2) im += (double)Ubase[2 * b + 1];
+    for (int f = 0; f < fac_count; ++f) apply_factor<T>(c, c.facs[fac_begin + f], nc == 1);
+    for (int b0 = threadIdx.x; b0 < B; b0 += UB * BLML_NT) {
+        double re[UB], im[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int b = b0 + u * BLML_NT;
+            re[u] = 0.0; im[u] = 0.0;
+            if (b < B) {
+                const cpx<T> x = c.X[b];
+                re[u] = (double)x.x;
+                im[u] = (double)x.y;
+                if (Ubase) {
+                    re[u] += (double)Ubase[nc * b];
+                    if (nc == 2) im[u] += (double)Ubase[2 * b + 1];
+                }
+            }
         }
-        U[nc * b] = (To)re;
-        if (nc == 2) U[2 * b + 1] = (To)im;
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int b = b0 + u * BLML_NT;
+            if (b < B) {
+                U[nc * b] = (To)re[u];
+                if (nc == 2) U[2 * b + 1] = (To)im[u];
+            }
+        }
     }
     __syncthreads();
 }

@@ -42,6 +42,11 @@ MATVEC_CASES = [
                                                                 periods=[128.0, 97.0])], 1.0, True),
     ('2d pball 127x17 complex', (127, 17), [dict(fc=[0.45, 0.45], kinds=['pball', 'pball'],
                                                  periods=[127.0, 17.0])], 1.0, True),
+    ('2d slits + dirichlet bg 64x45 real', (64, 45), [dict(fc=[0.35, 0.3], kinds=['slits:0.5', 'slits:0.4']),
+                                                      dict(fc=[3 / 64, 3 / 45], kinds=['dirichlet', 'dirichlet'],
+                                                           periods=[64.0, 45.0])], 1.0, False),
+    ('2d sinc x dirichlet real', (13, 16), [dict(fc=[0.4, 0.3], kinds=['sinc', 'dirichlet'],
+                                                 periods=[None, 16.0])], 1.0, False),
     ('3d sinc', (6, 7, 5), [dict(fc=[0.5, 0.4, 0.3], kinds=['sinc'] * 3)], 1.0, False),
     ('5d sinc 8^5', (8,) * 5, [dict(fc=[0.3] * 5, kinds=['sinc'] * 5)], 1.0, False),
 ]
