@@ -34,7 +34,7 @@ class Params(ctypes.Structure):
     _fields_ = [('tol', ctypes.c_double), ('max_sweeps', ctypes.c_int64), ('anderson', ctypes.c_int32),
                 ('stall', ctypes.c_int32), ('phase_noise', ctypes.c_double), ('ll_tol', ctypes.c_double),
                 ('ll_window', ctypes.c_int32), ('solver', ctypes.c_int32), ('lbfgs_m', ctypes.c_int32),
-                ('hist_cap', ctypes.c_int32)]
+                ('hist_cap', ctypes.c_int32), ('spread', ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):

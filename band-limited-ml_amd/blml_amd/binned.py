@@ -121,8 +121,11 @@ class FitBatch:
     """A plan for a batch of fits and its device buffers.  build() uploads everything; run() is one persistent
     launch (asynchronous on the current stream); results() copies the outputs back."""
 
-    def __init__(self, fitspecs, params, dtype, hist_cap=4096):
+    def __init__(self, fitspecs, params, dtype, hist_cap=4096, spread=0):
+        """spread (a batch of one fit): 0 spreads the fit over every CU when it has >= 2^18 values, 1 always,
+        -1 never."""
         self.fits = list(fitspecs)
+        self.spread = int(spread)
         self.params = params
         self.dtype = np.dtype(dtype)
         if self.dtype not in (np.float32, np.float64):
@@ -206,6 +209,7 @@ class FitBatch:
         prm.solver = 1 if P['solver'] == 'lbfgs' else 0
         prm.lbfgs_m = int(P['lbfgs_m'])
         prm.hist_cap = self.hist_cap
+        prm.spread = self.spread
         self._prm = prm
         plan = ctypes.c_void_p()
         _lib.check(lib.blml_plan_create(self._Fits, len(self.fits), self._Facs, self._nfacs,
@@ -273,6 +277,8 @@ class FitBatch:
         se = self.d_se.cpu().numpy()
         dens = self.d_dens.cpu().numpy()
         res = np.frombuffer(self.d_res.cpu().numpy().tobytes(), dtype=_lib.RESULT_DTYPE)
+        if np.any(res['reserved'] != 0):
+            raise RuntimeError("a grid barrier of the spread fit timed out: its results are void")
         mh = self.d_mh.cpu().numpy()
         rh = self.d_rh.cpu().numpy()
         out = []

@@ -83,3 +83,105 @@ __device__ __forceinline__ double block_sum1(double v, double *red) {
     block_sum<1>(a, red);
     return a[0];
 }
+
+__device__ __forceinline__ double block_max1(double v, double *red) { return block_max(v, red); }
+
+// ------------------------------------------------------------------------------------------------ teams
+// A team is the set of workgroups that runs one fit: one workgroup (rank 0 of 1, the batched path), or every
+// workgroup of a cooperative launch (a fit too large for one CU).  Element loops stride over the team; sync()
+// orders every team member's global writes before any member's later reads; sum / max reduce over the team in
+// FP64 and give every thread of every member the same totals (partials combined in rank order), so the solver's
+// scalar control flow stays identical across the team.
+
+#define BLML_BAR_SPIN (1L << 26)    // polls (each with a short sleep) before a grid barrier declares a timeout
+
+// bar[0]: arrivals, bar[1]: generation, bar[2]: set when a wait timed out (the results carry it to the host).
+// Every thread fences at agent scope on both sides, so plain global stores made before the barrier by any
+// workgroup are visible to loads made after it by any other (the L2s of the XCDs are not coherent by
+// themselves).  The wait is bounded: the kernel always ends, and a timeout is reported instead of hanging.
+__device__ inline void grid_barrier(unsigned *bar, int nranks) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == (unsigned)nranks - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long spins = 0;
+            // after any timeout every later wait is skipped, so a failed team still finishes promptly
+            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen &&
+                   __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > BLML_BAR_SPIN) {
+                    __hip_atomic_fetch_or(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    __threadfence();
+}
+
+struct Team {
+    int rank, nranks;
+    unsigned *bar;       // grid teams: the barrier words
+    double *part;        // grid teams: 2 * nranks * BLML_NRED partial sums (two buffers, alternating)
+    int parity;
+
+    __device__ long first() const { return (long)rank * BLML_NT + threadIdx.x; }
+    __device__ long stride() const { return (long)nranks * BLML_NT; }
+    __device__ bool lead() const { return rank == 0 && threadIdx.x == 0; }
+    __device__ void sync() const {
+        if (nranks > 1) grid_barrier(bar, nranks);
+        else __syncthreads();
+    }
+    // combines one value per member (already reduced over the workgroup) into red[i] for i < N, in rank order;
+    // op 0: sum, 1: max
+    template <int N>
+    __device__ void combine(const double (&v)[N], double *red, int op) {
+        double *buf = part + (long)parity * nranks * BLML_NRED;
+        parity ^= 1;
+        if (threadIdx.x == 0)
+            for (int i = 0; i < N; ++i) buf[(long)rank * BLML_NRED + i] = v[i];
+        grid_barrier(bar, nranks);
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        for (int i = wid; i < N; i += BLML_NWAVE) {
+            double a = op ? -INFINITY : 0.0;
+            for (int r = lane; r < nranks; r += 64) {
+                const double x = __hip_atomic_load(&buf[(long)r * BLML_NRED + i], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                a = op ? fmax(a, x) : a + x;
+            }
+            a = op ? wave_max(a) : wave_sum(a);
+            if (lane == 0) red[BLML_NWAVE * BLML_NRED - BLML_NRED + i] = a;
+        }
+        __syncthreads();
+    }
+    template <int N>
+    __device__ void sum(double (&v)[N], double *red) {
+        block_sum<N>(v, red);
+        if (nranks == 1) return;
+        combine<N>(v, red, 0);
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = red[BLML_NWAVE * BLML_NRED - BLML_NRED + i];
+        __syncthreads();
+    }
+    __device__ double sum1(double v, double *red) {
+        double a[1] = {v};
+        sum<1>(a, red);
+        return a[0];
+    }
+    __device__ double max(double v, double *red) {
+        double a[1] = {block_max(v, red)};
+        if (nranks == 1) return a[0];
+        combine<1>(a, red, 1);
+        a[0] = red[BLML_NWAVE * BLML_NRED - BLML_NRED];
+        __syncthreads();
+        return a[0];
+    }
+};
+
+__device__ inline Team block_team() { return Team{0, 1, nullptr, nullptr, 0}; }

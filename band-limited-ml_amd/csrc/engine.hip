@@ -127,6 +127,8 @@ __global__ void __launch_bounds__(BLML_NT) spec_kernel(DevPlan<T> p, int nfacs, 
         ps.spec = nullptr;
         ps.n_in = ps.n_out = ps.L;
         ps.pairs = false;
+        ps.rank = 0;
+        ps.nranks = 1;
         line_geom<T>(ext, fa.naxes, a, ps.outer, ps.inner);
         run_pass<T>(ps, cur, oth, p.tw, p.Lmax);
         cpx<T> *t = cur; cur = oth; oth = t;
@@ -137,7 +139,8 @@ __global__ void __launch_bounds__(BLML_NT) spec_kernel(DevPlan<T> p, int nfacs, 
 }
 
 template <typename T>
-__device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, MatvecCtx<T> &mv) {
+__device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, MatvecCtx<T> &mv, const Team &tm) {
+    mv.tm = tm;
     mv.d = f.d;
     for (int j = 0; j < BLML_MAXD; ++j) mv.shape[j] = j < f.d ? f.shape[j] : 1;
     mv.facs = p.facs;
@@ -150,12 +153,10 @@ __device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, Matvec
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BLML_NT) matvec_kernel(DevPlan<T> p, const T *E, T *U) {
-    const int fi = blockIdx.x;
-    if (fi >= p.nfits) return;
+__device__ void run_matvec(const DevPlan<T> &p, int fi, const Team &tm, const T *E, T *U) {
     const blml_fit_t f = p.fits[fi];
     MatvecCtx<T> mv;
-    make_matvec_ctx<T>(p, f, mv);
+    make_matvec_ctx<T>(p, f, mv, tm);
     const int nc = f.complex_g ? 2 : 1;
     for (int k = 0; k < f.K; ++k) {
         const long off = f.out_off + (long)k * f.B * nc;
@@ -163,7 +164,192 @@ __global__ void __launch_bounds__(BLML_NT) matvec_kernel(DevPlan<T> p, const T *
     }
 }
 
+// one workgroup per fit
+template <typename T>
+__global__ void __launch_bounds__(BLML_NT) matvec_kernel(DevPlan<T> p, const T *E, T *U) {
+    if ((int)blockIdx.x >= p.nfits) return;
+    run_matvec<T>(p, blockIdx.x, block_team(), E, U);
+}
 
+// a plan of one fit, spread over every workgroup of a cooperative launch
+template <typename T>
+__global__ void __launch_bounds__(BLML_NT) matvec_grid_kernel(DevPlan<T> p, unsigned *bar, double *part, const T *E,
+                                                             T *U) {
+    run_matvec<T>(p, 0, Team{(int)blockIdx.x, (int)gridDim.x, bar, part, 0}, E, U);
+}
+
+// One whole fit (start, solve, outputs), run by a team.
+template <typename T>
+__device__ void run_fit(const DevPlan<T> &p, int fi, const Team &team, const double *counts, const double *draws,
+                        const double *init, const double *signs, T *e_out, T *se_out, double *density,
+                        blml_result_t *results, double *merit_hist, double *rho_hist, double tick_hz) {
+    const blml_fit_t f = p.fits[fi];
+    FitSolver<T> S;
+    S.tm = team;
+    make_matvec_ctx<T>(p, f, S.mv, team);
+    S.f = &p.fits[fi];
+    S.P = &p.params;
+    S.B = f.B;
+    S.n = f.n;
+    for (int k = 0; k < BLML_MAXK; ++k) S.s[k] = f.s[k];
+    S.equal_s = f.equal_s != 0;
+    S.w = counts + f.counts_off;
+    S.sg = f.has_signs ? signs + f.signs_off : nullptr;
+    const int ncf = f.complex_g ? 2 : 1;
+    S.Vmax = (long)f.K * f.B * ncf;
+    S.pool = (double *)(p.ws + f.ws_off + pool_bytes_off(f, (int)sizeof(T)));
+    S.inc = sizeof(T) == 4;
+    S.gstep = 0.0;
+    S.t_mv = S.t_dir = S.t_ls = 0;
+    S.n_mv = 0;
+    const long long tfit0 = wall_clock64();
+    long long tstart = 0;
+    S.merit_hist = merit_hist ? merit_hist + f.hist_off : nullptr;
+    S.rho_hist = rho_hist ? rho_hist + f.hist_off : nullptr;
+    S.hist_cap = p.params.hist_cap;
+    S.nhist_m = S.nhist_r = 0;
+    S.ring_n = 0;
+    const bool psd_full = f.complex_g || f.K > 1;
+    const double pn = p.params.phase_noise;
+    const double *dr = draws + f.draws_off;
+    int start_sweeps = 0;
+    double *E = S.vec(0);
+    if (f.init_K > 0) {
+        // warm start (blml_binned.py:403-408): given modes, zero off the occupied bins, noise in the others
+        const double *E0 = init + f.init_off;
+        const int kk = f.init_K < f.K ? f.init_K : f.K;
+        for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
+            const bool o = S.w[b] > 0;
+            for (int k = 0; k < f.K; ++k) {
+                const long i = ((long)k * f.B + b) * ncf;
+                double re = 0.0, im = 0.0;
+                if (k < kk && o) {
+                    re = E0[((long)k * f.B + b) * 2];
+                    im = f.complex_g ? E0[((long)k * f.B + b) * 2 + 1] : 0.0;
+                }
+                E[i] = re;
+                if (ncf == 2) E[i + 1] = im;
+            }
+        }
+        S.tm.sync();
+        long dp = 0;
+        for (int k = kk; k < f.K; ++k) {
+            for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
+                const double a0 = ncf == 2 ? sqrt((double)E[2 * b] * E[2 * b] + (double)E[2 * b + 1] * E[2 * b + 1])
+                                           : fabs((double)E[b]);
+                const long i = ((long)k * f.B + b) * ncf;
+                if (ncf == 2) {
+                    E[i] = (pn * a0 * (dr[dp + b] / sqrt(2.0)));
+                    E[i + 1] = (pn * a0 * (dr[dp + f.B + b] / sqrt(2.0)));
+                } else {
+                    E[i] = (pn * a0 * dr[dp + b]);
+                }
+            }
+            dp += (long)f.B * ncf;
+        }
+        S.tm.sync();
+    } else {
+        // cold start: e0 = sqrt(c / s0) on occupied bins (times the signs)
+        for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
+            const double sgb = S.sg ? S.sg[b] : 1.0;
+            E[b] = S.w[b] > 0 ? sqrt(f.n * S.w[b] / f.s[0]) * sgb : 0.0;
+        }
+        S.tm.sync();
+        if (psd_full) {
+            // the real rank-1 solution under mode 0 (blml_binned.py:421-425)
+            S.K = 1; S.nc = 1; S.psd = false; S.rec = false;
+            const FitOut so = S.solve_cd();
+            start_sweeps = so.n_sweeps;
+            tstart = wall_clock64() - tfit0;
+            // phases and the other modes (blml_binned.py:429-436), assembled in slot 2 then moved to slot 0
+            double *E2 = S.vec(2);
+            const double *er = S.vec(0);
+            long dp = 0;
+            for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
+                if (f.complex_g) {
+                    double sn, cs;
+                    sincos(pn * dr[b], &sn, &cs);
+                    const double r = er[b];
+                    E2[2 * b] = (r * cs);
+                    E2[2 * b + 1] = (r * sn);
+                } else {
+                    E2[b] = er[b];
+                }
+            }
+            if (f.complex_g) dp = f.B;
+            S.tm.sync();
+            for (int k = 1; k < f.K; ++k) {
+                for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
+                    const double a0 = ncf == 2
+                        ? sqrt((double)E2[2 * b] * E2[2 * b] + (double)E2[2 * b + 1] * E2[2 * b + 1])
+                        : fabs((double)E2[b]);
+                    const long i = ((long)k * f.B + b) * ncf;
+                    if (ncf == 2) {
+                        E2[i] = (pn * a0 * (dr[dp + b] / sqrt(2.0)));
+                        E2[i + 1] = (pn * a0 * (dr[dp + f.B + b] / sqrt(2.0)));
+                    } else {
+                        E2[i] = (pn * a0 * dr[dp + b]);
+                    }
+                }
+                dp += (long)f.B * ncf;
+                S.tm.sync();
+            }
+            S.K = f.K; S.nc = ncf; S.psd = true;
+            S.copy(S.vec(0), E2);
+        }
+    }
+    S.K = f.K;
+    S.nc = ncf;
+    S.psd = psd_full;
+    S.rec = true;
+    S.nhist_m = S.nhist_r = 0;
+    FitOut o;
+    if (p.params.solver == 1 && psd_full)
+        o = (S.inc && p.params.lbfgs_m <= BLML_CMAX) ? S.solve_lbfgs_compact() : S.solve_lbfgs();
+    else
+        o = S.solve_cd();
+    // outputs (blml_binned.py:438-445)
+    const double *Ef = S.vec(0), *Uf = S.vec(1);
+    const long V = (long)f.K * f.B * ncf;
+    for (long i = S.tm.first(); i < V; i += S.tm.stride()) {
+        e_out[f.out_off + i] = (T)Ef[i];
+        se_out[f.out_off + i] = (T)Uf[i];
+    }
+    double ll = 0.0;
+    const double n2 = f.n * f.n;
+    for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
+        const double dens = S.Fb(Uf, b) / n2;
+        density[f.dens_off + b] = dens;
+        if (S.w[b] > 0) ll += S.w[b] * log(dens);
+    }
+    ll = S.tm.sum1(ll, g_sl.red);
+    const unsigned timed_out = S.tm.nranks > 1 ? __hip_atomic_load(&S.tm.bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    if (S.tm.lead()) {
+        blml_result_t r;
+        r.loglik = ll / f.n;
+        r.residual = o.residual;
+        r.n_sweeps = o.n_sweeps;
+        r.stop_reason = o.stop;
+        r.converged = o.converged;
+        r.n_anderson = o.n_and;
+        r.n_damped = o.n_damp;
+        r.n_gradient = o.n_grad;
+        r.start_sweeps = start_sweeps;
+        r.n_hist = S.nhist_m;
+        r.n_matvec = S.n_mv;
+        r.reserved = (int)timed_out;   // a grid barrier timed out: the results are void
+        r.seconds[0] = (double)(wall_clock64() - tfit0) / tick_hz;
+        r.seconds[1] = (double)tstart / tick_hz;
+        r.seconds[2] = (double)S.t_mv / tick_hz;
+        r.seconds[3] = (double)S.t_dir / tick_hz;
+        r.seconds[4] = (double)S.t_ls / tick_hz;
+        results[fi] = r;
+    }
+    S.tm.sync();
+}
+
+// The batched path: a persistent grid whose workgroups take fits from an atomic queue, longest first; one
+// workgroup runs one whole fit.
 template <typename T>
 __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *order, int *queue,
                                                       const double *counts, const double *draws,
@@ -177,193 +363,44 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
         const int qi = s_fit;
         __syncthreads();
         if (qi >= p.nfits) break;
-        const int fi = order[qi];
-        const blml_fit_t f = p.fits[fi];
-        FitSolver<T> S;
-        make_matvec_ctx<T>(p, f, S.mv);
-        S.f = &p.fits[fi];
-        S.P = &p.params;
-        S.B = f.B;
-        S.n = f.n;
-        for (int k = 0; k < BLML_MAXK; ++k) S.s[k] = f.s[k];
-        S.equal_s = f.equal_s != 0;
-        S.w = counts + f.counts_off;
-        S.sg = f.has_signs ? signs + f.signs_off : nullptr;
-        const int ncf = f.complex_g ? 2 : 1;
-        S.Vmax = (long)f.K * f.B * ncf;
-        S.pool = (double *)(p.ws + f.ws_off + pool_bytes_off(f, (int)sizeof(T)));
-        S.inc = sizeof(T) == 4;
-        S.gstep = 0.0;
-        S.t_mv = S.t_dir = S.t_ls = 0;
-        S.n_mv = 0;
-        const long long tfit0 = wall_clock64();
-        long long tstart = 0;
-        S.merit_hist = merit_hist ? merit_hist + f.hist_off : nullptr;
-        S.rho_hist = rho_hist ? rho_hist + f.hist_off : nullptr;
-        S.hist_cap = p.params.hist_cap;
-        S.nhist_m = S.nhist_r = 0;
-        S.ring_n = 0;
-        const bool psd_full = f.complex_g || f.K > 1;
-        const double pn = p.params.phase_noise;
-        const double *dr = draws + f.draws_off;
-        int start_sweeps = 0;
-        double *E = S.vec(0);
-        if (f.init_K > 0) {
-            // warm start (blml_binned.py:403-408): given modes, zero off the occupied bins, noise in the others
-            const double *E0 = init + f.init_off;
-            const int kk = f.init_K < f.K ? f.init_K : f.K;
-            for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
-                const bool o = S.w[b] > 0;
-                for (int k = 0; k < f.K; ++k) {
-                    const long i = ((long)k * f.B + b) * ncf;
-                    double re = 0.0, im = 0.0;
-                    if (k < kk && o) {
-                        re = E0[((long)k * f.B + b) * 2];
-                        im = f.complex_g ? E0[((long)k * f.B + b) * 2 + 1] : 0.0;
-                    }
-                    E[i] = re;
-                    if (ncf == 2) E[i + 1] = im;
-                }
-            }
-            __syncthreads();
-            long dp = 0;
-            for (int k = kk; k < f.K; ++k) {
-                for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
-                    const double a0 = ncf == 2 ? sqrt((double)E[2 * b] * E[2 * b] + (double)E[2 * b + 1] * E[2 * b + 1])
-                                               : fabs((double)E[b]);
-                    const long i = ((long)k * f.B + b) * ncf;
-                    if (ncf == 2) {
-                        E[i] = (pn * a0 * (dr[dp + b] / sqrt(2.0)));
-                        E[i + 1] = (pn * a0 * (dr[dp + f.B + b] / sqrt(2.0)));
-                    } else {
-                        E[i] = (pn * a0 * dr[dp + b]);
-                    }
-                }
-                dp += (long)f.B * ncf;
-            }
-            __syncthreads();
-        } else {
-            // cold start: e0 = sqrt(c / s0) on occupied bins (times the signs)
-            for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
-                const double sgb = S.sg ? S.sg[b] : 1.0;
-                E[b] = S.w[b] > 0 ? sqrt(f.n * S.w[b] / f.s[0]) * sgb : 0.0;
-            }
-            __syncthreads();
-            if (psd_full) {
-                // the real rank-1 solution under mode 0 (blml_binned.py:421-425)
-                S.K = 1; S.nc = 1; S.psd = false; S.rec = false;
-                const FitOut so = S.solve_cd();
-                start_sweeps = so.n_sweeps;
-                tstart = wall_clock64() - tfit0;
-                // phases and the other modes (blml_binned.py:429-436), assembled in slot 2 then moved to slot 0
-                double *E2 = S.vec(2);
-                const double *er = S.vec(0);
-                long dp = 0;
-                for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
-                    if (f.complex_g) {
-                        double sn, cs;
-                        sincos(pn * dr[b], &sn, &cs);
-                        const double r = er[b];
-                        E2[2 * b] = (r * cs);
-                        E2[2 * b + 1] = (r * sn);
-                    } else {
-                        E2[b] = er[b];
-                    }
-                }
-                if (f.complex_g) dp = f.B;
-                __syncthreads();
-                for (int k = 1; k < f.K; ++k) {
-                    for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
-                        const double a0 = ncf == 2
-                            ? sqrt((double)E2[2 * b] * E2[2 * b] + (double)E2[2 * b + 1] * E2[2 * b + 1])
-                            : fabs((double)E2[b]);
-                        const long i = ((long)k * f.B + b) * ncf;
-                        if (ncf == 2) {
-                            E2[i] = (pn * a0 * (dr[dp + b] / sqrt(2.0)));
-                            E2[i + 1] = (pn * a0 * (dr[dp + f.B + b] / sqrt(2.0)));
-                        } else {
-                            E2[i] = (pn * a0 * dr[dp + b]);
-                        }
-                    }
-                    dp += (long)f.B * ncf;
-                    __syncthreads();
-                }
-                S.K = f.K; S.nc = ncf; S.psd = true;
-                S.copy(S.vec(0), E2);
-            }
-        }
-        S.K = f.K;
-        S.nc = ncf;
-        S.psd = psd_full;
-        S.rec = true;
-        S.nhist_m = S.nhist_r = 0;
-        FitOut o;
-        if (p.params.solver == 1 && psd_full)
-            o = (S.inc && p.params.lbfgs_m <= BLML_CMAX) ? S.solve_lbfgs_compact() : S.solve_lbfgs();
-        else
-            o = S.solve_cd();
-        // outputs (blml_binned.py:438-445)
-        const double *Ef = S.vec(0), *Uf = S.vec(1);
-        const long V = (long)f.K * f.B * ncf;
-        for (long i = threadIdx.x; i < V; i += BLML_NT) {
-            e_out[f.out_off + i] = (T)Ef[i];
-            se_out[f.out_off + i] = (T)Uf[i];
-        }
-        double ll = 0.0;
-        const double n2 = f.n * f.n;
-        for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
-            const double dens = S.Fb(Uf, b) / n2;
-            density[f.dens_off + b] = dens;
-            if (S.w[b] > 0) ll += S.w[b] * log(dens);
-        }
-        ll = block_sum1(ll, g_sl.red);
-        if (threadIdx.x == 0) {
-            blml_result_t r;
-            r.loglik = ll / f.n;
-            r.residual = o.residual;
-            r.n_sweeps = o.n_sweeps;
-            r.stop_reason = o.stop;
-            r.converged = o.converged;
-            r.n_anderson = o.n_and;
-            r.n_damped = o.n_damp;
-            r.n_gradient = o.n_grad;
-            r.start_sweeps = start_sweeps;
-            r.n_hist = S.nhist_m;
-            r.n_matvec = S.n_mv;
-            r.reserved = 0;
-            r.seconds[0] = (double)(wall_clock64() - tfit0) / tick_hz;
-            r.seconds[1] = (double)tstart / tick_hz;
-            r.seconds[2] = (double)S.t_mv / tick_hz;
-            r.seconds[3] = (double)S.t_dir / tick_hz;
-            r.seconds[4] = (double)S.t_ls / tick_hz;
-            results[fi] = r;
-        }
-        __syncthreads();
+        run_fit<T>(p, order[qi], block_team(), counts, draws, init, signs, e_out, se_out, density, results,
+                   merit_hist, rho_hist, tick_hz);
     }
+}
+
+// A plan of one large fit: every workgroup of a cooperative launch works on it (grid barriers between the
+// solver's data-dependent steps, FP64 team reductions).
+template <typename T>
+__global__ void __launch_bounds__(BLML_NT) fit_grid_kernel(DevPlan<T> p, unsigned *bar, double *part,
+                                                           const double *counts, const double *draws,
+                                                           const double *init, const double *signs, T *e_out,
+                                                           T *se_out, double *density, blml_result_t *results,
+                                                           double *merit_hist, double *rho_hist, double tick_hz) {
+    run_fit<T>(p, 0, Team{(int)blockIdx.x, (int)gridDim.x, bar, part, 0}, counts, draws, init, signs, e_out,
+               se_out, density, results, merit_hist, rho_hist, tick_hz);
 }
 
 // certificate(): lambda_max(D^1/2 S_0 D^1/2) / n, D_b = |e_b|^2 / w_b on occupied bins, by power iteration with an
 // FP64 Rayleigh quotient.  Stops when the quotient changes by less than tol (relative) or after max_iter steps.
 template <typename T>
-__global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<T> p, const T *e_hat, const double *counts,
-                                                       int max_iter, double tol, double *lam_out, int *iters) {
+__device__ void run_cert(const DevPlan<T> &p, int fi, Team tm, const T *e_hat, const double *counts, int max_iter,
+                         double tol, double *lam_out, int *iters) {
     __shared__ double red[BLML_NWAVE * BLML_NRED];
-    const int fi = blockIdx.x;
-    if (fi >= p.nfits) return;
     const blml_fit_t f = p.fits[fi];
     if (!f.shared) {
-        if (threadIdx.x == 0) { lam_out[fi] = NAN; iters[fi] = 0; }
+        if (tm.lead()) { lam_out[fi] = NAN; iters[fi] = 0; }
         return;
     }
     MatvecCtx<T> mv;
-    make_matvec_ctx<T>(p, f, mv);
+    make_matvec_ctx<T>(p, f, mv, tm);
     const int nc = f.complex_g ? 2 : 1;
     const int B = f.B;
+    const long b0 = tm.first(), bs = tm.stride();
     double *dh = (double *)(p.ws + f.ws_off + pool_bytes_off(f, (int)sizeof(T)));   // B doubles
     double *x = dh + B, *y = x + B;
     const double *w = counts + f.counts_off;
     const T *E = e_hat + f.out_off;
-    for (int b = threadIdx.x; b < B; b += BLML_NT) {
+    for (long b = b0; b < B; b += bs) {
         double D = 0.0;
         if (w[b] > 0) {
             for (int k = 0; k < f.K; ++k) {
@@ -374,57 +411,68 @@ __global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<T> p, const T *e_
         }
         dh[b] = sqrt(D);
     }
-    __syncthreads();
+    tm.sync();
     double nrm = 0.0;
-    for (int b = threadIdx.x; b < B; b += BLML_NT) nrm += dh[b] * dh[b];
-    nrm = sqrt(block_sum1(nrm, red));
+    for (long b = b0; b < B; b += bs) nrm += dh[b] * dh[b];
+    nrm = sqrt(tm.sum1(nrm, red));
     if (!(nrm > 0)) {
-        if (threadIdx.x == 0) { lam_out[fi] = 0.0; iters[fi] = 0; }
+        if (tm.lead()) { lam_out[fi] = 0.0; iters[fi] = 0; }
         return;
     }
-    for (int b = threadIdx.x; b < B; b += BLML_NT) x[b] = dh[b] / nrm;
-    __syncthreads();
+    for (long b = b0; b < B; b += bs) x[b] = dh[b] / nrm;
+    tm.sync();
     double lam = 0.0, lam_prev = -INFINITY;
     int it = 0;
     for (it = 1; it <= max_iter; ++it) {
-        for (int b = threadIdx.x; b < B; b += BLML_NT) y[b] = dh[b] * x[b];
-        __syncthreads();
+        for (long b = b0; b < B; b += bs) y[b] = dh[b] * x[b];
+        tm.sync();
         matvec_mode<T, double, double>(mv, f.fac_begin[0], f.fac_count[0], y, y, B, 1);
-        double a[2] = {0.0, 0.0};
-        for (int b = threadIdx.x; b < B; b += BLML_NT) {
+        double a[3] = {0.0, 0.0, 0.0};
+        for (long b = b0; b < B; b += bs) {
             const double yb = dh[b] * (double)y[b];
             a[0] += (double)x[b] * yb;
             a[1] += yb * yb;
+            a[2] += (double)x[b] * (double)x[b];
         }
-        block_sum<2>(a, red);
-        double xx = 0.0;
-        for (int b = threadIdx.x; b < B; b += BLML_NT) xx += (double)x[b] * (double)x[b];
-        xx = block_sum1(xx, red);
-        lam = a[0] / xx;
+        tm.sum<3>(a, red);
+        lam = a[0] / a[2];
         const double ny = sqrt(a[1]);
         if (!(ny > 0)) break;
-        for (int b = threadIdx.x; b < B; b += BLML_NT) x[b] = dh[b] * y[b] / ny;
-        __syncthreads();
+        for (long b = b0; b < B; b += bs) x[b] = dh[b] * y[b] / ny;
+        tm.sync();
         if (fabs(lam - lam_prev) <= tol * fabs(lam)) break;
         lam_prev = lam;
     }
-    if (threadIdx.x == 0) {
+    if (tm.lead()) {
         lam_out[fi] = lam / f.n;
         iters[fi] = it > max_iter ? max_iter : it;
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<T> p, const T *e_hat, const double *counts,
+                                                       int max_iter, double tol, double *lam_out, int *iters) {
+    if ((int)blockIdx.x >= p.nfits) return;
+    run_cert<T>(p, blockIdx.x, block_team(), e_hat, counts, max_iter, tol, lam_out, iters);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLML_NT) cert_grid_kernel(DevPlan<T> p, unsigned *bar, double *part,
+                                                            const T *e_hat, const double *counts, int max_iter,
+                                                            double tol, double *lam_out, int *iters) {
+    run_cert<T>(p, 0, Team{(int)blockIdx.x, (int)gridDim.x, bar, part, 0}, e_hat, counts, max_iter, tol, lam_out,
+                iters);
+}
+
 // FP32 plans only: Se, density and loglik of every fit re-evaluated with an FP64 mat-vec of the FP32-converged
 // coefficients (the iterations run in FP32; the outputs' likelihood sum is FP64 end to end).  p carries the FP64
 // spectra and twiddles; its workspace is the FP32 plan's, reinterpreted (sized for it by layout()).
-__global__ void __launch_bounds__(BLML_NT) finish64_kernel(DevPlan<double> p, float *se_out,
-                                                          const double *counts, double *density,
-                                                          blml_result_t *results) {
+__device__ void run_finish64(const DevPlan<double> &p, int fi, Team tm, float *se_out, const double *counts,
+                             double *density, blml_result_t *results) {
     __shared__ double red[BLML_NWAVE * BLML_NRED];
-    const int fi = blockIdx.x;
-    if (fi >= p.nfits) return;
     const blml_fit_t f = p.fits[fi];
     MatvecCtx<double> mv;
+    mv.tm = tm;
     mv.d = f.d;
     for (int j = 0; j < BLML_MAXD; ++j) mv.shape[j] = j < f.d ? f.shape[j] : 1;
     mv.facs = p.facs;
@@ -443,11 +491,11 @@ __global__ void __launch_bounds__(BLML_NT) finish64_kernel(DevPlan<double> p, fl
         const long off = (long)k * f.B * nc;
         matvec_mode<double, double, double>(mv, f.fac_begin[k], f.fac_count[k], E64 + off, U64 + off, f.B, nc);
     }
-    for (long i = threadIdx.x; i < V; i += BLML_NT) se_out[f.out_off + i] = (float)U64[i];
+    for (long i = tm.first(); i < V; i += tm.stride()) se_out[f.out_off + i] = (float)U64[i];
     const double *w = counts + f.counts_off;
     const double n2 = f.n * f.n;
     double ll = 0.0;
-    for (int b = threadIdx.x; b < f.B; b += BLML_NT) {
+    for (long b = tm.first(); b < f.B; b += tm.stride()) {
         double F = 0.0;
         for (int k = 0; k < f.K; ++k) {
             const long i = ((long)k * f.B + b) * nc;
@@ -457,8 +505,21 @@ __global__ void __launch_bounds__(BLML_NT) finish64_kernel(DevPlan<double> p, fl
         density[f.dens_off + b] = dens;
         if (w[b] > 0) ll += w[b] * log(dens);
     }
-    ll = block_sum1(ll, red);
-    if (threadIdx.x == 0) results[fi].loglik = ll / f.n;
+    ll = tm.sum1(ll, red);
+    if (tm.lead()) results[fi].loglik = ll / f.n;
+}
+
+__global__ void __launch_bounds__(BLML_NT) finish64_kernel(DevPlan<double> p, float *se_out,
+                                                          const double *counts, double *density,
+                                                          blml_result_t *results) {
+    if ((int)blockIdx.x >= p.nfits) return;
+    run_finish64(p, blockIdx.x, block_team(), se_out, counts, density, results);
+}
+
+__global__ void __launch_bounds__(BLML_NT) finish64_grid_kernel(DevPlan<double> p, unsigned *bar, double *part,
+                                                               float *se_out, const double *counts, double *density,
+                                                               blml_result_t *results) {
+    run_finish64(p, 0, Team{(int)blockIdx.x, (int)gridDim.x, bar, part, 0}, se_out, counts, density, results);
 }
 
 // ------------------------------------------------------------------------------------------------ host plan
@@ -484,6 +545,10 @@ struct Plan {
     int Lmax = 2;
     int lds_bytes = 0;
     int grid = 1;
+    bool spread = false;       // one fit over every CU: cooperative launches of coop workgroups
+    int coop = 0;
+    unsigned *d_bar = nullptr;  // grid barrier words
+    double *d_part = nullptr;   // team partial sums (2 * coop * BLML_NRED)
     int64_t out_elems = 0, dens_elems = 0;
     int64_t ws_elems = 0;
     size_t esize = 4;
@@ -681,6 +746,8 @@ int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs,
     return BLML_OK;
 }
 
+constexpr long kSpreadMin = 1L << 18;   // values (K * B * c) from which a lone fit spreads over the device
+
 // raises `st` to the static LDS (bytes of __shared__ arrays) of kernel `kern`
 int static_lds(const void *kern, int &st) {
     hipFuncAttributes a;
@@ -778,13 +845,42 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
     if (khz > 0) pl.tick_hz = 1e3 * khz;
     if (occ < 1) occ = 1;
     pl.grid = std::max(1, std::min(pl.nfits, ncu * occ));
+    // a plan of one large fit: every workgroup that fits on the device at once works on it
+    const long V0 = vmax_of(pl.hfits[0]);
+    pl.spread = pl.nfits == 1 && (pl.params.spread > 0 || (pl.params.spread == 0 && V0 >= kSpreadMin));
+    if (pl.spread) {
+        int coop_ok = 0;
+        HIPCHECK(hipDeviceGetAttribute(&coop_ok, hipDeviceAttributeCooperativeLaunch, dev));
+        if (!coop_ok) return fail(BLML_ERUNTIME, "the device does not support cooperative launches");
+        int o1 = 0, o2 = 0, o3 = 0, o4 = 0;
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o1, fit_grid_kernel<T>, BLML_NT, pl.lds_bytes));
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o2, matvec_grid_kernel<T>, BLML_NT, pl.lds_bytes));
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o3, cert_grid_kernel<T>, BLML_NT, pl.lds_bytes));
+        o4 = o1;
+        if (sizeof(T) == 4)
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, finish64_grid_kernel, BLML_NT,
+                                                                  pl.lds64_bytes));
+        const int oc = std::min(std::min(o1, o2), std::min(o3, o4));
+        if (oc < 1) return fail(BLML_ERUNTIME, "no workgroup of the spread kernels fits a CU");
+        pl.coop = ncu * oc;
+        HIPCHECK(hipMalloc(&pl.d_bar, 4 * sizeof(unsigned)));
+        HIPCHECK(hipMalloc(&pl.d_part, sizeof(double) * 2 * (size_t)pl.coop * BLML_NRED));
+        HIPCHECK(hipMemset(pl.d_bar, 0, 4 * sizeof(unsigned)));
+    }
+    return BLML_OK;
+}
+
+// a cooperative launch of a spread plan's kernel (the barrier words start cleared)
+int launch_coop(Plan &pl, const void *kern, void **args, int lds, hipStream_t st) {
+    HIPCHECK(hipMemsetAsync(pl.d_bar, 0, 4 * sizeof(unsigned), st));
+    HIPCHECK(hipLaunchCooperativeKernel(kern, dim3(pl.coop), dim3(BLML_NT), args, (unsigned)lds, st));
     return BLML_OK;
 }
 
 void free_plan(Plan *pl) {
     if (!pl) return;
     void *ptrs[] = {pl->d_fits, pl->d_facs, pl->d_fdata, pl->d_freqs, pl->d_spec, pl->d_tw,
-                    pl->d_spec64, pl->d_tw64, pl->d_ws, pl->d_order, pl->d_queue};
+                    pl->d_spec64, pl->d_tw64, pl->d_ws, pl->d_order, pl->d_queue, pl->d_bar, pl->d_part};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete pl;
@@ -832,7 +928,7 @@ int blml_plan_info(void *plan, int64_t *out_elems, int64_t *ws_bytes, int32_t *l
     if (out_elems) *out_elems = pl->out_elems;
     if (ws_bytes) *ws_bytes = pl->ws_elems;
     if (lds_bytes) *lds_bytes = pl->lds_bytes;
-    if (grid) *grid = pl->grid;
+    if (grid) *grid = pl->spread ? -pl->coop : pl->grid;   // negative: one fit spread over that many workgroups
     return BLML_OK;
 }
 
@@ -840,6 +936,16 @@ int blml_matvec(void *plan, const void *E, void *U, void *stream) {
     Plan *pl = (Plan *)plan;
     if (!pl || !E || !U) return fail(BLML_EVALUE, "null argument");
     hipStream_t st = (hipStream_t)stream;
+    if (pl->spread) {
+        if (pl->dtype == BLML_F32) {
+            DevPlan<float> dp = dev_plan<float>(*pl);
+            void *args[] = {&dp, &pl->d_bar, &pl->d_part, &E, &U};
+            return launch_coop(*pl, (const void *)matvec_grid_kernel<float>, args, pl->lds_bytes, st);
+        }
+        DevPlan<double> dp = dev_plan<double>(*pl);
+        void *args[] = {&dp, &pl->d_bar, &pl->d_part, &E, &U};
+        return launch_coop(*pl, (const void *)matvec_grid_kernel<double>, args, pl->lds_bytes, st);
+    }
     if (pl->dtype == BLML_F32)
         matvec_kernel<float><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<float>(*pl), (const float *)E,
                                                                           (float *)U);
@@ -860,6 +966,24 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
         if (f.has_signs && !signs) return fail(BLML_EVALUE, "a fit with signs needs signs");
     }
     hipStream_t st = (hipStream_t)stream;
+    if (pl->spread) {
+        double tick = pl->tick_hz;
+        int rc;
+        if (pl->dtype == BLML_F32) {
+            DevPlan<float> dp = dev_plan<float>(*pl);
+            void *args[] = {&dp, &pl->d_bar, &pl->d_part, &counts, &draws, &init, &signs, &e_out, &se_out,
+                            &density, &results, &merit_hist, &rho_hist, &tick};
+            rc = launch_coop(*pl, (const void *)fit_grid_kernel<float>, args, pl->lds_bytes, st);
+            if (rc) return rc;
+            DevPlan<double> d64 = dev_plan64(*pl);
+            void *a64[] = {&d64, &pl->d_bar, &pl->d_part, &se_out, &counts, &density, &results};
+            return launch_coop(*pl, (const void *)finish64_grid_kernel, a64, pl->lds64_bytes, st);
+        }
+        DevPlan<double> dp = dev_plan<double>(*pl);
+        void *args[] = {&dp, &pl->d_bar, &pl->d_part, &counts, &draws, &init, &signs, &e_out, &se_out,
+                        &density, &results, &merit_hist, &rho_hist, &tick};
+        return launch_coop(*pl, (const void *)fit_grid_kernel<double>, args, pl->lds_bytes, st);
+    }
     HIPCHECK(hipMemsetAsync(pl->d_queue, 0, sizeof(int), st));
     if (pl->dtype == BLML_F32) {
         fit_kernel<float><<<pl->grid, BLML_NT, pl->lds_bytes, st>>>(
@@ -883,6 +1007,16 @@ int blml_certificate(void *plan, const void *e_hat, const double *counts, int32_
     if (!pl || !e_hat || !counts || !lam || !iters) return fail(BLML_EVALUE, "null argument");
     if (max_iter < 1) return fail(BLML_EVALUE, "max_iter must be >= 1");
     hipStream_t st = (hipStream_t)stream;
+    if (pl->spread) {
+        if (pl->dtype == BLML_F32) {
+            DevPlan<float> dp = dev_plan<float>(*pl);
+            void *args[] = {&dp, &pl->d_bar, &pl->d_part, &e_hat, &counts, &max_iter, &tol, &lam, &iters};
+            return launch_coop(*pl, (const void *)cert_grid_kernel<float>, args, pl->lds_bytes, st);
+        }
+        DevPlan<double> dp = dev_plan<double>(*pl);
+        void *args[] = {&dp, &pl->d_bar, &pl->d_part, &e_hat, &counts, &max_iter, &tol, &lam, &iters};
+        return launch_coop(*pl, (const void *)cert_grid_kernel<double>, args, pl->lds_bytes, st);
+    }
     if (pl->dtype == BLML_F32)
         cert_kernel<float><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<float>(*pl), (const float *)e_hat,
                                                                         counts, max_iter, tol, lam, iters);

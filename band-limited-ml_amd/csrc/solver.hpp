@@ -76,6 +76,7 @@ __device__ inline double secular_root(const double *R2, const double *s, int K, 
 
 template <typename T>
 struct FitSolver {
+    Team tm;               // who runs the fit: one workgroup, or every workgroup of a cooperative launch
     MatvecCtx<T> mv;
     const blml_fit_t *f;
     const blml_params_t *P;
@@ -127,24 +128,24 @@ struct FitSolver {
     __device__ double dot(const double *a, const double *b) {
         double acc = 0.0;
         const long n_ = V();
-        for (long i = threadIdx.x; i < n_; i += BLML_NT) acc += a[i] * b[i];
-        return block_sum1(acc, g_sl.red);
+        for (long i = tm.first(); i < n_; i += tm.stride()) acc += a[i] * b[i];
+        return tm.sum1(acc, g_sl.red);
     }
     __device__ void copy(double *dst, const double *src) {
         const long n_ = V();
-        for (long i = threadIdx.x; i < n_; i += BLML_NT) dst[i] = src[i];
-        __syncthreads();
+        for (long i = tm.first(); i < n_; i += tm.stride()) dst[i] = src[i];
+        tm.sync();
     }
     __device__ void scale(double *a, double sc) {
         const long n_ = V();
-        for (long i = threadIdx.x; i < n_; i += BLML_NT) a[i] = a[i] * sc;
-        __syncthreads();
+        for (long i = tm.first(); i < n_; i += tm.stride()) a[i] = a[i] * sc;
+        tm.sync();
     }
     // out = a + t * b
     __device__ void axpy(double *out, const double *a, double t, const double *b) {
         const long n_ = V();
-        for (long i = threadIdx.x; i < n_; i += BLML_NT) out[i] = a[i] + t * b[i];
-        __syncthreads();
+        for (long i = tm.first(); i < n_; i += tm.stride()) out[i] = a[i] + t * b[i];
+        tm.sync();
     }
     __device__ bool occ(int b) const { return w[b] > 0; }
     __device__ double cb(int b) const { return n * w[b]; }
@@ -162,16 +163,16 @@ struct FitSolver {
     __device__ double merit(const double *E, const double *U) {
         if (psd) {
             double a[2] = {0.0, 0.0};
-            for (int b = threadIdx.x; b < B; b += BLML_NT) a[0] += cb(b) * log(fmax(Fb(U, b), 1e-30));
+            for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) a[0] += cb(b) * log(fmax(Fb(U, b), 1e-30));
             const long n_ = V();
-            for (long i = threadIdx.x; i < n_; i += BLML_NT) a[1] += E[i] * U[i];
-            block_sum<2>(a, g_sl.red);
+            for (long i = tm.first(); i < n_; i += tm.stride()) a[1] += E[i] * U[i];
+            tm.sum<2>(a, g_sl.red);
             const double Q = a[1];
             if (!(Q > 0)) return -INFINITY;
             return a[0] / n - n * log(Q);
         }
         double a[3] = {0.0, 0.0, 0.0};
-        for (int b = threadIdx.x; b < B; b += BLML_NT) {
+        for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
             const double e = E[b], u = U[b];
             const double sgb = sg ? sg[b] : 1.0;
             a[0] += e * u;
@@ -180,7 +181,7 @@ struct FitSolver {
                 if (sgb * e <= 0) a[2] += 1.0;
             }
         }
-        block_sum<3>(a, g_sl.red);
+        tm.sum<3>(a, g_sl.red);
         if (a[2] > 0) return -INFINITY;
         return -(0.5 * a[0] - a[1]) / n;
     }
@@ -188,16 +189,16 @@ struct FitSolver {
     __device__ void update(const double *E, const double *U, double *out) {
         if (!psd) {
             const double s0 = s[0];
-            for (int b = threadIdx.x; b < B; b += BLML_NT) {
+            for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
                 const double sgb = sg ? sg[b] : 1.0;
                 const double R = U[b] - s0 * E[b];
                 const double t = occ(b) ? cd_root(sgb * R, cb(b), s0) : 0.0;
                 out[b] = sgb * t;
             }
-            __syncthreads();
+            tm.sync();
             return;
         }
-        for (int b = threadIdx.x; b < B; b += BLML_NT) {
+        for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
             double Rr[BLML_MAXK], Ri[BLML_MAXK], R2[BLML_MAXK];
             double a2 = 0.0;
             for (int k = 0; k < K; ++k) {
@@ -234,24 +235,24 @@ struct FitSolver {
                 if (nc == 2) out[i + 1] = Ri[k];
             }
         }
-        __syncthreads();
+        tm.sync();
     }
 
     __device__ void rescale(double *E, double *U) {
         const double Q = dot(E, U);
         const double sc = Q > 0 ? n / sqrt(Q) : 1.0;
         const long n_ = V();
-        for (long i = threadIdx.x; i < n_; i += BLML_NT) {
+        for (long i = tm.first(); i < n_; i += tm.stride()) {
             E[i] *= sc;
             U[i] *= sc;
         }
-        __syncthreads();
+        tm.sync();
     }
 
     // max over occupied bins of ||u_b (|e_b|^2 / c_b) - e_b|| / ||e_b|| of the rescaled pair (sc E, sc U)
     __device__ double resid(const double *E, const double *U, double sc) {
         double r = 0.0;
-        for (int b = threadIdx.x; b < B; b += BLML_NT) {
+        for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
             if (!occ(b)) continue;
             double nrm2 = 0.0;
             for (int k = 0; k < K; ++k) nrm2 += absq(E, k, b);
@@ -267,14 +268,14 @@ struct FitSolver {
             }
             r = fmax(r, sqrt(acc) / sqrt(fmax(nrm2, 1e-300)));
         }
-        return block_max(r, g_sl.red);
+        return tm.max(r, g_sl.red);
     }
 
     // L = sum w log F - n log Q (blml_binned.py:278-284); writes Vt = (w / F) U on occupied bins when Vt is given
     // (the gradient's mat-vec input) and Q to *Qout
     __device__ double lval(const double *E, const double *U, double *Vt, double *Qout) {
         double a[2] = {0.0, 0.0};
-        for (int b = threadIdx.x; b < B; b += BLML_NT) {
+        for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
             const bool o = occ(b);
             const double F = Fb(U, b);
             a[0] += w[b] * log(o ? F : 1.0);
@@ -287,7 +288,7 @@ struct FitSolver {
                 }
             }
         }
-        block_sum<2>(a, g_sl.red);
+        tm.sum<2>(a, g_sl.red);
         *Qout = a[1];
         if (!(a[1] > 0)) return -INFINITY;
         return a[0] - n * log(a[1]);
@@ -296,29 +297,27 @@ struct FitSolver {
     __device__ void lgrad_from(const double *U, double Q, const double *Vt, double *SVt, double *g) {
         Sall(Vt, SVt);
         const double nq = n / Q;
-        for (int b = threadIdx.x; b < B; b += BLML_NT) {
+        for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
             const bool o = occ(b);
             for (int k = 0; k < K; ++k) {
                 const long i = ((long)k * B + b) * nc;
                 for (int q = 0; q < nc; ++q) g[i + q] = o ? 2.0 * (SVt[i + q] - nq * U[i + q]) : 0.0;
             }
         }
-        __syncthreads();
+        tm.sync();
     }
 
     // ------------------------------------------------------------------ histories
     __device__ void push_merit(double v) {
-        if (threadIdx.x == 0) {
-            g_sl.ring[ring_n % BLML_RING] = v;
-            if (rec && merit_hist && nhist_m < hist_cap) merit_hist[nhist_m] = v;
-        }
+        if (threadIdx.x == 0) g_sl.ring[ring_n % BLML_RING] = v;
+        if (tm.lead() && rec && merit_hist && nhist_m < hist_cap) merit_hist[nhist_m] = v;
         ++ring_n;
         if (rec) ++nhist_m;
         __syncthreads();
     }
     __device__ double ring_back(int k) const { return g_sl.ring[(ring_n - 1 - k) % BLML_RING]; }
     __device__ void push_rho(double v) {
-        if (threadIdx.x == 0 && rec && rho_hist && nhist_r < hist_cap) rho_hist[nhist_r] = v;
+        if (tm.lead() && rec && rho_hist && nhist_r < hist_cap) rho_hist[nhist_r] = v;
         if (rec) ++nhist_r;
     }
 
@@ -330,7 +329,7 @@ struct FitSolver {
         double acc[20];
         for (int i = 0; i < 20; ++i) acc[i] = 0.0;
         const long n_ = V();
-        for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+        for (long e = tm.first(); e < n_; e += tm.stride()) {
             double dF[BLML_MAXAND];
             for (int i = 0; i < m; ++i) dF[i] = vec(slotF[i + 1])[e] - vec(slotF[i])[e];
             const double fe = f[e];
@@ -339,7 +338,7 @@ struct FitSolver {
                 for (int j = i; j < m; ++j) acc[p++] += dF[i] * dF[j];
             for (int i = 0; i < m; ++i) acc[p++] += dF[i] * fe;
         }
-        block_sum<20>(acc, g_sl.red);
+        tm.sum<20>(acc, g_sl.red);
         double G[BLML_MAXAND][BLML_MAXAND + 1];
         int p = 0;
         double tr = 0.0;
@@ -368,7 +367,7 @@ struct FitSolver {
         }
         for (int i = 0; i < m; ++i)
             if (!isfinite(gam[i])) return false;
-        for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+        for (long e = tm.first(); e < n_; e += tm.stride()) {
             double mix = 0.0;
             for (int i = 0; i < m; ++i) {
                 const double dX = vec(slotX[i + 1])[e] - vec(slotX[i])[e];
@@ -377,7 +376,7 @@ struct FitSolver {
             }
             EA[e] = E[e] + f[e] - mix;
         }
-        __syncthreads();
+        tm.sync();
         return true;
     }
 
@@ -392,14 +391,14 @@ struct FitSolver {
             lval(Eo, Uo, Vb, &Q);
             Sall(Vb, Gb);
             const double nq = n / Q;
-            for (int b = threadIdx.x; b < B; b += BLML_NT) {
+            for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
                 const bool o = occ(b);
                 for (int k = 0; k < K; ++k) {
                     const long i = ((long)k * B + b) * nc;
                     for (int q = 0; q < nc; ++q) Gb[i + q] = o ? Gb[i + q] - nq * Uo[i + q] : 0.0;
                 }
             }
-            __syncthreads();
+            tm.sync();
             Sall(Gb, SGb);
             const double g2 = dot(Gb, Gb);
             if (!(g2 > 0)) break;
@@ -469,11 +468,11 @@ struct FitSolver {
                 double *xs = vec(13 + slot), *fs = vec(13 + hcap + slot);
                 const double *Ec = vec(iE), *EJc = vec(iEJ);
                 const long n_ = V();
-                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                for (long e = tm.first(); e < n_; e += tm.stride()) {
                     xs[e] = Ec[e];
                     fs[e] = EJc[e] - Ec[e];
                 }
-                __syncthreads();
+                tm.sync();
                 if (hcnt >= 3) {
                     int sx[BLML_MAXAND + 1], sf[BLML_MAXAND + 1];
                     for (int i = 0; i < hcnt; ++i) {
@@ -505,11 +504,11 @@ struct FitSolver {
                     const long n_ = V();
                     double *EJ = vec(iEJ), *UJ = vec(iUJ);
                     const double *Ec = vec(iE), *Uc = vec(iU), *EPc = vec(iEP), *UPc = vec(iUP);
-                    for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                    for (long e = tm.first(); e < n_; e += tm.stride()) {
                         EJ[e] = Ec[e] + a * (EPc[e] - Ec[e]);
                         UJ[e] = Uc[e] + a * (UPc[e] - Uc[e]);
                     }
-                    __syncthreads();
+                    tm.sync();
                     mJ = merit(vec(iEJ), vec(iUJ));
                     if (mJ >= cur) { found = true; break; }
                     a *= 0.5;
@@ -605,8 +604,8 @@ struct FitSolver {
                 const double sc = 1e-3 * sqrt(ee / fmax(gg, 1e-300));
                 const long n_ = V();
                 const double *g = vec(ig);
-                for (long e = threadIdx.x; e < n_; e += BLML_NT) q[e] = g[e] * sc;
-                __syncthreads();
+                for (long e = tm.first(); e < n_; e += tm.stride()) q[e] = g[e] * sc;
+                tm.sync();
                 slope = dot(vec(ig), q);
             }
             const long long td1 = wall_clock64();
@@ -618,11 +617,11 @@ struct FitSolver {
                 const long n_ = V();
                 double *En = vec(iEn), *Un = vec(iUn);
                 const double *Ec = vec(iE), *Uc = vec(iU), *Sq = vec(iSVt);
-                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                for (long e = tm.first(); e < n_; e += tm.stride()) {
                     En[e] = Ec[e] + t * q[e];
                     if (inc) Un[e] = Uc[e] + t * Sq[e];
                 }
-                __syncthreads();
+                tm.sync();
                 if (!inc) Sall(vec(iEn), vec(iUn));
                 Ln = lval(vec(iEn), vec(iUn), nullptr, &Qn);
                 if (Ln >= L + 1e-4 * t * slope) { ok = true; break; }
@@ -645,7 +644,7 @@ struct FitSolver {
                 const double *En = vec(iEn), *Ec = vec(iE), *g = vec(ig), *gn = vec(ign);
                 double a3[3] = {0.0, 0.0, 0.0};
                 const long n_ = V();
-                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                for (long e = tm.first(); e < n_; e += tm.stride()) {
                     const double sv = En[e] - Ec[e];
                     const double yv = g[e] - gn[e];
                     Sv[e] = sv;
@@ -654,7 +653,7 @@ struct FitSolver {
                     a3[1] += sv * sv;
                     a3[2] += yv * yv;
                 }
-                block_sum<3>(a3, g_sl.red);
+                tm.sum<3>(a3, g_sl.red);
                 if (a3[0] > 1e-12 * sqrt(a3[1] * a3[2])) {
                     if (threadIdx.x == 0) {
                         g_sl.lb_r[slot] = 1.0 / a3[0];
@@ -679,7 +678,7 @@ struct FitSolver {
             const double Qc = dot(vec(iE), vec(iU));
             double *Ecn = vec(iEn);
             const double *Uc = vec(iU);
-            for (int b = threadIdx.x; b < B; b += BLML_NT) {
+            for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
                 const bool ob = occ(b);
                 const double F = Fb(Uc, b);
                 const double fac = ob ? (Qc / n) * w[b] / F : 0.0;
@@ -688,7 +687,7 @@ struct FitSolver {
                     for (int qq = 0; qq < nc; ++qq) Ecn[i + qq] = fac * Uc[i + qq];
                 }
             }
-            __syncthreads();
+            tm.sync();
             Snew(vec(iEn), vec(iUn), vec(iE), vec(iU));
             double Qd = 0.0;
             const double Lc = lval(vec(iEn), vec(iUn), nullptr, &Qd);
@@ -728,7 +727,7 @@ struct FitSolver {
                 Sp[i] = vec(9 + sl_);
                 Yp[i] = vec(9 + ph + sl_);
             }
-            for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+            for (long e = tm.first(); e < n_; e += tm.stride()) {
                 const double ge = gn[e];
 #pragma unroll
                 for (int i = 0; i < BLML_CCHUNK; ++i) {
@@ -738,7 +737,7 @@ struct FitSolver {
                     }
                 }
             }
-            block_sum<2 * BLML_CCHUNK>(acc, g_sl.red);
+            tm.sum<2 * BLML_CCHUNK>(acc, g_sl.red);
             if (threadIdx.x == 0)
                 for (int i = 0; i < nc_; ++i) { g_sl.csg[c0 + i] = acc[i]; g_sl.cyg[c0 + i] = acc[BLML_CCHUNK + i]; }
         }
@@ -798,7 +797,7 @@ struct FitSolver {
                 __syncthreads();
                 double acc = 0.0;
                 const double *g = vec(ig);
-                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                for (long e = tm.first(); e < n_; e += tm.stride()) {
                     double v = gam * g[e];
                     for (int i = 0; i < cnt; ++i) {
                         const int sl_ = (head + i) % ph;
@@ -807,18 +806,18 @@ struct FitSolver {
                     q[e] = v;
                     acc += g[e] * v;
                 }
-                slope = block_sum1(acc, g_sl.red);
+                slope = tm.sum1(acc, g_sl.red);
             }
             if (!(slope > 0)) {
                 cnt = 0; head = 0;
                 const double sc = 1e-3 * sqrt(ee / fmax(gg, 1e-300));
                 const double *g = vec(ig);
                 double acc = 0.0;
-                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                for (long e = tm.first(); e < n_; e += tm.stride()) {
                     q[e] = g[e] * sc;
                     acc += g[e] * q[e];
                 }
-                slope = block_sum1(acc, g_sl.red);
+                slope = tm.sum1(acc, g_sl.red);
             }
             const long long td1 = wall_clock64();
             t_dir += td1 - td0;
@@ -828,11 +827,11 @@ struct FitSolver {
             for (int tr = 0; tr < 40; ++tr) {
                 double *En = vec(iEn), *Un = vec(iUn);
                 const double *Ec = vec(iE), *Uc = vec(iU), *Sq = vec(iSVt);
-                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                for (long e = tm.first(); e < n_; e += tm.stride()) {
                     En[e] = Ec[e] + t * q[e];
                     Un[e] = Uc[e] + t * Sq[e];
                 }
-                __syncthreads();
+                tm.sync();
                 Ln = lval(vec(iEn), vec(iUn), nullptr, &Qn);
                 if (Ln >= L + 1e-4 * t * slope) { ok = true; break; }
                 t *= 0.5;
@@ -855,7 +854,7 @@ struct FitSolver {
             {
                 double *Sv = vec(9 + slot), *Yv = vec(9 + ph + slot);
                 const double *En = vec(iEn), *Ec = vec(iE), *g = vec(ig), *gn = vec(ign);
-                for (long e = threadIdx.x; e < n_; e += BLML_NT) {
+                for (long e = tm.first(); e < n_; e += tm.stride()) {
                     const double sv = En[e] - Ec[e];
                     const double yv = g[e] - gn[e];
                     Sv[e] = sv;
@@ -868,7 +867,7 @@ struct FitSolver {
                     a7[5] += gn[e] * gn[e];
                     a7[6] += En[e] * En[e];
                 }
-                block_sum<7>(a7, g_sl.red);
+                tm.sum<7>(a7, g_sl.red);
             }
             // update the small matrices (thread 0) and a, b for the new gradient
             const bool accept = a7[0] > 1e-12 * sqrt(a7[1] * a7[2]);
@@ -926,7 +925,7 @@ struct FitSolver {
             const double Qc = dot(vec(iE), vec(iU));
             double *Ecn = vec(iEn);
             const double *Uc = vec(iU);
-            for (int b = threadIdx.x; b < B; b += BLML_NT) {
+            for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
                 const bool ob = occ(b);
                 const double F = Fb(Uc, b);
                 const double fac = ob ? (Qc / n) * w[b] / F : 0.0;
@@ -935,7 +934,7 @@ struct FitSolver {
                     for (int qq = 0; qq < nc; ++qq) Ecn[i + qq] = fac * Uc[i + qq];
                 }
             }
-            __syncthreads();
+            tm.sync();
             Snew(vec(iEn), vec(iUn), vec(iE), vec(iU));
             double Qd = 0.0;
             const double Lc = lval(vec(iEn), vec(iUn), nullptr, &Qd);

@@ -235,6 +235,7 @@ struct Pass {
     int outer, inner;     // lines = outer * inner; element (o, i, r) at (o * n * inner + i * inner + r)
     const int *F;         // circular: frequencies (0 <= F < m)
     const T *spec;        // CONV ops: spectrum values (real)
+    int rank, nranks;     // the team member running this call: rounds rank, rank + nranks, ... of the lines
     bool pairs;           // CONV ops on real data: two lines per transform, as real and imaginary part (exact: the
                           // operator is real, so S (x + i y) = S x + i S y)
 };
@@ -337,7 +338,7 @@ __device__ __noinline__ void run_pass(const Pass<T> &p, const cpx<T> *src, cpx<T
     const int *li2 = p.pairs ? lofs_in2 : nullptr;
     const int *lo2 = p.pairs ? lofs_out2 : nullptr;
     const float inv_inner = 1.0f / (float)p.inner;
-    for (long base = 0; base < nlines; base += nl) {
+    for (long base = (long)p.rank * nl; base < nlines; base += (long)p.nranks * nl) {
         for (int l = threadIdx.x; l < nl; l += BLML_NT) {
             const long t = base + l;
             const long la = p.pairs ? 2 * t : t, lb = p.pairs ? 2 * t + 1 : -1;
@@ -429,6 +430,7 @@ struct MatvecCtx {
     const cpx<T> *tw;          // full-circle twiddles of the longest FFT
     int Lmax;
     cpx<T> *X, *Y;             // scratch (scratch_len complex each)
+    Team tm;                   // who runs the mat-vec (passes split their lines over the team)
 };
 
 template <typename T>
@@ -439,7 +441,8 @@ __device__ void mul_group_spec(const MatvecCtx<T> &c, const blml_factor_t &fa, c
     const bool all = fa.naxes == c.d;      // the group spans every axis: the spectrum index is the element index
     float inv[BLML_MAXD];
     for (int j = 0; j < c.d; ++j) inv[j] = 1.0f / (float)ext[j];
-    for (int idx = threadIdx.x; idx < total; idx += BLML_NT) {
+    for (long idx_ = c.tm.first(); idx_ < total; idx_ += c.tm.stride()) {
+        const int idx = (int)idx_;
         int g = idx;
         if (!all) {
             int rem = idx;
@@ -455,7 +458,7 @@ __device__ void mul_group_spec(const MatvecCtx<T> &c, const blml_factor_t &fa, c
         const T s = spec[g];
         buf[idx] = {buf[idx].x * s, buf[idx].y * s};
     }
-    __syncthreads();
+    c.tm.sync();
 }
 
 template <typename T>
@@ -472,6 +475,8 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa, boo
     if (fa.kind == BLML_FAC_LINE) {
         const int a = fa.axes[0];
         Pass<T> p;
+        p.rank = c.tm.rank;
+        p.nranks = c.tm.nranks;
         p.pairs = real;
         p.m = c.shape[a];
         p.n_in = p.n_out = p.m;
@@ -489,6 +494,7 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa, boo
             p.F = nullptr;
         }
         run_pass<T>(p, c.X, c.X, c.tw, c.Lmax);
+        c.tm.sync();
         return;
     }
     int ext[BLML_MAXD];
@@ -497,6 +503,8 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa, boo
     for (int ai = fa.naxes - 1; ai >= 0; --ai) {          // forward, last axis first
         const int a = fa.axes[ai];
         Pass<T> p;
+        p.rank = c.tm.rank;
+        p.nranks = c.tm.nranks;
         p.m = c.shape[a];
         p.n_in = ext[a];
         line_geom<T>(ext, c.d, a, p.outer, p.inner);
@@ -510,6 +518,7 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa, boo
         p.spec = nullptr;
         p.pairs = false;
         run_pass<T>(p, cur, oth, c.tw, c.Lmax);
+        c.tm.sync();
         ext[a] = p.n_out;
         cpx<T> *t = cur; cur = oth; oth = t;
     }
@@ -517,6 +526,8 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa, boo
     for (int ai = 0; ai < fa.naxes; ++ai) {               // inverse, first axis first
         const int a = fa.axes[ai];
         Pass<T> p;
+        p.rank = c.tm.rank;
+        p.nranks = c.tm.nranks;
         p.m = c.shape[a];
         p.n_in = ext[a];
         p.n_out = p.m;
@@ -529,6 +540,7 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa, boo
         p.spec = nullptr;
         p.pairs = false;
         run_pass<T>(p, cur, oth, c.tw, c.Lmax);
+        c.tm.sync();
         ext[a] = p.m;
         cpx<T> *t = cur; cur = oth; oth = t;
     }
@@ -542,11 +554,12 @@ template <typename T, typename Ti, typename To>
 __device__ void matvec_mode(const MatvecCtx<T> &c, const int fac_begin, const int fac_count, const Ti *E, To *U,
                             int B, int nc, const Ti *Ebase = nullptr, const To *Ubase = nullptr) {
     constexpr int UB = 4;                 // bins per thread in flight
-    for (int b0 = threadIdx.x; b0 < B; b0 += UB * BLML_NT) {
+    const long st = c.tm.stride();
+    for (long b0 = c.tm.first(); b0 < B; b0 += UB * st) {
         double re[UB], im[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            const int b = b0 + u * BLML_NT;
+            const long b = b0 + u * st;
             re[u] = 0.0; im[u] = 0.0;
             if (b < B) {
                 re[u] = (double)E[nc * b];
@@ -559,17 +572,17 @@ __device__ void matvec_mode(const MatvecCtx<T> &c, const int fac_begin, const in
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            const int b = b0 + u * BLML_NT;
+            const long b = b0 + u * st;
             if (b < B) c.X[b] = {(T)re[u], (T)im[u]};
         }
     }
-    __syncthreads();
+    c.tm.sync();
     for (int f = 0; f < fac_count; ++f) apply_factor<T>(c, c.facs[fac_begin + f], nc == 1);
-    for (int b0 = threadIdx.x; b0 < B; b0 += UB * BLML_NT) {
+    for (long b0 = c.tm.first(); b0 < B; b0 += UB * st) {
         double re[UB], im[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            const int b = b0 + u * BLML_NT;
+            const long b = b0 + u * st;
             re[u] = 0.0; im[u] = 0.0;
             if (b < B) {
                 const cpx<T> x = c.X[b];
@@ -583,12 +596,12 @@ __device__ void matvec_mode(const MatvecCtx<T> &c, const int fac_begin, const in
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            const int b = b0 + u * BLML_NT;
+            const long b = b0 + u * st;
             if (b < B) {
                 U[nc * b] = (To)re[u];
                 if (nc == 2) U[2 * b + 1] = (To)im[u];
             }
         }
     }
-    __syncthreads();
+    c.tm.sync();
 }

@@ -84,6 +84,8 @@ typedef struct {
     int32_t solver;                  /* 0 'cd', 1 'lbfgs' */
     int32_t lbfgs_m;                 /* <= 64 */
     int32_t hist_cap;                /* merit / rho history entries kept per fit */
+    int32_t spread;                  /* a plan of one fit: 0 spreads it over every CU when it has >= 2^18 values,
+                                        1 always, -1 never (one workgroup per fit) */
 } blml_params_t;
 
 /* Per-fit outputs (the scalar attributes of a fitted BLMLBinned, blml_binned.py:273-275, 351-354, 438-445). */
@@ -130,7 +132,8 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
 int blml_certificate(void *plan, const void *e_hat, const double *counts, int32_t max_iter, double tol,
                      double *lam, int32_t *iters, void *stream);
 
-/* Sizes the plan chose (for the caller's buffers and for reporting). */
+/* Sizes the plan chose (for the caller's buffers and for reporting).  grid: workgroups of the batched launch, or
+ * minus the workgroup count when the plan's one fit is spread over the device. */
 int blml_plan_info(void *plan, int64_t *out_elems, int64_t *ws_bytes, int32_t *lds_bytes, int32_t *grid);
 
 const char *blml_last_error(void);

@@ -1,0 +1,148 @@
+"""A lone fit spread over every CU (cooperative launch, grid barriers, FP64 team reductions) against the oracle and
+against the same fit run by one workgroup.  Tolerances: mat-vec as in test_gpu_engine (FP64 1e-12 of the Gram's
+row sums, FP32 2e-6 of max |S e|); fits: loglik 1e-8 relative between the two team shapes (the same iteration with
+the FP64 sums grouped differently; converged fits) and 1e-9 against the oracle's unique optimum (real rank 1); certificate 1e-6 relative to dense eigvalsh."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import blml_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _modes(mm, lib):
+    return [md for kw in mm for md in lib.make_modes(**kw)]
+
+
+PARAMS = dict(tol=1e-5, max_sweeps=None, anderson=5, stall=1000, phase_noise=0.1, ll_tol=None, ll_window=100,
+              solver='cd', lbfgs_m=20)
+
+SPREAD_MATVEC = [
+    ('1d sinc 4096', (4096,), [dict(fc=[0.05], kinds=['sinc'])], 1.0, False),
+    ('2d slits + dirichlet bg 64x45 real', (64, 45), [dict(fc=[0.35, 0.3], kinds=['slits:0.5', 'slits:0.4']),
+                                                      dict(fc=[3 / 64, 3 / 45], kinds=['dirichlet', 'dirichlet'],
+                                                           periods=[64.0, 45.0])], 1.0, False),
+    ('2d pball 31x31 + bg complex', (31, 31), [dict(fc=[[0.33, 0.33], [3 / 31, 3 / 31]], kinds=['pball', 'pball'],
+                                                    periods=[31.0, 31.0])], 1.0, True),
+    ('2d ball h=0.5', (11, 13), [dict(fc=[0.8, 0.8], kinds=['ball', 'ball'])], 0.5, False),
+    ('2d pball padded', (9, 10), [dict(fc=[0.4, 0.4], kinds=['pball', 'pball'], periods=[12.0, 15.0])], 1.0, False),
+    ('3d sinc x dirichlet complex', (20, 16, 9), [dict(fc=[0.3, 0.25, 0.4], kinds=['sinc', 'dirichlet', 'sinc'],
+                                                       periods=[None, 16.0, None])], 1.0, True),
+    ('5d sinc 10^5', (10,) * 5, [dict(fc=[0.3] * 5, kinds=['sinc'] * 5)], 1.0, False),
+]
+
+
+def _one_fit_batch(counts, modes, h, cplx, dtype, spread, params=PARAMS):
+    from blml_amd.binned import _FitSpec, FitBatch
+    fs = _FitSpec(counts, modes, h, 0.0, cplx, None, None, 0)
+    return fs, FitBatch([fs], params, dtype, spread=spread).build()
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32], ids=['f64', 'f32'])
+def test_spread_matvec_matches_oracle(dtype):
+    import torch
+    import blml_amd
+    rng = np.random.default_rng(11)
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    for name, shape, mm, h, cplx in SPREAD_MATVEC:
+        counts = rng.poisson(3.0, shape).astype(float) + 1
+        _note(f'spread matvec {name} dtype {np.dtype(dtype).name}')
+        fs, batch = _one_fit_batch(counts, _modes(mm, blml_amd), h, cplx, dtype, 1)
+        assert batch.info()['grid'] < 0                      # spread over the device
+        om = O.BLMLBinnedOracle()
+        om._setup(counts, _modes(mm, O), h, 0.0)
+        E = rng.standard_normal((fs.K,) + shape)
+        if cplx:
+            E = E + 1j * rng.standard_normal((fs.K,) + shape)
+        v = np.stack([E.real, E.imag], -1) if cplx else E[..., None]
+        U = batch.matvec(torch.from_numpy(v.reshape(-1).copy()).to('cuda', tdt)).double().cpu().numpy()
+        u = U.reshape((fs.K, fs.B, fs.nc))
+        u = u[..., 0] + 1j * u[..., 1] if cplx else u[..., 0]
+        for k in range(fs.K):
+            want = om._S(k, E[k]).ravel()
+            err = np.abs(u[k] - want).max()
+            if dtype == np.float64:
+                G = O.dense_gram(om.specs_[k], shape, om.h_) if fs.B <= 4096 else None
+                scale = np.abs(G).sum(1).max() if G is not None else np.abs(want).max() * 10
+                assert err < 1e-12 * max(1.0, scale) * max(1.0, np.abs(E).max()), (name, k, err)
+            else:
+                assert err < 2e-6 * np.abs(want).max() * max(1.0, np.log2(fs.B)), (name, k, err)
+        batch.close()
+
+
+def _field_counts(rng, shape, n):
+    g = np.meshgrid(*[np.arange(m, dtype=float) for m in shape], indexing='ij')
+    p = np.exp(-0.5 * sum(((x - m / 2.2) / (m / 4.5)) ** 2 for x, m in zip(g, shape))) + 0.05
+    p = p / p.sum()
+    return rng.multinomial(n, p.ravel()).reshape(shape).astype(float)
+
+
+SPREAD_FITS = [
+    # name, shape, modes, complex_g, solver, max_sweeps
+    ('2d sinc x sinc real cd', (48, 40), [dict(fc=[0.3, 0.25], kinds=['sinc', 'sinc'])], False, 'cd', 20000),
+    ('4d sinc real cd', (12, 10, 9, 8), [dict(fc=[0.35] * 4, kinds=['sinc'] * 4)], False, 'cd', 20000),
+    ('2d pball + bg complex lbfgs', (31, 31), [dict(fc=[[0.33, 0.33], [3 / 31, 3 / 31]], kinds=['pball', 'pball'],
+                                                    periods=[31.0, 31.0])], True, 'lbfgs', 5000),
+    ('2d slits + dirichlet bg complex lbfgs', (40, 36), [dict(fc=[0.35, 0.3], kinds=['slits:0.5', 'slits:0.4']),
+                                                         dict(fc=[3 / 40, 3 / 36], kinds=['dirichlet', 'dirichlet'],
+                                                              periods=[40.0, 36.0])], True, 'lbfgs', 5000),
+]
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32], ids=['f64', 'f32'])
+def test_spread_fits_match_one_workgroup(dtype):
+    import blml_amd
+    rng = np.random.default_rng(12)
+    for name, shape, mm, cplx, solver, ms in SPREAD_FITS:
+        counts = _field_counts(rng, shape, 20000)
+        # converged fits only: a trajectory stopped early by max_sweeps is chaotic in the grouping of its FP64 sums
+        # when the fit is non-convex
+        params = dict(PARAMS, solver=solver, max_sweeps=ms, tol=1e-8 if dtype == np.float64 else 1e-5,
+                      ll_tol=1e-10 if cplx else (None if dtype == np.float64 else 1e-12))
+        out = {}
+        for spread in (1, -1):
+            _note(f'spread fit {name} dtype {np.dtype(dtype).name} spread {spread}')
+            fs, batch = _one_fit_batch(counts, _modes(mm, blml_amd), 1.0, cplx, dtype, spread, params)
+            out[spread] = batch.run().results()[0]
+            batch.close()
+        a, b = out[1], out[-1]
+        rel = abs(a['loglik'] - b['loglik']) / abs(b['loglik'])
+        _note(f'   spread {a["loglik"]:.12f} ({a["n_sweeps"]} sweeps, {a["stop_reason"]}) one workgroup '
+              f'{b["loglik"]:.12f} ({b["n_sweeps"]}, {b["stop_reason"]}) rel {rel:.1e}')
+        assert a['stop_reason'] != 'max_sweeps' and b['stop_reason'] != 'max_sweeps', name
+        assert rel < (1e-8 if dtype == np.float64 else 1e-6), (name, rel)
+        if not cplx and len(mm) == 1:                          # convex: the oracle's unique optimum
+            r = O.BLMLBinnedOracle(tol=1e-9).fit(counts, _modes(mm, O))
+            rel_o = abs(a['loglik'] - r.loglik_) / abs(r.loglik_)
+            _note(f'   oracle {r.loglik_:.12f} rel {rel_o:.1e}')
+            assert rel_o < (1e-9 if dtype == np.float64 else 1e-6), (name, rel_o)
+
+
+def test_spread_certificate_matches_dense():
+    import torch
+    import blml_amd
+    rng = np.random.default_rng(13)
+    shape = (16, 16)
+    N = _field_counts(rng, shape, 4000)
+    mm = [dict(fc=[0.4, 0.4], kinds=['ball', 'ball'])]
+    params = dict(PARAMS, tol=1e-8)
+    fs, batch = _one_fit_batch(N, _modes(mm, blml_amd), 1.0, False, np.float64, 1, params)
+    r = batch.run().results()[0]
+    lam, its = batch.certificate(max_iter=100000, tol=1e-12)
+    batch.close()
+    occ = N.ravel() > 0
+    om = O.BLMLBinnedOracle()
+    om._setup(N, _modes(mm, O), 1.0, 0.0)
+    S = O.dense_gram(om.specs_[0], shape, om.h_)[np.ix_(occ, occ)]
+    E = np.asarray(r['e_hat']).reshape(1, -1)[:, occ]
+    D = (np.abs(E) ** 2).sum(0) / N.ravel()[occ]
+    want = np.linalg.eigvalsh(np.sqrt(D)[:, None] * S * np.sqrt(D)[None])[-1] / N.sum()
+    _note(f'spread certificate {lam[0]:.9f} dense {want:.9f} iters {its[0]}')
+    assert abs(lam[0] - want) < 1e-6 * want
