@@ -32,6 +32,21 @@ __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, 
 template <typename T>
 __device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
 
+// q = a / d, r = a % d for 0 <= a and 1 <= d, from a float reciprocal with one correction step when a < 2^24
+// (exact there; integer division is a long instruction sequence on the GPU), by integer division beyond.
+__device__ __forceinline__ int fdivmod(int a, int d, float inv_d, int &r) {
+    if (a >= (1 << 24)) {
+        const int q = a / d;
+        r = a - q * d;
+        return q;
+    }
+    int q = (int)((float)a * inv_d);
+    r = a - q * d;
+    if (r < 0) { --q; r += d; }
+    else if (r >= d) { ++q; r -= d; }
+    return q;
+}
+
 __device__ __forceinline__ double wave_sum(double a) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);

@@ -49,6 +49,9 @@ struct DevPlan {
     unsigned char *ws;     // per-fit workspaces, byte offsets fit.ws_off
     int nfits;
     blml_params_t params;
+    const SepPlan *sep;    // spread plans of separable modes: [0] for the mat-vec kernel, [1] for the fit kernel
+    const T *sep_tab;
+    const long *sep_off;
 };
 
 // A fit's workspace (bytes from ws_off): [X, Y: scratch_len complex values of the transform type]
@@ -139,8 +142,12 @@ __global__ void __launch_bounds__(BLML_NT) spec_kernel(DevPlan<T> p, int nfacs, 
 }
 
 template <typename T>
-__device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, MatvecCtx<T> &mv, const Team &tm) {
+__device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, MatvecCtx<T> &mv, const Team &tm,
+                                const SepPlan *sep = nullptr) {
     mv.tm = tm;
+    mv.sep = sep;
+    mv.sep_tab = p.sep_tab;
+    mv.sep_off = p.sep_off;
     mv.d = f.d;
     for (int j = 0; j < BLML_MAXD; ++j) mv.shape[j] = j < f.d ? f.shape[j] : 1;
     mv.facs = p.facs;
@@ -153,10 +160,10 @@ __device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, Matvec
 }
 
 template <typename T>
-__device__ void run_matvec(const DevPlan<T> &p, int fi, const Team &tm, const T *E, T *U) {
+__device__ void run_matvec(const DevPlan<T> &p, int fi, const Team &tm, const T *E, T *U, const SepPlan *sep) {
     const blml_fit_t f = p.fits[fi];
     MatvecCtx<T> mv;
-    make_matvec_ctx<T>(p, f, mv, tm);
+    make_matvec_ctx<T>(p, f, mv, tm, sep);
     const int nc = f.complex_g ? 2 : 1;
     for (int k = 0; k < f.K; ++k) {
         const long off = f.out_off + (long)k * f.B * nc;
@@ -168,14 +175,14 @@ __device__ void run_matvec(const DevPlan<T> &p, int fi, const Team &tm, const T 
 template <typename T>
 __global__ void __launch_bounds__(BLML_NT) matvec_kernel(DevPlan<T> p, const T *E, T *U) {
     if ((int)blockIdx.x >= p.nfits) return;
-    run_matvec<T>(p, blockIdx.x, block_team(), E, U);
+    run_matvec<T>(p, blockIdx.x, block_team(), E, U, nullptr);
 }
 
 // a plan of one fit, spread over every workgroup of a cooperative launch
 template <typename T>
 __global__ void __launch_bounds__(BLML_NT) matvec_grid_kernel(DevPlan<T> p, unsigned *bar, double *part, const T *E,
                                                              T *U) {
-    run_matvec<T>(p, 0, Team{(int)blockIdx.x, (int)gridDim.x, bar, part, 0}, E, U);
+    run_matvec<T>(p, 0, Team{(int)blockIdx.x, (int)gridDim.x, bar, part, 0}, E, U, p.sep);
 }
 
 // One whole fit (start, solve, outputs), run by a team.
@@ -186,7 +193,7 @@ __device__ void run_fit(const DevPlan<T> &p, int fi, const Team &team, const dou
     const blml_fit_t f = p.fits[fi];
     FitSolver<T> S;
     S.tm = team;
-    make_matvec_ctx<T>(p, f, S.mv, team);
+    make_matvec_ctx<T>(p, f, S.mv, team, team.nranks > 1 && p.sep ? p.sep + 1 : nullptr);
     S.f = &p.fits[fi];
     S.P = &p.params;
     S.B = f.B;
@@ -473,6 +480,7 @@ __device__ void run_finish64(const DevPlan<double> &p, int fi, Team tm, float *s
     const blml_fit_t f = p.fits[fi];
     MatvecCtx<double> mv;
     mv.tm = tm;
+    mv.sep = nullptr;
     mv.d = f.d;
     for (int j = 0; j < BLML_MAXD; ++j) mv.shape[j] = j < f.d ? f.shape[j] : 1;
     mv.facs = p.facs;
@@ -549,6 +557,11 @@ struct Plan {
     int coop = 0;
     unsigned *d_bar = nullptr;  // grid barrier words
     double *d_part = nullptr;   // team partial sums (2 * coop * BLML_NRED)
+    bool sep = false;           // spread plan of separable modes: tile passes with dense line operators
+    SepPlan *d_sep = nullptr;   // [0] mat-vec kernel, [1] fit kernel
+    void *d_sep_tab = nullptr;
+    long *d_sep_off = nullptr;
+    int lds_mv = 0, lds_fit = 0;   // dynamic LDS of the spread mat-vec and fit launches
     int64_t out_elems = 0, dens_elems = 0;
     int64_t ws_elems = 0;
     size_t esize = 4;
@@ -573,6 +586,9 @@ DevPlan<T> dev_plan(const Plan &pl) {
     p.ws = (unsigned char *)pl.d_ws;
     p.nfits = pl.nfits;
     p.params = pl.params;
+    p.sep = pl.sep ? pl.d_sep : nullptr;
+    p.sep_tab = (const T *)pl.d_sep_tab;
+    p.sep_off = pl.d_sep_off;
     return p;
 }
 
@@ -589,6 +605,9 @@ DevPlan<double> dev_plan64(const Plan &pl) {
     p.ws = (unsigned char *)pl.d_ws;
     p.nfits = pl.nfits;
     p.params = pl.params;
+    p.sep = nullptr;
+    p.sep_tab = nullptr;
+    p.sep_off = nullptr;
     return p;
 }
 
@@ -746,13 +765,158 @@ int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs,
     return BLML_OK;
 }
 
-constexpr long kSpreadMin = 1L << 18;   // values (K * B * c) from which a lone fit spreads over the device
+constexpr long kSpreadMin = 1L << 18;
+
+// The dense symmetric operator of one LINE factor on m points (FP64), zero-padded to the MFMA block multiple
+// P x P of separable.hpp.
+void sep_tables(const blml_factor_t &fa, int m, int blk, const double *fdata, const int32_t *freqs,
+                std::vector<double> &out) {
+    std::vector<double> M((size_t)m * m);
+    if (!fa.circ[0]) {
+        const double *row = fdata + fa.data_off;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) M[(size_t)i * m + j] = row[i > j ? i - j : j - i];
+    } else {
+        std::vector<double> c(m, 0.0);
+        const double *sp = fdata + fa.data_off;
+        const int32_t *F = freqs + fa.freq_off[0];
+        for (int k = 0; k <= m / 2; ++k) {
+            double acc = 0.0;
+            for (int q = 0; q < fa.tlen[0]; ++q) {
+                const long ph = ((long)F[q] * k) % m;
+                acc += sp[q] * std::cos(2.0 * M_PI * (double)ph / (double)m);
+            }
+            c[k] = acc;
+            c[(m - k) % m] = acc;
+        }
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) M[(size_t)i * m + j] = c[((i - j) % m + m) % m];
+    }
+    const int P = sep_table_dim(m, blk);
+    for (int i = 0; i < P; ++i)
+        for (int j = 0; j < P; ++j) out.push_back(i < m && j < m ? M[(size_t)i * m + j] : 0.0);
+}
+
+// One tile pass over axes [lo, hi) for tiles of at most tile_max values (rows of at least 16 values, or the whole
+// inner extent); false if the group does not fit.
+bool sep_make_pass(const int *m, int d, int nc, int lo, int hi, long tile_max, SepPass &ps) {
+    long rs = nc;
+    for (int j = hi; j < d; ++j) rs *= m[j];
+    long R = 1;
+    for (int j = lo; j < hi; ++j) R *= m[j];
+    const long wmin = hi == d ? rs : std::min<long>(rs, 16);
+    if (R * wmin > tile_max) return false;
+    long w;
+    if (hi == d || rs * R <= tile_max) {
+        w = rs;
+    } else {
+        w = 1;
+        while (w * 2 <= rs && R * w * 2 <= tile_max) w *= 2;
+    }
+    long nout = 1;
+    for (int j = 0; j < lo; ++j) nout *= m[j];
+    const long OB = std::min(std::max<long>(1, tile_max / (R * w)), nout);
+    ps = SepPass{};
+    ps.lo = lo;
+    ps.hi = hi;
+    ps.OB = (int)OB;
+    ps.R = (int)R;
+    ps.w = (int)w;
+    ps.rs = rs;
+    ps.nout = nout;
+    ps.nchunks = (rs + w - 1) / w;
+    ps.ntiles = ((nout + OB - 1) / OB) * ps.nchunks;
+    return true;
+}
+
+// Cuts the lattice into groups of consecutive axes, one tile pass each (innermost group first), with as few
+// passes as the tile size allows (every cut of d <= 5 axes is tried); false if some axis alone does not fit.
+bool sep_plan_passes(const int *m, int d, int nc, long tile_max, SepPlan &sp) {
+    int best = -1, best_np = BLML_SEP_MAXPASS + 1;
+    long best_tile = 0;
+    for (int cuts = 0; cuts < (1 << (d - 1)); ++cuts) {     // bit j set: a cut between axes j and j + 1
+        int np = 0, hi = d;
+        bool ok = true;
+        long mint = -1;
+        SepPass ps;
+        for (int lo = d - 1; lo >= 0 && ok; --lo) {
+            if (lo == 0 || (cuts >> (lo - 1) & 1)) {
+                ok = sep_make_pass(m, d, nc, lo, hi, tile_max, ps);
+                const long t = (long)ps.OB * ps.R * ps.w;
+                mint = mint < 0 ? t : std::min(mint, t);
+                ++np;
+                hi = lo;
+            }
+        }
+        if (ok && (np < best_np || (np == best_np && mint > best_tile))) {
+            best = cuts;
+            best_np = np;
+            best_tile = mint;
+        }
+    }
+    if (best < 0) return false;
+    sp = SepPlan{};
+    long big = 0;
+    int hi = d;
+    for (int lo = d - 1; lo >= 0; --lo) {
+        if (lo == 0 || (best >> (lo - 1) & 1)) {
+            SepPass &ps = sp.pass[sp.npass++];
+            sep_make_pass(m, d, nc, lo, hi, tile_max, ps);
+            big = std::max(big, (long)ps.OB * ps.R * ps.w);
+            hi = lo;
+        }
+    }
+    sp.tile = (int)(big + big / 32 + 1);
+    return true;
+}
 
 // raises `st` to the static LDS (bytes of __shared__ arrays) of kernel `kern`
 int static_lds(const void *kern, int &st) {
     hipFuncAttributes a;
     HIPCHECK(hipFuncGetAttributes(&a, kern));
     st = std::max(st, (int)a.sharedSizeBytes);
+    return BLML_OK;
+}
+
+// A spread plan whose modes are all LINE factors on axes of at most SepMfma<T>::MMAX points takes the separable
+// path: the factors' dense tables on the device and two pass plans sized for the LDS the mat-vec and fit kernels
+// leave free.
+template <typename T>
+int build_sep(Plan &pl, const double *fdata, const int32_t *freqs, int max_lds) {
+    const blml_fit_t &f = pl.hfits[0];
+    for (const blml_factor_t &fa : pl.hfacs)
+        if (fa.kind != BLML_FAC_LINE || f.shape[fa.axes[0]] > SepMfma<T>::MMAX) return BLML_OK;
+    std::vector<double> tab;
+    std::vector<long> off(std::max(1, pl.nfacs), 0);
+    const long pq_all = 0;   // the tables are read from global memory, not staged in LDS
+    for (int i = 0; i < pl.nfacs; ++i) {
+        off[i] = (long)tab.size();
+        sep_tables(pl.hfacs[i], f.shape[pl.hfacs[i].axes[0]], SepMfma<T>::BLK, fdata, freqs, tab);
+    }
+    const int nc = f.complex_g ? 2 : 1;
+    const size_t es = sizeof(T);
+    int st_mv = 0, st_fit = 0;
+    int rc = static_lds((const void *)matvec_grid_kernel<T>, st_mv);
+    if (!rc) rc = static_lds((const void *)fit_grid_kernel<T>, st_fit);
+    if (rc) return rc;
+    SepPlan sp[2];
+    const int stat[2] = {st_mv, st_fit};
+    for (int k = 0; k < 2; ++k) {
+        const long avail = ((long)max_lds - stat[k] - 64) / (long)es - pq_all - 2;
+        const long tile_max = avail * 32 / 33;
+        if (tile_max < 64 || !sep_plan_passes(f.shape, f.d, nc, tile_max, sp[k])) return BLML_OK;
+        sp[k].pq = (int)pq_all;
+    }
+    std::vector<T> tabT(tab.begin(), tab.end());
+    HIPCHECK(hipMalloc(&pl.d_sep, sizeof(SepPlan) * 2));
+    HIPCHECK(hipMalloc(&pl.d_sep_tab, sizeof(T) * std::max<size_t>(1, tabT.size())));
+    HIPCHECK(hipMalloc(&pl.d_sep_off, sizeof(long) * off.size()));
+    HIPCHECK(hipMemcpy(pl.d_sep, sp, sizeof(SepPlan) * 2, hipMemcpyHostToDevice));
+    if (!tabT.empty()) HIPCHECK(hipMemcpy(pl.d_sep_tab, tabT.data(), sizeof(T) * tabT.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(pl.d_sep_off, off.data(), sizeof(long) * off.size(), hipMemcpyHostToDevice));
+    pl.lds_mv = std::max(pl.lds_bytes, (int)((sp[0].tile + pq_all + 2) * es));
+    pl.lds_fit = std::max(pl.lds_bytes, (int)((sp[1].tile + pq_all + 2) * es));
+    pl.sep = true;
     return BLML_OK;
 }
 
@@ -866,6 +1030,21 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
         HIPCHECK(hipMalloc(&pl.d_bar, 4 * sizeof(unsigned)));
         HIPCHECK(hipMalloc(&pl.d_part, sizeof(double) * 2 * (size_t)pl.coop * BLML_NRED));
         HIPCHECK(hipMemset(pl.d_bar, 0, 4 * sizeof(unsigned)));
+        pl.lds_mv = pl.lds_fit = pl.lds_bytes;
+        rc = build_sep<T>(pl, fdata, freqs, max_lds);
+        if (rc) return rc;
+        if (pl.sep) {   // the spread launches use the separable path; their co-resident grid follows its LDS
+            int a = 0, b = 0;
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, fit_grid_kernel<T>, BLML_NT, pl.lds_fit));
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, matvec_grid_kernel<T>, BLML_NT, pl.lds_mv));
+            const int oc2 = std::min(std::min(a, b), oc);
+            if (oc2 < 1) return fail(BLML_ERUNTIME, "no workgroup of the separable spread kernels fits a CU");
+            if (oc2 < oc) {
+                pl.coop = ncu * oc2;
+                HIPCHECK(hipFree(pl.d_part));
+                HIPCHECK(hipMalloc(&pl.d_part, sizeof(double) * 2 * (size_t)pl.coop * BLML_NRED));
+            }
+        }
     }
     return BLML_OK;
 }
@@ -880,7 +1059,8 @@ int launch_coop(Plan &pl, const void *kern, void **args, int lds, hipStream_t st
 void free_plan(Plan *pl) {
     if (!pl) return;
     void *ptrs[] = {pl->d_fits, pl->d_facs, pl->d_fdata, pl->d_freqs, pl->d_spec, pl->d_tw,
-                    pl->d_spec64, pl->d_tw64, pl->d_ws, pl->d_order, pl->d_queue, pl->d_bar, pl->d_part};
+                    pl->d_spec64, pl->d_tw64, pl->d_ws, pl->d_order, pl->d_queue, pl->d_bar, pl->d_part,
+                    pl->d_sep, pl->d_sep_tab, pl->d_sep_off};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete pl;
@@ -940,11 +1120,11 @@ int blml_matvec(void *plan, const void *E, void *U, void *stream) {
         if (pl->dtype == BLML_F32) {
             DevPlan<float> dp = dev_plan<float>(*pl);
             void *args[] = {&dp, &pl->d_bar, &pl->d_part, &E, &U};
-            return launch_coop(*pl, (const void *)matvec_grid_kernel<float>, args, pl->lds_bytes, st);
+            return launch_coop(*pl, (const void *)matvec_grid_kernel<float>, args, pl->lds_mv, st);
         }
         DevPlan<double> dp = dev_plan<double>(*pl);
         void *args[] = {&dp, &pl->d_bar, &pl->d_part, &E, &U};
-        return launch_coop(*pl, (const void *)matvec_grid_kernel<double>, args, pl->lds_bytes, st);
+        return launch_coop(*pl, (const void *)matvec_grid_kernel<double>, args, pl->lds_mv, st);
     }
     if (pl->dtype == BLML_F32)
         matvec_kernel<float><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<float>(*pl), (const float *)E,
@@ -973,7 +1153,7 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
             DevPlan<float> dp = dev_plan<float>(*pl);
             void *args[] = {&dp, &pl->d_bar, &pl->d_part, &counts, &draws, &init, &signs, &e_out, &se_out,
                             &density, &results, &merit_hist, &rho_hist, &tick};
-            rc = launch_coop(*pl, (const void *)fit_grid_kernel<float>, args, pl->lds_bytes, st);
+            rc = launch_coop(*pl, (const void *)fit_grid_kernel<float>, args, pl->lds_fit, st);
             if (rc) return rc;
             DevPlan<double> d64 = dev_plan64(*pl);
             void *a64[] = {&d64, &pl->d_bar, &pl->d_part, &se_out, &counts, &density, &results};
@@ -982,7 +1162,7 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
         DevPlan<double> dp = dev_plan<double>(*pl);
         void *args[] = {&dp, &pl->d_bar, &pl->d_part, &counts, &draws, &init, &signs, &e_out, &se_out,
                         &density, &results, &merit_hist, &rho_hist, &tick};
-        return launch_coop(*pl, (const void *)fit_grid_kernel<double>, args, pl->lds_bytes, st);
+        return launch_coop(*pl, (const void *)fit_grid_kernel<double>, args, pl->lds_fit, st);
     }
     HIPCHECK(hipMemsetAsync(pl->d_queue, 0, sizeof(int), st));
     if (pl->dtype == BLML_F32) {

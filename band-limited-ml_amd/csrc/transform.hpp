@@ -16,6 +16,7 @@
 #pragma once
 #include "common.hpp"
 #include "blml_hip.h"
+#include "separable.hpp"
 
 enum {
     OP_CONV_PAD = 0,   // m -> m: zero-pad to L, FFT, * spec, IFFT, first m
@@ -97,21 +98,6 @@ template <int R>
 struct Log2 { static constexpr int v = R <= 1 ? 0 : 1 + Log2<R / 2>::v; };
 template <>
 struct Log2<1> { static constexpr int v = 0; };
-
-// q = a / d, r = a % d for 0 <= a and 1 <= d, from a float reciprocal with one correction step when a < 2^24
-// (exact there; integer division is a long instruction sequence on the GPU), by integer division beyond.
-__device__ __forceinline__ int fdivmod(int a, int d, float inv_d, int &r) {
-    if (a >= (1 << 24)) {
-        const int q = a / d;
-        r = a - q * d;
-        return q;
-    }
-    int q = (int)((float)a * inv_d);
-    r = a - q * d;
-    if (r < 0) { --q; r += d; }
-    else if (r >= d) { ++q; r -= d; }
-    return q;
-}
 
 // In-register R-point DFT (R = 2..16), natural order in and out: y_k = sum_r v_r exp(dir 2 pi i r k / R).
 template <typename T, int R>
@@ -431,7 +417,38 @@ struct MatvecCtx {
     int Lmax;
     cpx<T> *X, *Y;             // scratch (scratch_len complex each)
     Team tm;                   // who runs the mat-vec (passes split their lines over the team)
+    const SepPlan *sep;        // separable modes: dense line operators on LDS tiles (null: transform passes)
+    const T *sep_tab;          // per factor: P, Q, c_mid, middle row
+    const long *sep_off;
 };
+
+// U_k = S_k E_k of a separable mode by tile passes (see separable.hpp); the intermediate lives in c.X.
+template <typename T, typename Ti, typename To>
+__device__ void matvec_sep(const MatvecCtx<T> &c, const int fac_begin, const int fac_count, const Ti *E, To *U,
+                           int nc, const Ti *Ebase, const To *Ubase) {
+    int fac_of[BLML_MAXD];
+    for (int a = 0; a < BLML_MAXD; ++a) fac_of[a] = -1;
+    for (int f = 0; f < fac_count; ++f) fac_of[c.facs[fac_begin + f].axes[0]] = fac_begin + f;
+    T *Z = reinterpret_cast<T *>(c.X);
+    const int np = c.sep->npass;
+    for (int q = 0; q < np; ++q) {
+        const SepPass ps = c.sep->pass[q];
+        const bool first = q == 0, last = q == np - 1;
+        if (first && last)
+            sep_pass<T, Ti, To>(ps, c.d, nc, c.shape, fac_of, c.sep_tab, c.sep_off, E, Ebase, U, Ubase, c.tm.rank,
+                                c.tm.nranks);
+        else if (first)
+            sep_pass<T, Ti, T>(ps, c.d, nc, c.shape, fac_of, c.sep_tab, c.sep_off, E, Ebase, Z, (const T *)nullptr,
+                               c.tm.rank, c.tm.nranks);
+        else if (last)
+            sep_pass<T, T, To>(ps, c.d, nc, c.shape, fac_of, c.sep_tab, c.sep_off, Z, (const T *)nullptr, U, Ubase,
+                               c.tm.rank, c.tm.nranks);
+        else
+            sep_pass<T, T, T>(ps, c.d, nc, c.shape, fac_of, c.sep_tab, c.sep_off, Z, (const T *)nullptr, Z,
+                              (const T *)nullptr, c.tm.rank, c.tm.nranks);
+        c.tm.sync();
+    }
+}
 
 template <typename T>
 __device__ void mul_group_spec(const MatvecCtx<T> &c, const blml_factor_t &fa, cpx<T> *buf, const int *ext,
@@ -553,6 +570,10 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa, boo
 template <typename T, typename Ti, typename To>
 __device__ void matvec_mode(const MatvecCtx<T> &c, const int fac_begin, const int fac_count, const Ti *E, To *U,
                             int B, int nc, const Ti *Ebase = nullptr, const To *Ubase = nullptr) {
+    if (c.sep) {
+        matvec_sep<T, Ti, To>(c, fac_begin, fac_count, E, U, nc, Ebase, Ubase);
+        return;
+    }
     constexpr int UB = 4;                 // bins per thread in flight
     const long st = c.tm.stride();
     for (long b0 = c.tm.first(); b0 < B; b0 += UB * st) {

@@ -95,6 +95,29 @@ def uneven_fits(n_fits=N_FITS, seed=3):
     return out
 
 
+LARGE_SHAPE = (32,) * 5
+LARGE_H = 0.25                 # [-4, 4]^5 in 32 bins per axis
+LARGE_FC = 0.12 / LARGE_H      # 0.12 cycles per bin, in 1 / coordinate units (the fit runs with h = 0.25)
+
+
+def large_fit_counts(n=10_000_000, seed=2):
+    """Counts of the large single fit: n draws of a 4-component 5-D Gaussian mixture (means U[-2, 2]^5, diagonal
+    sds U[0.3, 0.8], equal weights; numpy default_rng(seed)), binned on 32^5 bins over [-4, 4]^5 (draws outside are
+    dropped).  Returns (counts, make_modes groups, fit kwargs)."""
+    rng = np.random.default_rng(seed)
+    d, nb = len(LARGE_SHAPE), LARGE_SHAPE[0]
+    means = rng.uniform(-2.0, 2.0, (4, d))
+    sds = rng.uniform(0.3, 0.8, (4, d))
+    comp = rng.integers(0, 4, n)
+    x = means[comp] + sds[comp] * rng.standard_normal((n, d))
+    idx = np.floor((x + 4.0) / LARGE_H).astype(np.int64)
+    keep = np.all((idx >= 0) & (idx < nb), axis=1)
+    flat = np.ravel_multi_index(tuple(idx[keep].T), LARGE_SHAPE)
+    counts = np.bincount(flat, minlength=nb ** d).astype(float).reshape(LARGE_SHAPE)
+    groups = [dict(fc=[LARGE_FC] * d, kinds=['sinc'] * d)]
+    return counts, groups, dict(h=LARGE_H, origin=-4.0)
+
+
 def modes_of(groups, lib):
     return [md for kw in groups for md in lib.make_modes(**kw)]
 
