@@ -111,13 +111,17 @@ __device__ __forceinline__ double block_max1(double v, double *red) { return blo
 #define BLML_BAR_SPIN (1L << 26)    // polls (each with a short sleep) before a grid barrier declares a timeout
 
 // bar[0]: arrivals, bar[1]: generation, bar[2]: set when a wait timed out (the results carry it to the host).
-// Every thread fences at agent scope on both sides, so plain global stores made before the barrier by any
-// workgroup are visible to loads made after it by any other (the L2s of the XCDs are not coherent by
-// themselves).  The wait is bounded: the kernel always ends, and a timeout is reported instead of hanging.
+// Every thread first waits until its own memory operations are complete (its stores have reached the L2); after
+// the workgroup barrier one thread fences at agent scope on both sides: the release writes the L2's dirty lines
+// back, the acquire invalidates this CU's caches, so plain global stores made before the barrier by any workgroup
+// are visible to loads made after it by any other (the L2s of the XCDs are not coherent by themselves).  One
+// fence pair per workgroup, not per wave: each is a cache-wide operation.  The wait is bounded: the kernel always
+// ends, and a timeout is reported instead of hanging.
 __device__ inline void grid_barrier(unsigned *bar, int nranks) {
-    __threadfence();
+    __builtin_amdgcn_s_waitcnt(0);      // vmcnt = expcnt = lgkmcnt = 0
     __syncthreads();
     if (threadIdx.x == 0) {
+        __threadfence();
         const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == (unsigned)nranks - 1) {
@@ -135,9 +139,9 @@ __device__ inline void grid_barrier(unsigned *bar, int nranks) {
                 }
             }
         }
+        __threadfence();
     }
     __syncthreads();
-    __threadfence();
 }
 
 struct Team {

@@ -146,6 +146,7 @@ __device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, Matvec
                                 const SepPlan *sep = nullptr) {
     mv.tm = tm;
     mv.sep = sep;
+    mv.sep_last_sync = true;
     mv.sep_tab = p.sep_tab;
     mv.sep_off = p.sep_off;
     mv.d = f.d;
@@ -164,6 +165,7 @@ __device__ void run_matvec(const DevPlan<T> &p, int fi, const Team &tm, const T 
     const blml_fit_t f = p.fits[fi];
     MatvecCtx<T> mv;
     make_matvec_ctx<T>(p, f, mv, tm, sep);
+    mv.sep_last_sync = f.K > 1;      // between modes only: the kernel's end orders the last one
     const int nc = f.complex_g ? 2 : 1;
     for (int k = 0; k < f.K; ++k) {
         const long off = f.out_off + (long)k * f.B * nc;
@@ -481,6 +483,7 @@ __device__ void run_finish64(const DevPlan<double> &p, int fi, Team tm, float *s
     MatvecCtx<double> mv;
     mv.tm = tm;
     mv.sep = nullptr;
+    mv.sep_last_sync = true;
     mv.d = f.d;
     for (int j = 0; j < BLML_MAXD; ++j) mv.shape[j] = j < f.d ? f.shape[j] : 1;
     mv.facs = p.facs;
@@ -797,9 +800,14 @@ void sep_tables(const blml_factor_t &fa, int m, int blk, const double *fdata, co
         for (int j = 0; j < P; ++j) out.push_back(i < m && j < m ? M[(size_t)i * m + j] : 0.0);
 }
 
-// One tile pass over axes [lo, hi) for tiles of at most tile_max values (rows of at least 16 values, or the whole
-// inner extent); false if the group does not fit.
-bool sep_make_pass(const int *m, int d, int nc, int lo, int hi, long tile_max, SepPass &ps) {
+// One tile pass over axes [lo, hi) in `avail` LDS values, which hold the tile (with its bank padding and dump
+// slots) and the staged tables of the pass's axes (stage[a] values each); rows of at least 16 values, or the whole
+// inner extent.  false if the group does not fit.
+bool sep_make_pass(const int *m, const int *stage, int d, int nc, int lo, int hi, long avail, SepPass &ps) {
+    long st_sum = 0;
+    for (int a = lo; a < hi; ++a) st_sum += stage[a];
+    const long tile_max = (avail - st_sum - 66) * 32 / 33;
+    if (tile_max < 64) return false;
     long rs = nc;
     for (int j = hi; j < d; ++j) rs *= m[j];
     long R = 1;
@@ -831,7 +839,7 @@ bool sep_make_pass(const int *m, int d, int nc, int lo, int hi, long tile_max, S
 
 // Cuts the lattice into groups of consecutive axes, one tile pass each (innermost group first), with as few
 // passes as the tile size allows (every cut of d <= 5 axes is tried); false if some axis alone does not fit.
-bool sep_plan_passes(const int *m, int d, int nc, long tile_max, SepPlan &sp) {
+bool sep_plan_passes(const int *m, const int *stage, int d, int nc, long avail, SepPlan &sp) {
     int best = -1, best_np = BLML_SEP_MAXPASS + 1;
     long best_tile = 0;
     for (int cuts = 0; cuts < (1 << (d - 1)); ++cuts) {     // bit j set: a cut between axes j and j + 1
@@ -841,7 +849,7 @@ bool sep_plan_passes(const int *m, int d, int nc, long tile_max, SepPlan &sp) {
         SepPass ps;
         for (int lo = d - 1; lo >= 0 && ok; --lo) {
             if (lo == 0 || (cuts >> (lo - 1) & 1)) {
-                ok = sep_make_pass(m, d, nc, lo, hi, tile_max, ps);
+                ok = sep_make_pass(m, stage, d, nc, lo, hi, avail, ps);
                 const long t = (long)ps.OB * ps.R * ps.w;
                 mint = mint < 0 ? t : std::min(mint, t);
                 ++np;
@@ -861,12 +869,15 @@ bool sep_plan_passes(const int *m, int d, int nc, long tile_max, SepPlan &sp) {
     for (int lo = d - 1; lo >= 0; --lo) {
         if (lo == 0 || (best >> (lo - 1) & 1)) {
             SepPass &ps = sp.pass[sp.npass++];
-            sep_make_pass(m, d, nc, lo, hi, tile_max, ps);
-            big = std::max(big, (long)ps.OB * ps.R * ps.w);
+            sep_make_pass(m, stage, d, nc, lo, hi, avail, ps);
+            long st_sum = 0;
+            for (int a = lo; a < hi; ++a) st_sum += stage[a];
+            const long t = (long)ps.OB * ps.R * ps.w;
+            big = std::max(big, t + t / 32 + 1 + 64 + st_sum);
             hi = lo;
         }
     }
-    sp.tile = (int)(big + big / 32 + 1);
+    sp.tile = (int)big;     // LDS values of the largest pass: tile, padding, dump slots, staged tables
     return true;
 }
 
@@ -888,7 +899,8 @@ int build_sep(Plan &pl, const double *fdata, const int32_t *freqs, int max_lds) 
         if (fa.kind != BLML_FAC_LINE || f.shape[fa.axes[0]] > SepMfma<T>::MMAX) return BLML_OK;
     std::vector<double> tab;
     std::vector<long> off(std::max(1, pl.nfacs), 0);
-    const long pq_all = 0;   // the tables are read from global memory, not staged in LDS
+    int stage[BLML_MAXD] = {0, 0, 0, 0, 0};
+    for (int a = 0; a < f.d; ++a) stage[a] = sep_stage_len<T>(f.shape[a]);
     for (int i = 0; i < pl.nfacs; ++i) {
         off[i] = (long)tab.size();
         sep_tables(pl.hfacs[i], f.shape[pl.hfacs[i].axes[0]], SepMfma<T>::BLK, fdata, freqs, tab);
@@ -902,10 +914,9 @@ int build_sep(Plan &pl, const double *fdata, const int32_t *freqs, int max_lds) 
     SepPlan sp[2];
     const int stat[2] = {st_mv, st_fit};
     for (int k = 0; k < 2; ++k) {
-        const long avail = ((long)max_lds - stat[k] - 64) / (long)es - pq_all - 2;
-        const long tile_max = avail * 32 / 33;
-        if (tile_max < 64 || !sep_plan_passes(f.shape, f.d, nc, tile_max, sp[k])) return BLML_OK;
-        sp[k].pq = (int)pq_all;
+        const long avail = ((long)max_lds - stat[k] - 64) / (long)es - 2;
+        if (!sep_plan_passes(f.shape, stage, f.d, nc, avail, sp[k])) return BLML_OK;
+        sp[k].pq = 0;
     }
     std::vector<T> tabT(tab.begin(), tab.end());
     HIPCHECK(hipMalloc(&pl.d_sep, sizeof(SepPlan) * 2));
@@ -914,8 +925,8 @@ int build_sep(Plan &pl, const double *fdata, const int32_t *freqs, int max_lds) 
     HIPCHECK(hipMemcpy(pl.d_sep, sp, sizeof(SepPlan) * 2, hipMemcpyHostToDevice));
     if (!tabT.empty()) HIPCHECK(hipMemcpy(pl.d_sep_tab, tabT.data(), sizeof(T) * tabT.size(), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(pl.d_sep_off, off.data(), sizeof(long) * off.size(), hipMemcpyHostToDevice));
-    pl.lds_mv = std::max(pl.lds_bytes, (int)((sp[0].tile + pq_all + 2) * es));
-    pl.lds_fit = std::max(pl.lds_bytes, (int)((sp[1].tile + pq_all + 2) * es));
+    pl.lds_mv = std::max(pl.lds_bytes, (int)((sp[0].tile + 2) * es));
+    pl.lds_fit = std::max(pl.lds_bytes, (int)((sp[1].tile + 2) * es));
     pl.sep = true;
     return BLML_OK;
 }

@@ -418,13 +418,15 @@ struct MatvecCtx {
     cpx<T> *X, *Y;             // scratch (scratch_len complex each)
     Team tm;                   // who runs the mat-vec (passes split their lines over the team)
     const SepPlan *sep;        // separable modes: dense line operators on LDS tiles (null: transform passes)
+    bool sep_last_sync;        // team barrier after a separable mat-vec's last pass (false: the kernel ends there)
     const T *sep_tab;          // per factor: P, Q, c_mid, middle row
     const long *sep_off;
 };
 
-// U_k = S_k E_k of a separable mode by tile passes (see separable.hpp); the intermediate lives in c.X.
+// U_k = S_k E_k of a separable mode by tile passes (see separable.hpp); the intermediate lives in c.X.  The one call
+// boundary of the separable path (its tile loops are inlined, so no registers are saved per tile or axis).
 template <typename T, typename Ti, typename To>
-__device__ void matvec_sep(const MatvecCtx<T> &c, const int fac_begin, const int fac_count, const Ti *E, To *U,
+__device__ __noinline__ void matvec_sep(const MatvecCtx<T> &c, const int fac_begin, const int fac_count, const Ti *E, To *U,
                            int nc, const Ti *Ebase, const To *Ubase) {
     int fac_of[BLML_MAXD];
     for (int a = 0; a < BLML_MAXD; ++a) fac_of[a] = -1;
@@ -446,7 +448,7 @@ __device__ void matvec_sep(const MatvecCtx<T> &c, const int fac_begin, const int
         else
             sep_pass<T, T, T>(ps, c.d, nc, c.shape, fac_of, c.sep_tab, c.sep_off, Z, (const T *)nullptr, Z,
                               (const T *)nullptr, c.tm.rank, c.tm.nranks);
-        c.tm.sync();
+        if (!last || c.sep_last_sync) c.tm.sync();
     }
 }
 
