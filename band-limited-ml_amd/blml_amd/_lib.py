@@ -50,7 +50,7 @@ RESULT_DTYPE = np.dtype([(name, (np.float64, 5) if name == 'seconds' else
 assert RESULT_DTYPE.itemsize == ctypes.sizeof(Result)
 
 EXPORTS = ('blml_plan_create', 'blml_plan_destroy', 'blml_matvec', 'blml_fit', 'blml_certificate',
-           'blml_plan_info', 'blml_last_error')
+           'blml_plan_info', 'blml_pair_rows', 'blml_pair_cols', 'blml_last_error')
 
 _lib = None
 
@@ -74,6 +74,8 @@ def load(path=None):
     lib.blml_certificate.argtypes = [vp, dp, dp, i32, ctypes.c_double, dp, dp, vp]
     lib.blml_plan_info.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i32),
                                    ctypes.POINTER(i32)]
+    lib.blml_pair_rows.argtypes = [vp, vp, vp, vp, i64, i64, vp, vp]
+    lib.blml_pair_cols.argtypes = [vp, vp, vp, i64, i64, i64, vp, vp, vp]
     lib.blml_last_error.restype = ctypes.c_char_p
     for name in EXPORTS[:-1]:
         getattr(lib, name).restype = ctypes.c_int

@@ -1,0 +1,168 @@
+"""Pair accumulation over lattice bins on the MI355X: the reference's blml_local_gpu.rows / .cols
+(blml_local_gpu.py:453-475) over the same table dicts (blml_kernels.ProductKernel.tables, blml_kernels.py:133-200),
+computed by libblml_hip.so (blml_pair_rows / blml_pair_cols).
+
+    s, ds = rows(ZA, ZB, w, tab)           s_r = sum_j S(ZA_r, ZB_j) w_j   (ds: zeros; derivative rows are not built)
+    out = cols(ZA, ZB, tab)                the block S(ZA_r, ZB_j) (R, M)
+    out, out_m = cols(ZA, ZB, tab, depmask=mask)   with the marginal over the columns not in mask
+
+Inputs are host arrays (integer lattice coordinates, int64 as the reference's); results are numpy.  f32=True reads
+float32 tables and multiplies in float32, with FP64 sums (the reference's f32 policy, blml_local_gpu.py:5-8).  The
+coordinates must lie inside the tables' spans: a bin outside them means the tables predate it (the reference's
+stale-table case) and raises RuntimeError before anything runs."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+__all__ = ['rows', 'cols', 'rank_coords']
+
+MAXD, MAXG = 8, 8
+RADIAL_Q = 4
+
+
+class PairTab(ctypes.Structure):
+    _fields_ = [('d', ctypes.c_int32), ('G', ctypes.c_int32), ('f32', ctypes.c_int32), ('reserved', ctypes.c_int32),
+                ('kind', ctypes.c_int32 * MAXD), ('grp', ctypes.c_int32 * MAXD), ('base', ctypes.c_int64 * MAXD),
+                ('J', ctypes.c_int64 * MAXD), ('mk', ctypes.c_int64 * MAXD), ('nbins', ctypes.c_int64 * MAXD),
+                ('nT', ctypes.c_int64), ('LR', ctypes.c_int64), ('T', ctypes.c_void_p), ('RT', ctypes.c_void_p)]
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("the pair kernels need a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def rank_coords(Z, tab):
+    """Coordinates as the row kernels read them: a rank column's value replaced by its rank among the table's values
+    (blml_kernels.py:203-217); a value the table lacks raises RuntimeError (stale tables)."""
+    Z = np.array(Z, np.int64, copy=True, ndmin=2)
+    for k, v in enumerate(tab['vals']):
+        if v is None:
+            continue
+        v = np.asarray(v, np.int64)
+        r = np.minimum(np.searchsorted(v, Z[:, k]), len(v) - 1)
+        if len(Z) and not np.array_equal(v[r], Z[:, k]):
+            raise RuntimeError(f"rank column {k}: a bin value is not in the kernel table (stale tables)")
+        Z[:, k] = r
+    return Z
+
+
+def _groups(tab):
+    grp = np.asarray(tab['grp'])
+    return int(np.atleast_2d(tab['R']).shape[0]) if (grp >= 0).any() else 0
+
+
+def _check_spans(A, B, tab):
+    """Every pair's table index inside its table (the kernels clamp; a clamp would be a stale table)."""
+    if not len(A) or not len(B):
+        return
+    kinds, Jk, mk, nbins, grp = (np.asarray(tab[k]) for k in ('kinds', 'Jk', 'mk', 'nbins', 'grp'))
+    lo_a, hi_a, lo_b, hi_b = A.min(0), A.max(0), B.min(0), B.max(0)
+    far = np.maximum(hi_a - lo_b, hi_b - lo_a)                  # the largest |za - zb| per column
+    G = _groups(tab)
+    LR = np.atleast_2d(tab['R']).shape[1]
+    r2max = np.zeros(max(G, 1), np.int64)
+    for k in range(A.shape[1]):
+        kd = int(kinds[k])
+        if kd == 0 and far[k] > Jk[k]:
+            raise RuntimeError(f"column {k}: offsets up to {far[k]} beyond the table's half width {Jk[k]} "
+                               "(stale tables)")
+        if kd == 3 and max(hi_a[k], hi_b[k]) >= mk[k]:
+            raise RuntimeError(f"rank column {k}: rank beyond the table (stale tables)")
+        if kd in (1, 4) and far[k] >= nbins[k] + (kd == 1):
+            raise RuntimeError(f"column {k}: offsets beyond one period (stale tables)")
+        if kd == 2:
+            r2max[grp[k]] += int(far[k]) ** 2
+    for g in range(G):
+        tor = bool(np.any((kinds == 4) & (grp == g)))
+        if not tor and np.sqrt(float(r2max[g])) * RADIAL_Q + 1 >= LR:
+            raise RuntimeError(f"radial group {g}: distances beyond its row (stale tables)")
+
+
+def _prepare(ZA, ZB, tab, f32, ranked):
+    torch = _torch()
+    A = np.ascontiguousarray(ZA if ranked else rank_coords(ZA, tab), np.int64)
+    B = np.ascontiguousarray(ZB if ranked else rank_coords(ZB, tab), np.int64)
+    A, B = np.atleast_2d(A), np.atleast_2d(B)
+    d = A.shape[1]
+    if d > MAXD or B.shape[1] != d:
+        raise ValueError(f"coordinates: the same number of columns for ZA and ZB, at most {MAXD}")
+    G = _groups(tab)
+    if G > MAXG:
+        raise ValueError(f"at most {MAXG} radial groups on this engine")
+    for Z in (A, B):
+        if len(Z) and (Z.min() < -(1 << 31) or Z.max() >= (1 << 31)):
+            raise ValueError("lattice coordinates beyond int32")
+    _check_spans(A, B, tab)
+    rt = torch.float32 if f32 else torch.float64
+    dT = torch.as_tensor(np.asarray(tab['T'], np.float64)).to('cuda', rt).contiguous()
+    dR = torch.as_tensor(np.atleast_2d(np.asarray(tab['R'], np.float64))[:max(G, 1)]).to('cuda', rt).contiguous()
+    t = PairTab()
+    t.d, t.G, t.f32 = d, G, int(bool(f32))
+    for k in range(d):
+        t.kind[k] = int(tab['kinds'][k])
+        t.grp[k] = int(tab['grp'][k])
+        t.base[k] = int(tab['base'][k])
+        t.J[k] = int(tab['Jk'][k])
+        t.mk[k] = int(tab['mk'][k])
+        t.nbins[k] = int(tab['nbins'][k])
+    t.nT = dT.numel()
+    t.LR = dR.shape[1]
+    t.T = dT.data_ptr()
+    t.RT = dR.data_ptr()
+    dA = torch.from_numpy(A.astype(np.int32)).to('cuda')
+    dB = torch.from_numpy(B.astype(np.int32)).to('cuda')
+    return t, dA, dB, (dT, dR)
+
+
+def rows(ZA, ZB, w, tab, grad=False, f32=False, ranked=False):
+    """(s, ds): s (R,) = sum over j with w_j != 0 of S(ZA_r, ZB_j) w_j (FP64); ds: zeros (R, max(nW, 1))."""
+    if grad:
+        raise NotImplementedError("derivative rows (grad=True) are not built on this engine")
+    torch = _torch()
+    t, dA, dB, keep = _prepare(ZA, ZB, tab, f32, ranked)
+    R, M = dA.shape[0], dB.shape[0]
+    wv = np.ascontiguousarray(w, np.float64).ravel()
+    if len(wv) != M:
+        raise ValueError("w: one weight per row of ZB")
+    dw = torch.from_numpy(wv).to('cuda')
+    s = torch.zeros(max(R, 1), dtype=torch.float64, device='cuda')
+    lib = _lib.load()
+    _lib.check(lib.blml_pair_rows(ctypes.byref(t), ctypes.c_void_p(dA.data_ptr()), ctypes.c_void_p(dB.data_ptr()),
+                                  ctypes.c_void_p(dw.data_ptr()), R, M, ctypes.c_void_p(s.data_ptr()),
+                                  ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    out = s.cpu().numpy()[:R]
+    del keep
+    return out, np.zeros((R, max(int(tab.get('nW', 0)), 1)))
+
+
+def cols(ZA, ZB, tab, out=None, out_m=None, depmask=0, f32=False, ranked=False):
+    """The block S(ZA_r, ZB_j) (R, M) in the tables' precision; with depmask also the marginal over the columns not
+    in depmask (returned as (out, out_m)).  out / out_m: optional host arrays to fill."""
+    torch = _torch()
+    t, dA, dB, keep = _prepare(ZA, ZB, tab, f32, ranked)
+    R, M = dA.shape[0], dB.shape[0]
+    rt = torch.float32 if f32 else torch.float64
+    dO = torch.empty(max(R * M, 1), dtype=rt, device='cuda')
+    dM = torch.empty(max(R * M, 1), dtype=rt, device='cuda') if depmask else None
+    lib = _lib.load()
+    _lib.check(lib.blml_pair_cols(ctypes.byref(t), ctypes.c_void_p(dA.data_ptr()), ctypes.c_void_p(dB.data_ptr()),
+                                  R, M, int(depmask), ctypes.c_void_p(dO.data_ptr()),
+                                  ctypes.c_void_p(dM.data_ptr() if dM is not None else 0),
+                                  ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    res = dO.cpu().numpy()[:R * M].reshape(R, M)
+    if out is not None:
+        out[...] = res
+        res = out
+    del keep
+    if not depmask:
+        return res
+    resm = dM.cpu().numpy()[:R * M].reshape(R, M)
+    if out_m is not None:
+        out_m[...] = resm
+        resm = out_m
+    return res, resm

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "blml_hip.h"
+#include "pairs.hpp"
 #include "solver.hpp"
 
 namespace {
@@ -1214,6 +1215,84 @@ int blml_certificate(void *plan, const void *e_hat, const double *counts, int32_
     else
         cert_kernel<double><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<double>(*pl), (const double *)e_hat,
                                                                          counts, max_iter, tol, lam, iters);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+// ------------------------------------------------------------------------------------------------ pair path
+static int pair_check(const blml_pairtab_t *t) {
+    if (!t) return fail(BLML_EVALUE, "null table");
+    if (t->d < 1 || t->d > BLML_PAIR_MAXD)
+        return fail(BLML_EVALUE, "pair tables: 1.." + std::to_string(BLML_PAIR_MAXD) + " columns on this engine");
+    if (t->G < 0 || t->G > BLML_PAIR_MAXG)
+        return fail(BLML_EVALUE, "pair tables: at most " + std::to_string(BLML_PAIR_MAXG) + " radial groups");
+    if (!t->T || t->nT < 1) return fail(BLML_EVALUE, "pair tables: empty T");
+    if (t->G > 0 && (!t->RT || t->LR < 2)) return fail(BLML_EVALUE, "pair tables: radial rows missing");
+    for (int k = 0; k < t->d; ++k) {
+        const int kd = t->kind[k];
+        if (kd < 0 || kd > 4) return fail(BLML_EVALUE, "pair tables: unknown column kind");
+        if ((kd == 2 || kd == 4) && (t->grp[k] < 0 || t->grp[k] >= t->G))
+            return fail(BLML_EVALUE, "pair tables: a radial column without its group");
+        if ((kd == 1 || kd == 4) && t->nbins[k] < 1) return fail(BLML_EVALUE, "pair tables: circular bins missing");
+    }
+    return BLML_OK;
+}
+
+int blml_pair_rows(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, const double *w, int64_t R,
+                   int64_t M, double *s, void *stream) {
+    int rc = pair_check(tab);
+    if (rc) return rc;
+    if (R < 0 || M < 0 || R >= (1L << 31) || M >= (1L << 31)) return fail(BLML_EVALUE, "R, M: 0 .. 2^31 - 1");
+    if ((R > 0 && (!ZA || !s)) || (M > 0 && (!ZB || !w))) return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (R == 0) return BLML_OK;
+    if (M == 0) {
+        HIPCHECK(hipMemsetAsync(s, 0, sizeof(double) * R, st));
+        return BLML_OK;
+    }
+    const size_t es = tab->f32 ? 4 : 8;
+    const bool tl = tab->nT * (long)es <= 32768;
+    const long tbytes = tl ? ((tab->nT * (long)es + 15) / 16) * 16 : 0;
+    const int lds = (int)(tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16 + BLML_PAIR_CH * 8);
+    const long rb = (R + BLML_PAIR_NT - 1) / BLML_PAIR_NT;
+    long nsl = std::max<long>(1, std::min<long>((2048 + rb - 1) / rb, (M + BLML_PAIR_CH - 1) / BLML_PAIR_CH));
+    long slice = ((M + nsl - 1) / nsl + BLML_PAIR_CH - 1) / BLML_PAIR_CH * BLML_PAIR_CH;
+    nsl = (M + slice - 1) / slice;
+    double *part = nullptr;
+    HIPCHECK(hipMallocAsync((void **)&part, sizeof(double) * nsl * R, st));
+    const dim3 grid((unsigned)rb, (unsigned)nsl);
+    const blml_pairtab_t t = *tab;
+    if (tab->f32) {
+        if (tl) pair_rows_kernel<float, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+        else pair_rows_kernel<float, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+    } else {
+        if (tl) pair_rows_kernel<double, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+        else pair_rows_kernel<double, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+    }
+    HIPCHECK(hipGetLastError());
+    pair_rows_sum_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, s);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipFreeAsync(part, st));
+    return BLML_OK;
+}
+
+int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, int64_t R, int64_t M,
+                   int64_t depmask, void *out, void *out_m, void *stream) {
+    int rc = pair_check(tab);
+    if (rc) return rc;
+    if (R < 0 || M < 0) return fail(BLML_EVALUE, "R, M must be >= 0");
+    if (R == 0 || M == 0) return BLML_OK;
+    if (!ZA || !ZB || !out || (depmask != 0 && !out_m)) return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const long n = R * M;
+    const unsigned blocks = (unsigned)std::min<long>((n + BLML_PAIR_NT - 1) / BLML_PAIR_NT, 16384);
+    const blml_pairtab_t t = *tab;
+    if (tab->f32)
+        pair_cols_kernel<float><<<blocks, BLML_PAIR_NT, 0, st>>>(t, ZA, ZB, R, M, depmask, (float *)out,
+                                                                 (float *)(out_m ? out_m : out));
+    else
+        pair_cols_kernel<double><<<blocks, BLML_PAIR_NT, 0, st>>>(t, ZA, ZB, R, M, depmask, (double *)out,
+                                                                  (double *)(out_m ? out_m : out));
     HIPCHECK(hipGetLastError());
     return BLML_OK;
 }

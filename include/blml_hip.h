@@ -132,6 +132,42 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
 int blml_certificate(void *plan, const void *e_hat, const double *counts, int32_t max_iter, double tol,
                      double *lam, int32_t *iters, void *stream);
 
+/* ---------------------------------------------------------------------------------------------------------------
+ * Pair accumulation over lattice bins (the reference's blml_local_gpu.rows / .cols, blml_local_gpu.py:42-105,
+ * 320-361, and their CPU twins blml_local.py:203-282): the kernel value of a pair of bins is the product of its
+ * columns' table entries -- an offset row (kind 0: T[base + za - zb + J]), a circular row (1: the min circular
+ * distance), a rank table (3: T[base + za * mk + zb], ranks as coordinates), a radial group's row on the r grid
+ * (2: linear interpolation at sqrt(sum off^2) * 4) or a periodic ball's torus table (4: the row-major circular
+ * offset) -- the layout of blml_kernels.ProductKernel.tables (blml_kernels.py:133-200).  Coordinates are int32
+ * (ranks for rank columns); tables REAL = float (f32 = 1) or double.  Sums are FP64. */
+#define BLML_PAIR_MAXD 8      /* columns */
+#define BLML_PAIR_MAXG 8      /* radial / torus groups */
+
+typedef struct {
+    int32_t d, G, f32, reserved;
+    int32_t kind[BLML_PAIR_MAXD];
+    int32_t grp[BLML_PAIR_MAXD];     /* group of a radial / torus column, else -1 */
+    int64_t base[BLML_PAIR_MAXD];
+    int64_t J[BLML_PAIR_MAXD];       /* offset rows: half width */
+    int64_t mk[BLML_PAIR_MAXD];      /* rank tables: values per side */
+    int64_t nbins[BLML_PAIR_MAXD];   /* circular rows and torus columns: bins per period */
+    int64_t nT;                      /* values of T */
+    int64_t LR;                      /* values per row of RT */
+    const void *T;                   /* device: the columns' rows, flat */
+    const void *RT;                  /* device: G rows of LR values (radial rows / torus tables) */
+} blml_pairtab_t;
+
+/* s_r = sum_j S(ZA_r, ZB_j) w_j (pairs with w_j == 0 skipped), FP64, in a fixed order (deterministic).  ZA (R x d),
+ * ZB (M x d) int32, w (M) double, s (R) double: device.  Coordinates must lie inside the tables' spans (checked on
+ * the host side of the binding; an index outside its table is clamped on the device and reported). */
+int blml_pair_rows(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, const double *w, int64_t R,
+                   int64_t M, double *s, void *stream);
+
+/* The explicit block out[r * M + j] = S(ZA_r, ZB_j) (REAL); with depmask != 0 also out_m = the product over the
+ * columns not in depmask (the reference's joint + marginal pass). */
+int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, int64_t R, int64_t M,
+                   int64_t depmask, void *out, void *out_m, void *stream);
+
 /* Sizes the plan chose (for the caller's buffers and for reporting).  grid: workgroups of the batched launch, or
  * minus the workgroup count when the plan's one fit is spread over the device. */
 int blml_plan_info(void *plan, int64_t *out_elems, int64_t *ws_bytes, int32_t *lds_bytes, int32_t *grid);

@@ -1,0 +1,191 @@
+"""CPU FP64 restatement of the reference's pair accumulation -- TEST INFRASTRUCTURE ONLY (the checker of the HIP pair
+kernels; imported by tests and the benchmark's CPU baseline, never by the product).
+
+Restated (numpy, float64):
+  rank_coords(Z, tab)        blml_kernels.py:203-217 (rank columns -> ranks among the table's values; stale -> error)
+  pair_rows(ZA, ZB, w, tab)  blml_local.py:203-208 with _rows_numba's pair loop (blml_local.py:230-253, no
+                             derivatives): s_r = sum_{j: w_j != 0} prod_u fac_u(r, j) w_j, the unit factors from
+                             _col_index (blml_local.py:86-96), radial rows by linear interpolation on the r grid
+                             (RADIAL_Q = 4, blml_trivial.py:218), torus tables at the row-major circular offset
+  pair_cols(ZA, ZB, tab, depmask)   blml_local.py:271-282 / blml_local_gpu.py:331-361 (joint and marginal)
+Table builders for the tests (the rows ProductKernel.tables holds, blml_kernels.py:133-200, for the kinds with
+closed forms): offset rows K(o h), o = -J..J (sinc, slits, unfolded dirichlet); circular rows K(m h), m = 0..nb-1
+(folded dirichlet); rank tables K((v_a - v_b) h) over the occupied values; a periodic ball's torus table over its
+columns' circular offsets.
+
+Pinning (tests/test_oracle_pairs.py): pair_rows / pair_cols against the dense Gram gram_int @ w
+(blml_kernels.py:243-251; the reference's own check, results/pball_parity.py:66-76) for tables built from the
+kernels' closed forms; the lookup of radial rows against a brute-force per-pair loop of the same rule."""
+import numpy as np
+
+from oracle.blml_oracle import kernel_eval
+
+RADIAL_Q = 4
+KIND = dict(offset=0, circular=1, radial=2, rank=3, torus=4)
+
+
+def rank_coords(Z, tab):
+    """Coordinates as the row kernels read them: a rank column's value replaced by its rank in tab['vals'][k]."""
+    Z = np.array(Z, np.int64, copy=True)
+    for k, v in enumerate(tab['vals']):
+        if v is None:
+            continue
+        v = np.asarray(v, np.int64)
+        r = np.minimum(np.searchsorted(v, Z[:, k]), len(v) - 1)
+        if len(Z) and not np.array_equal(v[r], Z[:, k]):
+            raise RuntimeError(f"rank column {k}: a bin value is not in the kernel table (stale tables)")
+        Z[:, k] = r
+    return Z
+
+
+def _factors(A, B, tab):
+    """Per-unit factor arrays (U, nA, nB) of every pair: column units from T, groups from R."""
+    A = np.asarray(A, np.int64)
+    B = np.asarray(B, np.int64)
+    d = A.shape[1]
+    kinds, base, Jk, mk, nbins, grp = (np.asarray(tab[k]) for k in ('kinds', 'base', 'Jk', 'mk', 'nbins', 'grp'))
+    T = np.asarray(tab['T'], float)
+    Rt = np.atleast_2d(np.asarray(tab['R'], float))
+    U = int(tab['U'])
+    fac = np.ones((U, len(A), len(B)))
+    G = int((grp >= 0).any() and Rt.shape[0])
+    r2 = np.zeros((max(G, 1), len(A), len(B)), np.int64)
+    tor = np.zeros(max(G, 1), bool)
+    for k in range(d):
+        za, zb = A[:, k][:, None], B[:, k][None, :]
+        off = za - zb
+        kd = int(kinds[k])
+        if kd == 2:
+            r2[grp[k]] += off * off
+        elif kd == 4:
+            nb = int(nbins[k])
+            r2[grp[k]] = r2[grp[k]] * nb + np.mod(off, nb)
+            tor[grp[k]] = True
+        else:
+            if kd == 0:
+                idx = base[k] + off + Jk[k]
+            elif kd == 3:
+                idx = base[k] + za * mk[k] + zb
+            else:
+                nb = int(nbins[k])
+                m = np.mod(off, nb)
+                idx = base[k] + np.minimum(m, nb - m)
+            fac[int(tab['dim_unit'][k])] = T[idx]
+    for g in range(G):
+        u = int(tab['grp_unit'][g])
+        if tor[g]:
+            fac[u] = Rt[g][r2[g]]
+        else:
+            t = np.sqrt(r2[g] * 1.0) * RADIAL_Q
+            i0 = t.astype(np.int64)
+            f = t - i0
+            fac[u] = Rt[g][i0] * (1.0 - f) + Rt[g][i0 + 1] * f
+    return fac
+
+
+def pair_rows(ZA, ZB, w, tab, chunk=512):
+    """s (R,): sum over sources with w_j != 0 of the unit-order product times w_j (float64)."""
+    A, B = rank_coords(ZA, tab), rank_coords(ZB, tab)
+    w = np.asarray(w, float)
+    keep = w != 0
+    B, w = B[keep], w[keep]
+    s = np.zeros(len(A))
+    for a0 in range(0, len(A), chunk):
+        fac = _factors(A[a0:a0 + chunk], B, tab)
+        prod = np.ones(fac.shape[1:])
+        for u in range(fac.shape[0]):
+            prod = prod * fac[u]
+        s[a0:a0 + chunk] = prod @ w
+    return s
+
+
+def pair_cols(ZA, ZB, tab, depmask=0):
+    """(out, out_m): the block S(ZA_r, ZB_j) and, with depmask, the product over the columns not in depmask (a
+    group on the dep side when any of its columns is) -- column order, then the groups (blml_local_gpu.py:341-357)."""
+    A, B = rank_coords(ZA, tab), rank_coords(ZB, tab)
+    fac = _factors(A, B, tab)
+    d = A.shape[1]
+    dep_unit = np.zeros(fac.shape[0], bool)
+    grp = np.asarray(tab['grp'])
+    for k in range(d):
+        if (depmask >> k) & 1:
+            dep_unit[int(tab['dim_unit'][k]) if grp[k] < 0 else int(tab['grp_unit'][grp[k]])] = True
+    v = np.ones(fac.shape[1:])
+    vd = np.ones(fac.shape[1:])
+    for u in range(fac.shape[0]):
+        if dep_unit[u]:
+            vd = vd * fac[u]
+        else:
+            v = v * fac[u]
+    return v * vd, v
+
+
+# ------------------------------------------------------------------------------------------------ table builders
+def build_tables(units, h, J, vals=None):
+    """A table dict in ProductKernel.tables' layout for per-column units and periodic-ball groups.
+    units: list of (code, kind, fc_cols, per_cols, dims): code 'offset' | 'circular' | 'rank' | 'torus', kind the
+    kernel_eval kind string ('sinc', 'slits:0.5', 'dirichlet', 'pball' ...), fc_cols / per_cols its cut-offs and
+    periods (0.0 where none), dims its columns; h: per-column step; J: per-column half width of offset rows; vals:
+    per-column occupied values of rank columns (else None)."""
+    d = len(h)
+    kinds = np.zeros(d, np.int64)
+    base = np.zeros(d, np.int64)
+    Jk = np.zeros(d, np.int64)
+    mk = np.zeros(d, np.int64)
+    nbins = np.zeros(d, np.int64)
+    grp = np.full(d, -1, np.int64)
+    dim_unit = np.full(d, -1, np.int64)
+    grp_unit = []
+    rows, rrows = [], []
+    rvals = [None] * d
+    at = 0
+    for ui, (code, kind, fc_cols, per_cols, dims) in enumerate(units):
+        if code == 'torus':
+            g = len(rrows)
+            P = np.asarray(per_cols, float)
+            ms = [int(round(p / h[k])) for p, k in zip(P, dims)]
+            offs = np.stack([g_.ravel() for g_ in np.meshgrid(*[np.arange(m) for m in ms], indexing='ij')], 1)
+            diffs = [offs[:, i] * h[k] for i, k in enumerate(dims)]
+            rrows.append(kernel_eval(kind, np.asarray(fc_cols, float), P, diffs))
+            grp[list(dims)] = g
+            kinds[list(dims)] = KIND['torus']
+            nbins[list(dims)] = ms
+            grp_unit.append(ui)
+            continue
+        k = dims[0]
+        dim_unit[k] = ui
+        if code == 'offset':
+            row = kernel_eval(kind, np.asarray(fc_cols, float), np.asarray(per_cols, float),
+                              [np.arange(-J[k], J[k] + 1) * h[k]])
+            Jk[k] = J[k]
+        elif code == 'circular':
+            nb = int(round(float(per_cols[0]) / h[k]))
+            row = kernel_eval(kind, np.asarray(fc_cols, float), np.asarray(per_cols, float), [np.arange(nb) * h[k]])
+            nbins[k] = nb
+        else:
+            v = np.asarray(vals[k], np.int64)
+            row = kernel_eval(kind, np.asarray(fc_cols, float), np.asarray(per_cols, float),
+                              [((v[:, None] - v[None, :]).ravel() * h[k]).astype(float)])
+            mk[k] = len(v)
+            rvals[k] = v
+        kinds[k] = KIND[code]
+        base[k] = at
+        rows.append(np.asarray(row, float))
+        at += len(row)
+    T = np.concatenate(rows) if rows else np.zeros(1)
+    LR = max([len(r) for r in rrows] or [2])
+    R = np.zeros((max(len(rrows), 1), LR))
+    for g, r in enumerate(rrows):
+        R[g, :len(r)] = r
+    return dict(T=T, base=base, Jk=Jk, mk=mk, vals=rvals, kinds=kinds, nbins=nbins, grp=grp, R=R,
+                dim_unit=dim_unit, grp_unit=np.asarray(grp_unit or [0], np.int64), U=len(units), nW=0)
+
+
+def dense_pair_gram(units, h, ZA, ZB):
+    """gram_int (blml_kernels.py:243-251): prod over units of K(diffs) at lattice differences times h."""
+    ZA, ZB = np.atleast_2d(ZA), np.atleast_2d(ZB)
+    G = np.ones((len(ZA), len(ZB)))
+    for code, kind, fc_cols, per_cols, dims in units:
+        diffs = [(ZA[:, k, None] - ZB[None, :, k]) * h[k] for k in dims]
+        G *= kernel_eval(kind, np.asarray(fc_cols, float), np.asarray(per_cols, float), diffs)
+    return G

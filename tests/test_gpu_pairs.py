@@ -1,0 +1,83 @@
+"""The HIP pair kernels (blml_pair_rows / blml_pair_cols through blml_amd.local) against the oracle's restatement of
+the reference's rows / cols (oracle/blml_pairs.py), on the same seeded inputs: FP64 tables to 1e-12 of the rows'
+absolute scale (sum_j |S_rj w_j|), FP32 tables to 1e-5 of it (FP32 products, FP64 sums); cols elementwise 1e-13
+(FP64) / 1e-6 (FP32) of max |S|."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import blml_pairs as OP
+from test_oracle_pairs import CASES, _bins
+
+pytestmark = pytest.mark.gpu
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _radial_tab(rng):
+    LR = 80
+    return dict(T=rng.standard_normal(30), base=np.array([0, 0, 0]), Jk=np.array([0, 0, 14]), mk=np.zeros(3, int),
+                vals=[None, None, None], kinds=np.array([2, 2, 0]), nbins=np.zeros(3, int), grp=np.array([0, 0, -1]),
+                R=rng.standard_normal(LR)[None], dim_unit=np.array([-1, -1, 1]), grp_unit=np.array([0]), U=2, nW=0)
+
+
+def _all_cases(rng):
+    out = []
+    for name, shape, units, h, J in CASES:
+        out.append((name, shape, OP.build_tables(units, np.asarray(h, float), J), 60, 90))
+    Z = _bins(rng, (40, 6), 25)
+    out.append(('rank x offset', (40, 6), OP.build_tables(
+        [('rank', 'sinc', [0.2], [0.0], [0]), ('offset', 'sinc', [0.5], [0.0], [1])], np.ones(2), (40, 6),
+        vals=[np.unique(Z[:, 0]), None]), None, None))
+    out.append(('radial random rows', (8, 8, 8), _radial_tab(rng), 40, 50))
+    # enough rows and sources for several slices, chunk tails and a table too big for LDS (float64, > 32 KB)
+    out.append(('3d slits large', (64, 64, 64), OP.build_tables(
+        [('offset', 'slits:0.5', [0.2], [0.0], [k]) for k in range(3)], np.ones(3), (1500, 1500, 1500)), 3000,
+        20000))
+    return out, Z
+
+
+@pytest.mark.parametrize('f32', [False, True], ids=['f64', 'f32'])
+def test_rows_and_cols_match_oracle(f32):
+    from blml_amd import local
+    rng = np.random.default_rng(21)
+    cases, Zrank = _all_cases(rng)
+    for name, shape, tab, nA, nB in cases:
+        _note(f'pairs {name} f32={f32}')
+        if nA is None:
+            ZA = ZB = Zrank
+        else:
+            ZA, ZB = _bins(rng, shape, nA), _bins(rng, shape, nB)
+        w = rng.standard_normal(len(ZB))
+        w[::5] = 0.0
+        want = OP.pair_rows(ZA, ZB, w, tab)
+        out_o, _ = OP.pair_cols(ZA[:50], ZB[:70], tab)
+        scale = (np.abs(out_o).max() if out_o.size else 1.0)
+        s, ds = local.rows(ZA, ZB, w, tab, f32=f32)
+        absrow = np.abs(OP.pair_cols(ZA[:20], ZB, tab)[0]) @ np.abs(w)
+        tol = (1e-5 if f32 else 1e-12) * max(float(np.max(absrow, initial=0.0)), 1e-300)
+        assert s.shape == (len(ZA),) and ds.shape[0] == len(ZA)
+        assert np.abs(s - want).max() <= tol * 10, (name, np.abs(s - want).max(), tol)
+        out = local.cols(ZA[:50], ZB[:70], tab, f32=f32)
+        assert np.abs(out - out_o).max() <= (1e-6 if f32 else 1e-13) * scale, name
+
+
+def test_cols_marginal_and_errors():
+    from blml_amd import local
+    rng = np.random.default_rng(22)
+    name, shape, units, h, J = CASES[4]
+    tab = OP.build_tables(units, np.asarray(h, float), J)
+    ZA, ZB = _bins(rng, shape, 40), _bins(rng, shape, 50)
+    out, marg = local.cols(ZA, ZB, tab, depmask=0b010)
+    o_out, o_marg = OP.pair_cols(ZA, ZB, tab, depmask=0b010)
+    assert np.abs(out - o_out).max() < 1e-13 and np.abs(marg - o_marg).max() < 1e-13
+    bad = ZA.copy()
+    bad[0, 0] = 100                        # beyond the offset row's half width: the tables predate this bin
+    with pytest.raises(RuntimeError):
+        local.rows(bad, ZB, np.ones(len(ZB)), tab)
+    with pytest.raises(NotImplementedError):
+        local.rows(ZA, ZB, np.ones(len(ZB)), tab, grad=True)

@@ -1,0 +1,86 @@
+"""The CPU restatement of the reference's pair accumulation (oracle/blml_pairs.py), pinned without the reference:
+rows and cols over tables built from the kernels' closed forms equal the dense Gram gram_int @ w
+(blml_kernels.py:243-251; results/pball_parity.py:66-76) to 1e-12; radial rows follow the interpolation rule of
+blml_local.py:250-252 (checked against a per-pair loop); rank columns reject values their table lacks."""
+import numpy as np
+import pytest
+
+from oracle import blml_pairs as OP
+
+# (name, shape of the lattice box, units, h, J) -- units as blml_pairs.build_tables takes them
+CASES = [
+    ('2d sinc x slits offset rows', (14, 11),
+     [('offset', 'sinc', [0.4], [0.0], [0]), ('offset', 'slits:0.5', [0.3], [0.0], [1])], (1.0, 1.0), (14, 11)),
+    ('1d dirichlet circular row', (16,), [('circular', 'dirichlet', [0.3], [16.0], [0])], (1.0,), (16,)),
+    ('2d dirichlet unfolded x sinc h', (9, 12),
+     [('offset', 'dirichlet', [0.5], [20.0], [0]), ('offset', 'sinc', [1.2], [0.0], [1])], (1.0, 0.25), (9, 12)),
+    ('2d pball torus', (10, 8), [('torus', 'pball', [0.45], [10.0, 8.0], [0, 1])], (1.0, 1.0), (10, 8)),
+    ('3d slits x dirichlet circular x sinc', (7, 10, 6),
+     [('offset', 'slits:0.4', [0.5], [0.0], [0]), ('circular', 'dirichlet:0.5', [0.6], [10.0], [1]),
+      ('offset', 'sinc', [0.3], [0.0], [2])], (1.0, 1.0, 1.0), (7, 10, 6)),
+]
+
+
+def _bins(rng, shape, n):
+    Z = np.stack([rng.integers(0, m, n) for m in shape], 1)
+    return np.unique(Z, axis=0)
+
+
+@pytest.mark.parametrize('name,shape,units,h,J', CASES, ids=[c[0] for c in CASES])
+def test_rows_and_cols_equal_dense_gram(name, shape, units, h, J):
+    rng = np.random.default_rng(3)
+    tab = OP.build_tables(units, np.asarray(h, float), J)
+    ZA, ZB = _bins(rng, shape, 60), _bins(rng, shape, 90)
+    w = rng.standard_normal(len(ZB))
+    w[::7] = 0.0
+    G = OP.dense_pair_gram(units, np.asarray(h, float), ZA, ZB)
+    s = OP.pair_rows(ZA, ZB, w, tab)
+    assert np.abs(s - G @ w).max() < 1e-12 * max(1.0, np.abs(G).sum(1).max()), name
+    out, _ = OP.pair_cols(ZA, ZB, tab)
+    assert np.abs(out - G).max() < 1e-13 * max(1.0, np.abs(G).max()), name
+
+
+def test_cols_marginal_is_the_other_columns():
+    units = CASES[4][2]
+    h = np.ones(3)
+    tab = OP.build_tables(units, h, CASES[4][4])
+    rng = np.random.default_rng(4)
+    ZA, ZB = _bins(rng, (7, 10, 6), 30), _bins(rng, (7, 10, 6), 40)
+    out, marg = OP.pair_cols(ZA, ZB, tab, depmask=0b010)
+    want = OP.dense_pair_gram([units[0], units[2]], h, ZA, ZB)
+    assert np.abs(marg - want).max() < 1e-13 and np.abs(out - OP.dense_pair_gram(units, h, ZA, ZB)).max() < 1e-13
+
+
+def test_rank_table_equals_offset_row():
+    """A rank column (the occupied values' pairs) gives the same kernel as the offset row it replaces."""
+    rng = np.random.default_rng(5)
+    Z = _bins(rng, (40, 6), 25)
+    vals = [np.unique(Z[:, 0]), None]
+    units = [('rank', 'sinc', [0.2], [0.0], [0]), ('offset', 'sinc', [0.5], [0.0], [1])]
+    tab = OP.build_tables(units, np.ones(2), (40, 6), vals=vals)
+    w = rng.standard_normal(len(Z))
+    G = OP.dense_pair_gram(units, np.ones(2), Z, Z)
+    assert np.abs(OP.pair_rows(Z, Z, w, tab) - G @ w).max() < 1e-12
+    with pytest.raises(RuntimeError):
+        OP.rank_coords(np.array([[41, 0]]), tab)
+
+
+def test_radial_rows_interpolate_per_pair():
+    """Radial rows: the value at sqrt(sum off^2) * 4 on the row, linear between grid points (random table)."""
+    rng = np.random.default_rng(6)
+    LR = 80
+    row = rng.standard_normal(LR)
+    tab = dict(T=rng.standard_normal(30), base=np.array([0, 0, 0]), Jk=np.array([0, 0, 14]), mk=np.zeros(3, int),
+               vals=[None, None, None], kinds=np.array([2, 2, 0]), nbins=np.zeros(3, int), grp=np.array([0, 0, -1]),
+               R=row[None], dim_unit=np.array([-1, -1, 1]), grp_unit=np.array([0]), U=2, nW=0)
+    ZA, ZB = _bins(rng, (8, 8, 8), 12), _bins(rng, (8, 8, 8), 15)
+    w = rng.standard_normal(len(ZB))
+    want = np.zeros(len(ZA))
+    for r in range(len(ZA)):
+        for j in range(len(ZB)):
+            o = ZA[r] - ZB[j]
+            t = np.sqrt(float(o[0] ** 2 + o[1] ** 2)) * 4
+            i0 = int(t)
+            f = t - i0
+            want[r] += (row[i0] * (1 - f) + row[i0 + 1] * f) * tab['T'][o[2] + 14] * w[j]
+    assert np.abs(OP.pair_rows(ZA, ZB, w, tab) - want).max() < 1e-12
