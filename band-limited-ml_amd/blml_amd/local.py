@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ['rows', 'cols', 'rank_coords']
+__all__ = ['rows', 'cols', 'rank_coords', 'lattice_tables']
 
 MAXD, MAXG = 8, 8
 RADIAL_Q = 4
@@ -49,6 +49,78 @@ def rank_coords(Z, tab):
             raise RuntimeError(f"rank column {k}: a bin value is not in the kernel table (stale tables)")
         Z[:, k] = r
     return Z
+
+
+def lattice_tables(kinds, fc, periods, h, Zi):
+    """Value tables in the layout of the reference's ProductKernel.tables (blml_kernels.py:133-200) for the bins Zi
+    (N, d) and per-column kinds: sinc / slits columns get an offset row K(o h), o = -J..J, J = ceil(1.25 span)
+    (the reference's 25% headroom), or a rank table over their occupied values when that is smaller
+    (len(vals)^2 < 2 J + 1, blml_trivial.py:405-413); dirichlet columns (folded on their period) a circular row
+    K(m h), m = 0..P/h - 1; a periodic ball over its columns (pball) its torus table over the columns' circular
+    offsets (row-major).  Radial balls are not built here."""
+    from .modes import split_kind
+    from .units import one_column_values, multi_values
+    kinds = list(kinds)
+    d = len(kinds)
+    fc = np.broadcast_to(np.asarray(fc, float), (d,))
+    periods = list(periods) if periods is not None else [None] * d
+    h = np.broadcast_to(np.asarray(h, float), (d,))
+    Zi = np.asarray(Zi, np.int64)
+    span = np.maximum(Zi.max(0) - Zi.min(0), 1) if len(Zi) else np.ones(d, np.int64)
+    alloc = np.ceil(span * 1.25).astype(np.int64)
+    kd_arr = np.zeros(d, np.int64)
+    base = np.zeros(d, np.int64)
+    Jk = np.zeros(d, np.int64)
+    mk = np.zeros(d, np.int64)
+    nbins = np.zeros(d, np.int64)
+    grp = np.full(d, -1, np.int64)
+    dim_unit = np.full(d, -1, np.int64)
+    vals = [None] * d
+    rows, rrows, grp_unit = [], [], []
+    single = [k for k in range(d) if split_kind(kinds[k])[0] != 'pball']
+    pcols = [k for k in range(d) if split_kind(kinds[k])[0] == 'pball']
+    at = 0
+    for ui, k in enumerate(single):
+        name = split_kind(kinds[k])[0]
+        dim_unit[k] = ui
+        if name in ('sinc', 'slits'):
+            J = int(alloc[k])
+            occ = np.unique(Zi[:, k])
+            if len(occ) ** 2 < 2 * J + 1:
+                row = one_column_values(kinds[k], fc[k], None, (occ[:, None] - occ[None, :]).ravel() * h[k])
+                kd_arr[k], mk[k], vals[k] = 3, len(occ), occ
+            else:
+                row = one_column_values(kinds[k], fc[k], None, np.arange(-J, J + 1) * h[k])
+                kd_arr[k], Jk[k] = 0, J
+        elif name == 'dirichlet':
+            nb = int(round(float(periods[k]) / h[k]))
+            if abs(nb * h[k] - float(periods[k])) > 1e-9 * float(periods[k]):
+                raise ValueError(f"column {k}: the period is not a whole number of bins")
+            row = one_column_values(kinds[k], fc[k], periods[k], np.arange(nb) * h[k])
+            kd_arr[k], nbins[k] = 1, nb
+        else:
+            raise NotImplementedError(f"lattice_tables: kind {kinds[k]!r} (radial rows are not built here)")
+        base[k] = at
+        rows.append(np.asarray(row, float))
+        at += len(row)
+    if pcols:
+        P = [float(periods[k]) for k in pcols]
+        ms = [int(round(p / h[k])) for p, k in zip(P, pcols)]
+        offs = np.stack([g.ravel() for g in np.meshgrid(*[np.arange(m) for m in ms], indexing='ij')], 1)
+        rrows.append(multi_values('pball', [fc[pcols[0]]] * len(pcols), P,
+                                  [offs[:, i] * h[k] for i, k in enumerate(pcols)]))
+        grp[pcols] = 0
+        kd_arr[pcols] = 4
+        nbins[pcols] = ms
+        grp_unit.append(len(single))
+    T = np.concatenate(rows) if rows else np.zeros(1)
+    LR = max([len(r) for r in rrows] or [2])
+    R = np.zeros((max(len(rrows), 1), LR))
+    for g, r in enumerate(rrows):
+        R[g, :len(r)] = r
+    return dict(T=T, base=base, Jk=Jk, mk=mk, vals=vals, kinds=kd_arr, nbins=nbins, grp=grp, R=R,
+                dim_unit=dim_unit, grp_unit=np.asarray(grp_unit or [0], np.int64), U=len(single) + (1 if pcols else 0),
+                nW=0)
 
 
 def _groups(tab):

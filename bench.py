@@ -13,8 +13,9 @@ A step = one launch that fits all 2048 fits from their cold start.  value = fits
 it: the plain FP64 method on the same GPU, the fraction of fits whose FP32 loglik is within 1e-6 of the FP64 run's
 and the fraction that is a local maximum to 1e-6 by the reference's polish test; the FP64 oracle on the host's
 cores (cpu_baseline); the large single fit (32^5 bins, product sinc, 1e7 draws of a 4-component Gaussian mixture,
-spread over every CU) in both precisions.  roofline: the Gram mat-vec of the large fit (the memory-bound case);
-roofline_batched: the batched mat-vec of the 2048 fits.  Prints one JSON line."""
+spread over every CU) in both precisions; the pair path's rows over the occupied bins of a 128^3 lattice (pairs/s).
+roofline: the Gram mat-vec of the large fit (the memory-bound case); roofline_batched: the batched mat-vec of the
+2048 fits.  Prints one JSON line."""
 import argparse
 import json
 import math
@@ -120,6 +121,25 @@ def large_fit_counts(n=10_000_000, seed=2):
     counts = np.bincount(flat, minlength=nb ** d).astype(float).reshape(LARGE_SHAPE)
     groups = [dict(fc=[LARGE_FC] * d, kinds=['sinc'] * d)]
     return counts, groups, dict(h=LARGE_H, origin=-4.0)
+
+
+PAIR_BINS = 128
+
+
+def pair_companion(n=2_000_000, seed=2):
+    """The pair path's lattice: n draws of the large fit's mixture in 3-D (means U[-2, 2]^3, sds U[0.3, 0.8], 4 equal
+    components, numpy default_rng(seed)) binned on 128^3 bins over [-4, 4]^3; returns the occupied bins' integer
+    coordinates (B, 3), their counts and the step."""
+    rng = np.random.default_rng(seed)
+    d, h = 3, 8.0 / PAIR_BINS
+    means = rng.uniform(-2.0, 2.0, (4, d))
+    sds = rng.uniform(0.3, 0.8, (4, d))
+    comp = rng.integers(0, 4, n)
+    x = means[comp] + sds[comp] * rng.standard_normal((n, d))
+    idx = np.floor((x + 4.0) / h).astype(np.int64)
+    idx = idx[np.all((idx >= 0) & (idx < PAIR_BINS), axis=1)]
+    Z, cnt = np.unique(idx, axis=0, return_counts=True)
+    return Z, cnt.astype(float), h
 
 
 def modes_of(groups, lib):
@@ -274,7 +294,34 @@ def main():
     del cp_src, cp_dst
     print(f'bench: large fit fp32 {large["fp32_seconds"]:.2f} s; mat-vec {t_l * 1e6:.1f} us', file=sys.stderr,
           flush=True)
+    # the pair path: rows s_r = sum_j S(z_r, z_j) w_j over the companion's occupied bins (slits per column),
+    # FP32 tables and products, FP64 sums; a row subset against every source
+    import ctypes
+    from blml_amd import _lib, local
+    Zp, wp, hp = pair_companion()
+    tabp = local.lattice_tables(['slits:0.5'] * 3, [0.2 / hp] * 3, None, [hp] * 3, Zp)
+    n_rows = min(len(Zp), 1 << 16)
+    pt, dA, dB, keep = local._prepare(Zp[:n_rows], Zp, tabp, True, False)
+    dw = torch.from_numpy(np.ascontiguousarray(wp)).to('cuda')
+    ds = torch.zeros(n_rows, dtype=torch.float64, device='cuda')
+    lib = _lib.load()
+    call = lambda: _lib.check(lib.blml_pair_rows(ctypes.byref(pt), ctypes.c_void_p(dA.data_ptr()),
+                                                 ctypes.c_void_p(dB.data_ptr()), ctypes.c_void_p(dw.data_ptr()),
+                                                 n_rows, len(Zp), ctypes.c_void_p(ds.data_ptr()),
+                                                 ctypes.c_void_p(st.cuda_stream)))
+    call()
+    t_p = event_time(call, 3)
+    pairs = dict(bins=int(len(Zp)), rows=int(n_rows), sources=int(len(Zp)), draws=2_000_000, lattice=[PAIR_BINS] * 3,
+                 kernel='slits fill 0.5, fc 0.2 cycles/bin per column (offset rows)', precision='f32 tables, f64 sums',
+                 seconds=t_p, pairs_per_s=n_rows * len(Zp) / t_p)
+    del keep
+    print(f'bench: pair rows {n_rows} x {len(Zp)} in {t_p * 1e3:.1f} ms', file=sys.stderr, flush=True)
     cpu = cpu_baseline(fits, args.cpu_seconds)
+    from oracle import blml_pairs as OP
+    t0 = time.perf_counter()
+    OP.pair_rows(Zp[:64], Zp, wp, tabp)
+    pairs['cpu_pairs_per_s'] = 64 * len(Zp) / (time.perf_counter() - t0)
+    pairs['cpu_sample'] = f'64 rows x {len(Zp)} sources, the FP64 numpy restatement, one process'
     ms = dt32 / args.steps * 1e3
     line = dict(metric='fits/sec at loglik tol 1e-6 (warm-start FP32 vs plain FP64) + mat-vec HBM GB/s',
                 value=len(fits) / (dt32 / args.steps), unit='fits/s', n_gpus=1, steps=args.steps,
@@ -282,7 +329,7 @@ def main():
                 dtype='f32 transforms, f64 reductions', data='synthetic',
                 config=dict(workload=f'{len(fits)} uneven 2-D binned fits (pball / slits + background, '
                                      'L-BFGS, cold start)', fits=len(fits), solver=SOLVER),
-                roofline=roof, cpu_baseline=cpu, roofline_batched=roof_batched, large_fit=large, **extra)
+                roofline=roof, cpu_baseline=cpu, roofline_batched=roof_batched, large_fit=large, pairs=pairs, **extra)
     print(json.dumps(line), flush=True)
 
 

@@ -84,3 +84,24 @@ def test_radial_rows_interpolate_per_pair():
             f = t - i0
             want[r] += (row[i0] * (1 - f) + row[i0 + 1] * f) * tab['T'][o[2] + 14] * w[j]
     assert np.abs(OP.pair_rows(ZA, ZB, w, tab) - want).max() < 1e-12
+
+
+def test_product_tables_match_oracle_tables():
+    """blml_amd.local.lattice_tables (the product's builder) gives the rows the oracle's closed-form builder gives,
+    so rows over either equal the dense Gram."""
+    from blml_amd import local
+    rng = np.random.default_rng(8)
+    Z = _bins(rng, (30, 12, 16), 200)
+    kinds = ['slits:0.5', 'sinc', 'dirichlet']
+    fc, per, h = [0.3, 0.45, 0.4], [None, None, 16.0], [1.0, 1.0, 1.0]
+    tab = local.lattice_tables(kinds, fc, per, h, Z)
+    units = [('offset', 'slits:0.5', [0.3], [0.0], [0]), ('offset', 'sinc', [0.45], [0.0], [1]),
+             ('circular', 'dirichlet', [0.4], [16.0], [2])]
+    w = rng.standard_normal(len(Z))
+    G = OP.dense_pair_gram(units, np.asarray(h), Z, Z)
+    assert np.abs(OP.pair_rows(Z, Z, w, tab) - G @ w).max() < 1e-12 * np.abs(G).sum(1).max()
+    Zp = _bins(rng, (10, 8), 50)
+    tabp = local.lattice_tables(['pball', 'pball'], [0.45, 0.45], [10.0, 8.0], [1.0, 1.0], Zp)
+    Gp = OP.dense_pair_gram([('torus', 'pball', [0.45], [10.0, 8.0], [0, 1])], np.ones(2), Zp, Zp)
+    wp = rng.standard_normal(len(Zp))
+    assert np.abs(OP.pair_rows(Zp, Zp, wp, tabp) - Gp @ wp).max() < 1e-12
