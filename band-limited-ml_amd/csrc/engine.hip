@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "blml_hip.h"
+#include "net.hpp"
 #include "pairs.hpp"
 #include "solver.hpp"
 
@@ -1293,6 +1294,93 @@ int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *
     else
         pair_cols_kernel<double><<<blocks, BLML_PAIR_NT, 0, st>>>(t, ZA, ZB, R, M, depmask, (double *)out,
                                                                   (double *)(out_m ? out_m : out));
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+// ------------------------------------------------------------------------------------------------ warm-start net
+static int net_check(int N, int Cin, int Cout, int H, int W) {
+    if (N < 1 || Cin < 1 || Cin > 32 || Cout < 1 || Cout > 32)
+        return fail(BLML_EVALUE, "net: 1..32 input and output channels, N >= 1");
+    if (W != BLML_NET_W || H < BLML_NET_RT || H % BLML_NET_RT)
+        return fail(BLML_EVALUE, "net: images of width 64 and even height");
+    return BLML_OK;
+}
+#define NET_GRID(n) (unsigned)(((n) + 255) / 256)
+
+int blml_net_conv(const void *X, const void *Wk, const float *bias, float *Z, void *A, int32_t N, int32_t Cin,
+                  int32_t Cout, int32_t H, int32_t W, int32_t gelu, void *stream) {
+    int rc = net_check(N, Cin, Cout, H, W);
+    if (rc) return rc;
+    if (!X || !Wk || !Z || (gelu && !A)) return fail(BLML_EVALUE, "null argument");
+    const int Kpad = (Cin * 9 + 7) / 8 * 8;
+    const int lds = (((Cin * BLML_NET_PATCH + 1 + 7) / 8) * 8 + 32 * Kpad) * 2 + Kpad * 4;
+    net_conv_kernel<<<dim3(H / BLML_NET_RT, N), BLML_NET_NT, lds, (hipStream_t)stream>>>(
+        (const bf16_t *)X, (const bf16_t *)Wk, bias, Z, (bf16_t *)A, Cin, Cout, H, gelu);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+int blml_net_conv_grad_w(const void *X, const void *dZ, float *dW, float *db, int32_t N, int32_t Cin, int32_t Cout,
+                         int32_t H, int32_t W, void *stream) {
+    int rc = net_check(N, Cin, Cout, H, W);
+    if (rc) return rc;
+    if (!X || !dZ || !dW || !db) return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int K = Cin * 9, len = Cout * K;
+    float *part = nullptr, *bpart = nullptr;
+    HIPCHECK(hipMallocAsync((void **)&part, sizeof(float) * (size_t)N * len, st));
+    HIPCHECK(hipMallocAsync((void **)&bpart, sizeof(float) * (size_t)N * Cout, st));
+    const int lds = ((Cin * BLML_NET_PATCH + 7) / 8) * 8 * 2 + 32 * BLML_NET_RT * BLML_NET_W * 2;
+    net_conv_dw_kernel<<<N, BLML_NET_NT, lds, st>>>((const bf16_t *)X, (const bf16_t *)dZ, part, Cin, Cout, H);
+    HIPCHECK(hipGetLastError());
+    net_sum_parts_kernel<<<NET_GRID(len), 256, 0, st>>>(part, N, len, dW);
+    HIPCHECK(hipGetLastError());
+    net_bias_grad_kernel<<<N * Cout, BLML_NET_NT, 0, st>>>((const bf16_t *)dZ, Cout, H * W, bpart);
+    HIPCHECK(hipGetLastError());
+    net_sum_parts_kernel<<<NET_GRID(Cout), 256, 0, st>>>(bpart, N, Cout, db);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipFreeAsync(part, st));
+    HIPCHECK(hipFreeAsync(bpart, st));
+    return BLML_OK;
+}
+
+int blml_net_flip(const void *Wk, void *Wf, int32_t Cin, int32_t Cout, void *stream) {
+    if (!Wk || !Wf || Cin < 1 || Cout < 1) return fail(BLML_EVALUE, "bad argument");
+    net_flip_kernel<<<NET_GRID(Cin * Cout * 9), 256, 0, (hipStream_t)stream>>>((const bf16_t *)Wk, (bf16_t *)Wf, Cin,
+                                                                                Cout);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+int blml_net_gelu_grad(const float *dA, const float *Z, void *dZ, int64_t n, void *stream) {
+    if (!dA || !Z || !dZ || n < 0) return fail(BLML_EVALUE, "bad argument");
+    if (n) net_gelu_bwd_kernel<<<NET_GRID(n), 256, 0, (hipStream_t)stream>>>(dA, Z, (bf16_t *)dZ, n);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+int blml_net_mse(const float *Z, const float *Y, const float *M, void *dZ, float *loss_part, int64_t n, float inv_msum,
+                 void *stream) {
+    if (!Z || !Y || !M || !dZ || !loss_part || n < 1) return fail(BLML_EVALUE, "bad argument");
+    net_mse_kernel<<<NET_GRID(n), BLML_NET_NT, 0, (hipStream_t)stream>>>(Z, Y, M, (bf16_t *)dZ, loss_part, n,
+                                                                          inv_msum);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+int blml_net_adam(float *p, const float *g, float *m, float *v, void *bf, int64_t n, float lr, float b1, float b2,
+                  float eps, int32_t step, void *stream) {
+    if (!p || !g || !m || !v || n < 1 || step < 1) return fail(BLML_EVALUE, "bad argument");
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    net_adam_kernel<<<NET_GRID(n), 256, 0, (hipStream_t)stream>>>(p, g, m, v, (bf16_t *)bf, n, lr, b1, b2, eps, c1, c2);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+int blml_net_to_bf16(const float *x, void *y, int64_t n, void *stream) {
+    if (!x || !y || n < 0) return fail(BLML_EVALUE, "bad argument");
+    if (n) net_to_bf16_kernel<<<NET_GRID(n), 256, 0, (hipStream_t)stream>>>(x, (bf16_t *)y, n);
     HIPCHECK(hipGetLastError());
     return BLML_OK;
 }

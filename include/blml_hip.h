@@ -168,6 +168,30 @@ int blml_pair_rows(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *
 int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, int64_t R, int64_t M,
                    int64_t depmask, void *out, void *out_m, void *stream);
 
+/* ---------------------------------------------------------------------------------------------------------------
+ * The warm-start network (no counterpart in the reference: the learned start of the fit, entering through
+ * BLMLBinned.fit(init=...), blml_binned.py:403-408).  NCHW images of width 64, even height; bf16 = 16-bit brain
+ * floats (uint16 storage); FP32 accumulation.  All pointers are device pointers. */
+
+/* Z = conv3x3(X, W) + b (same padding), W [Cout][Cin][3][3] bf16, X bf16, Z FP32; gelu != 0: A = bf16(GELU(Z)). */
+int blml_net_conv(const void *X, const void *Wk, const float *bias, float *Z, void *A, int32_t N, int32_t Cin,
+                  int32_t Cout, int32_t H, int32_t W, int32_t gelu, void *stream);
+/* dW [Cout][Cin][3][3] = sum over images and pixels of dZ (x) X's 3x3 windows; db [Cout] = sum of dZ (FP32). */
+int blml_net_conv_grad_w(const void *X, const void *dZ, float *dW, float *db, int32_t N, int32_t Cin, int32_t Cout,
+                         int32_t H, int32_t W, void *stream);
+/* Wf [Cin][Cout][3][3] = W [Cout][Cin] flipped in space: the data gradient is blml_net_conv(dZ, Wf). */
+int blml_net_flip(const void *Wk, void *Wf, int32_t Cin, int32_t Cout, void *stream);
+/* dZ = bf16(dA * GELU'(Z)) (n values). */
+int blml_net_gelu_grad(const float *dA, const float *Z, void *dZ, int64_t n, void *stream);
+/* masked squared loss: dZ = bf16(2 M (Z - Y) inv_msum), loss_part[b] = partial sums of M (Z - Y)^2 inv_msum
+ * (ceil(n / 256) floats). */
+int blml_net_mse(const float *Z, const float *Y, const float *M, void *dZ, float *loss_part, int64_t n, float inv_msum,
+                 void *stream);
+/* Adam step `step` (1-based) on FP32 weights p with gradient g and moments m, v; bf (may be NULL) gets bf16(p). */
+int blml_net_adam(float *p, const float *g, float *m, float *v, void *bf, int64_t n, float lr, float b1, float b2,
+                  float eps, int32_t step, void *stream);
+int blml_net_to_bf16(const float *x, void *y, int64_t n, void *stream);
+
 /* Sizes the plan chose (for the caller's buffers and for reporting).  grid: workgroups of the batched launch, or
  * minus the workgroup count when the plan's one fit is spread over the device. */
 int blml_plan_info(void *plan, int64_t *out_elems, int64_t *ws_bytes, int32_t *lds_bytes, int32_t *grid);
