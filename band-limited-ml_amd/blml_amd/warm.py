@@ -65,7 +65,8 @@ def describe(modes):
     sp = parse_modes(modes, 2)[0][0]
     kind = str(sp['kinds'][0])
     name, _, tag = kind.partition(':')
-    return float(name == 'pball'), float(np.mean(sp['fc'])), float(tag) if tag else 0.0
+    fill = float(tag) if tag and name in ('slits', 'dirichlet') else 0.0
+    return float(name == 'pball'), float(np.mean(sp['fc'])), fill
 
 
 def features(counts, modes):
@@ -281,9 +282,54 @@ def start_solutions(jobs, dtype=np.float32, max_sweeps=3000):
     return [np.asarray(r['e_hat'][0], float) for r in res], [float(fs.s[0]) for fs in specs]
 
 
+def evaluate(net, jobs, params, dtype=np.float32, seed=0):
+    """Cold and warm fits of jobs [(counts, modes, complex_g)], each as one batched launch.  The warm fit's cost
+    includes the features (host) and the network (device).  A warm fit that ends below the cold one's likelihood by
+    more than 1e-6 (relative) falls back to the cold result.  Returns a dict of totals and per-fit arrays."""
+    import torch
+    from .binned import _FitSpec, FitBatch
+    cold = [_FitSpec(c, m, 1.0, 0.0, cg, None, None, seed) for c, m, cg in jobs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bc = FitBatch(cold, params, dtype, hist_cap=0).build()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rc = bc.run().results()
+    t_cold = time.perf_counter() - t1
+    bc.close()
+    t2 = time.perf_counter()
+    X = np.stack([features(c, m) for c, m, _ in jobs])
+    pred = net.predict(X)
+    inits = [warm_init(fs, start_from_prediction(c, p, fs.s[0]), params['phase_noise'], seed)
+             for fs, (c, m, cg), p in zip(cold, jobs, pred)]
+    t_pred = time.perf_counter() - t2
+    warm = [_FitSpec(c, m, 1.0, 0.0, cg, None, e0, seed) for (c, m, cg), e0 in zip(jobs, inits)]
+    bw = FitBatch(warm, params, dtype, hist_cap=0).build()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    rw = bw.run().results()
+    t_warm = time.perf_counter() - t3
+    bw.close()
+    llc = np.array([r['loglik'] for r in rc])
+    llw = np.array([r['loglik'] for r in rw])
+    itc = np.array([r['n_sweeps'] + r['start_sweeps'] for r in rc])
+    itw = np.array([r['n_sweeps'] + r['start_sweeps'] for r in rw])
+    dtc = np.array([r['device_seconds']['fit'] for r in rc])
+    dtw = np.array([r['device_seconds']['fit'] for r in rw])
+    fallback = llw < llc - 1e-6 * np.abs(llc)
+    rel = np.abs(llw - llc) / np.abs(llc)
+    return dict(fits=len(jobs), cold_seconds=t_cold, warm_seconds=t_warm, predict_seconds=t_pred,
+                speedup=t_cold / (t_warm + t_pred), cold_iterations=float(itc.mean()), warm_iterations=float(itw.mean()),
+                cold_fit_device_s=float(dtc.sum()), warm_fit_device_s=float(dtw.sum()),
+                fallbacks=int(fallback.sum()), warm_better=int((llw > llc + 1e-6 * np.abs(llc)).sum()),
+                within_1e6=float(np.mean(rel <= 1e-6)), cold_build_seconds=t1 - t0)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description='train the warm-start network on fits the engine makes')
-    ap.add_argument('cmd', choices=['train'])
+    ap.add_argument('cmd', choices=['train', 'eval'])
+    ap.add_argument('--weights', default=WEIGHTS)
+    ap.add_argument('--eval-fits', type=int, default=2048)
     ap.add_argument('--fits', type=int, default=16384)
     ap.add_argument('--seed0', type=int, default=1000)
     ap.add_argument('--epochs', type=int, default=20)
@@ -293,6 +339,13 @@ def main(argv=None):
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
     import bench
     t0 = time.time()
+    if args.cmd == 'eval':
+        import json
+        net = WarmNet.load(args.weights)
+        jobs = [(c, bench.modes_of(g, __import__('blml_amd')), kw['complex_g'])
+                for c, g, kw, _ in bench.uneven_fits(args.eval_fits, seed=4)]
+        print(json.dumps(evaluate(net, jobs, bench.fit_params())), flush=True)
+        return
     X, Y, M = [], [], []
     for c0 in range(0, args.fits, 2048):
         seeds = range(args.seed0 + c0, args.seed0 + min(args.fits, c0 + 2048))

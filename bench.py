@@ -142,6 +142,12 @@ def pair_companion(n=2_000_000, seed=2):
     return Z, cnt.astype(float), h
 
 
+def fit_params():
+    """The solver keywords of the batched workload (FitBatch form)."""
+    return dict(tol=SOLVER['tol'], max_sweeps=SOLVER['max_sweeps'], anderson=5, stall=1000, phase_noise=0.1,
+                ll_tol=SOLVER['ll_tol'], ll_window=100, solver=SOLVER['solver'], lbfgs_m=SOLVER['lbfgs_m'])
+
+
 def modes_of(groups, lib):
     return [md for kw in groups for md in lib.make_modes(**kw)]
 
