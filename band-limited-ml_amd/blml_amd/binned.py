@@ -79,7 +79,13 @@ class _FitSpec:
         self.s = np.array(s)
         self.shared = len(chains) == 1
         self.equal_s = bool(np.allclose(self.s, self.s[0]))
-        # warm start (blml_binned.py:403-408) and the start's draws in the reference's order
+        self.warm = False
+        self.set_start(init, seed)
+
+    def set_start(self, init, seed):
+        """The start: init's e_hat_ (blml_binned.py:403-408) when its shape_ matches, else the cold start; and the
+        start's draws in the reference's order (blml_binned.py:415-436)."""
+        shape = self.shape
         self.init = None
         kk = 0
         if init is not None and getattr(init, 'e_hat_', None) is not None and tuple(init.shape_) == tuple(shape):
@@ -388,22 +394,45 @@ class BLMLBinned:
 
 
 def fit_many(jobs, tol=1e-5, max_sweeps=None, anderson=5, dtype=np.float64, phase_noise=0.1, seed=0, stall=1000,
-             ll_tol=None, ll_window=100, solver='cd', lbfgs_m=20, hist_cap=4096):
+             ll_tol=None, ll_window=100, solver='cd', lbfgs_m=20, hist_cap=4096, warm=None):
     """Fits many independent lattices in one persistent launch.  jobs: iterable of (counts, modes, fit kwargs)
-    with the keywords of BLMLBinned.fit.  Returns one fitted BLMLBinned per job (same attributes)."""
+    with the keywords of BLMLBinned.fit.  Returns one fitted BLMLBinned per job (same attributes).
+
+    warm: None (cold starts), True (the packaged warm-start network) or a blml_amd.warm.WarmNet.  The network
+    predicts the start of every 2-D job without init= (blml_amd.warm.apply); each such fit still runs to the
+    solver's tolerance, and one that ends without converging (stop reason 'stall' or 'max_sweeps') is fitted
+    again from the cold start in a second launch, whose result it keeps.  Fitted estimators carry warm_start_
+    (True when the kept result started from the network)."""
     est = BLMLBinned(tol=tol, max_sweeps=max_sweeps, anderson=anderson, dtype=dtype, phase_noise=phase_noise,
                      seed=seed, stall=stall, ll_tol=ll_tol, ll_window=ll_window, solver=solver, lbfgs_m=lbfgs_m)
     specs = []
-    for counts, modes, kw in jobs:
+    jobs_list = [(counts, modes, dict(kw)) for counts, modes, kw in jobs]
+    for counts, modes, kw in jobs_list:
         kw = dict(kw)
         specs.append(_FitSpec(counts, modes, kw.pop('h', 1.0), kw.pop('origin', 0.0), kw.pop('complex_g', False),
                               kw.pop('signs', None), kw.pop('init', None), seed))
         if kw:
             raise TypeError(f"unknown fit keywords {sorted(kw)}")
+    if warm is not None and warm is not False:
+        from . import warm as _warm
+        _warm.apply(_warm.WarmNet.load() if warm is True else warm, specs, phase_noise, seed)
     batch = FitBatch(specs, est._params(), dtype, hist_cap=hist_cap).build().run()
+    res = batch.results()
+    where = [(batch, i) for i in range(len(specs))]
+    redo = [i for i, (fs, r) in enumerate(zip(specs, res)) if fs.warm and r['stop_reason'] in ('stall', 'max_sweeps')]
+    if redo:
+        cold = []
+        for i in redo:
+            counts, modes, kw = jobs_list[i]
+            cold.append(_FitSpec(counts, modes, kw.get('h', 1.0), kw.get('origin', 0.0), kw.get('complex_g', False),
+                                 kw.get('signs', None), None, seed))
+        b2 = FitBatch(cold, est._params(), dtype, hist_cap=hist_cap).build().run()
+        for j, (i, r) in enumerate(zip(redo, b2.results())):
+            specs[i], res[i], where[i] = cold[j], r, (b2, j)
     out = []
-    for i, (fs, r) in enumerate(zip(specs, batch.results())):
+    for fs, r, (b, i) in zip(specs, res, where):
         m = BLMLBinned(tol=tol, max_sweeps=max_sweeps, anderson=anderson, dtype=dtype, phase_noise=phase_noise,
                        seed=seed, stall=stall, ll_tol=ll_tol, ll_window=ll_window, solver=solver, lbfgs_m=lbfgs_m)
-        out.append(m._adopt(fs, batch, r, i))
+        out.append(m._adopt(fs, b, r, i))
+        m.warm_start_ = fs.warm
     return out
