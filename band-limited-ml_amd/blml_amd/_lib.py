@@ -52,7 +52,7 @@ assert RESULT_DTYPE.itemsize == ctypes.sizeof(Result)
 EXPORTS = ('blml_plan_create', 'blml_plan_destroy', 'blml_matvec', 'blml_fit', 'blml_certificate',
            'blml_plan_info', 'blml_pair_rows', 'blml_pair_cols', 'blml_net_conv', 'blml_net_conv_grad_w',
            'blml_net_flip', 'blml_net_gelu_grad', 'blml_net_mse', 'blml_net_adam', 'blml_net_to_bf16',
-           'blml_last_error')
+           'blml_quick_sweep', 'blml_quick_exact', 'blml_quick_stats', 'blml_last_error')
 
 _lib = None
 
@@ -86,6 +86,10 @@ def load(path=None):
     lib.blml_net_mse.argtypes = [vp, vp, vp, vp, vp, i64, f32, vp]
     lib.blml_net_adam.argtypes = [vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp]
     lib.blml_net_to_bf16.argtypes = [vp, vp, i64, vp]
+    f64 = ctypes.c_double
+    lib.blml_quick_sweep.argtypes = [vp, vp, i64, vp, vp, vp, i64, f64, f64, i32, vp, vp, vp]
+    lib.blml_quick_exact.argtypes = [vp, vp, i64, vp, vp, vp, vp]
+    lib.blml_quick_stats.argtypes = [vp, vp, vp, i64, f64, vp, vp]
     lib.blml_last_error.restype = ctypes.c_char_p
     for name in EXPORTS[:-1]:
         getattr(lib, name).restype = ctypes.c_int

@@ -82,6 +82,15 @@ class _FitSpec:
         self.warm = False
         self.set_start(init, seed)
 
+    def set_start_solution(self, er):
+        """The cold start (its draws unchanged) with er, on the lattice, in place of the real rank-1 start solve
+        under mode 0 (blml_binned.py:421-425) -- or of e0 when the fit is not psd.  The engine reads it as
+        init_K = -1."""
+        e = np.zeros((1, self.B, 2))
+        e[0, :, 0] = np.asarray(er, float).ravel()
+        self.init = e
+        self.init_K = -1
+
     def set_start(self, init, seed):
         """The start: init's e_hat_ (blml_binned.py:403-408) when its shape_ matches, else the cold start; and the
         start's draws in the reference's order (blml_binned.py:415-436)."""
@@ -415,7 +424,7 @@ def fit_many(jobs, tol=1e-5, max_sweeps=None, anderson=5, dtype=np.float64, phas
             raise TypeError(f"unknown fit keywords {sorted(kw)}")
     if warm is not None and warm is not False:
         from . import warm as _warm
-        _warm.apply(_warm.WarmNet.load() if warm is True else warm, specs, phase_noise, seed)
+        _warm.apply(_warm.WarmNet.load() if warm is True else warm, specs)
     batch = FitBatch(specs, est._params(), dtype, hist_cap=hist_cap).build().run()
     res = batch.results()
     where = [(batch, i) for i in range(len(specs))]

@@ -6,8 +6,8 @@ coordinate-descent solve, blml_binned.py:415-427), then draws phases and the oth
 convex with a unique optimum, so it is a well-posed regression target: the network predicts
 log(e_b / sqrt(c_b / s_0)) on a 64 x 64 grid (c_b the counts, s_0 = K_0(0)), from log(1 + counts), the occupancy and
 four planes of the kernel's parameters.  The warm start is the cold start with that solve replaced by the prediction
--- the same phase and mode draws from default_rng(seed) -- and enters the fit through init= (blml_binned.py:403-408).
-The fit then runs to the plain method's tolerance as usual (fit_many(..., warm=True) applies it and fits again
+-- the same phase and mode draws from default_rng(seed), applied by the engine (a fit's init_K = -1).  The fit then
+runs to the plain method's tolerance as usual (fit_many(..., warm=True) applies it and fits again
 from the cold start any warm fit that ends without converging).  The C3-like fits have two modes and are
 non-convex: from another start they end at another local maximum, as an FP32 and an FP64 run from the same start
 do too, so a warm fit's likelihood is judged by the FP64 polish test (a local maximum) rather than by equality
@@ -17,10 +17,12 @@ Network: 4 convolutions 3 x 3 (6 -> 32 -> 32 -> 32 -> 1 channels), GELU between,
 accumulation and FP32 master weights; Adam.  Inputs are resampled to 64 x 64 and the output back to the lattice
 bilinearly (corners aligned).
 
-    python -m blml_amd.warm train --fits 16384 --out warm_net.npz     # data from the engine, then training
+    python -m blml_amd.warm train --fits 50000 --out warm_net.npz     # data from the engine, then training
+    python -m blml_amd.warm eval                                      # cold vs warm on 2048 held-out fits
 """
 import argparse
 import ctypes
+import functools
 import math
 import os
 import sys
@@ -43,6 +45,7 @@ def _torch():
 
 
 # ------------------------------------------------------------------------------------------------ grids
+@functools.lru_cache(maxsize=512)
 def _interp_matrix(n_out, n_in):
     """Linear interpolation from n_in points to n_out points, both spanning [0, 1] (corners aligned)."""
     A = np.zeros((n_out, n_in))
@@ -62,22 +65,22 @@ def resample(a, shape):
     return _interp_matrix(shape[0], a.shape[0]) @ a @ _interp_matrix(shape[1], a.shape[1]).T
 
 
-def describe(modes):
-    """(is_pball, fc, fill) of the signal mode from a make_modes / modes list (the first mode)."""
+def describe(modes, spec=None):
+    """(is_pball, fc, fill) of the signal mode: the first of a make_modes / modes list, or spec (a parsed mode)."""
     from .modes import parse_modes
-    sp = parse_modes(modes, 2)[0][0]
+    sp = spec if spec is not None else parse_modes(modes, 2)[0][0]
     kind = str(sp['kinds'][0])
     name, _, tag = kind.partition(':')
     fill = float(tag) if tag and name in ('slits', 'dirichlet') else 0.0
     return float(name == 'pball'), float(np.mean(sp['fc'])), fill
 
 
-def features(counts, modes):
-    """(6, 64, 64) float32 network input of one 2-D lattice."""
+def features(counts, modes, spec=None):
+    """(6, 64, 64) float32 network input of one 2-D lattice (spec: its first mode parsed already, else modes')."""
     c = np.asarray(counts, float)
     occ = (c > 0).astype(float)
     lc = np.log1p(c)
-    is_pb, fc, fill = describe(modes)
+    is_pb, fc, fill = describe(modes, spec)
     X = np.empty((6, GRID, GRID), np.float32)
     X[0] = resample(lc / max(lc.max(), 1e-12), (GRID, GRID))
     X[1] = resample(occ, (GRID, GRID))
@@ -105,25 +108,6 @@ def start_from_prediction(counts, pred64, s0):
     c = np.asarray(counts, float)
     r = resample(np.asarray(pred64, float), c.shape)
     return np.where(c > 0, np.sqrt(c / s0) * np.exp(r), 0.0)
-
-
-def warm_init(fs, er, phase_noise=0.1, seed=0):
-    """The fit's start with the real rank-1 solve replaced by er: E_0 = er exp(i pn z), E_k = pn |E_0| CN(0, 1) --
-    the cold start's draws in its order (blml_binned.py:429-436).  Returns an object with e_hat_ and shape_."""
-    from types import SimpleNamespace
-    rng = np.random.default_rng(seed)
-    sh = fs.shape
-    E = np.zeros((fs.K,) + sh, complex if fs.complex_g else float)
-    E0 = np.asarray(er, float).reshape(sh)
-    if fs.complex_g:
-        E0 = E0 * np.exp(1j * phase_noise * rng.standard_normal(sh))
-    E[0] = E0
-    for k in range(1, fs.K):
-        z = rng.standard_normal(sh)
-        if fs.complex_g:
-            z = z + 1j * rng.standard_normal(sh)
-        E[k] = phase_noise * np.abs(E0) * (z / (np.sqrt(2) if fs.complex_g else 1))
-    return SimpleNamespace(e_hat_=E, shape_=sh)
 
 
 # ------------------------------------------------------------------------------------------------ network
@@ -285,22 +269,22 @@ def start_solutions(jobs, dtype=np.float32, max_sweeps=3000):
     return [np.asarray(r['e_hat'][0], float) for r in res], [float(fs.s[0]) for fs in specs]
 
 
-def apply(net, specs, phase_noise=0.1, seed=0):
+def apply(net, specs):
     """Sets the network's warm start on every 2-D fit spec (blml_amd.binned._FitSpec) that has no init of the
-    caller's (a warm start of an earlier call is replaced): the
-    predicted real rank-1 solution under mode 0 (times the signs, when given) in place of the start solve, the
-    phases and other modes drawn as the cold start draws them.  Returns the number of specs set."""
+    caller's (a warm start of an earlier call is replaced): the predicted real rank-1 solution under mode 0 (times
+    the signs, when given) in place of the start solve; the engine draws the phases and the other modes from the
+    cold start's draws (blml_binned.py:429-436).  Returns the number of specs set."""
     idx = [i for i, fs in enumerate(specs) if fs.d == 2 and (fs.init is None or fs.warm)]
     if not idx:
         return 0
-    X = np.stack([features(specs[i].counts, specs[i].modes) for i in idx])
+    X = np.stack([features(specs[i].counts, None, specs[i].specs[0]) for i in idx])
     pred = net.predict(X)
     for i, p in zip(idx, pred):
         fs = specs[i]
         er = start_from_prediction(fs.counts, p, fs.s[0])
         if fs.signs is not None:
             er = er * fs.signs
-        fs.set_start(warm_init(fs, er, phase_noise, seed), seed)
+        fs.set_start_solution(er)
         fs.warm = True
     return len(idx)
 
@@ -327,7 +311,7 @@ def evaluate(net, jobs, params, dtype=np.float32, seed=0):
     rc, t_cold = _run_batch(mk(), params, dtype)
     specs = mk()
     t0 = time.perf_counter()
-    apply(net, specs, params['phase_noise'], seed)
+    apply(net, specs)
     t_pred = time.perf_counter() - t0
     rw, t_warm = _run_batch(specs, params, dtype)
     llc = np.array([r['loglik'] for r in rc])

@@ -20,6 +20,7 @@
 #include "blml_hip.h"
 #include "net.hpp"
 #include "pairs.hpp"
+#include "quick.hpp"
 #include "solver.hpp"
 
 namespace {
@@ -260,17 +261,21 @@ __device__ void run_fit(const DevPlan<T> &p, int fi, const Team &team, const dou
         }
         S.tm.sync();
     } else {
-        // cold start: e0 = sqrt(c / s0) on occupied bins (times the signs)
+        // cold start: e0 = sqrt(c / s0) on occupied bins (times the signs); with init_K < 0 the caller's real
+        // rank-1 solution stands in for e0 and, in a psd fit, for the start solve below
+        const double *er0 = f.init_K < 0 ? init + f.init_off : nullptr;
         for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
             const double sgb = S.sg ? S.sg[b] : 1.0;
-            E[b] = S.w[b] > 0 ? sqrt(f.n * S.w[b] / f.s[0]) * sgb : 0.0;
+            E[b] = S.w[b] > 0 ? (er0 ? er0[2 * b] : sqrt(f.n * S.w[b] / f.s[0]) * sgb) : 0.0;
         }
         S.tm.sync();
         if (psd_full) {
             // the real rank-1 solution under mode 0 (blml_binned.py:421-425)
             S.K = 1; S.nc = 1; S.psd = false; S.rec = false;
-            const FitOut so = S.solve_cd();
-            start_sweeps = so.n_sweeps;
+            if (!er0) {
+                const FitOut so = S.solve_cd();
+                start_sweeps = so.n_sweeps;
+            }
             tstart = wall_clock64() - tfit0;
             // phases and the other modes (blml_binned.py:429-436), assembled in slot 2 then moved to slot 0
             double *E2 = S.vec(2);
@@ -637,6 +642,7 @@ int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs,
         const blml_fit_t &f = pl.hfits[fi];
         if (f.d < 1 || f.d > BLML_MAXD) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": 1 <= d <= 5");
         if (f.K < 1 || f.K > BLML_MAXK) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": 1 <= K <= 8");
+        if (f.init_K < -1) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": init_K >= -1");
         long B = 1;
         for (int j = 0; j < f.d; ++j) {
             if (f.shape[j] < 1) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": empty lattice axis");
@@ -1155,7 +1161,7 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
     Plan *pl = (Plan *)plan;
     if (!pl || !counts || !draws || !e_out || !se_out || !density || !results) return fail(BLML_EVALUE, "null argument");
     for (const blml_fit_t &f : pl->hfits) {
-        if (f.init_K > 0 && !init) return fail(BLML_EVALUE, "a warm-started fit needs init");
+        if (f.init_K != 0 && !init) return fail(BLML_EVALUE, "a warm-started fit needs init");
         if (f.has_signs && !signs) return fail(BLML_EVALUE, "a fit with signs needs signs");
     }
     hipStream_t st = (hipStream_t)stream;
@@ -1239,6 +1245,37 @@ static int pair_check(const blml_pairtab_t *t) {
     return BLML_OK;
 }
 
+// the rows launch's shape: rb workgroups of rows, nsl slices of `slice` sources (a multiple of the LDS chunk)
+static void pair_rows_grid(long R, long M, long &rb, long &nsl, long &slice) {
+    rb = (R + BLML_PAIR_NT - 1) / BLML_PAIR_NT;
+    nsl = std::max<long>(1, std::min<long>((2048 + rb - 1) / rb, (M + BLML_PAIR_CH - 1) / BLML_PAIR_CH));
+    slice = ((M + nsl - 1) / nsl + BLML_PAIR_CH - 1) / BLML_PAIR_CH * BLML_PAIR_CH;
+    nsl = (M + slice - 1) / slice;
+}
+
+// s = rows(ZA, ZB, w) for R, M > 0 with a partials buffer of nsl * R values (pair_rows_grid)
+static int pair_rows_launch(const blml_pairtab_t &t, const int32_t *ZA, const int32_t *ZB, const double *w, long R,
+                            long M, double *part, double *s, hipStream_t st) {
+    long rb, nsl, slice;
+    pair_rows_grid(R, M, rb, nsl, slice);
+    const size_t es = t.f32 ? 4 : 8;
+    const bool tl = t.nT * (long)es <= 32768;
+    const long tbytes = tl ? ((t.nT * (long)es + 15) / 16) * 16 : 0;
+    const int lds = (int)(tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16 + BLML_PAIR_CH * 8);
+    const dim3 grid((unsigned)rb, (unsigned)nsl);
+    if (t.f32) {
+        if (tl) pair_rows_kernel<float, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+        else pair_rows_kernel<float, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+    } else {
+        if (tl) pair_rows_kernel<double, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+        else pair_rows_kernel<double, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+    }
+    HIPCHECK(hipGetLastError());
+    pair_rows_sum_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, s);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
 int blml_pair_rows(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, const double *w, int64_t R,
                    int64_t M, double *s, void *stream) {
     int rc = pair_check(tab);
@@ -1251,28 +1288,12 @@ int blml_pair_rows(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *
         HIPCHECK(hipMemsetAsync(s, 0, sizeof(double) * R, st));
         return BLML_OK;
     }
-    const size_t es = tab->f32 ? 4 : 8;
-    const bool tl = tab->nT * (long)es <= 32768;
-    const long tbytes = tl ? ((tab->nT * (long)es + 15) / 16) * 16 : 0;
-    const int lds = (int)(tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16 + BLML_PAIR_CH * 8);
-    const long rb = (R + BLML_PAIR_NT - 1) / BLML_PAIR_NT;
-    long nsl = std::max<long>(1, std::min<long>((2048 + rb - 1) / rb, (M + BLML_PAIR_CH - 1) / BLML_PAIR_CH));
-    long slice = ((M + nsl - 1) / nsl + BLML_PAIR_CH - 1) / BLML_PAIR_CH * BLML_PAIR_CH;
-    nsl = (M + slice - 1) / slice;
+    long rb, nsl, slice;
+    pair_rows_grid(R, M, rb, nsl, slice);
     double *part = nullptr;
     HIPCHECK(hipMallocAsync((void **)&part, sizeof(double) * nsl * R, st));
-    const dim3 grid((unsigned)rb, (unsigned)nsl);
-    const blml_pairtab_t t = *tab;
-    if (tab->f32) {
-        if (tl) pair_rows_kernel<float, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
-        else pair_rows_kernel<float, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
-    } else {
-        if (tl) pair_rows_kernel<double, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
-        else pair_rows_kernel<double, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
-    }
-    HIPCHECK(hipGetLastError());
-    pair_rows_sum_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, s);
-    HIPCHECK(hipGetLastError());
+    rc = pair_rows_launch(*tab, ZA, ZB, w, R, M, part, s, st);
+    if (rc) return rc;
     HIPCHECK(hipFreeAsync(part, st));
     return BLML_OK;
 }
@@ -1295,6 +1316,88 @@ int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *
         pair_cols_kernel<double><<<blocks, BLML_PAIR_NT, 0, st>>>(t, ZA, ZB, R, M, depmask, (double *)out,
                                                                   (double *)(out_m ? out_m : out));
     HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+// ------------------------------------------------------------------------------------------------ BLMLQuick counts
+int blml_quick_sweep(const blml_pairtab_t *tab, const int32_t *Z, int64_t B, const double *w, const double *sg,
+                     const int32_t *perm, int64_t Bs, double n, double sbb, int32_t safeguard, double *e, double *Se,
+                     void *stream) {
+    int rc = pair_check(tab);
+    if (rc) return rc;
+    if (B < 1 || B >= (1L << 31)) return fail(BLML_EVALUE, "B: 1 .. 2^31 - 1 bins");
+    if (Bs < 1 || Bs > B) return fail(BLML_EVALUE, "block size: 1 .. B");
+    if (!Z || !w || !perm || !e || !Se) return fail(BLML_EVALUE, "null argument");
+    if (!(n > 0) || !(sbb > 0)) return fail(BLML_EVALUE, "n and S_bb must be positive");
+    hipStream_t st = (hipStream_t)stream;
+    const int d = tab->d;
+    long rb, nsl, slice, nsl_r = 0;
+    pair_rows_grid(B, Bs, rb, nsl, slice);
+    if (B % Bs) pair_rows_grid(B, B % Bs, rb, nsl_r, slice);     // the last, shorter block
+    nsl = std::max(nsl, nsl_r);
+    // one allocation: partials (nsl B), v (B), eold, dlt, estar, src (Bs each), t, block coordinates (Bs d int32)
+    const size_t nd = (size_t)nsl * B + B + 4 * (size_t)Bs + 1;
+    char *buf = nullptr;
+    HIPCHECK(hipMallocAsync((void **)&buf, sizeof(double) * nd + sizeof(int32_t) * (size_t)Bs * d, st));
+    double *part = (double *)buf, *v = part + (size_t)nsl * B, *eold = v + B, *dlt = eold + Bs, *estar = dlt + Bs,
+           *src = estar + Bs, *tdev = src + Bs;
+    int32_t *Zb = (int32_t *)(tdev + 1);
+    for (long s0 = 0; s0 < B; s0 += Bs) {
+        const long m = std::min<long>(Bs, B - s0);
+        const int32_t *I = perm + s0;
+        quick_block_kernel<<<(unsigned)((m + BLML_QUICK_NT - 1) / BLML_QUICK_NT), BLML_QUICK_NT, 0, st>>>(
+            Z, d, I, (int)m, e, Se, w, sg, n, sbb, Zb, eold, dlt, estar, src);
+        HIPCHECK(hipGetLastError());
+        rc = pair_rows_launch(*tab, Z, Zb, src, B, m, part, v, st);
+        if (rc) return rc;
+        if (safeguard) {
+            quick_step_kernel<<<1, BLML_QUICK_NT, 0, st>>>(I, (int)m, Se, v, w, sg, eold, dlt, n, tdev);
+            HIPCHECK(hipGetLastError());
+        }
+        quick_apply_kernel<<<(unsigned)((B + BLML_QUICK_NT - 1) / BLML_QUICK_NT), BLML_QUICK_NT, 0, st>>>(
+            B, I, (int)m, v, sg, eold, dlt, estar, safeguard ? tdev : nullptr, e, Se);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipFreeAsync(buf, st));
+    return BLML_OK;
+}
+
+int blml_quick_exact(const blml_pairtab_t *tab, const int32_t *Z, int64_t B, const double *e, const double *sg,
+                     double *Se, void *stream) {
+    int rc = pair_check(tab);
+    if (rc) return rc;
+    if (B < 1 || B >= (1L << 31)) return fail(BLML_EVALUE, "B: 1 .. 2^31 - 1 bins");
+    if (!Z || !e || !Se) return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    long rb, nsl, slice;
+    pair_rows_grid(B, B, rb, nsl, slice);
+    double *buf = nullptr;
+    HIPCHECK(hipMallocAsync((void **)&buf, sizeof(double) * ((size_t)nsl * B + 2 * (size_t)B), st));
+    double *part = buf, *src = part + (size_t)nsl * B, *v = src + B;
+    const unsigned g = (unsigned)((B + 255) / 256);
+    quick_signed_kernel<<<g, 256, 0, st>>>(B, e, sg, src);
+    HIPCHECK(hipGetLastError());
+    rc = pair_rows_launch(*tab, Z, Z, src, B, B, part, v, st);
+    if (rc) return rc;
+    quick_signed_kernel<<<g, 256, 0, st>>>(B, v, sg, Se);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipFreeAsync(buf, st));
+    return BLML_OK;
+}
+
+int blml_quick_stats(const double *e, const double *Se, const double *w, int64_t B, double n, double *out,
+                     void *stream) {
+    if (B < 1) return fail(BLML_EVALUE, "B must be >= 1");
+    if (!e || !Se || !w || !out) return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int np = (int)std::min<long>((B + BLML_QUICK_NT - 1) / BLML_QUICK_NT, 1024);
+    double *part = nullptr;
+    HIPCHECK(hipMallocAsync((void **)&part, sizeof(double) * 4 * np, st));
+    quick_stats_kernel<<<np, BLML_QUICK_NT, 0, st>>>(B, e, Se, w, n, part);
+    HIPCHECK(hipGetLastError());
+    quick_stats_sum_kernel<<<1, 64, 0, st>>>(part, np, out);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipFreeAsync(part, st));
     return BLML_OK;
 }
 

@@ -197,7 +197,7 @@ def main():
     def warm_step():
         """One step of the headline: the network's warm starts (features, prediction, start draws), the plan's
         upload and one batched launch to the solver's tolerance."""
-        W.apply(net, specs, params['phase_noise'], 0)
+        W.apply(net, specs)
         b = FitBatch(specs, params, np.float32, hist_cap=0).build().run()
         return b
 
@@ -214,9 +214,8 @@ def main():
     rw = bw.results()
     bw.close()
     print(f'bench: warm fp32 {dtw / args.steps * 1e3:.1f} ms/step', file=sys.stderr, flush=True)
-    for fs in specs:                         # back to the cold start
-        fs.set_start(None, 0)
-        fs.warm = False
+    for fs in specs:                         # back to the cold start (the draws are the cold start's already)
+        fs.init, fs.init_K, fs.warm = None, 0, False
 
     def timed(dtype):
         b = FitBatch(specs, params, dtype, hist_cap=0).build()

@@ -56,7 +56,9 @@ typedef struct {
     int32_t shape[BLML_MAXD];
     int32_t fac_begin[BLML_MAXK];
     int32_t fac_count[BLML_MAXK];
-    int32_t init_K;                  /* 0: cold start (blml_binned.py:415-427); >0: warm start with that many modes */
+    int32_t init_K;                  /* 0: cold start (blml_binned.py:415-427); >0: warm start with that many modes
+                                        (:403-408); -1: the cold start with init's real rank-1 solution (layout
+                                        (1, B, 2)) in place of its start solve, the draws as the cold start's */
     int32_t has_signs;
     double n;                        /* total count */
     double s[BLML_MAXK];             /* K_k(0) */
@@ -191,6 +193,22 @@ int blml_net_mse(const float *Z, const float *Y, const float *M, void *dZ, float
 int blml_net_adam(float *p, const float *g, float *m, float *v, void *bf, int64_t n, float lr, float b1, float b2,
                   float eps, int32_t step, void *stream);
 int blml_net_to_bf16(const float *x, void *y, int64_t n, void *stream);
+
+/* ---- BLMLQuick on counts on a fixed lattice, real rank 1 (blml_gpu.py:249-429; blml_trivial.py:3516-3598) ----
+ * Bins Z (B x tab->d int32, ranked as for blml_pair_rows), counts w (B), n = sum w, sbb = S(z, z), optional signs
+ * sg (B, +-1; NULL: none); e, Se (B, FP64, device) are updated in place. */
+/* One sweep of block-Jacobi CD (blml_gpu.py:340-373): blocks of Bs bins in the order perm (B int32, the sweep's
+ * permutation), each moved to its one-coordinate minimisers; safeguard != 0: the block-level step-size rule. */
+int blml_quick_sweep(const blml_pairtab_t *tab, const int32_t *Z, int64_t B, const double *w, const double *sg,
+                     const int32_t *perm, int64_t Bs, double n, double sbb, int32_t safeguard, double *e, double *Se,
+                     void *stream);
+/* Se = S e from scratch (exact_Se, blml_gpu.py:249-258): FP64 sums. */
+int blml_quick_exact(const blml_pairtab_t *tab, const int32_t *Z, int64_t B, const double *e, const double *sg,
+                     double *Se, void *stream);
+/* out (4 doubles, device) = (e . Se, sum w log max(e, 1e-300), max residual, count of non-finite e)
+ * (objective and residual, blml_gpu.py:262-276). */
+int blml_quick_stats(const double *e, const double *Se, const double *w, int64_t B, double n, double *out,
+                     void *stream);
 
 /* Sizes the plan chose (for the caller's buffers and for reporting).  grid: workgroups of the batched launch, or
  * minus the workgroup count when the plan's one fit is spread over the device. */
