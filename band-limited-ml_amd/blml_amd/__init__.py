@@ -1,5 +1,6 @@
 """MI355X engine for the binned BLML fit (drop-in for blml_binned.BLMLBinned's fit path)."""
 from .binned import BLMLBinned, FitBatch, fit_many
 from .modes import make_modes, parse_modes
+from .quick import BLMLQuick
 
-__all__ = ['BLMLBinned', 'FitBatch', 'fit_many', 'make_modes', 'parse_modes']
+__all__ = ['BLMLBinned', 'BLMLQuick', 'FitBatch', 'fit_many', 'make_modes', 'parse_modes']
