@@ -14,6 +14,8 @@
 // rs = nc * prod m[hi:], i.e. element (ob, r, col) of tile (o0, c0) is at (o0 + ob) R rs + r rs + c0 + col.
 // The innermost pass reads contiguous blocks; outer passes read rows of w contiguous values (w a power of two).
 #pragma once
+#include <type_traits>
+
 #include "common.hpp"
 
 #define BLML_SEP_MAXPASS 5
@@ -133,10 +135,71 @@ __device__ __forceinline__ void sep_axis_mfma(int toff, int nvals, int m, int st
     }
 }
 
+// The same product for lines of exactly BLK values whose blocks pair up evenly (nlines a multiple of 2 BLK) and
+// whose stride st is 1 or a multiple of 32.  Then position j of a line starting at tile index base sits at LDS
+// sep_lds(base) + j stp: with st a multiple of 32, adding j st leaves base's low five bits alone, so the bank
+// padding grows by j st / 32 (stp = st + st / 32); with st = 1 a line is 32-aligned (base = line BLK, a multiple
+// of 32 for BLK = 32; for BLK = 16 the line stays inside one 32-value run), so stp = 1.  Every row is below m and
+// every line exists: no clamps and no dump slots.
+#ifndef BLML_SEP_FULL_NH
+#define BLML_SEP_FULL_NH 2
+#endif
+template <typename T>
+__device__ __forceinline__ void sep_axis_mfma_full(int nvals, int st, int aoff) {
+    using F = SepMfma<T>;
+    constexpr int BLK = F::BLK, KMAX = BLK / F::KS, NH = BLML_SEP_FULL_NH;
+    st = __builtin_amdgcn_readfirstlane(st);
+    nvals = __builtin_amdgcn_readfirstlane(nvals);
+    T *tile = reinterpret_cast<T *>(g_dyn_lds);
+    const T *At = reinterpret_cast<const T *>(g_dyn_lds) + aoff;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane % BLK, lk = lane / BLK;
+    T a[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) a[k] = At[li * (BLK + 1) + k * F::KS + lk];
+    const int nblk = nvals / (BLK * BLK);
+    const int span = BLK * st;
+    const int stp = st == 1 ? 1 : st + (st >> 5);
+    const float inv_st = 1.0f / (float)st;
+    int rowoff[F::NACC];
+#pragma unroll
+    for (int r = 0; r < F::NACC; ++r) rowoff[r] = F::row(r, lane) * stp;
+    for (int b0 = NH * wave; b0 < nblk; b0 += NH * BLML_NWAVE) {
+        int lb[NH];
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+            int lo;
+            const int hi = fdivmod((b0 + h) * BLK + li, st, inv_st, lo);
+            lb[h] = sep_lds(hi * span + lo);
+        }
+        T xv[NH][KMAX];
+#pragma unroll
+        for (int h = 0; h < NH; ++h)
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) xv[h][k] = tile[lb[h] + (k * F::KS + lk) * stp];
+        typename F::acc_t acc[NH];
+#pragma unroll
+        for (int h = 0; h < NH; ++h)
+#pragma unroll
+            for (int r = 0; r < F::NACC; ++r) acc[h][r] = (T)0;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+            for (int h = 0; h < NH; ++h) acc[h] = F::step(a[k], xv[h][k], acc[h]);
+#pragma unroll
+        for (int h = 0; h < NH; ++h)
+#pragma unroll
+            for (int r = 0; r < F::NACC; ++r) tile[lb[h] + rowoff[r]] = acc[h][r];
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void sep_axis(int toff, int nvals, int m, int st, int aoff) {
-    if (m <= SepMfma<T>::BLK) sep_axis_mfma<T, 1>(toff, nvals, m, st, aoff);
-    else if (m <= 2 * SepMfma<T>::BLK) sep_axis_mfma<T, 2>(toff, nvals, m, st, aoff);
+    constexpr int BLK = SepMfma<T>::BLK;
+    if (toff == 0 && m == BLK && nvals % (BLML_SEP_FULL_NH * BLK * BLK) == 0 && (st == 1 || st % 32 == 0))
+        sep_axis_mfma_full<T>(nvals, st, aoff);
+    else if (m <= BLK) sep_axis_mfma<T, 1>(toff, nvals, m, st, aoff);
+    else if (m <= 2 * BLK) sep_axis_mfma<T, 2>(toff, nvals, m, st, aoff);
 }
 
 // LDS values a staged table of an axis of length m takes (P rows of P + 1)
@@ -209,6 +272,173 @@ __device__ __forceinline__ void sep_move(const SepPass &ps, long gbase, int obv,
     }
 }
 
+// A whole tile moved with neither a base nor a type change, UB values per thread in flight.  Tile value
+// e = e0 + t + NT u (e0 a multiple of UB NT) sits at LDS sep_lds(e0 + t) + (NT + NT / 32) u, since NT u is a
+// multiple of 32.  Its global offset is linear in u: contiguous tiles (w == rs) at gbase + e; row tiles with w
+// dividing NT at gbase + row rs + col with col = t mod w and the row growing by NT / w per u; row tiles with w a
+// multiple of UB NT at gbase + row rs + (e0 mod w) + t + NT u (the chunk stays inside one row).  The caller checks
+// that the tile is whole and that tsize is a multiple of UB NT, so every access is one the general move makes.
+// global-memory views of plain pointers (global loads and stores count on vmcnt only, so the LDS waits of the
+// tile's products do not wait for them)
+template <typename T>
+using sep_gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+using sep_gcptr = const __attribute__((address_space(1))) T *;
+
+#ifndef BLML_SEP_FAST_UB
+#define BLML_SEP_FAST_UB 16
+#endif
+__device__ __forceinline__ bool sep_fast_move_ok(const SepPass &ps, int obv, int wv) {
+    const long tsize = (long)ps.OB * ps.R * ps.w;
+    return obv == ps.OB && wv == ps.w && tsize % (BLML_SEP_FAST_UB * BLML_NT) == 0 &&
+           (ps.w == ps.rs || ps.w <= BLML_NT || ps.w % (BLML_SEP_FAST_UB * BLML_NT) == 0);
+}
+
+template <typename T, bool LOAD>
+__device__ __forceinline__ void sep_move_fast(const SepPass &ps, long gbase, const T *src, T *dst) {
+    constexpr int UB = BLML_SEP_FAST_UB;
+    T *tile = reinterpret_cast<T *>(g_dyn_lds);
+    const int tsize = ps.OB * ps.R * ps.w;
+    const int t = threadIdx.x;
+    const int lw = 31 - __clz(ps.w);
+    const int mode = ps.w == ps.rs ? 0 : (ps.w <= BLML_NT ? 1 : 2);
+    const long ustep = mode == 1 ? (long)(BLML_NT >> lw) * ps.rs : (long)BLML_NT;
+    constexpr int lstep = BLML_NT + BLML_NT / 32;
+    for (int e0 = 0; e0 < tsize; e0 += UB * BLML_NT) {
+        long g0;
+        if (mode == 0) g0 = gbase + e0 + t;
+        else if (mode == 1) g0 = gbase + (long)((e0 + t) >> lw) * ps.rs + (t & (ps.w - 1));
+        else g0 = gbase + (long)(e0 >> lw) * ps.rs + (e0 & (ps.w - 1)) + t;
+        const int l0 = sep_lds(e0 + t);
+        T v[UB];
+        if (LOAD) {
+            sep_gcptr<T> p = (sep_gcptr<T>)(src + g0);
+#pragma unroll
+            for (int u = 0; u < UB; ++u) v[u] = p[u * ustep];
+#pragma unroll
+            for (int u = 0; u < UB; ++u) tile[l0 + lstep * u] = v[u];
+        } else {
+            sep_gptr<T> p = (sep_gptr<T>)(dst + g0);
+#pragma unroll
+            for (int u = 0; u < UB; ++u) v[u] = tile[l0 + lstep * u];
+#pragma unroll
+            for (int u = 0; u < UB; ++u) p[u * ustep] = v[u];
+        }
+    }
+}
+
+// A pass whose tiles are all whole, of NV NT values, with neither bases nor type changes, software-pipelined: the
+// next tile's values are loaded into registers (NV per thread) while the matrix cores work on the current one in
+// LDS, and a tile's results leave through registers in chunks (the stores drain while the loop goes on).  Tile
+// value e = t + NT i sits at LDS sep_lds(t) + (NT + NT / 32) i and at global offset start + i step:
+//   w == rs (contiguous):   start = gbase + t,                         step = NT
+//   w divides NT (rows):    start = gbase + (t / w) rs + (t mod w),    step = (NT / w) rs
+template <typename T, int NV>
+__device__ __forceinline__ void sep_pass_pipe(const SepPass &ps, const int *shape, const int *fac_of, const int *aoff,
+                                              const T *src, T *dst, int rank, int nranks) {
+    T *tile = reinterpret_cast<T *>(g_dyn_lds);
+    const int t = threadIdx.x;
+    // the pass's axes (innermost first): stride and staged-table offset, held in registers so that nothing inside
+    // the tile loop reads memory other than the tiles (a read there would wait for the prefetched loads)
+    int nax = 0, ast[BLML_MAXD], aao[BLML_MAXD];
+    {
+        int st = ps.w;
+        for (int a = ps.hi - 1; a >= ps.lo; --a) {
+            if (fac_of[a] >= 0 && shape[a] > 1) {
+                ast[nax] = st;
+                aao[nax] = aoff[a];
+                ++nax;
+            }
+            st *= shape[a];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < BLML_MAXD; ++k) {
+        ast[k] = __builtin_amdgcn_readfirstlane(k < nax ? ast[k] : 1);
+        aao[k] = __builtin_amdgcn_readfirstlane(k < nax ? aao[k] : 0);
+    }
+    nax = __builtin_amdgcn_readfirstlane(nax);
+    const long rs = ((long)__builtin_amdgcn_readfirstlane((int)(ps.rs >> 32)) << 32) |
+                    (unsigned)__builtin_amdgcn_readfirstlane((int)ps.rs);
+    const int w = __builtin_amdgcn_readfirstlane(ps.w);
+    const long R = __builtin_amdgcn_readfirstlane(ps.R), OB = __builtin_amdgcn_readfirstlane(ps.OB);
+    const long nchunks = ps.nchunks, ntiles = ps.ntiles;
+    const bool contig = w == rs;
+    const int lw = 31 - __clz(w);
+    const long step = contig ? (long)BLML_NT : (long)(BLML_NT >> lw) * rs;
+    const long toff = contig ? t : (long)(t >> lw) * rs + (t & (w - 1));
+    const int l0 = sep_lds(t);
+    constexpr int lstep = BLML_NT + BLML_NT / 32;
+    const int tsize = NV * BLML_NT;
+    T v[NV];
+    long tn = rank;
+    if (tn < ntiles) {
+        const long to = tn / nchunks, tc = tn - to * nchunks;
+        sep_gcptr<T> p = (sep_gcptr<T>)(src + (to * OB * R * rs + tc * w + toff));
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            v[i] = *p;
+            p += step;
+        }
+    }
+    for (; tn < ntiles; tn += nranks) {
+        const long to = tn / nchunks, tc = tn - to * nchunks;
+        const long gbase = to * OB * R * rs + tc * w;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) tile[l0 + lstep * i] = v[i];
+        __syncthreads();
+        const long tnext = tn + nranks;
+        if (tnext < ntiles) {
+            const long to2 = tnext / nchunks, tc2 = tnext - to2 * nchunks;
+            sep_gcptr<T> p = (sep_gcptr<T>)(src + (to2 * OB * R * rs + tc2 * w + toff));
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                v[i] = *p;
+                p += step;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < BLML_MAXD; ++k) {
+            if (k < nax) {
+                sep_axis_mfma_full<T>(tsize, ast[k], aao[k]);
+                __syncthreads();
+            }
+        }
+        sep_gptr<T> q = (sep_gptr<T>)(dst + (gbase + toff));
+        constexpr int CH = 16;
+#pragma unroll
+        for (int i0 = 0; i0 < NV; i0 += CH) {
+            T o[CH];
+#pragma unroll
+            for (int i = 0; i < CH; ++i) o[i] = tile[l0 + lstep * (i0 + i)];
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                *q = o[i];
+                q += step;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// whether every tile of the pass is whole, of NV NT values, contiguous or in rows of w dividing NT, and every axis
+// of the pass takes the full-block product (m == BLK, stride 1 or a multiple of 32)
+template <typename T>
+__device__ __forceinline__ bool sep_pipe_ok(const SepPass &ps, int nv, const int *shape, const int *fac_of) {
+    constexpr int BLK = SepMfma<T>::BLK;
+    const long tsize = (long)ps.OB * ps.R * ps.w;
+    if (!(tsize == (long)nv * BLML_NT && ps.nout % ps.OB == 0 && ps.rs % ps.w == 0 &&
+          (ps.w == ps.rs || (ps.w <= BLML_NT && BLML_NT % ps.w == 0))))
+        return false;
+    if (tsize % (BLML_SEP_FULL_NH * BLK * BLK)) return false;
+    long st = ps.w;
+    for (int a = ps.hi - 1; a >= ps.lo; --a) {
+        if (fac_of[a] >= 0 && shape[a] > 1 && !(shape[a] == BLK && (st == 1 || st % 32 == 0))) return false;
+        st *= shape[a];
+    }
+    return true;
+}
+
 // One pass of a separable mode: dst = (prod over the pass's axes of M_a) src.  The first pass may subtract a base
 // from its input, the last may add a base to its output (the incremental form U = Ubase + S (E - Ebase)); values
 // are converted Ti -> T on load and T -> To on store (the base arithmetic in FP64).  Tiles are dealt to the team's
@@ -266,13 +496,25 @@ __device__ __forceinline__ void sep_pass(const SepPass &ps, int d, int nc, const
         }
         __syncthreads();
     }
+    if (std::is_same<Ti, T>::value && std::is_same<To, T>::value && !srcb && !dstb) {
+        constexpr int NV = sizeof(T) == 4 ? 64 : 32;          // 128 KB tiles
+        if (sep_pipe_ok<T>(ps, NV, shape, fac_of)) {
+            sep_pass_pipe<T, NV>(ps, shape, fac_of, aoff, reinterpret_cast<const T *>(src), reinterpret_cast<T *>(dst),
+                                 rank, nranks);
+            return;
+        }
+    }
     for (long t = rank; t < ps.ntiles; t += nranks) {
         const long to = t / ps.nchunks, tc = t - to * ps.nchunks;
         const long o0 = to * ps.OB, c0 = tc * ps.w;
         const int obv = (int)((ps.nout - o0) < ps.OB ? (ps.nout - o0) : ps.OB);
         const int wv = (int)((ps.rs - c0) < ps.w ? (ps.rs - c0) : ps.w);
         const long gbase = o0 * ps.R * ps.rs + c0;
-        sep_move<T, Ti, true>(ps, gbase, obv, wv, src, srcb, nullptr, nullptr);
+        const bool fast = sep_fast_move_ok(ps, obv, wv);
+        if (std::is_same<Ti, T>::value && !srcb && fast)
+            sep_move_fast<T, true>(ps, gbase, reinterpret_cast<const T *>(src), nullptr);
+        else
+            sep_move<T, Ti, true>(ps, gbase, obv, wv, src, srcb, nullptr, nullptr);
         __syncthreads();
         // the pass's axes, innermost first: axis a has tile stride w * prod m[a+1:hi]
         int st = ps.w;
@@ -283,7 +525,10 @@ __device__ __forceinline__ void sep_pass(const SepPass &ps, int d, int nc, const
             }
             st *= shape[a];
         }
-        sep_move<T, To, false>(ps, gbase, obv, wv, nullptr, nullptr, dst, dstb);
+        if (std::is_same<To, T>::value && !dstb && fast)
+            sep_move_fast<T, false>(ps, gbase, nullptr, reinterpret_cast<T *>(dst));
+        else
+            sep_move<T, To, false>(ps, gbase, obv, wv, nullptr, nullptr, dst, dstb);
         __syncthreads();
     }
 }

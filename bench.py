@@ -124,6 +124,7 @@ def large_fit_counts(n=10_000_000, seed=2):
 
 
 PAIR_BINS = 128
+QUICK_SWEEPS = 5               # sweeps of the companion fit
 
 
 def pair_companion(n=2_000_000, seed=2):
@@ -171,6 +172,51 @@ def cpu_baseline(fits, seconds=20.0):
                        f'{dt:.1f} s')
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                               'matvec_grid_kernel_32x5_sinc_traffic.md')
+
+
+def measured_traffic():
+    """HBM bytes per launch of the 32^5 mat-vec from its rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, each
+    scaled by the same counter's reading for a copy of known size), or None when the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith('traffic_bytes_per_launch:'):
+                    return float(line.split(':', 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+def large_matvec(torch, blml_amd, _FitSpec, FitBatch, params, event_time, reps=20):
+    """The 32^5 single fit's plan and its mat-vec timed with HIP events on the launch stream.  Returns
+    (spec, fit params, description, batch, roofline dict, seconds per mat-vec)."""
+    lc, lg, lkw = large_fit_counts()
+    lfs = _FitSpec(lc, modes_of(lg, blml_amd), lkw['h'], lkw['origin'], False, None, None, 0)
+    lparams = dict(params, solver='lbfgs', lbfgs_m=10, ll_tol=1e-10, max_sweeps=3000)
+    large = dict(bins=int(lfs.B), shape=list(LARGE_SHAPE), modes='sinc fc=0.12 cycles/bin per axis, h=0.25',
+                 counts=int(lc.sum()), solver='cd (real rank 1; the reference ignores lbfgs there)')
+    bl = FitBatch([lfs], lparams, np.float32, hist_cap=0).build()
+    El = torch.randn(lfs.B, device='cuda', dtype=torch.float32)
+    bl.matvec(El)
+    t_l = event_time(lambda: bl.matvec(El), reps)
+    alg_l = 2 * lfs.B * 4
+    roof = dict(bound='hbm', achieved=alg_l / t_l / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s', frac=alg_l / t_l / HBM_PEAK,
+                traffic=measured_traffic(), kernel='matvec_grid_kernel<float>', bytes_per_launch=alg_l,
+                us_per_launch=t_l * 1e6, workload='32^5 product-sinc mat-vec (separable tile passes, FP32)')
+    return lfs, lparams, large, bl, roof, t_l
+
+
+def copy_known_size(torch):
+    """A plain device copy of 256 MiB (its FETCH_SIZE / WRITE_SIZE readings scale the counters)."""
+    cp_src = torch.empty(64 << 20, dtype=torch.float32, device='cuda')
+    cp_dst = torch.empty_like(cp_src)
+    cp_dst.copy_(cp_src)
+    torch.cuda.synchronize()
+    del cp_src, cp_dst
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -179,6 +225,8 @@ def main():
     ap.add_argument('--fits', type=int, default=N_FITS)
     ap.add_argument('--no-fp64', action='store_true', help='skip the plain FP64 reference run')
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
+    ap.add_argument('--only', choices=['all', 'matvec'], default='all',
+                    help='matvec: only the 32^5 mat-vec (timed) and the 256 MiB copy, for counter runs')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (independent fits '
@@ -187,6 +235,23 @@ def main():
     import torch
     import blml_amd
     from blml_amd.binned import _FitSpec, FitBatch
+    if args.only == 'matvec':
+        st0 = torch.cuda.current_stream()
+
+        def ev_time(fn, reps):
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record(st0)
+            for _ in range(reps):
+                fn()
+            ev1.record(st0)
+            torch.cuda.synchronize()
+            return ev0.elapsed_time(ev1) / 1e3 / reps
+
+        *_, bl, roof, _ = large_matvec(torch, blml_amd, _FitSpec, FitBatch, fit_params(), ev_time, args.steps)
+        bl.close()
+        copy_known_size(torch)
+        print(json.dumps(dict(roofline=roof)), flush=True)
+        return
     fits = uneven_fits(args.fits)
     print(f'bench: {len(fits)} fits, {sum(c.size for c, *_ in fits)} bins', file=sys.stderr, flush=True)
     specs = [_FitSpec(c, modes_of(g, blml_amd), 1.0, 0.0, kw['complex_g'], None, None, 0) for c, g, kw, _ in fits]
@@ -295,19 +360,7 @@ def main():
                         bytes_per_launch=alg, us_per_launch=t_mv * 1e6)
     b32.close()
     # the large single fit: 32^5 bins spread over every CU; its Gram mat-vec is the memory-bound roofline
-    lc, lg, lkw = large_fit_counts()
-    lfs = _FitSpec(lc, modes_of(lg, blml_amd), lkw['h'], lkw['origin'], False, None, None, 0)
-    lparams = dict(params, solver='lbfgs', lbfgs_m=10, ll_tol=1e-10, max_sweeps=3000)
-    large = dict(bins=int(lfs.B), shape=list(LARGE_SHAPE), modes='sinc fc=0.12 cycles/bin per axis, h=0.25',
-                 counts=int(lc.sum()), solver='cd (real rank 1; the reference ignores lbfgs there)')
-    bl = FitBatch([lfs], lparams, np.float32, hist_cap=0).build()
-    El = torch.randn(lfs.B, device='cuda', dtype=torch.float32)
-    bl.matvec(El)
-    t_l = event_time(lambda: bl.matvec(El), 20)
-    alg_l = 2 * lfs.B * 4
-    roof = dict(bound='hbm', achieved=alg_l / t_l / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s', frac=alg_l / t_l / HBM_PEAK,
-                traffic=None, kernel='matvec_grid_kernel<float>', bytes_per_launch=alg_l, us_per_launch=t_l * 1e6,
-                workload='32^5 product-sinc mat-vec (separable tile passes, FP32)')
+    lfs, lparams, large, bl, roof, t_l = large_matvec(torch, blml_amd, _FitSpec, FitBatch, params, event_time)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     rl = bl.run().results()[0]
@@ -323,11 +376,7 @@ def main():
                      rel_loglik=abs(rl['loglik'] - rl64['loglik']) / abs(rl64['loglik']))
         bl64.close()
     # a plain device copy of known size (256 MiB), the scale of rocprof's FETCH_SIZE / WRITE_SIZE counters
-    cp_src = torch.empty(64 << 20, dtype=torch.float32, device='cuda')
-    cp_dst = torch.empty_like(cp_src)
-    cp_dst.copy_(cp_src)
-    torch.cuda.synchronize()
-    del cp_src, cp_dst
+    copy_known_size(torch)
     print(f'bench: large fit fp32 {large["fp32_seconds"]:.2f} s; mat-vec {t_l * 1e6:.1f} us', file=sys.stderr,
           flush=True)
     # the pair path: rows s_r = sum_j S(z_r, z_j) w_j over the companion's occupied bins (slits per column),
@@ -352,6 +401,21 @@ def main():
                  seconds=t_p, pairs_per_s=n_rows * len(Zp) / t_p)
     del keep
     print(f'bench: pair rows {n_rows} x {len(Zp)} in {t_p * 1e3:.1f} ms', file=sys.stderr, flush=True)
+    # the companion fit through the pair path: BLMLQuick on the lattice's counts (real rank 1, FP32 tables, block
+    # 4096), a capped number of sweeps -- a band-pass slits kernel (fill 0.5) leaves the real rank-1 objective
+    # without an interior optimum, so the fit runs to the cap and is scaled to e'Se = n^2
+    from blml_amd.quick import BLMLQuick
+    modes_p = [{j: dict(kind='slits', fc=0.2 / hp, fill=0.5, h=hp) for j in range(3)}]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    qf = BLMLQuick(tol=1e-4, max_sweeps=QUICK_SWEEPS, seed=0).fit(Zp * hp, modes_p, sample_weight=wp)
+    t_q = time.perf_counter() - t0
+    passes = qf.n_sweeps_ + 2                     # the sweeps, the first exact S e and the capped one
+    pairs['quick_fit'] = dict(estimator='BLMLQuick(tol=1e-4, block_size=4096, dtype=float32)', bins=int(qf.n_bins_),
+                              sweeps=int(qf.n_sweeps_), seconds=t_q, full_passes=passes,
+                              pairs_per_s=passes * float(qf.n_bins_) ** 2 / t_q, converged=bool(qf.converged_),
+                              residual_history=[float(r) for r in qf.rho_history_])
+    print(f'bench: companion fit {qf.n_sweeps_} sweeps in {t_q:.2f} s', file=sys.stderr, flush=True)
     cpu = cpu_baseline(fits, args.cpu_seconds)
     from oracle import blml_pairs as OP
     t0 = time.perf_counter()

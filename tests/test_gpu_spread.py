@@ -36,6 +36,10 @@ SPREAD_MATVEC = [
     ('3d sinc x dirichlet complex', (20, 16, 9), [dict(fc=[0.3, 0.25, 0.4], kinds=['sinc', 'dirichlet', 'sinc'],
                                                        periods=[None, 16.0, None])], 1.0, True),
     ('5d sinc 10^5', (10,) * 5, [dict(fc=[0.3] * 5, kinds=['sinc'] * 5)], 1.0, False),
+    # whole tiles and whole line blocks (the separable path's fast moves and full-block products)
+    ('4d sinc 32^4', (32,) * 4, [dict(fc=[0.3] * 4, kinds=['sinc'] * 4)], 1.0, False),
+    ('3d sinc 32^3 complex', (32,) * 3, [dict(fc=[0.3, 0.25, 0.35], kinds=['sinc'] * 3)], 1.0, True),
+    ('5d sinc 16^5', (16,) * 5, [dict(fc=[0.3] * 5, kinds=['sinc'] * 5)], 1.0, False),
 ]
 
 
