@@ -173,7 +173,7 @@ def cpu_baseline(fits, seconds=20.0):
 
 
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
-                               'matvec_grid_kernel_32x5_sinc_traffic.md')
+                               'matvec_grid_kernel_32x5_sinc_traffic.txt')
 
 
 def measured_traffic():
