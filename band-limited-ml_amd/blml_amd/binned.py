@@ -247,6 +247,17 @@ class FitBatch:
         self.d_rh = torch.zeros(max(1, self.sizes['hist']), dtype=f64, device=dev)
         return self
 
+    def refresh_init(self):
+        """Copies the specs' current inits to the device (the same layout as at build(): every spec keeps its
+        init_K, so only the values change; a warm-start refresh between launches)."""
+        inits = [fs.init.ravel() for fs in self.fits if fs.init is not None]
+        new = np.concatenate(inits) if inits else None
+        if (new is None) != (self.d_init is None) or (new is not None and new.size != self.d_init.numel()):
+            raise ValueError("refresh_init: the inits' layout changed since build()")
+        if new is not None:
+            self.d_init.copy_(_torch().from_numpy(new))
+        return self
+
     def info(self):
         lib = _lib.load()
         a, b = ctypes.c_int64(), ctypes.c_int64()

