@@ -32,6 +32,7 @@ struct SolverLds {
     double cR[BLML_CMAX * BLML_CMAX], cYY[BLML_CMAX * BLML_CMAX];
     double ca[BLML_CMAX + 1], cb[BLML_CMAX + 1], csg[BLML_CMAX + 1], cyg[BLML_CMAX + 1];
     double cp1[BLML_CMAX], cp2[BLML_CMAX];
+    double cu[BLML_CMAX], crc[BLML_CMAX + 1], cyc[BLML_CMAX + 1];   // thread 0's work arrays for the small systems
 };
 
 // The solver's LDS (reduction scratch, merit ring, L-BFGS scalars): a file-scope __shared__ object, so every access
@@ -99,6 +100,9 @@ struct FitSolver {
     int n_mv;
 
     __device__ double *vec(int i) const { return pool + (long)i * Vmax; }
+    // the compact L-BFGS form's history pairs, stored in FP32 in their FP64 slots (FP32 plans only): half the
+    // bytes of the two passes over the history per iteration; s.y and y.y are formed in FP64 before the rounding
+    __device__ float *hvec(int i) const { return reinterpret_cast<float *>(pool + (long)i * Vmax); }
     __device__ long V() const { return (long)K * B * nc; }
 
     // ------------------------------------------------------------------ mat-vecs
@@ -720,20 +724,21 @@ struct FitSolver {
 #pragma unroll
             for (int i = 0; i < 2 * BLML_CCHUNK; ++i) acc[i] = 0.0;
             const int nc_ = cnt - c0 < BLML_CCHUNK ? cnt - c0 : BLML_CCHUNK;
-            const double *Sp[BLML_CCHUNK], *Yp[BLML_CCHUNK];
+            const float *Sp[BLML_CCHUNK], *Yp[BLML_CCHUNK];
 #pragma unroll
             for (int i = 0; i < BLML_CCHUNK; ++i) {
-                const int sl_ = (head + c0 + (i < nc_ ? i : 0)) % ph;
-                Sp[i] = vec(9 + sl_);
-                Yp[i] = vec(9 + ph + sl_);
+                int sl_ = head + c0 + (i < nc_ ? i : 0);     // head < ph and c0 + i < ph: one wrap at most
+                if (sl_ >= ph) sl_ -= ph;
+                Sp[i] = hvec(9 + sl_);
+                Yp[i] = hvec(9 + ph + sl_);
             }
             for (long e = tm.first(); e < n_; e += tm.stride()) {
                 const double ge = gn[e];
 #pragma unroll
                 for (int i = 0; i < BLML_CCHUNK; ++i) {
                     if (i < nc_) {
-                        acc[i] += Sp[i][e] * ge;
-                        acc[BLML_CCHUNK + i] += Yp[i][e] * ge;
+                        acc[i] += (double)Sp[i][e] * ge;
+                        acc[BLML_CCHUNK + i] += (double)Yp[i][e] * ge;
                     }
                 }
             }
@@ -775,36 +780,73 @@ struct FitSolver {
             double *q = vec(iq);
             double slope = 0.0;
             if (cnt > 0) {
-                // small systems, solved redundantly by every thread from LDS
-                double u[BLML_CMAX], w[BLML_CMAX];
-                for (int i = cnt - 1; i >= 0; --i) {            // R u = a (upper triangular)
-                    double v = g_sl.ca[i];
-                    for (int j = i + 1; j < cnt; ++j) v -= g_sl.cR[i * BLML_CMAX + j] * u[j];
-                    u[i] = v / g_sl.cR[i * BLML_CMAX + i];
+                // the small systems, solved by thread 0 with its work arrays in LDS (arrays indexed at run time
+                // in private memory would live in scratch); p1 = w and p2 = -gamma u go to LDS for everyone
+                if (threadIdx.x == 0) {
+                    double *u = g_sl.cu, *w = g_sl.cp1;
+                    for (int i = cnt - 1; i >= 0; --i) {            // R u = a (upper triangular)
+                        double v = g_sl.ca[i];
+                        for (int j = i + 1; j < cnt; ++j) v -= g_sl.cR[i * BLML_CMAX + j] * u[j];
+                        u[i] = v / g_sl.cR[i * BLML_CMAX + i];
+                    }
+                    for (int i = 0; i < cnt; ++i) {                  // w = (D + gamma Y^T Y) u - gamma b
+                        double v = g_sl.cR[i * BLML_CMAX + i] * u[i] - gam * g_sl.cb[i];
+                        for (int j = 0; j < cnt; ++j) v += gam * g_sl.cYY[i * BLML_CMAX + j] * u[j];
+                        w[i] = v;
+                    }
+                    for (int i = 0; i < cnt; ++i) {                  // R^T p1 = w (lower triangular)
+                        double v = w[i];
+                        for (int j = 0; j < i; ++j) v -= g_sl.cR[j * BLML_CMAX + i] * w[j];
+                        w[i] = v / g_sl.cR[i * BLML_CMAX + i];
+                    }
+                    for (int i = 0; i < cnt; ++i) g_sl.cp2[i] = -gam * u[i];
                 }
-                for (int i = 0; i < cnt; ++i) {                  // w = (D + gamma Y^T Y) u - gamma b
-                    double v = g_sl.cR[i * BLML_CMAX + i] * u[i] - gam * g_sl.cb[i];
-                    for (int j = 0; j < cnt; ++j) v += gam * g_sl.cYY[i * BLML_CMAX + j] * u[j];
-                    w[i] = v;
-                }
-                for (int i = 0; i < cnt; ++i) {                  // R^T p1 = w (lower triangular)
-                    double v = w[i];
-                    for (int j = 0; j < i; ++j) v -= g_sl.cR[j * BLML_CMAX + i] * w[j];
-                    w[i] = v / g_sl.cR[i * BLML_CMAX + i];
-                }
-                if (threadIdx.x == 0)
-                    for (int i = 0; i < cnt; ++i) { g_sl.cp1[i] = w[i]; g_sl.cp2[i] = -gam * u[i]; }
                 __syncthreads();
+                // q = gam g + sum_i (p1_i s_i + p2_i y_i), slope = g . q: UE elements per thread and the pairs in
+                // chunks of BLML_CCHUNK, so 2 BLML_CCHUNK UE loads are in flight (the additions in pair order)
+                constexpr int UE = 2;
                 double acc = 0.0;
                 const double *g = vec(ig);
-                for (long e = tm.first(); e < n_; e += tm.stride()) {
-                    double v = gam * g[e];
-                    for (int i = 0; i < cnt; ++i) {
-                        const int sl_ = (head + i) % ph;
-                        v += g_sl.cp1[i] * vec(9 + sl_)[e] + g_sl.cp2[i] * vec(9 + ph + sl_)[e];
+                const long stp = tm.stride();
+                for (long e0 = tm.first(); e0 < n_; e0 += UE * stp) {
+                    double v[UE], gv[UE];
+#pragma unroll
+                    for (int u = 0; u < UE; ++u) {
+                        const long e = e0 + u * stp;
+                        gv[u] = e < n_ ? g[e] : 0.0;
+                        v[u] = gam * gv[u];
                     }
-                    q[e] = v;
-                    acc += g[e] * v;
+                    for (int c0 = 0; c0 < cnt; c0 += BLML_CCHUNK) {
+                        const int nc_ = cnt - c0 < BLML_CCHUNK ? cnt - c0 : BLML_CCHUNK;
+                        const float *Sp[BLML_CCHUNK], *Yp[BLML_CCHUNK];
+                        double a1[BLML_CCHUNK], a2[BLML_CCHUNK];
+#pragma unroll
+                        for (int i = 0; i < BLML_CCHUNK; ++i) {
+                            int sl_ = head + c0 + (i < nc_ ? i : 0);
+                            if (sl_ >= ph) sl_ -= ph;
+                            Sp[i] = hvec(9 + sl_);
+                            Yp[i] = hvec(9 + ph + sl_);
+                            a1[i] = g_sl.cp1[c0 + (i < nc_ ? i : 0)];
+                            a2[i] = g_sl.cp2[c0 + (i < nc_ ? i : 0)];
+                        }
+#pragma unroll
+                        for (int u = 0; u < UE; ++u) {
+                            const long e = e0 + u * stp;
+                            if (e < n_) {
+#pragma unroll
+                                for (int i = 0; i < BLML_CCHUNK; ++i)
+                                    if (i < nc_) v[u] += a1[i] * (double)Sp[i][e] + a2[i] * (double)Yp[i][e];
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < UE; ++u) {
+                        const long e = e0 + u * stp;
+                        if (e < n_) {
+                            q[e] = v[u];
+                            acc += gv[u] * v[u];
+                        }
+                    }
                 }
                 slope = tm.sum1(acc, g_sl.red);
             }
@@ -852,13 +894,13 @@ struct FitSolver {
             const int slot = (head + cnt) % ph;
             double a7[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
             {
-                double *Sv = vec(9 + slot), *Yv = vec(9 + ph + slot);
+                float *Sv = hvec(9 + slot), *Yv = hvec(9 + ph + slot);
                 const double *En = vec(iEn), *Ec = vec(iE), *g = vec(ig), *gn = vec(ign);
                 for (long e = tm.first(); e < n_; e += tm.stride()) {
                     const double sv = En[e] - Ec[e];
                     const double yv = g[e] - gn[e];
-                    Sv[e] = sv;
-                    Yv[e] = yv;
+                    Sv[e] = (float)sv;
+                    Yv[e] = (float)yv;
                     a7[0] += sv * yv;
                     a7[1] += sv * sv;
                     a7[2] += yv * yv;
@@ -873,7 +915,7 @@ struct FitSolver {
             const bool accept = a7[0] > 1e-12 * sqrt(a7[1] * a7[2]);
             if (threadIdx.x == 0) {
                 // R's new column and Y^T Y's new row / column for the kept pairs, from the previous a, b
-                double rc[BLML_CMAX + 1], yc[BLML_CMAX + 1];
+                double *rc = g_sl.crc, *yc = g_sl.cyc;
                 for (int i = 0; i < cnt; ++i) {
                     rc[i] = g_sl.ca[i] - g_sl.csg[i];
                     yc[i] = g_sl.cb[i] - g_sl.cyg[i];
