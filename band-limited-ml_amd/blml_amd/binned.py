@@ -263,7 +263,10 @@ class FitBatch:
         a, b = ctypes.c_int64(), ctypes.c_int64()
         c, d = ctypes.c_int32(), ctypes.c_int32()
         _lib.check(lib.blml_plan_info(self.plan, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
-        return dict(out_elems=a.value, ws_bytes=b.value, lds_bytes=c.value, grid=d.value)
+        k = ctypes.c_int32()
+        _lib.check(lib.blml_plan_matvec_kernel(self.plan, ctypes.byref(k)))
+        return dict(out_elems=a.value, ws_bytes=b.value, lds_bytes=c.value, grid=d.value,
+                    matvec_kernel=('matvec_kernel', 'matvec_grid_kernel', 'matvec_pipe_kernel')[k.value])
 
     def stream(self):
         return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)

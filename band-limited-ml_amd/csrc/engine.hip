@@ -190,6 +190,44 @@ __global__ void __launch_bounds__(BLML_NT) matvec_grid_kernel(DevPlan<T> p, unsi
     run_matvec<T>(p, 0, Team{(int)blockIdx.x, (int)gridDim.x, bar, part, 0}, E, U, p.sep);
 }
 
+// The stand-alone mat-vec of a separable spread plan whose every pass is a pipelined whole-tile pass (the host
+// checks it): only that path is compiled in, so the kernel's registers serve it alone and nothing is saved to
+// scratch.  Passes of a mode are separated by grid barriers, modes too.
+template <typename T>
+__global__ void __launch_bounds__(BLML_NT) matvec_pipe_kernel(DevPlan<T> p, unsigned *bar, double *part, const T *E,
+                                                             T *U) {
+    const Team tm{(int)blockIdx.x, (int)gridDim.x, bar, part, 0};
+    const blml_fit_t *fp = p.fits;                     // read in place: a local copy indexed by k would be scratch
+    const int nc = fp->complex_g ? 2 : 1;
+    const int K = fp->K;
+    const long B = fp->B;
+    const SepPlan *sp = p.sep;
+    T *Z = reinterpret_cast<T *>(p.ws + fp->ws_off);
+    int shape[BLML_MAXD];
+#pragma unroll
+    for (int a = 0; a < BLML_MAXD; ++a) shape[a] = fp->shape[a];
+    for (int k = 0; k < K; ++k) {
+        const long off = fp->out_off + (long)k * B * nc;
+        const int fb = fp->fac_begin[k], fcnt = fp->fac_count[k];
+        int fac_of[BLML_MAXD];                          // indexed by constants only
+#pragma unroll
+        for (int a = 0; a < BLML_MAXD; ++a) {
+            int fa = -1;
+            for (int j = 0; j < fcnt; ++j)
+                if (p.facs[fb + j].axes[0] == a) fa = fb + j;
+            fac_of[a] = fa;
+        }
+        const int np = sp->npass;
+        for (int q = 0; q < np; ++q) {
+            const SepPass ps = sp->pass[q];
+            const T *src = q == 0 ? E + off : Z;
+            T *dst = q == np - 1 ? U + off : Z;
+            sep_pass_pipe_only<T>(ps, shape, fac_of, p.sep_tab, p.sep_off, src, dst, tm.rank, tm.nranks);
+            if (q < np - 1 || k < K - 1) tm.sync();
+        }
+    }
+}
+
 // One whole fit (start, solve, outputs), run by a team.
 template <typename T>
 __device__ void run_fit(const DevPlan<T> &p, int fi, const Team &team, const double *counts, const double *draws,
@@ -569,6 +607,7 @@ struct Plan {
     double *d_part = nullptr;   // team partial sums (2 * coop * BLML_NRED)
     bool sep = false;           // spread plan of separable modes: tile passes with dense line operators
     SepPlan *d_sep = nullptr;   // [0] mat-vec kernel, [1] fit kernel
+    bool pipe_mv = false;       // the stand-alone mat-vec takes matvec_pipe_kernel (every pass pipelined)
     void *d_sep_tab = nullptr;
     long *d_sep_off = nullptr;
     int lds_mv = 0, lds_fit = 0;   // dynamic LDS of the spread mat-vec and fit launches
@@ -897,6 +936,27 @@ int static_lds(const void *kern, int &st) {
     return BLML_OK;
 }
 
+// Whether every pass of a mat-vec plan takes the pipelined whole-tile path (the device's sep_pipe_ok, asked of every
+// axis of the lattice): then the stand-alone mat-vec runs matvec_pipe_kernel, which compiles in that path alone.
+template <typename T>
+bool sep_pipe_host(const SepPlan &sp, const int *shape) {
+    constexpr int BLK = SepMfma<T>::BLK;
+    const long NV = sizeof(T) == 4 ? 64 : 32;
+    for (int q = 0; q < sp.npass; ++q) {
+        const SepPass &ps = sp.pass[q];
+        const long tsize = (long)ps.OB * ps.R * ps.w;
+        if (tsize != NV * BLML_NT || ps.nout % ps.OB || ps.rs % ps.w) return false;
+        if (!(ps.w == ps.rs || (ps.w <= BLML_NT && BLML_NT % ps.w == 0))) return false;
+        if (tsize % ((long)BLML_SEP_FULL_NH * BLK * BLK)) return false;
+        long st = ps.w;
+        for (int a = ps.hi - 1; a >= ps.lo; --a) {
+            if (shape[a] > 1 && !(shape[a] == BLK && (st == 1 || st % 32 == 0))) return false;
+            st *= shape[a];
+        }
+    }
+    return true;
+}
+
 // A spread plan whose modes are all LINE factors on axes of at most SepMfma<T>::MMAX points takes the separable
 // path: the factors' dense tables on the device and two pass plans sized for the LDS the mat-vec and fit kernels
 // leave free.
@@ -936,6 +996,10 @@ int build_sep(Plan &pl, const double *fdata, const int32_t *freqs, int max_lds) 
     pl.lds_mv = std::max(pl.lds_bytes, (int)((sp[0].tile + 2) * es));
     pl.lds_fit = std::max(pl.lds_bytes, (int)((sp[1].tile + 2) * es));
     pl.sep = true;
+    int st_pipe = 0;
+    rc = static_lds((const void *)matvec_pipe_kernel<T>, st_pipe);
+    if (rc) return rc;
+    pl.pipe_mv = st_pipe <= st_mv && sep_pipe_host<T>(sp[0], f.shape);
     return BLML_OK;
 }
 
@@ -1057,6 +1121,11 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, fit_grid_kernel<T>, BLML_NT, pl.lds_fit));
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, matvec_grid_kernel<T>, BLML_NT, pl.lds_mv));
             const int oc2 = std::min(std::min(a, b), oc);
+            if (pl.pipe_mv) {
+                int c = 0;
+                HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, matvec_pipe_kernel<T>, BLML_NT, pl.lds_mv));
+                pl.pipe_mv = c >= std::max(oc2, 1);
+            }
             if (oc2 < 1) return fail(BLML_ERUNTIME, "no workgroup of the separable spread kernels fits a CU");
             if (oc2 < oc) {
                 pl.coop = ncu * oc2;
@@ -1131,6 +1200,13 @@ int blml_plan_info(void *plan, int64_t *out_elems, int64_t *ws_bytes, int32_t *l
     return BLML_OK;
 }
 
+int blml_plan_matvec_kernel(void *plan, int32_t *which) {
+    Plan *pl = (Plan *)plan;
+    if (!pl || !which) return fail(BLML_EVALUE, "null argument");
+    *which = pl->spread ? (pl->pipe_mv ? 2 : 1) : 0;
+    return BLML_OK;
+}
+
 int blml_matvec(void *plan, const void *E, void *U, void *stream) {
     Plan *pl = (Plan *)plan;
     if (!pl || !E || !U) return fail(BLML_EVALUE, "null argument");
@@ -1139,11 +1215,13 @@ int blml_matvec(void *plan, const void *E, void *U, void *stream) {
         if (pl->dtype == BLML_F32) {
             DevPlan<float> dp = dev_plan<float>(*pl);
             void *args[] = {&dp, &pl->d_bar, &pl->d_part, &E, &U};
-            return launch_coop(*pl, (const void *)matvec_grid_kernel<float>, args, pl->lds_mv, st);
+            return launch_coop(*pl, pl->pipe_mv ? (const void *)matvec_pipe_kernel<float>
+                                                : (const void *)matvec_grid_kernel<float>, args, pl->lds_mv, st);
         }
         DevPlan<double> dp = dev_plan<double>(*pl);
         void *args[] = {&dp, &pl->d_bar, &pl->d_part, &E, &U};
-        return launch_coop(*pl, (const void *)matvec_grid_kernel<double>, args, pl->lds_mv, st);
+        return launch_coop(*pl, pl->pipe_mv ? (const void *)matvec_pipe_kernel<double>
+                                            : (const void *)matvec_grid_kernel<double>, args, pl->lds_mv, st);
     }
     if (pl->dtype == BLML_F32)
         matvec_kernel<float><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<float>(*pl), (const float *)E,

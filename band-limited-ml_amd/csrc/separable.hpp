@@ -161,9 +161,7 @@ __device__ __forceinline__ void sep_axis_mfma_full(int nvals, int st, int aoff) 
     const int span = BLK * st;
     const int stp = st == 1 ? 1 : st + (st >> 5);
     const float inv_st = 1.0f / (float)st;
-    int rowoff[F::NACC];
-#pragma unroll
-    for (int r = 0; r < F::NACC; ++r) rowoff[r] = F::row(r, lane) * stp;
+    const int lane_row = F::row(0, lane) * stp;     // row(r, lane) = row(r, 0) + row(0, lane): a uniform part per r
     for (int b0 = NH * wave; b0 < nblk; b0 += NH * BLML_NWAVE) {
         int lb[NH];
 #pragma unroll
@@ -189,7 +187,7 @@ __device__ __forceinline__ void sep_axis_mfma_full(int nvals, int st, int aoff) 
 #pragma unroll
         for (int h = 0; h < NH; ++h)
 #pragma unroll
-            for (int r = 0; r < F::NACC; ++r) tile[lb[h] + rowoff[r]] = acc[h][r];
+            for (int r = 0; r < F::NACC; ++r) tile[lb[h] + lane_row + F::row(r, 0) * stp] = acc[h][r];
     }
 }
 
@@ -338,26 +336,23 @@ __device__ __forceinline__ void sep_pass_pipe(const SepPass &ps, const int *shap
                                               const T *src, T *dst, int rank, int nranks) {
     T *tile = reinterpret_cast<T *>(g_dyn_lds);
     const int t = threadIdx.x;
-    // the pass's axes (innermost first): stride and staged-table offset, held in registers so that nothing inside
-    // the tile loop reads memory other than the tiles (a read there would wait for the prefetched loads)
-    int nax = 0, ast[BLML_MAXD], aao[BLML_MAXD];
+    // the pass's axes: per axis (indexed by constants only, so the arrays stay in registers) whether it has an
+    // operator, its tile stride and its staged-table offset; nothing inside the tile loop reads memory other than
+    // the tiles (a read there would wait for the prefetched loads)
+    bool act[BLML_MAXD];
+    int ast[BLML_MAXD], aao[BLML_MAXD];
     {
-        int st = ps.w;
-        for (int a = ps.hi - 1; a >= ps.lo; --a) {
-            if (fac_of[a] >= 0 && shape[a] > 1) {
-                ast[nax] = st;
-                aao[nax] = aoff[a];
-                ++nax;
-            }
-            st *= shape[a];
+        const int lo = __builtin_amdgcn_readfirstlane(ps.lo), hi = __builtin_amdgcn_readfirstlane(ps.hi);
+        int st = __builtin_amdgcn_readfirstlane(ps.w);
+#pragma unroll
+        for (int a = BLML_MAXD - 1; a >= 0; --a) {
+            const bool in = a >= lo && a < hi;
+            act[a] = in && fac_of[a] >= 0 && shape[a] > 1;
+            ast[a] = __builtin_amdgcn_readfirstlane(st);
+            aao[a] = __builtin_amdgcn_readfirstlane(act[a] ? aoff[a] : 0);
+            if (in) st *= shape[a];
         }
     }
-#pragma unroll
-    for (int k = 0; k < BLML_MAXD; ++k) {
-        ast[k] = __builtin_amdgcn_readfirstlane(k < nax ? ast[k] : 1);
-        aao[k] = __builtin_amdgcn_readfirstlane(k < nax ? aao[k] : 0);
-    }
-    nax = __builtin_amdgcn_readfirstlane(nax);
     const long rs = ((long)__builtin_amdgcn_readfirstlane((int)(ps.rs >> 32)) << 32) |
                     (unsigned)__builtin_amdgcn_readfirstlane((int)ps.rs);
     const int w = __builtin_amdgcn_readfirstlane(ps.w);
@@ -398,9 +393,9 @@ __device__ __forceinline__ void sep_pass_pipe(const SepPass &ps, const int *shap
             }
         }
 #pragma unroll
-        for (int k = 0; k < BLML_MAXD; ++k) {
-            if (k < nax) {
-                sep_axis_mfma_full<T>(tsize, ast[k], aao[k]);
+        for (int a = BLML_MAXD - 1; a >= 0; --a) {          // innermost axis first
+            if (act[a]) {
+                sep_axis_mfma_full<T>(tsize, ast[a], aao[a]);
                 __syncthreads();
             }
         }
@@ -439,6 +434,87 @@ __device__ __forceinline__ bool sep_pipe_ok(const SepPass &ps, int nv, const int
     return true;
 }
 
+// Stages the operator tables of a pass's axes in LDS after the tile and its dump slots (one batched loop over all
+// of them: the loads of a batch are in flight together) and returns each axis's table offset in aoff (-1: none).
+template <typename T>
+__device__ __forceinline__ void sep_stage_tables(const SepPass &ps, const int *shape, const int *fac_of, const T *tabs,
+                                                 const long *tab_off, int *aoff) {
+    const int tsize = ps.OB * ps.R * ps.w;
+    int tdim[BLML_MAXD], tcum[BLML_MAXD + 1];
+    const T *tsrc[BLML_MAXD];
+    int nst = 0;
+    int o = sep_lds(tsize) + 1 + 64;
+    tcum[0] = 0;
+    for (int a = ps.lo; a < ps.hi; ++a) {
+        aoff[a] = -1;
+        if (fac_of[a] < 0 || shape[a] < 2) continue;
+        const int P = sep_table_dim(shape[a], SepMfma<T>::BLK);
+        aoff[a] = o;
+        tdim[nst] = P;
+        tsrc[nst] = tabs + tab_off[fac_of[a]];
+        tcum[nst + 1] = tcum[nst] + P * P;
+        ++nst;
+        o += sep_stage_len<T>(shape[a]);
+    }
+    T *lds = reinterpret_cast<T *>(g_dyn_lds);
+    constexpr int SB = 8;
+    for (int j0 = threadIdx.x; j0 < tcum[nst]; j0 += SB * BLML_NT) {
+        T v[SB];
+        int dsto[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const int j = j0 + u * BLML_NT;
+            dsto[u] = -1;
+            v[u] = (T)0;
+            if (j < tcum[nst]) {
+                int q = 0, ao = sep_lds(tsize) + 1 + 64;
+                while (j >= tcum[q + 1]) {
+                    ao += tdim[q] * (tdim[q] + 1);
+                    ++q;
+                }
+                const int i = j - tcum[q], P = tdim[q];
+                v[u] = tsrc[q][i];
+                dsto[u] = ao + (i / P) * (P + 1) + i % P;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u)
+            if (dsto[u] >= 0) lds[dsto[u]] = v[u];
+    }
+    __syncthreads();
+}
+
+// A pass known to take the pipelined whole-tile path (the stand-alone mat-vec kernel's host checks every pass).
+// shape and fac_of are arrays the caller indexes by constants only; the tables are staged axis by axis in the
+// generic staging's layout (ascending axes, rows of P + 1), with every per-axis array indexed by constants, so
+// nothing here lives in scratch memory.
+template <typename T>
+__device__ __forceinline__ void sep_pass_pipe_only(const SepPass &ps, const int *shape, const int *fac_of,
+                                                   const T *tabs, const long *tab_off, const T *src, T *dst, int rank,
+                                                   int nranks) {
+    int aoff[BLML_MAXD];
+    {
+        T *lds = reinterpret_cast<T *>(g_dyn_lds);
+        const int lo = __builtin_amdgcn_readfirstlane(ps.lo), hi = __builtin_amdgcn_readfirstlane(ps.hi);
+        int o = sep_lds(ps.OB * ps.R * ps.w) + 1 + 64;
+#pragma unroll
+        for (int a = 0; a < BLML_MAXD; ++a) {
+            aoff[a] = -1;
+            if (a >= lo && a < hi && fac_of[a] >= 0 && shape[a] > 1) {
+                const int P = sep_table_dim(shape[a], SepMfma<T>::BLK);
+                const int lp = 31 - __clz(P);
+                const T *tsrc = tabs + tab_off[fac_of[a]];
+                aoff[a] = o;
+                for (int i = threadIdx.x; i < P * P; i += BLML_NT) lds[o + (i >> lp) * (P + 1) + (i & (P - 1))] = tsrc[i];
+                o += P * (P + 1);
+            }
+        }
+        __syncthreads();
+    }
+    constexpr int NV = sizeof(T) == 4 ? 64 : 32;          // 128 KB tiles
+    sep_pass_pipe<T, NV>(ps, shape, fac_of, aoff, src, dst, rank, nranks);
+}
+
 // One pass of a separable mode: dst = (prod over the pass's axes of M_a) src.  The first pass may subtract a base
 // from its input, the last may add a base to its output (the incremental form U = Ubase + S (E - Ebase)); values
 // are converted Ti -> T on load and T -> To on store (the base arithmetic in FP64).  Tiles are dealt to the team's
@@ -450,52 +526,8 @@ __device__ __forceinline__ void sep_pass(const SepPass &ps, int d, int nc, const
     (void)nc;
     (void)d;
     const int tsize = ps.OB * ps.R * ps.w;
-    // the operator tables of the pass's axes, staged once in LDS after the tile and its dump slots (one batched
-    // loop over all of them: the loads of a batch are in flight together)
-    int aoff[BLML_MAXD], tdim[BLML_MAXD], tcum[BLML_MAXD + 1];
-    const T *tsrc[BLML_MAXD];
-    int nst = 0;
-    {
-        int o = sep_lds(tsize) + 1 + 64;
-        tcum[0] = 0;
-        for (int a = ps.lo; a < ps.hi; ++a) {
-            aoff[a] = -1;
-            if (fac_of[a] < 0 || shape[a] < 2) continue;
-            const int P = sep_table_dim(shape[a], SepMfma<T>::BLK);
-            aoff[a] = o;
-            tdim[nst] = P;
-            tsrc[nst] = tabs + tab_off[fac_of[a]];
-            tcum[nst + 1] = tcum[nst] + P * P;
-            ++nst;
-            o += sep_stage_len<T>(shape[a]);
-        }
-        T *lds = reinterpret_cast<T *>(g_dyn_lds);
-        constexpr int SB = 8;
-        for (int j0 = threadIdx.x; j0 < tcum[nst]; j0 += SB * BLML_NT) {
-            T v[SB];
-            int dsto[SB];
-#pragma unroll
-            for (int u = 0; u < SB; ++u) {
-                const int j = j0 + u * BLML_NT;
-                dsto[u] = -1;
-                v[u] = (T)0;
-                if (j < tcum[nst]) {
-                    int q = 0, ao = sep_lds(tsize) + 1 + 64;
-                    while (j >= tcum[q + 1]) {
-                        ao += tdim[q] * (tdim[q] + 1);
-                        ++q;
-                    }
-                    const int i = j - tcum[q], P = tdim[q];
-                    v[u] = tsrc[q][i];
-                    dsto[u] = ao + (i / P) * (P + 1) + i % P;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < SB; ++u)
-                if (dsto[u] >= 0) lds[dsto[u]] = v[u];
-        }
-        __syncthreads();
-    }
+    int aoff[BLML_MAXD];
+    sep_stage_tables<T>(ps, shape, fac_of, tabs, tab_off, aoff);
     if (std::is_same<Ti, T>::value && std::is_same<To, T>::value && !srcb && !dstb) {
         constexpr int NV = sizeof(T) == 4 ? 64 : 32;          // 128 KB tiles
         if (sep_pipe_ok<T>(ps, NV, shape, fac_of)) {

@@ -423,11 +423,10 @@ struct MatvecCtx {
     const long *sep_off;
 };
 
-// U_k = S_k E_k of a separable mode by tile passes (see separable.hpp); the intermediate lives in c.X.  The one call
-// boundary of the separable path (its tile loops are inlined, so no registers are saved per tile or axis).
+// U_k = S_k E_k of a separable mode by tile passes (see separable.hpp); the intermediate lives in c.X.
 template <typename T, typename Ti, typename To>
-__device__ __noinline__ void matvec_sep(const MatvecCtx<T> &c, const int fac_begin, const int fac_count, const Ti *E, To *U,
-                           int nc, const Ti *Ebase, const To *Ubase) {
+__device__ __forceinline__ void matvec_sep_body(const MatvecCtx<T> &c, const int fac_begin, const int fac_count,
+                                                const Ti *E, To *U, int nc, const Ti *Ebase, const To *Ubase) {
     int fac_of[BLML_MAXD];
     for (int a = 0; a < BLML_MAXD; ++a) fac_of[a] = -1;
     for (int f = 0; f < fac_count; ++f) fac_of[c.facs[fac_begin + f].axes[0]] = fac_begin + f;
@@ -450,6 +449,14 @@ __device__ __noinline__ void matvec_sep(const MatvecCtx<T> &c, const int fac_beg
                               (const T *)nullptr, c.tm.rank, c.tm.nranks);
         if (!last || c.sep_last_sync) c.tm.sync();
     }
+}
+
+// The call boundary of the separable path inside the solver kernels (its tile loops are inlined, so no registers are
+// saved per tile or axis; the stand-alone mat-vec kernel inlines the body itself and saves none at all).
+template <typename T, typename Ti, typename To>
+__device__ __noinline__ void matvec_sep(const MatvecCtx<T> &c, const int fac_begin, const int fac_count, const Ti *E,
+                                        To *U, int nc, const Ti *Ebase, const To *Ubase) {
+    matvec_sep_body<T, Ti, To>(c, fac_begin, fac_count, E, U, nc, Ebase, Ubase);
 }
 
 template <typename T>

@@ -172,15 +172,16 @@ def cpu_baseline(fits, seconds=20.0):
                        f'{dt:.1f} s')
 
 
-TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
-                               'matvec_grid_kernel_32x5_sinc_traffic.txt')
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles')
 
 
-def measured_traffic():
-    """HBM bytes per launch of the 32^5 mat-vec from its rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, each
-    scaled by the same counter's reading for a copy of known size), or None when the summary is absent."""
+def measured_traffic(kernel):
+    """HBM bytes per launch of the 32^5 mat-vec's kernel from its rocprofv3 counter summary
+    (profiles/<kernel>_32x5_sinc_traffic.txt: FETCH_SIZE + WRITE_SIZE, each scaled by the same counter's reading for
+    a copy of known size), or None when no summary of that kernel exists."""
+    name = kernel.split('<')[0]
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(os.path.join(PROFILES, f'{name}_32x5_sinc_traffic.txt')) as f:
             for line in f:
                 if line.startswith('traffic_bytes_per_launch:'):
                     return float(line.split(':', 1)[1])
@@ -202,8 +203,9 @@ def large_matvec(torch, blml_amd, _FitSpec, FitBatch, params, event_time, reps=2
     bl.matvec(El)
     t_l = event_time(lambda: bl.matvec(El), reps)
     alg_l = 2 * lfs.B * 4
+    kname = bl.info()['matvec_kernel'] + '<float>'
     roof = dict(bound='hbm', achieved=alg_l / t_l / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s', frac=alg_l / t_l / HBM_PEAK,
-                traffic=measured_traffic(), kernel='matvec_grid_kernel<float>', bytes_per_launch=alg_l,
+                traffic=measured_traffic(kname), kernel=kname, bytes_per_launch=alg_l,
                 us_per_launch=t_l * 1e6, workload='32^5 product-sinc mat-vec (separable tile passes, FP32)')
     return lfs, lparams, large, bl, roof, t_l
 

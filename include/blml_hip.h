@@ -210,6 +210,11 @@ int blml_quick_exact(const blml_pairtab_t *tab, const int32_t *Z, int64_t B, con
 int blml_quick_stats(const double *e, const double *Se, const double *w, int64_t B, double n, double *out,
                      void *stream);
 
+/* Which kernel blml_matvec launches for the plan (reporting; no counterpart in the reference): 0 matvec_kernel (one
+ * workgroup per fit), 1 matvec_grid_kernel (one fit over the device), 2 matvec_pipe_kernel (one separable fit whose
+ * every tile pass is pipelined). */
+int blml_plan_matvec_kernel(void *plan, int32_t *which);
+
 /* Sizes the plan chose (for the caller's buffers and for reporting).  grid: workgroups of the batched launch, or
  * minus the workgroup count when the plan's one fit is spread over the device. */
 int blml_plan_info(void *plan, int64_t *out_elems, int64_t *ws_bytes, int32_t *lds_bytes, int32_t *grid);

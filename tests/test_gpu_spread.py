@@ -40,7 +40,31 @@ SPREAD_MATVEC = [
     ('4d sinc 32^4', (32,) * 4, [dict(fc=[0.3] * 4, kinds=['sinc'] * 4)], 1.0, False),
     ('3d sinc 32^3 complex', (32,) * 3, [dict(fc=[0.3, 0.25, 0.35], kinds=['sinc'] * 3)], 1.0, True),
     ('5d sinc 16^5', (16,) * 5, [dict(fc=[0.3] * 5, kinds=['sinc'] * 5)], 1.0, False),
+    ('3d sinc 32^3 (one pipelined pass)', (32,) * 3, [dict(fc=[0.3, 0.2, 0.35], kinds=['sinc'] * 3)], 1.0, False),
 ]
+
+
+def test_spread_matvec_32x5_fp32_matches_fp64():
+    """The 32^5 product-sinc mat-vec (the FP32 plan takes the pipelined whole-tile kernel) against the FP64 plan's
+    (general separable path, pinned by the oracle at the sizes above): 2e-6 of max |S e| times log2 B."""
+    import torch
+    import blml_amd
+    shape = (32,) * 5
+    counts = np.ones(shape)
+    mm = [dict(fc=[0.48] * 5, kinds=['sinc'] * 5)]
+    _note('spread matvec 32^5 fp32 vs fp64')
+    fs32, b32 = _one_fit_batch(counts, _modes(mm, blml_amd), 0.25, False, np.float32, 1)
+    fs64, b64 = _one_fit_batch(counts, _modes(mm, blml_amd), 0.25, False, np.float64, 1)
+    g = torch.Generator(device='cuda').manual_seed(5)
+    E = torch.randn(fs32.B, device='cuda', dtype=torch.float64, generator=g)
+    U32 = b32.matvec(E.float()).double()
+    U64 = b64.matvec(E)
+    err = float((U32 - U64).abs().max())
+    scale = float(U64.abs().max())
+    _note(f'spread matvec 32^5: max err {err:.3e} of {scale:.3e}')
+    assert err < 2e-6 * scale * np.log2(fs32.B)
+    b32.close()
+    b64.close()
 
 
 def _one_fit_batch(counts, modes, h, cplx, dtype, spread, params=PARAMS):
