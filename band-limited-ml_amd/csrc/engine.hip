@@ -414,11 +414,14 @@ __global__ void __launch_bounds__(BLML_NT) fit_kernel(DevPlan<T> p, const int *o
     for (;;) {
         if (threadIdx.x == 0) s_fit = atomicAdd(queue, 1);
         __syncthreads();
-        const int qi = s_fit;
+        // the fit's index is uniform: readfirstlane lets the compiler load its table row with scalar loads and
+        // keep everything derived from it in scalar registers (an LDS or vector-loaded value counts as divergent)
+        const int qi = __builtin_amdgcn_readfirstlane(s_fit);
         __syncthreads();
         if (qi >= p.nfits) break;
-        run_fit<T>(p, order[qi], block_team(), counts, draws, init, signs, e_out, se_out, density, results,
-                   merit_hist, rho_hist, tick_hz);
+        const int fi = __builtin_amdgcn_readfirstlane(order[qi]);
+        run_fit<T>(p, fi, block_team(), counts, draws, init, signs, e_out, se_out, density, results, merit_hist,
+                   rho_hist, tick_hz);
     }
 }
 
