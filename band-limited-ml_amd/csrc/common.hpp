@@ -112,25 +112,32 @@ __device__ __forceinline__ double block_max1(double v, double *red) { return blo
 
 // bar[0]: arrivals, bar[1]: generation, bar[2]: set when a wait timed out (the results carry it to the host).
 // Every thread first waits until its own memory operations are complete (its stores have reached the L2); after
-// the workgroup barrier one thread fences at agent scope on both sides: the release writes the L2's dirty lines
-// back, the acquire invalidates this CU's caches, so plain global stores made before the barrier by any workgroup
-// are visible to loads made after it by any other (the L2s of the XCDs are not coherent by themselves).  One
-// fence pair per workgroup, not per wave: each is a cache-wide operation.  The wait is bounded: the kernel always
-// ends, and a timeout is reported instead of hanging.
+// the workgroup barrier one thread publishes them with a release fence at agent scope (the L2's dirty lines are
+// written back: the L2s of the XCDs are not coherent by themselves), arrives, waits, and after the wait takes one
+// acquire fence at agent scope (this XCD's L2 and the CU's caches are invalidated), so plain global stores made
+// before the barrier by any workgroup are visible to loads made after it by any other.  The barrier words are
+// only touched by agent-scope atomics, which are coherent without fences: the polls are relaxed loads, so a
+// waiting workgroup does not invalidate its L2 once per poll while the others are still writing back.  The
+// generation is read to completion before the arrival is counted (else the last arrival could overtake it and
+// the wait would never see a change), and the reset of the count completes before the generation advances.
+// One fence pair per workgroup, not per wave: each is a cache-wide operation.  The wait is bounded: the kernel
+// always ends, and a timeout is reported instead of hanging.
 __device__ inline void grid_barrier(unsigned *bar, int nranks) {
     __builtin_amdgcn_s_waitcnt(0);      // vmcnt = expcnt = lgkmcnt = 0
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);
+        const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == (unsigned)nranks - 1) {
             __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_s_waitcnt(0);
+            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             long spins = 0;
             // after any timeout every later wait is skipped, so a failed team still finishes promptly
-            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen &&
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen &&
                    __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
                 __builtin_amdgcn_s_sleep(2);
                 if (++spins > BLML_BAR_SPIN) {
@@ -139,7 +146,7 @@ __device__ inline void grid_barrier(unsigned *bar, int nranks) {
                 }
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }

@@ -217,12 +217,33 @@ __global__ void __launch_bounds__(BLML_NT) matvec_pipe_kernel(DevPlan<T> p, unsi
                 if (p.facs[fb + j].axes[0] == a) fa = fb + j;
             fac_of[a] = fa;
         }
+        // every axis's table is staged once per mode, after the largest tile of any pass (the host sized the LDS)
         const int np = sp->npass;
+        int o = 0;
+        for (int q = 0; q < np; ++q) {
+            const int t = sp->pass[q].OB * sp->pass[q].R * sp->pass[q].w;
+            o = max(o, sep_lds(t) + 1 + 64);
+        }
+        constexpr int BLK = SepMfma<T>::BLK;
+        int aoff[BLML_MAXD];
+#pragma unroll
+        for (int a = 0; a < BLML_MAXD; ++a) {
+            aoff[a] = -1;
+            if (fac_of[a] >= 0 && shape[a] > 1) {
+                aoff[a] = o;
+                o += BLK * (BLK + 1);
+            }
+        }
+        constexpr int NV = sizeof(T) == 4 ? 64 : 32;          // 128 KB tiles
         for (int q = 0; q < np; ++q) {
             const SepPass ps = sp->pass[q];
             const T *src = q == 0 ? E + off : Z;
             T *dst = q == np - 1 ? U + off : Z;
-            sep_pass_pipe_only<T>(ps, shape, fac_of, p.sep_tab, p.sep_off, src, dst, tm.rank, tm.nranks);
+            if (q == 0)
+                sep_pass_pipe<T, NV>(ps, shape, fac_of, aoff, src, dst, tm.rank, tm.nranks,
+                                     SepStageAll<T>{fac_of, p.sep_tab, p.sep_off, aoff});
+            else
+                sep_pass_pipe<T, NV>(ps, shape, fac_of, aoff, src, dst, tm.rank, tm.nranks, SepNoStage{});
             if (q < np - 1 || k < K - 1) tm.sync();
         }
     }
@@ -614,6 +635,7 @@ struct Plan {
     void *d_sep_tab = nullptr;
     long *d_sep_off = nullptr;
     int lds_mv = 0, lds_fit = 0;   // dynamic LDS of the spread mat-vec and fit launches
+    int lds_pipe = 0;              // dynamic LDS of matvec_pipe_kernel (largest tile, then every axis's table)
     int64_t out_elems = 0, dens_elems = 0;
     int64_t ws_elems = 0;
     size_t esize = 4;
@@ -949,7 +971,8 @@ bool sep_pipe_host(const SepPlan &sp, const int *shape) {
         const SepPass &ps = sp.pass[q];
         const long tsize = (long)ps.OB * ps.R * ps.w;
         if (tsize != NV * BLML_NT || ps.nout % ps.OB || ps.rs % ps.w) return false;
-        if (!(ps.w == ps.rs || (ps.w <= BLML_NT && BLML_NT % ps.w == 0))) return false;
+        if (!(ps.w == ps.rs || (ps.w <= BLML_NT && BLML_NT % ps.w == 0 && ps.w % (16 / (long)sizeof(T)) == 0)))
+            return false;
         if (tsize % ((long)BLML_SEP_FULL_NH * BLK * BLK)) return false;
         long st = ps.w;
         for (int a = ps.hi - 1; a >= ps.lo; --a) {
@@ -1002,7 +1025,16 @@ int build_sep(Plan &pl, const double *fdata, const int32_t *freqs, int max_lds) 
     int st_pipe = 0;
     rc = static_lds((const void *)matvec_pipe_kernel<T>, st_pipe);
     if (rc) return rc;
-    pl.pipe_mv = st_pipe <= st_mv && sep_pipe_host<T>(sp[0], f.shape);
+    long pipe_vals = 0, all_tabs = 0;
+    for (int q = 0; q < sp[0].npass; ++q) {
+        const long t = (long)sp[0].pass[q].OB * sp[0].pass[q].R * sp[0].pass[q].w;
+        pipe_vals = std::max(pipe_vals, t + t / 32 + 1 + 64);
+    }
+    for (int a = 0; a < f.d; ++a)
+        if (f.shape[a] > 1) all_tabs += (long)SepMfma<T>::BLK * (SepMfma<T>::BLK + 1);
+    const long lds_pipe = (pipe_vals + all_tabs + 2) * (long)es;
+    pl.lds_pipe = (int)std::min<long>(lds_pipe, 1L << 30);
+    pl.pipe_mv = st_pipe <= st_mv && lds_pipe + st_pipe <= max_lds && sep_pipe_host<T>(sp[0], f.shape);
     return BLML_OK;
 }
 
@@ -1126,7 +1158,7 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
             const int oc2 = std::min(std::min(a, b), oc);
             if (pl.pipe_mv) {
                 int c = 0;
-                HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, matvec_pipe_kernel<T>, BLML_NT, pl.lds_mv));
+                HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, matvec_pipe_kernel<T>, BLML_NT, pl.lds_pipe));
                 pl.pipe_mv = c >= std::max(oc2, 1);
             }
             if (oc2 < 1) return fail(BLML_ERUNTIME, "no workgroup of the separable spread kernels fits a CU");
@@ -1215,16 +1247,24 @@ int blml_matvec(void *plan, const void *E, void *U, void *stream) {
     if (!pl || !E || !U) return fail(BLML_EVALUE, "null argument");
     hipStream_t st = (hipStream_t)stream;
     if (pl->spread) {
+        // the pipelined kernel moves 16-byte vectors: every mode's slice of E and U must start 16-byte aligned
+        const blml_fit_t &f = pl->hfits[0];
+        const long mb = (long)f.B * (f.complex_g ? 2 : 1) * pl->esize;
+        const bool al16 = (((unsigned long)E | (unsigned long)U) & 15) == 0 && (f.out_off * pl->esize) % 16 == 0 &&
+                          (f.K == 1 || mb % 16 == 0);
+        const bool pipe = pl->pipe_mv && al16;
         if (pl->dtype == BLML_F32) {
             DevPlan<float> dp = dev_plan<float>(*pl);
             void *args[] = {&dp, &pl->d_bar, &pl->d_part, &E, &U};
-            return launch_coop(*pl, pl->pipe_mv ? (const void *)matvec_pipe_kernel<float>
-                                                : (const void *)matvec_grid_kernel<float>, args, pl->lds_mv, st);
+            return launch_coop(*pl, pipe ? (const void *)matvec_pipe_kernel<float>
+                                         : (const void *)matvec_grid_kernel<float>, args,
+                               pipe ? pl->lds_pipe : pl->lds_mv, st);
         }
         DevPlan<double> dp = dev_plan<double>(*pl);
         void *args[] = {&dp, &pl->d_bar, &pl->d_part, &E, &U};
-        return launch_coop(*pl, pl->pipe_mv ? (const void *)matvec_pipe_kernel<double>
-                                            : (const void *)matvec_grid_kernel<double>, args, pl->lds_mv, st);
+        return launch_coop(*pl, pipe ? (const void *)matvec_pipe_kernel<double>
+                                     : (const void *)matvec_grid_kernel<double>, args,
+                           pipe ? pl->lds_pipe : pl->lds_mv, st);
     }
     if (pl->dtype == BLML_F32)
         matvec_kernel<float><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<float>(*pl), (const float *)E,

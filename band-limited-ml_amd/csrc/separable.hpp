@@ -109,6 +109,7 @@ __device__ __forceinline__ void sep_axis_mfma(int toff, int nvals, int m, int st
         for (int h = 0; h < NH; ++h)
 #pragma unroll
             for (int k = 0; k < KMAX; ++k) xv[h][k] = tile[sep_lds(base[h] + min(k * F::KS + lk, m - 1) * st)];
+        __builtin_amdgcn_sched_barrier(0);        // all B operands in flight before the first product
         typename F::acc_t acc[NH][MB];
 #pragma unroll
         for (int h = 0; h < NH; ++h)
@@ -175,6 +176,9 @@ __device__ __forceinline__ void sep_axis_mfma_full(int nvals, int st, int aoff) 
         for (int h = 0; h < NH; ++h)
 #pragma unroll
             for (int k = 0; k < KMAX; ++k) xv[h][k] = tile[lb[h] + (k * F::KS + lk) * stp];
+        // every B operand is read before the first product: left to itself the scheduler sinks each LDS read to
+        // just before its product, which then waits out a full LDS round trip per product
+        __builtin_amdgcn_sched_barrier(0);
         typename F::acc_t acc[NH];
 #pragma unroll
         for (int h = 0; h < NH; ++h)
@@ -327,13 +331,22 @@ __device__ __forceinline__ void sep_move_fast(const SepPass &ps, long gbase, con
 
 // A pass whose tiles are all whole, of NV NT values, with neither bases nor type changes, software-pipelined: the
 // next tile's values are loaded into registers (NV per thread) while the matrix cores work on the current one in
-// LDS, and a tile's results leave through registers in chunks (the stores drain while the loop goes on).  Tile
-// value e = t + NT i sits at LDS sep_lds(t) + (NT + NT / 32) i and at global offset start + i step:
-//   w == rs (contiguous):   start = gbase + t,                         step = NT
-//   w divides NT (rows):    start = gbase + (t / w) rs + (t mod w),    step = (NT / w) rs
-template <typename T, int NV>
+// LDS, and a tile's results leave through registers in chunks (the stores drain while the loop goes on).  Global
+// memory moves in 16-byte accesses (VW = 16 / sizeof(T) values), so a thread has NV / VW loads and as many stores
+// in flight per tile: well inside the 63 vector-memory operations a wave can count, so the next tile's loads never
+// wait for this tile's stores to retire.  Tile value e = VW (t + NT i) + j (j < VW) sits at LDS
+// sep_lds(VW t) + j + (VW NT + VW NT / 32) i (a run of VW values never crosses a 32-value pad boundary) and at
+// global offset start + j + i step:
+//   w == rs (contiguous):             start = gbase + VW t,                              step = VW NT
+//   w divides NT, VW divides w (rows): start = gbase + (VW t / w) rs + (VW t mod w),      step = (VW NT / w) rs
+// The caller guarantees src and dst 16-byte aligned and every tile's first value at an offset that VW divides.
+// stage() runs once, after the first tile's loads are issued: it may write LDS outside the tile (operator tables).
+template <typename T, int NV, typename Stage>
 __device__ __forceinline__ void sep_pass_pipe(const SepPass &ps, const int *shape, const int *fac_of, const int *aoff,
-                                              const T *src, T *dst, int rank, int nranks) {
+                                              const T *src, T *dst, int rank, int nranks, const Stage &stage) {
+    constexpr int VW = 16 / (int)sizeof(T);
+    constexpr int NQ = NV / VW;
+    typedef T vec_t __attribute__((ext_vector_type(VW)));
     T *tile = reinterpret_cast<T *>(g_dyn_lds);
     const int t = threadIdx.x;
     // the pass's axes: per axis (indexed by constants only, so the arrays stay in registers) whether it has an
@@ -360,36 +373,39 @@ __device__ __forceinline__ void sep_pass_pipe(const SepPass &ps, const int *shap
     const long nchunks = ps.nchunks, ntiles = ps.ntiles;
     const bool contig = w == rs;
     const int lw = 31 - __clz(w);
-    const long step = contig ? (long)BLML_NT : (long)(BLML_NT >> lw) * rs;
-    const long toff = contig ? t : (long)(t >> lw) * rs + (t & (w - 1));
-    const int l0 = sep_lds(t);
-    constexpr int lstep = BLML_NT + BLML_NT / 32;
+    const long qstep = contig ? (long)BLML_NT : (long)((VW * BLML_NT) >> lw) * (rs / VW);   // in VW-value units
+    const long toff = contig ? (long)VW * t : (long)((VW * t) >> lw) * rs + ((VW * t) & (w - 1));
+    const int l0 = sep_lds(VW * t);
+    constexpr int lstep = VW * BLML_NT + VW * BLML_NT / 32;
     const int tsize = NV * BLML_NT;
-    T v[NV];
+    vec_t v[NQ];
     long tn = rank;
     if (tn < ntiles) {
         const long to = tn / nchunks, tc = tn - to * nchunks;
-        sep_gcptr<T> p = (sep_gcptr<T>)(src + (to * OB * R * rs + tc * w + toff));
+        sep_gcptr<vec_t> p = (sep_gcptr<vec_t>)(src + (to * OB * R * rs + tc * w + toff));
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
+        for (int i = 0; i < NQ; ++i) {
             v[i] = *p;
-            p += step;
+            p += qstep;
         }
     }
+    stage();            // tables staged while the first tile's loads are in flight (the loop's barrier covers both)
     for (; tn < ntiles; tn += nranks) {
         const long to = tn / nchunks, tc = tn - to * nchunks;
         const long gbase = to * OB * R * rs + tc * w;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) tile[l0 + lstep * i] = v[i];
+        for (int i = 0; i < NQ; ++i)
+#pragma unroll
+            for (int j = 0; j < VW; ++j) tile[l0 + lstep * i + j] = v[i][j];
         __syncthreads();
         const long tnext = tn + nranks;
         if (tnext < ntiles) {
             const long to2 = tnext / nchunks, tc2 = tnext - to2 * nchunks;
-            sep_gcptr<T> p = (sep_gcptr<T>)(src + (to2 * OB * R * rs + tc2 * w + toff));
+            sep_gcptr<vec_t> p = (sep_gcptr<vec_t>)(src + (to2 * OB * R * rs + tc2 * w + toff));
 #pragma unroll
-            for (int i = 0; i < NV; ++i) {
+            for (int i = 0; i < NQ; ++i) {
                 v[i] = *p;
-                p += step;
+                p += qstep;
             }
         }
 #pragma unroll
@@ -399,31 +415,35 @@ __device__ __forceinline__ void sep_pass_pipe(const SepPass &ps, const int *shap
                 __syncthreads();
             }
         }
-        sep_gptr<T> q = (sep_gptr<T>)(dst + (gbase + toff));
-        constexpr int CH = 16;
+        sep_gptr<vec_t> q = (sep_gptr<vec_t>)(dst + (gbase + toff));
+        constexpr int CH = 4;
 #pragma unroll
-        for (int i0 = 0; i0 < NV; i0 += CH) {
-            T o[CH];
+        for (int i0 = 0; i0 < NQ; i0 += CH) {
+            vec_t o[CH];
 #pragma unroll
-            for (int i = 0; i < CH; ++i) o[i] = tile[l0 + lstep * (i0 + i)];
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int j = 0; j < VW; ++j) o[i][j] = tile[l0 + lstep * (i0 + i) + j];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
                 *q = o[i];
-                q += step;
+                q += qstep;
             }
         }
         __syncthreads();
     }
 }
 
-// whether every tile of the pass is whole, of NV NT values, contiguous or in rows of w dividing NT, and every axis
-// of the pass takes the full-block product (m == BLK, stride 1 or a multiple of 32)
+// whether every tile of the pass is whole, of NV NT values, contiguous or in rows of w dividing NT (and divisible
+// by the 16-byte access's value count), and every axis of the pass takes the full-block product (m == BLK, stride
+// 1 or a multiple of 32)
 template <typename T>
 __device__ __forceinline__ bool sep_pipe_ok(const SepPass &ps, int nv, const int *shape, const int *fac_of) {
     constexpr int BLK = SepMfma<T>::BLK;
     const long tsize = (long)ps.OB * ps.R * ps.w;
+    constexpr int VW = 16 / (int)sizeof(T);
     if (!(tsize == (long)nv * BLML_NT && ps.nout % ps.OB == 0 && ps.rs % ps.w == 0 &&
-          (ps.w == ps.rs || (ps.w <= BLML_NT && BLML_NT % ps.w == 0))))
+          (ps.w == ps.rs || (ps.w <= BLML_NT && BLML_NT % ps.w == 0 && ps.w % VW == 0))))
         return false;
     if (tsize % (BLML_SEP_FULL_NH * BLK * BLK)) return false;
     long st = ps.w;
@@ -484,36 +504,46 @@ __device__ __forceinline__ void sep_stage_tables(const SepPass &ps, const int *s
     __syncthreads();
 }
 
-// A pass known to take the pipelined whole-tile path (the stand-alone mat-vec kernel's host checks every pass).
-// shape and fac_of are arrays the caller indexes by constants only; the tables are staged axis by axis in the
-// generic staging's layout (ascending axes, rows of P + 1), with every per-axis array indexed by constants, so
-// nothing here lives in scratch memory.
+struct SepNoStage {
+    __device__ __forceinline__ void operator()() const {}
+};
+
+// Stages the operator tables of every axis with aoff[a] >= 0 (each BLK x BLK, the pipelined path's only table
+// size) in LDS at aoff[a], rows of BLK + 1: every table's loads are issued before the first LDS write, so the whole
+// staging costs about one memory latency.  The arrays are the caller's and indexed by constants only.
 template <typename T>
-__device__ __forceinline__ void sep_pass_pipe_only(const SepPass &ps, const int *shape, const int *fac_of,
-                                                   const T *tabs, const long *tab_off, const T *src, T *dst, int rank,
-                                                   int nranks) {
-    int aoff[BLML_MAXD];
-    {
+struct SepStageAll {
+    const int *fac_of;
+    const T *tabs;
+    const long *tab_off;
+    const int *aoff;
+    __device__ __forceinline__ void operator()() const {
+        constexpr int BLK = SepMfma<T>::BLK, PP = BLK * BLK, NS = (PP + BLML_NT - 1) / BLML_NT;
         T *lds = reinterpret_cast<T *>(g_dyn_lds);
-        const int lo = __builtin_amdgcn_readfirstlane(ps.lo), hi = __builtin_amdgcn_readfirstlane(ps.hi);
-        int o = sep_lds(ps.OB * ps.R * ps.w) + 1 + 64;
+        T tv[BLML_MAXD][NS];
 #pragma unroll
         for (int a = 0; a < BLML_MAXD; ++a) {
-            aoff[a] = -1;
-            if (a >= lo && a < hi && fac_of[a] >= 0 && shape[a] > 1) {
-                const int P = sep_table_dim(shape[a], SepMfma<T>::BLK);
-                const int lp = 31 - __clz(P);
-                const T *tsrc = tabs + tab_off[fac_of[a]];
-                aoff[a] = o;
-                for (int i = threadIdx.x; i < P * P; i += BLML_NT) lds[o + (i >> lp) * (P + 1) + (i & (P - 1))] = tsrc[i];
-                o += P * (P + 1);
+            if (aoff[a] >= 0) {
+                const T *ts = tabs + tab_off[fac_of[a]];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const int i = threadIdx.x + s * BLML_NT;
+                    tv[a][s] = i < PP ? ts[i] : (T)0;
+                }
             }
         }
-        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < BLML_MAXD; ++a) {
+            if (aoff[a] >= 0) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const int i = threadIdx.x + s * BLML_NT;
+                    if (i < PP) lds[aoff[a] + (i / BLK) * (BLK + 1) + i % BLK] = tv[a][s];
+                }
+            }
+        }
     }
-    constexpr int NV = sizeof(T) == 4 ? 64 : 32;          // 128 KB tiles
-    sep_pass_pipe<T, NV>(ps, shape, fac_of, aoff, src, dst, rank, nranks);
-}
+};
 
 // One pass of a separable mode: dst = (prod over the pass's axes of M_a) src.  The first pass may subtract a base
 // from its input, the last may add a base to its output (the incremental form U = Ubase + S (E - Ebase)); values
@@ -530,9 +560,10 @@ __device__ __forceinline__ void sep_pass(const SepPass &ps, int d, int nc, const
     sep_stage_tables<T>(ps, shape, fac_of, tabs, tab_off, aoff);
     if (std::is_same<Ti, T>::value && std::is_same<To, T>::value && !srcb && !dstb) {
         constexpr int NV = sizeof(T) == 4 ? 64 : 32;          // 128 KB tiles
-        if (sep_pipe_ok<T>(ps, NV, shape, fac_of)) {
+        const bool al16 = ((reinterpret_cast<unsigned long>(src) | reinterpret_cast<unsigned long>(dst)) & 15) == 0;
+        if (al16 && sep_pipe_ok<T>(ps, NV, shape, fac_of)) {
             sep_pass_pipe<T, NV>(ps, shape, fac_of, aoff, reinterpret_cast<const T *>(src), reinterpret_cast<T *>(dst),
-                                 rank, nranks);
+                                 rank, nranks, SepNoStage{});
             return;
         }
     }
