@@ -190,6 +190,56 @@ struct FitSolver {
         return -(0.5 * a[0] - a[1]) / n;
     }
 
+    // Real rank one (psd false: K = nc = 1, so V() = B): the merit terms of one bin, as merit() sums them.
+    __device__ __forceinline__ void merit_terms(int b, double e, double u, double (&a)[3]) const {
+        const double sgb = sg ? sg[b] : 1.0;
+        a[0] += e * u;
+        if (occ(b)) {
+            a[1] += cb(b) * log(fabs(e));
+            if (sgb * e <= 0) a[2] += 1.0;
+        }
+    }
+    __device__ double merit_of(double (&a)[3]) {
+        tm.sum<3>(a, g_sl.red);
+        if (a[2] > 0) return -INFINITY;
+        return -(0.5 * a[0] - a[1]) / n;
+    }
+    // Real rank one: EJ = E + t (EP - E), UJ = U + t (UP - U), and merit(EJ, UJ) summed in the same pass (the
+    // values and the order of the sums are merit()'s, so the result is the same to the bit).
+    __device__ double lerp_merit(double *EJ, double *UJ, const double *E, const double *U, const double *EP,
+                                 const double *UP, double t) {
+        double a[3] = {0.0, 0.0, 0.0};
+        for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
+            const double e = E[b] + t * (EP[b] - E[b]);
+            const double u = U[b] + t * (UP[b] - U[b]);
+            EJ[b] = e;
+            UJ[b] = u;
+            merit_terms(b, e, u, a);
+        }
+        return merit_of(a);
+    }
+    // Real rank one: rescale(E, U), then merit(E, U) and resid(E, U, 1) of the rescaled pair, both summed in the
+    // pass that scales (one pass over the bins instead of three; values and sum orders are theirs).
+    __device__ void rescale_merit_resid(double *E, double *U, double &mer, double &res) {
+        const double Q = dot(E, U);
+        const double sc = Q > 0 ? n / sqrt(Q) : 1.0;
+        double a[3] = {0.0, 0.0, 0.0};
+        double r = 0.0;
+        for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
+            const double e = E[b] * sc, u = U[b] * sc;
+            E[b] = e;
+            U[b] = u;
+            merit_terms(b, e, u, a);
+            if (occ(b)) {
+                const double nrm2 = e * e;
+                const double v = u * (nrm2 / cb(b)) - e;
+                r = fmax(r, sqrt(v * v) / sqrt(fmax(nrm2, 1e-300)));
+            }
+        }
+        mer = merit_of(a);
+        res = tm.max(r, g_sl.red);
+    }
+
     __device__ void update(const double *E, const double *U, double *out) {
         if (!psd) {
             const double s0 = s[0];
@@ -508,12 +558,16 @@ struct FitSolver {
                     const long n_ = V();
                     double *EJ = vec(iEJ), *UJ = vec(iUJ);
                     const double *Ec = vec(iE), *Uc = vec(iU), *EPc = vec(iEP), *UPc = vec(iUP);
-                    for (long e = tm.first(); e < n_; e += tm.stride()) {
-                        EJ[e] = Ec[e] + a * (EPc[e] - Ec[e]);
-                        UJ[e] = Uc[e] + a * (UPc[e] - Uc[e]);
+                    if (!psd) {
+                        mJ = lerp_merit(EJ, UJ, Ec, Uc, EPc, UPc, a);
+                    } else {
+                        for (long e = tm.first(); e < n_; e += tm.stride()) {
+                            EJ[e] = Ec[e] + a * (EPc[e] - Ec[e]);
+                            UJ[e] = Uc[e] + a * (UPc[e] - Uc[e]);
+                        }
+                        tm.sync();
+                        mJ = merit(vec(iEJ), vec(iUJ));
                     }
-                    tm.sync();
-                    mJ = merit(vec(iEJ), vec(iUJ));
                     if (mJ >= cur) { found = true; break; }
                     a *= 0.5;
                 }
@@ -529,11 +583,16 @@ struct FitSolver {
                     }
                 }
             }
-            rescale(vec(iEJ), vec(iUJ));
-            { int t = iE; iE = iEJ; iEJ = t; t = iU; iU = iUJ; iUJ = t; }
-            cur = psd ? mJ : merit(vec(iE), vec(iU));
+            if (!psd) {
+                rescale_merit_resid(vec(iEJ), vec(iUJ), cur, res);
+                { int t = iE; iE = iEJ; iEJ = t; t = iU; iU = iUJ; iUJ = t; }
+            } else {
+                rescale(vec(iEJ), vec(iUJ));
+                { int t = iE; iE = iEJ; iEJ = t; t = iU; iU = iUJ; iUJ = t; }
+                cur = mJ;
+                res = resid(vec(iE), vec(iU), 1.0);
+            }
             push_merit(cur);
-            res = resid(vec(iE), vec(iU), 1.0);
             push_rho(res);
             if (res < 0.9 * best_res || cur > best_m + 1e-13 * fabs(best_m)) {
                 best_res = fmin(res, best_res);
