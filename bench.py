@@ -175,13 +175,13 @@ def cpu_baseline(fits, seconds=20.0):
 PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles')
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of the 32^5 mat-vec's kernel from its rocprofv3 counter summary
-    (profiles/<kernel>_32x5_sinc_traffic.txt: FETCH_SIZE + WRITE_SIZE, each scaled by the same counter's reading for
+def measured_traffic(kernel, workload='32x5_sinc'):
+    """HBM bytes per launch of a mat-vec kernel from its rocprofv3 counter summary
+    (profiles/<kernel>_<workload>_traffic.txt: FETCH_SIZE + WRITE_SIZE, each scaled by the same counter's reading for
     a copy of known size), or None when no summary of that kernel exists."""
     name = kernel.split('<')[0]
     try:
-        with open(os.path.join(PROFILES, f'{name}_32x5_sinc_traffic.txt')) as f:
+        with open(os.path.join(PROFILES, f'{name}_{workload}_traffic.txt')) as f:
             for line in f:
                 if line.startswith('traffic_bytes_per_launch:'):
                     return float(line.split(':', 1)[1])
@@ -210,6 +210,20 @@ def large_matvec(torch, blml_amd, _FitSpec, FitBatch, params, event_time, reps=2
     return lfs, lparams, large, bl, roof, t_l
 
 
+def batched_matvec(torch, batch, specs, event_time, reps):
+    """The batched Gram mat-vec of every fit and mode of a plan (one workgroup per fit) timed with HIP events on the
+    launch stream; its roofline dict."""
+    g = torch.Generator(device='cuda').manual_seed(3)
+    E = torch.randn(batch.sizes['out'], device='cuda', dtype=torch.float32, generator=g)
+    batch.matvec(E)
+    t_mv = event_time(lambda: batch.matvec(E), reps)
+    alg = sum(2 * fs.K * fs.nc * fs.B * 4 for fs in specs)
+    kname = batch.info()['matvec_kernel'] + '<float>'
+    return dict(bound='hbm', achieved=alg / t_mv / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s', frac=alg / t_mv / HBM_PEAK,
+                traffic=measured_traffic(kname, 'c3_2048_fits'), kernel=kname, bytes_per_launch=alg,
+                us_per_launch=t_mv * 1e6)
+
+
 def copy_known_size(torch):
     """A plain device copy of 256 MiB (its FETCH_SIZE / WRITE_SIZE readings scale the counters)."""
     cp_src = torch.empty(64 << 20, dtype=torch.float32, device='cuda')
@@ -227,8 +241,9 @@ def main():
     ap.add_argument('--fits', type=int, default=N_FITS)
     ap.add_argument('--no-fp64', action='store_true', help='skip the plain FP64 reference run')
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
-    ap.add_argument('--only', choices=['all', 'matvec'], default='all',
-                    help='matvec: only the 32^5 mat-vec (timed) and the 256 MiB copy, for counter runs')
+    ap.add_argument('--only', choices=['all', 'matvec', 'batched'], default='all',
+                    help='matvec: only the 32^5 mat-vec (timed) and the 256 MiB copy, for counter runs; batched: '
+                         'only the batched mat-vec of the 2048 fits (timed) and the copy')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (independent fits '
@@ -237,7 +252,7 @@ def main():
     import torch
     import blml_amd
     from blml_amd.binned import _FitSpec, FitBatch
-    if args.only == 'matvec':
+    if args.only != 'all':
         st0 = torch.cuda.current_stream()
 
         def ev_time(fn, reps):
@@ -249,8 +264,16 @@ def main():
             torch.cuda.synchronize()
             return ev0.elapsed_time(ev1) / 1e3 / reps
 
-        *_, bl, roof, _ = large_matvec(torch, blml_amd, _FitSpec, FitBatch, fit_params(), ev_time, args.steps)
-        bl.close()
+        if args.only == 'matvec':
+            *_, bl, roof, _ = large_matvec(torch, blml_amd, _FitSpec, FitBatch, fit_params(), ev_time, args.steps)
+            bl.close()
+        else:
+            fits = uneven_fits(args.fits)
+            specs = [_FitSpec(c, modes_of(g, blml_amd), 1.0, 0.0, kw['complex_g'], None, None, 0)
+                     for c, g, kw, _ in fits]
+            bb = FitBatch(specs, fit_params(), np.float32, hist_cap=0).build()
+            roof = batched_matvec(torch, bb, specs, ev_time, args.steps)
+            bb.close()
         copy_known_size(torch)
         print(json.dumps(dict(roofline=roof)), flush=True)
         return
@@ -361,13 +384,7 @@ def main():
         return ev0.elapsed_time(ev1) / 1e3 / reps
 
     # the batched Gram mat-vec of the workload (all fits, all modes), timed with HIP events on the launch stream
-    E = b32.d_e.clone()
-    b32.matvec(E)
-    t_mv = event_time(lambda: b32.matvec(E), 20)
-    alg = sum(2 * fs.K * fs.nc * fs.B * 4 for fs in specs)
-    roof_batched = dict(bound='hbm', achieved=alg / t_mv / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
-                        frac=alg / t_mv / HBM_PEAK, traffic=None, kernel='matvec_kernel<float>',
-                        bytes_per_launch=alg, us_per_launch=t_mv * 1e6)
+    roof_batched = batched_matvec(torch, b32, specs, event_time, 20)
     b32.close()
     # the large single fit: 32^5 bins spread over every CU; its Gram mat-vec is the memory-bound roofline
     lfs, lparams, large, bl, roof, t_l = large_matvec(torch, blml_amd, _FitSpec, FitBatch, params, event_time)
