@@ -67,6 +67,29 @@ def test_spread_matvec_32x5_fp32_matches_fp64():
     b64.close()
 
 
+def test_spread_matvec_unaligned_input_takes_the_general_kernel():
+    """The pipelined 32^5 kernel moves 16-byte vectors, so blml_matvec hands an input that is not 16-byte aligned
+    to the general separable kernel: the same products of the same tiles, so the same U."""
+    import torch
+    import blml_amd
+    shape = (32,) * 5
+    mm = [dict(fc=[0.48] * 5, kinds=['sinc'] * 5)]
+    _note('spread matvec 32^5 fp32, aligned and unaligned input')
+    fs, b = _one_fit_batch(np.ones(shape), _modes(mm, blml_amd), 0.25, False, np.float32, 1)
+    assert b.info()['matvec_kernel'] == 'matvec_pipe_kernel'
+    g = torch.Generator(device='cuda').manual_seed(7)
+    buf = torch.randn(fs.B + 1, device='cuda', dtype=torch.float32, generator=g)
+    E_al = buf[1:].clone()                 # a fresh allocation: aligned
+    E_un = buf[1:]                         # 4 bytes past an aligned allocation
+    assert E_al.data_ptr() % 16 == 0 and E_un.data_ptr() % 16 == 4
+    U_al = b.matvec(E_al)
+    U_un = b.matvec(E_un)
+    err = float((U_al - U_un).abs().max())
+    _note(f'spread matvec 32^5 aligned vs unaligned: max diff {err:.3e}')
+    assert err <= 1e-6 * float(U_al.abs().max())
+    b.close()
+
+
 def _one_fit_batch(counts, modes, h, cplx, dtype, spread, params=PARAMS):
     from blml_amd.binned import _FitSpec, FitBatch
     fs = _FitSpec(counts, modes, h, 0.0, cplx, None, None, 0)
