@@ -481,7 +481,7 @@ struct FitSolver {
     __device__ FitOut solve_cd() {
         // slots: 0 E, 1 U, 2 EJ, 3 UJ, 4 EA, 5 UA, 6 EP, 7 UP, 8 V, 9 G, 10 SG, 11 En, 12 Un, 13.. history
         int iE = 0, iU = 1, iEJ = 2, iUJ = 3, iEA = 4, iUA = 5;
-        const int iEP = 6, iUP = 7;
+        int iEP = 6, iUP = 7;
         const int mA = P->anderson;
         const int hcap = mA + 1;
         int hhead = 0, hcnt = 0;
@@ -509,6 +509,7 @@ struct FitSolver {
             update(vec(iE), vec(iU), vec(iEJ));
             Snew(vec(iEJ), vec(iUJ), vec(iE), vec(iU));
             double mJ = merit(vec(iEJ), vec(iUJ));
+            bool plain = true;          // EJ, UJ still hold the Jacobi update of this sweep
             if (mA) {
                 // f = EJ - E; hX.append(E); hF.append(f); keep the last mA + 1
                 int slot;
@@ -540,6 +541,7 @@ struct FitSolver {
                             int t = iEJ; iEJ = iEA; iEA = t;
                             t = iUJ; iUJ = iUA; iUA = t;
                             mJ = mAc;
+                            plain = false;
                             ++o.n_and;
                         }
                     } else {
@@ -550,8 +552,15 @@ struct FitSolver {
             if (mJ < cur - 1e-12 * fabs(cur)) {
                 hcnt = 0; hhead = 0;
                 ++o.n_damp;
-                update(vec(iE), vec(iU), vec(iEP));
-                Snew(vec(iEP), vec(iUP), vec(iE), vec(iU));
+                // the damped step runs along the Jacobi update (blml_binned.py:252-255 recomputes it; when EJ is
+                // still that update its slots are taken over, the same values without the update and mat-vec)
+                if (plain) {
+                    int t = iEP; iEP = iEJ; iEJ = t;
+                    t = iUP; iUP = iUJ; iUJ = t;
+                } else {
+                    update(vec(iE), vec(iU), vec(iEP));
+                    Snew(vec(iEP), vec(iUP), vec(iE), vec(iU));
+                }
                 double a = 0.5;
                 bool found = false;
                 while (a > 1e-8) {
