@@ -347,6 +347,37 @@ struct FitSolver {
         if (!(a[1] > 0)) return -INFINITY;
         return a[0] - n * log(a[1]);
     }
+    // A line-search trial in one pass over the bins: En = E + t q, Un = U + t Sq, then lval(En, Un, Vt, Qout) (the
+    // same values in the same order: every value of bin b is this thread's own, written before it is read back).
+    __device__ double lerp_lval(double *En, double *Un, const double *E, const double *U, const double *q,
+                                const double *Sq, double t, double *Vt, double *Qout) {
+        double a[2] = {0.0, 0.0};
+        for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
+            for (int k = 0; k < K; ++k) {
+                const long i = ((long)k * B + b) * nc;
+                for (int c = 0; c < nc; ++c) {
+                    En[i + c] = E[i + c] + t * q[i + c];
+                    Un[i + c] = U[i + c] + t * Sq[i + c];
+                }
+            }
+            const bool o = occ(b);
+            const double F = Fb(Un, b);
+            a[0] += w[b] * log(o ? F : 1.0);
+            const double fac = o ? w[b] / F : 0.0;
+            for (int k = 0; k < K; ++k) {
+                const long i = ((long)k * B + b) * nc;
+                for (int c = 0; c < nc; ++c) {
+                    a[1] += En[i + c] * Un[i + c];
+                    Vt[i + c] = fac * Un[i + c];
+                }
+            }
+        }
+        tm.sum<2>(a, g_sl.red);
+        *Qout = a[1];
+        if (!(a[1] > 0)) return -INFINITY;
+        return a[0] - n * log(a[1]);
+    }
+
     // g = 2 (S Vt - n U / Q) on occupied bins (blml_binned.py:285-288); SVt: scratch
     __device__ void lgrad_from(const double *U, double Q, const double *Vt, double *SVt, double *g) {
         Sall(Vt, SVt);
@@ -935,14 +966,8 @@ struct FitSolver {
             double t = 1.0, Ln = -INFINITY, Qn = 0.0;
             bool ok = false;
             for (int tr = 0; tr < 40; ++tr) {
-                double *En = vec(iEn), *Un = vec(iUn);
-                const double *Ec = vec(iE), *Uc = vec(iU), *Sq = vec(iSVt);
-                for (long e = tm.first(); e < n_; e += tm.stride()) {
-                    En[e] = Ec[e] + t * q[e];
-                    Un[e] = Uc[e] + t * Sq[e];
-                }
-                tm.sync();
-                Ln = lval(vec(iEn), vec(iUn), nullptr, &Qn);
+                // the trial's Vt = (w / F) Un too: the accepted trial's is the gradient's mat-vec input
+                Ln = lerp_lval(vec(iEn), vec(iUn), vec(iE), vec(iU), q, vec(iSVt), t, vec(iVt), &Qn);
                 if (Ln >= L + 1e-4 * t * slope) { ok = true; break; }
                 t *= 0.5;
             }
@@ -954,7 +979,6 @@ struct FitSolver {
                 continue;
             }
             reset = false;
-            lval(vec(iEn), vec(iUn), vec(iVt), &Qn);
             lgrad_from(vec(iUn), Qn, vec(iVt), vec(iSVt), vec(ign));
             // the kept pairs' dots with the new gradient
             dots_history(head, ph, cnt, vec(ign));
