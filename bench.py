@@ -31,6 +31,7 @@ for _p in (ROOT, os.path.join(ROOT, 'band-limited-ml_amd')):
         sys.path.insert(0, _p)
 
 HBM_PEAK = 8.0e12
+MFMA_FP32_PEAK = 157.3e12      # dense FP32 matrix-core peak of the MI355X
 N_FITS = 2048
 SOLVER = dict(solver='lbfgs', lbfgs_m=20, tol=1e-5, ll_tol=1e-9, max_sweeps=3000)
 
@@ -204,9 +205,13 @@ def large_matvec(torch, blml_amd, _FitSpec, FitBatch, params, event_time, reps=2
     t_l = event_time(lambda: bl.matvec(El), reps)
     alg_l = 2 * lfs.B * 4
     kname = bl.info()['matvec_kernel'] + '<float>'
+    # the dense line operators' arithmetic: one multiply-add per input value of the line, per value and axis
+    flops = 2 * lfs.B * sum(LARGE_SHAPE)
     roof = dict(bound='hbm', achieved=alg_l / t_l / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s', frac=alg_l / t_l / HBM_PEAK,
                 traffic=measured_traffic(kname), kernel=kname, bytes_per_launch=alg_l,
-                us_per_launch=t_l * 1e6, workload='32^5 product-sinc mat-vec (separable tile passes, FP32)')
+                us_per_launch=t_l * 1e6, workload='32^5 product-sinc mat-vec (separable tile passes, FP32)',
+                flops_per_launch=flops, mfma_achieved=flops / t_l / 1e12, mfma_peak=MFMA_FP32_PEAK / 1e12,
+                mfma_unit='TFLOP/s', mfma_frac=flops / t_l / MFMA_FP32_PEAK)
     return lfs, lparams, large, bl, roof, t_l
 
 
