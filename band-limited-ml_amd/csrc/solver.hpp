@@ -392,6 +392,38 @@ struct FitSolver {
         tm.sync();
     }
 
+    // lgrad_from(U, Q, Vt, SVt, g), and resid(E, U, Q > 0 ? n / sqrt(Q) : 1) of the same point summed in its pass
+    // over the bins (resid()'s values; a maximum does not depend on the order, so the residual is the same to the bit)
+    __device__ double lgrad_resid(const double *E, const double *U, double Q, const double *Vt, double *SVt,
+                                  double *g) {
+        Sall(Vt, SVt);
+        const double nq = n / Q;
+        const double sc = Q > 0 ? n / sqrt(Q) : 1.0;
+        double r = 0.0;
+        for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
+            const bool o = occ(b);
+            for (int k = 0; k < K; ++k) {
+                const long i = ((long)k * B + b) * nc;
+                for (int q = 0; q < nc; ++q) g[i + q] = o ? 2.0 * (SVt[i + q] - nq * U[i + q]) : 0.0;
+            }
+            if (!o) continue;
+            double nrm2 = 0.0;
+            for (int k = 0; k < K; ++k) nrm2 += absq(E, k, b);
+            nrm2 *= sc * sc;
+            const double fac = nrm2 / cb(b);
+            double acc = 0.0;
+            for (int k = 0; k < K; ++k) {
+                const long i = ((long)k * B + b) * nc;
+                for (int q = 0; q < nc; ++q) {
+                    const double v = sc * U[i + q] * fac - sc * E[i + q];
+                    acc += v * v;
+                }
+            }
+            r = fmax(r, sqrt(acc) / sqrt(fmax(nrm2, 1e-300)));
+        }
+        return tm.max(r, g_sl.red);
+    }
+
     // ------------------------------------------------------------------ histories
     __device__ void push_merit(double v) {
         if (threadIdx.x == 0) g_sl.ring[ring_n % BLML_RING] = v;
@@ -979,7 +1011,7 @@ struct FitSolver {
                 continue;
             }
             reset = false;
-            lgrad_from(vec(iUn), Qn, vec(iVt), vec(iSVt), vec(ign));
+            const double res_n = lgrad_resid(vec(iEn), vec(iUn), Qn, vec(iVt), vec(iSVt), vec(ign));
             // the kept pairs' dots with the new gradient
             dots_history(head, ph, cnt, vec(ign));
             // the candidate pair s = En - E, y = g - gn and its scalars
@@ -1050,7 +1082,7 @@ struct FitSolver {
             { int tt = iE; iE = iEn; iEn = tt; tt = iU; iU = iUn; iUn = tt; tt = ig; ig = ign; ign = tt; }
             L = Ln;
             Q = Qn;
-            res = resid(vec(iE), vec(iU), Q > 0 ? n / sqrt(Q) : 1.0);
+            res = res_n;
             push_rho(res);
             push_merit(L / n);
         }
