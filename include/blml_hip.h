@@ -212,7 +212,8 @@ int blml_quick_stats(const double *e, const double *Se, const double *w, int64_t
 
 /* Which kernel blml_matvec launches for the plan (reporting; no counterpart in the reference): 0 matvec_kernel (one
  * workgroup per fit), 1 matvec_grid_kernel (one fit over the device), 2 matvec_pipe_kernel (one separable fit whose
- * every tile pass is pipelined). */
+ * every tile pass is pipelined; it moves 16-byte vectors, so a call whose E or U slices are not 16-byte aligned
+ * runs matvec_grid_kernel instead, with the same results). */
 int blml_plan_matvec_kernel(void *plan, int32_t *which);
 
 /* Sizes the plan chose (for the caller's buffers and for reporting).  grid: workgroups of the batched launch, or
