@@ -182,3 +182,30 @@ def test_struct_layouts_match_header(tmp_path):
         assert got[(cname, 'size')] == ctypes.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_fit_input_errors_mirror_reference():
+    """The fit's input checks run on the host before any device work and raise what blml_binned.py:119-124,
+    383-400 raise (ValueError), plus this engine's own limits (at most 5 axes, 8 modes)."""
+    from blml_amd.binned import _FitSpec
+    modes = M.make_modes([0.3], ['sinc'])
+    ok = np.ones(12)
+    spec = lambda c, **kw: _FitSpec(c, kw.pop('modes', modes), 1.0, 0.0, kw.pop('complex_g', False),
+                                    kw.pop('signs', None), None, 0)
+    assert spec(ok).B == 12
+    for bad in (np.array(3.0), -ok, np.where(np.arange(12) == 3, np.nan, 1.0), np.where(np.arange(12) == 5, np.inf, 1.0)):
+        with pytest.raises(ValueError, match='non-negative finite'):
+            spec(bad)
+    with pytest.raises(ValueError, match='no observations'):
+        spec(np.zeros(12))
+    with pytest.raises(ValueError, match='phases'):
+        spec(ok, complex_g=True, signs=np.ones(12))
+    with pytest.raises(ValueError, match='nonzero on every occupied bin'):
+        spec(ok, signs=np.where(np.arange(12) == 2, 0.0, 1.0))
+    assert spec(np.where(np.arange(12) == 2, 0.0, 1.0), signs=np.where(np.arange(12) == 2, 0.0, -1.0)).signs is not None
+    with pytest.raises(ValueError, match='dimensions'):
+        spec(np.ones((2,) * 6), modes=M.make_modes([0.3] * 6, ['sinc'] * 6))
+    with pytest.raises(ValueError, match='modes'):
+        spec(ok, modes=M.make_modes([[0.3]] * 9, ['sinc']))
+    with pytest.raises(ValueError, match='dep'):
+        M.make_modes([0.3], ['sinc'], dep=[0])
