@@ -573,7 +573,10 @@ struct FitSolver {
             Snew(vec(iEJ), vec(iUJ), vec(iE), vec(iU));
             double mJ = merit(vec(iEJ), vec(iUJ));
             bool plain = true;          // EJ, UJ still hold the Jacobi update of this sweep
-            if (mA) {
+            // a sweep that will damp (no ascent) with fewer than 3 pairs after this one tries no Anderson step, and the
+            // damped step clears the history: its pair is not stored (the state after the sweep is the same)
+            const bool skip_pair = hcnt + 1 < 3 && mJ < cur - 1e-12 * fabs(cur);
+            if (mA && !skip_pair) {
                 // f = EJ - E; hX.append(E); hF.append(f); keep the last mA + 1
                 int slot;
                 if (hcnt < hcap) {
