@@ -164,7 +164,8 @@ struct FitSolver {
     }
 
     // ------------------------------------------------------------------ merit, update, rescale, residual
-    __device__ double merit(const double *E, const double *U) {
+    // Qout (real rank one): sum e u of the pair, the same value dot(E, U) gives (same products, order and reduction)
+    __device__ double merit(const double *E, const double *U, double *Qout = nullptr) {
         if (psd) {
             double a[2] = {0.0, 0.0};
             for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) a[0] += cb(b) * log(fmax(Fb(U, b), 1e-30));
@@ -186,6 +187,7 @@ struct FitSolver {
             }
         }
         tm.sum<3>(a, g_sl.red);
+        if (Qout) *Qout = a[0];
         if (a[2] > 0) return -INFINITY;
         return -(0.5 * a[0] - a[1]) / n;
     }
@@ -207,7 +209,7 @@ struct FitSolver {
     // Real rank one: EJ = E + t (EP - E), UJ = U + t (UP - U), and merit(EJ, UJ) summed in the same pass (the
     // values and the order of the sums are merit()'s, so the result is the same to the bit).
     __device__ double lerp_merit(double *EJ, double *UJ, const double *E, const double *U, const double *EP,
-                                 const double *UP, double t) {
+                                 const double *UP, double t, double *Qout) {
         double a[3] = {0.0, 0.0, 0.0};
         for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
             const double e = E[b] + t * (EP[b] - E[b]);
@@ -216,12 +218,14 @@ struct FitSolver {
             UJ[b] = u;
             merit_terms(b, e, u, a);
         }
-        return merit_of(a);
+        const double m = merit_of(a);
+        *Qout = a[0];
+        return m;
     }
     // Real rank one: rescale(E, U), then merit(E, U) and resid(E, U, 1) of the rescaled pair, both summed in the
-    // pass that scales (one pass over the bins instead of three; values and sum orders are theirs).
-    __device__ void rescale_merit_resid(double *E, double *U, double &mer, double &res) {
-        const double Q = dot(E, U);
+    // pass that scales (one pass over the bins instead of three; values and sum orders are theirs).  Q: dot(E, U),
+    // known from the merit evaluation of (E, U); Qout: the rescaled pair's.
+    __device__ void rescale_merit_resid(double *E, double *U, double Q, double &mer, double &res, double &Qout) {
         const double sc = Q > 0 ? n / sqrt(Q) : 1.0;
         double a[3] = {0.0, 0.0, 0.0};
         double r = 0.0;
@@ -237,6 +241,7 @@ struct FitSolver {
             }
         }
         mer = merit_of(a);
+        Qout = a[0];
         res = tm.max(r, g_sl.red);
     }
 
@@ -551,7 +556,8 @@ struct FitSolver {
         FitOut o{};
         Sall(vec(iE), vec(iU));
         rescale(vec(iE), vec(iU));
-        double cur = merit(vec(iE), vec(iU));
+        double Qcur = 0.0;              // real rank one: dot(E, U) of the current pair, from its merit evaluation
+        double cur = merit(vec(iE), vec(iU), &Qcur);
         long it = 0;
         double res = resid(vec(iE), vec(iU), 1.0);
         push_rho(res);
@@ -571,7 +577,8 @@ struct FitSolver {
             ++it;
             update(vec(iE), vec(iU), vec(iEJ));
             Snew(vec(iEJ), vec(iUJ), vec(iE), vec(iU));
-            double mJ = merit(vec(iEJ), vec(iUJ));
+            double QJ = 0.0;            // real rank one: dot(EJ, UJ)
+            double mJ = merit(vec(iEJ), vec(iUJ), &QJ);
             bool plain = true;          // EJ, UJ still hold the Jacobi update of this sweep
             // a sweep that will damp (no ascent) with fewer than 3 pairs after this one tries no Anderson step, and the
             // damped step clears the history: its pair is not stored (the state after the sweep is the same)
@@ -602,11 +609,13 @@ struct FitSolver {
                     }
                     if (anderson(sx, sf, hcnt, vec(iE), fs, vec(iEA))) {
                         Snew(vec(iEA), vec(iUA), vec(iE), vec(iU));
-                        const double mAc = merit(vec(iEA), vec(iUA));
+                        double QA = 0.0;
+                        const double mAc = merit(vec(iEA), vec(iUA), &QA);
                         if (mAc > mJ) {
                             int t = iEJ; iEJ = iEA; iEA = t;
                             t = iUJ; iUJ = iUA; iUA = t;
                             mJ = mAc;
+                            QJ = QA;
                             plain = false;
                             ++o.n_and;
                         }
@@ -634,7 +643,7 @@ struct FitSolver {
                     double *EJ = vec(iEJ), *UJ = vec(iUJ);
                     const double *Ec = vec(iE), *Uc = vec(iU), *EPc = vec(iEP), *UPc = vec(iUP);
                     if (!psd) {
-                        mJ = lerp_merit(EJ, UJ, Ec, Uc, EPc, UPc, a);
+                        mJ = lerp_merit(EJ, UJ, Ec, Uc, EPc, UPc, a, &QJ);
                     } else {
                         for (long e = tm.first(); e < n_; e += tm.stride()) {
                             EJ[e] = Ec[e] + a * (EPc[e] - Ec[e]);
@@ -655,11 +664,12 @@ struct FitSolver {
                         copy(vec(iEJ), vec(iE));
                         copy(vec(iUJ), vec(iU));
                         mJ = cur;
+                        QJ = Qcur;
                     }
                 }
             }
             if (!psd) {
-                rescale_merit_resid(vec(iEJ), vec(iUJ), cur, res);
+                rescale_merit_resid(vec(iEJ), vec(iUJ), QJ, cur, res, Qcur);
                 { int t = iE; iE = iEJ; iEJ = t; t = iU; iU = iUJ; iUJ = t; }
             } else {
                 rescale(vec(iEJ), vec(iUJ));
