@@ -397,15 +397,24 @@ def main():
     t0 = time.perf_counter()
     rl = bl.run().results()[0]
     large.update(fp32_seconds=time.perf_counter() - t0, loglik_fp32=rl['loglik'], sweeps_fp32=rl['n_sweeps'],
-                 stop_fp32=rl['stop_reason'])
+                 stop_fp32=rl['stop_reason'], residual_fp32=rl['residual'])
+    # the certificate lambda_max(D^1/2 S D^1/2) / n of the result (blml_binned.py:501-518; <= 1 + tol only at a
+    # stationary point whose rank-1 lift is optimal)
+    lam, its = bl.certificate(max_iter=2000, tol=1e-9)
+    large.update(certificate_fp32=float(lam[0]), certificate_iters_fp32=int(its[0]))
     bl.close()
     if not args.no_fp64:
         bl64 = FitBatch([lfs], lparams, np.float64, hist_cap=0).build()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         rl64 = bl64.run().results()[0]
-        large.update(fp64_seconds=time.perf_counter() - t0, loglik_fp64=rl64['loglik'], sweeps_fp64=rl64['n_sweeps'],
-                     rel_loglik=abs(rl['loglik'] - rl64['loglik']) / abs(rl64['loglik']))
+        t64 = time.perf_counter() - t0
+        lam64, _ = bl64.certificate(max_iter=2000, tol=1e-9)
+        d64 = np.asarray(rl64['density'])
+        large.update(fp64_seconds=t64, loglik_fp64=rl64['loglik'], sweeps_fp64=rl64['n_sweeps'],
+                     rel_loglik=abs(rl['loglik'] - rl64['loglik']) / abs(rl64['loglik']),
+                     density_max_rel_err=float(np.abs(np.asarray(rl['density']) - d64).max() / np.abs(d64).max()),
+                     certificate_fp64=float(lam64[0]))
         bl64.close()
     # a plain device copy of known size (256 MiB), the scale of rocprof's FETCH_SIZE / WRITE_SIZE counters
     copy_known_size(torch)
