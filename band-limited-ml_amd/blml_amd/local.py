@@ -57,7 +57,9 @@ def lattice_tables(kinds, fc, periods, h, Zi):
     (the reference's 25% headroom), or a rank table over their occupied values when that is smaller
     (len(vals)^2 < 2 J + 1, blml_trivial.py:405-413); dirichlet columns (folded on their period) a circular row
     K(m h), m = 0..P/h - 1; a periodic ball over its columns (pball) its torus table over the columns' circular
-    offsets (row-major).  Radial balls are not built here."""
+    offsets (row-major); a ball over its columns (one radial group per kind tag, blml_trivial.py radial_groups) its
+    row on the grid r = j / RADIAL_Q lattice steps, j < floor(sqrt(sum span^2) RADIAL_Q) + 2, with fc h equal across
+    the group (blml_trivial.py:284-303; the row kernels interpolate it linearly in r)."""
     from .modes import split_kind
     from .units import one_column_values, multi_values
     kinds = list(kinds)
@@ -77,8 +79,13 @@ def lattice_tables(kinds, fc, periods, h, Zi):
     dim_unit = np.full(d, -1, np.int64)
     vals = [None] * d
     rows, rrows, grp_unit = [], [], []
-    single = [k for k in range(d) if split_kind(kinds[k])[0] != 'pball']
+    single = [k for k in range(d) if split_kind(kinds[k])[0] not in ('pball', 'ball')]
     pcols = [k for k in range(d) if split_kind(kinds[k])[0] == 'pball']
+    bgroups = {}                                  # ball columns by kind tag, in order of first appearance
+    for k in range(d):
+        name, tag = split_kind(kinds[k])
+        if name == 'ball':
+            bgroups.setdefault(tag, []).append(k)
     at = 0
     for ui, k in enumerate(single):
         name = split_kind(kinds[k])[0]
@@ -113,13 +120,24 @@ def lattice_tables(kinds, fc, periods, h, Zi):
         kd_arr[pcols] = 4
         nbins[pcols] = ms
         grp_unit.append(len(single))
+    for dims in bgroups.values():
+        c = fc[dims] * h[dims]
+        if not np.allclose(c, c[0], rtol=1e-9, atol=0):
+            raise ValueError(f"radial group dims {dims}: fc*h must be equal across the group, got {c}")
+        r2max = int((alloc[dims] ** 2).sum())
+        r = np.arange(int(np.floor(np.sqrt(r2max) * RADIAL_Q)) + 2, dtype=float) / RADIAL_Q
+        diffs = [r * h[dims[0]]] + [np.zeros_like(r)] * (len(dims) - 1)     # along the first column: K is radial
+        grp[dims] = len(rrows)
+        kd_arr[dims] = 2
+        rrows.append(multi_values('ball', [fc[k] for k in dims], None, diffs))
+        grp_unit.append(len(single) + len(grp_unit))
     T = np.concatenate(rows) if rows else np.zeros(1)
     LR = max([len(r) for r in rrows] or [2])
     R = np.zeros((max(len(rrows), 1), LR))
     for g, r in enumerate(rrows):
         R[g, :len(r)] = r
     return dict(T=T, base=base, Jk=Jk, mk=mk, vals=vals, kinds=kd_arr, nbins=nbins, grp=grp, R=R,
-                dim_unit=dim_unit, grp_unit=np.asarray(grp_unit or [0], np.int64), U=len(single) + (1 if pcols else 0),
+                dim_unit=dim_unit, grp_unit=np.asarray(grp_unit or [0], np.int64), U=len(single) + len(grp_unit),
                 nW=0)
 
 

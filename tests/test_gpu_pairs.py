@@ -34,6 +34,10 @@ def _all_cases(rng):
         [('rank', 'sinc', [0.2], [0.0], [0]), ('offset', 'sinc', [0.5], [0.0], [1])], np.ones(2), (40, 6),
         vals=[np.unique(Z[:, 0]), None]), None, None))
     out.append(('radial random rows', (8, 8, 8), _radial_tab(rng), 40, 50))
+    from blml_amd import local
+    Zb = _bins(rng, (20, 18, 9), 400)
+    out.append(('ball x sinc (radial group from lattice_tables)', (20, 18, 9),
+                local.lattice_tables(['ball', 'ball', 'sinc'], [0.3, 0.3, 0.4], None, [1.0, 1.0, 1.0], Zb), 70, 120))
     # enough rows and sources for several slices, chunk tails and a table too big for LDS (float64, > 32 KB)
     out.append(('3d slits large', (64, 64, 64), OP.build_tables(
         [('offset', 'slits:0.5', [0.2], [0.0], [k]) for k in range(3)], np.ones(3), (1500, 1500, 1500)), 3000,

@@ -5,6 +5,7 @@ blml_local.py:250-252 (checked against a per-pair loop); rank columns reject val
 import numpy as np
 import pytest
 
+from oracle import blml_oracle as O
 from oracle import blml_pairs as OP
 
 # (name, shape of the lattice box, units, h, J) -- units as blml_pairs.build_tables takes them
@@ -105,3 +106,36 @@ def test_product_tables_match_oracle_tables():
     Gp = OP.dense_pair_gram([('torus', 'pball', [0.45], [10.0, 8.0], [0, 1])], np.ones(2), Zp, Zp)
     wp = rng.standard_normal(len(Zp))
     assert np.abs(OP.pair_rows(Zp, Zp, wp, tabp) - Gp @ wp).max() < 1e-12
+
+
+def test_ball_tables_radial_rows():
+    """A ball over columns (one radial group) next to a sinc column: lattice_tables' radial row is the oracle's ball
+    (kernel_eval, the reference's closed form) on the grid r = j / 4 lattice steps to 1e-14, so rows through it
+    equal the exact Gram at pairs whose distance lies on the grid (a shift along one column) and stay within the
+    linear interpolation's error bound elsewhere; fc h must be equal across the group."""
+    from blml_amd import local
+    rng = np.random.default_rng(9)
+    Z = _bins(rng, (14, 12, 10), 120)
+    fc, h = [0.35, 0.35, 0.4], [1.0, 1.0, 1.0]
+    tab = local.lattice_tables(['ball', 'ball', 'sinc'], fc, None, h, Z)
+    assert list(tab['kinds']) == [2, 2, 0] and int(tab['U']) == 2 and list(tab['grp']) == [0, 0, -1]
+    row = np.asarray(tab['R'])[0]
+    r = np.arange(len(row)) / OP.RADIAL_Q
+    want = O.kernel_eval('ball', [0.35, 0.35], [0.0, 0.0], [r, 0.0 * r])
+    assert np.abs(row - want).max() < 1e-14 * np.abs(want).max()
+    span = Z.max(0) - Z.min(0)
+    assert (len(row) - 2) / OP.RADIAL_Q >= np.sqrt((span[:2] ** 2).sum())   # every pair's distance is on the row
+    units = [('offset', 'ball', [0.35, 0.35], [0.0, 0.0], [0, 1]), ('offset', 'sinc', [0.4], [0.0], [2])]
+    ZA = Z.copy()
+    ZB = Z.copy()
+    ZB[:, 0] += 1                                      # distances on the grid: |off| = integer when column 1 matches
+    ZB[:, 1] = ZA[:, 1]
+    w = rng.standard_normal(len(Z))
+    G = OP.dense_pair_gram(units, np.asarray(h), ZA, ZB)
+    got = OP.pair_rows(ZA, ZB, w, local.lattice_tables(['ball', 'ball', 'sinc'], fc, None, h, np.vstack([ZA, ZB])))
+    # pairs with both column offsets nonzero fall between grid points: |K'' dr^2 / 8| with dr = 1/4, |K''| <= fc^2 pi^2 K0
+    bound = (0.35 ** 2 * np.pi ** 2 * 0.35 ** 2 * (1.0 / OP.RADIAL_Q) ** 2 / 8) * np.abs(
+        OP.dense_pair_gram(units[1:], np.asarray(h), ZA, ZB)).sum(1).max() * np.abs(w).max()
+    assert np.abs(got - G @ w).max() <= bound
+    with pytest.raises(ValueError, match='equal across the group'):
+        local.lattice_tables(['ball', 'ball'], [0.35, 0.3], None, [1.0, 1.0], Z[:, :2])
