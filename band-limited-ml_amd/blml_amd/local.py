@@ -2,7 +2,8 @@
 (blml_local_gpu.py:453-475) over the same table dicts (blml_kernels.ProductKernel.tables, blml_kernels.py:133-200),
 computed by libblml_hip.so (blml_pair_rows / blml_pair_cols).
 
-    s, ds = rows(ZA, ZB, w, tab)           s_r = sum_j S(ZA_r, ZB_j) w_j   (ds: zeros; derivative rows are not built)
+    s, ds = rows(ZA, ZB, w, tab)           s_r = sum_j S(ZA_r, ZB_j) w_j   (ds: zeros)
+    s, ds = rows(ZA, ZB, w, tab, grad=True)   with ds_rq = sum_j dS_q(ZA_r, ZB_j) w_j for tab's derivative entries
     out = cols(ZA, ZB, tab)                the block S(ZA_r, ZB_j) (R, M)
     out, out_m = cols(ZA, ZB, tab, depmask=mask)   with the marginal over the columns not in mask
 
@@ -29,6 +30,17 @@ class PairTab(ctypes.Structure):
                 ('nT', ctypes.c_int64), ('LR', ctypes.c_int64), ('T', ctypes.c_void_p), ('RT', ctypes.c_void_p)]
 
 
+MAXW = 16
+
+
+class PairGrad(ctypes.Structure):
+    _fields_ = [('nW', ctypes.c_int32), ('U', ctypes.c_int32), ('reserved', ctypes.c_int32 * 2),
+                ('wunit', ctypes.c_int32 * MAXW), ('wcol', ctypes.c_int32 * MAXW), ('wgrp', ctypes.c_int32 * MAXW),
+                ('dim_unit', ctypes.c_int32 * MAXD), ('grp_unit', ctypes.c_int32 * MAXG),
+                ('dbase', ctypes.c_int64 * MAXW), ('ndT', ctypes.c_int64), ('dT', ctypes.c_void_p),
+                ('dR', ctypes.c_void_p)]
+
+
 def _torch():
     import torch
     if not torch.cuda.is_available():
@@ -51,7 +63,7 @@ def rank_coords(Z, tab):
     return Z
 
 
-def lattice_tables(kinds, fc, periods, h, Zi):
+def lattice_tables(kinds, fc, periods, h, Zi, want=()):
     """Value tables in the layout of the reference's ProductKernel.tables (blml_kernels.py:133-200) for the bins Zi
     (N, d) and per-column kinds: sinc / slits columns get an offset row K(o h), o = -J..J, J = ceil(1.25 span)
     (the reference's 25% headroom), or a rank table over their occupied values when that is smaller
@@ -59,9 +71,18 @@ def lattice_tables(kinds, fc, periods, h, Zi):
     K(m h), m = 0..P/h - 1; a periodic ball over its columns (pball) its torus table over the columns' circular
     offsets (row-major); a ball over its columns (one radial group per kind tag, blml_trivial.py radial_groups) its
     row on the grid r = j / RADIAL_Q lattice steps, j < floor(sqrt(sum span^2) RADIAL_Q) + 2, with fc h equal across
-    the group (blml_trivial.py:284-303; the row kernels interpolate it linearly in r)."""
+    the group (blml_trivial.py:284-303; the row kernels interpolate it linearly in r).
+
+    Units: the one-column units in column order, then the periodic ball (if any), then the balls by kind tag.
+    want: derivative entries (unit, p) for rows(grad=True), p a parameter (dK/dlog p: 'fc' of sinc / slits / ball,
+    'period' of dirichlet) or a coordinate shift ('shift' or ('shift', i), dK/dt_i), built as the reference's
+    ProductKernel._assemble (blml_kernels.py:165-182): a column's row over raw offsets -J..J (its rank table for a
+    rank column) at dT[dbase[q]:]; a ball's row on its r grid in dR[q] (a shift row divided by the offset of its
+    column, which the row kernels multiply back, blml_trivial.py:317-327); a periodic ball's shift as its torus
+    table of dK/dt_i."""
     from .modes import split_kind
-    from .units import one_column_values, multi_values
+    from .units import (one_column_values, multi_values, one_column_dvalues, shift_index, ball_profile,
+                        ball_profile_dlog, pball_dshift)
     kinds = list(kinds)
     d = len(kinds)
     fc = np.broadcast_to(np.asarray(fc, float), (d,))
@@ -78,7 +99,7 @@ def lattice_tables(kinds, fc, periods, h, Zi):
     grp = np.full(d, -1, np.int64)
     dim_unit = np.full(d, -1, np.int64)
     vals = [None] * d
-    rows, rrows, grp_unit = [], [], []
+    rows, rrows, grp_unit, uinfo = [], [], [], []
     single = [k for k in range(d) if split_kind(kinds[k])[0] not in ('pball', 'ball')]
     pcols = [k for k in range(d) if split_kind(kinds[k])[0] == 'pball']
     bgroups = {}                                  # ball columns by kind tag, in order of first appearance
@@ -90,6 +111,8 @@ def lattice_tables(kinds, fc, periods, h, Zi):
     for ui, k in enumerate(single):
         name = split_kind(kinds[k])[0]
         dim_unit[k] = ui
+        uinfo.append(('col', k))
+        Jk[k] = int(alloc[k])                     # every column's raw-offset half width (its derivative rows')
         if name in ('sinc', 'slits'):
             J = int(alloc[k])
             occ = np.unique(Zi[:, k])
@@ -98,7 +121,7 @@ def lattice_tables(kinds, fc, periods, h, Zi):
                 kd_arr[k], mk[k], vals[k] = 3, len(occ), occ
             else:
                 row = one_column_values(kinds[k], fc[k], None, np.arange(-J, J + 1) * h[k])
-                kd_arr[k], Jk[k] = 0, J
+                kd_arr[k] = 0
         elif name == 'dirichlet':
             nb = int(round(float(periods[k]) / h[k]))
             if abs(nb * h[k] - float(periods[k])) > 1e-9 * float(periods[k]):
@@ -120,6 +143,7 @@ def lattice_tables(kinds, fc, periods, h, Zi):
         kd_arr[pcols] = 4
         nbins[pcols] = ms
         grp_unit.append(len(single))
+        uinfo.append(('pball', pcols, P, ms, offs))
     for dims in bgroups.values():
         c = fc[dims] * h[dims]
         if not np.allclose(c, c[0], rtol=1e-9, atol=0):
@@ -131,14 +155,64 @@ def lattice_tables(kinds, fc, periods, h, Zi):
         kd_arr[dims] = 2
         rrows.append(multi_values('ball', [fc[k] for k in dims], None, diffs))
         grp_unit.append(len(single) + len(grp_unit))
+        uinfo.append(('ball', dims, r, float(c[0])))
+    nW = len(want)
+    wunit = np.zeros(max(nW, 1), np.int64)
+    wcol = np.full(max(nW, 1), -1, np.int64)
+    wgrp = np.full(max(nW, 1), -1, np.int64)
+    dbase = np.zeros(max(nW, 1), np.int64)
+    drows, drr, dat = [], {}, 0
+    for q, (ui, p) in enumerate(want):
+        ui = int(ui)
+        if not 0 <= ui < len(uinfo):
+            raise ValueError(f"derivative entry {q}: unit {ui} of {len(uinfo)}")
+        info = uinfo[ui]
+        wunit[q] = ui
+        i = shift_index(p)
+        if info[0] == 'col':
+            k = info[1]
+            wcol[q] = k
+            if kd_arr[k] == 3:
+                t = (vals[k][:, None] - vals[k][None, :]).ravel() * h[k]
+            else:
+                t = np.arange(-Jk[k], Jk[k] + 1) * h[k]
+            row = np.asarray(one_column_dvalues(kinds[k], fc[k], periods[k], p, t), float)
+            dbase[q] = dat
+            drows.append(row)
+            dat += len(row)
+        elif info[0] == 'pball':
+            pc, P, ms, offs = info[1:]
+            if i is None:
+                raise ValueError("pball: no parameter gradient (fc in whole harmonics, fixed periods); shifts only")
+            wgrp[q], wcol[q] = grp[pc[0]], pc[i]
+            drr[q] = pball_dshift([fc[pc[0]]] * len(pc), P, i, [offs[:, j] * h[k] for j, k in enumerate(pc)])
+        else:
+            dims, r, cg = info[1:]
+            D = len(dims)
+            sg = cg * r
+            pf = float(np.prod(fc[dims]))
+            wgrp[q] = grp[dims[0]]
+            if i is None:
+                if p != 'fc':
+                    raise ValueError(f"ball has no parameter {p!r}")
+                drr[q] = pf * (D * ball_profile(sg, D) + ball_profile_dlog(sg, D))
+            else:
+                wcol[q] = dims[i]
+                out = pf * fc[dims[i]] * ball_profile_dlog(sg, D) / (cg * np.where(r > 0, r * r, 1.0))
+                out[0] = out[1]                   # offset 0 there: the entry only bounds the interpolation
+                drr[q] = out
     T = np.concatenate(rows) if rows else np.zeros(1)
-    LR = max([len(r) for r in rrows] or [2])
+    LR = max([len(r) for r in rrows + list(drr.values())] or [2])
     R = np.zeros((max(len(rrows), 1), LR))
     for g, r in enumerate(rrows):
         R[g, :len(r)] = r
+    dR = np.zeros((max(nW, 1), LR))
+    for q, r in drr.items():
+        dR[q, :len(r)] = r
     return dict(T=T, base=base, Jk=Jk, mk=mk, vals=vals, kinds=kd_arr, nbins=nbins, grp=grp, R=R,
                 dim_unit=dim_unit, grp_unit=np.asarray(grp_unit or [0], np.int64), U=len(single) + len(grp_unit),
-                nW=0)
+                dT=np.concatenate(drows) if drows else np.zeros(1), dbase=dbase, dR=dR, wunit=wunit, wcol=wcol,
+                wgrp=wgrp, nW=nW)
 
 
 def _groups(tab):
@@ -209,10 +283,76 @@ def _prepare(ZA, ZB, tab, f32, ranked):
     return t, dA, dB, (dT, dR)
 
 
+def _check_dspans(A, B, tab):
+    """A column entry's derivative row is over raw offsets -J..J (or its rank table): every pair inside it."""
+    if not len(A) or not len(B):
+        return
+    kinds, Jk = np.asarray(tab['kinds']), np.asarray(tab['Jk'])
+    far = np.maximum(A.max(0) - B.min(0), B.max(0) - A.min(0))
+    for q in range(int(tab['nW'])):
+        k = int(tab['wcol'][q])
+        if int(tab['wgrp'][q]) < 0 and kinds[k] != 3 and far[k] > Jk[k]:
+            raise RuntimeError(f"derivative entry {q}: offsets up to {far[k]} in column {k} beyond its row's half "
+                               f"width {Jk[k]} (stale tables)")
+
+
+def _rows_grad(ZA, ZB, w, tab, f32, ranked):
+    torch = _torch()
+    nW, U = int(tab['nW']), int(tab['U'])
+    if nW > MAXW or U > MAXD:
+        raise ValueError(f"derivative rows: {U} kernel units (max {MAXD}), {nW} derivative entries (max {MAXW})")
+    A = np.atleast_2d(np.asarray(ZA if ranked else rank_coords(ZA, tab), np.int64))
+    B = np.atleast_2d(np.asarray(ZB if ranked else rank_coords(ZB, tab), np.int64))
+    if A.shape[1] == B.shape[1]:
+        _check_dspans(A, B, tab)
+    t, dA, dB, keep = _prepare(A, B, tab, f32, True)
+    R, M = dA.shape[0], dB.shape[0]
+    wv = np.ascontiguousarray(w, np.float64).ravel()
+    if len(wv) != M:
+        raise ValueError("w: one weight per row of ZB")
+    rt = torch.float32 if f32 else torch.float64
+    ddT = torch.as_tensor(np.asarray(tab['dT'], np.float64).ravel()).to('cuda', rt).contiguous()
+    dRh = np.atleast_2d(np.asarray(tab['dR'], np.float64))
+    if dRh.shape[1] != t.LR:                      # derivative rows on the value rows' length
+        pad = np.zeros((dRh.shape[0], t.LR))
+        n = min(t.LR, dRh.shape[1])
+        pad[:, :n] = dRh[:, :n]
+        dRh = pad
+    ddR = torch.as_tensor(dRh).to('cuda', rt).contiguous()
+    g = PairGrad()
+    g.nW, g.U = nW, U
+    for q in range(nW):
+        g.wunit[q] = int(tab['wunit'][q])
+        g.wcol[q] = int(tab['wcol'][q])
+        g.wgrp[q] = int(tab['wgrp'][q])
+        g.dbase[q] = int(tab['dbase'][q])
+    for k in range(A.shape[1]):
+        g.dim_unit[k] = int(tab['dim_unit'][k])
+    G = _groups(tab)
+    for c in range(G):
+        g.grp_unit[c] = int(tab['grp_unit'][c])
+    g.ndT = ddT.numel()
+    g.dT = ddT.data_ptr()
+    g.dR = ddR.data_ptr()
+    dw = torch.from_numpy(wv).to('cuda')
+    s = torch.zeros(max(R, 1), dtype=torch.float64, device='cuda')
+    ds = torch.zeros(max(R * nW, 1), dtype=torch.float64, device='cuda')
+    lib = _lib.load()
+    _lib.check(lib.blml_pair_rows_grad(ctypes.byref(t), ctypes.byref(g), ctypes.c_void_p(dA.data_ptr()),
+                                       ctypes.c_void_p(dB.data_ptr()), ctypes.c_void_p(dw.data_ptr()), R, M,
+                                       ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(ds.data_ptr()),
+                                       ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    out, dout = s.cpu().numpy()[:R], ds.cpu().numpy()[:R * nW].reshape(R, nW)
+    del keep, ddT, ddR
+    return out, dout
+
+
 def rows(ZA, ZB, w, tab, grad=False, f32=False, ranked=False):
-    """(s, ds): s (R,) = sum over j with w_j != 0 of S(ZA_r, ZB_j) w_j (FP64); ds: zeros (R, max(nW, 1))."""
-    if grad:
-        raise NotImplementedError("derivative rows (grad=True) are not built on this engine")
+    """(s, ds): s (R,) = sum over j with w_j != 0 of S(ZA_r, ZB_j) w_j (FP64); ds (R, max(nW, 1)): with grad and
+    derivative entries in the tables (lattice_tables(want=...)), ds[r, q] = sum_j dS_q(ZA_r, ZB_j) w_j, dS_q entry q's
+    derivative of its unit's factor times the other units' factors (blml_local_gpu.py:94-103, 423-429); else zeros."""
+    if grad and int(tab.get('nW', 0)) > 0:
+        return _rows_grad(ZA, ZB, w, tab, f32, ranked)
     torch = _torch()
     t, dA, dB, keep = _prepare(ZA, ZB, tab, f32, ranked)
     R, M = dA.shape[0], dB.shape[0]

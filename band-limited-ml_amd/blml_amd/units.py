@@ -108,6 +108,104 @@ def multi_values(name, fcs, periods, diffs):
     return sc * acc
 
 
+def shift_index(p):
+    """The column (within its unit) of a coordinate derivative p = 'shift' (0) or ('shift', i); None for a parameter
+    (blml_trivial.py:357-361)."""
+    if isinstance(p, str):
+        return 0 if p == 'shift' else None
+    return int(p[1]) if isinstance(p, tuple) and len(p) == 2 and p[0] == 'shift' else None
+
+
+def _sinc_dt(f, t):
+    """dK/dt of fc sinc(fc t): (pi f t cos(pi f t) - sin(pi f t)) / (pi t^2), the series -f (pi f)^2 t / 3 where
+    |pi f t| < 1e-4 (blml_trivial.py:379-387)."""
+    pt = np.pi * f * t
+    small = np.abs(pt) < 1e-4
+    ts = np.where(small, 1.0, t)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        full = (pt * np.cos(pt) - np.sin(pt)) / (np.pi * ts * ts)
+    return np.where(small, -f * (np.pi * f) ** 2 * t / 3.0, full)
+
+
+def _dirichlet_D(M, u):
+    """D(u) = sin(M u / 2) / sin(u / 2) and D'(u), with D = M, D' = 0 where |sin(u / 2)| < 1e-9
+    (blml_trivial.py:500-507)."""
+    s2 = np.sin(u / 2)
+    small = np.abs(s2) < 1e-9
+    ss = np.where(small, 1.0, s2)
+    Dv = np.where(small, float(M), np.sin(M * u / 2) / ss)
+    dD = np.where(small, 0.0, (0.5 * M * np.cos(M * u / 2) * ss - 0.5 * np.sin(M * u / 2) * np.cos(u / 2)) / ss ** 2)
+    return Dv, dD
+
+
+def one_column_dvalues(kind, fc, period, p, t):
+    """A derivative of a one-column unit at raw coordinate offsets t: dK/dlog p for a parameter p, dK/dt for
+    p = 'shift' (sinc blml_trivial.py:372-387, 392-395; slits :445-460; dirichlet :517-543).  Parameters without a
+    gradient (slits / dirichlet fill, dirichlet fc: whole harmonics) raise ValueError."""
+    name, tag = split_kind(kind)
+    t = np.asarray(t, float)
+    shift = shift_index(p) is not None
+    if name == 'sinc':
+        if shift:
+            return _sinc_dt(fc, t)
+        if p != 'fc':
+            raise ValueError(f"sinc has no parameter {p!r}")
+        return fc * np.cos(np.pi * fc * t)
+    if name == 'slits':
+        fi = (1.0 - float(tag)) * fc
+        if shift:
+            return _sinc_dt(fc, t) - _sinc_dt(fi, t) if fi > 0 else _sinc_dt(fc, t)
+        if p != 'fc':
+            raise ValueError("slits fill is a fixed shape: no gradient" if p == 'fill' else
+                             f"slits has no parameter {p!r}")
+        return fc * np.cos(np.pi * fc * t) - fi * np.cos(np.pi * fi * t)
+    if name == 'dirichlet':
+        fill = float(tag) if tag else None
+        P = float(period)
+        M = 2 * dirichlet_N(fc, P) + 1
+        Mi = dirichlet_Mi(fc, P, fill)
+        u = 2 * np.pi * t / P
+        Dv, dD = _dirichlet_D(M, u)
+        if Mi:
+            Di, dDi = _dirichlet_D(Mi, u)
+            Dv, dD = Dv - Di, dD - dDi
+        if shift:
+            return (1.0 / P) * dD * (2 * np.pi / P)
+        if p != 'period':
+            raise ValueError("dirichlet fc moves in whole harmonics: no gradient" if p == 'fc' else
+                             "dirichlet fill is a fixed shape: no gradient" if p == 'fill' else
+                             f"dirichlet has no parameter {p!r}")
+        return -(Dv + u * dD) / P
+    raise ValueError(f"not a one-column kind: {kind!r}")
+
+
+def ball_profile_dlog(s, D):
+    """s p'(s) of the ball profile: -pi (1/2)^(D/2) s^(1 - D/2) J_{D/2 + 1}(pi s), 0 at s = 0
+    (blml_trivial.py:146-155)."""
+    from scipy.special import jv
+    s = np.asarray(s, float)
+    out = np.zeros(s.shape)
+    nz = s > 0
+    out[nz] = -np.pi * 0.5 ** (D / 2) * s[nz] ** (1 - D / 2) * jv(D / 2 + 1, np.pi * s[nz])
+    return out
+
+
+def pball_dshift(fcs, periods, i, diffs):
+    """dK/dt_i of a periodic ball at per-column offsets: -(1 / prod P) sum_k (2 pi k_i / P_i) sin(2 pi sum_j k_j t_j
+    / P_j) (blml_trivial.py:672-689)."""
+    H, sc = pball_harmonics(fcs[0], periods)
+    acc = 0.0 * sum(diffs)
+    for k in H:
+        if not k[i]:
+            continue
+        ph = 0.0
+        for j, P in enumerate(periods):
+            if k[j]:
+                ph = ph + (2 * np.pi * k[j] / P) * diffs[j]
+        acc = acc - (2 * np.pi * k[i] / periods[i]) * np.sin(ph)
+    return sc * acc
+
+
 def unit_at_zero(kind, dims, spec):
     name = split_kind(kind)[0]
     fc = np.asarray(spec['fc'], float)

@@ -1419,6 +1419,91 @@ int blml_pair_rows(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *
     return BLML_OK;
 }
 
+int blml_pair_rows_grad(const blml_pairtab_t *tab, const blml_pairgrad_t *grad, const int32_t *ZA, const int32_t *ZB,
+                        const double *w, int64_t R, int64_t M, double *s, double *ds, void *stream) {
+    int rc = pair_check(tab);
+    if (rc) return rc;
+    if (!grad) return fail(BLML_EVALUE, "null derivative entries");
+    const blml_pairgrad_t &g = *grad;
+    if (g.nW < 1 || g.nW > BLML_PAIR_MAXW)
+        return fail(BLML_EVALUE, "derivative entries: 1.." + std::to_string(BLML_PAIR_MAXW) + " on this engine");
+    if (g.U < 1 || g.U > BLML_PAIR_MAXD) return fail(BLML_EVALUE, "derivative rows: 1.." +
+                                                     std::to_string(BLML_PAIR_MAXD) + " kernel units");
+    pair_grad_dev gd{};
+    gd.nW = g.nW;
+    gd.U = g.U;
+    for (int k = 0; k < BLML_PAIR_MAXD; ++k) gd.dim_unit[k] = -1;
+    for (int k = 0; k < tab->d; ++k) {
+        const int kd = tab->kind[k];
+        if (kd == 2 || kd == 4) continue;
+        if (g.dim_unit[k] < 0 || g.dim_unit[k] >= g.U) return fail(BLML_EVALUE, "derivative rows: a column's unit");
+        gd.dim_unit[k] = g.dim_unit[k];
+    }
+    for (int c = 0; c < tab->G; ++c) {
+        if (g.grp_unit[c] < 0 || g.grp_unit[c] >= g.U) return fail(BLML_EVALUE, "derivative rows: a group's unit");
+        gd.grp_unit[c] = g.grp_unit[c];
+    }
+    bool cols = false, grps = false;
+    for (int q = 0; q < g.nW; ++q) {
+        const int gq = g.wgrp[q], kc = g.wcol[q];
+        if (g.wunit[q] < 0 || g.wunit[q] >= g.U) return fail(BLML_EVALUE, "derivative entry: unit out of range");
+        gd.wunit[q] = g.wunit[q];
+        gd.wgrp[q] = gq;
+        gd.wcol[q] = kc;
+        if (gq >= 0) {
+            if (gq >= tab->G || (kc >= 0 && (kc >= tab->d || tab->grp[kc] != gq)))
+                return fail(BLML_EVALUE, "derivative entry: a group's entry outside its group");
+            if (kc < -1) return fail(BLML_EVALUE, "derivative entry: column out of range");
+            grps = true;
+        } else {
+            if (kc < 0 || kc >= tab->d || tab->kind[kc] == 2 || tab->kind[kc] == 4)
+                return fail(BLML_EVALUE, "derivative entry: a column entry needs a column of kind 0, 1 or 3");
+            if (g.dbase[q] < 0 || g.dbase[q] >= g.ndT) return fail(BLML_EVALUE, "derivative entry: dbase outside dT");
+            gd.wrank[q] = tab->kind[kc] == 3;
+            gd.wJ[q] = tab->J[kc];
+            gd.wmk[q] = tab->mk[kc];
+            gd.dbase[q] = g.dbase[q];
+            cols = true;
+        }
+    }
+    if (cols && !g.dT) return fail(BLML_EVALUE, "derivative rows: dT missing");
+    if (grps && (!g.dR || tab->LR < 2)) return fail(BLML_EVALUE, "derivative rows: dR missing");
+    gd.ndT = g.ndT;
+    gd.dT = g.dT;
+    gd.dR = g.dR;
+    if (R < 0 || M < 0 || R >= (1L << 31) || M >= (1L << 31)) return fail(BLML_EVALUE, "R, M: 0 .. 2^31 - 1");
+    if ((R > 0 && (!ZA || !s || !ds)) || (M > 0 && (!ZB || !w))) return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (R == 0) return BLML_OK;
+    if (M == 0) {
+        HIPCHECK(hipMemsetAsync(s, 0, sizeof(double) * R, st));
+        HIPCHECK(hipMemsetAsync(ds, 0, sizeof(double) * R * g.nW, st));
+        return BLML_OK;
+    }
+    long rb, nsl, slice;
+    pair_rows_grid(R, M, rb, nsl, slice);
+    double *part = nullptr;
+    HIPCHECK(hipMallocAsync((void **)&part, sizeof(double) * nsl * (1 + g.nW) * R, st));
+    const blml_pairtab_t t = *tab;
+    const size_t es = t.f32 ? 4 : 8;
+    const bool tl = t.nT * (long)es <= 32768;
+    const long tbytes = tl ? ((t.nT * (long)es + 15) / 16) * 16 : 0;
+    const int lds = (int)(tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16 + BLML_PAIR_CH * 8);
+    const dim3 grid((unsigned)rb, (unsigned)nsl);
+    if (t.f32) {
+        if (tl) pair_rows_grad_kernel<float, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+        else pair_rows_grad_kernel<float, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+    } else {
+        if (tl) pair_rows_grad_kernel<double, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+        else pair_rows_grad_kernel<double, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+    }
+    HIPCHECK(hipGetLastError());
+    pair_rows_grad_sum_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, g.nW, s, ds);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipFreeAsync(part, st));
+    return BLML_OK;
+}
+
 int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, int64_t R, int64_t M,
                    int64_t depmask, void *out, void *out_m, void *stream) {
     int rc = pair_check(tab);

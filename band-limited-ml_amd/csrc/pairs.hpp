@@ -33,6 +33,18 @@ __device__ __forceinline__ long pair_col_index(int kd, long za, long zb, long ba
     return i < 0 ? 0 : (i >= nT ? nT - 1 : i);
 }
 
+// A group's factor from its row of LR values: a torus table read at the row-major circular offset r2, or a row on
+// the r grid interpolated linearly at sqrt(r2) * RADIAL_Q (indexes clamped into the row).
+template <typename R_>
+__device__ __forceinline__ R_ pair_group_row(const R_ *row, long LR, long r2, bool tor) {
+    if (tor) return row[r2 < LR ? r2 : LR - 1];
+    const double x = sqrt((double)r2) * BLML_RADIAL_Q;
+    long i0 = (long)x;
+    const double f = x - (double)i0;
+    if (i0 > LR - 2) i0 = LR - 2;
+    return (R_)((double)row[i0] * (1.0 - f) + (double)row[i0 + 1] * f);
+}
+
 // S(za, zb) = vn * vd: vd the product over the columns in depmask (groups with any column there), vn over the
 // others; zb: a pointer to d coordinates.
 template <typename R_>
@@ -74,17 +86,7 @@ __device__ __forceinline__ void pair_value(const blml_pairtab_t &t, const R_ *T,
 #pragma unroll
     for (int g = 0; g < BLML_PAIR_MAXG; ++g) {
         if (g < t.G) {
-            R_ tv;
-            if (tor[g]) {
-                const long i = r2[g] < t.LR ? r2[g] : t.LR - 1;
-                tv = RT[(long)g * t.LR + i];
-            } else {
-                const double x = sqrt((double)r2[g]) * BLML_RADIAL_Q;
-                long i0 = (long)x;
-                const double f = x - (double)i0;
-                if (i0 > t.LR - 2) i0 = t.LR - 2;
-                tv = (R_)((double)RT[(long)g * t.LR + i0] * (1.0 - f) + (double)RT[(long)g * t.LR + i0 + 1] * f);
-            }
+            const R_ tv = pair_group_row<R_>(RT + (long)g * t.LR, t.LR, r2[g], tor[g]);
             if (gdep[g]) vd *= tv;
             else v *= tv;
         }
@@ -142,6 +144,165 @@ __global__ void pair_rows_sum_kernel(const double *part, int R, int nslice, doub
     double a = 0.0;
     for (int y = 0; y < nslice; ++y) a += part[(long)y * R + r];
     s[r] = a;
+}
+
+// ------------------------------------------------------------------------------------------------ derivative rows
+// The derivative entries as the kernel reads them: blml_pairgrad_t with each column entry's table geometry resolved
+// on the host (rank table or raw offsets, its half width and rank size).
+struct pair_grad_dev {
+    int nW, U;
+    int wunit[BLML_PAIR_MAXW], wcol[BLML_PAIR_MAXW], wgrp[BLML_PAIR_MAXW], wrank[BLML_PAIR_MAXW];
+    int dim_unit[BLML_PAIR_MAXD], grp_unit[BLML_PAIR_MAXG];
+    long dbase[BLML_PAIR_MAXW], wJ[BLML_PAIR_MAXW], wmk[BLML_PAIR_MAXW];
+    long ndT;
+    const void *dT, *dR;
+};
+
+// part[(slice * (1 + nW) + 0) * R + r] = the slice's sum of S(ZA_r, ZB_j) w_j, part[(slice * (1 + nW) + 1 + q) * R + r]
+// = its sum of dS_q(ZA_r, ZB_j) w_j: dS_q = entry q's derivative value times the product of the other units' factors
+// (blml_local_gpu.py:21-40, 94-103).  The value is the product in unit order.  Staging as pair_rows_kernel.
+template <typename R_, bool TL>
+__global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_grad_kernel(blml_pairtab_t t, pair_grad_dev gd, const int *ZA,
+                                                                      const int *ZB, const double *w, int R, int M,
+                                                                      int slice, double *part) {
+    unsigned char *lds = g_dyn_lds;
+    const long tbytes = TL ? ((t.nT * (long)sizeof(R_) + 15) / 16) * 16 : 0;
+    R_ *lT = reinterpret_cast<R_ *>(lds);
+    int *lz = reinterpret_cast<int *>(lds + tbytes);
+    double *lw = reinterpret_cast<double *>(lds + tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16);
+    const R_ *T = reinterpret_cast<const R_ *>(t.T);
+    if (TL) {
+        for (long i = threadIdx.x; i < t.nT; i += BLML_PAIR_NT) lT[i] = T[i];
+        T = lT;
+    }
+    const R_ *RT = reinterpret_cast<const R_ *>(t.RT);
+    const R_ *dT = reinterpret_cast<const R_ *>(gd.dT);
+    const R_ *dR = reinterpret_cast<const R_ *>(gd.dR);
+    const int d = t.d;
+    const int r = blockIdx.x * BLML_PAIR_NT + threadIdx.x;
+    const bool has = r < R;
+    int za[BLML_PAIR_MAXD];
+#pragma unroll
+    for (int k = 0; k < BLML_PAIR_MAXD; ++k) za[k] = (has && k < d) ? ZA[(long)r * d + k] : 0;
+    const long j0 = (long)blockIdx.y * slice;
+    const long j1 = j0 + slice < M ? j0 + slice : M;
+    double acc = 0.0, dacc[BLML_PAIR_MAXW];
+#pragma unroll
+    for (int q = 0; q < BLML_PAIR_MAXW; ++q) dacc[q] = 0.0;
+    for (long c0 = j0; c0 < j1; c0 += BLML_PAIR_CH) {
+        const int n = (int)(j1 - c0 < BLML_PAIR_CH ? j1 - c0 : BLML_PAIR_CH);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n * d; i += BLML_PAIR_NT) lz[i] = ZB[c0 * d + i];
+        for (int i = threadIdx.x; i < n; i += BLML_PAIR_NT) lw[i] = w[c0 + i];
+        __syncthreads();
+        for (int jj = 0; jj < n; ++jj) {
+            const double wj = lw[jj];
+            if (wj == 0.0) continue;
+            const int *zb = lz + jj * d;
+            R_ fac[BLML_PAIR_MAXD];
+            long r2[BLML_PAIR_MAXG];
+            bool tor[BLML_PAIR_MAXG];
+#pragma unroll
+            for (int u = 0; u < BLML_PAIR_MAXD; ++u) fac[u] = (R_)1;
+#pragma unroll
+            for (int g = 0; g < BLML_PAIR_MAXG; ++g) { r2[g] = 0; tor[g] = false; }
+#pragma unroll
+            for (int k = 0; k < BLML_PAIR_MAXD; ++k) {
+                if (k < d) {
+                    const int kd = t.kind[k];
+                    const long a = za[k], b = zb[k], off = a - b;
+                    if (kd == 2 || kd == 4) {
+                        const int gk = t.grp[k];
+#pragma unroll
+                        for (int g = 0; g < BLML_PAIR_MAXG; ++g) {
+                            if (g == gk) {
+                                if (kd == 2) {
+                                    r2[g] += off * off;
+                                } else {
+                                    const long nb = t.nbins[k];
+                                    r2[g] = r2[g] * nb + (off < 0 ? off + nb : off);
+                                    tor[g] = true;
+                                }
+                            }
+                        }
+                    } else {
+                        const R_ tv = T[pair_col_index(kd, a, b, t.base[k], t.J[k], t.mk[k], t.nbins[k], t.nT)];
+                        const int uk = gd.dim_unit[k];
+#pragma unroll
+                        for (int u = 0; u < BLML_PAIR_MAXD; ++u)
+                            if (u == uk) fac[u] = tv;
+                    }
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < BLML_PAIR_MAXG; ++g) {
+                if (g < t.G) {
+                    const R_ tv = pair_group_row<R_>(RT + (long)g * t.LR, t.LR, r2[g], tor[g]);
+                    const int ug = gd.grp_unit[g];
+#pragma unroll
+                    for (int u = 0; u < BLML_PAIR_MAXD; ++u)
+                        if (u == ug) fac[u] = tv;
+                }
+            }
+            R_ v = (R_)1;
+#pragma unroll
+            for (int u = 0; u < BLML_PAIR_MAXD; ++u)
+                if (u < gd.U) v *= fac[u];
+            acc += (double)v * wj;
+#pragma unroll
+            for (int q = 0; q < BLML_PAIR_MAXW; ++q) {
+                if (q < gd.nW) {
+                    const int uq = gd.wunit[q], g = gd.wgrp[q], kc = gd.wcol[q];
+                    long a = 0;
+#pragma unroll
+                    for (int k = 0; k < BLML_PAIR_MAXD; ++k)
+                        if (k == kc) a = za[k];
+                    const long b = kc >= 0 ? zb[kc] : 0;
+                    R_ dv;
+                    if (g >= 0) {
+                        long rg = 0;
+                        bool tg = false;
+#pragma unroll
+                        for (int gg = 0; gg < BLML_PAIR_MAXG; ++gg)
+                            if (gg == g) { rg = r2[gg]; tg = tor[gg]; }
+                        dv = pair_group_row<R_>(dR + (long)q * t.LR, t.LR, rg, tg);
+                        if (!tg && kc >= 0) dv *= (R_)(a - b);
+                    } else {
+                        long i = gd.wrank[q] ? gd.dbase[q] + a * gd.wmk[q] + b : gd.dbase[q] + a - b + gd.wJ[q];
+                        i = i < 0 ? 0 : (i >= gd.ndT ? gd.ndT - 1 : i);
+                        dv = dT[i];
+                    }
+                    R_ ex = (R_)1;
+#pragma unroll
+                    for (int u = 0; u < BLML_PAIR_MAXD; ++u)
+                        if (u < gd.U && u != uq) ex *= fac[u];
+                    dacc[q] += (double)(dv * ex) * wj;
+                }
+            }
+        }
+    }
+    if (has) {
+        const long at = (long)blockIdx.y * (1 + gd.nW);
+        part[at * R + r] = acc;
+#pragma unroll
+        for (int q = 0; q < BLML_PAIR_MAXW; ++q)
+            if (q < gd.nW) part[(at + 1 + q) * R + r] = dacc[q];
+    }
+}
+
+// s[r], ds[r * nW + q]: the slices' partials added in slice order
+__global__ void pair_rows_grad_sum_kernel(const double *part, int R, int nslice, int nW, double *s, double *ds) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const long stride = (long)(1 + nW) * R;
+    double a = 0.0;
+    for (int y = 0; y < nslice; ++y) a += part[y * stride + r];
+    s[r] = a;
+    for (int q = 0; q < nW; ++q) {
+        a = 0.0;
+        for (int y = 0; y < nslice; ++y) a += part[y * stride + (long)(1 + q) * R + r];
+        ds[(long)r * nW + q] = a;
+    }
 }
 
 // out[r * M + j] = S(ZA_r, ZB_j); with depmask, out_m[r * M + j] = the product over the columns not in depmask.

@@ -165,6 +165,34 @@ typedef struct {
 int blml_pair_rows(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, const double *w, int64_t R,
                    int64_t M, double *s, void *stream);
 
+/* Derivative rows (the reference's rows(..., grad=True): rows_kernel's dval and its prefix / suffix products,
+ * blml_local_gpu.py:21-40, 94-103; blml_kernels.py:165-182 builds the rows).  Entry q is a derivative of unit
+ * wunit[q]'s factor: for a column (wgrp[q] < 0) the row at dT[dbase[q]:] of column wcol[q], read at
+ * dbase[q] + za * mk + zb for a rank column, else at dbase[q] + za - zb + J (raw offsets, circular columns too); for
+ * a group (wgrp[q] = g) the row dR[q] (LR values, as tab->RT): a torus table read as is at the group's row-major
+ * circular offset, or a row on the r grid interpolated as the value rows and, when wcol[q] >= 0 (a coordinate
+ * shift), times the offset za - zb of column wcol[q].  Units: the columns' factors (dim_unit; -1 for a group's
+ * column) and the groups' (grp_unit), U in all.  dT, dR: device, REAL as tab->T. */
+#define BLML_PAIR_MAXW 16     /* derivative entries per launch */
+
+typedef struct {
+    int32_t nW, U, reserved[2];
+    int32_t wunit[BLML_PAIR_MAXW];
+    int32_t wcol[BLML_PAIR_MAXW];
+    int32_t wgrp[BLML_PAIR_MAXW];
+    int32_t dim_unit[BLML_PAIR_MAXD];
+    int32_t grp_unit[BLML_PAIR_MAXG];
+    int64_t dbase[BLML_PAIR_MAXW];
+    int64_t ndT;                     /* values of dT */
+    const void *dT;
+    const void *dR;                  /* nW rows of tab->LR values (rows of column entries unused) */
+} blml_pairgrad_t;
+
+/* blml_pair_rows with derivatives: s as blml_pair_rows, ds[r * nW + q] = sum_j dS_q(ZA_r, ZB_j) w_j (FP64), dS_q
+ * the product of entry q's derivative value and the other units' factors (blml_local_gpu.py:94-103). */
+int blml_pair_rows_grad(const blml_pairtab_t *tab, const blml_pairgrad_t *grad, const int32_t *ZA, const int32_t *ZB,
+                        const double *w, int64_t R, int64_t M, double *s, double *ds, void *stream);
+
 /* The explicit block out[r * M + j] = S(ZA_r, ZB_j) (REAL); with depmask != 0 also out_m = the product over the
  * columns not in depmask (the reference's joint + marginal pass). */
 int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, int64_t R, int64_t M,

@@ -8,6 +8,8 @@ Restated (numpy, float64):
                              _col_index (blml_local.py:86-96), radial rows by linear interpolation on the r grid
                              (RADIAL_Q = 4, blml_trivial.py:218), torus tables at the row-major circular offset
   pair_cols(ZA, ZB, tab, depmask)   blml_local.py:271-282 / blml_local_gpu.py:331-361 (joint and marginal)
+  pair_rows_grad(ZA, ZB, w, tab)   with grad (blml_local.py:99-116 _dval, 146-191; blml_local_gpu.py:21-40, 94-103):
+                             ds[r, q] = sum_j dval_q(r, j) prod_{u != wunit[q]} fac_u(r, j) w_j
 Table builders for the tests (the rows ProductKernel.tables holds, blml_kernels.py:133-200, for the kinds with
 closed forms): offset rows K(o h), o = -J..J (sinc, slits, unfolded dirichlet); circular rows K(m h), m = 0..nb-1
 (folded dirichlet); rank tables K((v_a - v_b) h) over the occupied values; a periodic ball's torus table over its
@@ -15,7 +17,9 @@ columns' circular offsets.
 
 Pinning (tests/test_oracle_pairs.py): pair_rows / pair_cols against the dense Gram gram_int @ w
 (blml_kernels.py:243-251; the reference's own check, results/pball_parity.py:66-76) for tables built from the
-kernels' closed forms; the lookup of radial rows against a brute-force per-pair loop of the same rule."""
+kernels' closed forms; the lookup of radial rows against a brute-force per-pair loop of the same rule; pair_rows_grad
+over the derivative tables of blml_amd.local.lattice_tables against central finite differences of the dense Gram
+(the kernels' closed forms at fc e^+-eps, or at coordinate offsets +-eps), which pins the tables and the lookup."""
 import numpy as np
 
 from oracle.blml_oracle import kernel_eval
@@ -97,6 +101,69 @@ def pair_rows(ZA, ZB, w, tab, chunk=512):
             prod = prod * fac[u]
         s[a0:a0 + chunk] = prod @ w
     return s
+
+
+def _group_r2(A, B, tab, g):
+    """The group's r^2 (radial) or row-major circular offset (torus) of every pair, and whether it is a torus."""
+    kinds, nbins, grp = (np.asarray(tab[k]) for k in ('kinds', 'nbins', 'grp'))
+    r2 = np.zeros((len(A), len(B)), np.int64)
+    tor = False
+    for k in range(A.shape[1]):
+        if grp[k] != g:
+            continue
+        off = A[:, k][:, None] - B[:, k][None, :]
+        if kinds[k] == 4:
+            r2 = r2 * int(nbins[k]) + np.mod(off, int(nbins[k]))
+            tor = True
+        else:
+            r2 = r2 + off * off
+    return r2, tor
+
+
+def _dval(q, A, B, tab):
+    """Entry q's derivative value of every pair (blml_local.py:99-116 _dval): a column's row (raw offsets, or its rank
+    table), a group's row on the r grid (times the offset of column wcol[q] for a shift entry), or a torus table
+    read as is."""
+    g, k = int(tab['wgrp'][q]), int(tab['wcol'][q])
+    if g >= 0:
+        row = np.atleast_2d(np.asarray(tab['dR'], float))[q]
+        r2, tor = _group_r2(A, B, tab, g)
+        if tor:
+            return row[r2]
+        t = np.sqrt(r2 * 1.0) * RADIAL_Q
+        i0 = t.astype(np.int64)
+        f = t - i0
+        v = row[i0] * (1.0 - f) + row[i0 + 1] * f
+        if k >= 0:
+            v = v * (A[:, k][:, None] - B[:, k][None, :])
+        return v
+    dT = np.asarray(tab['dT'], float)
+    za, zb = A[:, k][:, None], B[:, k][None, :]
+    at = int(tab['dbase'][q])
+    if int(tab['kinds'][k]) == 3:
+        return dT[at + za * int(tab['mk'][k]) + zb]
+    return dT[at + za - zb + int(tab['Jk'][k])]
+
+
+def pair_rows_grad(ZA, ZB, w, tab, chunk=256):
+    """(s, ds): pair_rows and ds[r, q] = sum over sources with w_j != 0 of entry q's derivative value times the other
+    units' factors, times w_j (blml_local.py:146-191 _rows_numba with grad; blml_local_gpu.py:94-103)."""
+    A, B = rank_coords(ZA, tab), rank_coords(ZB, tab)
+    w = np.asarray(w, float)
+    keep = w != 0
+    B, w = B[keep], w[keep]
+    nW = int(tab['nW'])
+    s = np.zeros(len(A))
+    ds = np.zeros((len(A), max(nW, 1)))
+    for a0 in range(0, len(A), chunk):
+        Ac = A[a0:a0 + chunk]
+        fac = _factors(Ac, B, tab)
+        s[a0:a0 + chunk] = np.prod(fac, axis=0) @ w
+        for q in range(nW):
+            u = int(tab['wunit'][q])
+            ex = np.prod(np.delete(fac, u, axis=0), axis=0)
+            ds[a0:a0 + chunk, q] = (_dval(q, Ac, B, tab) * ex) @ w
+    return s, ds
 
 
 def pair_cols(ZA, ZB, tab, depmask=0):

@@ -83,5 +83,48 @@ def test_cols_marginal_and_errors():
     bad[0, 0] = 100                        # beyond the offset row's half width: the tables predate this bin
     with pytest.raises(RuntimeError):
         local.rows(bad, ZB, np.ones(len(ZB)), tab)
-    with pytest.raises(NotImplementedError):
-        local.rows(ZA, ZB, np.ones(len(ZB)), tab, grad=True)
+    s, ds = local.rows(ZA, ZB, np.ones(len(ZB)), tab, grad=True)      # no derivative entries: ds zeros
+    assert np.array_equal(s, local.rows(ZA, ZB, np.ones(len(ZB)), tab)[0]) and ds.shape == (len(ZA), 1) and not ds.any()
+
+
+@pytest.mark.parametrize('f32', [False, True], ids=['f64', 'f32'])
+def test_derivative_rows_match_oracle(f32):
+    """rows(grad=True) (blml_pair_rows_grad) against the oracle's pair_rows_grad on the same tables: column entries
+    (raw-offset rows, a rank table, a dirichlet period row; the full 16 entries), a ball group's fc and shift rows,
+    a periodic ball's torus shift rows, and a large case with many slices and a table beyond LDS.  s to the value
+    rows' bar; ds[:, q] to 1e-11 (FP64) / 1e-4 (FP32) of max |ds[:, q]|."""
+    from blml_amd import local
+    from test_oracle_pairs import _column_case
+    rng = np.random.default_rng(41)
+    Z, kinds, fc, periods, want, _, _ = _column_case(rng)
+    cases = [('columns', local.lattice_tables(kinds, fc, periods, np.ones(4), Z, want=want), Z, Z),
+             ('columns, 16 entries', local.lattice_tables(kinds, fc, periods, np.ones(4), Z, want=want + want[::-1]),
+              Z, Z)]
+    Zb = _bins(rng, (20, 18, 9), 300)
+    cases.append(('ball x sinc', local.lattice_tables(
+        ['ball', 'ball', 'sinc'], [0.3, 0.3, 0.4], None, np.ones(3), Zb,
+        want=[(1, 'fc'), (1, ('shift', 0)), (1, ('shift', 1)), (0, 'fc'), (0, 'shift')]), Zb[:120], Zb))
+    Zp = _bins(rng, (10, 8), 60)
+    cases.append(('pball', local.lattice_tables(['pball', 'pball'], [0.45, 0.45], [10.0, 8.0], [1.0, 1.0], Zp,
+                                                want=[(0, 'shift'), (0, ('shift', 1))]), Zp, Zp))
+    ZL = np.concatenate([_bins(rng, (1500, 1500, 1500), 12000), np.array([[0, 0, 0], [1499, 1499, 1499]])])
+    cases.append(('3d slits large', local.lattice_tables(
+        ['slits:0.5'] * 3, [0.2] * 3, None, np.ones(3), ZL,
+        want=[(k, p) for k in range(3) for p in ('fc', 'shift')]), ZL[:1000], ZL))
+    for name, tab, ZA, ZB in cases:
+        _note(f'derivative rows {name} f32={f32} nA={len(ZA)} nB={len(ZB)} nW={tab["nW"]}')
+        w = rng.standard_normal(len(ZB))
+        w[::5] = 0.0
+        s, ds = local.rows(ZA, ZB, w, tab, grad=True, f32=f32)
+        s0, ds0 = OP.pair_rows_grad(ZA, ZB, w, tab)
+        scale = np.abs(OP.pair_rows(ZA, ZB, np.abs(w), dict(tab, T=np.abs(tab['T']), R=np.abs(tab['R'])))).max()
+        assert np.abs(s - s0).max() <= (1e-5 if f32 else 1e-12) * scale, name
+        assert ds.shape == ds0.shape
+        for q in range(int(tab['nW'])):
+            ref = np.abs(ds0[:, q]).max()
+            assert np.abs(ds[:, q] - ds0[:, q]).max() <= (1e-4 if f32 else 1e-11) * ref, (name, q)
+    big = local.lattice_tables(kinds, fc, periods, np.ones(4), Z, want=want * 3)
+    with pytest.raises(ValueError, match='max 16'):
+        local.rows(Z, Z, np.ones(len(Z)), big, grad=True)
+    s, ds = local.rows(Z, Z, np.ones(len(Z)), local.lattice_tables(kinds, fc, periods, np.ones(4), Z), grad=True)
+    assert ds.shape == (len(Z), 1) and not ds.any()

@@ -139,3 +139,114 @@ def test_ball_tables_radial_rows():
     assert np.abs(got - G @ w).max() <= bound
     with pytest.raises(ValueError, match='equal across the group'):
         local.lattice_tables(['ball', 'ball'], [0.35, 0.3], None, [1.0, 1.0], Z[:, :2])
+
+
+# ------------------------------------------------------------------------------------------------ derivative rows
+def _fd_rows(units, h, ZA, ZB, w, ui, how, eps=1e-5):
+    """Central finite difference of the dense Gram's rows (the kernels' closed forms, oracle.kernel_eval) in one
+    unit's log fc ('logfc', a group's fc scaled together), log period ('logP') or coordinate offset ('t', i)."""
+    def gram(delta):
+        G = np.ones((len(ZA), len(ZB)))
+        for u, (kind, fcs, pers, dims) in enumerate(units):
+            fcs, pers = np.asarray(fcs, float), np.asarray(pers, float)
+            diffs = [(ZA[:, k, None] - ZB[None, :, k]) * h[k] for k in dims]
+            if u == ui:
+                if how == 'logfc':
+                    fcs = fcs * np.exp(delta)
+                elif how == 'logP':
+                    pers = pers * np.exp(delta)
+                else:
+                    diffs[how[1]] = diffs[how[1]] + delta
+            G = G * O.kernel_eval(kind, fcs, pers, diffs)
+        return G
+    return (gram(eps) - gram(-eps)) @ w / (2 * eps)
+
+
+def _column_case(rng):
+    """sinc, slits, dirichlet (folded on P = 16) and a rank column (three occupied values) with every derivative
+    entry the reference allows for them."""
+    n = 70
+    Z = np.stack([rng.integers(0, 14, n), rng.integers(0, 12, n), rng.integers(0, 16, n),
+                  rng.choice([0, 7, 19], n)], 1)
+    kinds, fc, periods = ['sinc', 'slits:0.5', 'dirichlet', 'sinc'], [0.4, 0.45, 0.4, 0.3], [None, None, 16.0, None]
+    want = [(0, 'fc'), (0, 'shift'), (1, 'fc'), (1, 'shift'), (2, 'period'), (2, 'shift'), (3, 'fc'), (3, 'shift')]
+    units = [('sinc', [0.4], [0.0], [0]), ('slits:0.5', [0.45], [0.0], [1]), ('dirichlet', [0.4], [16.0], [2]),
+             ('sinc', [0.3], [0.0], [3])]
+    how = {'fc': 'logfc', 'period': 'logP', 'shift': ('t', 0)}
+    return Z, kinds, fc, periods, want, units, [(u, how[p]) for u, p in want]
+
+
+def test_derivative_rows_match_finite_differences():
+    """rows(grad=True) through the derivative tables of lattice_tables (raw-offset rows, a rank table, a dirichlet
+    period row) equals central differences of the dense Gram in each unit's log fc / log period / offset to 1e-7 of
+    the rows' scale; the value rows are unchanged.  A wrong sign, a dropped term or a transposed offset fails it."""
+    from blml_amd import local
+    rng = np.random.default_rng(31)
+    Z, kinds, fc, periods, want, units, fd = _column_case(rng)
+    h = np.ones(4)
+    tab = local.lattice_tables(kinds, fc, periods, h, Z, want=want)
+    assert int(tab['nW']) == len(want) and int(tab['kinds'][3]) == 3 and int(tab['kinds'][2]) == 1
+    w = rng.standard_normal(len(Z))
+    w[::4] = 0.0
+    s, ds = OP.pair_rows_grad(Z, Z, w, tab)
+    assert np.abs(s - OP.pair_rows(Z, Z, w, tab)).max() < 1e-14 * np.abs(s).max()
+    for q, (ui, how) in enumerate(fd):
+        ref = _fd_rows(units, h, Z, Z, w, ui, how)
+        assert np.abs(ds[:, q] - ref).max() < 1e-7 * np.abs(ref).max(), (q, want[q])
+
+
+def test_derivative_rows_ball_and_pball():
+    """A ball group's fc and shift rows (on the r grid): exact at pairs on the grid (one column offset 0) to 1e-7, and
+    within the linear interpolation's error elsewhere (3e-2 of the scale; a wrong term is O(1)); its table entries
+    are the closed forms' differences at the grid points.  A periodic ball's shift rows (torus tables) to 1e-7."""
+    from blml_amd import local
+    rng = np.random.default_rng(32)
+    kinds, fc, h = ['ball', 'ball', 'sinc'], [0.35, 0.35, 0.4], np.ones(3)
+    want = [(1, 'fc'), (1, ('shift', 0)), (1, ('shift', 1)), (0, 'fc'), (0, 'shift')]
+    units = [('sinc', [0.4], [0.0], [2]), ('ball', [0.35, 0.35], [0.0, 0.0], [0, 1])]
+    fd = [(1, 'logfc'), (1, ('t', 0)), (1, ('t', 1)), (0, 'logfc'), (0, ('t', 0))]
+    Z = _bins(rng, (14, 12, 10), 90)
+    tab = local.lattice_tables(kinds, fc, None, h, Z, want=want)
+    r = np.arange(np.asarray(tab['dR']).shape[1]) / OP.RADIAL_Q
+    drow = np.asarray(tab['dR'])
+    for q, how in ((0, 'logfc'), (1, ('t', 0))):
+        e = 1e-5
+        if how == 'logfc':
+            ref = (O.kernel_eval('ball', [0.35 * np.exp(e)] * 2, [0, 0], [r, 0 * r]) -
+                   O.kernel_eval('ball', [0.35 * np.exp(-e)] * 2, [0, 0], [r, 0 * r])) / (2 * e)
+            got = drow[q]
+        else:                                      # the shift row times the offset r along column 0
+            ref = (O.kernel_eval('ball', [0.35] * 2, [0, 0], [r + e, 0 * r]) -
+                   O.kernel_eval('ball', [0.35] * 2, [0, 0], [r - e, 0 * r])) / (2 * e)
+            got = drow[q] * r
+        assert np.abs(got - ref).max() < 1e-8 * np.abs(ref).max(), q
+    w = rng.standard_normal(len(Z))
+    s, ds = OP.pair_rows_grad(Z, Z, w, tab)
+    for q, (ui, how) in enumerate(fd):
+        ref = _fd_rows(units, h, Z, Z, w, ui, how)
+        assert np.abs(ds[:, q] - ref).max() < 3e-2 * np.abs(ref).max(), (q, want[q])
+    Zg = Z.copy()
+    Zg[:, 1] = 5                                   # every pair's distance |off_0| on the grid
+    tabg = local.lattice_tables(kinds, fc, None, h, Zg, want=want)
+    s, ds = OP.pair_rows_grad(Zg, Zg, w, tabg)
+    for q, (ui, how) in enumerate(fd):
+        if q == 2:
+            assert np.abs(ds[:, q]).max() == 0.0   # d/dt_1 at offset 0 along column 1
+            continue
+        ref = _fd_rows(units, h, Zg, Zg, w, ui, how)
+        assert np.abs(ds[:, q] - ref).max() < 1e-7 * np.abs(ref).max(), (q, want[q])
+    Zp = _bins(rng, (10, 8), 50)
+    tabp = local.lattice_tables(['pball', 'pball'], [0.45, 0.45], [10.0, 8.0], [1.0, 1.0], Zp,
+                                want=[(0, 'shift'), (0, ('shift', 1))])
+    wp = rng.standard_normal(len(Zp))
+    s, ds = OP.pair_rows_grad(Zp, Zp, wp, tabp)
+    unitp = [('pball', [0.45, 0.45], [10.0, 8.0], [0, 1])]
+    for q in range(2):
+        ref = _fd_rows(unitp, np.ones(2), Zp, Zp, wp, 0, ('t', q))
+        assert np.abs(ds[:, q] - ref).max() < 1e-7 * np.abs(ref).max(), q
+    with pytest.raises(ValueError, match='shifts only'):
+        local.lattice_tables(['pball', 'pball'], [0.45, 0.45], [10.0, 8.0], [1.0, 1.0], Zp, want=[(0, 'fc')])
+    with pytest.raises(ValueError, match='whole harmonics'):
+        local.lattice_tables(['dirichlet'], [0.4], [16.0], [1.0], Zp[:, :1], want=[(0, 'fc')])
+    with pytest.raises(ValueError, match='fixed shape'):
+        local.lattice_tables(['slits:0.5'], [0.4], None, [1.0], Zp[:, :1], want=[(0, 'fill')])
