@@ -4,6 +4,7 @@ computed by libblml_hip.so (blml_pair_rows / blml_pair_cols).
 
     s, ds = rows(ZA, ZB, w, tab)           s_r = sum_j S(ZA_r, ZB_j) w_j   (ds: zeros)
     s, ds = rows(ZA, ZB, w, tab, grad=True)   with ds_rq = sum_j dS_q(ZA_r, ZB_j) w_j for tab's derivative entries
+    s, ds = rows_multi(ZA, ZB, Wt, tab, umask, grad)   several densities (unit masks) in one pass
     out = cols(ZA, ZB, tab)                the block S(ZA_r, ZB_j) (R, M)
     out, out_m = cols(ZA, ZB, tab, depmask=mask)   with the marginal over the columns not in mask
 
@@ -17,7 +18,7 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ['rows', 'cols', 'rank_coords', 'lattice_tables']
+__all__ = ['rows', 'rows_multi', 'cols', 'rank_coords', 'lattice_tables']
 
 MAXD, MAXG = 8, 8
 RADIAL_Q = 4
@@ -30,7 +31,7 @@ class PairTab(ctypes.Structure):
                 ('nT', ctypes.c_int64), ('LR', ctypes.c_int64), ('T', ctypes.c_void_p), ('RT', ctypes.c_void_p)]
 
 
-MAXW = 16
+MAXW, MAXM = 16, 4
 
 
 class PairGrad(ctypes.Structure):
@@ -296,23 +297,25 @@ def _check_dspans(A, B, tab):
                                f"width {Jk[k]} (stale tables)")
 
 
-def _rows_grad(ZA, ZB, w, tab, f32, ranked):
+def _rows_fused(ZA, ZB, Wt, tab, umask, grad, f32, ranked):
+    """s (R, Mm), ds (R, Mm, nW) through blml_pair_rows_grad (umask None: one density of every unit) or
+    blml_pair_rows_multi."""
     torch = _torch()
-    nW, U = int(tab['nW']), int(tab['U'])
-    if nW > MAXW or U > MAXD:
-        raise ValueError(f"derivative rows: {U} kernel units (max {MAXD}), {nW} derivative entries (max {MAXW})")
+    nW, U = (int(tab['nW']) if grad else 0), int(tab['U'])
+    Wt = np.ascontiguousarray(np.asarray(Wt, np.float64).reshape(len(np.atleast_2d(ZB)), -1))
+    Mm = Wt.shape[1]
+    if Mm > MAXM or Mm * nW > MAXW or U > MAXD:
+        raise ValueError(f"pair rows: {U} kernel units (max {MAXD}), {nW} derivative entries x {Mm} densities "
+                         f"(max {MAXW}), {Mm} densities (max {MAXM})")
     A = np.atleast_2d(np.asarray(ZA if ranked else rank_coords(ZA, tab), np.int64))
     B = np.atleast_2d(np.asarray(ZB if ranked else rank_coords(ZB, tab), np.int64))
-    if A.shape[1] == B.shape[1]:
+    if A.shape[1] == B.shape[1] and nW:
         _check_dspans(A, B, tab)
     t, dA, dB, keep = _prepare(A, B, tab, f32, True)
     R, M = dA.shape[0], dB.shape[0]
-    wv = np.ascontiguousarray(w, np.float64).ravel()
-    if len(wv) != M:
-        raise ValueError("w: one weight per row of ZB")
     rt = torch.float32 if f32 else torch.float64
-    ddT = torch.as_tensor(np.asarray(tab['dT'], np.float64).ravel()).to('cuda', rt).contiguous()
-    dRh = np.atleast_2d(np.asarray(tab['dR'], np.float64))
+    ddT = torch.as_tensor(np.asarray(tab.get('dT', np.zeros(1)), np.float64).ravel()).to('cuda', rt).contiguous()
+    dRh = np.atleast_2d(np.asarray(tab.get('dR', np.zeros((1, 2))), np.float64))
     if dRh.shape[1] != t.LR:                      # derivative rows on the value rows' length
         pad = np.zeros((dRh.shape[0], t.LR))
         n = min(t.LR, dRh.shape[1])
@@ -328,23 +331,45 @@ def _rows_grad(ZA, ZB, w, tab, f32, ranked):
         g.dbase[q] = int(tab['dbase'][q])
     for k in range(A.shape[1]):
         g.dim_unit[k] = int(tab['dim_unit'][k])
-    G = _groups(tab)
-    for c in range(G):
+    for c in range(_groups(tab)):
         g.grp_unit[c] = int(tab['grp_unit'][c])
     g.ndT = ddT.numel()
     g.dT = ddT.data_ptr()
     g.dR = ddR.data_ptr()
-    dw = torch.from_numpy(wv).to('cuda')
-    s = torch.zeros(max(R, 1), dtype=torch.float64, device='cuda')
-    ds = torch.zeros(max(R * nW, 1), dtype=torch.float64, device='cuda')
+    dw = torch.from_numpy(Wt.ravel().copy()).to('cuda')
+    s = torch.zeros(max(R * Mm, 1), dtype=torch.float64, device='cuda')
+    ds = torch.zeros(max(R * Mm * nW, 1), dtype=torch.float64, device='cuda')
     lib = _lib.load()
-    _lib.check(lib.blml_pair_rows_grad(ctypes.byref(t), ctypes.byref(g), ctypes.c_void_p(dA.data_ptr()),
-                                       ctypes.c_void_p(dB.data_ptr()), ctypes.c_void_p(dw.data_ptr()), R, M,
-                                       ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(ds.data_ptr()),
-                                       ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
-    out, dout = s.cpu().numpy()[:R], ds.cpu().numpy()[:R * nW].reshape(R, nW)
+    P = ctypes.c_void_p
+    st = P(torch.cuda.current_stream().cuda_stream)
+    if umask is None:
+        rc = lib.blml_pair_rows_grad(ctypes.byref(t), ctypes.byref(g), P(dA.data_ptr()), P(dB.data_ptr()),
+                                     P(dw.data_ptr()), R, M, P(s.data_ptr()), P(ds.data_ptr()), st)
+    else:
+        um = np.ascontiguousarray(np.asarray(umask, np.int64).reshape(Mm, U) != 0, np.int32)
+        rc = lib.blml_pair_rows_multi(ctypes.byref(t), ctypes.byref(g), P(dA.data_ptr()), P(dB.data_ptr()),
+                                      P(dw.data_ptr()), um.ctypes.data_as(P), Mm, R, M, P(s.data_ptr()),
+                                      P(ds.data_ptr()), st)
+    _lib.check(rc)
+    out = s.cpu().numpy()[:R * Mm].reshape(R, Mm)
+    dout = ds.cpu().numpy()[:R * Mm * nW].reshape(R, Mm, nW)
     del keep, ddT, ddR
+    if nW == 0:
+        dout = np.zeros((R, Mm, max(int(tab.get('nW', 0)), 1)))
     return out, dout
+
+
+def rows_multi(ZA, ZB, Wt, tab, umask, grad=False, f32=False, ranked=False):
+    """Several densities in one pass (blml_local_gpu.py:107-180, 432-439): s (R, Mm) with s[r, m] = sum_j
+    prod_{u: umask[m, u]} fac_u(ZA_r, ZB_j) Wt[j, m] (the joint and marginals: units left out of a density count
+    as 1), and ds (R, Mm, max(nW, 1)): with grad, each density's derivative rows (0 where its mask leaves the
+    entry's unit out), else zeros.  At most 4 densities and 16 densities x derivative entries."""
+    Wt = np.asarray(Wt, np.float64)
+    if Wt.ndim != 2 or Wt.shape[0] != len(np.atleast_2d(ZB)):
+        raise ValueError("Wt: (M, densities), one row per row of ZB")
+    if np.shape(umask) != (Wt.shape[1], int(tab['U'])):
+        raise ValueError(f"umask: (densities, units) = ({Wt.shape[1]}, {int(tab['U'])})")
+    return _rows_fused(ZA, ZB, Wt, tab, umask, grad, f32, ranked)
 
 
 def rows(ZA, ZB, w, tab, grad=False, f32=False, ranked=False):
@@ -352,7 +377,11 @@ def rows(ZA, ZB, w, tab, grad=False, f32=False, ranked=False):
     derivative entries in the tables (lattice_tables(want=...)), ds[r, q] = sum_j dS_q(ZA_r, ZB_j) w_j, dS_q entry q's
     derivative of its unit's factor times the other units' factors (blml_local_gpu.py:94-103, 423-429); else zeros."""
     if grad and int(tab.get('nW', 0)) > 0:
-        return _rows_grad(ZA, ZB, w, tab, f32, ranked)
+        wv = np.ascontiguousarray(w, np.float64).ravel()
+        if len(wv) != len(np.atleast_2d(ZB)):
+            raise ValueError("w: one weight per row of ZB")
+        s, ds = _rows_fused(ZA, ZB, wv[:, None], tab, None, True, f32, ranked)
+        return s[:, 0], ds[:, 0, :]
     torch = _torch()
     t, dA, dB, keep = _prepare(ZA, ZB, tab, f32, ranked)
     R, M = dA.shape[0], dB.shape[0]

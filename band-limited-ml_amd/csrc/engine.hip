@@ -1419,19 +1419,32 @@ int blml_pair_rows(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *
     return BLML_OK;
 }
 
-int blml_pair_rows_grad(const blml_pairtab_t *tab, const blml_pairgrad_t *grad, const int32_t *ZA, const int32_t *ZB,
-                        const double *w, int64_t R, int64_t M, double *s, double *ds, void *stream) {
+// rows for Mm densities with derivative entries: Wt (M x Mm, device), umask (Mm x U, host; NULL: every unit in
+// every density), s (R x Mm), ds (R x Mm x nW)
+static int pair_multi_run(const blml_pairtab_t *tab, const blml_pairgrad_t *grad, const int32_t *ZA,
+                          const int32_t *ZB, const double *Wt, const int32_t *umask, int Mm, int64_t R, int64_t M,
+                          double *s, double *ds, hipStream_t st) {
     int rc = pair_check(tab);
     if (rc) return rc;
-    if (!grad) return fail(BLML_EVALUE, "null derivative entries");
+    if (!grad) return fail(BLML_EVALUE, "null unit / derivative entries");
     const blml_pairgrad_t &g = *grad;
-    if (g.nW < 1 || g.nW > BLML_PAIR_MAXW)
-        return fail(BLML_EVALUE, "derivative entries: 1.." + std::to_string(BLML_PAIR_MAXW) + " on this engine");
+    if (Mm < 1 || Mm > BLML_PAIR_MAXM)
+        return fail(BLML_EVALUE, "densities: 1.." + std::to_string(BLML_PAIR_MAXM) + " per pass on this engine");
+    if (g.nW < 0 || (long)Mm * g.nW > BLML_PAIR_MAXW)
+        return fail(BLML_EVALUE, "derivative entries: densities x entries at most " +
+                                     std::to_string(BLML_PAIR_MAXW) + " on this engine");
     if (g.U < 1 || g.U > BLML_PAIR_MAXD) return fail(BLML_EVALUE, "derivative rows: 1.." +
                                                      std::to_string(BLML_PAIR_MAXD) + " kernel units");
-    pair_grad_dev gd{};
-    gd.nW = g.nW;
+    pair_multi_dev gd{};
+    gd.Mm = Mm;
+    gd.nE = Mm * g.nW;
     gd.U = g.U;
+    for (int m = 0; m < Mm; ++m) {
+        unsigned um = 0;
+        for (int u = 0; u < g.U; ++u)
+            if (!umask || umask[(long)m * g.U + u]) um |= 1u << u;
+        gd.umask[m] = um;
+    }
     for (int k = 0; k < BLML_PAIR_MAXD; ++k) gd.dim_unit[k] = -1;
     for (int k = 0; k < tab->d; ++k) {
         const int kd = tab->kind[k];
@@ -1447,23 +1460,30 @@ int blml_pair_rows_grad(const blml_pairtab_t *tab, const blml_pairgrad_t *grad, 
     for (int q = 0; q < g.nW; ++q) {
         const int gq = g.wgrp[q], kc = g.wcol[q];
         if (g.wunit[q] < 0 || g.wunit[q] >= g.U) return fail(BLML_EVALUE, "derivative entry: unit out of range");
-        gd.wunit[q] = g.wunit[q];
-        gd.wgrp[q] = gq;
-        gd.wcol[q] = kc;
         if (gq >= 0) {
-            if (gq >= tab->G || (kc >= 0 && (kc >= tab->d || tab->grp[kc] != gq)))
+            if (gq >= tab->G || kc < -1 || (kc >= 0 && (kc >= tab->d || tab->grp[kc] != gq)))
                 return fail(BLML_EVALUE, "derivative entry: a group's entry outside its group");
-            if (kc < -1) return fail(BLML_EVALUE, "derivative entry: column out of range");
             grps = true;
         } else {
             if (kc < 0 || kc >= tab->d || tab->kind[kc] == 2 || tab->kind[kc] == 4)
                 return fail(BLML_EVALUE, "derivative entry: a column entry needs a column of kind 0, 1 or 3");
             if (g.dbase[q] < 0 || g.dbase[q] >= g.ndT) return fail(BLML_EVALUE, "derivative entry: dbase outside dT");
-            gd.wrank[q] = tab->kind[kc] == 3;
-            gd.wJ[q] = tab->J[kc];
-            gd.wmk[q] = tab->mk[kc];
-            gd.dbase[q] = g.dbase[q];
             cols = true;
+        }
+        for (int m = 0; m < Mm; ++m) {
+            const int e = m * g.nW + q;
+            gd.em[e] = m;
+            gd.eq[e] = q;
+            gd.eumask[e] = gd.umask[m];
+            gd.eunit[e] = g.wunit[q];
+            gd.egrp[e] = gq;
+            gd.ecol[e] = kc;
+            if (gq < 0) {
+                gd.erank[e] = tab->kind[kc] == 3;
+                gd.eJ[e] = tab->J[kc];
+                gd.emk[e] = tab->mk[kc];
+                gd.edbase[e] = g.dbase[q];
+            }
         }
     }
     if (cols && !g.dT) return fail(BLML_EVALUE, "derivative rows: dT missing");
@@ -1472,36 +1492,49 @@ int blml_pair_rows_grad(const blml_pairtab_t *tab, const blml_pairgrad_t *grad, 
     gd.dT = g.dT;
     gd.dR = g.dR;
     if (R < 0 || M < 0 || R >= (1L << 31) || M >= (1L << 31)) return fail(BLML_EVALUE, "R, M: 0 .. 2^31 - 1");
-    if ((R > 0 && (!ZA || !s || !ds)) || (M > 0 && (!ZB || !w))) return fail(BLML_EVALUE, "null argument");
-    hipStream_t st = (hipStream_t)stream;
+    if ((R > 0 && (!ZA || !s || (gd.nE > 0 && !ds))) || (M > 0 && (!ZB || !Wt))) return fail(BLML_EVALUE, "null argument");
     if (R == 0) return BLML_OK;
     if (M == 0) {
-        HIPCHECK(hipMemsetAsync(s, 0, sizeof(double) * R, st));
-        HIPCHECK(hipMemsetAsync(ds, 0, sizeof(double) * R * g.nW, st));
+        HIPCHECK(hipMemsetAsync(s, 0, sizeof(double) * R * Mm, st));
+        if (gd.nE > 0) HIPCHECK(hipMemsetAsync(ds, 0, sizeof(double) * R * gd.nE, st));
         return BLML_OK;
     }
     long rb, nsl, slice;
     pair_rows_grid(R, M, rb, nsl, slice);
     double *part = nullptr;
-    HIPCHECK(hipMallocAsync((void **)&part, sizeof(double) * nsl * (1 + g.nW) * R, st));
+    HIPCHECK(hipMallocAsync((void **)&part, sizeof(double) * nsl * (Mm + gd.nE) * R, st));
     const blml_pairtab_t t = *tab;
     const size_t es = t.f32 ? 4 : 8;
     const bool tl = t.nT * (long)es <= 32768;
     const long tbytes = tl ? ((t.nT * (long)es + 15) / 16) * 16 : 0;
-    const int lds = (int)(tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16 + BLML_PAIR_CH * 8);
+    const int lds = (int)(tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16 +
+                          BLML_PAIR_CH * BLML_PAIR_MAXM * 8);
     const dim3 grid((unsigned)rb, (unsigned)nsl);
     if (t.f32) {
-        if (tl) pair_rows_grad_kernel<float, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
-        else pair_rows_grad_kernel<float, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+        if (tl) pair_rows_multi_kernel<float, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, Wt, (int)R, (int)M, (int)slice, part);
+        else pair_rows_multi_kernel<float, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, Wt, (int)R, (int)M, (int)slice, part);
     } else {
-        if (tl) pair_rows_grad_kernel<double, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
-        else pair_rows_grad_kernel<double, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+        if (tl) pair_rows_multi_kernel<double, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, Wt, (int)R, (int)M, (int)slice, part);
+        else pair_rows_multi_kernel<double, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, Wt, (int)R, (int)M, (int)slice, part);
     }
     HIPCHECK(hipGetLastError());
-    pair_rows_grad_sum_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, g.nW, s, ds);
+    pair_rows_multi_sum_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, Mm, gd.nE, s, ds);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipFreeAsync(part, st));
     return BLML_OK;
+}
+
+int blml_pair_rows_grad(const blml_pairtab_t *tab, const blml_pairgrad_t *grad, const int32_t *ZA, const int32_t *ZB,
+                        const double *w, int64_t R, int64_t M, double *s, double *ds, void *stream) {
+    if (grad && grad->nW < 1) return fail(BLML_EVALUE, "derivative rows: no derivative entries");
+    return pair_multi_run(tab, grad, ZA, ZB, w, nullptr, 1, R, M, s, ds, (hipStream_t)stream);
+}
+
+int blml_pair_rows_multi(const blml_pairtab_t *tab, const blml_pairgrad_t *grad, const int32_t *ZA, const int32_t *ZB,
+                         const double *Wt, const int32_t *umask, int32_t Mm, int64_t R, int64_t M, double *s, double *ds,
+                         void *stream) {
+    if (!umask) return fail(BLML_EVALUE, "null unit mask");
+    return pair_multi_run(tab, grad, ZA, ZB, Wt, umask, Mm, R, M, s, ds, (hipStream_t)stream);
 }
 
 int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, int64_t R, int64_t M,

@@ -146,25 +146,32 @@ __global__ void pair_rows_sum_kernel(const double *part, int R, int nslice, doub
     s[r] = a;
 }
 
-// ------------------------------------------------------------------------------------------------ derivative rows
-// The derivative entries as the kernel reads them: blml_pairgrad_t with each column entry's table geometry resolved
-// on the host (rank table or raw offsets, its half width and rank size).
-struct pair_grad_dev {
-    int nW, U;
-    int wunit[BLML_PAIR_MAXW], wcol[BLML_PAIR_MAXW], wgrp[BLML_PAIR_MAXW], wrank[BLML_PAIR_MAXW];
+// ------------------------------------------------------------------------------------------------ fused rows
+// Several densities in one pass, with derivatives (the reference's rows_multi_kernel, blml_local_gpu.py:107-180, and
+// rows_kernel's grad branch, :94-103): density m has weights Wt[j * Mm + m] and a unit mask (bit u set: unit u's
+// factor enters its product, the others count as 1).  Accumulator entry e is density em[e]'s sum of derivative
+// entry eq[e]: its derivative value times the product of the density's other units, zero when the density's mask
+// leaves the entry's unit out.  Every per-entry value is resolved on the host, so the kernel indexes its arrays with
+// compile-time indexes only.
+
+struct pair_multi_dev {
+    int Mm, nE, U;
+    unsigned umask[BLML_PAIR_MAXM];
+    unsigned eumask[BLML_PAIR_MAXW];
+    int em[BLML_PAIR_MAXW], eq[BLML_PAIR_MAXW], eunit[BLML_PAIR_MAXW], ecol[BLML_PAIR_MAXW], egrp[BLML_PAIR_MAXW];
+    int erank[BLML_PAIR_MAXW];
     int dim_unit[BLML_PAIR_MAXD], grp_unit[BLML_PAIR_MAXG];
-    long dbase[BLML_PAIR_MAXW], wJ[BLML_PAIR_MAXW], wmk[BLML_PAIR_MAXW];
+    long edbase[BLML_PAIR_MAXW], eJ[BLML_PAIR_MAXW], emk[BLML_PAIR_MAXW];
     long ndT;
     const void *dT, *dR;
 };
 
-// part[(slice * (1 + nW) + 0) * R + r] = the slice's sum of S(ZA_r, ZB_j) w_j, part[(slice * (1 + nW) + 1 + q) * R + r]
-// = its sum of dS_q(ZA_r, ZB_j) w_j: dS_q = entry q's derivative value times the product of the other units' factors
-// (blml_local_gpu.py:21-40, 94-103).  The value is the product in unit order.  Staging as pair_rows_kernel.
+// part[(slice * (Mm + nE) + m) * R + r] = the slice's sum of density m's S(ZA_r, ZB_j) Wt[j, m];
+// part[(slice * (Mm + nE) + Mm + e) * R + r] = its sum for accumulator entry e.  Staging as pair_rows_kernel.
 template <typename R_, bool TL>
-__global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_grad_kernel(blml_pairtab_t t, pair_grad_dev gd, const int *ZA,
-                                                                      const int *ZB, const double *w, int R, int M,
-                                                                      int slice, double *part) {
+__global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_multi_kernel(blml_pairtab_t t, pair_multi_dev gd, const int *ZA,
+                                                                       const int *ZB, const double *Wt, int R, int M,
+                                                                       int slice, double *part) {
     unsigned char *lds = g_dyn_lds;
     const long tbytes = TL ? ((t.nT * (long)sizeof(R_) + 15) / 16) * 16 : 0;
     R_ *lT = reinterpret_cast<R_ *>(lds);
@@ -178,7 +185,7 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_grad_kernel(blml_pairt
     const R_ *RT = reinterpret_cast<const R_ *>(t.RT);
     const R_ *dT = reinterpret_cast<const R_ *>(gd.dT);
     const R_ *dR = reinterpret_cast<const R_ *>(gd.dR);
-    const int d = t.d;
+    const int d = t.d, Mm = gd.Mm;
     const int r = blockIdx.x * BLML_PAIR_NT + threadIdx.x;
     const bool has = r < R;
     int za[BLML_PAIR_MAXD];
@@ -186,18 +193,24 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_grad_kernel(blml_pairt
     for (int k = 0; k < BLML_PAIR_MAXD; ++k) za[k] = (has && k < d) ? ZA[(long)r * d + k] : 0;
     const long j0 = (long)blockIdx.y * slice;
     const long j1 = j0 + slice < M ? j0 + slice : M;
-    double acc = 0.0, dacc[BLML_PAIR_MAXW];
+    double acc[BLML_PAIR_MAXM], dacc[BLML_PAIR_MAXW];
 #pragma unroll
-    for (int q = 0; q < BLML_PAIR_MAXW; ++q) dacc[q] = 0.0;
+    for (int m = 0; m < BLML_PAIR_MAXM; ++m) acc[m] = 0.0;
+#pragma unroll
+    for (int e = 0; e < BLML_PAIR_MAXW; ++e) dacc[e] = 0.0;
     for (long c0 = j0; c0 < j1; c0 += BLML_PAIR_CH) {
         const int n = (int)(j1 - c0 < BLML_PAIR_CH ? j1 - c0 : BLML_PAIR_CH);
         __syncthreads();
         for (int i = threadIdx.x; i < n * d; i += BLML_PAIR_NT) lz[i] = ZB[c0 * d + i];
-        for (int i = threadIdx.x; i < n; i += BLML_PAIR_NT) lw[i] = w[c0 + i];
+        for (int i = threadIdx.x; i < n * Mm; i += BLML_PAIR_NT) lw[i] = Wt[c0 * Mm + i];
         __syncthreads();
         for (int jj = 0; jj < n; ++jj) {
-            const double wj = lw[jj];
-            if (wj == 0.0) continue;
+            const double *wj = lw + jj * Mm;
+            bool any = false;
+#pragma unroll
+            for (int m = 0; m < BLML_PAIR_MAXM; ++m)
+                if (m < Mm && wj[m] != 0.0) any = true;
+            if (!any) continue;
             const int *zb = lz + jj * d;
             R_ fac[BLML_PAIR_MAXD];
             long r2[BLML_PAIR_MAXG];
@@ -244,64 +257,76 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_grad_kernel(blml_pairt
                         if (u == ug) fac[u] = tv;
                 }
             }
-            R_ v = (R_)1;
 #pragma unroll
-            for (int u = 0; u < BLML_PAIR_MAXD; ++u)
-                if (u < gd.U) v *= fac[u];
-            acc += (double)v * wj;
-#pragma unroll
-            for (int q = 0; q < BLML_PAIR_MAXW; ++q) {
-                if (q < gd.nW) {
-                    const int uq = gd.wunit[q], g = gd.wgrp[q], kc = gd.wcol[q];
-                    long a = 0;
-#pragma unroll
-                    for (int k = 0; k < BLML_PAIR_MAXD; ++k)
-                        if (k == kc) a = za[k];
-                    const long b = kc >= 0 ? zb[kc] : 0;
-                    R_ dv;
-                    if (g >= 0) {
-                        long rg = 0;
-                        bool tg = false;
-#pragma unroll
-                        for (int gg = 0; gg < BLML_PAIR_MAXG; ++gg)
-                            if (gg == g) { rg = r2[gg]; tg = tor[gg]; }
-                        dv = pair_group_row<R_>(dR + (long)q * t.LR, t.LR, rg, tg);
-                        if (!tg && kc >= 0) dv *= (R_)(a - b);
-                    } else {
-                        long i = gd.wrank[q] ? gd.dbase[q] + a * gd.wmk[q] + b : gd.dbase[q] + a - b + gd.wJ[q];
-                        i = i < 0 ? 0 : (i >= gd.ndT ? gd.ndT - 1 : i);
-                        dv = dT[i];
-                    }
-                    R_ ex = (R_)1;
+            for (int m = 0; m < BLML_PAIR_MAXM; ++m) {
+                if (m < Mm && wj[m] != 0.0) {
+                    const unsigned um = gd.umask[m];
+                    R_ v = (R_)1;
 #pragma unroll
                     for (int u = 0; u < BLML_PAIR_MAXD; ++u)
-                        if (u < gd.U && u != uq) ex *= fac[u];
-                    dacc[q] += (double)(dv * ex) * wj;
+                        if (u < gd.U && ((um >> u) & 1u)) v *= fac[u];
+                    acc[m] += (double)v * wj[m];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < BLML_PAIR_MAXW; ++e) {
+                if (e < gd.nE) {
+                    const double we = wj[gd.em[e]];
+                    const unsigned um = gd.eumask[e];
+                    const int uq = gd.eunit[e];
+                    if (we != 0.0 && ((um >> uq) & 1u)) {
+                        const int g = gd.egrp[e], kc = gd.ecol[e];
+                        long a = 0;
+#pragma unroll
+                        for (int k = 0; k < BLML_PAIR_MAXD; ++k)
+                            if (k == kc) a = za[k];
+                        const long b = kc >= 0 ? zb[kc] : 0;
+                        R_ dv;
+                        if (g >= 0) {
+                            long rg = 0;
+                            bool tg = false;
+#pragma unroll
+                            for (int gg = 0; gg < BLML_PAIR_MAXG; ++gg)
+                                if (gg == g) { rg = r2[gg]; tg = tor[gg]; }
+                            dv = pair_group_row<R_>(dR + (long)gd.eq[e] * t.LR, t.LR, rg, tg);
+                            if (!tg && kc >= 0) dv *= (R_)(a - b);
+                        } else {
+                            long i = gd.erank[e] ? gd.edbase[e] + a * gd.emk[e] + b : gd.edbase[e] + a - b + gd.eJ[e];
+                            i = i < 0 ? 0 : (i >= gd.ndT ? gd.ndT - 1 : i);
+                            dv = dT[i];
+                        }
+                        R_ ex = (R_)1;
+#pragma unroll
+                        for (int u = 0; u < BLML_PAIR_MAXD; ++u)
+                            if (u < gd.U && u != uq && ((um >> u) & 1u)) ex *= fac[u];
+                        dacc[e] += (double)(dv * ex) * we;
+                    }
                 }
             }
         }
     }
     if (has) {
-        const long at = (long)blockIdx.y * (1 + gd.nW);
-        part[at * R + r] = acc;
+        const long at = (long)blockIdx.y * (Mm + gd.nE);
 #pragma unroll
-        for (int q = 0; q < BLML_PAIR_MAXW; ++q)
-            if (q < gd.nW) part[(at + 1 + q) * R + r] = dacc[q];
+        for (int m = 0; m < BLML_PAIR_MAXM; ++m)
+            if (m < Mm) part[(at + m) * R + r] = acc[m];
+#pragma unroll
+        for (int e = 0; e < BLML_PAIR_MAXW; ++e)
+            if (e < gd.nE) part[(at + Mm + e) * R + r] = dacc[e];
     }
 }
 
-// s[r], ds[r * nW + q]: the slices' partials added in slice order
-__global__ void pair_rows_grad_sum_kernel(const double *part, int R, int nslice, int nW, double *s, double *ds) {
+// s[r * Mm + m], ds[r * nE + e]: the slices' partials added in slice order
+__global__ void pair_rows_multi_sum_kernel(const double *part, int R, int nslice, int Mm, int nE, double *s,
+                                           double *ds) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
-    const long stride = (long)(1 + nW) * R;
-    double a = 0.0;
-    for (int y = 0; y < nslice; ++y) a += part[y * stride + r];
-    s[r] = a;
-    for (int q = 0; q < nW; ++q) {
-        a = 0.0;
-        for (int y = 0; y < nslice; ++y) a += part[y * stride + (long)(1 + q) * R + r];
-        ds[(long)r * nW + q] = a;
+    const long stride = (long)(Mm + nE) * R;
+    for (int m = 0; m < Mm + nE; ++m) {
+        double a = 0.0;
+        for (int y = 0; y < nslice; ++y) a += part[y * stride + (long)m * R + r];
+        if (m < Mm) s[(long)r * Mm + m] = a;
+        else ds[(long)r * nE + (m - Mm)] = a;
     }
 }
 

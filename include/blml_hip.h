@@ -193,6 +193,17 @@ typedef struct {
 int blml_pair_rows_grad(const blml_pairtab_t *tab, const blml_pairgrad_t *grad, const int32_t *ZA, const int32_t *ZB,
                         const double *w, int64_t R, int64_t M, double *s, double *ds, void *stream);
 
+/* Several densities in one pass (the reference's rows_multi(ZA, ZB, Wt, tab, umask, grad),
+ * blml_local_gpu.py:107-180, 432-439; the fused joint and marginals): Wt (M x Mm, row-major, device) the densities'
+ * weights, umask (Mm x grad->U int32, HOST) 1 where a unit's factor enters density m (the others count as 1).
+ * s (R x Mm) = sum_j S_m(ZA_r, ZB_j) Wt[j, m]; with derivative entries (grad->nW > 0, Mm * nW <= BLML_PAIR_MAXW)
+ * ds (R x Mm x nW) as blml_pair_rows_grad per density, 0 where the density's mask leaves the entry's unit out.
+ * grad->nW = 0: no derivatives (ds unused; grad still gives the units). */
+#define BLML_PAIR_MAXM 4      /* densities per fused pass */
+int blml_pair_rows_multi(const blml_pairtab_t *tab, const blml_pairgrad_t *grad, const int32_t *ZA, const int32_t *ZB,
+                         const double *Wt, const int32_t *umask, int32_t Mm, int64_t R, int64_t M, double *s, double *ds,
+                         void *stream);
+
 /* The explicit block out[r * M + j] = S(ZA_r, ZB_j) (REAL); with depmask != 0 also out_m = the product over the
  * columns not in depmask (the reference's joint + marginal pass). */
 int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, int64_t R, int64_t M,

@@ -10,6 +10,8 @@ Restated (numpy, float64):
   pair_cols(ZA, ZB, tab, depmask)   blml_local.py:271-282 / blml_local_gpu.py:331-361 (joint and marginal)
   pair_rows_grad(ZA, ZB, w, tab)   with grad (blml_local.py:99-116 _dval, 146-191; blml_local_gpu.py:21-40, 94-103):
                              ds[r, q] = sum_j dval_q(r, j) prod_{u != wunit[q]} fac_u(r, j) w_j
+  pair_rows_multi(ZA, ZB, Wt, tab, umask, grad)   blml_local.py:212-268 / blml_local_gpu.py:107-180: one density per
+                             column of Wt, its product over the units its mask row keeps
 Table builders for the tests (the rows ProductKernel.tables holds, blml_kernels.py:133-200, for the kinds with
 closed forms): offset rows K(o h), o = -J..J (sinc, slits, unfolded dirichlet); circular rows K(m h), m = 0..nb-1
 (folded dirichlet); rank tables K((v_a - v_b) h) over the occupied values; a periodic ball's torus table over its
@@ -19,7 +21,8 @@ Pinning (tests/test_oracle_pairs.py): pair_rows / pair_cols against the dense Gr
 (blml_kernels.py:243-251; the reference's own check, results/pball_parity.py:66-76) for tables built from the
 kernels' closed forms; the lookup of radial rows against a brute-force per-pair loop of the same rule; pair_rows_grad
 over the derivative tables of blml_amd.local.lattice_tables against central finite differences of the dense Gram
-(the kernels' closed forms at fc e^+-eps, or at coordinate offsets +-eps), which pins the tables and the lookup."""
+(the kernels' closed forms at fc e^+-eps, or at coordinate offsets +-eps), which pins the tables and the lookup;
+pair_rows_multi's marginal densities against the dense Gram of their units alone (and its finite differences)."""
 import numpy as np
 
 from oracle.blml_oracle import kernel_eval
@@ -163,6 +166,33 @@ def pair_rows_grad(ZA, ZB, w, tab, chunk=256):
             u = int(tab['wunit'][q])
             ex = np.prod(np.delete(fac, u, axis=0), axis=0)
             ds[a0:a0 + chunk, q] = (_dval(q, Ac, B, tab) * ex) @ w
+    return s, ds
+
+
+def pair_rows_multi(ZA, ZB, Wt, tab, umask, grad=False, chunk=256):
+    """(s (R, Mm), ds (R, Mm, max(nW, 1))): several densities at once (blml_local.py:212-268 _rows_numba_m,
+    blml_local_gpu.py:107-180): density m multiplies the units with umask[m, u] != 0 (the others count as 1) with
+    weights Wt[:, m]; a derivative entry whose unit the mask leaves out stays 0."""
+    A, B = rank_coords(ZA, tab), rank_coords(ZB, tab)
+    Wt = np.asarray(Wt, float)
+    umask = np.asarray(umask) != 0
+    keep = (Wt != 0).any(1)
+    B, Wt = B[keep], Wt[keep]
+    Mm, nW = Wt.shape[1], int(tab['nW'])
+    s = np.zeros((len(A), Mm))
+    ds = np.zeros((len(A), Mm, max(nW, 1)))
+    for a0 in range(0, len(A), chunk):
+        Ac = A[a0:a0 + chunk]
+        fac = _factors(Ac, B, tab)
+        units = np.arange(fac.shape[0])
+        dv = [_dval(q, Ac, B, tab) for q in range(nW)] if grad else []
+        for m in range(Mm):
+            s[a0:a0 + chunk, m] = np.prod(fac[umask[m]], axis=0) @ Wt[:, m]
+            for q in range(len(dv)):
+                u = int(tab['wunit'][q])
+                if umask[m, u]:
+                    ex = np.prod(fac[umask[m] & (units != u)], axis=0)
+                    ds[a0:a0 + chunk, m, q] = (dv[q] * ex) @ Wt[:, m]
     return s, ds
 
 

@@ -128,3 +128,44 @@ def test_derivative_rows_match_oracle(f32):
         local.rows(Z, Z, np.ones(len(Z)), big, grad=True)
     s, ds = local.rows(Z, Z, np.ones(len(Z)), local.lattice_tables(kinds, fc, periods, np.ones(4), Z), grad=True)
     assert ds.shape == (len(Z), 1) and not ds.any()
+
+
+@pytest.mark.parametrize('f32', [False, True], ids=['f64', 'f32'])
+def test_fused_densities_match_oracle(f32):
+    """rows_multi (blml_pair_rows_multi) against the oracle's pair_rows_multi: the joint and two marginals with five
+    derivative entries (15 accumulators), four densities without derivatives over a ball group and a sinc column,
+    and the many-slice case; bars as the derivative rows."""
+    from blml_amd import local
+    from test_oracle_pairs import _column_case
+    rng = np.random.default_rng(43)
+    Z, kinds, fc, periods, want, _, _ = _column_case(rng)
+    Zb = _bins(rng, (20, 18, 9), 300)
+    ZL = np.concatenate([_bins(rng, (1500, 1500, 1500), 9000), np.array([[0, 0, 0], [1499, 1499, 1499]])])
+    cases = [('columns, 3 densities x 5 entries', local.lattice_tables(kinds, fc, periods, np.ones(4), Z, want=want[:5]),
+              Z, Z, np.array([[1, 1, 1, 1], [1, 0, 1, 0], [0, 1, 0, 1]]), True),
+             ('ball x sinc, 4 densities', local.lattice_tables(['ball', 'ball', 'sinc'], [0.3, 0.3, 0.4], None,
+                                                              np.ones(3), Zb), Zb[:150], Zb,
+              np.array([[1, 1], [1, 0], [0, 1], [0, 0]]), False),
+             ('3d slits large, 2 densities x 6 entries', local.lattice_tables(
+                 ['slits:0.5'] * 3, [0.2] * 3, None, np.ones(3), ZL,
+                 want=[(k, p) for k in range(3) for p in ('fc', 'shift')]), ZL[:800], ZL,
+              np.array([[1, 1, 1], [1, 0, 1]]), True)]
+    for name, tab, ZA, ZB, umask, grad in cases:
+        _note(f'fused rows {name} f32={f32} nA={len(ZA)} nB={len(ZB)}')
+        Wt = rng.standard_normal((len(ZB), umask.shape[0]))
+        Wt[::5] = 0.0
+        Wt[1::4, 0] = 0.0
+        s, ds = local.rows_multi(ZA, ZB, Wt, tab, umask, grad=grad, f32=f32)
+        s0, ds0 = OP.pair_rows_multi(ZA, ZB, Wt, tab, umask, grad=grad)
+        assert s.shape == s0.shape and ds.shape == ds0.shape, name
+        for m in range(umask.shape[0]):
+            ref = np.abs(s0[:, m]).max()
+            assert np.abs(s[:, m] - s0[:, m]).max() <= (1e-4 if f32 else 1e-11) * ref, (name, m)
+            for q in range(ds0.shape[2]):
+                ref = np.abs(ds0[:, m, q]).max()
+                assert np.abs(ds[:, m, q] - ds0[:, m, q]).max() <= (1e-4 if f32 else 1e-11) * ref, (name, m, q)
+    tab = cases[0][1]
+    with pytest.raises(ValueError, match='max 16'):
+        local.rows_multi(Z, Z, np.ones((len(Z), 4)), tab, np.ones((4, 4)), grad=True)
+    with pytest.raises(ValueError, match='umask'):
+        local.rows_multi(Z, Z, np.ones((len(Z), 2)), tab, np.ones((2, 3)))

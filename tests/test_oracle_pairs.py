@@ -250,3 +250,34 @@ def test_derivative_rows_ball_and_pball():
         local.lattice_tables(['dirichlet'], [0.4], [16.0], [1.0], Zp[:, :1], want=[(0, 'fc')])
     with pytest.raises(ValueError, match='fixed shape'):
         local.lattice_tables(['slits:0.5'], [0.4], None, [1.0], Zp[:, :1], want=[(0, 'fill')])
+
+
+def test_fused_densities_are_the_marginal_grams():
+    """pair_rows_multi: the joint mask gives pair_rows_grad; a marginal mask gives the dense Gram of its units alone
+    (closed forms, 1e-12) and that Gram's finite differences for the entries on its units (1e-7), zero for the
+    others; a source whose weights are all zero is skipped."""
+    from blml_amd import local
+    rng = np.random.default_rng(33)
+    Z, kinds, fc, periods, want, units, fd = _column_case(rng)
+    h = np.ones(4)
+    tab = local.lattice_tables(kinds, fc, periods, h, Z, want=want)
+    umask = np.array([[1, 1, 1, 1], [1, 0, 1, 0], [0, 1, 0, 0]])
+    Wt = rng.standard_normal((len(Z), 3))
+    Wt[::6] = 0.0
+    Wt[1::7, 1] = 0.0
+    s, ds = OP.pair_rows_multi(Z, Z, Wt, tab, umask, grad=True)
+    s0, ds0 = OP.pair_rows_grad(Z, Z, Wt[:, 0], tab)
+    assert np.abs(s[:, 0] - s0).max() < 1e-14 * np.abs(s0).max() and np.abs(ds[:, 0] - ds0).max() < 1e-14 * np.abs(ds0).max()
+    for m in (1, 2):
+        keep = [u for u in range(4) if umask[m, u]]
+        sub = [units[u] for u in keep]
+        G = np.ones((len(Z), len(Z)))
+        for kind, fcs, pers, dims in sub:
+            G = G * O.kernel_eval(kind, np.asarray(fcs), np.asarray(pers), [(Z[:, k, None] - Z[None, :, k]) * h[k] for k in dims])
+        assert np.abs(s[:, m] - G @ Wt[:, m]).max() < 1e-12 * np.abs(G).sum(1).max()
+        for q, (ui, how) in enumerate(fd):
+            if not umask[m, ui]:
+                assert not ds[:, m, q].any()
+                continue
+            ref = _fd_rows(sub, h, Z, Z, Wt[:, m], keep.index(ui), how)
+            assert np.abs(ds[:, m, q] - ref).max() < 1e-7 * np.abs(ref).max(), (m, q)
