@@ -1384,13 +1384,17 @@ static int pair_rows_launch(const blml_pairtab_t &t, const int32_t *ZA, const in
     const long tbytes = tl ? ((t.nT * (long)es + 15) / 16) * 16 : 0;
     const int lds = (int)(tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16 + BLML_PAIR_CH * 8);
     const dim3 grid((unsigned)rb, (unsigned)nsl);
+#define BLML_ROWS_LAUNCH(R_, TL, HASG) \
+    pair_rows_kernel<R_, TL, HASG><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part)
+    const bool hasg = t.G > 0;          // pair_check: a column of kind 2 or 4 needs a group
     if (t.f32) {
-        if (tl) pair_rows_kernel<float, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
-        else pair_rows_kernel<float, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+        if (tl) { if (hasg) BLML_ROWS_LAUNCH(float, true, true); else BLML_ROWS_LAUNCH(float, true, false); }
+        else { if (hasg) BLML_ROWS_LAUNCH(float, false, true); else BLML_ROWS_LAUNCH(float, false, false); }
     } else {
-        if (tl) pair_rows_kernel<double, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
-        else pair_rows_kernel<double, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part);
+        if (tl) { if (hasg) BLML_ROWS_LAUNCH(double, true, true); else BLML_ROWS_LAUNCH(double, true, false); }
+        else { if (hasg) BLML_ROWS_LAUNCH(double, false, true); else BLML_ROWS_LAUNCH(double, false, false); }
     }
+#undef BLML_ROWS_LAUNCH
     HIPCHECK(hipGetLastError());
     pair_rows_sum_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, s);
     HIPCHECK(hipGetLastError());
@@ -1510,13 +1514,17 @@ static int pair_multi_run(const blml_pairtab_t *tab, const blml_pairgrad_t *grad
     const int lds = (int)(tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16 +
                           BLML_PAIR_CH * BLML_PAIR_MAXM * 8);
     const dim3 grid((unsigned)rb, (unsigned)nsl);
+#define BLML_MULTI_LAUNCH(R_, TL, NE) \
+    pair_rows_multi_kernel<R_, TL, NE><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, Wt, (int)R, (int)M, (int)slice, part)
+    const bool small = gd.nE <= 4;
     if (t.f32) {
-        if (tl) pair_rows_multi_kernel<float, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, Wt, (int)R, (int)M, (int)slice, part);
-        else pair_rows_multi_kernel<float, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, Wt, (int)R, (int)M, (int)slice, part);
+        if (tl) { if (small) BLML_MULTI_LAUNCH(float, true, 4); else BLML_MULTI_LAUNCH(float, true, BLML_PAIR_MAXW); }
+        else { if (small) BLML_MULTI_LAUNCH(float, false, 4); else BLML_MULTI_LAUNCH(float, false, BLML_PAIR_MAXW); }
     } else {
-        if (tl) pair_rows_multi_kernel<double, true><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, Wt, (int)R, (int)M, (int)slice, part);
-        else pair_rows_multi_kernel<double, false><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, Wt, (int)R, (int)M, (int)slice, part);
+        if (tl) { if (small) BLML_MULTI_LAUNCH(double, true, 4); else BLML_MULTI_LAUNCH(double, true, BLML_PAIR_MAXW); }
+        else { if (small) BLML_MULTI_LAUNCH(double, false, 4); else BLML_MULTI_LAUNCH(double, false, BLML_PAIR_MAXW); }
     }
+#undef BLML_MULTI_LAUNCH
     HIPCHECK(hipGetLastError());
     pair_rows_multi_sum_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, Mm, gd.nE, s, ds);
     HIPCHECK(hipGetLastError());

@@ -46,8 +46,9 @@ __device__ __forceinline__ R_ pair_group_row(const R_ *row, long LR, long r2, bo
 }
 
 // S(za, zb) = vn * vd: vd the product over the columns in depmask (groups with any column there), vn over the
-// others; zb: a pointer to d coordinates.
-template <typename R_>
+// others; zb: a pointer to d coordinates.  HASG = false: tables without groups (every column of kind 0, 1 or 3, as
+// the binding checks), the group code compiled out.
+template <typename R_, bool HASG = true>
 __device__ __forceinline__ void pair_value(const blml_pairtab_t &t, const R_ *T, const R_ *RT, const int *za,
                                            const int *zb, long depmask, R_ &vn, R_ &vdep) {
     R_ v = (R_)1, vd = (R_)1;
@@ -61,7 +62,7 @@ __device__ __forceinline__ void pair_value(const blml_pairtab_t &t, const R_ *T,
             const int kd = t.kind[k];
             const long a = za[k], b = zb[k], off = a - b;
             const bool dep = (depmask >> k) & 1;
-            if (kd == 2 || kd == 4) {
+            if (HASG && (kd == 2 || kd == 4)) {
                 const int gk = t.grp[k];
 #pragma unroll
                 for (int g = 0; g < BLML_PAIR_MAXG; ++g) {
@@ -85,7 +86,7 @@ __device__ __forceinline__ void pair_value(const blml_pairtab_t &t, const R_ *T,
     }
 #pragma unroll
     for (int g = 0; g < BLML_PAIR_MAXG; ++g) {
-        if (g < t.G) {
+        if (HASG && g < t.G) {
             const R_ tv = pair_group_row<R_>(RT + (long)g * t.LR, t.LR, r2[g], tor[g]);
             if (gdep[g]) vd *= tv;
             else v *= tv;
@@ -96,7 +97,7 @@ __device__ __forceinline__ void pair_value(const blml_pairtab_t &t, const R_ *T,
 }
 
 // part[slice][r] = sum over the slice's sources j (w_j != 0) of S(ZA_r, ZB_j) w_j.  LDS: [T if TL][ZB chunk][w chunk].
-template <typename R_, bool TL>
+template <typename R_, bool TL, bool HASG>
 __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_kernel(blml_pairtab_t t, const int *ZA, const int *ZB,
                                                                  const double *w, int R, int M, int slice,
                                                                  double *part) {
@@ -130,7 +131,7 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_kernel(blml_pairtab_t 
             const double wj = lw[jj];
             if (wj == 0.0) continue;
             R_ vn, vd;
-            pair_value<R_>(t, T, RT, za, lz + jj * d, 0, vn, vd);
+            pair_value<R_, HASG>(t, T, RT, za, lz + jj * d, 0, vn, vd);
             acc += (double)vn * wj;
         }
     }
@@ -168,7 +169,8 @@ struct pair_multi_dev {
 
 // part[(slice * (Mm + nE) + m) * R + r] = the slice's sum of density m's S(ZA_r, ZB_j) Wt[j, m];
 // part[(slice * (Mm + nE) + Mm + e) * R + r] = its sum for accumulator entry e.  Staging as pair_rows_kernel.
-template <typename R_, bool TL>
+// NE: the accumulator entries the instantiation holds (nE <= NE; 4 keeps the common small passes light on registers).
+template <typename R_, bool TL, int NE>
 __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_multi_kernel(blml_pairtab_t t, pair_multi_dev gd, const int *ZA,
                                                                        const int *ZB, const double *Wt, int R, int M,
                                                                        int slice, double *part) {
@@ -193,11 +195,11 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_multi_kernel(blml_pair
     for (int k = 0; k < BLML_PAIR_MAXD; ++k) za[k] = (has && k < d) ? ZA[(long)r * d + k] : 0;
     const long j0 = (long)blockIdx.y * slice;
     const long j1 = j0 + slice < M ? j0 + slice : M;
-    double acc[BLML_PAIR_MAXM], dacc[BLML_PAIR_MAXW];
+    double acc[BLML_PAIR_MAXM], dacc[NE];
 #pragma unroll
     for (int m = 0; m < BLML_PAIR_MAXM; ++m) acc[m] = 0.0;
 #pragma unroll
-    for (int e = 0; e < BLML_PAIR_MAXW; ++e) dacc[e] = 0.0;
+    for (int e = 0; e < NE; ++e) dacc[e] = 0.0;
     for (long c0 = j0; c0 < j1; c0 += BLML_PAIR_CH) {
         const int n = (int)(j1 - c0 < BLML_PAIR_CH ? j1 - c0 : BLML_PAIR_CH);
         __syncthreads();
@@ -269,7 +271,7 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_multi_kernel(blml_pair
                 }
             }
 #pragma unroll
-            for (int e = 0; e < BLML_PAIR_MAXW; ++e) {
+            for (int e = 0; e < NE; ++e) {
                 if (e < gd.nE) {
                     const double we = wj[gd.em[e]];
                     const unsigned um = gd.eumask[e];
@@ -311,7 +313,7 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_multi_kernel(blml_pair
         for (int m = 0; m < BLML_PAIR_MAXM; ++m)
             if (m < Mm) part[(at + m) * R + r] = acc[m];
 #pragma unroll
-        for (int e = 0; e < BLML_PAIR_MAXW; ++e)
+        for (int e = 0; e < NE; ++e)
             if (e < gd.nE) part[(at + Mm + e) * R + r] = dacc[e];
     }
 }

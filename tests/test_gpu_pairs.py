@@ -111,6 +111,8 @@ def test_derivative_rows_match_oracle(f32):
     cases.append(('3d slits large', local.lattice_tables(
         ['slits:0.5'] * 3, [0.2] * 3, None, np.ones(3), ZL,
         want=[(k, p) for k in range(3) for p in ('fc', 'shift')]), ZL[:1000], ZL))
+    cases.append(('3d slits large, 2 entries', local.lattice_tables(
+        ['slits:0.5'] * 3, [0.2] * 3, None, np.ones(3), ZL, want=[(0, 'fc'), (2, 'shift')]), ZL[:500], ZL))
     for name, tab, ZA, ZB in cases:
         _note(f'derivative rows {name} f32={f32} nA={len(ZA)} nB={len(ZB)} nW={tab["nW"]}')
         w = rng.standard_normal(len(ZB))
