@@ -238,6 +238,49 @@ def copy_known_size(torch):
     del cp_src, cp_dst
 
 
+def pair_leg(torch, st, event_time):
+    """The pair path on the 128^3 companion: the rows of 65,536 occupied bins against every one (pairs/s) and the
+    companion BLMLQuick fit through the same kernels."""
+    # the pair path: rows s_r = sum_j S(z_r, z_j) w_j over the companion's occupied bins (slits per column),
+    # FP32 tables and products, FP64 sums; a row subset against every source
+    import ctypes
+    from blml_amd import _lib, local
+    Zp, wp, hp = pair_companion()
+    tabp = local.lattice_tables(['slits:0.5'] * 3, [0.2 / hp] * 3, None, [hp] * 3, Zp)
+    n_rows = min(len(Zp), 1 << 16)
+    pt, dA, dB, keep = local._prepare(Zp[:n_rows], Zp, tabp, True, False)
+    dw = torch.from_numpy(np.ascontiguousarray(wp)).to('cuda')
+    ds = torch.zeros(n_rows, dtype=torch.float64, device='cuda')
+    lib = _lib.load()
+    call = lambda: _lib.check(lib.blml_pair_rows(ctypes.byref(pt), ctypes.c_void_p(dA.data_ptr()),
+                                                 ctypes.c_void_p(dB.data_ptr()), ctypes.c_void_p(dw.data_ptr()),
+                                                 n_rows, len(Zp), ctypes.c_void_p(ds.data_ptr()),
+                                                 ctypes.c_void_p(st.cuda_stream)))
+    call()
+    t_p = event_time(call, 3)
+    pairs = dict(bins=int(len(Zp)), rows=int(n_rows), sources=int(len(Zp)), draws=2_000_000, lattice=[PAIR_BINS] * 3,
+                 kernel='slits fill 0.5, fc 0.2 cycles/bin per column (offset rows)', precision='f32 tables, f64 sums',
+                 seconds=t_p, pairs_per_s=n_rows * len(Zp) / t_p)
+    del keep
+    print(f'bench: pair rows {n_rows} x {len(Zp)} in {t_p * 1e3:.1f} ms', file=sys.stderr, flush=True)
+    # the companion fit through the pair path: BLMLQuick on the lattice's counts (real rank 1, FP32 tables, block
+    # 4096), a capped number of sweeps -- a band-pass slits kernel (fill 0.5) leaves the real rank-1 objective
+    # without an interior optimum, so the fit runs to the cap and is scaled to e'Se = n^2
+    from blml_amd.quick import BLMLQuick
+    modes_p = [{j: dict(kind='slits', fc=0.2 / hp, fill=0.5, h=hp) for j in range(3)}]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    qf = BLMLQuick(tol=1e-4, max_sweeps=QUICK_SWEEPS, seed=0).fit(Zp * hp, modes_p, sample_weight=wp)
+    t_q = time.perf_counter() - t0
+    passes = qf.n_sweeps_ + 2                     # the sweeps, the first exact S e and the capped one
+    pairs['quick_fit'] = dict(estimator='BLMLQuick(tol=1e-4, block_size=4096, dtype=float32)', bins=int(qf.n_bins_),
+                              sweeps=int(qf.n_sweeps_), seconds=t_q, full_passes=passes,
+                              pairs_per_s=passes * float(qf.n_bins_) ** 2 / t_q, converged=bool(qf.converged_),
+                              residual_history=[float(r) for r in qf.rho_history_])
+    print(f'bench: companion fit {qf.n_sweeps_} sweeps in {t_q:.2f} s', file=sys.stderr, flush=True)
+    return pairs, Zp, wp, tabp
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -246,9 +289,10 @@ def main():
     ap.add_argument('--fits', type=int, default=N_FITS)
     ap.add_argument('--no-fp64', action='store_true', help='skip the plain FP64 reference run')
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
-    ap.add_argument('--only', choices=['all', 'matvec', 'batched'], default='all',
+    ap.add_argument('--only', choices=['all', 'matvec', 'batched', 'pairs'], default='all',
                     help='matvec: only the 32^5 mat-vec (timed) and the 256 MiB copy, for counter runs; batched: '
-                         'only the batched mat-vec of the 2048 fits (timed) and the copy')
+                         'only the batched mat-vec of the 2048 fits (timed) and the copy; pairs: only the pair '
+                         'rows and the companion fit')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (independent fits '
@@ -269,6 +313,10 @@ def main():
             torch.cuda.synchronize()
             return ev0.elapsed_time(ev1) / 1e3 / reps
 
+        if args.only == 'pairs':
+            pairs, *_ = pair_leg(torch, st0, ev_time)
+            print(json.dumps(dict(pairs=pairs)), flush=True)
+            return
         if args.only == 'matvec':
             *_, bl, roof, _ = large_matvec(torch, blml_amd, _FitSpec, FitBatch, fit_params(), ev_time, args.steps)
             bl.close()
@@ -420,43 +468,7 @@ def main():
     copy_known_size(torch)
     print(f'bench: large fit fp32 {large["fp32_seconds"]:.2f} s; mat-vec {t_l * 1e6:.1f} us', file=sys.stderr,
           flush=True)
-    # the pair path: rows s_r = sum_j S(z_r, z_j) w_j over the companion's occupied bins (slits per column),
-    # FP32 tables and products, FP64 sums; a row subset against every source
-    import ctypes
-    from blml_amd import _lib, local
-    Zp, wp, hp = pair_companion()
-    tabp = local.lattice_tables(['slits:0.5'] * 3, [0.2 / hp] * 3, None, [hp] * 3, Zp)
-    n_rows = min(len(Zp), 1 << 16)
-    pt, dA, dB, keep = local._prepare(Zp[:n_rows], Zp, tabp, True, False)
-    dw = torch.from_numpy(np.ascontiguousarray(wp)).to('cuda')
-    ds = torch.zeros(n_rows, dtype=torch.float64, device='cuda')
-    lib = _lib.load()
-    call = lambda: _lib.check(lib.blml_pair_rows(ctypes.byref(pt), ctypes.c_void_p(dA.data_ptr()),
-                                                 ctypes.c_void_p(dB.data_ptr()), ctypes.c_void_p(dw.data_ptr()),
-                                                 n_rows, len(Zp), ctypes.c_void_p(ds.data_ptr()),
-                                                 ctypes.c_void_p(st.cuda_stream)))
-    call()
-    t_p = event_time(call, 3)
-    pairs = dict(bins=int(len(Zp)), rows=int(n_rows), sources=int(len(Zp)), draws=2_000_000, lattice=[PAIR_BINS] * 3,
-                 kernel='slits fill 0.5, fc 0.2 cycles/bin per column (offset rows)', precision='f32 tables, f64 sums',
-                 seconds=t_p, pairs_per_s=n_rows * len(Zp) / t_p)
-    del keep
-    print(f'bench: pair rows {n_rows} x {len(Zp)} in {t_p * 1e3:.1f} ms', file=sys.stderr, flush=True)
-    # the companion fit through the pair path: BLMLQuick on the lattice's counts (real rank 1, FP32 tables, block
-    # 4096), a capped number of sweeps -- a band-pass slits kernel (fill 0.5) leaves the real rank-1 objective
-    # without an interior optimum, so the fit runs to the cap and is scaled to e'Se = n^2
-    from blml_amd.quick import BLMLQuick
-    modes_p = [{j: dict(kind='slits', fc=0.2 / hp, fill=0.5, h=hp) for j in range(3)}]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    qf = BLMLQuick(tol=1e-4, max_sweeps=QUICK_SWEEPS, seed=0).fit(Zp * hp, modes_p, sample_weight=wp)
-    t_q = time.perf_counter() - t0
-    passes = qf.n_sweeps_ + 2                     # the sweeps, the first exact S e and the capped one
-    pairs['quick_fit'] = dict(estimator='BLMLQuick(tol=1e-4, block_size=4096, dtype=float32)', bins=int(qf.n_bins_),
-                              sweeps=int(qf.n_sweeps_), seconds=t_q, full_passes=passes,
-                              pairs_per_s=passes * float(qf.n_bins_) ** 2 / t_q, converged=bool(qf.converged_),
-                              residual_history=[float(r) for r in qf.rho_history_])
-    print(f'bench: companion fit {qf.n_sweeps_} sweeps in {t_q:.2f} s', file=sys.stderr, flush=True)
+    pairs, Zp, wp, tabp = pair_leg(torch, st, event_time)
     cpu = cpu_baseline(fits, args.cpu_seconds)
     from oracle import blml_pairs as OP
     t0 = time.perf_counter()
