@@ -1384,16 +1384,23 @@ static int pair_rows_launch(const blml_pairtab_t &t, const int32_t *ZA, const in
     const long tbytes = tl ? ((t.nT * (long)es + 15) / 16) * 16 : 0;
     const int lds = (int)(tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16 + BLML_PAIR_CH * 8);
     const dim3 grid((unsigned)rb, (unsigned)nsl);
-#define BLML_ROWS_LAUNCH(R_, TL, HASG) \
-    pair_rows_kernel<R_, TL, HASG><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part)
-    const bool hasg = t.G > 0;          // pair_check: a column of kind 2 or 4 needs a group
+#define BLML_ROWS_LAUNCH(R_, TL, NG) \
+    pair_rows_kernel<R_, TL, NG><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part)
+#define BLML_ROWS_BY_G(R_, TL) \
+    do { \
+        if (t.G == 0) BLML_ROWS_LAUNCH(R_, TL, 0); \
+        else if (t.G == 1) BLML_ROWS_LAUNCH(R_, TL, 1); \
+        else BLML_ROWS_LAUNCH(R_, TL, BLML_PAIR_MAXG); \
+    } while (0)
+    // pair_check: a column of kind 2 or 4 names a group < G, so G = 0 means no such column
     if (t.f32) {
-        if (tl) { if (hasg) BLML_ROWS_LAUNCH(float, true, true); else BLML_ROWS_LAUNCH(float, true, false); }
-        else { if (hasg) BLML_ROWS_LAUNCH(float, false, true); else BLML_ROWS_LAUNCH(float, false, false); }
+        if (tl) BLML_ROWS_BY_G(float, true);
+        else BLML_ROWS_BY_G(float, false);
     } else {
-        if (tl) { if (hasg) BLML_ROWS_LAUNCH(double, true, true); else BLML_ROWS_LAUNCH(double, true, false); }
-        else { if (hasg) BLML_ROWS_LAUNCH(double, false, true); else BLML_ROWS_LAUNCH(double, false, false); }
+        if (tl) BLML_ROWS_BY_G(double, true);
+        else BLML_ROWS_BY_G(double, false);
     }
+#undef BLML_ROWS_BY_G
 #undef BLML_ROWS_LAUNCH
     HIPCHECK(hipGetLastError());
     pair_rows_sum_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, s);

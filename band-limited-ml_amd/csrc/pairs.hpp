@@ -46,26 +46,27 @@ __device__ __forceinline__ R_ pair_group_row(const R_ *row, long LR, long r2, bo
 }
 
 // S(za, zb) = vn * vd: vd the product over the columns in depmask (groups with any column there), vn over the
-// others; zb: a pointer to d coordinates.  HASG = false: tables without groups (every column of kind 0, 1 or 3, as
-// the binding checks), the group code compiled out.
-template <typename R_, bool HASG = true>
+// others; zb: a pointer to d coordinates.  NG: the groups the instantiation holds (t.G <= NG); NG = 0 for tables
+// without groups (every column of kind 0, 1 or 3, as the binding checks) compiles the group code out.
+template <typename R_, int NG = BLML_PAIR_MAXG>
 __device__ __forceinline__ void pair_value(const blml_pairtab_t &t, const R_ *T, const R_ *RT, const int *za,
                                            const int *zb, long depmask, R_ &vn, R_ &vdep) {
+    constexpr int NGA = NG > 0 ? NG : 1;
     R_ v = (R_)1, vd = (R_)1;
-    long r2[BLML_PAIR_MAXG];
-    bool tor[BLML_PAIR_MAXG], gdep[BLML_PAIR_MAXG];
+    long r2[NGA];
+    bool tor[NGA], gdep[NGA];
 #pragma unroll
-    for (int g = 0; g < BLML_PAIR_MAXG; ++g) { r2[g] = 0; tor[g] = false; gdep[g] = false; }
+    for (int g = 0; g < NGA; ++g) { r2[g] = 0; tor[g] = false; gdep[g] = false; }
 #pragma unroll
     for (int k = 0; k < BLML_PAIR_MAXD; ++k) {
         if (k < t.d) {
             const int kd = t.kind[k];
             const long a = za[k], b = zb[k], off = a - b;
             const bool dep = (depmask >> k) & 1;
-            if (HASG && (kd == 2 || kd == 4)) {
+            if (NG > 0 && (kd == 2 || kd == 4)) {
                 const int gk = t.grp[k];
 #pragma unroll
-                for (int g = 0; g < BLML_PAIR_MAXG; ++g) {
+                for (int g = 0; g < NGA; ++g) {
                     if (g == gk) {
                         if (kd == 2) {
                             r2[g] += off * off;
@@ -85,8 +86,8 @@ __device__ __forceinline__ void pair_value(const blml_pairtab_t &t, const R_ *T,
         }
     }
 #pragma unroll
-    for (int g = 0; g < BLML_PAIR_MAXG; ++g) {
-        if (HASG && g < t.G) {
+    for (int g = 0; g < NGA; ++g) {
+        if (NG > 0 && g < t.G) {
             const R_ tv = pair_group_row<R_>(RT + (long)g * t.LR, t.LR, r2[g], tor[g]);
             if (gdep[g]) vd *= tv;
             else v *= tv;
@@ -97,7 +98,7 @@ __device__ __forceinline__ void pair_value(const blml_pairtab_t &t, const R_ *T,
 }
 
 // part[slice][r] = sum over the slice's sources j (w_j != 0) of S(ZA_r, ZB_j) w_j.  LDS: [T if TL][ZB chunk][w chunk].
-template <typename R_, bool TL, bool HASG>
+template <typename R_, bool TL, int NG>
 __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_kernel(blml_pairtab_t t, const int *ZA, const int *ZB,
                                                                  const double *w, int R, int M, int slice,
                                                                  double *part) {
@@ -131,7 +132,7 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_kernel(blml_pairtab_t 
             const double wj = lw[jj];
             if (wj == 0.0) continue;
             R_ vn, vd;
-            pair_value<R_, HASG>(t, T, RT, za, lz + jj * d, 0, vn, vd);
+            pair_value<R_, NG>(t, T, RT, za, lz + jj * d, 0, vn, vd);
             acc += (double)vn * wj;
         }
     }

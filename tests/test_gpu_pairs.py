@@ -38,6 +38,10 @@ def _all_cases(rng):
     Zb = _bins(rng, (20, 18, 9), 400)
     out.append(('ball x sinc (radial group from lattice_tables)', (20, 18, 9),
                 local.lattice_tables(['ball', 'ball', 'sinc'], [0.3, 0.3, 0.4], None, [1.0, 1.0, 1.0], Zb), 70, 120))
+    Z4 = _bins(rng, (9, 8, 10, 7), 400)
+    out.append(('two ball groups (lattice_tables)', (9, 8, 10, 7),
+                local.lattice_tables(['ball:a', 'ball:a', 'ball:b', 'ball:b'], [0.3, 0.3, 0.35, 0.35], None,
+                                     [1.0] * 4, Z4), 90, 150))
     # enough rows and sources for several slices, chunk tails and a table too big for LDS (float64, > 32 KB)
     out.append(('3d slits large', (64, 64, 64), OP.build_tables(
         [('offset', 'slits:0.5', [0.2], [0.0], [k]) for k in range(3)], np.ones(3), (1500, 1500, 1500)), 3000,
@@ -107,6 +111,10 @@ def test_derivative_rows_match_oracle(f32):
     Zp = _bins(rng, (10, 8), 60)
     cases.append(('pball', local.lattice_tables(['pball', 'pball'], [0.45, 0.45], [10.0, 8.0], [1.0, 1.0], Zp,
                                                 want=[(0, 'shift'), (0, ('shift', 1))]), Zp, Zp))
+    Z4 = _bins(rng, (9, 8, 10, 7), 300)
+    cases.append(('two ball groups', local.lattice_tables(
+        ['ball:a', 'ball:a', 'ball:b', 'ball:b'], [0.3, 0.3, 0.35, 0.35], None, [1.0] * 4, Z4,
+        want=[(0, 'fc'), (1, ('shift', 1)), (1, 'fc')]), Z4[:100], Z4))
     ZL = np.concatenate([_bins(rng, (1500, 1500, 1500), 12000), np.array([[0, 0, 0], [1499, 1499, 1499]])])
     cases.append(('3d slits large', local.lattice_tables(
         ['slits:0.5'] * 3, [0.2] * 3, None, np.ones(3), ZL,
