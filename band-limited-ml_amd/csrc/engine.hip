@@ -1521,9 +1521,15 @@ static int pair_multi_run(const blml_pairtab_t *tab, const blml_pairgrad_t *grad
     const int lds = (int)(tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16 +
                           BLML_PAIR_CH * BLML_PAIR_MAXM * 8);
     const dim3 grid((unsigned)rb, (unsigned)nsl);
+#define BLML_MULTI_LAUNCH_G(R_, TL, NE, NG) \
+    pair_rows_multi_kernel<R_, TL, NE, NG><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, Wt, (int)R, (int)M, (int)slice, part)
 #define BLML_MULTI_LAUNCH(R_, TL, NE) \
-    pair_rows_multi_kernel<R_, TL, NE><<<grid, BLML_PAIR_NT, lds, st>>>(t, gd, ZA, ZB, Wt, (int)R, (int)M, (int)slice, part)
-    const bool small = gd.nE <= 4;
+    do { \
+        if (t.G == 0) BLML_MULTI_LAUNCH_G(R_, TL, NE, 0); \
+        else if (t.G == 1) BLML_MULTI_LAUNCH_G(R_, TL, NE, 1); \
+        else BLML_MULTI_LAUNCH_G(R_, TL, NE, BLML_PAIR_MAXG); \
+    } while (0)
+    const bool small = gd.nE <= 4;     // a group entry (egrp >= 0) needs t.G >= 1, checked above
     if (t.f32) {
         if (tl) { if (small) BLML_MULTI_LAUNCH(float, true, 4); else BLML_MULTI_LAUNCH(float, true, BLML_PAIR_MAXW); }
         else { if (small) BLML_MULTI_LAUNCH(float, false, 4); else BLML_MULTI_LAUNCH(float, false, BLML_PAIR_MAXW); }
@@ -1532,6 +1538,7 @@ static int pair_multi_run(const blml_pairtab_t *tab, const blml_pairgrad_t *grad
         else { if (small) BLML_MULTI_LAUNCH(double, false, 4); else BLML_MULTI_LAUNCH(double, false, BLML_PAIR_MAXW); }
     }
 #undef BLML_MULTI_LAUNCH
+#undef BLML_MULTI_LAUNCH_G
     HIPCHECK(hipGetLastError());
     pair_rows_multi_sum_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, Mm, gd.nE, s, ds);
     HIPCHECK(hipGetLastError());

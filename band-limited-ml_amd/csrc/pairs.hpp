@@ -170,8 +170,9 @@ struct pair_multi_dev {
 
 // part[(slice * (Mm + nE) + m) * R + r] = the slice's sum of density m's S(ZA_r, ZB_j) Wt[j, m];
 // part[(slice * (Mm + nE) + Mm + e) * R + r] = its sum for accumulator entry e.  Staging as pair_rows_kernel.
-// NE: the accumulator entries the instantiation holds (nE <= NE; 4 keeps the common small passes light on registers).
-template <typename R_, bool TL, int NE>
+// NE: the accumulator entries the instantiation holds (nE <= NE; 4 keeps the common small passes light on registers);
+// NG: the groups it holds (t.G <= NG; 0 compiles the group code out), as pair_rows_kernel.
+template <typename R_, bool TL, int NE, int NG>
 __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_multi_kernel(blml_pairtab_t t, pair_multi_dev gd, const int *ZA,
                                                                        const int *ZB, const double *Wt, int R, int M,
                                                                        int slice, double *part) {
@@ -215,22 +216,23 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_multi_kernel(blml_pair
                 if (m < Mm && wj[m] != 0.0) any = true;
             if (!any) continue;
             const int *zb = lz + jj * d;
+            constexpr int NGA = NG > 0 ? NG : 1;
             R_ fac[BLML_PAIR_MAXD];
-            long r2[BLML_PAIR_MAXG];
-            bool tor[BLML_PAIR_MAXG];
+            long r2[NGA];
+            bool tor[NGA];
 #pragma unroll
             for (int u = 0; u < BLML_PAIR_MAXD; ++u) fac[u] = (R_)1;
 #pragma unroll
-            for (int g = 0; g < BLML_PAIR_MAXG; ++g) { r2[g] = 0; tor[g] = false; }
+            for (int g = 0; g < NGA; ++g) { r2[g] = 0; tor[g] = false; }
 #pragma unroll
             for (int k = 0; k < BLML_PAIR_MAXD; ++k) {
                 if (k < d) {
                     const int kd = t.kind[k];
                     const long a = za[k], b = zb[k], off = a - b;
-                    if (kd == 2 || kd == 4) {
+                    if (NG > 0 && (kd == 2 || kd == 4)) {
                         const int gk = t.grp[k];
 #pragma unroll
-                        for (int g = 0; g < BLML_PAIR_MAXG; ++g) {
+                        for (int g = 0; g < NGA; ++g) {
                             if (g == gk) {
                                 if (kd == 2) {
                                     r2[g] += off * off;
@@ -251,8 +253,8 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_multi_kernel(blml_pair
                 }
             }
 #pragma unroll
-            for (int g = 0; g < BLML_PAIR_MAXG; ++g) {
-                if (g < t.G) {
+            for (int g = 0; g < NGA; ++g) {
+                if (NG > 0 && g < t.G) {
                     const R_ tv = pair_group_row<R_>(RT + (long)g * t.LR, t.LR, r2[g], tor[g]);
                     const int ug = gd.grp_unit[g];
 #pragma unroll
@@ -285,11 +287,11 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_multi_kernel(blml_pair
                             if (k == kc) a = za[k];
                         const long b = kc >= 0 ? zb[kc] : 0;
                         R_ dv;
-                        if (g >= 0) {
+                        if (NG > 0 && g >= 0) {
                             long rg = 0;
                             bool tg = false;
 #pragma unroll
-                            for (int gg = 0; gg < BLML_PAIR_MAXG; ++gg)
+                            for (int gg = 0; gg < NGA; ++gg)
                                 if (gg == g) { rg = r2[gg]; tg = tor[gg]; }
                             dv = pair_group_row<R_>(dR + (long)gd.eq[e] * t.LR, t.LR, rg, tg);
                             if (!tg && kc >= 0) dv *= (R_)(a - b);
