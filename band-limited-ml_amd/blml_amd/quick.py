@@ -16,8 +16,10 @@ Each block's S[:, I] dlt is one pair-rows pass of libblml_hip.so (blml_quick_swe
 reference's ProductKernel.tables layout in LDS, FP64 sums, and no B x |I| block in memory.  e and Se are FP64 on
 the device; dtype picks the tables' precision (float32: the reference's f32 policy, products in FP32).
 
-Not built on this engine (they raise NotImplementedError): complex_g / rank K > 1 (the reference's _psd_solve),
-conditional dep columns, init (warm starts), and points not already binned (modes without h)."""
+init= warm-starts from a fit on the same lattice (the previous e at shared bins, each new bin at its 1-D
+minimiser).  Not built on this engine (they raise NotImplementedError): complex_g / rank K > 1 (the reference's
+_psd_solve), conditional dep columns, init from a fit on another lattice (re-binning), and points not already
+binned (modes without h)."""
 import ctypes
 import os
 
@@ -92,8 +94,6 @@ class BLMLQuick:
             if signs is not None:
                 raise ValueError("complex_g / rank > 1 / modes estimate the phases; do not pass signs as well")
             raise NotImplementedError("BLMLQuick(engine='hip'): complex / rank-K counts fits are not built")
-        if init is not None:
-            raise NotImplementedError("BLMLQuick(engine='hip'): warm starts (init=) are not built")
         keep = w > 0
         sg = None
         if signs is not None:
@@ -111,10 +111,11 @@ class BLMLQuick:
         Zi = np.round(Z / h).astype(np.int64)
         if np.abs(Zi * h - Z).max() > 1e-6 * h.max():
             raise ValueError("bin centers must lie on the lattice h * integers")
+        e0, new = self._warm(init, h, Zi, sg)
         sp = specs[0]
         tab = local.lattice_tables(sp['kinds'], sp['fc'], sp['periods'], h, Zi)
         f32 = self.dtype == np.float32
-        out = self._solve(Zi, w, n, sg, tab, f32)
+        out = self._solve(Zi, w, n, sg, tab, f32, e0, new)
         e, Se, n_sweeps, converged, max_rho, rho_hist, obj_hist, safeguard = out
         if sg is not None:
             e = e * sg
@@ -130,7 +131,31 @@ class BLMLQuick:
         self._h_vec, self._Zi, self._tab = h, Zi, tab
         return self
 
-    def _solve(self, Zi, w, n, sg, tab, f32):
+    def _warm(self, init, h, Zi, sg):
+        """init= on the same lattice (blml_gpu.py:145-153, 200-208): the previous fit's e at the bins both share
+        (by lattice coordinates; |e| with signs), 0 at new bins (mask `new`, started by _solve).  A fit on another
+        lattice would be re-binned through init.sqrt_density (blml_gpu.py:213-219): not built here."""
+        self.rebinned_, self.n_new_bins_ = False, 0
+        if init is None:
+            return None, None
+        hp = getattr(init, '_h_vec', None)
+        if hp is None or getattr(init, 'e_hat_', None) is None:
+            raise ValueError("init: a fitted BLMLQuick")
+        if np.shape(hp) != np.shape(h) or not np.allclose(hp, h):
+            raise NotImplementedError("BLMLQuick(engine='hip'): warm starts from a fit on another lattice "
+                                      "(re-binning through init.sqrt_density) are not built")
+        if np.ndim(init.e_hat_) != 1 or np.iscomplexobj(init.e_hat_):
+            raise NotImplementedError("BLMLQuick(engine='hip'): complex / rank-K warm starts are not built")
+        key = lambda Z: [z.tobytes() for z in np.ascontiguousarray(Z, np.int64)]
+        prev = dict(zip(key(init._Zi), np.asarray(init.e_hat_, float)))
+        e0 = np.array([prev.get(k, np.nan) for k in key(Zi)])
+        new = np.isnan(e0)
+        if sg is not None:
+            e0 = np.abs(e0)
+        self.n_new_bins_ = int(new.sum())
+        return np.where(new, 0.0, e0), new
+
+    def _solve(self, Zi, w, n, sg, tab, f32, e0=None, new=None):
         torch = _torch()
         lib = _lib.load()
         B = len(Zi)
@@ -147,7 +172,8 @@ class BLMLQuick:
             raise RuntimeError(f"S(z, z) = {sbb}: the kernel's value at zero must be positive")
         dw = torch.from_numpy(np.ascontiguousarray(w, np.float64)).cuda()
         dsg = None if sg is None else torch.from_numpy(np.ascontiguousarray(sg, np.float64)).cuda()
-        e = torch.ones(B, dtype=torch.float64, device='cuda')
+        e = (torch.ones(B, dtype=torch.float64, device='cuda') if e0 is None else
+             torch.from_numpy(np.ascontiguousarray(e0, np.float64)).cuda())
         Se = torch.empty_like(e)
         stats = torch.zeros(4, dtype=torch.float64, device='cuda')
         rng = np.random.default_rng(self.seed)
@@ -167,6 +193,14 @@ class BLMLQuick:
                                             P(Se), st))
 
         exact()
+        if new is not None and new.any():
+            # new bins of a warm start: the exact 1-D minimiser of the objective in e_b with the others fixed,
+            # root of S_bb e^2 + b e - n w_b with b = (S e)_b (blml_gpu.py:304-320)
+            J = torch.from_numpy(np.flatnonzero(new)).cuda()
+            b, cc = Se[J], n * dw[J]
+            disc = torch.sqrt(b * b + 4.0 * sbb * cc)
+            e[J] = torch.where(b >= 0, 2.0 * cc / (b + disc), (-b + disc) / (2.0 * sbb))
+            exact()
         obj_prev = measure()[0]
         safeguard, done, conv = False, False, False
         n_sweeps, max_rho = 0, np.inf

@@ -10,11 +10,14 @@ Restated (numpy, float64, a dense Gram matrix G over the occupied bins in place 
                           (the largest t in 2^-k, k < 31, with dPhi(t) <= 0, blml_gpu.py:352-366) for good; Se = G e
                           afresh when the residual max|Se e / (n w) - 1| first drops under tol; the stall guard
                           (< 1% over 50 sweeps); max_sweeps, then e scaled to e'Se = n^2 (blml_gpu.py:432-439)
+  warm_start(e_prev, new, G, w)   init= on the same lattice (blml_gpu.py:200-208, 304-320): the previous e at the
+                          shared bins, each new bin at root(Se_b, n w_b, S_bb) from e with the new bins at 0
   newton_solve(G, w)      the unique minimiser of the same convex Phi by damped Newton (an independent solver)
 
 Pinning (tests/test_oracle_quick.py): root by its quadratic's residual and the 1-D objective's derivative; the
 fixed point of quick_fit by newton_solve (the problem is convex, so both must reach the same e); each safeguarded
-block step against a direct evaluation of Phi; Se = G e at the end.  The trajectory itself (sweep counts, when the
+block step against a direct evaluation of Phi; Se = G e at the end; a warm-started fit by the same Newton fixed
+point, and each new bin's start by the 1-D objective's derivative.  The trajectory itself (sweep counts, when the
 safeguard switches on) follows the reference's code and no reference test holds it: parity unpinned beyond the
 fixed point."""
 import numpy as np
@@ -57,17 +60,30 @@ def sweep(G, w, n, sbb, e, Se, perm, Bs, safe, log_steps=None):
             Se += v
 
 
-def quick_fit(G, w, tol=1e-4, block_size=4096, seed=0, max_sweeps=None, log_steps=None):
+def warm_start(e_prev, new, G, w):
+    """A warm start on the same lattice (blml_gpu.py:200-208, 304-320): the previous fit's e at the bins both fits
+    share (new: the mask of bins it lacks, whose e starts at 0), then each new bin at the exact 1-D minimiser of
+    the objective with the others fixed, root(b, n w_b, S_bb) with b = (S e)_b, all from the same S e."""
+    G = np.asarray(G, float)
+    w = np.asarray(w, float)
+    e = np.where(new, 0.0, np.asarray(e_prev, float))
+    if np.any(new):
+        Se = G @ e
+        e[new] = root(Se[new], float(w.sum()) * w[new], float(G[0, 0]))
+    return e
+
+
+def quick_fit(G, w, tol=1e-4, block_size=4096, seed=0, max_sweeps=None, log_steps=None, e0=None):
     """(e, Se, n_sweeps, converged, max_rho, rho_hist, obj_hist, safeguard) of the real rank-1 counts fit with Gram
     matrix G (B x B, the signs already in it: D S D) and counts w (B,).  log_steps: a list that receives
-    (e_before, dlt, t, I) for every safeguarded block step."""
+    (e_before, dlt, t, I) for every safeguarded block step.  e0: the start (warm_start), else ones."""
     G = np.asarray(G, float)
     w = np.asarray(w, float)
     B = len(w)
     n = float(w.sum())
     sbb = float(G[0, 0])
     rng = np.random.default_rng(seed)
-    e = np.ones(B)
+    e = np.ones(B) if e0 is None else np.array(e0, float)
     Se = G @ e
     Bs = max(1, min(int(block_size), -(-B // 8)))
 

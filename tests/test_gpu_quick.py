@@ -135,3 +135,35 @@ def test_capped_fit_and_sqrt_density():
     assert abs(q.e_hat_ @ q.Se_ - n * n) < 1e-9 * n * n
     g = q.sqrt_density(q.Z_)
     assert np.abs(g - q.Se_ / n).max() < 1e-9 * np.abs(q.Se_ / n).max()
+
+
+def test_warm_start_matches_oracle():
+    """init= on the same lattice (FP64 tables): a second fit on other bins (some gone, some new, counts moved) warm-
+    started from the first equals the oracle's quick_fit from warm_start(the first fit's e) -- the same sweeps, e to
+    1e-9 -- and reaches the cold fit's solution; a fit on another lattice raises NotImplementedError."""
+    from blml_amd.quick import BLMLQuick
+    name, shape, h, cols, units, bs, seed, signed = CASES[1]
+    Z, w = _problem(shape, 8000, seed, zero_frac=0.0)
+    rng = np.random.default_rng(11)
+    m1, m2 = rng.uniform(size=len(w)) < 0.85, rng.uniform(size=len(w)) < 0.85
+    hv = np.asarray(h, float)
+    Z1, w1 = Z[m1 & (w > 0)], w[m1 & (w > 0)]
+    Z2, w2 = Z[m2 & (w > 0)], w[m2 & (w > 0)] + rng.poisson(3.0, int((m2 & (w > 0)).sum()))
+    _note(f'quick warm start: {len(w1)} -> {len(w2)} bins')
+    kw = dict(tol=1e-8, block_size=bs, seed=seed, dtype=np.float64)
+    q1 = BLMLQuick(**kw).fit(Z1 * hv, _modes(cols, hv), sample_weight=w1)
+    q2 = BLMLQuick(**kw).fit(Z2 * hv, _modes(cols, hv), sample_weight=w2, init=q1)
+    key = lambda Z: [z.tobytes() for z in np.ascontiguousarray(Z, np.int64)]
+    prev = dict(zip(key(Z1), q1.e_hat_))
+    e_prev = np.array([prev.get(k, np.nan) for k in key(Z2)])
+    new = np.isnan(e_prev)
+    assert q2.n_new_bins_ == int(new.sum()) > 0 and (~new).any()
+    G = OP.dense_pair_gram(units, hv, Z2, Z2)
+    e0 = OQ.warm_start(np.where(new, 0.0, e_prev), new, G, w2)
+    e, Se, ns, conv, rho, rh, oh, safe = OQ.quick_fit(G, w2, tol=1e-8, block_size=bs, seed=seed, e0=e0)
+    assert conv and q2.converged_ and q2.n_sweeps_ == ns, (q2.n_sweeps_, ns)
+    assert np.abs(q2.e_hat_ - e).max() < 1e-9 * np.abs(e).max()
+    cold = BLMLQuick(**kw).fit(Z2 * hv, _modes(cols, hv), sample_weight=w2)
+    assert np.abs(q2.e_hat_ - cold.e_hat_).max() < 1e-6 * np.abs(cold.e_hat_).max()
+    with pytest.raises(NotImplementedError, match='another lattice'):
+        BLMLQuick(**kw).fit(Z2 * hv * 2, _modes(cols, hv * 2), sample_weight=w2, init=q1)

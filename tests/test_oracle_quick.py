@@ -86,3 +86,32 @@ def test_capped_fit_is_normalised():
     n = w.sum()
     assert abs(e @ (G @ e) - n * n) < 1e-9 * n * n
     assert np.abs(Se - G @ e).max() < 1e-9 * np.abs(Se).max()
+
+
+def test_warm_start_reaches_the_convex_minimiser():
+    """init= on the same lattice: a fit's e carried to a second problem (some bins gone, some new, counts moved):
+    each new bin starts where the 1-D objective's derivative in it vanishes (the others fixed), and the warm fit
+    reaches the second problem's Newton fixed point, in fewer sweeps than the cold fit."""
+    name, shape, units, h, total = CASES[1]
+    Z, w, G = problem(shape, units, h, total, 1)
+    rng = np.random.default_rng(5)
+    m1, m2 = rng.uniform(size=len(w)) < 0.85, rng.uniform(size=len(w)) < 0.85
+    Z1, w1, G1 = Z[m1], w[m1], G[np.ix_(m1, m1)]
+    Z2, w2, G2 = Z[m2], w[m2] + rng.poisson(3.0, int(m2.sum())), G[np.ix_(m2, m2)]
+    e1 = OQ.quick_fit(G1, w1, tol=1e-10, block_size=37, seed=2)[0]
+    key = lambda Z: [z.tobytes() for z in np.ascontiguousarray(Z)]
+    prev = dict(zip(key(Z1), e1))
+    e_prev = np.array([prev.get(k, np.nan) for k in key(Z2)])
+    new = np.isnan(e_prev)
+    assert new.any() and (~new).any(), (new.sum(), len(new))
+    e0 = OQ.warm_start(e_prev, new, G2, w2)
+    n2 = w2.sum()
+    Se0 = G2 @ np.where(new, 0.0, e0)                     # the others' part at the new bins
+    sbb = G2[0, 0]
+    assert np.abs(sbb * e0[new] + Se0[new] - n2 * w2[new] / e0[new]).max() < 1e-9 * np.abs(Se0[new]).max()
+    assert np.array_equal(e0[~new], e_prev[~new])
+    warm = OQ.quick_fit(G2, w2, tol=1e-10, block_size=37, seed=2, e0=e0)
+    cold = OQ.quick_fit(G2, w2, tol=1e-10, block_size=37, seed=2)
+    en = OQ.newton_solve(G2, w2)
+    assert warm[3] and np.abs(warm[0] - en).max() < 1e-7 * np.abs(en).max()
+    assert warm[2] < cold[2], (warm[2], cold[2])
