@@ -1570,12 +1570,19 @@ int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *
     const long n = R * M;
     const unsigned blocks = (unsigned)std::min<long>((n + BLML_PAIR_NT - 1) / BLML_PAIR_NT, 16384);
     const blml_pairtab_t t = *tab;
-    if (tab->f32)
-        pair_cols_kernel<float><<<blocks, BLML_PAIR_NT, 0, st>>>(t, ZA, ZB, R, M, depmask, (float *)out,
-                                                                 (float *)(out_m ? out_m : out));
-    else
-        pair_cols_kernel<double><<<blocks, BLML_PAIR_NT, 0, st>>>(t, ZA, ZB, R, M, depmask, (double *)out,
-                                                                  (double *)(out_m ? out_m : out));
+#define BLML_COLS_LAUNCH(R_, NG) \
+    pair_cols_kernel<R_, NG><<<blocks, BLML_PAIR_NT, 0, st>>>(t, ZA, ZB, R, M, depmask, (R_ *)out, \
+                                                             (R_ *)(out_m ? out_m : out))
+#define BLML_COLS_BY_G(R_) \
+    do { \
+        if (t.G == 0) BLML_COLS_LAUNCH(R_, 0); \
+        else if (t.G == 1) BLML_COLS_LAUNCH(R_, 1); \
+        else BLML_COLS_LAUNCH(R_, BLML_PAIR_MAXG); \
+    } while (0)
+    if (tab->f32) BLML_COLS_BY_G(float);
+    else BLML_COLS_BY_G(double);
+#undef BLML_COLS_BY_G
+#undef BLML_COLS_LAUNCH
     HIPCHECK(hipGetLastError());
     return BLML_OK;
 }

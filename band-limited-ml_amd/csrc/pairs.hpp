@@ -336,7 +336,7 @@ __global__ void pair_rows_multi_sum_kernel(const double *part, int R, int nslice
 }
 
 // out[r * M + j] = S(ZA_r, ZB_j); with depmask, out_m[r * M + j] = the product over the columns not in depmask.
-template <typename R_>
+template <typename R_, int NG>
 __global__ void __launch_bounds__(BLML_PAIR_NT) pair_cols_kernel(blml_pairtab_t t, const int *ZA, const int *ZB,
                                                                  long R, long M, long depmask, R_ *out, R_ *out_m) {
     const R_ *T = reinterpret_cast<const R_ *>(t.T);
@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_cols_kernel(blml_pairtab_t 
             zb[k] = k < d ? ZB[j * d + k] : 0;
         }
         R_ vn, vd;
-        pair_value<R_>(t, T, RT, za, zb, depmask, vn, vd);
+        pair_value<R_, NG>(t, T, RT, za, zb, depmask, vn, vd);
         if (depmask != 0) out_m[idx] = vn;
         out[idx] = vn * vd;
     }
