@@ -300,6 +300,14 @@ def _check_dspans(A, B, tab):
 def _rows_fused(ZA, ZB, Wt, tab, umask, grad, f32, ranked):
     """s (R, Mm), ds (R, Mm, nW) through blml_pair_rows_grad (umask None: one density of every unit) or
     blml_pair_rows_multi."""
+    launch, result = fused_rows_launcher(ZA, ZB, Wt, tab, umask, grad, f32, ranked)
+    launch()
+    return result()
+
+
+def fused_rows_launcher(ZA, ZB, Wt, tab, umask=None, grad=True, f32=False, ranked=False):
+    """(launch, result): the device inputs of _rows_fused uploaded once; launch() runs the kernels on the current
+    stream (no host synchronisation), result() returns (s, ds) as _rows_fused does."""
     torch = _torch()
     nW, U = (int(tab['nW']) if grad else 0), int(tab['U'])
     Wt = np.ascontiguousarray(np.asarray(Wt, np.float64).reshape(len(np.atleast_2d(ZB)), -1))
@@ -341,22 +349,29 @@ def _rows_fused(ZA, ZB, Wt, tab, umask, grad, f32, ranked):
     ds = torch.zeros(max(R * Mm * nW, 1), dtype=torch.float64, device='cuda')
     lib = _lib.load()
     P = ctypes.c_void_p
-    st = P(torch.cuda.current_stream().cuda_stream)
-    if umask is None:
-        rc = lib.blml_pair_rows_grad(ctypes.byref(t), ctypes.byref(g), P(dA.data_ptr()), P(dB.data_ptr()),
-                                     P(dw.data_ptr()), R, M, P(s.data_ptr()), P(ds.data_ptr()), st)
-    else:
-        um = np.ascontiguousarray(np.asarray(umask, np.int64).reshape(Mm, U) != 0, np.int32)
-        rc = lib.blml_pair_rows_multi(ctypes.byref(t), ctypes.byref(g), P(dA.data_ptr()), P(dB.data_ptr()),
-                                      P(dw.data_ptr()), um.ctypes.data_as(P), Mm, R, M, P(s.data_ptr()),
-                                      P(ds.data_ptr()), st)
-    _lib.check(rc)
-    out = s.cpu().numpy()[:R * Mm].reshape(R, Mm)
-    dout = ds.cpu().numpy()[:R * Mm * nW].reshape(R, Mm, nW)
-    del keep, ddT, ddR
-    if nW == 0:
-        dout = np.zeros((R, Mm, max(int(tab.get('nW', 0)), 1)))
-    return out, dout
+    um = None if umask is None else np.ascontiguousarray(np.asarray(umask, np.int64).reshape(Mm, U) != 0, np.int32)
+
+    def launch():
+        st = P(torch.cuda.current_stream().cuda_stream)
+        if um is None:
+            rc = lib.blml_pair_rows_grad(ctypes.byref(t), ctypes.byref(g), P(dA.data_ptr()), P(dB.data_ptr()),
+                                         P(dw.data_ptr()), R, M, P(s.data_ptr()), P(ds.data_ptr()), st)
+        else:
+            rc = lib.blml_pair_rows_multi(ctypes.byref(t), ctypes.byref(g), P(dA.data_ptr()), P(dB.data_ptr()),
+                                          P(dw.data_ptr()), um.ctypes.data_as(P), Mm, R, M, P(s.data_ptr()),
+                                          P(ds.data_ptr()), st)
+        _lib.check(rc)
+
+    def result():
+        held = (keep, ddT, ddR)                   # the device tables stay alive until the results are read
+        out = s.cpu().numpy()[:R * Mm].reshape(R, Mm)
+        dout = ds.cpu().numpy()[:R * Mm * nW].reshape(R, Mm, nW)
+        del held
+        if nW == 0:
+            dout = np.zeros((R, Mm, max(int(tab.get('nW', 0)), 1)))
+        return out, dout
+
+    return launch, result
 
 
 def rows_multi(ZA, ZB, Wt, tab, umask, grad=False, f32=False, ranked=False):

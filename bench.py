@@ -263,6 +263,18 @@ def pair_leg(torch, st, event_time):
                  seconds=t_p, pairs_per_s=n_rows * len(Zp) / t_p)
     del keep
     print(f'bench: pair rows {n_rows} x {len(Zp)} in {t_p * 1e3:.1f} ms', file=sys.stderr, flush=True)
+    # derivative rows on the same bins (grad=True): dS/dlog fc of each column, three FP64 accumulators per row
+    tabg = local.lattice_tables(['slits:0.5'] * 3, [0.2 / hp] * 3, None, [hp] * 3, Zp,
+                                want=[(k, 'fc') for k in range(3)])
+    n_g = 1 << 14
+    launch_g, result_g = local.fused_rows_launcher(Zp[:n_g], Zp, wp[:, None], tabg, None, True, True)
+    launch_g()
+    t_g = event_time(launch_g, 3)
+    result_g()
+    pairs['grad_rows'] = dict(rows=n_g, sources=int(len(Zp)), entries=3, seconds=t_g,
+                              pairs_per_s=n_g * len(Zp) / t_g)
+    print(f'bench: derivative rows {n_g} x {len(Zp)} (3 entries) in {t_g * 1e3:.1f} ms', file=sys.stderr,
+          flush=True)
     # the companion fit through the pair path: BLMLQuick on the lattice's counts (real rank 1, FP32 tables, block
     # 4096), a capped number of sweeps -- a band-pass slits kernel (fill 0.5) leaves the real rank-1 objective
     # without an interior optimum, so the fit runs to the cap and is scaled to e'Se = n^2
