@@ -350,8 +350,10 @@ def fused_rows_launcher(ZA, ZB, Wt, tab, umask=None, grad=True, f32=False, ranke
     lib = _lib.load()
     P = ctypes.c_void_p
     um = None if umask is None else np.ascontiguousarray(np.asarray(umask, np.int64).reshape(Mm, U) != 0, np.int32)
+    held = (keep, ddT, ddR)                       # device tables the kernels read: alive with either closure
 
     def launch():
+        assert held is not None
         st = P(torch.cuda.current_stream().cuda_stream)
         if um is None:
             rc = lib.blml_pair_rows_grad(ctypes.byref(t), ctypes.byref(g), P(dA.data_ptr()), P(dB.data_ptr()),
@@ -363,10 +365,9 @@ def fused_rows_launcher(ZA, ZB, Wt, tab, umask=None, grad=True, f32=False, ranke
         _lib.check(rc)
 
     def result():
-        held = (keep, ddT, ddR)                   # the device tables stay alive until the results are read
+        assert held is not None
         out = s.cpu().numpy()[:R * Mm].reshape(R, Mm)
         dout = ds.cpu().numpy()[:R * Mm * nW].reshape(R, Mm, nW)
-        del held
         if nW == 0:
             dout = np.zeros((R, Mm, max(int(tab.get('nW', 0)), 1)))
         return out, dout
