@@ -130,7 +130,7 @@ def lattice_tables(kinds, fc, periods, h, Zi, want=()):
             row = one_column_values(kinds[k], fc[k], periods[k], np.arange(nb) * h[k])
             kd_arr[k], nbins[k] = 1, nb
         else:
-            raise NotImplementedError(f"lattice_tables: kind {kinds[k]!r} (radial rows are not built here)")
+            raise NotImplementedError(f"lattice_tables: kind {kinds[k]!r} has no lattice row on this engine")
         base[k] = at
         rows.append(np.asarray(row, float))
         at += len(row)
