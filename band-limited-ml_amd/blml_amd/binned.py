@@ -288,7 +288,9 @@ class FitBatch:
                                            self.stream()))
         return U
 
-    def certificate(self, max_iter=20000, tol=1e-12):
+    def certificate(self, max_iter=20000, tol=1e-6):
+        """lambda_max(D^1/2 S_0 D^1/2) / n of every fit (NaN without a shared kernel) and the power iterations used
+        (negative: max_iter reached before the eigen-residual fell below tol * lambda)."""
         torch = _torch()
         lam = torch.zeros(len(self.fits), dtype=torch.float64, device='cuda')
         its = torch.zeros(len(self.fits), dtype=torch.int32, device='cuda')
@@ -406,12 +408,13 @@ class BLMLBinned:
 
     def certificate(self, k=1, tol=1e-6):
         """lambda_max(D^1/2 S D^1/2) / n (shared kernel only, blml_binned.py:501-518), by power iteration on the
-        device; the iteration stops when the FP64 Rayleigh quotient changes by less than tol * 1e-6 (relative)."""
+        device with FP64 transforms; tol is eigsh's: the iteration stops when the eigen-residual of the unit iterate
+        is at most tol * lambda.  certificate_iters_ holds the iterations (negative if 100000 were not enough)."""
         if not self.shared_:
             raise ValueError("certificate: shared kernel only (per-mode kernels: no convex lift)")
         if k != 1:
             raise ValueError("certificate: k = 1 (the largest eigenvalue) on this engine")
-        lam, its = self._batch.certificate(max_iter=100000, tol=tol * 1e-6)
+        lam, its = self._batch.certificate(max_iter=100000, tol=tol)
         self.certificate_iters_ = int(its[self._index])
         return float(lam[self._index])
 

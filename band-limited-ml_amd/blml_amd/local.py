@@ -74,7 +74,8 @@ def lattice_tables(kinds, fc, periods, h, Zi, want=()):
     row on the grid r = j / RADIAL_Q lattice steps, j < floor(sqrt(sum span^2) RADIAL_Q) + 2, with fc h equal across
     the group (blml_trivial.py:284-303; the row kernels interpolate it linearly in r).
 
-    Units: the one-column units in column order, then the periodic ball (if any), then the balls by kind tag.
+    Units: the one-column units in column order, then the multi-column units (balls and periodic balls) by kind tag
+    in order of first appearance, group g being the g-th of them (the reference's kernel_units / radial_groups).
     want: derivative entries (unit, p) for rows(grad=True), p a parameter (dK/dlog p: 'fc' of sinc / slits / ball,
     'period' of dirichlet) or a coordinate shift ('shift' or ('shift', i), dK/dt_i), built as the reference's
     ProductKernel._assemble (blml_kernels.py:165-182): a column's row over raw offsets -J..J (its rank table for a
@@ -102,12 +103,16 @@ def lattice_tables(kinds, fc, periods, h, Zi, want=()):
     vals = [None] * d
     rows, rrows, grp_unit, uinfo = [], [], [], []
     single = [k for k in range(d) if split_kind(kinds[k])[0] not in ('pball', 'ball')]
-    pcols = [k for k in range(d) if split_kind(kinds[k])[0] == 'pball']
-    bgroups = {}                                  # ball columns by kind tag, in order of first appearance
+    # the multi-column units (balls and periodic balls) by kind tag, in order of first appearance; group g is the
+    # g-th of them (blml_trivial.py:251-261, kernel_units :330-335)
+    rgroups = {}
     for k in range(d):
         name, tag = split_kind(kinds[k])
-        if name == 'ball':
-            bgroups.setdefault(tag, []).append(k)
+        if name in ('ball', 'pball'):
+            g = rgroups.setdefault(tag, (name, []))
+            if g[0] != name:
+                raise ValueError(f"radial group {tag!r} mixes kinds {g[0]!r} and {name!r}")
+            g[1].append(k)
     at = 0
     for ui, k in enumerate(single):
         name = split_kind(kinds[k])[0]
@@ -134,28 +139,28 @@ def lattice_tables(kinds, fc, periods, h, Zi, want=()):
         base[k] = at
         rows.append(np.asarray(row, float))
         at += len(row)
-    if pcols:
-        P = [float(periods[k]) for k in pcols]
-        ms = [int(round(p / h[k])) for p, k in zip(P, pcols)]
-        offs = np.stack([g.ravel() for g in np.meshgrid(*[np.arange(m) for m in ms], indexing='ij')], 1)
-        rrows.append(multi_values('pball', [fc[pcols[0]]] * len(pcols), P,
-                                  [offs[:, i] * h[k] for i, k in enumerate(pcols)]))
-        grp[pcols] = 0
-        kd_arr[pcols] = 4
-        nbins[pcols] = ms
-        grp_unit.append(len(single))
-        uinfo.append(('pball', pcols, P, ms, offs))
-    for dims in bgroups.values():
+    for name, dims in rgroups.values():
+        g = len(rrows)
+        grp[dims] = g
+        grp_unit.append(len(single) + g)
+        if name == 'pball':
+            P = [float(periods[k]) for k in dims]
+            ms = [int(round(p / h[k])) for p, k in zip(P, dims)]
+            offs = np.stack([x.ravel() for x in np.meshgrid(*[np.arange(m) for m in ms], indexing='ij')], 1)
+            rrows.append(multi_values('pball', [fc[k] for k in dims], P,
+                                      [offs[:, i] * h[k] for i, k in enumerate(dims)]))
+            kd_arr[dims] = 4
+            nbins[dims] = ms
+            uinfo.append(('pball', dims, P, ms, offs))
+            continue
         c = fc[dims] * h[dims]
         if not np.allclose(c, c[0], rtol=1e-9, atol=0):
             raise ValueError(f"radial group dims {dims}: fc*h must be equal across the group, got {c}")
         r2max = int((alloc[dims] ** 2).sum())
         r = np.arange(int(np.floor(np.sqrt(r2max) * RADIAL_Q)) + 2, dtype=float) / RADIAL_Q
         diffs = [r * h[dims[0]]] + [np.zeros_like(r)] * (len(dims) - 1)     # along the first column: K is radial
-        grp[dims] = len(rrows)
         kd_arr[dims] = 2
         rrows.append(multi_values('ball', [fc[k] for k in dims], None, diffs))
-        grp_unit.append(len(single) + len(grp_unit))
         uinfo.append(('ball', dims, r, float(c[0])))
     nW = len(want)
     wunit = np.zeros(max(nW, 1), np.int64)
@@ -186,7 +191,7 @@ def lattice_tables(kinds, fc, periods, h, Zi, want=()):
             if i is None:
                 raise ValueError("pball: no parameter gradient (fc in whole harmonics, fixed periods); shifts only")
             wgrp[q], wcol[q] = grp[pc[0]], pc[i]
-            drr[q] = pball_dshift([fc[pc[0]]] * len(pc), P, i, [offs[:, j] * h[k] for j, k in enumerate(pc)])
+            drr[q] = pball_dshift([fc[k] for k in pc], P, i, [offs[:, j] * h[k] for j, k in enumerate(pc)])
         else:
             dims, r, cg = info[1:]
             D = len(dims)
@@ -236,8 +241,8 @@ def _check_spans(A, B, tab):
         if kd == 0 and far[k] > Jk[k]:
             raise RuntimeError(f"column {k}: offsets up to {far[k]} beyond the table's half width {Jk[k]} "
                                "(stale tables)")
-        if kd == 3 and max(hi_a[k], hi_b[k]) >= mk[k]:
-            raise RuntimeError(f"rank column {k}: rank beyond the table (stale tables)")
+        if kd == 3 and (max(hi_a[k], hi_b[k]) >= mk[k] or min(lo_a[k], lo_b[k]) < 0):
+            raise RuntimeError(f"rank column {k}: rank outside the table's 0..{int(mk[k]) - 1} (stale tables)")
         if kd in (1, 4) and far[k] >= nbins[k] + (kd == 1):
             raise RuntimeError(f"column {k}: offsets beyond one period (stale tables)")
         if kd == 2:

@@ -38,6 +38,25 @@ int fail(int code, const std::string &msg) {
         if (e_ != hipSuccess) return fail(BLML_ERUNTIME, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Device scratch of one call, allocated on the call's stream and given back on it however the call returns (an
+// error return included); release() is the checked release of the normal path.
+struct StreamScratch {
+    void *p = nullptr;
+    hipStream_t st;
+    explicit StreamScratch(hipStream_t s) : st(s) {}
+    StreamScratch(const StreamScratch &) = delete;
+    StreamScratch &operator=(const StreamScratch &) = delete;
+    ~StreamScratch() {
+        if (p) (void)hipFreeAsync(p, st);
+    }
+    hipError_t alloc(size_t bytes) { return hipMallocAsync(&p, bytes, st); }
+    hipError_t release() {
+        const hipError_t e = p ? hipFreeAsync(p, st) : hipSuccess;
+        p = nullptr;
+        return e;
+    }
+};
+
 
 // ------------------------------------------------------------------------------------------------ kernels
 template <typename T>
@@ -458,26 +477,34 @@ __global__ void __launch_bounds__(BLML_NT) fit_grid_kernel(DevPlan<T> p, unsigne
                se_out, density, results, merit_hist, rho_hist, tick_hz);
 }
 
-// certificate(): lambda_max(D^1/2 S_0 D^1/2) / n, D_b = |e_b|^2 / w_b on occupied bins, by power iteration with an
-// FP64 Rayleigh quotient.  Stops when the quotient changes by less than tol (relative) or after max_iter steps.
-template <typename T>
-__device__ void run_cert(const DevPlan<T> &p, int fi, Team tm, const T *e_hat, const double *counts, int max_iter,
-                         double tol, double *lam_out, int *iters) {
+// certificate(): lambda_max(D^1/2 S_0 D^1/2) / n, D_b = |e_b|^2 / w_b on occupied bins, by power iteration with FP64
+// transforms and an FP64 Rayleigh quotient.  It stops when the eigen-residual of the unit iterate x and its quotient
+// lam is small, ||A x - lam x|| <= tol |lam| (ARPACK's convergence test for a Ritz pair, the test eigsh applies at
+// blml_binned.py:510-518; the quotient's error is then at most tol^2 lam^2 / gap), or after max_iter steps.  Te: the
+// plan's coefficient type; an FP32 plan runs through its FP64 view, with the finishing pass's FP64 scratch (xoff
+// bytes into the fit's workspace).  iters: the steps taken, negated when max_iter ended the iteration.
+template <typename Te>
+__device__ void run_cert(const DevPlan<double> &p, int fi, Team tm, const Te *e_hat, const double *counts,
+                         int max_iter, double tol, double *lam_out, int *iters) {
     __shared__ double red[BLML_NWAVE * BLML_NRED];
     const blml_fit_t f = p.fits[fi];
     if (!f.shared) {
         if (tm.lead()) { lam_out[fi] = NAN; iters[fi] = 0; }
         return;
     }
-    MatvecCtx<T> mv;
-    make_matvec_ctx<T>(p, f, mv, tm);
+    MatvecCtx<double> mv;
+    make_matvec_ctx<double>(p, f, mv, tm);
+    if (sizeof(Te) == 4) {
+        mv.X = (cpx<double> *)(p.ws + f.ws_off + fin_bytes_off(p.params, f, 4));
+        mv.Y = mv.X + f.scratch_len;
+    }
     const int nc = f.complex_g ? 2 : 1;
     const int B = f.B;
     const long b0 = tm.first(), bs = tm.stride();
-    double *dh = (double *)(p.ws + f.ws_off + pool_bytes_off(f, (int)sizeof(T)));   // B doubles
+    double *dh = (double *)(p.ws + f.ws_off + pool_bytes_off(f, (int)sizeof(Te)));   // B doubles
     double *x = dh + B, *y = x + B;
     const double *w = counts + f.counts_off;
-    const T *E = e_hat + f.out_off;
+    const Te *E = e_hat + f.out_off;
     for (long b = b0; b < B; b += bs) {
         double D = 0.0;
         if (w[b] > 0) {
@@ -499,47 +526,55 @@ __device__ void run_cert(const DevPlan<T> &p, int fi, Team tm, const T *e_hat, c
     }
     for (long b = b0; b < B; b += bs) x[b] = dh[b] / nrm;
     tm.sync();
-    double lam = 0.0, lam_prev = -INFINITY;
+    double lam = 0.0;
     int it = 0;
+    bool conv = false;
     for (it = 1; it <= max_iter; ++it) {
         for (long b = b0; b < B; b += bs) y[b] = dh[b] * x[b];
         tm.sync();
-        matvec_mode<T, double, double>(mv, f.fac_begin[0], f.fac_count[0], y, y, B, 1);
+        matvec_mode<double, double, double>(mv, f.fac_begin[0], f.fac_count[0], y, y, B, 1);
         double a[3] = {0.0, 0.0, 0.0};
         for (long b = b0; b < B; b += bs) {
-            const double yb = dh[b] * (double)y[b];
-            a[0] += (double)x[b] * yb;
+            const double yb = dh[b] * y[b];
+            a[0] += x[b] * yb;
             a[1] += yb * yb;
-            a[2] += (double)x[b] * (double)x[b];
+            a[2] += x[b] * x[b];
         }
         tm.sum<3>(a, red);
         lam = a[0] / a[2];
         const double ny = sqrt(a[1]);
         if (!(ny > 0)) break;
-        for (long b = b0; b < B; b += bs) x[b] = dh[b] * y[b] / ny;
-        tm.sync();
-        if (fabs(lam - lam_prev) <= tol * fabs(lam)) break;
-        lam_prev = lam;
+        // the residual of (x, lam), summed in the pass that forms the next iterate A x / ||A x|| (each thread reads
+        // and writes its own bins only)
+        double r2 = 0.0;
+        for (long b = b0; b < B; b += bs) {
+            const double yb = dh[b] * y[b];
+            const double v = yb - lam * x[b];
+            r2 += v * v;
+            x[b] = yb / ny;
+        }
+        const double r = sqrt(tm.sum1(r2, red) / a[2]);
+        if (r <= tol * fabs(lam)) { conv = true; break; }
     }
     if (tm.lead()) {
         lam_out[fi] = lam / f.n;
-        iters[fi] = it > max_iter ? max_iter : it;
+        iters[fi] = conv ? it : -(it > max_iter ? max_iter : it);
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<T> p, const T *e_hat, const double *counts,
+template <typename Te>
+__global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<double> p, const Te *e_hat, const double *counts,
                                                        int max_iter, double tol, double *lam_out, int *iters) {
     if ((int)blockIdx.x >= p.nfits) return;
-    run_cert<T>(p, blockIdx.x, block_team(), e_hat, counts, max_iter, tol, lam_out, iters);
+    run_cert<Te>(p, blockIdx.x, block_team(), e_hat, counts, max_iter, tol, lam_out, iters);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(BLML_NT) cert_grid_kernel(DevPlan<T> p, unsigned *bar, double *part,
-                                                            const T *e_hat, const double *counts, int max_iter,
+template <typename Te>
+__global__ void __launch_bounds__(BLML_NT) cert_grid_kernel(DevPlan<double> p, unsigned *bar, double *part,
+                                                            const Te *e_hat, const double *counts, int max_iter,
                                                             double tol, double *lam_out, int *iters) {
-    run_cert<T>(p, 0, Team{(int)blockIdx.x, (int)gridDim.x, bar, part, 0}, e_hat, counts, max_iter, tol, lam_out,
-                iters);
+    run_cert<Te>(p, 0, Team{(int)blockIdx.x, (int)gridDim.x, bar, part, 0}, e_hat, counts, max_iter, tol, lam_out,
+                 iters);
 }
 
 // FP32 plans only: Se, density and loglik of every fit re-evaluated with an FP64 mat-vec of the FP32-converged
@@ -1052,7 +1087,8 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
     HIPCHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     rc = static_lds((const void *)fit_kernel<T>, st);
     if (!rc) rc = static_lds((const void *)matvec_kernel<T>, st);
-    if (!rc) rc = static_lds((const void *)cert_kernel<T>, st);
+    if (!rc && sizeof(T) == 8) rc = static_lds((const void *)cert_kernel<double>, st);
+    if (!rc && sizeof(T) == 4) rc = static_lds((const void *)cert_kernel<float>, st64);
     if (!rc) rc = static_lds((const void *)spec_kernel<T>, st);
     if (!rc && sizeof(T) == 4) rc = static_lds((const void *)finish64_kernel, st64);
     if (!rc && sizeof(T) == 4) rc = static_lds((const void *)spec_kernel<double>, st64);
@@ -1137,7 +1173,12 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
         int o1 = 0, o2 = 0, o3 = 0, o4 = 0;
         HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o1, fit_grid_kernel<T>, BLML_NT, pl.lds_bytes));
         HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o2, matvec_grid_kernel<T>, BLML_NT, pl.lds_bytes));
-        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o3, cert_grid_kernel<T>, BLML_NT, pl.lds_bytes));
+        if (sizeof(T) == 8)
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o3, cert_grid_kernel<double>, BLML_NT,
+                                                                  pl.lds_bytes));
+        else
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o3, cert_grid_kernel<float>, BLML_NT,
+                                                                  pl.lds64_bytes));
         o4 = o1;
         if (sizeof(T) == 4)
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, finish64_grid_kernel, BLML_NT,
@@ -1326,23 +1367,23 @@ int blml_certificate(void *plan, const void *e_hat, const double *counts, int32_
     Plan *pl = (Plan *)plan;
     if (!pl || !e_hat || !counts || !lam || !iters) return fail(BLML_EVALUE, "null argument");
     if (max_iter < 1) return fail(BLML_EVALUE, "max_iter must be >= 1");
+    if (!(tol >= 0)) return fail(BLML_EVALUE, "tol must be >= 0");
     hipStream_t st = (hipStream_t)stream;
+    // FP64 transforms for every plan: an FP32 plan's FP64 view (its finishing pass's spectra, twiddles and scratch)
+    const bool f32 = pl->dtype == BLML_F32;
+    DevPlan<double> dp = f32 ? dev_plan64(*pl) : dev_plan<double>(*pl);
+    const int lds = f32 ? pl->lds64_bytes : pl->lds_bytes;
     if (pl->spread) {
-        if (pl->dtype == BLML_F32) {
-            DevPlan<float> dp = dev_plan<float>(*pl);
-            void *args[] = {&dp, &pl->d_bar, &pl->d_part, &e_hat, &counts, &max_iter, &tol, &lam, &iters};
-            return launch_coop(*pl, (const void *)cert_grid_kernel<float>, args, pl->lds_bytes, st);
-        }
-        DevPlan<double> dp = dev_plan<double>(*pl);
         void *args[] = {&dp, &pl->d_bar, &pl->d_part, &e_hat, &counts, &max_iter, &tol, &lam, &iters};
-        return launch_coop(*pl, (const void *)cert_grid_kernel<double>, args, pl->lds_bytes, st);
+        return launch_coop(*pl, f32 ? (const void *)cert_grid_kernel<float> : (const void *)cert_grid_kernel<double>,
+                           args, lds, st);
     }
-    if (pl->dtype == BLML_F32)
-        cert_kernel<float><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<float>(*pl), (const float *)e_hat,
-                                                                        counts, max_iter, tol, lam, iters);
+    if (f32)
+        cert_kernel<float><<<pl->nfits, BLML_NT, lds, st>>>(dp, (const float *)e_hat, counts, max_iter, tol, lam,
+                                                             iters);
     else
-        cert_kernel<double><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<double>(*pl), (const double *)e_hat,
-                                                                         counts, max_iter, tol, lam, iters);
+        cert_kernel<double><<<pl->nfits, BLML_NT, lds, st>>>(dp, (const double *)e_hat, counts, max_iter, tol, lam,
+                                                              iters);
     HIPCHECK(hipGetLastError());
     return BLML_OK;
 }
@@ -1422,11 +1463,11 @@ int blml_pair_rows(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *
     }
     long rb, nsl, slice;
     pair_rows_grid(R, M, rb, nsl, slice);
-    double *part = nullptr;
-    HIPCHECK(hipMallocAsync((void **)&part, sizeof(double) * nsl * R, st));
-    rc = pair_rows_launch(*tab, ZA, ZB, w, R, M, part, s, st);
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(double) * nsl * R));
+    rc = pair_rows_launch(*tab, ZA, ZB, w, R, M, (double *)sc.p, s, st);
     if (rc) return rc;
-    HIPCHECK(hipFreeAsync(part, st));
+    HIPCHECK(sc.release());
     return BLML_OK;
 }
 
@@ -1512,8 +1553,9 @@ static int pair_multi_run(const blml_pairtab_t *tab, const blml_pairgrad_t *grad
     }
     long rb, nsl, slice;
     pair_rows_grid(R, M, rb, nsl, slice);
-    double *part = nullptr;
-    HIPCHECK(hipMallocAsync((void **)&part, sizeof(double) * nsl * (Mm + gd.nE) * R, st));
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(double) * nsl * (Mm + gd.nE) * R));
+    double *part = (double *)sc.p;
     const blml_pairtab_t t = *tab;
     const size_t es = t.f32 ? 4 : 8;
     const bool tl = t.nT * (long)es <= 32768;
@@ -1542,7 +1584,7 @@ static int pair_multi_run(const blml_pairtab_t *tab, const blml_pairgrad_t *grad
     HIPCHECK(hipGetLastError());
     pair_rows_multi_sum_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, Mm, gd.nE, s, ds);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipFreeAsync(part, st));
+    HIPCHECK(sc.release());
     return BLML_OK;
 }
 
@@ -1605,9 +1647,9 @@ int blml_quick_sweep(const blml_pairtab_t *tab, const int32_t *Z, int64_t B, con
     nsl = std::max(nsl, nsl_r);
     // one allocation: partials (nsl B), v (B), eold, dlt, estar, src (Bs each), t, block coordinates (Bs d int32)
     const size_t nd = (size_t)nsl * B + B + 4 * (size_t)Bs + 1;
-    char *buf = nullptr;
-    HIPCHECK(hipMallocAsync((void **)&buf, sizeof(double) * nd + sizeof(int32_t) * (size_t)Bs * d, st));
-    double *part = (double *)buf, *v = part + (size_t)nsl * B, *eold = v + B, *dlt = eold + Bs, *estar = dlt + Bs,
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(double) * nd + sizeof(int32_t) * (size_t)Bs * d));
+    double *part = (double *)sc.p, *v = part + (size_t)nsl * B, *eold = v + B, *dlt = eold + Bs, *estar = dlt + Bs,
            *src = estar + Bs, *tdev = src + Bs;
     int32_t *Zb = (int32_t *)(tdev + 1);
     for (long s0 = 0; s0 < B; s0 += Bs) {
@@ -1626,7 +1668,7 @@ int blml_quick_sweep(const blml_pairtab_t *tab, const int32_t *Z, int64_t B, con
             B, I, (int)m, v, sg, eold, dlt, estar, safeguard ? tdev : nullptr, e, Se);
         HIPCHECK(hipGetLastError());
     }
-    HIPCHECK(hipFreeAsync(buf, st));
+    HIPCHECK(sc.release());
     return BLML_OK;
 }
 
@@ -1639,9 +1681,9 @@ int blml_quick_exact(const blml_pairtab_t *tab, const int32_t *Z, int64_t B, con
     hipStream_t st = (hipStream_t)stream;
     long rb, nsl, slice;
     pair_rows_grid(B, B, rb, nsl, slice);
-    double *buf = nullptr;
-    HIPCHECK(hipMallocAsync((void **)&buf, sizeof(double) * ((size_t)nsl * B + 2 * (size_t)B), st));
-    double *part = buf, *src = part + (size_t)nsl * B, *v = src + B;
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(double) * ((size_t)nsl * B + 2 * (size_t)B)));
+    double *part = (double *)sc.p, *src = part + (size_t)nsl * B, *v = src + B;
     const unsigned g = (unsigned)((B + 255) / 256);
     quick_signed_kernel<<<g, 256, 0, st>>>(B, e, sg, src);
     HIPCHECK(hipGetLastError());
@@ -1649,7 +1691,7 @@ int blml_quick_exact(const blml_pairtab_t *tab, const int32_t *Z, int64_t B, con
     if (rc) return rc;
     quick_signed_kernel<<<g, 256, 0, st>>>(B, v, sg, Se);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipFreeAsync(buf, st));
+    HIPCHECK(sc.release());
     return BLML_OK;
 }
 
@@ -1659,13 +1701,14 @@ int blml_quick_stats(const double *e, const double *Se, const double *w, int64_t
     if (!e || !Se || !w || !out) return fail(BLML_EVALUE, "null argument");
     hipStream_t st = (hipStream_t)stream;
     const int np = (int)std::min<long>((B + BLML_QUICK_NT - 1) / BLML_QUICK_NT, 1024);
-    double *part = nullptr;
-    HIPCHECK(hipMallocAsync((void **)&part, sizeof(double) * 4 * np, st));
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(double) * 4 * np));
+    double *part = (double *)sc.p;
     quick_stats_kernel<<<np, BLML_QUICK_NT, 0, st>>>(B, e, Se, w, n, part);
     HIPCHECK(hipGetLastError());
     quick_stats_sum_kernel<<<1, 64, 0, st>>>(part, np, out);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipFreeAsync(part, st));
+    HIPCHECK(sc.release());
     return BLML_OK;
 }
 
@@ -1699,9 +1742,10 @@ int blml_net_conv_grad_w(const void *X, const void *dZ, float *dW, float *db, in
     if (!X || !dZ || !dW || !db) return fail(BLML_EVALUE, "null argument");
     hipStream_t st = (hipStream_t)stream;
     const int K = Cin * 9, len = Cout * K;
-    float *part = nullptr, *bpart = nullptr;
-    HIPCHECK(hipMallocAsync((void **)&part, sizeof(float) * (size_t)N * len, st));
-    HIPCHECK(hipMallocAsync((void **)&bpart, sizeof(float) * (size_t)N * Cout, st));
+    StreamScratch sc_w(st), sc_b(st);
+    HIPCHECK(sc_w.alloc(sizeof(float) * (size_t)N * len));
+    HIPCHECK(sc_b.alloc(sizeof(float) * (size_t)N * Cout));
+    float *part = (float *)sc_w.p, *bpart = (float *)sc_b.p;
     const int lds = ((Cin * BLML_NET_PATCH + 7) / 8) * 8 * 2 + 32 * BLML_NET_RT * BLML_NET_W * 2;
     net_conv_dw_kernel<<<N, BLML_NET_NT, lds, st>>>((const bf16_t *)X, (const bf16_t *)dZ, part, Cin, Cout, H);
     HIPCHECK(hipGetLastError());
@@ -1711,8 +1755,8 @@ int blml_net_conv_grad_w(const void *X, const void *dZ, float *dW, float *db, in
     HIPCHECK(hipGetLastError());
     net_sum_parts_kernel<<<NET_GRID(Cout), 256, 0, st>>>(bpart, N, Cout, db);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipFreeAsync(part, st));
-    HIPCHECK(hipFreeAsync(bpart, st));
+    HIPCHECK(sc_w.release());
+    HIPCHECK(sc_b.release());
     return BLML_OK;
 }
 

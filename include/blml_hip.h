@@ -128,9 +128,11 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
              void *e_out, void *se_out, double *density, blml_result_t *results, double *merit_hist,
              double *rho_hist, void *stream);
 
-/* certificate(): lambda_max(D^1/2 S_0 D^1/2) / n per fit by power iteration on the device with an FP64 Rayleigh
- * quotient (blml_binned.py:501-518).  e_hat: plan dtype (as written by blml_fit).  lam: nfits doubles (NaN for
- * a fit without a shared kernel).  iters: nfits int32 (iterations used).                                      */
+/* certificate(): lambda_max(D^1/2 S_0 D^1/2) / n per fit (blml_binned.py:501-518, which calls eigsh(k=1, tol=tol)) by
+ * power iteration on the device, FP64 transforms (an FP32 plan uses its FP64 spectra) and an FP64 Rayleigh
+ * quotient; a fit stops when its eigen-residual ||A x - lam x|| <= tol * lam for the unit iterate x (ARPACK's test
+ * for a Ritz pair) or after max_iter steps.  e_hat: plan dtype (as written by blml_fit).  lam: nfits doubles (NaN
+ * for a fit without a shared kernel).  iters: nfits int32, the steps used, negative when max_iter ended them.   */
 int blml_certificate(void *plan, const void *e_hat, const double *counts, int32_t max_iter, double tol,
                      double *lam, int32_t *iters, void *stream);
 

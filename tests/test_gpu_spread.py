@@ -201,7 +201,7 @@ def test_spread_certificate_matches_dense():
     params = dict(PARAMS, tol=1e-8)
     fs, batch = _one_fit_batch(N, _modes(mm, blml_amd), 1.0, False, np.float64, 1, params)
     r = batch.run().results()[0]
-    lam, its = batch.certificate(max_iter=100000, tol=1e-12)
+    lam, its = batch.certificate(max_iter=100000, tol=1e-9)
     batch.close()
     occ = N.ravel() > 0
     om = O.BLMLBinnedOracle()

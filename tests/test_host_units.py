@@ -209,3 +209,20 @@ def test_fit_input_errors_mirror_reference():
         spec(ok, modes=M.make_modes([[0.3]] * 9, ['sinc']))
     with pytest.raises(ValueError, match='dep'):
         M.make_modes([0.3], ['sinc'], dep=[0])
+
+
+def test_pair_spans_reject_ranks_outside_the_table():
+    """A ranked=True caller's rank column must hold ranks 0 .. mk - 1: a negative rank would be clamped into another
+    column's rows on the device, so the host check raises (the reference's stale-table RuntimeError,
+    blml_kernels.py:214-215) for ranks above and below the table."""
+    from blml_amd import local
+    Z = np.stack([np.array([0, 3, 7, 40]), np.arange(4)], 1)          # column 0: 4 values over a span of 40
+    tab = local.lattice_tables(['sinc', 'sinc'], [0.3, 0.3], None, [1.0, 1.0], Z)
+    assert int(tab['kinds'][0]) == 3 and int(tab['mk'][0]) == 4
+    A = local.rank_coords(Z, tab)
+    local._check_spans(A, A, tab)
+    for bad in (-1, 4):
+        B = A.copy()
+        B[0, 0] = bad
+        with pytest.raises(RuntimeError, match='rank outside'):
+            local._check_spans(A, B, tab)

@@ -141,6 +141,45 @@ def test_ball_tables_radial_rows():
         local.lattice_tables(['ball', 'ball'], [0.35, 0.3], None, [1.0, 1.0], Z[:, :2])
 
 
+def test_multi_column_groups_by_tag_in_order():
+    """Two periodic balls with their own tags get a torus table each (their own fc), and a ball listed before a
+    periodic ball is group 0 / unit 0 as in the reference's radial_groups and kernel_units (blml_trivial.py:251-261,
+    330-335); rows over either table equal the dense Gram of the units."""
+    from blml_amd import local
+    rng = np.random.default_rng(21)
+    Z = _bins(rng, (10, 8, 12, 6), 150)
+    kinds = ['pball:a', 'pball:a', 'pball:b', 'pball:b']
+    fc, per, h = [0.3, 0.3, 0.45, 0.45], [10.0, 8.0, 12.0, 6.0], [1.0] * 4
+    tab = local.lattice_tables(kinds, fc, per, h, Z)
+    assert list(tab['grp']) == [0, 0, 1, 1] and int(tab['U']) == 2 and list(tab['grp_unit']) == [0, 1]
+    units = [('torus', 'pball', [0.3], [10.0, 8.0], [0, 1]), ('torus', 'pball', [0.45], [12.0, 6.0], [2, 3])]
+    w = rng.standard_normal(len(Z))
+    G = OP.dense_pair_gram(units, np.asarray(h), Z, Z)
+    assert np.abs(OP.pair_rows(Z, Z, w, tab) - G @ w).max() < 1e-12 * max(1.0, np.abs(G).sum(1).max())
+    # a ball before a periodic ball, then a sinc column: units sinc (0), ball (1), pball (2); groups ball 0, pball 1
+    Zb = np.stack([rng.integers(0, 9, 140), rng.integers(0, 9, 140), rng.integers(0, 10, 140),
+                   rng.integers(0, 8, 140), rng.integers(0, 7, 140)], 1)
+    Zb = np.unique(Zb, axis=0)
+    kinds_b = ['ball:r', 'ball:r', 'pball:p', 'pball:p', 'sinc']
+    fc_b, per_b, h_b = [0.35, 0.35, 0.4, 0.4, 0.3], [None, None, 10.0, 8.0, None], [1.0] * 5
+    tb = local.lattice_tables(kinds_b, fc_b, per_b, h_b, Zb)
+    assert list(tb['grp']) == [0, 0, 1, 1, -1] and list(tb['grp_unit']) == [1, 2]
+    assert list(tb['kinds']) == [2, 2, 4, 4, 0] and int(tb['U']) == 3 and list(tb['dim_unit']) == [-1, -1, -1, -1, 0]
+    # every pair's ball distance on the radial grid (column 1 the same for every bin): the ball's row is exact there
+    ZA = Zb.copy()
+    ZA[:, 1] = 4
+    ZB = ZA.copy()
+    ZB[:, 0] = (ZA[:, 0] + 3) % 9
+    tb = local.lattice_tables(kinds_b, fc_b, per_b, h_b, np.vstack([ZA, ZB]))
+    ub = [('offset', 'ball', [0.35, 0.35], [0.0, 0.0], [0, 1]), ('torus', 'pball', [0.4], [10.0, 8.0], [2, 3]),
+          ('offset', 'sinc', [0.3], [0.0], [4])]
+    wb = rng.standard_normal(len(ZB))
+    Gb = OP.dense_pair_gram(ub, np.asarray(h_b), ZA, ZB)
+    assert np.abs(OP.pair_rows(ZA, ZB, wb, tb) - Gb @ wb).max() < 1e-12 * max(1.0, np.abs(Gb).sum(1).max())
+    with pytest.raises(ValueError, match='mixes kinds'):
+        local.lattice_tables(['ball:x', 'pball:x'], [0.3, 0.3], [None, 8.0], [1.0, 1.0], Z[:, :2])
+
+
 # ------------------------------------------------------------------------------------------------ derivative rows
 def _fd_rows(units, h, ZA, ZB, w, ui, how, eps=1e-5):
     """Central finite difference of the dense Gram's rows (the kernels' closed forms, oracle.kernel_eval) in one
