@@ -208,10 +208,8 @@ class FitBatch:
         signs = [fs.signs.ravel() for fs in self.fits if fs.signs is not None]
         self.signs = np.concatenate(signs) if signs else None
 
-    def build(self):
-        """Creates the plan (spectra on the device, workspace) and uploads the inputs."""
-        torch = _torch()
-        lib = _lib.load()
+    def params_struct(self):
+        """The batch's solver knobs as the C-ABI's blml_params_t."""
         P = self.params
         prm = _lib.Params()
         prm.tol = float(P['tol'])
@@ -225,7 +223,13 @@ class FitBatch:
         prm.lbfgs_m = int(P['lbfgs_m'])
         prm.hist_cap = self.hist_cap
         prm.spread = self.spread
-        self._prm = prm
+        return prm
+
+    def build(self):
+        """Creates the plan (spectra on the device, workspace) and uploads the inputs."""
+        torch = _torch()
+        lib = _lib.load()
+        prm = self._prm = self.params_struct()
         plan = ctypes.c_void_p()
         _lib.check(lib.blml_plan_create(self._Fits, len(self.fits), self._Facs, self._nfacs,
                                         self._fdata.ctypes.data, len(self._fdata), self._freqs.ctypes.data,

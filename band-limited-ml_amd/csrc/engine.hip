@@ -21,16 +21,10 @@
 #include "net.hpp"
 #include "pairs.hpp"
 #include "quick.hpp"
-#include "solver.hpp"
+#include "fit.hpp"
+#include "plan_layout.hpp"
 
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string &msg) {
-    g_err = msg;
-    return code;
-}
 
 #define HIPCHECK(x)                                                                              \
     do {                                                                                         \
@@ -60,38 +54,6 @@ struct StreamScratch {
 
 // ------------------------------------------------------------------------------------------------ kernels
 template <typename T>
-struct DevPlan {
-    const blml_fit_t *fits;
-    const blml_factor_t *facs;
-    const double *fdata;
-    const int *freqs;
-    T *spec;
-    const cpx<T> *tw;
-    int Lmax;
-    unsigned char *ws;     // per-fit workspaces, byte offsets fit.ws_off
-    int nfits;
-    blml_params_t params;
-    const SepPlan *sep;    // spread plans of separable modes: [0] for the mat-vec kernel, [1] for the fit kernel
-    const T *sep_tab;
-    const long *sep_off;
-};
-
-// A fit's workspace (bytes from ws_off): [X, Y: scratch_len complex values of the transform type]
-// [pool: n_slots * Vmax FP64 state vectors][FP32 plans: X64, Y64, the finishing pass's FP64 scratch].
-__host__ __device__ inline long align256(long v) { return (v + 255) / 256 * 256; }
-__host__ __device__ inline int n_slots(const blml_params_t &P, const blml_fit_t &f) {
-    const int cd = 13 + 2 * (P.anderson + 1);
-    const int lb = 9 + 2 * (P.lbfgs_m + 1);
-    const bool psd = f.complex_g || f.K > 1;
-    return (P.solver == 1 && psd) ? (cd > lb ? cd : lb) : cd;
-}
-__host__ __device__ inline long pool_bytes_off(const blml_fit_t &f, int esize) { return align256(4L * f.scratch_len * esize); }
-__host__ __device__ inline long vmax_of(const blml_fit_t &f) { return (long)f.K * f.B * (f.complex_g ? 2 : 1); }
-__host__ __device__ inline long fin_bytes_off(const blml_params_t &P, const blml_fit_t &f, int esize) {
-    return align256(pool_bytes_off(f, esize) + 8L * n_slots(P, f) * vmax_of(f));
-}
-
-template <typename T>
 __global__ void twiddle_kernel(cpx<T> *tw, int Lmax) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < Lmax) {
@@ -101,98 +63,11 @@ __global__ void twiddle_kernel(cpx<T> *tw, int Lmax) {
     }
 }
 
-// Spectrum of one factor.  LINE padded: Kf[q] = (t0 + 2 sum_{o=1}^{m-1} t_o cos(2 pi q o / L)) / L, the DFT of the
-// even, zero-padded row, in FP64 with exact integer phase reduction.  Circular factors: the given values.
-// GROUP padded: the forward FFT of the table over its power-of-two lattice (real, the table being even), times
-// prod 1/L.
 template <typename T>
 __global__ void __launch_bounds__(BLML_NT) spec_kernel(DevPlan<T> p, int nfacs, const int *fac_m, cpx<T> *tmp,
                                                        const long *tmp_off) {
-    const int fi = blockIdx.x;
-    if (fi >= nfacs) return;
-    const blml_factor_t fa = p.facs[fi];
-    T *spec = p.spec + fa.spec_off;
-    const double *data = p.fdata + fa.data_off;
-    if (fa.kind == BLML_FAC_LINE && !fa.circ[0]) {
-        const int m = fac_m[fi * BLML_MAXD], L = fa.tlen[0];
-        for (int q = threadIdx.x; q < L; q += BLML_NT) {
-            double acc = data[0];
-            long ph = 0;
-            for (int o = 1; o < m; ++o) {
-                ph += q;
-                if (ph >= L) ph -= L;
-                acc += 2.0 * data[o] * cospi(2.0 * (double)ph / (double)L);
-            }
-            spec[q] = (T)(acc / (double)L);
-        }
-        return;
-    }
-    long total = 1;
-    for (int a = 0; a < fa.naxes; ++a) total *= fa.tlen[a];
-    bool allcirc = true;
-    for (int a = 0; a < fa.naxes; ++a) allcirc = allcirc && fa.circ[a];
-    if (allcirc) {
-        for (long i = threadIdx.x; i < total; i += BLML_NT) spec[i] = (T)data[i];
-        return;
-    }
-    // padded group: FFT of the table along each of its axes (the table's own lattice: ext = tlen)
-    cpx<T> *X = tmp + tmp_off[fi], *Y = X + total;
-    for (long i = threadIdx.x; i < total; i += BLML_NT) X[i] = {(T)data[i], (T)0};
-    __syncthreads();
-    int ext[BLML_MAXD];
-    for (int a = 0; a < fa.naxes; ++a) ext[a] = fa.tlen[a];
-    cpx<T> *cur = X, *oth = Y;
-    for (int a = fa.naxes - 1; a >= 0; --a) {
-        Pass<T> ps;
-        ps.op = OP_FFT_FULL;
-        ps.L = fa.tlen[a];
-        ps.m = ps.L;
-        ps.nF = 0;
-        ps.F = nullptr;
-        ps.spec = nullptr;
-        ps.n_in = ps.n_out = ps.L;
-        ps.pairs = false;
-        ps.rank = 0;
-        ps.nranks = 1;
-        line_geom<T>(ext, fa.naxes, a, ps.outer, ps.inner);
-        run_pass<T>(ps, cur, oth, p.tw, p.Lmax);
-        cpx<T> *t = cur; cur = oth; oth = t;
-    }
-    double inv = 1.0;
-    for (int a = 0; a < fa.naxes; ++a) inv /= (double)fa.tlen[a];
-    for (long i = threadIdx.x; i < total; i += BLML_NT) spec[i] = (T)((double)cur[i].x * inv);
-}
-
-template <typename T>
-__device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, MatvecCtx<T> &mv, const Team &tm,
-                                const SepPlan *sep = nullptr) {
-    mv.tm = tm;
-    mv.sep = sep;
-    mv.sep_last_sync = true;
-    mv.sep_tab = p.sep_tab;
-    mv.sep_off = p.sep_off;
-    mv.d = f.d;
-    for (int j = 0; j < BLML_MAXD; ++j) mv.shape[j] = j < f.d ? f.shape[j] : 1;
-    mv.facs = p.facs;
-    mv.spec = p.spec;
-    mv.freqs = p.freqs;
-    mv.tw = p.tw;
-    mv.Lmax = p.Lmax;
-    mv.X = (cpx<T> *)(p.ws + f.ws_off);
-    mv.Y = mv.X + f.scratch_len;
-}
-
-template <typename T>
-__device__ void run_matvec(const DevPlan<T> &p, int fi, const Team &tm, const T *E, T *U, const SepPlan *sep) {
-    const blml_fit_t f = p.fits[fi];
-    MatvecCtx<T> mv;
-    make_matvec_ctx<T>(p, f, mv, tm, sep);
-    mv.sep_last_sync = f.K > 1;      // between modes only: the kernel's end orders the last one
-    const int nc = f.complex_g ? 2 : 1;
-    for (int k = 0; k < f.K; ++k) {
-        const long off = f.out_off + (long)k * f.B * nc;
-        matvec_mode<T, T, T>(mv, f.fac_begin[k], f.fac_count[k], E + off, U + off, f.B, nc);
-    }
+    if ((int)blockIdx.x >= nfacs) return;
+    spec_factor<T>(p, blockIdx.x, fac_m, tmp, tmp_off);
 }
 
 // one workgroup per fit
@@ -268,180 +143,6 @@ __global__ void __launch_bounds__(BLML_NT) matvec_pipe_kernel(DevPlan<T> p, unsi
     }
 }
 
-// One whole fit (start, solve, outputs), run by a team.
-template <typename T>
-__device__ void run_fit(const DevPlan<T> &p, int fi, const Team &team, const double *counts, const double *draws,
-                        const double *init, const double *signs, T *e_out, T *se_out, double *density,
-                        blml_result_t *results, double *merit_hist, double *rho_hist, double tick_hz) {
-    const blml_fit_t f = p.fits[fi];
-    FitSolver<T> S;
-    S.tm = team;
-    make_matvec_ctx<T>(p, f, S.mv, team, team.nranks > 1 && p.sep ? p.sep + 1 : nullptr);
-    S.f = &p.fits[fi];
-    S.P = &p.params;
-    S.B = f.B;
-    S.n = f.n;
-    for (int k = 0; k < BLML_MAXK; ++k) S.s[k] = f.s[k];
-    S.equal_s = f.equal_s != 0;
-    S.w = counts + f.counts_off;
-    S.sg = f.has_signs ? signs + f.signs_off : nullptr;
-    const int ncf = f.complex_g ? 2 : 1;
-    S.Vmax = (long)f.K * f.B * ncf;
-    S.pool = (double *)(p.ws + f.ws_off + pool_bytes_off(f, (int)sizeof(T)));
-    S.inc = sizeof(T) == 4;
-    S.gstep = 0.0;
-    S.t_mv = S.t_dir = S.t_ls = 0;
-    S.n_mv = 0;
-    const long long tfit0 = wall_clock64();
-    long long tstart = 0;
-    S.merit_hist = merit_hist ? merit_hist + f.hist_off : nullptr;
-    S.rho_hist = rho_hist ? rho_hist + f.hist_off : nullptr;
-    S.hist_cap = p.params.hist_cap;
-    S.nhist_m = S.nhist_r = 0;
-    S.ring_n = 0;
-    const bool psd_full = f.complex_g || f.K > 1;
-    const double pn = p.params.phase_noise;
-    const double *dr = draws + f.draws_off;
-    int start_sweeps = 0;
-    double *E = S.vec(0);
-    if (f.init_K > 0) {
-        // warm start (blml_binned.py:403-408): given modes, zero off the occupied bins, noise in the others
-        const double *E0 = init + f.init_off;
-        const int kk = f.init_K < f.K ? f.init_K : f.K;
-        for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
-            const bool o = S.w[b] > 0;
-            for (int k = 0; k < f.K; ++k) {
-                const long i = ((long)k * f.B + b) * ncf;
-                double re = 0.0, im = 0.0;
-                if (k < kk && o) {
-                    re = E0[((long)k * f.B + b) * 2];
-                    im = f.complex_g ? E0[((long)k * f.B + b) * 2 + 1] : 0.0;
-                }
-                E[i] = re;
-                if (ncf == 2) E[i + 1] = im;
-            }
-        }
-        S.tm.sync();
-        long dp = 0;
-        for (int k = kk; k < f.K; ++k) {
-            for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
-                const double a0 = ncf == 2 ? sqrt((double)E[2 * b] * E[2 * b] + (double)E[2 * b + 1] * E[2 * b + 1])
-                                           : fabs((double)E[b]);
-                const long i = ((long)k * f.B + b) * ncf;
-                if (ncf == 2) {
-                    E[i] = (pn * a0 * (dr[dp + b] / sqrt(2.0)));
-                    E[i + 1] = (pn * a0 * (dr[dp + f.B + b] / sqrt(2.0)));
-                } else {
-                    E[i] = (pn * a0 * dr[dp + b]);
-                }
-            }
-            dp += (long)f.B * ncf;
-        }
-        S.tm.sync();
-    } else {
-        // cold start: e0 = sqrt(c / s0) on occupied bins (times the signs); with init_K < 0 the caller's real
-        // rank-1 solution stands in for e0 and, in a psd fit, for the start solve below
-        const double *er0 = f.init_K < 0 ? init + f.init_off : nullptr;
-        for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
-            const double sgb = S.sg ? S.sg[b] : 1.0;
-            E[b] = S.w[b] > 0 ? (er0 ? er0[2 * b] : sqrt(f.n * S.w[b] / f.s[0]) * sgb) : 0.0;
-        }
-        S.tm.sync();
-        if (psd_full) {
-            // the real rank-1 solution under mode 0 (blml_binned.py:421-425)
-            S.K = 1; S.nc = 1; S.psd = false; S.rec = false;
-            if (!er0) {
-                const FitOut so = S.solve_cd();
-                start_sweeps = so.n_sweeps;
-            }
-            tstart = wall_clock64() - tfit0;
-            // phases and the other modes (blml_binned.py:429-436), assembled in slot 2 then moved to slot 0
-            double *E2 = S.vec(2);
-            const double *er = S.vec(0);
-            long dp = 0;
-            for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
-                if (f.complex_g) {
-                    double sn, cs;
-                    sincos(pn * dr[b], &sn, &cs);
-                    const double r = er[b];
-                    E2[2 * b] = (r * cs);
-                    E2[2 * b + 1] = (r * sn);
-                } else {
-                    E2[b] = er[b];
-                }
-            }
-            if (f.complex_g) dp = f.B;
-            S.tm.sync();
-            for (int k = 1; k < f.K; ++k) {
-                for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
-                    const double a0 = ncf == 2
-                        ? sqrt((double)E2[2 * b] * E2[2 * b] + (double)E2[2 * b + 1] * E2[2 * b + 1])
-                        : fabs((double)E2[b]);
-                    const long i = ((long)k * f.B + b) * ncf;
-                    if (ncf == 2) {
-                        E2[i] = (pn * a0 * (dr[dp + b] / sqrt(2.0)));
-                        E2[i + 1] = (pn * a0 * (dr[dp + f.B + b] / sqrt(2.0)));
-                    } else {
-                        E2[i] = (pn * a0 * dr[dp + b]);
-                    }
-                }
-                dp += (long)f.B * ncf;
-                S.tm.sync();
-            }
-            S.K = f.K; S.nc = ncf; S.psd = true;
-            S.copy(S.vec(0), E2);
-        }
-    }
-    S.K = f.K;
-    S.nc = ncf;
-    S.psd = psd_full;
-    S.rec = true;
-    S.nhist_m = S.nhist_r = 0;
-    FitOut o;
-    if (p.params.solver == 1 && psd_full)
-        o = (S.inc && p.params.lbfgs_m <= BLML_CMAX) ? S.solve_lbfgs_compact() : S.solve_lbfgs();
-    else
-        o = S.solve_cd();
-    // outputs (blml_binned.py:438-445)
-    const double *Ef = S.vec(0), *Uf = S.vec(1);
-    const long V = (long)f.K * f.B * ncf;
-    for (long i = S.tm.first(); i < V; i += S.tm.stride()) {
-        e_out[f.out_off + i] = (T)Ef[i];
-        se_out[f.out_off + i] = (T)Uf[i];
-    }
-    double ll = 0.0;
-    const double n2 = f.n * f.n;
-    for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
-        const double dens = S.Fb(Uf, b) / n2;
-        density[f.dens_off + b] = dens;
-        if (S.w[b] > 0) ll += S.w[b] * log(dens);
-    }
-    ll = S.tm.sum1(ll, g_sl.red);
-    const unsigned timed_out = S.tm.nranks > 1 ? __hip_atomic_load(&S.tm.bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-    if (S.tm.lead()) {
-        blml_result_t r;
-        r.loglik = ll / f.n;
-        r.residual = o.residual;
-        r.n_sweeps = o.n_sweeps;
-        r.stop_reason = o.stop;
-        r.converged = o.converged;
-        r.n_anderson = o.n_and;
-        r.n_damped = o.n_damp;
-        r.n_gradient = o.n_grad;
-        r.start_sweeps = start_sweeps;
-        r.n_hist = S.nhist_m;
-        r.n_matvec = S.n_mv;
-        r.reserved = (int)timed_out;   // a grid barrier timed out: the results are void
-        r.seconds[0] = (double)(wall_clock64() - tfit0) / tick_hz;
-        r.seconds[1] = (double)tstart / tick_hz;
-        r.seconds[2] = (double)S.t_mv / tick_hz;
-        r.seconds[3] = (double)S.t_dir / tick_hz;
-        r.seconds[4] = (double)S.t_ls / tick_hz;
-        results[fi] = r;
-    }
-    S.tm.sync();
-}
-
 // The batched path: a persistent grid whose workgroups take fits from an atomic queue, longest first; one
 // workgroup runs one whole fit.
 template <typename T>
@@ -477,91 +178,6 @@ __global__ void __launch_bounds__(BLML_NT) fit_grid_kernel(DevPlan<T> p, unsigne
                se_out, density, results, merit_hist, rho_hist, tick_hz);
 }
 
-// certificate(): lambda_max(D^1/2 S_0 D^1/2) / n, D_b = |e_b|^2 / w_b on occupied bins, by power iteration with FP64
-// transforms and an FP64 Rayleigh quotient.  It stops when the eigen-residual of the unit iterate x and its quotient
-// lam is small, ||A x - lam x|| <= tol |lam| (ARPACK's convergence test for a Ritz pair, the test eigsh applies at
-// blml_binned.py:510-518; the quotient's error is then at most tol^2 lam^2 / gap), or after max_iter steps.  Te: the
-// plan's coefficient type; an FP32 plan runs through its FP64 view, with the finishing pass's FP64 scratch (xoff
-// bytes into the fit's workspace).  iters: the steps taken, negated when max_iter ended the iteration.
-template <typename Te>
-__device__ void run_cert(const DevPlan<double> &p, int fi, Team tm, const Te *e_hat, const double *counts,
-                         int max_iter, double tol, double *lam_out, int *iters) {
-    __shared__ double red[BLML_NWAVE * BLML_NRED];
-    const blml_fit_t f = p.fits[fi];
-    if (!f.shared) {
-        if (tm.lead()) { lam_out[fi] = NAN; iters[fi] = 0; }
-        return;
-    }
-    MatvecCtx<double> mv;
-    make_matvec_ctx<double>(p, f, mv, tm);
-    if (sizeof(Te) == 4) {
-        mv.X = (cpx<double> *)(p.ws + f.ws_off + fin_bytes_off(p.params, f, 4));
-        mv.Y = mv.X + f.scratch_len;
-    }
-    const int nc = f.complex_g ? 2 : 1;
-    const int B = f.B;
-    const long b0 = tm.first(), bs = tm.stride();
-    double *dh = (double *)(p.ws + f.ws_off + pool_bytes_off(f, (int)sizeof(Te)));   // B doubles
-    double *x = dh + B, *y = x + B;
-    const double *w = counts + f.counts_off;
-    const Te *E = e_hat + f.out_off;
-    for (long b = b0; b < B; b += bs) {
-        double D = 0.0;
-        if (w[b] > 0) {
-            for (int k = 0; k < f.K; ++k) {
-                const long i = ((long)k * B + b) * nc;
-                D += (double)E[i] * E[i] + (nc == 2 ? (double)E[i + 1] * E[i + 1] : 0.0);
-            }
-            D /= w[b];
-        }
-        dh[b] = sqrt(D);
-    }
-    tm.sync();
-    double nrm = 0.0;
-    for (long b = b0; b < B; b += bs) nrm += dh[b] * dh[b];
-    nrm = sqrt(tm.sum1(nrm, red));
-    if (!(nrm > 0)) {
-        if (tm.lead()) { lam_out[fi] = 0.0; iters[fi] = 0; }
-        return;
-    }
-    for (long b = b0; b < B; b += bs) x[b] = dh[b] / nrm;
-    tm.sync();
-    double lam = 0.0;
-    int it = 0;
-    bool conv = false;
-    for (it = 1; it <= max_iter; ++it) {
-        for (long b = b0; b < B; b += bs) y[b] = dh[b] * x[b];
-        tm.sync();
-        matvec_mode<double, double, double>(mv, f.fac_begin[0], f.fac_count[0], y, y, B, 1);
-        double a[3] = {0.0, 0.0, 0.0};
-        for (long b = b0; b < B; b += bs) {
-            const double yb = dh[b] * y[b];
-            a[0] += x[b] * yb;
-            a[1] += yb * yb;
-            a[2] += x[b] * x[b];
-        }
-        tm.sum<3>(a, red);
-        lam = a[0] / a[2];
-        const double ny = sqrt(a[1]);
-        if (!(ny > 0)) break;
-        // the residual of (x, lam), summed in the pass that forms the next iterate A x / ||A x|| (each thread reads
-        // and writes its own bins only)
-        double r2 = 0.0;
-        for (long b = b0; b < B; b += bs) {
-            const double yb = dh[b] * y[b];
-            const double v = yb - lam * x[b];
-            r2 += v * v;
-            x[b] = yb / ny;
-        }
-        const double r = sqrt(tm.sum1(r2, red) / a[2]);
-        if (r <= tol * fabs(lam)) { conv = true; break; }
-    }
-    if (tm.lead()) {
-        lam_out[fi] = lam / f.n;
-        iters[fi] = conv ? it : -(it > max_iter ? max_iter : it);
-    }
-}
-
 template <typename Te>
 __global__ void __launch_bounds__(BLML_NT) cert_kernel(DevPlan<double> p, const Te *e_hat, const double *counts,
                                                        int max_iter, double tol, double *lam_out, int *iters) {
@@ -577,53 +193,6 @@ __global__ void __launch_bounds__(BLML_NT) cert_grid_kernel(DevPlan<double> p, u
                  iters);
 }
 
-// FP32 plans only: Se, density and loglik of every fit re-evaluated with an FP64 mat-vec of the FP32-converged
-// coefficients (the iterations run in FP32; the outputs' likelihood sum is FP64 end to end).  p carries the FP64
-// spectra and twiddles; its workspace is the FP32 plan's, reinterpreted (sized for it by layout()).
-__device__ void run_finish64(const DevPlan<double> &p, int fi, Team tm, float *se_out, const double *counts,
-                             double *density, blml_result_t *results) {
-    __shared__ double red[BLML_NWAVE * BLML_NRED];
-    const blml_fit_t f = p.fits[fi];
-    MatvecCtx<double> mv;
-    mv.tm = tm;
-    mv.sep = nullptr;
-    mv.sep_last_sync = true;
-    mv.d = f.d;
-    for (int j = 0; j < BLML_MAXD; ++j) mv.shape[j] = j < f.d ? f.shape[j] : 1;
-    mv.facs = p.facs;
-    mv.spec = p.spec;
-    mv.freqs = p.freqs;
-    mv.tw = p.tw;
-    mv.Lmax = p.Lmax;
-    mv.X = (cpx<double> *)(p.ws + f.ws_off + fin_bytes_off(p.params, f, 4));
-    mv.Y = mv.X + f.scratch_len;
-    const int nc = f.complex_g ? 2 : 1;
-    const long V = (long)f.K * f.B * nc;
-    // the fit's FP64 state: E in pool slot 0; U64 overwrites slot 1
-    const double *E64 = (const double *)(p.ws + f.ws_off + pool_bytes_off(f, 4));
-    double *U64 = (double *)E64 + vmax_of(f);
-    for (int k = 0; k < f.K; ++k) {
-        const long off = (long)k * f.B * nc;
-        matvec_mode<double, double, double>(mv, f.fac_begin[k], f.fac_count[k], E64 + off, U64 + off, f.B, nc);
-    }
-    for (long i = tm.first(); i < V; i += tm.stride()) se_out[f.out_off + i] = (float)U64[i];
-    const double *w = counts + f.counts_off;
-    const double n2 = f.n * f.n;
-    double ll = 0.0;
-    for (long b = tm.first(); b < f.B; b += tm.stride()) {
-        double F = 0.0;
-        for (int k = 0; k < f.K; ++k) {
-            const long i = ((long)k * f.B + b) * nc;
-            F += U64[i] * U64[i] + (nc == 2 ? U64[i + 1] * U64[i + 1] : 0.0);
-        }
-        const double dens = F / n2;
-        density[f.dens_off + b] = dens;
-        if (w[b] > 0) ll += w[b] * log(dens);
-    }
-    ll = tm.sum1(ll, red);
-    if (tm.lead()) results[fi].loglik = ll / f.n;
-}
-
 __global__ void __launch_bounds__(BLML_NT) finish64_kernel(DevPlan<double> p, float *se_out,
                                                           const double *counts, double *density,
                                                           blml_result_t *results) {
@@ -637,357 +206,6 @@ __global__ void __launch_bounds__(BLML_NT) finish64_grid_kernel(DevPlan<double> 
     run_finish64(p, 0, Team{(int)blockIdx.x, (int)gridDim.x, bar, part, 0}, se_out, counts, density, results);
 }
 
-// ------------------------------------------------------------------------------------------------ host plan
-struct Plan {
-    int dtype;
-    int nfits, nfacs;
-    blml_params_t params;
-    std::vector<blml_fit_t> hfits;
-    std::vector<blml_factor_t> hfacs;
-    blml_fit_t *d_fits = nullptr;
-    blml_factor_t *d_facs = nullptr;
-    double *d_fdata = nullptr;
-    int *d_freqs = nullptr;
-    void *d_spec = nullptr;
-    void *d_tw = nullptr;
-    void *d_spec64 = nullptr;   // FP32 plans: FP64 spectra and twiddles for the finishing pass
-    void *d_tw64 = nullptr;
-    int lds64_bytes = 0;
-    double tick_hz = 1e8;      // wall_clock64() rate
-    void *d_ws = nullptr;
-    int *d_order = nullptr;
-    int *d_queue = nullptr;
-    int Lmax = 2;
-    int lds_bytes = 0;
-    int grid = 1;
-    bool spread = false;       // one fit over every CU: cooperative launches of coop workgroups
-    int coop = 0;
-    unsigned *d_bar = nullptr;  // grid barrier words
-    double *d_part = nullptr;   // team partial sums (2 * coop * BLML_NRED)
-    bool sep = false;           // spread plan of separable modes: tile passes with dense line operators
-    SepPlan *d_sep = nullptr;   // [0] mat-vec kernel, [1] fit kernel
-    bool pipe_mv = false;       // the stand-alone mat-vec takes matvec_pipe_kernel (every pass pipelined)
-    void *d_sep_tab = nullptr;
-    long *d_sep_off = nullptr;
-    int lds_mv = 0, lds_fit = 0;   // dynamic LDS of the spread mat-vec and fit launches
-    int lds_pipe = 0;              // dynamic LDS of matvec_pipe_kernel (largest tile, then every axis's table)
-    int64_t out_elems = 0, dens_elems = 0;
-    int64_t ws_elems = 0;
-    size_t esize = 4;
-};
-
-int ceil_pow2(int v) {
-    int p = 1;
-    while (p < v) p <<= 1;
-    return p;
-}
-
-template <typename T>
-DevPlan<T> dev_plan(const Plan &pl) {
-    DevPlan<T> p;
-    p.fits = pl.d_fits;
-    p.facs = pl.d_facs;
-    p.fdata = pl.d_fdata;
-    p.freqs = pl.d_freqs;
-    p.spec = (T *)pl.d_spec;
-    p.tw = (const cpx<T> *)pl.d_tw;
-    p.Lmax = pl.Lmax;
-    p.ws = (unsigned char *)pl.d_ws;
-    p.nfits = pl.nfits;
-    p.params = pl.params;
-    p.sep = pl.sep ? pl.d_sep : nullptr;
-    p.sep_tab = (const T *)pl.d_sep_tab;
-    p.sep_off = pl.d_sep_off;
-    return p;
-}
-
-// the FP64 view of an FP32 plan used by the finishing pass
-DevPlan<double> dev_plan64(const Plan &pl) {
-    DevPlan<double> p;
-    p.fits = pl.d_fits;
-    p.facs = pl.d_facs;
-    p.fdata = pl.d_fdata;
-    p.freqs = pl.d_freqs;
-    p.spec = (double *)pl.d_spec64;
-    p.tw = (const cpx<double> *)pl.d_tw64;
-    p.Lmax = pl.Lmax;
-    p.ws = (unsigned char *)pl.d_ws;
-    p.nfits = pl.nfits;
-    p.params = pl.params;
-    p.sep = nullptr;
-    p.sep_tab = nullptr;
-    p.sep_off = nullptr;
-    return p;
-}
-
-// validates the tables and fills in spec_off, ws_off, ws_len, scratch_len; returns the LDS each launch needs
-int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs, int64_t n_freqs,
-           std::vector<int> &fac_m, std::vector<long> &tmp_off, long &tmp_total, int64_t &spec_total) {
-    const blml_params_t &P = pl.params;
-    if (!(P.anderson >= 0 && P.anderson <= BLML_MAXAND))
-        return fail(BLML_EVALUE, "anderson: 0.." + std::to_string(BLML_MAXAND) + " on this engine");
-    if (!(P.lbfgs_m >= 1 && P.lbfgs_m <= BLML_MAXHIST))
-        return fail(BLML_EVALUE, "lbfgs_m: 1.." + std::to_string(BLML_MAXHIST) + " on this engine");
-    if (!(P.ll_window >= 1 && P.ll_window < BLML_RING))
-        return fail(BLML_EVALUE, "ll_window: 1.." + std::to_string(BLML_RING - 1) + " on this engine");
-    if (P.hist_cap < 0) return fail(BLML_EVALUE, "hist_cap must be >= 0");
-    fac_m.assign((size_t)pl.nfacs * BLML_MAXD, 0);
-    std::vector<int> fac_owner(pl.nfacs, -1);
-    int lds_elems = 0;
-    int Lmax = 2;
-    spec_total = 0;
-    // factors: sizes and spectra offsets (owner fit gives the axis lengths)
-    for (int fi = 0; fi < pl.nfits; ++fi) {
-        const blml_fit_t &f = pl.hfits[fi];
-        if (f.d < 1 || f.d > BLML_MAXD) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": 1 <= d <= 5");
-        if (f.K < 1 || f.K > BLML_MAXK) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": 1 <= K <= 8");
-        if (f.init_K < -1) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": init_K >= -1");
-        long B = 1;
-        for (int j = 0; j < f.d; ++j) {
-            if (f.shape[j] < 1) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": empty lattice axis");
-            B *= f.shape[j];
-        }
-        if (B != f.B || B > (1L << 30)) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": B mismatch");
-        for (int k = 0; k < f.K; ++k) {
-            if (f.fac_begin[k] < 0 || f.fac_count[k] < 0 || f.fac_begin[k] + f.fac_count[k] > pl.nfacs)
-                return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": factor range");
-            for (int q = 0; q < f.fac_count[k]; ++q) {
-                const int id = f.fac_begin[k] + q;
-                if (fac_owner[id] >= 0 && fac_owner[id] != fi)
-                    return fail(BLML_EVALUE, "factor shared between fits");
-                fac_owner[id] = fi;
-            }
-        }
-    }
-    tmp_off.assign(pl.nfacs, 0);
-    tmp_total = 0;
-    for (int id = 0; id < pl.nfacs; ++id) {
-        blml_factor_t &fa = pl.hfacs[id];
-        const int fi = fac_owner[id];
-        if (fi < 0) return fail(BLML_EVALUE, "factor " + std::to_string(id) + " belongs to no fit");
-        const blml_fit_t &f = pl.hfits[fi];
-        if (fa.naxes < 1 || fa.naxes > f.d) return fail(BLML_EVALUE, "factor axes");
-        if (fa.kind == BLML_FAC_LINE && fa.naxes != 1) return fail(BLML_EVALUE, "LINE factor with several axes");
-        long total = 1;
-        int ncirc = 0;
-        for (int a = 0; a < fa.naxes; ++a) {
-            const int ax = fa.axes[a];
-            if (ax < 0 || ax >= f.d || (a > 0 && ax <= fa.axes[a - 1])) return fail(BLML_EVALUE, "factor axes");
-            const int m = f.shape[ax];
-            fac_m[(size_t)id * BLML_MAXD + a] = m;
-            const int tl = fa.tlen[a];
-            if (fa.circ[a]) {
-                ++ncirc;
-                if (tl < 1 || tl > m) return fail(BLML_EVALUE, "circular factor: 1 <= frequencies <= m");
-                if (fa.freq_off[a] < 0 || fa.freq_off[a] + tl > n_freqs) return fail(BLML_EVALUE, "freq range");
-                for (int q = 0; q < tl; ++q) {
-                    const int v = freqs[fa.freq_off[a] + q];
-                    if (v < 0 || v >= m) return fail(BLML_EVALUE, "frequency outside [0, m)");
-                }
-            } else {
-                if (tl < 2 * m - 1 || tl != ceil_pow2(tl) || tl < 2)
-                    return fail(BLML_EVALUE, "padded factor: power-of-two length >= 2m - 1");
-                if (tl > 8192)
-                    return fail(BLML_EVALUE, "padded lattice axis longer than 4096 bins: not supported by this engine");
-                Lmax = std::max(Lmax, tl);
-            }
-            total *= tl;
-        }
-        if (ncirc != 0 && ncirc != fa.naxes)
-            return fail(BLML_EVALUE, "a multi-column unit with both circular and padded axes is not supported");
-        const bool line_pad = fa.kind == BLML_FAC_LINE && !fa.circ[0];
-        const long need = line_pad ? fac_m[(size_t)id * BLML_MAXD] : total;
-        if (fa.data_off < 0 || fa.data_off + need > n_fdata) return fail(BLML_EVALUE, "factor data range");
-        fa.spec_off = spec_total;
-        spec_total += total;
-        if (fa.kind == BLML_FAC_GROUP && ncirc == 0) {
-            tmp_off[id] = tmp_total;
-            tmp_total += 2 * total;
-            // the spectrum's forward FFTs over the table's own lattice (spec_kernel)
-            for (int a = 0; a < fa.naxes; ++a)
-                lds_elems = std::max(lds_elems, pass_lds_elems(OP_FFT_FULL, fa.tlen[a], fa.tlen[a], 0,
-                                                               total / fa.tlen[a]));
-        }
-    }
-    // fits: scratch (largest intermediate of any factor chain) and workspace
-    int64_t ws = 0;
-    for (int fi = 0; fi < pl.nfits; ++fi) {
-        blml_fit_t &f = pl.hfits[fi];
-        long scratch = f.B;
-        // replay every pass of the fit's mat-vecs: scratch = the largest intermediate, LDS = the largest round
-        for (int k = 0; k < f.K; ++k) {
-            for (int q = 0; q < f.fac_count[k]; ++q) {
-                const blml_factor_t &fa = pl.hfacs[f.fac_begin[k] + q];
-                if (fa.kind == BLML_FAC_LINE) {
-                    const int m = f.shape[fa.axes[0]];
-                    const long nl = f.B / m;
-                    lds_elems = std::max(lds_elems, fa.circ[0] ? pass_lds_elems(OP_CONV_CIRC, m, 0, fa.tlen[0], nl)
-                                                               : pass_lds_elems(OP_CONV_PAD, m, fa.tlen[0], 0, nl));
-                    continue;
-                }
-                long ext[BLML_MAXD];
-                for (int j = 0; j < f.d; ++j) ext[j] = f.shape[j];
-                auto lines_of = [&](int axis) {
-                    long t = 1;
-                    for (int j = 0; j < f.d; ++j)
-                        if (j != axis) t *= ext[j];
-                    return t;
-                };
-                for (int a = fa.naxes - 1; a >= 0; --a) {
-                    const int ax = fa.axes[a], m = f.shape[ax];
-                    lds_elems = std::max(lds_elems, fa.circ[a]
-                        ? pass_lds_elems(OP_DFT_FWD, m, 0, fa.tlen[a], lines_of(ax))
-                        : pass_lds_elems(OP_FFT_FWD, m, fa.tlen[a], 0, lines_of(ax)));
-                    ext[ax] = fa.tlen[a];
-                    long t = 1;
-                    for (int j = 0; j < f.d; ++j) t *= ext[j];
-                    scratch = std::max(scratch, t);
-                }
-                for (int a = 0; a < fa.naxes; ++a) {
-                    const int ax = fa.axes[a], m = f.shape[ax];
-                    lds_elems = std::max(lds_elems, fa.circ[a]
-                        ? pass_lds_elems(OP_DFT_INV, m, 0, fa.tlen[a], lines_of(ax))
-                        : pass_lds_elems(OP_FFT_INV, m, fa.tlen[a], 0, lines_of(ax)));
-                    ext[ax] = m;
-                }
-            }
-        }
-        if (scratch >= (1L << 31) || (long)f.B * 2 >= (1L << 31))
-            return fail(BLML_EVALUE, "a fit's transform intermediate exceeds 2^31 values");
-        f.scratch_len = scratch;
-        const long Vmax = vmax_of(f);
-        // bytes: transform scratch, FP64 state pool (also the certificate's 3 B doubles: n_slots >= 3), and for
-        // FP32 plans the finishing pass's FP64 scratch
-        long len = fin_bytes_off(P, f, (int)pl.esize);
-        if (pl.esize == 4) len += 32L * scratch;
-        len = align256(len);
-        f.ws_off = ws;
-        f.ws_len = len;
-        ws += len;
-        pl.out_elems = std::max<int64_t>(pl.out_elems, f.out_off + Vmax);
-        pl.dens_elems = std::max<int64_t>(pl.dens_elems, f.dens_off + f.B);
-    }
-    pl.ws_elems = ws;
-    pl.Lmax = Lmax;
-    lds_elems = std::max(lds_elems, 16);
-    pl.lds_bytes = lds_elems * (int)(2 * pl.esize);
-    pl.lds64_bytes = lds_elems * 16;
-    return BLML_OK;
-}
-
-constexpr long kSpreadMin = 1L << 18;
-
-// The dense symmetric operator of one LINE factor on m points (FP64), zero-padded to the MFMA block multiple
-// P x P of separable.hpp.
-void sep_tables(const blml_factor_t &fa, int m, int blk, const double *fdata, const int32_t *freqs,
-                std::vector<double> &out) {
-    std::vector<double> M((size_t)m * m);
-    if (!fa.circ[0]) {
-        const double *row = fdata + fa.data_off;
-        for (int i = 0; i < m; ++i)
-            for (int j = 0; j < m; ++j) M[(size_t)i * m + j] = row[i > j ? i - j : j - i];
-    } else {
-        std::vector<double> c(m, 0.0);
-        const double *sp = fdata + fa.data_off;
-        const int32_t *F = freqs + fa.freq_off[0];
-        for (int k = 0; k <= m / 2; ++k) {
-            double acc = 0.0;
-            for (int q = 0; q < fa.tlen[0]; ++q) {
-                const long ph = ((long)F[q] * k) % m;
-                acc += sp[q] * std::cos(2.0 * M_PI * (double)ph / (double)m);
-            }
-            c[k] = acc;
-            c[(m - k) % m] = acc;
-        }
-        for (int i = 0; i < m; ++i)
-            for (int j = 0; j < m; ++j) M[(size_t)i * m + j] = c[((i - j) % m + m) % m];
-    }
-    const int P = sep_table_dim(m, blk);
-    for (int i = 0; i < P; ++i)
-        for (int j = 0; j < P; ++j) out.push_back(i < m && j < m ? M[(size_t)i * m + j] : 0.0);
-}
-
-// One tile pass over axes [lo, hi) in `avail` LDS values, which hold the tile (with its bank padding and dump
-// slots) and the staged tables of the pass's axes (stage[a] values each); rows of at least 16 values, or the whole
-// inner extent.  false if the group does not fit.
-bool sep_make_pass(const int *m, const int *stage, int d, int nc, int lo, int hi, long avail, SepPass &ps) {
-    long st_sum = 0;
-    for (int a = lo; a < hi; ++a) st_sum += stage[a];
-    const long tile_max = (avail - st_sum - 66) * 32 / 33;
-    if (tile_max < 64) return false;
-    long rs = nc;
-    for (int j = hi; j < d; ++j) rs *= m[j];
-    long R = 1;
-    for (int j = lo; j < hi; ++j) R *= m[j];
-    const long wmin = hi == d ? rs : std::min<long>(rs, 16);
-    if (R * wmin > tile_max) return false;
-    long w;
-    if (hi == d || rs * R <= tile_max) {
-        w = rs;
-    } else {
-        w = 1;
-        while (w * 2 <= rs && R * w * 2 <= tile_max) w *= 2;
-    }
-    long nout = 1;
-    for (int j = 0; j < lo; ++j) nout *= m[j];
-    const long OB = std::min(std::max<long>(1, tile_max / (R * w)), nout);
-    ps = SepPass{};
-    ps.lo = lo;
-    ps.hi = hi;
-    ps.OB = (int)OB;
-    ps.R = (int)R;
-    ps.w = (int)w;
-    ps.rs = rs;
-    ps.nout = nout;
-    ps.nchunks = (rs + w - 1) / w;
-    ps.ntiles = ((nout + OB - 1) / OB) * ps.nchunks;
-    return true;
-}
-
-// Cuts the lattice into groups of consecutive axes, one tile pass each (innermost group first), with as few
-// passes as the tile size allows (every cut of d <= 5 axes is tried); false if some axis alone does not fit.
-bool sep_plan_passes(const int *m, const int *stage, int d, int nc, long avail, SepPlan &sp) {
-    int best = -1, best_np = BLML_SEP_MAXPASS + 1;
-    long best_tile = 0;
-    for (int cuts = 0; cuts < (1 << (d - 1)); ++cuts) {     // bit j set: a cut between axes j and j + 1
-        int np = 0, hi = d;
-        bool ok = true;
-        long mint = -1;
-        SepPass ps;
-        for (int lo = d - 1; lo >= 0 && ok; --lo) {
-            if (lo == 0 || (cuts >> (lo - 1) & 1)) {
-                ok = sep_make_pass(m, stage, d, nc, lo, hi, avail, ps);
-                const long t = (long)ps.OB * ps.R * ps.w;
-                mint = mint < 0 ? t : std::min(mint, t);
-                ++np;
-                hi = lo;
-            }
-        }
-        if (ok && (np < best_np || (np == best_np && mint > best_tile))) {
-            best = cuts;
-            best_np = np;
-            best_tile = mint;
-        }
-    }
-    if (best < 0) return false;
-    sp = SepPlan{};
-    long big = 0;
-    int hi = d;
-    for (int lo = d - 1; lo >= 0; --lo) {
-        if (lo == 0 || (best >> (lo - 1) & 1)) {
-            SepPass &ps = sp.pass[sp.npass++];
-            sep_make_pass(m, stage, d, nc, lo, hi, avail, ps);
-            long st_sum = 0;
-            for (int a = lo; a < hi; ++a) st_sum += stage[a];
-            const long t = (long)ps.OB * ps.R * ps.w;
-            big = std::max(big, t + t / 32 + 1 + 64 + st_sum);
-            hi = lo;
-        }
-    }
-    sp.tile = (int)big;     // LDS values of the largest pass: tile, padding, dump slots, staged tables
-    return true;
-}
-
 // raises `st` to the static LDS (bytes of __shared__ arrays) of kernel `kern`
 int static_lds(const void *kern, int &st) {
     hipFuncAttributes a;
@@ -996,80 +214,25 @@ int static_lds(const void *kern, int &st) {
     return BLML_OK;
 }
 
-// Whether every pass of a mat-vec plan takes the pipelined whole-tile path (the device's sep_pipe_ok, asked of every
-// axis of the lattice): then the stand-alone mat-vec runs matvec_pipe_kernel, which compiles in that path alone.
-template <typename T>
-bool sep_pipe_host(const SepPlan &sp, const int *shape) {
-    constexpr int BLK = SepMfma<T>::BLK;
-    const long NV = sizeof(T) == 4 ? 64 : 32;
-    for (int q = 0; q < sp.npass; ++q) {
-        const SepPass &ps = sp.pass[q];
-        const long tsize = (long)ps.OB * ps.R * ps.w;
-        if (tsize != NV * BLML_NT || ps.nout % ps.OB || ps.rs % ps.w) return false;
-        if (!(ps.w == ps.rs || (ps.w <= BLML_NT && BLML_NT % ps.w == 0 && ps.w % (16 / (long)sizeof(T)) == 0)))
-            return false;
-        if (tsize % ((long)BLML_SEP_FULL_NH * BLK * BLK)) return false;
-        long st = ps.w;
-        for (int a = ps.hi - 1; a >= ps.lo; --a) {
-            if (shape[a] > 1 && !(shape[a] == BLK && (st == 1 || st % 32 == 0))) return false;
-            st *= shape[a];
-        }
-    }
-    return true;
-}
-
 // A spread plan whose modes are all LINE factors on axes of at most SepMfma<T>::MMAX points takes the separable
 // path: the factors' dense tables on the device and two pass plans sized for the LDS the mat-vec and fit kernels
-// leave free.
+// leave free (sep_plan_host).
 template <typename T>
 int build_sep(Plan &pl, const double *fdata, const int32_t *freqs, int max_lds) {
-    const blml_fit_t &f = pl.hfits[0];
-    for (const blml_factor_t &fa : pl.hfacs)
-        if (fa.kind != BLML_FAC_LINE || f.shape[fa.axes[0]] > SepMfma<T>::MMAX) return BLML_OK;
-    std::vector<double> tab;
-    std::vector<long> off(std::max(1, pl.nfacs), 0);
-    int stage[BLML_MAXD] = {0, 0, 0, 0, 0};
-    for (int a = 0; a < f.d; ++a) stage[a] = sep_stage_len<T>(f.shape[a]);
-    for (int i = 0; i < pl.nfacs; ++i) {
-        off[i] = (long)tab.size();
-        sep_tables(pl.hfacs[i], f.shape[pl.hfacs[i].axes[0]], SepMfma<T>::BLK, fdata, freqs, tab);
-    }
-    const int nc = f.complex_g ? 2 : 1;
-    const size_t es = sizeof(T);
-    int st_mv = 0, st_fit = 0;
+    int st_mv = 0, st_fit = 0, st_pipe = 0;
     int rc = static_lds((const void *)matvec_grid_kernel<T>, st_mv);
     if (!rc) rc = static_lds((const void *)fit_grid_kernel<T>, st_fit);
+    if (!rc) rc = static_lds((const void *)matvec_pipe_kernel<T>, st_pipe);
     if (rc) return rc;
-    SepPlan sp[2];
-    const int stat[2] = {st_mv, st_fit};
-    for (int k = 0; k < 2; ++k) {
-        const long avail = ((long)max_lds - stat[k] - 64) / (long)es - 2;
-        if (!sep_plan_passes(f.shape, stage, f.d, nc, avail, sp[k])) return BLML_OK;
-        sp[k].pq = 0;
-    }
-    std::vector<T> tabT(tab.begin(), tab.end());
+    SepHost<T> sh;
+    if (!sep_plan_host<T>(pl, fdata, freqs, max_lds, st_mv, st_fit, st_pipe, sh)) return BLML_OK;
     HIPCHECK(hipMalloc(&pl.d_sep, sizeof(SepPlan) * 2));
-    HIPCHECK(hipMalloc(&pl.d_sep_tab, sizeof(T) * std::max<size_t>(1, tabT.size())));
-    HIPCHECK(hipMalloc(&pl.d_sep_off, sizeof(long) * off.size()));
-    HIPCHECK(hipMemcpy(pl.d_sep, sp, sizeof(SepPlan) * 2, hipMemcpyHostToDevice));
-    if (!tabT.empty()) HIPCHECK(hipMemcpy(pl.d_sep_tab, tabT.data(), sizeof(T) * tabT.size(), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(pl.d_sep_off, off.data(), sizeof(long) * off.size(), hipMemcpyHostToDevice));
-    pl.lds_mv = std::max(pl.lds_bytes, (int)((sp[0].tile + 2) * es));
-    pl.lds_fit = std::max(pl.lds_bytes, (int)((sp[1].tile + 2) * es));
-    pl.sep = true;
-    int st_pipe = 0;
-    rc = static_lds((const void *)matvec_pipe_kernel<T>, st_pipe);
-    if (rc) return rc;
-    long pipe_vals = 0, all_tabs = 0;
-    for (int q = 0; q < sp[0].npass; ++q) {
-        const long t = (long)sp[0].pass[q].OB * sp[0].pass[q].R * sp[0].pass[q].w;
-        pipe_vals = std::max(pipe_vals, t + t / 32 + 1 + 64);
-    }
-    for (int a = 0; a < f.d; ++a)
-        if (f.shape[a] > 1) all_tabs += (long)SepMfma<T>::BLK * (SepMfma<T>::BLK + 1);
-    const long lds_pipe = (pipe_vals + all_tabs + 2) * (long)es;
-    pl.lds_pipe = (int)std::min<long>(lds_pipe, 1L << 30);
-    pl.pipe_mv = st_pipe <= st_mv && lds_pipe + st_pipe <= max_lds && sep_pipe_host<T>(sp[0], f.shape);
+    HIPCHECK(hipMalloc(&pl.d_sep_tab, sizeof(T) * std::max<size_t>(1, sh.tab.size())));
+    HIPCHECK(hipMalloc(&pl.d_sep_off, sizeof(long) * sh.off.size()));
+    HIPCHECK(hipMemcpy(pl.d_sep, sh.sp, sizeof(SepPlan) * 2, hipMemcpyHostToDevice));
+    if (!sh.tab.empty())
+        HIPCHECK(hipMemcpy(pl.d_sep_tab, sh.tab.data(), sizeof(T) * sh.tab.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(pl.d_sep_off, sh.off.data(), sizeof(long) * sh.off.size(), hipMemcpyHostToDevice));
     return BLML_OK;
 }
 

@@ -1,0 +1,108 @@
+"""The engine's fit path compiled for the CPU under AddressSanitizer (tests/cpu_build: the device headers with a CPU
+stand-in for HIP, every buffer, LDS object and workspace region bounded).  A replay stops at the first access
+outside an allocation, so these runs check every index of the paths they take; their results are checked against
+the oracle (real rank 1: the unique optimum, loglik 1e-9 relative), against the one-workgroup run (a spread fit on
+a grid of 4 workgroups, separable tile passes on emulated matrix cores: loglik 1e-9) and against dense eigvalsh
+(certificate, 1e-6).  No GPU."""
+import os
+import shutil
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import blml_oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, 'cpu_build'))
+
+CXX = os.environ.get('BLML_REPLAY_CXX', '/opt/rocm/llvm/bin/clang++')
+pytestmark = pytest.mark.skipif(not shutil.which(CXX) and not os.path.exists(CXX),
+                                reason='no clang++ for the CPU build')
+
+PARAMS = dict(tol=1e-8, max_sweeps=None, anderson=5, stall=1000, phase_noise=0.1, ll_tol=None, ll_window=100,
+              solver='cd', lbfgs_m=20)
+
+
+@pytest.fixture(scope='module')
+def replay():
+    import replay as R
+    R.build()
+    return R
+
+
+def _batch(counts, modes, cplx, dtype, spread, params, h=1.0):
+    from blml_amd.binned import _FitSpec, FitBatch
+    fs = _FitSpec(counts, modes, h, 0.0, cplx, None, None, 0)
+    return FitBatch([fs], params, dtype, spread=spread)
+
+
+def _field_counts(rng, shape, n):
+    g = np.meshgrid(*[np.arange(m, dtype=float) for m in shape], indexing='ij')
+    p = np.exp(-0.5 * sum(((x - m / 2.2) / (m / 4.5)) ** 2 for x, m in zip(g, shape))) + 0.05
+    return rng.multinomial(n, (p / p.sum()).ravel()).reshape(shape).astype(float)
+
+
+def test_real_rank1_fit_matches_oracle(replay):
+    import blml_amd
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.normal(-1.5, 0.5, 3000), rng.normal(1.8, 0.8, 2000)])
+    counts = np.histogram(x, bins=512, range=(-6, 6))[0].astype(float)
+    out = replay.run(_batch(counts, blml_amd.make_modes([0.05], ['sinc']), False, np.float64, -1, PARAMS))
+    r = out['results'][0]
+    ref = O.BLMLBinnedOracle(tol=1e-8).fit(counts, O.make_modes([0.05], ['sinc']))
+    assert abs(r['loglik'] - ref.loglik_) < 1e-9 * abs(ref.loglik_)
+    assert int(r['stop_reason']) == 0 and ref.stop_reason_ == 'tol'
+    occ = counts > 0
+    assert np.max(np.abs(out['density'][:counts.size] - ref.density_)[occ] / ref.density_[occ]) < 1e-6
+
+
+def test_complex_slits_lbfgs_one_workgroup_in_bounds(replay):
+    """The FP64 one-workgroup L-BFGS fit of a 40 x 36 complex slits lattice with a dirichlet background (padded
+    power-of-two FFT lines of 128 values, band-restricted DFTs, two modes), capped at 30 iterations: every access
+    of its start solve, L-BFGS iterations and outputs stays inside its allocation."""
+    import blml_amd
+    rng = np.random.default_rng(12)
+    counts = _field_counts(rng, (40, 36), 20000)
+    modes = (blml_amd.make_modes([0.35, 0.3], ['slits:0.5', 'slits:0.4'])
+             + blml_amd.make_modes([3 / 40, 3 / 36], ['dirichlet', 'dirichlet'], periods=[40.0, 36.0]))
+    params = dict(PARAMS, solver='lbfgs', max_sweeps=30, ll_tol=1e-10)
+    out = replay.run(_batch(counts, modes, True, np.float64, -1, params))
+    r = out['results'][0]
+    assert int(r['n_sweeps']) == 30 and np.isfinite(r['loglik'])
+    assert np.all(np.isfinite(out['density'][:counts.size]))
+
+
+def test_spread_separable_fit_equals_one_workgroup(replay):
+    """A 4-D sinc fit spread over a grid of 4 workgroups (FP64 separable tile passes on emulated matrix cores, grid
+    barriers, rank-ordered team sums) against the same fit on one workgroup (transform passes)."""
+    import blml_amd
+    rng = np.random.default_rng(3)
+    counts = _field_counts(rng, (12, 10, 9, 8), 20000)
+    modes = blml_amd.make_modes([0.35] * 4, ['sinc'] * 4)
+    a = replay.run(_batch(counts, modes, False, np.float64, 1, PARAMS), ranks=4)
+    b = replay.run(_batch(counts, modes, False, np.float64, -1, PARAMS))
+    assert 'separable tile passes' in a['log']
+    la, lb = a['results'][0]['loglik'], b['results'][0]['loglik']
+    assert abs(la - lb) < 1e-9 * abs(lb)
+
+
+def test_fp32_plan_and_certificate(replay):
+    """An FP32 plan (FP32 transforms, the FP64 finishing pass) and the certificate of an FP64 fit (power iteration
+    with the eigen-residual stop) against dense eigvalsh."""
+    import blml_amd
+    rng = np.random.default_rng(2)
+    N = _field_counts(rng, (16, 16), 4000)
+    modes = blml_amd.make_modes([0.4, 0.4], ['ball', 'ball'])
+    out32 = replay.run(_batch(N, modes, False, np.float32, -1, dict(PARAMS, tol=1e-5)))
+    out = replay.run(_batch(N, modes, False, np.float64, -1, PARAMS), cert_iters=20000, cert_tol=1e-9)
+    r32, r = out32['results'][0], out['results'][0]
+    assert abs(r32['loglik'] - r['loglik']) < 1e-6 * abs(r['loglik'])
+    om = O.BLMLBinnedOracle()
+    om._setup(N, O.make_modes([0.4, 0.4], ['ball', 'ball']), 1.0, 0.0)
+    occ = N.ravel() > 0
+    S = O.dense_gram(om.specs_[0], N.shape, om.h_)[np.ix_(occ, occ)]
+    E = out['e'][:N.size][occ]
+    D = E ** 2 / N.ravel()[occ]
+    want = np.linalg.eigvalsh(np.sqrt(D)[:, None] * S * np.sqrt(D)[None])[-1] / N.sum()
+    assert out['iters'][0] > 0 and abs(out['lam'][0] - want) < 1e-6 * want
