@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -632,6 +633,66 @@ int blml_pair_rows(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *
     if (rc) return rc;
     HIPCHECK(sc.release());
     return BLML_OK;
+}
+
+// ---- blml_pair_rows_noalloc: blml_pair_rows' launch with its partial sums in a fixed array of the library's own.
+// The partials of one call, nsl * R doubles (pair_rows_grid), live in `scratch`: one array per GPU (the code
+// object's copy on each device), held until the process ends.  Host code reaches that memory only through the
+// address hipGetSymbolAddress gives on each device, kept per device in g_scratch_addr (the array's name on the host
+// is a stand-in, not a device address).  The rows kernel writes every (row, slice) partial before the sum kernel
+// reads it, so nothing needs clearing between calls; every call on one GPU uses the same array, so calls that
+// overlap on one GPU race.
+static constexpr long kNoAllocDoubles = BLML_PAIR_NOALLOC_MAX;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_addr[kMaxDevices] = {};
+
+// the current device's address of `scratch`, looked up on the first call on that device (with it current)
+static int scratch_addr(double **out) {
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(BLML_ERUNTIME, "device index beyond the library's table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(BLML_ERUNTIME, "hipGetSymbolAddress(scratch) gave no address");
+        g_scratch_addr[dev] = p;
+    }
+    *out = (double *)g_scratch_addr[dev];
+    return BLML_OK;
+}
+
+static __global__ void fill_zero_kernel(double *s, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) s[i] = 0.0;
+}
+
+int blml_pair_rows_noalloc(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, const double *w,
+                           int64_t R, int64_t M, double *s, void *stream) {
+    int rc = pair_check(tab);
+    if (rc) return rc;
+    if (R < 0 || M < 0 || R >= (1L << 31) || M >= (1L << 31)) return fail(BLML_EVALUE, "R, M: 0 .. 2^31 - 1");
+    if ((R > 0 && (!ZA || !s)) || (M > 0 && (!ZB || !w))) return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (R == 0) return BLML_OK;
+    if (M == 0) {      // a fill kernel, not a memset node: the caller may be capturing a graph
+        fill_zero_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(s, R);
+        HIPCHECK(hipGetLastError());
+        return BLML_OK;
+    }
+    long rb, nsl, slice;
+    pair_rows_grid(R, M, rb, nsl, slice);
+    if (nsl * R > kNoAllocDoubles)
+        return fail(BLML_EVALUE, "blml_pair_rows_noalloc: " + std::to_string(R) + " rows need " +
+                                     std::to_string(nsl * R) + " partial sums, more than its fixed " +
+                                     std::to_string(kNoAllocDoubles) + "; use blml_pair_rows");
+    double *part = nullptr;
+    rc = scratch_addr(&part);
+    if (rc) return rc;
+    return pair_rows_launch(*tab, ZA, ZB, w, R, M, part, s, st);
 }
 
 // rows for Mm densities with derivative entries: Wt (M x Mm, device), umask (Mm x U, host; NULL: every unit in

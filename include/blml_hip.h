@@ -167,6 +167,16 @@ typedef struct {
 int blml_pair_rows(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, const double *w, int64_t R,
                    int64_t M, double *s, void *stream);
 
+/* blml_pair_rows without any device allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same kernels
+ * and the same results as blml_pair_rows, with the FP64 partial sums in a fixed array of the library's own on each
+ * GPU (BLML_PAIR_NOALLOC_MAX doubles, 16 MiB, held until the process ends): a call whose partials need more -- more
+ * than 2^21 rows, or fewer rows whose slices need more -- fails with BLML_EVALUE.  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (other streams, other host threads) race: serialize them. */
+#define BLML_PAIR_NOALLOC_MAX (1L << 21)
+int blml_pair_rows_noalloc(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, const double *w,
+                           int64_t R, int64_t M, double *s, void *stream);
+
 /* Derivative rows (the reference's rows(..., grad=True): rows_kernel's dval and its prefix / suffix products,
  * blml_local_gpu.py:21-40, 94-103; blml_kernels.py:165-182 builds the rows).  Entry q is a derivative of unit
  * wunit[q]'s factor: for a column (wgrp[q] < 0) the row at dT[dbase[q]:] of column wcol[q], read at
