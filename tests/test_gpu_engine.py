@@ -138,10 +138,14 @@ def test_real_rank1_fits_match_oracle(dtype):
 
 
 def test_golden_fits_fp64():
-    """The FP64 engine against the reference's own outputs (tests/golden/make_golden.py)."""
+    """The FP64 engine against the reference's own outputs (tests/golden/make_golden.py): likelihood and density as
+    below, the reference's stop reason, and the sweep count inside the spread that rounding alone gives the
+    reference's own algorithm (tests/golden/sweep_envelope.json: the oracle, which reproduces the reference's counts
+    exactly, with its mat-vecs perturbed at the last bit of FP64), widened by 15%."""
     import blml_amd
     z = np.load(GOLD, allow_pickle=False)
     meta = json.loads(str(z['meta']))
+    env = json.load(open(os.path.join(os.path.dirname(GOLD), 'sweep_envelope.json')))['cases']
     for i, (name, counts, mm, est_kw, fit_kw) in enumerate(golden_cases()):
         _note(f'golden fit {i}: {name}')
         m = blml_amd.BLMLBinned(dtype=np.float64, **est_kw).fit(counts, _modes(mm, blml_amd), **fit_kw)
@@ -152,6 +156,10 @@ def test_golden_fits_fp64():
         _note(f'   loglik rel {rel:.2e} density max-rel {drel:.2e} sweeps {m.n_sweeps_} '
               f'(ref {int(z[f"c{i}_n_sweeps"])}) stop {m.stop_reason_} (ref {meta[i]["stop_reason"]})')
         assert list(m.pad_) == meta[i]['pad'] and m.shared_ == meta[i]['shared']
+        assert m.stop_reason_ == meta[i]['stop_reason'], (name, m.stop_reason_)
+        lo = min(env[i]['min'], env[i]['ref_n_sweeps'])
+        hi = max(env[i]['max'], env[i]['ref_n_sweeps'])
+        assert env[i]['name'] == name and 0.85 * lo <= m.n_sweeps_ <= 1.15 * hi, (name, m.n_sweeps_, lo, hi)
         if meta[i]['stop_reason'] == 'max_sweeps':
             assert rel < 1e-7, (name, rel)
         else:

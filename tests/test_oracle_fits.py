@@ -34,7 +34,7 @@ def test_oracle_matches_reference_golden(i):
     d_ref = z[f'c{i}_density']; occ = counts > 0
     assert np.max(np.abs(m.density_ - d_ref)[occ] / d_ref[occ]) < 1e-6
     assert m.stop_reason_ == meta[i]['stop_reason']
-    assert abs(m.n_sweeps_ - int(z[f'c{i}_n_sweeps'])) <= 2
+    assert m.n_sweeps_ == int(z[f'c{i}_n_sweeps'])
 
 
 def _hist(rng, shape, n, f):
