@@ -82,14 +82,15 @@ class _FitSpec:
         self.warm = False
         self.set_start(init, seed)
 
-    def set_start_solution(self, er):
+    def set_start_solution(self, er, solve=False):
         """The cold start (its draws unchanged) with er, on the lattice, in place of the real rank-1 start solve
-        under mode 0 (blml_binned.py:421-425) -- or of e0 when the fit is not psd.  The engine reads it as
-        init_K = -1."""
+        under mode 0 (blml_binned.py:421-425) -- or of e0 when the fit is not psd -- read by the engine as
+        init_K = -1; with solve=True er is the initial point of that start solve instead, which still runs to its
+        tolerance (init_K = -2: the convex solve's optimum, reached from nearer)."""
         e = np.zeros((1, self.B, 2))
         e[0, :, 0] = np.asarray(er, float).ravel()
         self.init = e
-        self.init_K = -1
+        self.init_K = -2 if solve else -1
 
     def set_start(self, init, seed):
         """The start: init's e_hat_ (blml_binned.py:403-408) when its shape_ matches, else the cold start; and the
@@ -304,6 +305,16 @@ class FitBatch:
                                                 self.stream()))
         return lam.cpu().numpy(), its.cpu().numpy()
 
+    def scalars(self):
+        """The per-fit blml_result_t rows (loglik, residual, sweeps, stop reason, ...) as a numpy structured array,
+        without the vectors (a small copy: what a judge of the results needs)."""
+        torch = _torch()
+        torch.cuda.synchronize()
+        res = np.frombuffer(self.d_res.cpu().numpy().tobytes(), dtype=_lib.RESULT_DTYPE)
+        if np.any(res['reserved'] != 0):
+            raise RuntimeError("a grid barrier of the spread fit timed out: its results are void")
+        return res
+
     def results(self):
         """Per-fit host outputs: list of dicts with e_hat, Se, density and the scalar attributes."""
         torch = _torch()
@@ -352,6 +363,57 @@ class FitBatch:
             self.close()
         except Exception:
             pass
+
+
+class Polish:
+    """The plain FP64 method started at another batch's results, on the device: the reference's polish test
+    (results/pball_parity.py:92-113 -- a result is a local maximum of the FP64 method when the method started there
+    moves loglik by less than 1e-6 relative).  The FP64 plan (every fit of `src` warm-started with all its modes,
+    init= semantics, blml_binned.py:403-408) is built once; run() copies src's current coefficients into its init
+    on the device (no host round trip) and fits; rel() gives (polished - src) / |src| per fit."""
+
+    def __init__(self, src, params, max_sweeps=200):
+        torch = _torch()
+        from types import SimpleNamespace
+        specs = []
+        for fs in src.fits:
+            cdt = np.complex128 if fs.complex_g else np.float64
+            init = SimpleNamespace(e_hat_=np.zeros((fs.K,) + tuple(fs.shape), cdt), shape_=fs.shape)
+            p = _FitSpec.__new__(_FitSpec)
+            p.__dict__.update(fs.__dict__)
+            p.set_start(init, 0)
+            specs.append(p)
+        self.src = src
+        self.batch = FitBatch(specs, dict(params, max_sweeps=max_sweeps), np.float64, hist_cap=0).build()
+        # where each coefficient of src's output goes in the polish init: (K, B, 2) doubles per fit
+        srcs, dsts, imag0 = [], [], []
+        for F, G, fs in zip(src._Fits, self.batch._Fits, src.fits):
+            V = fs.K * fs.B * fs.nc
+            j = np.arange(V, dtype=np.int64)
+            srcs.append(F.out_off + j)
+            if fs.nc == 2:
+                dsts.append(G.init_off + j)
+            else:
+                dsts.append(G.init_off + 2 * j)
+        self._src_idx = torch.from_numpy(np.concatenate(srcs)).to('cuda')
+        self._dst_idx = torch.from_numpy(np.concatenate(dsts)).to('cuda')
+
+    def run(self):
+        """Copies src's coefficients into the polish init and launches the FP64 fits (asynchronous)."""
+        d = self.batch.d_init
+        d.zero_()
+        d.index_copy_(0, self._dst_idx, self.src.d_e.index_select(0, self._src_idx).double())
+        self.batch.run()
+        return self
+
+    def rel(self, src_scalars=None):
+        """(polished loglik - src loglik) / |src loglik| per fit, and the polish's blml_result_t rows."""
+        a = self.src.scalars() if src_scalars is None else src_scalars
+        b = self.batch.scalars()
+        return (b['loglik'] - a['loglik']) / np.abs(a['loglik']), b
+
+    def close(self):
+        self.batch.close()
 
 
 class BLMLBinned:
@@ -424,15 +486,17 @@ class BLMLBinned:
 
 
 def fit_many(jobs, tol=1e-5, max_sweeps=None, anderson=5, dtype=np.float64, phase_noise=0.1, seed=0, stall=1000,
-             ll_tol=None, ll_window=100, solver='cd', lbfgs_m=20, hist_cap=4096, warm=None):
+             ll_tol=None, ll_window=100, solver='cd', lbfgs_m=20, hist_cap=4096, warm=None, judge=True):
     """Fits many independent lattices in one persistent launch.  jobs: iterable of (counts, modes, fit kwargs)
     with the keywords of BLMLBinned.fit.  Returns one fitted BLMLBinned per job (same attributes).
 
     warm: None (cold starts), True (the packaged warm-start network) or a blml_amd.warm.WarmNet.  The network
     predicts the start of every 2-D job without init= (blml_amd.warm.apply); each such fit still runs to the
-    solver's tolerance, and one that ends without converging (stop reason 'stall' or 'max_sweeps') is fitted
-    again from the cold start in a second launch, whose result it keeps.  Fitted estimators carry warm_start_
-    (True when the kept result started from the network)."""
+    solver's tolerance and is then judged: with judge=True the plain FP64 method is started at its result (one more
+    launch, Polish) and a fit it moves by 1e-6 relative or more -- not a local maximum of the plain method to that
+    tolerance -- or one that ended without converging (stop reason 'stall' or 'max_sweeps') is fitted again from
+    the cold start in a second launch, whose result it keeps.  Fitted estimators carry warm_start_ (True when the
+    kept result started from the network) and, for judged fits, polish_rel_ (the FP64 polish's relative change)."""
     est = BLMLBinned(tol=tol, max_sweeps=max_sweeps, anderson=anderson, dtype=dtype, phase_noise=phase_noise,
                      seed=seed, stall=stall, ll_tol=ll_tol, ll_window=ll_window, solver=solver, lbfgs_m=lbfgs_m)
     specs = []
@@ -449,7 +513,14 @@ def fit_many(jobs, tol=1e-5, max_sweeps=None, anderson=5, dtype=np.float64, phas
     batch = FitBatch(specs, est._params(), dtype, hist_cap=hist_cap).build().run()
     res = batch.results()
     where = [(batch, i) for i in range(len(specs))]
-    redo = [i for i, (fs, r) in enumerate(zip(specs, res)) if fs.warm and r['stop_reason'] in ('stall', 'max_sweeps')]
+    prel = [None] * len(specs)
+    if judge and any(fs.warm for fs in specs):
+        pol = Polish(batch, est._params())
+        rel, _ = pol.run().rel()
+        pol.close()
+        prel = [float(r) if fs.warm else None for r, fs in zip(rel, specs)]
+    redo = [i for i, (fs, r) in enumerate(zip(specs, res)) if fs.warm and (
+        r['stop_reason'] in ('stall', 'max_sweeps') or (prel[i] is not None and abs(prel[i]) >= 1e-6))]
     if redo:
         cold = []
         for i in redo:
@@ -465,4 +536,7 @@ def fit_many(jobs, tol=1e-5, max_sweeps=None, anderson=5, dtype=np.float64, phas
                        seed=seed, stall=stall, ll_tol=ll_tol, ll_window=ll_window, solver=solver, lbfgs_m=lbfgs_m)
         out.append(m._adopt(fs, b, r, i))
         m.warm_start_ = fs.warm
+    for m, p in zip(out, prel):
+        if m.warm_start_:
+            m.polish_rel_ = p
     return out

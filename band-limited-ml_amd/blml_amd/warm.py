@@ -269,11 +269,13 @@ def start_solutions(jobs, dtype=np.float32, max_sweeps=3000):
     return [np.asarray(r['e_hat'][0], float) for r in res], [float(fs.s[0]) for fs in specs]
 
 
-def apply(net, specs):
+def apply(net, specs, solve=False):
     """Sets the network's warm start on every 2-D fit spec (blml_amd.binned._FitSpec) that has no init of the
     caller's (a warm start of an earlier call is replaced): the predicted real rank-1 solution under mode 0 (times
     the signs, when given) in place of the start solve; the engine draws the phases and the other modes from the
-    cold start's draws (blml_binned.py:429-436).  Returns the number of specs set."""
+    cold start's draws (blml_binned.py:429-436).  solve=True: the prediction is the initial point of the start
+    solve instead, which still runs to its tolerance (the same convex optimum, from nearer).  Returns the number of
+    specs set."""
     idx = [i for i, fs in enumerate(specs) if fs.d == 2 and (fs.init is None or fs.warm)]
     if not idx:
         return 0
@@ -284,7 +286,7 @@ def apply(net, specs):
         er = start_from_prediction(fs.counts, p, fs.s[0])
         if fs.signs is not None:
             er = er * fs.signs
-        fs.set_start_solution(er)
+        fs.set_start_solution(er, solve)
         fs.warm = True
     return len(idx)
 

@@ -203,7 +203,8 @@ __device__ void run_fit(const DevPlan<T> &p, int fi, const Team &team, const dou
         S.tm.sync();
     } else {
         // cold start: e0 = sqrt(c / s0) on occupied bins (times the signs); with init_K < 0 the caller's real
-        // rank-1 solution stands in for e0 and, in a psd fit, for the start solve below
+        // rank-1 vector stands in for e0 -- and, in a psd fit with init_K = -1, for the start solve below, which
+        // with init_K = -2 starts from it instead (the same convex solve to the same tolerance, from nearer)
         const double *er0 = f.init_K < 0 ? init + f.init_off : nullptr;
         for (int b = (int)S.tm.first(); b < f.B; b += (int)S.tm.stride()) {
             const double sgb = S.sg ? S.sg[b] : 1.0;
@@ -213,7 +214,7 @@ __device__ void run_fit(const DevPlan<T> &p, int fi, const Team &team, const dou
         if (psd_full) {
             // the real rank-1 solution under mode 0 (blml_binned.py:421-425)
             S.K = 1; S.nc = 1; S.psd = false; S.rec = false;
-            if (!er0) {
+            if (!er0 || f.init_K == -2) {
                 const FitOut so = S.solve_cd();
                 start_sweeps = so.n_sweeps;
             }

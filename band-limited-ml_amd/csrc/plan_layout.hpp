@@ -122,7 +122,7 @@ int layout(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs,
         const blml_fit_t &f = pl.hfits[fi];
         if (f.d < 1 || f.d > BLML_MAXD) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": 1 <= d <= 5");
         if (f.K < 1 || f.K > BLML_MAXK) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": 1 <= K <= 8");
-        if (f.init_K < -1) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": init_K >= -1");
+        if (f.init_K < -2) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": init_K >= -2");
         long B = 1;
         for (int j = 0; j < f.d; ++j) {
             if (f.shape[j] < 1) return fail(BLML_EVALUE, "fit " + std::to_string(fi) + ": empty lattice axis");

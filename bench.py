@@ -9,10 +9,13 @@ coefficients on the signal band's harmonics, density |g|^2 normalised with a 5% 
 counts with totals log-uniform in [1e3, 1e6].  numpy default_rng(3).  L-BFGS (lbfgs_m 20), FP32 transforms with
 FP64 reductions; the plain FP64 method is run on the same GPU as the reference point of the metric.
 
-A step = one launch that fits all 2048 fits from their cold start.  value = fits / s of the FP32 engine.  Beside
-it: the plain FP64 method on the same GPU, the fraction of fits whose FP32 loglik is within 1e-6 of the FP64 run's
-and the fraction that is a local maximum to 1e-6 by the reference's polish test; the FP64 oracle on the host's
-cores (cpu_baseline); the large single fit (32^5 bins, product sinc, 1e7 draws of a 4-component Gaussian mixture,
+A step = the learned warm starts of all 2048 fits (host features, the network on the device, resampling, upload),
+one persistent FP32 launch that fits every one to the solver's tolerance, the judge -- the plain FP64 method
+started at every result, one launch (the reference's polish test: a fit it moves by >= 1e-6 relative is no local
+maximum of the plain method) -- and the cold refit of each rejected fit.  value = fits / s of that step.  Beside
+it: the cold FP32 launch and the plain FP64 method (cold) on the same GPU, the fraction of kept fits whose loglik is
+within 1e-6 of the FP64 run's and the fraction that is a local maximum to 1e-6 by the polish test, the plans' build
+times; the FP64 oracle on the host's cores (cpu_baseline); the large single fit (32^5 bins, product sinc, 1e7 draws of a 4-component Gaussian mixture,
 spread over every CU) in both precisions; the pair path's rows over the occupied bins of a 128^3 lattice (pairs/s).
 roofline: the Gram mat-vec of the large fit (the memory-bound case); roofline_batched: the batched mat-vec of the
 2048 fits.  Prints one JSON line."""
@@ -349,16 +352,40 @@ def main():
     from blml_amd import warm as W
     net = W.WarmNet.load()
 
-    # the warm-start plan: counts, tables and draws resident on the device before the clock starts; a step is the
-    # network's warm starts (host features, the network on the device, resampling), their upload, and one batched
-    # launch to the solver's tolerance
+    # the warm-start plan and its judge: counts, tables and draws resident on the device before the clock starts
+    # (their build time is reported); a step is the network's warm starts (host features, the network on the device,
+    # resampling), their upload, one batched FP32 launch to the solver's tolerance, the judge -- the plain FP64
+    # method started at every result (Polish, one launch; the reference's polish test) -- and the cold refit of
+    # every fit the judge rejects (moved by >= 1e-6, or not converged), its plan built inside the step
+    from blml_amd.binned import Polish
     W.apply(net, specs)
+    torch.cuda.synchronize()
+    tb0 = time.perf_counter()
     bw = FitBatch(specs, params, np.float32, hist_cap=0).build()
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - tb0
+    pol = Polish(bw, params)
+    torch.cuda.synchronize()
+    t_build_judge = time.perf_counter() - tb0 - t_build
+    judged = {}
 
     def warm_step():
         W.apply(net, specs)
         bw.refresh_init().run()
+        sw = bw.scalars()
+        rel, _ = pol.run().rel(sw)
+        redo = [i for i in range(len(specs)) if abs(rel[i]) >= 1e-6 or
+                _lib.STOP_REASONS[int(sw['stop_reason'][i])] in ('stall', 'max_sweeps')]
+        ll = sw['loglik'].copy()
+        if redo:
+            cold = [_FitSpec(fits[i][0], modes_of(fits[i][1], blml_amd), 1.0, 0.0, fits[i][2]['complex_g'], None,
+                             None, 0) for i in redo]
+            bc = FitBatch(cold, params, np.float32, hist_cap=0).build().run()
+            ll[redo] = bc.scalars()['loglik']
+            bc.close()
+        judged.update(loglik=ll, fallbacks=len(redo), polish_rel=rel, warm=sw)
 
+    from blml_amd import _lib
     for _ in range(args.warmup):
         warm_step()
     torch.cuda.synchronize()
@@ -373,9 +400,13 @@ def main():
     bw.run()
     torch.cuda.synchronize()
     t_launch = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    pol.run().rel()
+    t_judge = time.perf_counter() - t2
+    pol.close()
     bw.close()
-    print(f'bench: warm fp32 {dtw / args.steps * 1e3:.1f} ms/step ({t_launch * 1e3:.1f} ms of it the launch)',
-          file=sys.stderr, flush=True)
+    print(f'bench: warm fp32 {dtw / args.steps * 1e3:.1f} ms/step ({t_launch * 1e3:.1f} ms of it the launch, '
+          f'{t_judge * 1e3:.1f} ms the FP64 judge, {judged["fallbacks"]} cold fallbacks)', file=sys.stderr, flush=True)
     for fs in specs:                         # back to the cold start (the draws are the cold start's already)
         fs.init, fs.init_K, fs.warm = None, 0, False
 
@@ -391,10 +422,14 @@ def main():
     r32 = b32.results()
     print(f'bench: cold fp32 {dt32 * 1e3:.1f} ms', file=sys.stderr, flush=True)
     ll32 = np.array([r['loglik'] for r in r32])
-    llw = np.array([r['loglik'] for r in rw])
+    llw = np.asarray(judged['loglik'])         # the kept results: warm where the judge passed them, else cold
     relw = (llw - ll32) / np.abs(ll32)
     extra = dict(cold_fp32_fits_per_s=len(fits) / dt32, warm_speedup_vs_cold_fp32=dt32 / (dtw / args.steps),
-                 warm_launch_seconds=t_launch, warm_start_seconds=dtw / args.steps - t_launch,
+                 warm_launch_seconds=t_launch, judge_seconds=t_judge,
+                 warm_start_seconds=dtw / args.steps - t_launch - t_judge,
+                 cold_fallbacks=int(judged['fallbacks']),
+                 judge_pass_fraction=float(np.mean(np.abs(judged['polish_rel']) < 1e-6)),
+                 plan_build_seconds=dict(warm_fp32=t_build, judge_fp64=t_build_judge),
                  warm_iterations=float(np.mean([r['n_sweeps'] + r['start_sweeps'] for r in rw])),
                  cold_iterations=float(np.mean([r['n_sweeps'] + r['start_sweeps'] for r in r32])),
                  warm_vs_cold_better=int((relw > 1e-6).sum()), warm_vs_cold_worse=int((relw < -1e-6).sum()),

@@ -58,7 +58,8 @@ typedef struct {
     int32_t fac_count[BLML_MAXK];
     int32_t init_K;                  /* 0: cold start (blml_binned.py:415-427); >0: warm start with that many modes
                                         (:403-408); -1: the cold start with init's real rank-1 solution (layout
-                                        (1, B, 2)) in place of its start solve, the draws as the cold start's */
+                                        (1, B, 2)) in place of its start solve, the draws as the cold start's;
+                                        -2: the same, init being the start solve's initial point (the solve runs) */
     int32_t has_signs;
     double n;                        /* total count */
     double s[BLML_MAXK];             /* K_k(0) */
