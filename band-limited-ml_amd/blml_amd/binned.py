@@ -472,6 +472,43 @@ class BLMLBinned:
                         indexing='ij')
         return np.stack([x.ravel() for x in g], 1)
 
+    def sqrt_density(self, X, chunk=None):
+        """g_k(x) = (1/n) sum_b e_bk K_k(x - z_b) at points X (Nq, d) over the occupied bins (blml_binned.py:452-466):
+        (Nq,) for rank 1, else (Nq, K); complex for complex_g fits.  The sums run on the device (blml_point_rows,
+        the units' closed forms, FP64); chunk is accepted for compatibility."""
+        from .points import point_rows, point_units
+        X = np.atleast_2d(np.asarray(X, float))
+        if X.shape[1] != self.d_:
+            raise ValueError(f"X: points with {self.d_} coordinates")
+        occ = np.asarray(self.occ_).ravel()
+        Zb = self.centres()[occ]
+        E = np.asarray(self.e_hat_).reshape(self.K_, -1)[:, occ]
+        out = np.zeros((len(X), self.K_), dtype=complex if self.complex_g_ else float)
+        for k, sp in enumerate(self.specs_):
+            units, harm = point_units(sp)
+            W = np.stack([E[k].real, E[k].imag], 1) if self.complex_g_ else np.asarray(E[k].real, float)
+            S = point_rows(units, harm, X, Zb, W)
+            out[:, k] = (S[:, 0] + 1j * S[:, 1] if self.complex_g_ else S[:, 0]) / self.n_
+        return out[:, 0] if self.K_ == 1 else out
+
+    def predict(self, X):
+        """The density f(x) = sum_k |g_k(x)|^2 at points X (blml_binned.py:468-470)."""
+        G = self.sqrt_density(X)
+        return np.abs(G) ** 2 if G.ndim == 1 else (np.abs(G) ** 2).sum(1)
+
+    def predict_grid(self, r):
+        """f on the lattice refined r times per dim (origin, steps h / r; the last fine point is the last bin):
+        shape ((m_j - 1) r + 1, ...) (blml_binned.py:472-499).  Evaluated at the fine points by the device's direct
+        sums: the reference's shifted-table FFT pairs give the same values (the shift never reaches the table's
+        mask, and circular columns wrap the periodic kernel)."""
+        r = int(r)
+        if r < 1:
+            raise ValueError("predict_grid: r >= 1")
+        fine = tuple((m - 1) * r + 1 for m in self.shape_)
+        g = np.meshgrid(*[self.origin_[j] + self.h_[j] * np.arange(s) / r for j, s in enumerate(fine)], indexing='ij')
+        X = np.stack([x.ravel() for x in g], 1)
+        return self.predict(X).reshape(fine)
+
     def certificate(self, k=1, tol=1e-6):
         """lambda_max(D^1/2 S D^1/2) / n (shared kernel only, blml_binned.py:501-518), by power iteration on the
         device with FP64 transforms; tol is eigsh's: the iteration stops when the eigen-residual of the unit iterate

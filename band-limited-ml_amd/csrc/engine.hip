@@ -21,6 +21,7 @@
 #include "blml_hip.h"
 #include "net.hpp"
 #include "pairs.hpp"
+#include "points.hpp"
 #include "quick.hpp"
 #include "fit.hpp"
 #include "plan_layout.hpp"
@@ -850,6 +851,56 @@ int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *
 #undef BLML_COLS_BY_G
 #undef BLML_COLS_LAUNCH
     HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+// ------------------------------------------------------------------------------------------------ points
+int blml_point_rows(const blml_pointunit_t *units, int32_t U, int32_t d, const int32_t *harm, const double *X,
+                    const double *Z, const double *W, int32_t nw, int64_t Nq, int64_t M, double *S, void *stream) {
+    if (!units || U < 1 || U > BLML_POINT_MAXU) return fail(BLML_EVALUE, "point rows: 1..8 kernel units");
+    if (d < 1 || d > BLML_MAXD) return fail(BLML_EVALUE, "point rows: 1..5 columns");
+    if (nw != 1 && nw != 2) return fail(BLML_EVALUE, "point rows: nw 1 (real) or 2 (complex)");
+    if (Nq < 0 || M < 0 || Nq >= (1L << 31) || M >= (1L << 31)) return fail(BLML_EVALUE, "Nq, M: 0 .. 2^31 - 1");
+    if ((Nq > 0 && (!X || !S)) || (M > 0 && (!Z || !W))) return fail(BLML_EVALUE, "null argument");
+    PointTab t;
+    t.d = d;
+    t.U = U;
+    for (int u = 0; u < U; ++u) {
+        const blml_pointunit_t &a = units[u];
+        if (a.kind < 0 || a.kind > 4) return fail(BLML_EVALUE, "point rows: unknown unit kind");
+        const int nc = a.kind == 3 || a.kind == 4 ? a.ncol : 1;
+        if (nc < 1 || nc > d || (a.kind == 3 && nc > 5)) return fail(BLML_EVALUE, "point rows: unit columns");
+        for (int i = 0; i < nc; ++i)
+            if (a.col[i] < 0 || a.col[i] >= d) return fail(BLML_EVALUE, "point rows: a unit's column outside 0..d-1");
+        if (a.kind == 2 && !(a.period[0] > 0)) return fail(BLML_EVALUE, "point rows: dirichlet period");
+        if (a.kind == 4 && (a.nharm < 0 || (a.nharm > 0 && (!harm || a.hoff < 0))))
+            return fail(BLML_EVALUE, "point rows: periodic ball harmonics");
+        t.u[u] = a;
+        t.u[u].ncol = nc;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (Nq == 0) return BLML_OK;
+    if (M == 0) {
+        fill_zero_kernel<<<(unsigned)((Nq * nw + 255) / 256), 256, 0, st>>>(S, Nq * nw);
+        HIPCHECK(hipGetLastError());
+        return BLML_OK;
+    }
+    const long rb = (Nq + BLML_POINT_NT - 1) / BLML_POINT_NT;
+    long nsl = std::max<long>(1, std::min<long>((2048 + rb - 1) / rb, (M + BLML_POINT_NT - 1) / BLML_POINT_NT));
+    const long slice = ((M + nsl - 1) / nsl + BLML_POINT_NT - 1) / BLML_POINT_NT * BLML_POINT_NT;
+    nsl = (M + slice - 1) / slice;
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(double) * nsl * Nq * nw));
+    double *part = (double *)sc.p;
+    const dim3 grid((unsigned)rb, (unsigned)nsl);
+    if (nw == 1)
+        point_rows_kernel<1><<<grid, BLML_POINT_NT, 0, st>>>(t, harm, X, Z, W, (int)Nq, (int)M, (int)slice, part);
+    else
+        point_rows_kernel<2><<<grid, BLML_POINT_NT, 0, st>>>(t, harm, X, Z, W, (int)Nq, (int)M, (int)slice, part);
+    HIPCHECK(hipGetLastError());
+    point_rows_sum_kernel<<<(unsigned)((Nq * nw + 255) / 256), 256, 0, st>>>(part, (int)Nq, (int)nsl, nw, S);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(sc.release());
     return BLML_OK;
 }
 

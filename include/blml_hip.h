@@ -223,6 +223,33 @@ int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *
                    int64_t depmask, void *out, void *out_m, void *stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * The continuous kernel at arbitrary points: BLMLBinned.sqrt_density / predict / predict_grid (blml_binned.py:452-499,
+ * which evaluate the units' closed forms at x - z on the host) as s_q = sum_j K(x_q - z_j) w_j.  A unit: kind 0 sinc
+ * fc sinc(fc t) (blml_trivial.py:368-370), 1 slits fc sinc(fc t) - fi sinc(fi t) (:442-444), 2 dirichlet
+ * scale (sin(pi M t/P) - sin(pi Mi t/P)) / sin(pi t/P) on the folded distance (:509-515), 3 ball
+ * prod fc (1/2)^(D/2) J_{D/2}(pi s) / s^(D/2), s = ||fc t||, D = ncol <= 5 (:135-143, 588-591), 4 periodic ball
+ * scale sum_k cos(2 pi sum_i k_i t_i / P_i) over its nharm harmonics (:656-665). */
+#define BLML_POINT_MAXU 8     /* units per kernel */
+typedef struct {
+    int32_t kind;
+    int32_t ncol;
+    int32_t col[BLML_MAXD];
+    int32_t M, Mi;           /* dirichlet: 2N + 1 and the removed inner 2Ni + 1 (0: none) */
+    int32_t nharm;           /* periodic ball: harmonics at harm[hoff], ncol int32 each */
+    int32_t reserved;
+    int64_t hoff;
+    double fc[BLML_MAXD];    /* per column of the unit (one-column kinds: fc[0]) */
+    double fi;               /* slits: (1 - fill) fc */
+    double period[BLML_MAXD];
+    double scale;            /* dirichlet 1/P, periodic ball 1/prod P */
+} blml_pointunit_t;
+
+/* S (Nq x nw) = sum_j K(X_q - Z_j) W_j for the product of the U units (host array), X (Nq x d), Z (M x d) and W
+ * (M x nw, nw = 1 real or 2 complex) doubles and harm int32 (device); FP64 sums in a fixed order. */
+int blml_point_rows(const blml_pointunit_t *units, int32_t U, int32_t d, const int32_t *harm, const double *X,
+                    const double *Z, const double *W, int32_t nw, int64_t Nq, int64_t M, double *S, void *stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * The warm-start network (no counterpart in the reference: the learned start of the fit, entering through
  * BLMLBinned.fit(init=...), blml_binned.py:403-408).  NCHW images of width 64, even height; bf16 = 16-bit brain
  * floats (uint16 storage); FP32 accumulation.  All pointers are device pointers. */

@@ -307,6 +307,7 @@ class _Kern:
         for mk in masks:
             T = T * mk
         T = np.broadcast_to(T, tuple(pad)).astype(np.float64)
+        self.spec, self.diffs, self.masks = spec, diffs, masks
         self.T = T
         self.s = float(T[(0,) * d])
         self.Kf = sfft.rfftn(T, s=tuple(pad), workers=-1)
@@ -618,6 +619,50 @@ class BLMLBinnedOracle:
         g = np.meshgrid(*[self.origin_[j] + self.h_[j] * np.arange(m) for j, m in enumerate(self.shape_)],
                         indexing='ij')
         return np.stack([x.ravel() for x in g], 1)
+
+    def sqrt_density(self, X, chunk=4096):
+        """g_k(x) = (1/n) sum_b e_bk K_k(x - z_b) at points X (Nq, d), direct sums over the occupied bins
+        (blml_binned.py:452-466): (Nq,) for rank 1, else (Nq, K)."""
+        X = np.atleast_2d(np.asarray(X, float)); Zb = self.centres()
+        E = np.asarray(self.e_hat_).reshape(self.K_, -1); occ = np.asarray(self.occ_).ravel()
+        Zb, E = Zb[occ], E[:, occ]
+        out = np.zeros((len(X), self.K_), dtype=complex if self.complex_g_ else float)
+        for i in range(0, len(X), chunk):
+            diffs = X[i:i + chunk, None, :] - Zb[None, :, :]
+            for k, sp in enumerate(self.specs_):
+                Kv = spec_eval(sp, self.d_, [diffs[..., j] for j in range(self.d_)])
+                out[i:i + chunk, k] = Kv @ E[k] / self.n_
+        return out[:, 0] if self.K_ == 1 else out
+
+    def predict(self, X):
+        G = self.sqrt_density(X)
+        return np.abs(G) ** 2 if G.ndim == 1 else (np.abs(G) ** 2).sum(1)
+
+    def predict_grid(self, r):
+        """f on the lattice refined r times per dim, one FFT pair per sub-lattice shift and mode: the mode's table at
+        the offsets shifted by h sh / r (blml_binned.py:468-499)."""
+        import itertools
+        d = self.d_; sl = tuple(slice(0, m) for m in self.shape_)
+        fine = tuple((m - 1) * r + 1 for m in self.shape_)
+        out = np.zeros(fine)
+        for sh in itertools.product(range(r), repeat=d):
+            dens = 0.0
+            for k, kk in enumerate(self.kern_):
+                diffs = [kk.diffs[j] + self.h_[j] * sh[j] / r for j in range(d)]
+                T = spec_eval(kk.spec, d, diffs)
+                for mk in kk.masks:
+                    T = T * mk
+                Kf = sfft.rfftn(np.broadcast_to(T, self.pad_).astype(np.float64), s=self.pad_, workers=self.workers)
+                e = np.asarray(self.e_hat_[k])
+                parts = [e.real, e.imag] if np.iscomplexobj(e) else [e]
+                g = [sfft.irfftn(sfft.rfftn(p, s=self.pad_, workers=self.workers) * Kf, s=self.pad_,
+                                 workers=self.workers)[sl] for p in parts]
+                gk = g[0] + 1j * g[1] if len(g) == 2 else g[0]
+                dens = dens + np.abs(gk / self.n_) ** 2
+            idx = tuple(slice(s_, None, r) for s_ in sh)
+            tgt = out[idx]
+            out[idx] = dens[tuple(slice(0, n_) for n_ in tgt.shape)]
+        return out
 
     def certificate(self, k=1, tol=1e-6):
         """lambda_max(D^1/2 S D^1/2) / n by ARPACK on FP64 FFT mat-vecs (blml_binned.py:501-518)."""

@@ -118,3 +118,27 @@ def test_validation_errors():
         m.fit(np.zeros(8), O.make_modes([0.3], ['sinc']))
     with pytest.raises(ValueError):
         O.BLMLBinnedOracle(solver='newton')
+
+
+@pytest.mark.parametrize('kinds,fc,periods,shape,cplx', [
+    (['ball', 'ball'], [0.4, 0.4], None, (14, 12), True),
+    (['pball', 'pball'], [0.45, 0.45], [14.0, 12.0], (14, 12), False),
+    (['sinc', 'dirichlet'], [0.4, 0.3], [None, 12.0], (14, 12), True),
+    (['slits:0.5'], [0.35], None, (30,), False),
+], ids=['ball complex', 'pball', 'sinc x dirichlet complex', 'slits 1d'])
+def test_predict_grid_is_the_direct_sum_at_the_fine_points(kinds, fc, periods, shape, cplx):
+    """The oracle's predict_grid (one FFT pair per sub-lattice shift, blml_binned.py:468-499) equals its predict
+    (direct sums of the closed forms, :452-470) at the fine lattice's points -- the reference's own check
+    (binned_gpu_job.py:79-82) -- so each pins the other."""
+    rng = np.random.default_rng(4)
+    N = rng.poisson(6.0, shape).astype(float)
+    m = O.BLMLBinnedOracle(tol=1e-6, max_sweeps=300).fit(N, O.make_modes(fc, kinds, periods), complex_g=cplx,
+                                                         h=0.7, origin=-1.0)
+    r = 3
+    G = m.predict_grid(r)
+    fine = np.meshgrid(*[m.origin_[j] + m.h_[j] * np.arange(s) / r for j, s in enumerate(G.shape)], indexing='ij')
+    X = np.stack([x.ravel() for x in fine], 1)
+    P = m.predict(X)
+    assert np.max(np.abs(G.ravel() - P)) < 1e-10 * np.max(P)
+    # the bins themselves: the fit's density_
+    assert np.allclose(G[tuple(slice(0, None, r) for _ in shape)], m.density_, rtol=1e-9, atol=1e-12 * m.density_.max())
