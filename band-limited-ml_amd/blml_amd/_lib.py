@@ -53,7 +53,7 @@ EXPORTS = ('blml_plan_create', 'blml_plan_destroy', 'blml_matvec', 'blml_fit', '
            'blml_plan_info', 'blml_pair_rows', 'blml_pair_rows_grad', 'blml_pair_rows_multi', 'blml_pair_cols', 'blml_net_conv', 'blml_net_conv_grad_w',
            'blml_net_flip', 'blml_net_gelu_grad', 'blml_net_mse', 'blml_net_adam', 'blml_net_to_bf16',
            'blml_quick_sweep', 'blml_quick_exact', 'blml_quick_stats', 'blml_plan_matvec_kernel',
-           'blml_pair_rows_noalloc', 'blml_point_rows', 'blml_last_error')
+           'blml_pair_rows_noalloc', 'blml_point_rows', 'blml_grad_dots', 'blml_last_error')
 
 _lib = None
 
@@ -81,6 +81,7 @@ def load(path=None):
     lib.blml_pair_rows.argtypes = [vp, vp, vp, vp, i64, i64, vp, vp]
     lib.blml_pair_rows_noalloc.argtypes = [vp, vp, vp, vp, i64, i64, vp, vp]
     lib.blml_point_rows.argtypes = [vp, i32, i32, vp, vp, vp, vp, i32, i64, i64, vp, vp]
+    lib.blml_grad_dots.argtypes = [i32, i32, i32, i64, vp, vp, vp, i32, vp, vp, ctypes.c_double, vp, vp]
     lib.blml_pair_cols.argtypes = [vp, vp, vp, i64, i64, i64, vp, vp, vp]
     lib.blml_pair_rows_grad.argtypes = [vp, vp, vp, vp, vp, i64, i64, vp, vp, vp]
     lib.blml_pair_rows_multi.argtypes = [vp, vp, vp, vp, vp, vp, i32, i64, i64, vp, vp, vp]

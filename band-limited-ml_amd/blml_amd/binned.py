@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from .modes import parse_modes
-from .units import circular_dims, mode_factors, next_fast_len5
+from .units import circular_dims, mode_factor_dlogs, mode_factors, next_fast_len5
 
 __all__ = ['BLMLBinned', 'fit_many', 'FitBatch']
 
@@ -446,6 +446,7 @@ class BLMLBinned:
 
     def _adopt(self, fs, batch, r, index):
         self._batch, self._index = batch, index
+        self._gplans = None
         self.h_, self.origin_, self.pad_, self.shape_, self.d_ = fs.h, fs.origin, fs.pad, fs.shape, fs.d
         self.specs_, self.modes_, self.K_, self.shared_ = fs.specs, fs.modes, fs.K, fs.shared
         self.s_ = fs.s
@@ -508,6 +509,92 @@ class BLMLBinned:
         g = np.meshgrid(*[self.origin_[j] + self.h_[j] * np.arange(s) / r for j, s in enumerate(fine)], indexing='ij')
         X = np.stack([x.ravel() for x in g], 1)
         return self.predict(X).reshape(fine)
+
+    # ---------------------------------------------------------------- fc gradients (blml_binned.py:520-563, 685-693)
+    def _grad_plans(self):
+        """The derivative mat-vecs of this fit, built once: [(FitBatch, entry modes)] (at most 8 entries a plan, each
+        mode k's chain with unit u's table replaced by dK/dlog fc_u) and per mode the entry of each unit (None where
+        fc moves in whole harmonics)."""
+        if getattr(self, '_gplans', None) is not None:
+            return self._gplans
+        fs = self._batch.fits[self._index]
+        circ = circular_dims(fs.specs, fs.shape, fs.h)
+        chains, emode, where = [], [], []
+        for k, sp in enumerate(fs.specs):
+            _, facs = mode_factors(sp, fs.shape, fs.h, circ)
+            row = []
+            for u, g in enumerate(mode_factor_dlogs(sp, fs.shape, fs.h, circ)):
+                if g is None:
+                    row.append(None)
+                    continue
+                ch = list(facs)
+                ch[u] = g
+                row.append(len(chains))
+                chains.append(ch)
+                emode.append(k)
+            where.append(row)
+        plans = []
+        for c0 in range(0, len(chains), _lib.MAXK):
+            ds = _FitSpec.__new__(_FitSpec)
+            ds.__dict__.update(fs.__dict__)
+            part = chains[c0:c0 + _lib.MAXK]
+            ds.chains, ds.K, ds.mode_chain = part, len(part), list(range(len(part)))
+            ds.s, ds.shared, ds.equal_s = np.ones(len(part)), False, True
+            ds.init, ds.init_K, ds.signs, ds.draws = None, 0, None, np.zeros(1)
+            plans.append((FitBatch([ds], self._params(), self.dtype, hist_cap=0).build(), emode[c0:c0 + _lib.MAXK]))
+        self._gplans = (plans, where)
+        return self._gplans
+
+    def _dots(self, counts, eps, plan=None, emode=()):
+        """blml_grad_dots over this fit (and the derivative mat-vecs of one plan): host array of 4 + 2 nE doubles."""
+        torch = _torch()
+        b, i = self._batch, self._index
+        F = b._Fits[i]
+        fs = b.fits[i]
+        V = fs.K * fs.B * fs.nc
+        E = b.d_e[F.out_off:F.out_off + V]
+        U = b.d_se[F.out_off:F.out_off + V]
+        m = np.asarray(counts, float).reshape(-1)
+        if m.size != fs.B:
+            raise ValueError("counts: on the fit's lattice")
+        dm = torch.from_numpy(np.ascontiguousarray(m)).to('cuda')
+        nE = len(emode)
+        dU = None
+        if nE:
+            step_ = fs.B * fs.nc
+            Ed = torch.cat([b.d_e[F.out_off + k * step_:F.out_off + (k + 1) * step_] for k in emode])
+            dU = plan.matvec(Ed)
+        out = torch.zeros(4 + 2 * nE, dtype=torch.float64, device='cuda')
+        em = (ctypes.c_int32 * max(1, nE))(*emode)
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+        _lib.check(_lib.load().blml_grad_dots(b.code, fs.K, fs.nc, fs.B, ptr(E), ptr(U), None if dU is None else ptr(dU),
+                                              nE, em, ptr(dm), float(eps), ptr(out), b.stream()))
+        return out.cpu().numpy()
+
+    def param_grad(self, counts, eps=0.0):
+        """d/dlog fc_u of sum_b m_b log(F_b / Z + eps) / sum m at the current coefficients, for counts m on the
+        lattice (blml_binned.py:543-563): a list per mode of arrays, one entry per unit (nan where fc moves in whole
+        harmonics).  The derivative mat-vecs and the reductions run on the device (blml_matvec over a plan of the
+        derivative kernels, blml_grad_dots)."""
+        plans, where = self._grad_plans()
+        vals = []
+        for plan, emode in plans:
+            o = self._dots(counts, eps, plan, emode)
+            Z, M, fz = o[0], o[1], o[2] / o[1]
+            for j in range(len(emode)):
+                vals.append(2.0 * o[4 + 2 * j] / M - fz * o[5 + 2 * j] / Z)
+        return [np.array([np.nan if e is None else vals[e] for e in row]) for row in where]
+
+    def fc_grad(self):
+        """d loglik_ / d log fc of every (mode, unit) at the fitted coefficients (blml_binned.py:520-530): the
+        training counts' param_grad."""
+        return self.param_grad(self.w_)
+
+    def log_density(self, counts, eps=0.0):
+        """Mean over counts m on the lattice of log(f_b + eps), f = F / Z (blml_binned.py:685-693; loglik_ for the
+        training counts, a new batch's prequential score)."""
+        o = self._dots(counts, eps)
+        return float(o[3] / o[1])
 
     def certificate(self, k=1, tol=1e-6):
         """lambda_max(D^1/2 S D^1/2) / n (shared kernel only, blml_binned.py:501-518), by power iteration on the

@@ -283,3 +283,40 @@ def mode_factors(spec, shape, h, circ):
             facs.append(dict(kind='group', axes=list(dims), circ=[0] * len(dims), tlen=Ls, freqs=[None] * len(dims),
                              data=np.ascontiguousarray(T, dtype=float).ravel()))
     return s, facs
+
+
+def mode_factor_dlogs(spec, shape, h, circ):
+    """Per unit of mode_factors(spec, ...) (same order): the factor of dK/dlog fc_u (the unit's table replaced by its
+    derivative: blml_binned.py:92-101's dKf, one table per unit), or None where fc moves in whole harmonics
+    (dirichlet, periodic ball) or the unit is on circular dims."""
+    _, facs = mode_factors(spec, shape, h, circ)
+    out = []
+    fc = np.asarray(spec['fc'], float)
+    for (kind, dims), fa in zip(units_of(spec['kinds']), facs):
+        name = split_kind(kind)[0]
+        if name in ('dirichlet', 'pball') or any(fa['circ']):
+            out.append(None)
+            continue
+        g = dict(fa)
+        if fa['kind'] == 'line':
+            j = dims[0]
+            g['data'] = np.asarray(one_column_dvalues(kind, fc[j], spec['periods'][j], 'fc',
+                                                      np.arange(shape[j]) * h[j]), float)
+        else:                               # a ball over its columns: prod fc (D p(s) + s p'(s)), the same mask
+            Ls = fa['tlen']
+            offs = []
+            for a, j in enumerate(dims):
+                L = Ls[a]
+                i = np.arange(L)
+                off = np.where(i < (L + 1) // 2, i, i - L).astype(float)
+                offs.append((off, np.abs(off) <= shape[j] - 1))
+            grids = np.meshgrid(*[o[0] * h[j] for o, j in zip(offs, dims)], indexing='ij')
+            masks = np.meshgrid(*[o[1] for o in offs], indexing='ij')
+            sg = np.sqrt(sum((fc[j] * x) ** 2 for x, j in zip(grids, dims)))
+            D = len(dims)
+            T = float(np.prod(fc[dims])) * (D * ball_profile(sg, D) + ball_profile_dlog(sg, D))
+            for mk in masks:
+                T = T * mk
+            g['data'] = np.ascontiguousarray(T, dtype=float).ravel()
+        out.append(g)
+    return out

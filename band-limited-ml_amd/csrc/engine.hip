@@ -22,6 +22,7 @@
 #include "net.hpp"
 #include "pairs.hpp"
 #include "points.hpp"
+#include "grad.hpp"
 #include "quick.hpp"
 #include "fit.hpp"
 #include "plan_layout.hpp"
@@ -902,6 +903,55 @@ int blml_point_rows(const blml_pointunit_t *units, int32_t U, int32_t d, const i
     HIPCHECK(hipGetLastError());
     HIPCHECK(sc.release());
     return BLML_OK;
+}
+
+// ------------------------------------------------------------------------------------------------ gradients
+}  // extern "C"
+template <typename T>
+static int grad_dots_t(const GradArgs &g, const T *E, const T *U, const T *dU, const double *m, double *out,
+                       hipStream_t st) {
+    const long V = (long)g.K * g.B * g.nc;
+    const int nb1 = (int)std::min<long>(1024, (V + BLML_GRAD_NT - 1) / BLML_GRAD_NT);
+    const int nb2 = (int)std::min<long>(1024, (g.B + BLML_GRAD_NT - 1) / BLML_GRAD_NT);
+    const int W = 2 + 2 * g.nE;
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(double) * (2L * nb1 + (long)W * nb2)));
+    double *p1 = (double *)sc.p, *p2 = p1 + 2L * nb1;
+    grad_pass1_kernel<T><<<nb1, BLML_GRAD_NT, 0, st>>>(g, E, U, m, p1);
+    HIPCHECK(hipGetLastError());
+    grad_sum_kernel<<<1, 64, 0, st>>>(p1, nb1, 2, out);
+    HIPCHECK(hipGetLastError());
+    grad_pass2_kernel<T><<<nb2, BLML_GRAD_NT, 0, st>>>(g, E, U, dU, m, out, p2);
+    HIPCHECK(hipGetLastError());
+    grad_sum_kernel<<<(W + 63) / 64, 64, 0, st>>>(p2, nb2, W, out + 2);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(sc.release());
+    return BLML_OK;
+}
+extern "C" {
+
+int blml_grad_dots(int32_t dtype, int32_t K, int32_t nc, int64_t B, const void *E, const void *U, const void *dU,
+                   int32_t nE, const int32_t *emode, const double *m, double eps, double *out, void *stream) {
+    if (dtype != BLML_F32 && dtype != BLML_F64) return fail(BLML_EVALUE, "dtype: BLML_F32 or BLML_F64");
+    if (K < 1 || K > BLML_MAXK || (nc != 1 && nc != 2) || B < 1) return fail(BLML_EVALUE, "grad dots: K, nc, B");
+    if (nE < 0 || nE > BLML_GRAD_MAXE) return fail(BLML_EVALUE, "grad dots: 0..16 derivative entries");
+    if (!E || !U || !m || !out || (nE > 0 && (!dU || !emode))) return fail(BLML_EVALUE, "null argument");
+    if (!(eps >= 0)) return fail(BLML_EVALUE, "grad dots: eps >= 0");
+    GradArgs g;
+    g.K = K;
+    g.nE = nE;
+    g.nc = nc;
+    g.B = B;
+    g.eps = eps;
+    for (int j = 0; j < BLML_GRAD_MAXE; ++j) g.emode[j] = 0;
+    for (int j = 0; j < nE; ++j) {
+        if (emode[j] < 0 || emode[j] >= K) return fail(BLML_EVALUE, "grad dots: an entry's mode outside 0..K-1");
+        g.emode[j] = emode[j];
+    }
+    hipStream_t st = (hipStream_t)stream;
+    return dtype == BLML_F32 ? grad_dots_t<float>(g, (const float *)E, (const float *)U, (const float *)dU, m, out, st)
+                             : grad_dots_t<double>(g, (const double *)E, (const double *)U, (const double *)dU, m, out,
+                                                   st);
 }
 
 // ------------------------------------------------------------------------------------------------ BLMLQuick counts

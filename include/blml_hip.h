@@ -222,6 +222,15 @@ int blml_pair_rows_multi(const blml_pairtab_t *tab, const blml_pairgrad_t *grad,
 int blml_pair_cols(const blml_pairtab_t *tab, const int32_t *ZA, const int32_t *ZB, int64_t R, int64_t M,
                    int64_t depmask, void *out, void *out_m, void *stream);
 
+/* The reductions of BLMLBinned.param_grad and log_density (blml_binned.py:543-563, 685-693) over one fit's K modes:
+ * E, U = S E (K * B * nc values of the plan dtype, mode-major), derivative mat-vecs dU (nE * B * nc values, entry j a
+ * derivative of mode emode[j], HOST array) and counts m (B doubles).  out (device, 4 + 2 nE doubles) = [Z = Re sum
+ * conj(E) U, M = sum m, sum wn F, sum_{m > 0} m log max(F / Z + eps, 1e-300), then per entry sum wn Re(conj(U_k) dU_j)
+ * and Re sum conj(E_k) dU_j], F = sum_k |U_k|^2, wn = m / max(F + eps Z, 1e-300) on m > 0; FP64, fixed order. */
+#define BLML_GRAD_MAXE 16
+int blml_grad_dots(int32_t dtype, int32_t K, int32_t nc, int64_t B, const void *E, const void *U, const void *dU,
+                   int32_t nE, const int32_t *emode, const double *m, double eps, double *out, void *stream);
+
 /* ---------------------------------------------------------------------------------------------------------------
  * The continuous kernel at arbitrary points: BLMLBinned.sqrt_density / predict / predict_grid (blml_binned.py:452-499,
  * which evaluate the units' closed forms at x - z on the host) as s_q = sum_j K(x_q - z_j) w_j.  A unit: kind 0 sinc

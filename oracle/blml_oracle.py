@@ -620,6 +620,59 @@ class BLMLBinnedOracle:
                         indexing='ij')
         return np.stack([x.ravel() for x in g], 1)
 
+    def _dKf_fd(self, k, u, rel=1e-5):
+        """The spectrum of dK/dlog fc of unit u of mode k (the unit's columns scaled together), the table by a central
+        difference in log fc (error ~ rel^2); None where fc moves in whole harmonics (dirichlet, pball)."""
+        sp = self.specs_[k]
+        kind, dims = kernel_units(sp['kinds'], self.d_)[u]
+        if _split(kind)[0] in ('dirichlet', 'pball'):
+            return None
+        kk = self.kern_[k]
+        T = []
+        for sgn in (1.0, -1.0):
+            fc = np.array(sp['fc'], float)
+            fc[dims] = fc[dims] * np.exp(sgn * rel)
+            T2 = spec_eval(dict(sp, fc=list(fc)), self.d_, kk.diffs)
+            for mk in kk.masks:
+                T2 = T2 * mk
+            T.append(np.broadcast_to(T2, self.pad_).astype(np.float64))
+        return sfft.rfftn((T[0] - T[1]) / (2 * rel), s=self.pad_, workers=self.workers)
+
+    def _apply(self, Kf, e):
+        sl = tuple(slice(0, m) for m in self.shape_)
+        conv = lambda x: sfft.irfftn(sfft.rfftn(x, s=self.pad_, workers=self.workers) * Kf, s=self.pad_,
+                                     workers=self.workers)[sl]
+        return conv(e.real) + 1j * conv(e.imag) if np.iscomplexobj(e) else conv(e)
+
+    def param_grad(self, counts, eps=0.0):
+        """d/dlog fc_u of sum m log(F / Z + eps) / sum m at the current coefficients (blml_binned.py:543-563), with the
+        derivative spectra from central differences of the tables (_dKf_fd)."""
+        E, U = np.asarray(self.e_hat_), np.asarray(self.Se_)
+        m = np.asarray(counts, float); M = float(m.sum())
+        F = (np.abs(U) ** 2).sum(0); Z = float(np.real(np.sum(np.conj(E) * U)))
+        wn = np.where(m > 0, m / np.maximum(F + eps * Z, 1e-300), 0.0)
+        fz = float(np.sum(wn * F)) / M
+        out = []
+        for k, sp in enumerate(self.specs_):
+            gk = []
+            for u in range(len(kernel_units(sp['kinds'], self.d_))):
+                dKf = self._dKf_fd(k, u)
+                if dKf is None:
+                    gk.append(np.nan)
+                    continue
+                dU = self._apply(dKf, E[k])
+                dZ = float(np.real(np.sum(np.conj(E[k]) * dU)))
+                gk.append(2 * float(np.sum(wn * np.real(np.conj(U[k]) * dU))) / M - fz * dZ / Z)
+            out.append(np.array(gk))
+        return out
+
+    def log_density(self, counts, eps=0.0):
+        """Mean over counts m of log(F / Z + eps) (blml_binned.py:685-693)."""
+        U = np.asarray(self.Se_); F = (np.abs(U) ** 2).sum(0)
+        Z = float(np.real(np.sum(np.conj(np.asarray(self.e_hat_)) * U)))
+        m = np.asarray(counts, float); occ = m > 0
+        return float((m[occ] * np.log(np.maximum(F[occ] / Z + eps, 1e-300))).sum()) / float(m.sum())
+
     def sqrt_density(self, X, chunk=4096):
         """g_k(x) = (1/n) sum_b e_bk K_k(x - z_b) at points X (Nq, d), direct sums over the occupied bins
         (blml_binned.py:452-466): (Nq,) for rank 1, else (Nq, K)."""

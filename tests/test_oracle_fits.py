@@ -142,3 +142,27 @@ def test_predict_grid_is_the_direct_sum_at_the_fine_points(kinds, fc, periods, s
     assert np.max(np.abs(G.ravel() - P)) < 1e-10 * np.max(P)
     # the bins themselves: the fit's density_
     assert np.allclose(G[tuple(slice(0, None, r) for _ in shape)], m.density_, rtol=1e-9, atol=1e-12 * m.density_.max())
+
+
+@pytest.mark.parametrize('kinds,fc,cplx', [(['sinc'], [0.3], False), (['ball', 'ball'], [0.4, 0.4], False),
+                                           (['slits:0.5', 'sinc'], [0.45, 0.35], True)],
+                         ids=['sinc', 'ball', 'slits x sinc complex'])
+def test_param_grad_is_the_refit_derivative(kinds, fc, cplx):
+    """fc_grad (param_grad of the training counts at the fitted coefficients, blml_binned.py:520-563) is the
+    derivative of the refitted loglik (the envelope theorem: the coefficients are stationary), the reference's own
+    check (binned_gpu_job.py:84-86): the sum over units against a central difference of refits at fc e^{+-d}."""
+    rng = np.random.default_rng(8)
+    shape = (40,) if len(kinds) == 1 else (18, 16)
+    g = np.meshgrid(*[np.arange(s_, dtype=float) for s_ in shape], indexing='ij')
+    p = np.exp(-0.5 * sum(((x - s_ / 2.3) / (s_ / 6)) ** 2 for x, s_ in zip(g, shape))) + 0.05
+    N = rng.multinomial(6000, (p / p.sum()).ravel()).reshape(shape).astype(float)
+    mk = lambda f: O.make_modes(list(f), kinds)
+    kw = dict(tol=1e-11, max_sweeps=20000)
+    m = O.BLMLBinnedOracle(**kw).fit(N, mk(fc), complex_g=cplx)
+    G = m.param_grad(N)[0]
+    d = 1e-4
+    ll = [O.BLMLBinnedOracle(**kw).fit(N, mk(np.asarray(fc) * np.exp(s * d)), complex_g=cplx, init=m).loglik_
+          for s in (1, -1)]
+    fd = (ll[0] - ll[1]) / (2 * d)
+    assert abs(np.nansum(G) - fd) < 1e-5 * max(1.0, abs(fd)), (G, fd)
+    assert abs(m.log_density(N) - m.loglik_) < 1e-12 * abs(m.loglik_)
