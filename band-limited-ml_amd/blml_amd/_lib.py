@@ -53,7 +53,9 @@ EXPORTS = ('blml_plan_create', 'blml_plan_destroy', 'blml_matvec', 'blml_fit', '
            'blml_plan_info', 'blml_pair_rows', 'blml_pair_rows_grad', 'blml_pair_rows_multi', 'blml_pair_cols', 'blml_net_conv', 'blml_net_conv_grad_w',
            'blml_net_flip', 'blml_net_gelu_grad', 'blml_net_mse', 'blml_net_adam', 'blml_net_to_bf16',
            'blml_quick_sweep', 'blml_quick_exact', 'blml_quick_stats', 'blml_plan_matvec_kernel',
-           'blml_pair_rows_noalloc', 'blml_point_rows', 'blml_grad_dots', 'blml_last_error')
+           'blml_pair_rows_noalloc', 'blml_point_rows', 'blml_grad_dots', 'blml_quick_psd_sweep',
+           'blml_quick_psd_exact', 'blml_quick_psd_stats', 'blml_quick_psd_grad', 'blml_quick_psd_wscale',
+           'blml_vec_gram', 'blml_vec_lincomb', 'blml_lbfgs_twoloop', 'blml_last_error')
 
 _lib = None
 
@@ -97,6 +99,14 @@ def load(path=None):
     lib.blml_quick_sweep.argtypes = [vp, vp, i64, vp, vp, vp, i64, f64, f64, i32, vp, vp, vp]
     lib.blml_quick_exact.argtypes = [vp, vp, i64, vp, vp, vp, vp]
     lib.blml_quick_stats.argtypes = [vp, vp, vp, i64, f64, vp, vp]
+    lib.blml_quick_psd_sweep.argtypes = [vp, i32, vp, i64, i32, i32, vp, vp, i64, f64, vp, i32, vp, vp, vp]
+    lib.blml_quick_psd_exact.argtypes = [vp, i32, vp, i64, i32, i32, vp, vp, vp]
+    lib.blml_quick_psd_stats.argtypes = [vp, vp, vp, i64, i32, i32, f64, f64, vp, vp]
+    lib.blml_quick_psd_grad.argtypes = [vp, i32, vp, i64, i32, i32, vp, f64, vp, vp, vp, vp]
+    lib.blml_quick_psd_wscale.argtypes = [vp, vp, i64, i32, i32, f64, vp, vp]
+    lib.blml_vec_gram.argtypes = [i32, vp, i32, vp, i64, vp, vp]
+    lib.blml_vec_lincomb.argtypes = [i32, vp, vp, i32, i64, vp, vp]
+    lib.blml_lbfgs_twoloop.argtypes = [i32, vp, vp, vp, f64, i64, vp, vp, vp]
     lib.blml_last_error.restype = ctypes.c_char_p
     for name in EXPORTS[:-1]:
         getattr(lib, name).restype = ctypes.c_int

@@ -1,25 +1,35 @@
 """BLMLQuick(engine='hip') on counts recorded on a fixed lattice: the counts path of the reference's GPU estimator
-(blml_gpu.py:61-520, _BLMLQuickCUDA.fit with h in the modes; CPU twin blml_trivial.py:3516-3598), real rank 1.
+(blml_gpu.py:61-520 and 565-744, _BLMLQuickCUDA.fit with h in the modes; CPU twin blml_trivial.py:3516-3598).
 
     q = BLMLQuick(tol=1e-4).fit(Z, make_modes(...) with h per column, sample_weight=counts)
     q.e_hat_, q.Se_, q.n_sweeps_, q.converged_, q.residual_, q.rho_history_, q.obj_history_
 
-With e_b = w_b c_b the BLML stationarity condition is (S e)_b e_b = n w_b over the occupied bins (bins with w = 0
-are dropped); g(x) = (1/n) sum_b e_b K(x, z_b), and e'Se = n^2 at the solution.  The solver is the reference's
-block-Jacobi coordinate descent (blml_gpu.py:320-441): each sweep visits the bins in the order of
+Real rank 1: with e_b = w_b c_b the BLML stationarity condition is (S e)_b e_b = n w_b over the occupied bins
+(bins with w = 0 are dropped); g(x) = (1/n) sum_b e_b K(x, z_b), and e'Se = n^2 at the solution.  The solver is
+the reference's block-Jacobi coordinate descent (blml_gpu.py:320-441): each sweep visits the bins in the order of
 default_rng(seed).permutation(B), in blocks of min(block_size, ceil(B / 8)); plain sweeps until one raises the
 objective, which is then undone and redone with the block-level step-size safeguard for good; Se is recomputed
 exactly (FP64 sums) when the residual first drops under tol; a stall guard (< 1% over 50 sweeps) and max_sweeps
 (then e is scaled to e'Se = n^2) end it otherwise.
 
-Each block's S[:, I] dlt is one pair-rows pass of libblml_hip.so (blml_quick_sweep): the kernel tables of the
-reference's ProductKernel.tables layout in LDS, FP64 sums, and no B x |I| block in memory.  e and Se are FP64 on
-the device; dtype picks the tables' precision (float32: the reference's f32 policy, products in FP32).
+Complex / rank K (complex_g, or several modes; blml_gpu.py:461-486, 565-744): the real fit above under mode 0's
+kernel, then E (B, K) from its solution -- phases exp(i phase_noise N(0, 1)) on column 0, the other columns
+phase_noise |e| CN(0, 1), drawn from the same generator -- and block-Jacobi CD of L = sum w log F - n log Q with
+each row at its own stationary point (closed form for one shared kernel, the secular root for per-mode kernels),
+Anderson mixing over a fixed visiting order and the likelihood safeguard; or solver='lbfgs' (lbfgs_psd,
+blml_trivial.py:2590-2651).  Every vector of the solver lives on the device; the host keeps the control flow and
+the Anderson step's small least-squares system, as the reference does.
 
-init= warm-starts from a fit on the same lattice (the previous e at shared bins, each new bin at its 1-D
-minimiser).  Not built on this engine (they raise NotImplementedError): complex_g / rank K > 1 (the reference's
-_psd_solve), conditional dep columns, init from a fit on another lattice (re-binning), and points not already
-binned (modes without h)."""
+Each block's S[:, I] d is one pair-rows pass of libblml_hip.so (the kernel tables of the reference's
+ProductKernel.tables layout in LDS, FP64 sums, no B x |I| block in memory).  e and Se are FP64 on the device;
+dtype picks the tables' precision (float32: the reference's f32 policy, products in FP32).
+
+init= warm-starts a real rank-1 fit from a fit on the same lattice (the previous e at shared bins, each new bin at
+its 1-D minimiser) or on another lattice (re-binned: e_b = w_b / g_prev(z_b) where g_prev is not ~0,
+blml_gpu.py:213-219).  sqrt_density / predict evaluate the exact continuous kernel at any points
+(blml_point_rows).  Not built on this engine (they raise NotImplementedError): the complex start search
+(n_starts other than (0, 0) for a complex fit with no periodic column, blml_trivial.py:1605-1676), warm starts of
+complex / rank-K fits, conditional dep columns, and points not already binned (modes without h)."""
 import ctypes
 import os
 
@@ -74,6 +84,8 @@ class BLMLQuick:
         self.tol, self.max_sweeps, self.block_size = tol, max_sweeps, block_size
         self.seed, self.verbose, self.bin_scale, self.engine = seed, verbose, bin_scale, engine
         self.dtype = np.dtype(dtype or os.environ.get('BLML_DTYPE') or 'float32')
+        # the complex / rank-K fit's knobs the reference reads with getattr defaults (blml_gpu.py:576, 656, 472)
+        self.phase_noise, self.anderson, self.n_starts = 0.1, 5, (3, 2)
 
     # ---------------------------------------------------------------------------------------------------- fit
     def fit(self, X, modes, init=None, rebin_tol=1.0, signs=None, complex_g=False, sample_weight=None):
@@ -93,7 +105,17 @@ class BLMLQuick:
         if psd:
             if signs is not None:
                 raise ValueError("complex_g / rank > 1 / modes estimate the phases; do not pass signs as well")
-            raise NotImplementedError("BLMLQuick(engine='hip'): complex / rank-K counts fits are not built")
+            if init is not None:
+                raise NotImplementedError("BLMLQuick(engine='hip'): warm starts of complex / rank-K fits are not "
+                                          "built")
+            if len(specs) > _lib.MAXK:
+                raise ValueError(f"at most {_lib.MAXK} modes on this engine")
+            nh, ns = self.n_starts
+            periodic = any(str(k).partition(':')[0] in ('dirichlet', 'pball') for k in specs[0]['kinds'])
+            if complex_g and nh + ns > 0 and not periodic:
+                raise NotImplementedError("BLMLQuick(engine='hip'): the complex start search (n_starts = "
+                                          f"{tuple(self.n_starts)}) is not built; set n_starts = (0, 0) for the "
+                                          "one-start fit (the real root with phases)")
         keep = w > 0
         sg = None
         if signs is not None:
@@ -111,11 +133,18 @@ class BLMLQuick:
         Zi = np.round(Z / h).astype(np.int64)
         if np.abs(Zi * h - Z).max() > 1e-6 * h.max():
             raise ValueError("bin centers must lie on the lattice h * integers")
-        e0, new = self._warm(init, h, Zi, sg)
+        e0, new = self._warm(init, h, Zi, Z, w, sg)
         sp = specs[0]
         tab = local.lattice_tables(sp['kinds'], sp['fc'], sp['periods'], h, Zi)
         f32 = self.dtype == np.float32
-        out = self._solve(Zi, w, n, sg, tab, f32, e0, new)
+        rng = np.random.default_rng(self.seed)
+        out = self._solve(Zi, w, n, sg, tab, f32, e0, new, rng)
+        self.mode_specs_, self.aa_accepted_ = None, None
+        K = len(specs)
+        if psd:
+            shared = all(_same_spec(q, specs[0]) for q in specs)
+            out = self._psd(Zi, w, n, specs if not shared else specs[:1], h, f32, out[0], K, bool(complex_g), rng)
+            self.mode_specs_ = None if shared else [dict(q) for q in specs]
         e, Se, n_sweeps, converged, max_rho, rho_hist, obj_hist, safeguard = out
         if sg is not None:
             e = e * sg
@@ -126,26 +155,33 @@ class BLMLQuick:
         self.n_sweeps_, self.converged_, self.residual_ = n_sweeps, converged, max_rho
         self.safeguard_used_ = safeguard
         self.rho_history_, self.obj_history_ = rho_hist, obj_hist
-        self.complex_g_, self.rank_, self.dep_ = False, 1, None
+        self.complex_g_, self.rank_, self.dep_ = bool(complex_g), K, None
         self.kinds_, self.periods_, self._fc_arr = sp['kinds'], sp['periods'], sp['fc']
+        self._specs = specs
         self._h_vec, self._Zi, self._tab = h, Zi, tab
         return self
 
-    def _warm(self, init, h, Zi, sg):
-        """init= on the same lattice (blml_gpu.py:145-153, 200-208): the previous fit's e at the bins both share
-        (by lattice coordinates; |e| with signs), 0 at new bins (mask `new`, started by _solve).  A fit on another
-        lattice would be re-binned through init.sqrt_density (blml_gpu.py:213-219): not built here."""
+    def _warm(self, init, h, Zi, Z, w, sg):
+        """init= for a real rank-1 fit.  On the same lattice (blml_gpu.py:145-153, 200-208): the previous fit's e at
+        the bins both share (by lattice coordinates; |e| with signs), 0 at new bins (mask `new`, started by _solve).
+        On another lattice, re-binned (blml_gpu.py:213-219): e_b = w_b / g_prev(z_b) (s g_prev with signs) from the
+        previous fit's exact kernel at the new bins, the bins where g_prev ~ 0 (below 1e-3 of its largest size)
+        started by _solve."""
         self.rebinned_, self.n_new_bins_ = False, 0
         if init is None:
             return None, None
         hp = getattr(init, '_h_vec', None)
         if hp is None or getattr(init, 'e_hat_', None) is None:
             raise ValueError("init: a fitted BLMLQuick")
-        if np.shape(hp) != np.shape(h) or not np.allclose(hp, h):
-            raise NotImplementedError("BLMLQuick(engine='hip'): warm starts from a fit on another lattice "
-                                      "(re-binning through init.sqrt_density) are not built")
         if np.ndim(init.e_hat_) != 1 or np.iscomplexobj(init.e_hat_):
             raise NotImplementedError("BLMLQuick(engine='hip'): complex / rank-K warm starts are not built")
+        if np.shape(hp) != np.shape(h) or not np.allclose(hp, h):
+            g = np.asarray(init.sqrt_density(Z), float)
+            if sg is not None:
+                g = g * sg
+            new = ~(g > 1e-3 * max(float(np.abs(g).max()), 1e-300))
+            self.rebinned_, self.n_new_bins_ = True, int(new.sum())
+            return np.where(new, 0.0, w / np.where(new, 1.0, g)), new
         key = lambda Z: [z.tobytes() for z in np.ascontiguousarray(Z, np.int64)]
         prev = dict(zip(key(init._Zi), np.asarray(init.e_hat_, float)))
         e0 = np.array([prev.get(k, np.nan) for k in key(Zi)])
@@ -155,7 +191,7 @@ class BLMLQuick:
         self.n_new_bins_ = int(new.sum())
         return np.where(new, 0.0, e0), new
 
-    def _solve(self, Zi, w, n, sg, tab, f32, e0=None, new=None):
+    def _solve(self, Zi, w, n, sg, tab, f32, e0=None, new=None, rng=None):
         torch = _torch()
         lib = _lib.load()
         B = len(Zi)
@@ -176,7 +212,7 @@ class BLMLQuick:
              torch.from_numpy(np.ascontiguousarray(e0, np.float64)).cuda())
         Se = torch.empty_like(e)
         stats = torch.zeros(4, dtype=torch.float64, device='cuda')
-        rng = np.random.default_rng(self.seed)
+        rng = np.random.default_rng(self.seed) if rng is None else rng
         Bs = max(1, min(int(self.block_size), -(-B // 8)))
 
         def exact():
@@ -252,17 +288,331 @@ class BLMLQuick:
         return (e.cpu().numpy(), Se.cpu().numpy() if exact_se else None, n_sweeps, conv, max_rho, rho_hist,
                 obj_hist, safeguard)
 
+    # ------------------------------------------------------------------------------------- complex / rank K
+    def _psd(self, Zi, w, n, tspecs, h, f32, e_real, K, complex_g, rng):
+        """The rank-K / complex fit from the real solution e_real (blml_gpu.py:565-744): tspecs holds one spec
+        (a kernel shared by every mode) or one per mode.  Returns _solve's tuple with E (B, K) (B, for K = 1)."""
+        torch = _torch()
+        lib = _lib.load()
+        B = len(Zi)
+        nc = 2 if complex_g else 1
+        V = B * K * nc
+        st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        P = lambda x: ctypes.c_void_p(x.data_ptr())
+        tabs, Zs, keep = [], [], []
+        for q in tspecs:
+            tq = local.lattice_tables(q['kinds'], q['fc'], q['periods'], h, Zi)
+            t, dZ, _, kt = local._prepare(Zi, Zi, tq, f32, False)
+            tabs.append(t)
+            Zs.append(dZ)
+            keep.append(kt)
+        ntab = len(tabs)
+        tarr = (local.PairTab * ntab)(*tabs)
+        zarr = (ctypes.c_void_p * ntab)(*[z.data_ptr() for z in Zs])
+        sbb = []
+        one = torch.ones(1, dtype=torch.float64, device='cuda')
+        s1 = torch.zeros(1, dtype=torch.float64, device='cuda')
+        for t, dZ in zip(tabs, Zs):
+            _lib.check(lib.blml_pair_rows(ctypes.byref(t), P(dZ), P(dZ), P(one), 1, 1, P(s1), st))
+            sbb.append(float(s1.item()))
+        if not all(v > 0 for v in sbb):
+            raise RuntimeError(f"S(z, z) = {sbb}: the kernel's value at zero must be positive")
+        sbbs = np.array(sbb * (K if ntab == 1 else 1), np.float64)
+        sarr = (ctypes.c_double * K)(*sbbs)
+        dw = torch.from_numpy(np.ascontiguousarray(w, np.float64)).cuda()
+        stats = torch.zeros(4, dtype=torch.float64, device='cuda')
+        e0 = _psd_start(np.asarray(e_real, np.float64), K, complex_g, rng, float(self.phase_noise))
+        host = np.stack([e0.real, e0.imag], -1) if complex_g else e0[..., None]
+        E = torch.from_numpy(np.ascontiguousarray(host, np.float64)).cuda().reshape(-1)
+        SE = torch.empty_like(E)
+        Bs = max(1, min(int(self.block_size), -(-B // 8)))
+
+        def ptrs(xs):
+            return (ctypes.c_void_p * len(xs))(*[x.data_ptr() for x in xs])
+
+        def exact(X, Y):
+            _lib.check(lib.blml_quick_psd_exact(tarr, ntab, zarr, B, K, nc, P(X), P(Y), st))
+
+        def measure(X, Y, scale=1.0):
+            _lib.check(lib.blml_quick_psd_stats(P(X), P(Y), P(dw), B, K, nc, n, scale, P(stats), st))
+            tr, wl, rho, bad = stats.cpu().numpy()
+            return float(tr), (float(wl) - n * np.log(max(float(tr), 1e-300))) / n, float(rho), bad == 0
+
+        def comb(xs, c, out, mode=0):
+            _lib.check(lib.blml_vec_lincomb(len(xs), ptrs(xs), (ctypes.c_double * len(c))(*c), mode, V, P(out), st))
+            return out
+
+        def gram(xs, A):
+            A = np.ascontiguousarray(A, np.float64)
+            out = torch.empty(A.shape[0] ** 2, dtype=torch.float64, device='cuda')
+            _lib.check(lib.blml_vec_gram(len(xs), ptrs(xs), A.shape[0], A.ctypes.data_as(ctypes.c_void_p), V,
+                                         P(out), st))
+            return out.cpu().numpy().reshape(A.shape[0], A.shape[0])
+
+        exact(E, SE)
+        tr = measure(E, SE)[0]
+        sc = n / np.sqrt(max(tr, 1e-300))
+        comb([E], [sc], E)
+        comb([SE], [sc], SE)
+        if self.solver == 'lbfgs':
+            out = self._lbfgs_psd(E, SE, n, dw, tarr, ntab, zarr, B, K, nc, st, exact, measure, comb, gram)
+        else:
+            out = self._psd_cd(E, SE, n, dw, tarr, ntab, zarr, B, K, nc, Bs, sarr, st, rng, exact, measure, comb,
+                               gram)
+        Ef, SEf = out[0], out[1]
+        del keep
+        to_np = lambda X: _to_host(X, B, K, nc)
+        e = to_np(Ef)
+        Se = None if SEf is None else to_np(SEf)
+        if K == 1:
+            e = e[:, 0]
+            Se = None if Se is None else Se[:, 0]
+        return (e, Se) + tuple(out[2:])
+
+    def _psd_cd(self, E, SE, n, dw, tarr, ntab, zarr, B, K, nc, Bs, sarr, st, rng, exact, measure, comb, gram):
+        """Block-Jacobi CD with Anderson mixing and the likelihood safeguard (blml_gpu.py:650-744)."""
+        lib = _lib.load()
+        torch = _torch()
+        P = lambda x: ctypes.c_void_p(x.data_ptr())
+        aa_m = int(self.anderson)
+        perm_fixed = rng.permutation(B) if aa_m else None
+
+        def sweep(safe):
+            perm = perm_fixed if perm_fixed is not None else rng.permutation(B)
+            dp = torch.from_numpy(perm.astype(np.int32)).cuda()
+            _lib.check(lib.blml_quick_psd_sweep(tarr, ntab, zarr, B, K, nc, P(dw), P(dp), Bs, n, sarr, int(safe),
+                                                P(E), P(SE), st))
+
+        hG, hSG, hR = [], [], []
+        self.aa_accepted_ = 0
+        safeguard, n_sweeps, converged = False, 0, False
+        rho_hist, obj_hist = [], []
+        L_prev = measure(E, SE)[1]
+        cap, max_rho = self.max_sweeps, measure(E, SE)[2]
+        SE_end = None
+        while True:
+            if cap is not None and n_sweeps >= cap:
+                break
+            n_sweeps += 1
+            E_save, SE_save = E.clone(), SE.clone()
+            sweep(safeguard)
+            L_now = measure(E, SE)[1]
+            if not safeguard and L_now < L_prev - 1e-10 * abs(L_prev):
+                E.copy_(E_save)
+                SE.copy_(SE_save)
+                safeguard = True
+                sweep(True)
+                L_now = measure(E, SE)[1]
+            if aa_m:
+                hG.append(E.clone())
+                hSG.append(SE.clone())
+                hR.append(comb([E, E_save], [1.0, -1.0], torch.empty_like(E)))
+                if len(hG) > aa_m + 1:
+                    hG.pop(0)
+                    hSG.pop(0)
+                    hR.pop(0)
+                if len(hG) >= 2:
+                    L = len(hR)
+                    A = np.zeros((L, L))
+                    for j in range(L - 1):
+                        A[j, j + 1], A[j, j] = 1.0, -1.0
+                    A[L - 1, L - 1] = 1.0
+                    Gm = gram(hR, A)
+                    G, b = Gm[:L - 1, :L - 1], Gm[:L - 1, L - 1]
+                    gam = np.linalg.lstsq(G + 1e-12 * np.trace(G) * np.eye(len(G)), b, rcond=None)[0]
+                    E_aa = comb(hG, list(gam) + [0.0], torch.empty_like(E), mode=1)
+                    SE_aa = comb(hSG, list(gam) + [0.0], torch.empty_like(SE), mode=1)
+                    _, L_aa, _, fin = measure(E_aa, SE_aa)
+                    L_aa = L_aa if fin else -np.inf
+                    if L_aa >= L_now:
+                        E, SE, L_now = E_aa, SE_aa, L_aa
+                        self.aa_accepted_ += 1
+                    else:
+                        hG, hSG, hR = hG[-1:], hSG[-1:], hR[-1:]
+            L_prev = L_now
+            _, _, rho, fin = measure(E, SE)
+            if not fin:
+                max_rho = np.inf
+                rho_hist.append(max_rho)
+                break
+            max_rho = rho
+            if max_rho < self.tol:
+                exact(E, SE)
+                max_rho = measure(E, SE)[2]
+            rho_hist.append(max_rho)
+            obj_hist.append(-L_now)
+            if self.verbose and n_sweeps % 50 == 0:
+                print(f"  psd sweep {n_sweeps}: max|rho|={max_rho:.2e} -L/n={-L_now:.6f}", flush=True)
+            if max_rho < self.tol:
+                converged = True
+                break
+            if (len(rho_hist) > 500 and rho_hist[-1] > 0.99 * min(rho_hist[:-500])
+                    and obj_hist[-501] - obj_hist[-1] < 1e-9 * abs(obj_hist[-1])):
+                break
+        if not converged and cap is not None and measure(E, SE)[3]:     # capped: e'Se = n^2
+            exact(E, SE)
+            tr = measure(E, SE)[0]
+            sc = float(n / np.sqrt(max(tr, 1e-300)))
+            comb([E], [sc], E)
+            comb([SE], [sc], SE)
+            _, L_end, max_rho, _ = measure(E, SE)
+            converged = max_rho < self.tol
+            obj_hist.append(-L_end)
+            rho_hist.append(max_rho)
+            SE_end = SE
+        elif converged:
+            SE_end = SE
+        return E, SE_end, n_sweeps, converged, max_rho, rho_hist, obj_hist, safeguard
+
+    def _lbfgs_psd(self, e, Se, n, dw, tarr, ntab, zarr, B, K, nc, st, exact, measure, comb, gram):
+        """solver='lbfgs' (lbfgs_psd, blml_trivial.py:2590-2651): the two-loop recursion and every dot on the
+        device, the line search's trial points through exact S e passes."""
+        lib = _lib.load()
+        torch = _torch()
+        P = lambda x: ctypes.c_void_p(x.data_ptr())
+        V = B * K * nc
+        I2 = np.eye(2)
+
+        def grad(x, Sx):
+            G = torch.empty_like(x)
+            _lib.check(lib.blml_quick_psd_grad(tarr, ntab, zarr, B, K, nc, P(dw), n, P(x), P(Sx), P(G), st))
+            return G
+
+        def rho_of(x, Sx):
+            tr = measure(x, Sx)[0]
+            return measure(x, Sx, n / np.sqrt(max(tr, 1e-300)))[2]
+
+        L = measure(e, Se)[1] * n
+        g = grad(e, Se)
+        S_, Y_, R_, SY = [], [], [], []
+        it = 0
+        cap = np.inf if self.max_sweeps is None else self.max_sweeps
+        max_rho = rho_of(e, Se)
+        rho_hist, obj_hist = [max_rho], [-L / n]
+        reset = llstop = False
+        mem = int(self.lbfgs_m)
+        q = torch.empty_like(e)
+        while max_rho >= self.tol and it < cap:
+            if len(obj_hist) > 10 and obj_hist[-11] - obj_hist[-1] < 1e-11:
+                llstop = True
+                break
+            it += 1
+            if S_:
+                gamma = SY[-1][0] / SY[-1][1]
+                _lib.check(lib.blml_lbfgs_twoloop(len(S_), (ctypes.c_void_p * len(S_))(*[x.data_ptr() for x in S_]),
+                                                  (ctypes.c_void_p * len(Y_))(*[x.data_ptr() for x in Y_]),
+                                                  (ctypes.c_double * len(R_))(*R_), gamma, V, P(g), P(q), st))
+            else:
+                eg = gram([e, g], I2)
+                comb([g], [1e-3 * np.sqrt(eg[0, 0] / max(eg[1, 1], 1e-300))], q)
+            slope = float(gram([g, q], I2)[0, 1])
+            if not slope > 0:
+                S_, Y_, R_, SY = [], [], [], []
+                eg = gram([e, g], I2)
+                comb([g], [1e-3 * np.sqrt(eg[0, 0] / max(eg[1, 1], 1e-300))], q)
+                slope = float(gram([g, q], I2)[0, 1])
+            t = 1.0
+            ok = False
+            en, Sen = torch.empty_like(e), torch.empty_like(e)
+            for _ in range(40):
+                comb([e, q], [1.0, t], en)
+                exact(en, Sen)
+                Ln = measure(en, Sen)[1] * n
+                if Ln >= L + 1e-4 * t * slope:
+                    ok = True
+                    break
+                t *= 0.5
+            if not ok:
+                if reset or not S_:
+                    break
+                S_, Y_, R_, SY = [], [], [], []
+                reset = True
+                continue
+            reset = False
+            gn = grad(en, Sen)
+            s_ = comb([en, e], [1.0, -1.0], torch.empty_like(e))
+            y_ = comb([g, gn], [1.0, -1.0], torch.empty_like(e))
+            Gm = gram([s_, y_], I2)
+            ss, sy, yy = Gm[0, 0], Gm[0, 1], Gm[1, 1]
+            if sy > 1e-12 * np.sqrt(ss * yy):
+                S_.append(s_)
+                Y_.append(y_)
+                R_.append(1.0 / sy)
+                SY.append((sy, yy))
+                if len(S_) > mem:
+                    S_.pop(0)
+                    Y_.pop(0)
+                    R_.pop(0)
+                    SY.pop(0)
+            e, Se, L, g = en, Sen, Ln, gn
+            max_rho = rho_of(e, Se)
+            rho_hist.append(max_rho)
+            obj_hist.append(-L / n)
+            if self.verbose and it % 100 == 0:
+                print(f"  lbfgs {it}: max|rho|={max_rho:.2e} -L/n={-L / n:.9f}", flush=True)
+        Q = measure(e, Se)[0]
+        ec, Sec = torch.empty_like(e), torch.empty_like(e)
+        _lib.check(lib.blml_quick_psd_wscale(P(Se), P(dw), B, K, nc, Q / n, P(ec), st))
+        exact(ec, Sec)
+        if measure(ec, Sec)[1] * n >= L - 1e-9 * n:
+            e, Se = ec, Sec
+        tr = measure(e, Se)[0]
+        sc = n / np.sqrt(max(tr, 1e-300))
+        comb([e], [sc], e)
+        comb([Se], [sc], Se)
+        _, L_end, max_rho, _ = measure(e, Se)
+        converged = max_rho < self.tol or llstop
+        rho_hist.append(max_rho)
+        obj_hist.append(-L_end)
+        return e, Se, it, converged, max_rho, rho_hist, obj_hist, False
+
     # ---------------------------------------------------------------------------------------------- queries
     def sqrt_density(self, Z):
-        """g at lattice points Z (on the fit's lattice): (1/n) sum_b e_b K(z, z_b), by one pair-rows pass."""
+        """g(x) = (1/n) sum_b e_b K(x, z_b) at points Z (any points: the exact continuous kernel from the units'
+        closed forms, blml_point_rows, as the reference's sqrt_density, blml_gpu.py:955-989): (Nq,) for rank 1,
+        else (Nq, K); complex for complex fits."""
+        from .points import point_rows, point_units
         Zq = np.atleast_2d(np.asarray(Z, float))
-        Zqi = np.round(Zq / self._h_vec).astype(np.int64)
-        if np.abs(Zqi * self._h_vec - Zq).max() > 1e-6 * self._h_vec.max():
-            raise NotImplementedError("sqrt_density: points off the fit's lattice are not built on this engine")
-        tab = local.lattice_tables(self.kinds_, self._fc_arr, self.periods_, self._h_vec, np.vstack([self._Zi, Zqi]))
-        s, _ = local.rows(Zqi, self._Zi, self.e_hat_, tab, f32=self.dtype == np.float32)
-        return s / self.n_samples_
+        if Zq.shape[1] != self.dim_:
+            raise ValueError(f"Z: points with {self.dim_} coordinates")
+        E = np.asarray(self.e_hat_)
+        E = E[:, None] if E.ndim == 1 else E
+        K = E.shape[1]
+        cplx = np.iscomplexobj(E)
+        out = np.zeros((len(Zq), K), dtype=complex if cplx else float)
+        for k in range(K):
+            sp = self._specs[k] if self.mode_specs_ is not None else self._specs[0]
+            units, harm = point_units(sp)
+            W = np.stack([E[:, k].real, E[:, k].imag], 1) if cplx else np.asarray(E[:, k], float)
+            S = point_rows(units, harm, Zq, self.Z_, W)
+            out[:, k] = (S[:, 0] + 1j * S[:, 1] if cplx else S[:, 0]) / self.n_samples_
+        return out[:, 0] if K == 1 else out
 
     def predict(self, Z):
-        """The density at lattice points Z: g(z)^2."""
-        return self.sqrt_density(Z) ** 2
+        """The density f(x) = sum_k |g_k(x)|^2 at points Z (blml_gpu.py:892-897)."""
+        g = self.sqrt_density(Z)
+        return (np.abs(g) ** 2).sum(1) if g.ndim == 2 else np.abs(g) ** 2
+
+
+def _same_spec(a, b):
+    """One kernel: the same kinds, periods and fc in every column (the reference's _shared_kernel)."""
+    return (list(a['kinds']) == list(b['kinds']) and list(a['periods']) == list(b['periods'])
+            and np.allclose(np.asarray(a['fc'], float), np.asarray(b['fc'], float)))
+
+
+def _psd_start(e_real, K, complex_g, rng, pn):
+    """E0 (B, K) of the rank-K / complex fit (blml_gpu.py:582-590): column 0 the real solution, with phases
+    exp(i pn N(0, 1)) when complex; the other columns pn |e| times CN(0, 1) (real: N(0, 1)), drawn in that order."""
+    B = len(e_real)
+    e0 = np.zeros((B, K), dtype=complex if complex_g else np.float64)
+    e0[:, 0] = e_real * (np.exp(1j * pn * rng.standard_normal(B)) if complex_g else 1.0)
+    for k in range(1, K):
+        noise = ((rng.standard_normal(B) + 1j * rng.standard_normal(B)) / np.sqrt(2) if complex_g
+                 else rng.standard_normal(B))
+        e0[:, k] = pn * e_real * noise
+    return e0
+
+
+def _to_host(X, B, K, nc):
+    a = X.reshape(B, K, nc).cpu().numpy()
+    return a[..., 0] + 1j * a[..., 1] if nc == 2 else a[..., 0].copy()

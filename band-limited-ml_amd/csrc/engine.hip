@@ -24,6 +24,7 @@
 #include "points.hpp"
 #include "grad.hpp"
 #include "quick.hpp"
+#include "quick_psd.hpp"
 #include "fit.hpp"
 #include "plan_layout.hpp"
 
@@ -1032,6 +1033,314 @@ int blml_quick_stats(const double *e, const double *Se, const double *w, int64_t
     quick_stats_kernel<<<np, BLML_QUICK_NT, 0, st>>>(B, e, Se, w, n, part);
     HIPCHECK(hipGetLastError());
     quick_stats_sum_kernel<<<1, 64, 0, st>>>(part, np, out);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(sc.release());
+    return BLML_OK;
+}
+
+// ------------------------------------------------------------------------------- complex / rank-K counts fits
+// out[r * ostride + ooff + i] = sum_j S(ZA_r, ZB_j) Wt[j * nw + i] for i < nw <= 16; part: nsl * nw * R doubles
+// (pair_rows_grid(R, M))
+static int psd_rows_launch(const blml_pairtab_t &t, const int32_t *ZA, const int32_t *ZB, const double *Wt, int nw,
+                           long R, long M, double *part, double *out, int ostride, int ooff, hipStream_t st) {
+    long rb, nsl, slice;
+    pair_rows_grid(R, M, rb, nsl, slice);
+    const size_t es = t.f32 ? 4 : 8;
+    const bool tl = t.nT * (long)es <= 32768;
+    const int NWl = nw <= 2 ? 2 : nw <= 4 ? 4 : nw <= 8 ? 8 : 16;
+    const long tbytes = tl ? ((t.nT * (long)es + 15) / 16) * 16 : 0;
+    const int lds = (int)(tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16 + BLML_PAIR_CH * 8 * NWl);
+    const dim3 grid((unsigned)rb, (unsigned)nsl);
+#define BLML_NW_LAUNCH(R_, TL, NG, NW) \
+    pair_rows_nw_kernel<R_, TL, NG, NW><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, Wt, nw, (int)R, (int)M, \
+                                                                         (int)slice, part)
+#define BLML_NW_BY_W(R_, TL, NG) \
+    do { \
+        if (NWl == 2) BLML_NW_LAUNCH(R_, TL, NG, 2); \
+        else if (NWl == 4) BLML_NW_LAUNCH(R_, TL, NG, 4); \
+        else if (NWl == 8) BLML_NW_LAUNCH(R_, TL, NG, 8); \
+        else BLML_NW_LAUNCH(R_, TL, NG, 16); \
+    } while (0)
+#define BLML_NW_BY_G(R_, TL) \
+    do { \
+        if (t.G == 0) BLML_NW_BY_W(R_, TL, 0); \
+        else BLML_NW_BY_W(R_, TL, BLML_PAIR_MAXG); \
+    } while (0)
+    if (t.f32) {
+        if (tl) BLML_NW_BY_G(float, true);
+        else BLML_NW_BY_G(float, false);
+    } else {
+        if (tl) BLML_NW_BY_G(double, true);
+        else BLML_NW_BY_G(double, false);
+    }
+#undef BLML_NW_BY_G
+#undef BLML_NW_BY_W
+#undef BLML_NW_LAUNCH
+    HIPCHECK(hipGetLastError());
+    const long tot = R * nw;
+    pair_rows_nw_sum_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, nw, ostride, ooff,
+                                                                         out);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+static int psd_check(const blml_pairtab_t *tabs, int32_t ntab, const int32_t *const *Z, int64_t B, int32_t K,
+                     int32_t nc) {
+    if (!tabs || !Z) return fail(BLML_EVALUE, "null argument");
+    if (K < 1 || K > BLML_QUICK_MAXK) return fail(BLML_EVALUE, "K: 1 .. " + std::to_string(BLML_QUICK_MAXK));
+    if (nc != 1 && nc != 2) return fail(BLML_EVALUE, "nc: 1 (real) or 2 (complex)");
+    if (ntab != 1 && ntab != K) return fail(BLML_EVALUE, "tables: one shared, or one per mode");
+    if (B < 1 || B >= (1L << 30)) return fail(BLML_EVALUE, "B: 1 .. 2^30 - 1 bins");
+    for (int q = 0; q < ntab; ++q) {
+        const int rc = pair_check(tabs + q);
+        if (rc) return rc;
+        if (tabs[q].d != tabs[0].d) return fail(BLML_EVALUE, "tables: the same number of columns");
+        if (!Z[q]) return fail(BLML_EVALUE, "null bins");
+    }
+    return BLML_OK;
+}
+
+// SE (ostride K nc) = S src: src (B, K, nc) for a shared table (one pass of K nc columns); per-mode tables read
+// byk, the sources by mode (K, B, nc), one pass of nc columns each
+static int psd_apply_all(const blml_pairtab_t *tabs, int32_t ntab, const int32_t *const *Z, long B, int K, int nc,
+                         const double *src, const double *byk, double *part, double *SE, hipStream_t st) {
+    if (ntab == 1) return psd_rows_launch(tabs[0], Z[0], Z[0], src, K * nc, B, B, part, SE, K * nc, 0, st);
+    for (int k = 0; k < K; ++k) {
+        const int rc = psd_rows_launch(tabs[k], Z[k], Z[k], byk + (size_t)k * B * nc, nc, B, B, part, SE, K * nc,
+                                       k * nc, st);
+        if (rc) return rc;
+    }
+    return BLML_OK;
+}
+
+static size_t psd_part_len(long R, long M, int nw) {
+    long rb, nsl, slice;
+    pair_rows_grid(R, M, rb, nsl, slice);
+    return (size_t)nsl * nw * R;
+}
+
+int blml_quick_psd_exact(const blml_pairtab_t *tabs, int32_t ntab, const int32_t *const *Z, int64_t B, int32_t K,
+                         int32_t nc, const double *E, double *SE, void *stream) {
+    int rc = psd_check(tabs, ntab, Z, B, K, nc);
+    if (rc) return rc;
+    if (!E || !SE) return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t V = (size_t)B * K * nc;
+    const size_t np = psd_part_len(B, B, ntab == 1 ? K * nc : nc);
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(double) * (np + (ntab == 1 ? 0 : V))));
+    double *part = (double *)sc.p, *byk = part + np;
+    if (ntab > 1) {
+        psd_by_mode_kernel<<<(unsigned)((V + 255) / 256), 256, 0, st>>>(B, K, nc, E, byk);
+        HIPCHECK(hipGetLastError());
+    }
+    rc = psd_apply_all(tabs, ntab, Z, B, K, nc, E, byk, part, SE, st);
+    if (rc) return rc;
+    HIPCHECK(sc.release());
+    return BLML_OK;
+}
+
+int blml_quick_psd_sweep(const blml_pairtab_t *tabs, int32_t ntab, const int32_t *const *Z, int64_t B, int32_t K,
+                         int32_t nc, const double *w, const int32_t *perm, int64_t Bs, double n, const double *s,
+                         int32_t safeguard, double *E, double *SE, void *stream) {
+    int rc = psd_check(tabs, ntab, Z, B, K, nc);
+    if (rc) return rc;
+    if (!w || !perm || !s || !E || !SE) return fail(BLML_EVALUE, "null argument");
+    if (Bs < 1 || Bs > B) return fail(BLML_EVALUE, "block size: 1 .. B");
+    if (!(n > 0)) return fail(BLML_EVALUE, "n must be positive");
+    PsdArgs a;
+    a.K = K;
+    a.nc = nc;
+    a.shared = ntab == 1;
+    a.n = n;
+    for (int k = 0; k < BLML_QUICK_MAXK; ++k) a.s[k] = k < K ? s[k] : 0.0;
+    for (int k = 0; k < K; ++k)
+        if (!(a.s[k] > 0)) return fail(BLML_EVALUE, "S_k(z, z) must be positive");
+    hipStream_t st = (hipStream_t)stream;
+    const int d = tabs[0].d, Vn = K * nc;
+    const int nw = ntab == 1 ? Vn : nc;
+    size_t np = psd_part_len(B, Bs, nw);
+    if (B % Bs) np = std::max(np, psd_part_len(B, B % Bs, nw));
+    const int nbs = (int)std::min<long>((B + BLML_PSD_NT - 1) / BLML_PSD_NT, 512);
+    // partials, V (B K nc), EI, d, d by mode (Bs K nc each), step partials, t; then the block's bins per table
+    const size_t nd = np + (size_t)B * Vn + 3 * (size_t)Bs * Vn + (size_t)nbs * BLML_PSD_NACC + 1;
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(double) * nd + sizeof(int32_t) * (size_t)ntab * Bs * d));
+    double *part = (double *)sc.p, *Vv = part + np, *EI = Vv + (size_t)B * Vn, *dl = EI + (size_t)Bs * Vn,
+           *Wm = dl + (size_t)Bs * Vn, *spart = Wm + (size_t)Bs * Vn, *tdev = spart + (size_t)nbs * BLML_PSD_NACC;
+    int32_t *Zb = (int32_t *)(tdev + 1);
+    for (long s0 = 0; s0 < B; s0 += Bs) {
+        const long m = std::min<long>(Bs, B - s0);
+        const int32_t *I = perm + s0;
+        for (int q = 0; q < ntab; ++q) {
+            psd_gather_kernel<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(Z[q], d, I, (int)m, Zb + (size_t)q * Bs * d);
+            HIPCHECK(hipGetLastError());
+        }
+        psd_block_kernel<<<(unsigned)((m + BLML_PSD_NT - 1) / BLML_PSD_NT), BLML_PSD_NT, 0, st>>>(a, I, (int)m, E, SE,
+                                                                                              w, EI, dl, Wm);
+        HIPCHECK(hipGetLastError());
+        if (ntab == 1) {
+            rc = psd_rows_launch(tabs[0], Z[0], Zb, dl, Vn, B, m, part, Vv, Vn, 0, st);
+            if (rc) return rc;
+        } else {
+            for (int k = 0; k < K; ++k) {
+                rc = psd_rows_launch(tabs[k], Z[k], Zb + (size_t)k * Bs * d, Wm + (size_t)k * m * nc, nc, B, m, part,
+                                     Vv, Vn, k * nc, st);
+                if (rc) return rc;
+            }
+        }
+        if (safeguard) {
+            psd_step_part_kernel<<<nbs, BLML_PSD_NT, 0, st>>>(a, B, I, (int)m, E, SE, Vv, w, dl, spart);
+            HIPCHECK(hipGetLastError());
+            psd_step_pick_kernel<<<1, 64, 0, st>>>(spart, nbs, n, tdev);
+            HIPCHECK(hipGetLastError());
+        }
+        const long nval = (long)B * Vn;
+        psd_apply_kernel<<<(unsigned)((nval + BLML_PSD_NT - 1) / BLML_PSD_NT), BLML_PSD_NT, 0, st>>>(
+            nval, Vn, I, (int)m, Vv, EI, dl, safeguard ? tdev : nullptr, E, SE);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(sc.release());
+    return BLML_OK;
+}
+
+int blml_quick_psd_stats(const double *E, const double *SE, const double *w, int64_t B, int32_t K, int32_t nc,
+                         double n, double scale, double *out, void *stream) {
+    if (B < 1 || K < 1 || K > BLML_QUICK_MAXK || (nc != 1 && nc != 2)) return fail(BLML_EVALUE, "B, K, nc");
+    if (!E || !SE || !w || !out) return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int np = (int)std::min<long>((B + BLML_PSD_NT - 1) / BLML_PSD_NT, 1024);
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(double) * 4 * np));
+    double *part = (double *)sc.p;
+    psd_stats_kernel<<<np, BLML_PSD_NT, 0, st>>>(B, K * nc, E, SE, w, n, scale, part);
+    HIPCHECK(hipGetLastError());
+    quick_stats_sum_kernel<<<1, 64, 0, st>>>(part, np, out);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(sc.release());
+    return BLML_OK;
+}
+
+int blml_quick_psd_wscale(const double *SE, const double *w, int64_t B, int32_t K, int32_t nc, double coef,
+                          double *out, void *stream) {
+    if (B < 1 || K < 1 || K > BLML_QUICK_MAXK || (nc != 1 && nc != 2)) return fail(BLML_EVALUE, "B, K, nc");
+    if (!SE || !w || !out) return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    psd_wscale_kernel<<<(unsigned)((B + BLML_PSD_NT - 1) / BLML_PSD_NT), BLML_PSD_NT, 0, st>>>(B, K, nc, SE, w, coef,
+                                                                                            0, out);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+int blml_quick_psd_grad(const blml_pairtab_t *tabs, int32_t ntab, const int32_t *const *Z, int64_t B, int32_t K,
+                        int32_t nc, const double *w, double n, const double *E, const double *SE, double *G,
+                        void *stream) {
+    int rc = psd_check(tabs, ntab, Z, B, K, nc);
+    if (rc) return rc;
+    if (!w || !E || !SE || !G) return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t V = (size_t)B * K * nc;
+    const size_t np = psd_part_len(B, B, ntab == 1 ? K * nc : nc);
+    const int ns = (int)std::min<long>((B + BLML_PSD_NT - 1) / BLML_PSD_NT, 1024);
+    StreamScratch sc(st);
+    // partials, sources, S sources, stats partials, stats
+    HIPCHECK(sc.alloc(sizeof(double) * (np + 2 * V + 4 * (size_t)ns + 4)));
+    double *part = (double *)sc.p, *src = part + np, *Vv = src + V, *spart = Vv + V, *stats = spart + 4 * (size_t)ns;
+    psd_stats_kernel<<<ns, BLML_PSD_NT, 0, st>>>(B, K * nc, E, SE, w, n, 1.0, spart);
+    HIPCHECK(hipGetLastError());
+    quick_stats_sum_kernel<<<1, 64, 0, st>>>(spart, ns, stats);
+    HIPCHECK(hipGetLastError());
+    psd_wscale_kernel<<<(unsigned)((B + BLML_PSD_NT - 1) / BLML_PSD_NT), BLML_PSD_NT, 0, st>>>(B, K, nc, SE, w, 1.0,
+                                                                                            ntab > 1, src);
+    HIPCHECK(hipGetLastError());
+    rc = psd_apply_all(tabs, ntab, Z, B, K, nc, src, src, part, Vv, st);
+    if (rc) return rc;
+    psd_grad_kernel<<<(unsigned)((V + 255) / 256), 256, 0, st>>>((long)V, Vv, SE, stats, n, G);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(sc.release());
+    return BLML_OK;
+}
+
+static int vec_args(int32_t nv, const double *const *X, VecArgs &a) {
+    if (nv < 1 || nv > BLML_VEC_MAXV || !X) return fail(BLML_EVALUE, "vectors: 1 .. " + std::to_string(BLML_VEC_MAXV));
+    a.nv = nv;
+    a.ny = 0;
+    for (int j = 0; j < BLML_VEC_MAXV; ++j) {
+        a.X[j] = j < nv ? X[j] : nullptr;
+        a.c[j] = 0.0;
+        for (int i = 0; i < BLML_VEC_MAXV; ++i) a.A[i][j] = 0.0;
+    }
+    for (int j = 0; j < nv; ++j)
+        if (!X[j]) return fail(BLML_EVALUE, "null vector");
+    return BLML_OK;
+}
+
+int blml_vec_gram(int32_t nv, const double *const *X, int32_t ny, const double *A, int64_t len, double *out,
+                  void *stream) {
+    VecArgs a;
+    int rc = vec_args(nv, X, a);
+    if (rc) return rc;
+    if (ny < 1 || ny > BLML_VEC_MAXV || !A || !out || len < 1) return fail(BLML_EVALUE, "ny, A, out, len");
+    a.ny = ny;
+    for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nv; ++j) a.A[i][j] = A[i * nv + j];
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = (int)std::min<long>((len + BLML_PSD_NT - 1) / BLML_PSD_NT, 256);
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(double) * (size_t)nb * ny * ny));
+    vec_gram_part_kernel<<<nb, BLML_PSD_NT, 0, st>>>(a, len, (double *)sc.p);
+    HIPCHECK(hipGetLastError());
+    vec_gram_sum_kernel<<<1, 64, 0, st>>>((const double *)sc.p, nb, ny, out);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(sc.release());
+    return BLML_OK;
+}
+
+int blml_vec_lincomb(int32_t nv, const double *const *X, const double *c, int32_t mode, int64_t len, double *out,
+                     void *stream) {
+    VecArgs a;
+    int rc = vec_args(nv, X, a);
+    if (rc) return rc;
+    if (!c || !out || len < 1 || (mode != 0 && mode != 1)) return fail(BLML_EVALUE, "c, out, len, mode");
+    for (int j = 0; j < nv; ++j) a.c[j] = c[j];
+    hipStream_t st = (hipStream_t)stream;
+    vec_lincomb_kernel<<<(unsigned)((len + BLML_PSD_NT - 1) / BLML_PSD_NT), BLML_PSD_NT, 0, st>>>(a, mode, len, out);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+int blml_lbfgs_twoloop(int32_t m, const double *const *S, const double *const *Y, const double *rho, double gamma,
+                       int64_t len, const double *g, double *q, void *stream) {
+    if (m < 0 || m > 64 || len < 1 || !g || !q || (m > 0 && (!S || !Y || !rho)))
+        return fail(BLML_EVALUE, "m: 0 .. 64 pairs, len, g, q");
+    for (int i = 0; i < m; ++i)
+        if (!S[i] || !Y[i]) return fail(BLML_EVALUE, "null pair");
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = (int)std::min<long>((len + BLML_PSD_NT - 1) / BLML_PSD_NT, 256);
+    const unsigned ge = (unsigned)((len + 255) / 256);
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(double) * ((size_t)nb + 2 * (size_t)m + 1)));
+    double *part = (double *)sc.p, *al = part + nb, *cf = al + m;
+    HIPCHECK(hipMemcpyAsync(q, g, sizeof(double) * len, hipMemcpyDeviceToDevice, st));
+    for (int i = m - 1; i >= 0; --i) {              // a_i = rho_i s_i.q, q -= a_i y_i
+        vec_dot_part_kernel<<<nb, BLML_PSD_NT, 0, st>>>(S[i], q, len, part);
+        vec_dot_final_kernel<<<1, 64, 0, st>>>(part, nb, rho[i], 0, nullptr, al + i);
+        vec_axpy_dev_kernel<<<ge, 256, 0, st>>>(len, al + i, -1.0, Y[i], q);
+    }
+    HIPCHECK(hipGetLastError());
+    {
+        VecArgs a;
+        const double *X1[1] = {q};
+        int rc = vec_args(1, X1, a);
+        if (rc) return rc;
+        a.c[0] = gamma;
+        vec_lincomb_kernel<<<(unsigned)((len + BLML_PSD_NT - 1) / BLML_PSD_NT), BLML_PSD_NT, 0, st>>>(a, 0, len, q);
+    }
+    for (int i = 0; i < m; ++i) {                   // q += (a_i - rho_i y_i.q) s_i
+        vec_dot_part_kernel<<<nb, BLML_PSD_NT, 0, st>>>(Y[i], q, len, part);
+        vec_dot_final_kernel<<<1, 64, 0, st>>>(part, nb, rho[i], 1, al + i, cf + i);
+        vec_axpy_dev_kernel<<<ge, 256, 0, st>>>(len, cf + i, 1.0, S[i], q);
+    }
     HIPCHECK(hipGetLastError());
     HIPCHECK(sc.release());
     return BLML_OK;

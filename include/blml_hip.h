@@ -298,6 +298,48 @@ int blml_quick_exact(const blml_pairtab_t *tab, const int32_t *Z, int64_t B, con
 int blml_quick_stats(const double *e, const double *Se, const double *w, int64_t B, double n, double *out,
                      void *stream);
 
+/* ---- BLMLQuick's complex / rank-K counts fits (blml_gpu.py:565-744 _psd_solve; blml_trivial.py:2590-2651) ----
+ * E, SE: (B, K, nc) FP64 on the device (nc = 2 interleaved complex, 1 real; K <= BLML_QUICK_MAXK), the layout of
+ * the reference's (B, K) arrays.  tabs: ntab pair tables, 1 (one kernel shared by every mode) or K (per-mode
+ * kernels, mode k's S_k in tabs[k]); Z: ntab host-held pointers to each table's ranked bins (B x d int32, device).
+ * w: counts (B), n = sum w; s: S_k(z, z) per mode (K host doubles). */
+#define BLML_QUICK_MAXK 8
+/* One sweep of _psd_solve (blml_gpu.py:650-679): blocks of Bs bins in the order perm (B int32), each row moved to
+ * its own stationary point (closed form for one table, the secular root for per-mode tables), SE += S[:, I] d;
+ * safeguard != 0: the step the likelihood allows (the first of 2^-k, k < 31, not lowering L). */
+int blml_quick_psd_sweep(const blml_pairtab_t *tabs, int32_t ntab, const int32_t *const *Z, int64_t B, int32_t K,
+                         int32_t nc, const double *w, const int32_t *perm, int64_t Bs, double n, const double *s,
+                         int32_t safeguard, double *E, double *SE, void *stream);
+/* SE = S E from scratch (exact_Se, blml_gpu.py:598-603): FP64 sums. */
+int blml_quick_psd_exact(const blml_pairtab_t *tabs, int32_t ntab, const int32_t *const *Z, int64_t B, int32_t K,
+                         int32_t nc, const double *E, double *SE, void *stream);
+/* out (4 doubles, device) = (Re<E, SE>, sum w log max(F, 1e-300), max residual, non-finite count of E) at
+ * (scale E, scale SE) (loglik and residual, blml_gpu.py:606-620). */
+int blml_quick_psd_stats(const double *E, const double *SE, const double *w, int64_t B, int32_t K, int32_t nc,
+                         double n, double scale, double *out, void *stream);
+/* G = 2 (S ((w / max(F, 1e-300)) SE) - (n / Q) SE), Q = Re<E, SE> (lbfgs_psd's grad, blml_trivial.py:2606-2608). */
+int blml_quick_psd_grad(const blml_pairtab_t *tabs, int32_t ntab, const int32_t *const *Z, int64_t B, int32_t K,
+                        int32_t nc, const double *w, double n, const double *E, const double *SE, double *G,
+                        void *stream);
+/* out = (coef w / max(F, 1e-300)) SE (the canonical form e = (Q / n) (w / F) S e, blml_trivial.py:2645-2647). */
+int blml_quick_psd_wscale(const double *SE, const double *w, int64_t B, int32_t K, int32_t nc, double coef,
+                          double *out, void *stream);
+
+/* FP64 vector work of the Anderson mixing and L-BFGS above (the reference's CuPy array expressions,
+ * blml_gpu.py:705-718, blml_trivial.py:2617-2641).  X: nv <= BLML_VEC_MAXV host-held device pointers of len values.
+ * blml_vec_gram: out (ny x ny, device) = the Gram matrix of Y_i = sum_j A[i * nv + j] X_j (A host, ny <= 8).
+ * blml_vec_lincomb: mode 0 out = sum_j c_j X_j; mode 1 out = X_{nv-1} - sum_{j<nv-1} c_j (X_{j+1} - X_j)
+ * (Anderson's combination); out may be one of the X.
+ * blml_lbfgs_twoloop: q = H g by the two-loop recursion over m pairs (S, Y, rho = 1 / s.y, oldest first, host
+ * rho), the middle scaling gamma; the dots stay on the device. */
+#define BLML_VEC_MAXV 8
+int blml_vec_gram(int32_t nv, const double *const *X, int32_t ny, const double *A, int64_t len, double *out,
+                  void *stream);
+int blml_vec_lincomb(int32_t nv, const double *const *X, const double *c, int32_t mode, int64_t len, double *out,
+                     void *stream);
+int blml_lbfgs_twoloop(int32_t m, const double *const *S, const double *const *Y, const double *rho, double gamma,
+                       int64_t len, const double *g, double *q, void *stream);
+
 /* Which kernel blml_matvec launches for the plan (reporting; no counterpart in the reference): 0 matvec_kernel (one
  * workgroup per fit), 1 matvec_grid_kernel (one fit over the device), 2 matvec_pipe_kernel (one separable fit whose
  * every tile pass is pipelined; it moves 16-byte vectors, so a call whose E or U slices are not 16-byte aligned

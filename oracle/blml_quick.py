@@ -73,16 +73,17 @@ def warm_start(e_prev, new, G, w):
     return e
 
 
-def quick_fit(G, w, tol=1e-4, block_size=4096, seed=0, max_sweeps=None, log_steps=None, e0=None):
+def quick_fit(G, w, tol=1e-4, block_size=4096, seed=0, max_sweeps=None, log_steps=None, e0=None, rng=None):
     """(e, Se, n_sweeps, converged, max_rho, rho_hist, obj_hist, safeguard) of the real rank-1 counts fit with Gram
     matrix G (B x B, the signs already in it: D S D) and counts w (B,).  log_steps: a list that receives
-    (e_before, dlt, t, I) for every safeguarded block step.  e0: the start (warm_start), else ones."""
+    (e_before, dlt, t, I) for every safeguarded block step.  e0: the start (warm_start), else ones.  rng: the
+    generator of the sweeps' permutations (default_rng(seed) when None; a rank-K fit shares one with its start)."""
     G = np.asarray(G, float)
     w = np.asarray(w, float)
     B = len(w)
     n = float(w.sum())
     sbb = float(G[0, 0])
-    rng = np.random.default_rng(seed)
+    rng = np.random.default_rng(seed) if rng is None else rng
     e = np.ones(B) if e0 is None else np.array(e0, float)
     Se = G @ e
     Bs = max(1, min(int(block_size), -(-B // 8)))
@@ -170,3 +171,316 @@ def newton_solve(G, w, tol=1e-13, max_iter=200):
         if np.max(np.abs(t * step) / e) < tol:
             break
     return e
+
+
+# ------------------------------------------------------------------------------------------ complex / rank K
+def psd_start(e_real, K, complex_g, rng, phase_noise=0.1, ws=None):
+    """The start of the rank-K / complex counts fit (blml_gpu.py:575-590): column 0 the real solution with phases
+    exp(i pn N(0, 1)) (complex), or the warm start ws's columns; the missing columns pn |e0| times CN(0, 1) (real:
+    N(0, 1)), drawn from rng in that order."""
+    B = len(e_real) if ws is None else len(ws)
+    e0 = np.zeros((B, K), dtype=complex if complex_g else np.float64)
+    if ws is not None:
+        ws = np.asarray(ws, dtype=complex if complex_g else np.float64)
+        ws = ws[:, None] if ws.ndim == 1 else ws
+        k0 = min(K, ws.shape[1])
+        e0[:, :k0] = ws[:, :k0]
+        er = np.abs(ws[:, 0])
+    else:
+        k0 = 1
+        er = np.asarray(e_real, np.float64)
+        e0[:, 0] = er * (np.exp(1j * phase_noise * rng.standard_normal(B)) if complex_g else 1.0)
+    for k in range(k0, K):
+        noise = ((rng.standard_normal(B) + 1j * rng.standard_normal(B)) / np.sqrt(2) if complex_g
+                 else rng.standard_normal(B))
+        e0[:, k] = phase_noise * er * noise
+    return e0
+
+
+def psd_row_update(R, EI, cc, sbbs, shared):
+    """The block's new rows (blml_gpu.py:622-640): shared kernel e_b = t R_b / |R_b| with
+    t = 2 c / (|R| + sqrt(|R|^2 + 4 s c)) (EI's direction where R_b = 0); per-mode kernels e_bk = R_bk / (lam - s_k),
+    lam = smax + x, x by doubling from 1 (at most 200 times, while phi(x) >= c) then 60 bisections of
+    phi(x) = (smax + x) sum_k |R_bk|^2 / (smax + x - s_k)^2 against c."""
+    if shared:
+        s0 = float(sbbs[0])
+        ar = np.sqrt((np.abs(R) ** 2).sum(1))
+        t = 2.0 * cc / (ar + np.sqrt(ar * ar + 4.0 * s0 * cc))
+        en = np.sqrt((np.abs(EI) ** 2).sum(1))
+        with np.errstate(invalid='ignore', divide='ignore'):
+            dirn = np.where((ar > 0)[:, None], R / np.where(ar > 0, ar, 1.0)[:, None],
+                            EI / np.maximum(en, 1e-30)[:, None])
+        return t[:, None] * dirn
+    a2 = np.abs(R) ** 2
+    s = np.asarray(sbbs, float)
+    smax = float(s.max())
+    lo, hi = np.zeros(len(cc)), np.ones(len(cc))
+    phi = lambda x: (smax + x) * (a2 / (smax + x[:, None] - s[None]) ** 2).sum(1)
+    for _ in range(200):
+        big = phi(hi) >= cc
+        if not big.any():
+            break
+        hi = np.where(big, hi * 2, hi)
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        up = phi(mid) > cc
+        lo = np.where(up, mid, lo)
+        hi = np.where(up, hi, mid)
+    lam = smax + 0.5 * (lo + hi)
+    return R / (lam[:, None] - s[None])
+
+
+def psd_loglik(E, SE, w, n):
+    F = (np.abs(SE) ** 2).sum(1)
+    tr = float(np.real(np.vdot(E, SE)))
+    return float(w @ np.log(np.maximum(F, 1e-300)) - n * np.log(max(tr, 1e-300))) / n
+
+
+def psd_residual(E, SE, w, n):
+    """max_b |S e_b |e_b|^2 / (n w_b) - e_b| / |e_b| (blml_gpu.py:616-620)."""
+    en2 = (np.abs(E) ** 2).sum(1)
+    dev = SE * (en2 / (n * w))[:, None] - E
+    return float(np.max(np.sqrt((np.abs(dev) ** 2).sum(1) / np.maximum(en2, 1e-300))))
+
+
+def psd_apply(Gs, D, cols=None):
+    """S restricted to the columns cols (all bins: None) times D (m, K): one Gram for every mode, or one per mode."""
+    sl = slice(None) if cols is None else cols
+    if len(Gs) == 1:
+        return Gs[0][:, sl] @ D
+    return np.stack([Gs[k][:, sl] @ D[:, k] for k in range(D.shape[1])], 1)
+
+
+def psd_cd(Gs, w, e0, Bs, rng, tol, max_sweeps=None, anderson=5):
+    """Block-Jacobi CD with Anderson mixing and the likelihood safeguard of the rank-K / complex counts fit
+    (blml_gpu.py:591-744) with dense Grams Gs (one: shared kernel, else one per mode).  e0: the start (psd_start).
+    Returns (E, SE, n_sweeps, converged, max_rho, rho_hist, obj_hist, safeguard, aa_accepted, SE_exact)."""
+    w = np.asarray(w, float)
+    n = float(w.sum())
+    B, K = e0.shape
+    shared = len(Gs) == 1
+    sbbs = np.array([float(G[0, 0]) for G in Gs] * (K if shared else 1))
+    E = np.array(e0)
+    SE = psd_apply(Gs, E)
+    sc = n / np.sqrt(max(float(np.real(np.vdot(E, SE))), 1e-300))
+    E *= sc
+    SE *= sc
+    steps = 0.5 ** np.arange(0, 31)
+    perm_fixed = rng.permutation(B) if anderson else None
+    cplx = np.iscomplexobj(E)
+
+    def sweep(safeguard):
+        nonlocal E, SE
+        perm = perm_fixed if perm_fixed is not None else rng.permutation(B)
+        for start in range(0, B, Bs):
+            I = perm[start:start + Bs]
+            EI = E[I]
+            R = SE[I] - sbbs[None] * EI
+            d = psd_row_update(R, EI, n * w[I], sbbs, shared) - EI
+            V = psd_apply(Gs, d, I)
+            if safeguard:
+                a0 = float(np.real(np.vdot(E, SE)))
+                a1 = float(np.real(np.vdot(d, SE[I])) + np.real(np.vdot(E, V)))
+                a2 = float(np.real(np.vdot(d, V[I])))
+                F = (np.abs(SE[None] + steps[:, None, None] * V[None]) ** 2).sum(2)
+                Ls = np.log(np.maximum(F, 1e-300)) @ w - n * np.log(np.maximum(a0 + a1 * steps + a2 * steps ** 2,
+                                                                                1e-300))
+                L0 = float(w @ np.log(np.maximum((np.abs(SE) ** 2).sum(1), 1e-300))) - n * np.log(max(a0, 1e-300))
+                ok = Ls >= L0
+                st = float(steps[int(np.argmax(ok))]) if ok.any() else 0.0
+                E[I] = EI + st * d
+                SE += st * V
+            else:
+                E[I] = EI + d
+                SE += V
+
+    flat = (lambda a: np.concatenate([a.real.ravel(), a.imag.ravel()])) if cplx else (lambda a: a.ravel())
+    hG, hSG, hR = [], [], []
+    aa_acc = 0
+    safeguard, n_sweeps, converged = False, 0, False
+    rho_hist, obj_hist = [], []
+    L_prev = psd_loglik(E, SE, w, n)
+    max_rho = psd_residual(E, SE, w, n)
+    SE_end = None
+    while True:
+        if max_sweeps is not None and n_sweeps >= max_sweeps:
+            break
+        n_sweeps += 1
+        E_save, SE_save = E.copy(), SE.copy()
+        sweep(safeguard)
+        L_now = psd_loglik(E, SE, w, n)
+        if not safeguard and L_now < L_prev - 1e-10 * abs(L_prev):
+            E, SE = E_save, SE_save
+            safeguard = True
+            sweep(True)
+            L_now = psd_loglik(E, SE, w, n)
+        if anderson:
+            hG.append(E.copy())
+            hSG.append(SE.copy())
+            hR.append(flat(E - E_save).astype(np.float64))
+            if len(hG) > anderson + 1:
+                hG.pop(0)
+                hSG.pop(0)
+                hR.pop(0)
+            if len(hG) >= 2:
+                dR = np.stack([hR[j + 1] - hR[j] for j in range(len(hR) - 1)], 1)
+                G = dR.T @ dR
+                b = dR.T @ hR[-1]
+                gam = np.linalg.lstsq(G + 1e-12 * np.trace(G) * np.eye(len(G)), b, rcond=None)[0]
+                E_aa, SE_aa = hG[-1].copy(), hSG[-1].copy()
+                for j, gj in enumerate(gam):
+                    E_aa -= float(gj) * (hG[j + 1] - hG[j])
+                    SE_aa -= float(gj) * (hSG[j + 1] - hSG[j])
+                L_aa = psd_loglik(E_aa, SE_aa, w, n) if np.all(np.isfinite(E_aa)) else -np.inf
+                if L_aa >= L_now:
+                    E, SE, L_now = E_aa, SE_aa, L_aa
+                    aa_acc += 1
+                else:
+                    hG, hSG, hR = hG[-1:], hSG[-1:], hR[-1:]
+        L_prev = L_now
+        if not np.all(np.isfinite(E)):
+            max_rho = np.inf
+            rho_hist.append(max_rho)
+            break
+        max_rho = psd_residual(E, SE, w, n)
+        if max_rho < tol:
+            SE = psd_apply(Gs, E)
+            max_rho = psd_residual(E, SE, w, n)
+        rho_hist.append(max_rho)
+        obj_hist.append(-L_now)
+        if max_rho < tol:
+            converged = True
+            break
+        if (len(rho_hist) > 500 and rho_hist[-1] > 0.99 * min(rho_hist[:-500])
+                and obj_hist[-501] - obj_hist[-1] < 1e-9 * abs(obj_hist[-1])):
+            break
+    if not converged and max_sweeps is not None and np.all(np.isfinite(E)):
+        SE = psd_apply(Gs, E)
+        sc = float(n / np.sqrt(max(float(np.real(np.vdot(E, SE))), 1e-300)))
+        E, SE = E * sc, SE * sc
+        max_rho = psd_residual(E, SE, w, n)
+        converged = max_rho < tol
+        obj_hist.append(-psd_loglik(E, SE, w, n))
+        rho_hist.append(max_rho)
+        SE_end = SE
+    elif converged:
+        SE_end = SE
+    return E, SE, n_sweeps, converged, max_rho, rho_hist, obj_hist, safeguard, aa_acc, SE_end
+
+
+def lbfgs_psd(Gs, w, e0, tol, max_iter, mem):
+    """solver='lbfgs' of the rank-K / complex counts fit (blml_trivial.py:2590-2651, called at blml_gpu.py:605-609)
+    on dense Grams: L-BFGS ascent of L = sum w log F - n log Q from e0 scaled to Q = n^2, Armijo c1 = 1e-4 with at
+    most 40 halvings from the unit step (the first direction 1e-3 |e| / |g| times g), pairs kept with
+    s.y > 1e-12 |s| |y|, one memory reset on a failed line search, the 1e-11 ll stop over 10 iterations, then the
+    canonical form e = (Q / n) (w / F) S e kept unless it loses likelihood.  Returns
+    (E, SE, iterations, converged, max_rho, rho_hist, obj_hist)."""
+    w = np.asarray(w, float)
+    n = float(w.sum())
+    SE_of = lambda e: psd_apply(Gs, e)
+    dot = lambda a, b: float(np.real(np.vdot(a, b)))
+    loglik = lambda e, Se: psd_loglik(e, Se, w, n)
+    residual = lambda e, Se: psd_residual(e, Se, w, n)
+    e = np.array(e0)
+    Se = SE_of(e)
+    sc = n / np.sqrt(max(dot(e, Se), 1e-300))
+    e, Se = e * sc, Se * sc
+
+    def grad(e, Se):
+        F = (np.abs(Se) ** 2).sum(1)
+        Q = dot(e, Se)
+        return 2 * (SE_of((w / np.maximum(F, 1e-300))[:, None] * Se) - (n / Q) * Se)
+
+    def nrm(e, Se):
+        s = n / np.sqrt(max(dot(e, Se), 1e-300))
+        return e * s, Se * s
+
+    L = loglik(e, Se) * n
+    g = grad(e, Se)
+    S_, Y_, R_ = [], [], []
+    it = 0
+    cap = np.inf if max_iter is None else max_iter
+    max_rho = residual(*nrm(e, Se))
+    rho_hist, obj_hist = [max_rho], [-L / n]
+    reset = llstop = False
+    while max_rho >= tol and it < cap:
+        if len(obj_hist) > 10 and obj_hist[-11] - obj_hist[-1] < 1e-11:
+            llstop = True
+            break
+        it += 1
+        q = g.copy()
+        al = []
+        for s_, y_, r_ in zip(reversed(S_), reversed(Y_), reversed(R_)):
+            a = r_ * dot(s_, q)
+            al.append(a)
+            q = q - a * y_
+        q = q * (dot(S_[-1], Y_[-1]) / dot(Y_[-1], Y_[-1]) if S_ else 1e-3 * np.sqrt(dot(e, e) / max(dot(g, g), 1e-300)))
+        for (s_, y_, r_), a in zip(zip(S_, Y_, R_), reversed(al)):
+            q = q + (a - r_ * dot(y_, q)) * s_
+        slope = dot(g, q)
+        if not slope > 0:
+            S_, Y_, R_ = [], [], []
+            q = g * (1e-3 * np.sqrt(dot(e, e) / max(dot(g, g), 1e-300)))
+            slope = dot(g, q)
+        t = 1.0
+        ok = False
+        for _ in range(40):
+            en = e + t * q
+            Sen = SE_of(en)
+            Ln = loglik(en, Sen) * n
+            if Ln >= L + 1e-4 * t * slope:
+                ok = True
+                break
+            t *= 0.5
+        if not ok:
+            if reset or not S_:
+                break
+            S_, Y_, R_ = [], [], []
+            reset = True
+            continue
+        reset = False
+        gn = grad(en, Sen)
+        s_, y_ = en - e, g - gn
+        sy = dot(s_, y_)
+        if sy > 1e-12 * np.sqrt(dot(s_, s_) * dot(y_, y_)):
+            S_.append(s_)
+            Y_.append(y_)
+            R_.append(1.0 / sy)
+            if len(S_) > mem:
+                S_.pop(0)
+                Y_.pop(0)
+                R_.pop(0)
+        e, Se, L, g = en, Sen, Ln, gn
+        max_rho = residual(*nrm(e, Se))
+        rho_hist.append(max_rho)
+        obj_hist.append(-L / n)
+    F = (np.abs(Se) ** 2).sum(1)
+    Q = dot(e, Se)
+    ec = (Q / n) * (w / np.maximum(F, 1e-300))[:, None] * Se
+    Sec = SE_of(ec)
+    if loglik(ec, Sec) * n >= L - 1e-9 * n:
+        e, Se = ec, Sec
+    e, Se = nrm(e, Se)
+    max_rho = residual(e, Se)
+    converged = max_rho < tol or llstop
+    rho_hist.append(max_rho)
+    obj_hist.append(-loglik(e, Se))
+    return e, Se, it, converged, max_rho, rho_hist, obj_hist
+
+
+def psd_fit(Gs, w, tol=1e-4, block_size=4096, seed=0, max_sweeps=None, solver='cd', lbfgs_m=20, anderson=5,
+            phase_noise=0.1, K=1, complex_g=False, ws=None):
+    """The whole rank-K / complex counts fit (blml_gpu.py:332-441 then 461-486 with n_starts = (0, 0)): the real
+    rank-1 fit under mode 0's Gram (quick_fit, whose sweeps draw from the same default_rng(seed) first), then
+    psd_start and psd_cd or lbfgs_psd.  ws: a warm start (B, K') replacing the real start's column 0."""
+    G0 = np.asarray(Gs[0], float)
+    w = np.asarray(w, float)
+    B = len(w)
+    rng = np.random.default_rng(seed)
+    Bs = max(1, min(int(block_size), -(-B // 8)))
+    e_real = quick_fit(G0, w, tol, block_size, max_sweeps=max_sweeps, rng=rng)[0]
+    e0 = psd_start(e_real, K, complex_g, rng, phase_noise, ws)
+    if solver == 'lbfgs':
+        return lbfgs_psd(Gs, w, e0, tol, max_sweeps, lbfgs_m)
+    return psd_cd(Gs, w, e0, Bs, rng, tol, max_sweeps, anderson)
+

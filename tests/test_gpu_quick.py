@@ -140,7 +140,7 @@ def test_capped_fit_and_sqrt_density():
 def test_warm_start_matches_oracle():
     """init= on the same lattice (FP64 tables): a second fit on other bins (some gone, some new, counts moved) warm-
     started from the first equals the oracle's quick_fit from warm_start(the first fit's e) -- the same sweeps, e to
-    1e-9 -- and reaches the cold fit's solution; a fit on another lattice raises NotImplementedError."""
+    1e-9 -- and reaches the cold fit's solution (warm starts from another lattice: test_gpu_quick_psd.py)."""
     from blml_amd.quick import BLMLQuick
     name, shape, h, cols, units, bs, seed, signed = CASES[1]
     Z, w = _problem(shape, 8000, seed, zero_frac=0.0)
@@ -165,5 +165,3 @@ def test_warm_start_matches_oracle():
     assert np.abs(q2.e_hat_ - e).max() < 1e-9 * np.abs(e).max()
     cold = BLMLQuick(**kw).fit(Z2 * hv, _modes(cols, hv), sample_weight=w2)
     assert np.abs(q2.e_hat_ - cold.e_hat_).max() < 1e-6 * np.abs(cold.e_hat_).max()
-    with pytest.raises(NotImplementedError, match='another lattice'):
-        BLMLQuick(**kw).fit(Z2 * hv * 2, _modes(cols, hv * 2), sample_weight=w2, init=q1)

@@ -1,0 +1,176 @@
+"""BLMLQuick's complex / rank-K counts fits on the GPU (blml_amd/quick.py, libblml_hip.so blml_quick_psd_* and
+blml_vec_* / blml_lbfgs_twoloop) against the oracle's restatement of _psd_solve / lbfgs_psd (oracle/blml_quick.py)
+with dense Gram matrices of the same kernels (oracle/blml_pairs.py):
+- FP64 tables, one start (n_starts = (0, 0), as the reference's own parity check, results/binned_parity.py:52),
+  for a shared and a per-mode kernel, real rank 2 and complex, 1-D sinc, 2-D sinc x dirichlet and a 2-D periodic
+  ball, CD and L-BFGS: the same draws from default_rng(seed), so the same path -- the residual histories agree to
+  1e-6 (or 1e-11 absolute) entry by entry; where they agree to the end, the same count and stop, the likelihood to 1e-10 and E to 1e-7
+  of its scale.  Where rounding moves the stop (a secular root's bisection or an L-BFGS direction turning on the
+  last bits, then the stall rule or the 1e-11 likelihood stop firing a few iterations apart), the paths agree for
+  at least the first quarter (up to 100 entries), the stop kind is the same, the likelihood agrees to 1e-10
+  (converged CD) or 1e-8, and the density at the bins to 1e-3 (E's components in the near-null space of S stay
+  where each path left them, as lbfgs_psd's docstring says);
+- FP32 tables: the likelihood within 1e-6 of the FP64 oracle's;
+- sqrt_density at the bins is Se / n (the exact kernel at lattice points); a complex fit with the start search
+  asked for raises NotImplementedError;
+- a real fit warm-started from a fit on another lattice (re-binned through the exact kernel) reaches the cold
+  fit's fixed point."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import blml_pairs as OP
+from oracle import blml_quick as OQ
+
+pytestmark = pytest.mark.gpu
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _hist(shape, total, f, seed=1):
+    rng = np.random.default_rng(seed)
+    g = np.meshgrid(*[np.arange(m, dtype=float) for m in shape], indexing='ij')
+    p = f(*g)
+    N = rng.multinomial(total, (p / p.sum()).ravel()).reshape(shape).astype(float)
+    Z = np.stack([x.ravel() for x in np.meshgrid(*[np.arange(m) for m in shape], indexing='ij')], 1)
+    return Z, N.ravel()
+
+
+c1 = lambda x: np.exp(-0.5 * ((x - 14) / 3) ** 2) + 0.5 * np.exp(-0.5 * ((x - 27) / 2) ** 2) + 0.02
+c2 = lambda x, y: (np.exp(-0.5 * (((x - 11) / 2.5) ** 2 + ((y - 13) / 3) ** 2))
+                   + 0.4 * np.exp(-0.5 * (((x - 17) / 2) ** 2 + ((y - 6) / 2) ** 2)) + 0.01)
+c3 = lambda x, y: np.exp(np.cos(2 * np.pi * x / 16) + 0.6 * np.sin(2 * np.pi * y / 16)) + 0.1
+
+# name, shape, density, product columns per mode (without h), oracle units per mode, complex
+CASES = [
+    ('1d sinc complex', (40,), c1, [[dict(fc=0.35)]], [[('offset', 'sinc', [0.35], [0.0], [0])]], True),
+    ('1d sinc rank2 shared complex', (40,), c1, [[dict(fc=0.35)]] * 2, [[('offset', 'sinc', [0.35], [0.0], [0])]],
+     True),
+    ('1d sinc rank2 shared real', (40,), c1, [[dict(fc=0.35)]] * 2, [[('offset', 'sinc', [0.35], [0.0], [0])]],
+     False),
+    ('1d sinc rank2 per-mode complex', (40,), c1, [[dict(fc=0.35)], [dict(fc=0.12)]],
+     [[('offset', 'sinc', [0.35], [0.0], [0])], [('offset', 'sinc', [0.12], [0.0], [0])]], True),
+    ('2d sinc x dirichlet complex', (24, 24), c2, [[dict(fc=0.4), dict(kind='dirichlet', fc=0.3, period=24.0)]],
+     [[('offset', 'sinc', [0.4], [0.0], [0]), ('circular', 'dirichlet', [0.3], [24.0], [1])]], True),
+    ('2d periodic ball rank2 per-mode complex', (16, 16), c3,
+     [[{(0, 1): dict(kind='pball', fc=0.4, period=16.0)}], [{(0, 1): dict(kind='pball', fc=0.2, period=16.0)}]],
+     [[('torus', 'pball', [0.4, 0.4], [16.0, 16.0], [0, 1])], [('torus', 'pball', [0.2, 0.2], [16.0, 16.0], [0, 1])]],
+     True),
+]
+
+
+def _modes(cols_per_mode):
+    out = []
+    for cols in cols_per_mode:
+        if len(cols) == 1 and isinstance(cols[0], dict) and any(isinstance(k, tuple) for k in cols[0]):
+            out.append({key: dict(c, h=1.0) for key, c in cols[0].items()})
+        else:
+            out.append({j: dict(c, h=1.0) for j, c in enumerate(cols)})
+    return out
+
+
+def _fit_both(case, solver, tol, dtype):
+    from blml_amd.quick import BLMLQuick
+    name, shape, f, cols, units, cplx = case
+    Z, w = _hist(shape, 3000 if len(shape) == 1 else 8000, f)
+    keep = w > 0
+    K = len(cols)
+    _note(f'psd quick: {name} ({solver}, {np.dtype(dtype).name}), {int(keep.sum())} bins, K = {K}')
+    q = BLMLQuick(tol=tol, dtype=dtype, solver=solver, seed=0)
+    q.n_starts = (0, 0)
+    q.fit(Z.astype(float), _modes(cols), sample_weight=w, complex_g=cplx)
+    Gs = [OP.dense_pair_gram(u, np.ones(len(shape)), Z[keep], Z[keep]) for u in units]
+    o = OQ.psd_fit(Gs, w[keep], tol=tol, seed=0, solver=solver, K=K, complex_g=cplx)
+    return q, o, Gs, w[keep]
+
+
+def _split(a, b, rtol):
+    """The first index where two residual histories differ by more than rtol (their common length if none); an
+    absolute 1e-11 as well: near convergence the residual is a cancellation (S e |e|^2 / (n w) - e at 1e-7 of e),
+    whose relative rounding is about 1e-5."""
+    m = min(len(a), len(b))
+    bad = np.flatnonzero(~np.isclose(np.asarray(a[:m]), np.asarray(b[:m]), rtol=rtol, atol=1e-11))
+    return int(bad[0]) if len(bad) else m
+
+
+@pytest.mark.parametrize('solver', ['cd', 'lbfgs'])
+@pytest.mark.parametrize('case', CASES, ids=[c[0] for c in CASES])
+def test_psd_fit_matches_oracle_fp64(case, solver):
+    q, o, Gs, w = _fit_both(case, solver, 1e-7, np.float64)
+    n = w.sum()
+    Eo, SEo, its, conv = o[0], o[1], o[2], o[3]
+    E = q.e_hat_ if q.e_hat_.ndim == 2 else q.e_hat_[:, None]
+    SEx = OQ.psd_apply(Gs, E)
+    l_q, l_o = OQ.psd_loglik(E, SEx, w, n), OQ.psd_loglik(Eo, OQ.psd_apply(Gs, Eo), w, n)
+    fq, fo = (np.abs(SEx) ** 2).sum(1), (np.abs(OQ.psd_apply(Gs, Eo)) ** 2).sum(1)
+    split = _split(q.rho_history_, o[5], 1e-6)
+    _note(f'   engine: {q.n_sweeps_} {"sweeps" if solver == "cd" else "iterations"}, converged {q.converged_}, '
+          f'residual {q.residual_:.3e}, loglik {l_q:.12f}; oracle {its}, {conv}, {o[4]:.3e}, {l_o:.12f}; '
+          f'max |dE| / max |E| {np.abs(E - Eo).max() / np.abs(Eo).max():.2e}, density max rel '
+          f'{np.abs(fq / fo - 1).max():.2e}; residual histories equal to 1e-6 for the first {split} entries')
+    assert q.converged_ == conv
+    if q.n_sweeps_ == its:            # the same path to the end
+        assert split == len(o[5]) and abs(l_q - l_o) < 1e-10 * abs(l_o)
+        assert np.abs(E - Eo).max() < 1e-7 * np.abs(Eo).max()
+    else:                             # rounding moved the stop: the same path for a long stretch, the same answer
+        assert split >= min(100, its // 4), split
+        assert abs(l_q - l_o) < (1e-10 if conv and solver == 'cd' else 1e-8) * abs(l_o)
+        assert np.abs(fq / fo - 1).max() < 1e-3
+    if q.Se_ is not None:
+        Se = q.Se_ if q.Se_.ndim == 2 else q.Se_[:, None]
+        assert np.abs(Se - SEx).max() < 1e-9 * np.abs(SEx).max()
+
+
+@pytest.mark.parametrize('case', [CASES[0], CASES[3], CASES[4]], ids=[CASES[i][0] for i in (0, 3, 4)])
+def test_psd_fit_fp32_tables(case):
+    q, o, Gs, w = _fit_both(case, 'cd', 1e-6, np.float32)
+    n = w.sum()
+    E = q.e_hat_ if q.e_hat_.ndim == 2 else q.e_hat_[:, None]
+    l_q = OQ.psd_loglik(E, OQ.psd_apply(Gs, E), w, n)
+    l_o = OQ.psd_loglik(o[0], OQ.psd_apply(Gs, o[0]), w, n)
+    _note(f'   FP32 tables: loglik {l_q:.10f} against the FP64 oracle {l_o:.10f} ({q.n_sweeps_} / {o[2]} sweeps)')
+    assert q.converged_
+    assert abs(l_q - l_o) < 1e-6 * abs(l_o)
+
+
+def test_psd_queries_and_start_search():
+    from blml_amd.quick import BLMLQuick
+    case = CASES[1]
+    Z, w = _hist((40,), 3000, c1)
+    q = BLMLQuick(tol=1e-7, dtype=np.float64)
+    q.n_starts = (0, 0)
+    q.fit(Z.astype(float), _modes(case[3]), sample_weight=w, complex_g=True)
+    g = q.sqrt_density(q.Z_)
+    assert g.shape == (len(q.Z_), 2) and np.iscomplexobj(g)
+    assert np.abs(g - q.Se_ / q.n_samples_).max() < 1e-10 * np.abs(g).max()
+    f = q.predict(q.Z_ + 0.37)
+    assert f.shape == (len(q.Z_),) and np.all(f >= 0)
+    with pytest.raises(NotImplementedError):
+        BLMLQuick(tol=1e-6).fit(Z.astype(float), _modes(case[3]), sample_weight=w, complex_g=True)
+
+
+def test_rebinned_warm_start_reaches_the_cold_fit():
+    """A fit on a 2-unit lattice warm-starts one on the 1-unit lattice (re-binned: e = w / g_prev at the new bins);
+    the real rank-1 problem is convex, so it ends at the cold fit's fixed point."""
+    from blml_amd.quick import BLMLQuick
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.normal(30, 5, 4000), rng.normal(55, 3, 2000)])
+    mk = lambda h: [{0: dict(fc=0.15, h=h)}]
+
+    def binned(h):
+        zi, cnt = np.unique(np.round(x / h).astype(int), return_counts=True)
+        return (zi * h).astype(float)[:, None], cnt.astype(float)
+
+    Z2, w2 = binned(2.0)
+    Z1, w1 = binned(1.0)
+    coarse = BLMLQuick(tol=1e-9, dtype=np.float64).fit(Z2, mk(2.0), sample_weight=w2)
+    warm = BLMLQuick(tol=1e-9, dtype=np.float64).fit(Z1, mk(1.0), sample_weight=w1, init=coarse)
+    cold = BLMLQuick(tol=1e-9, dtype=np.float64).fit(Z1, mk(1.0), sample_weight=w1)
+    _note(f'rebinned warm start: {warm.n_sweeps_} sweeps ({warm.n_new_bins_} bins started at their 1-D root) '
+          f'against {cold.n_sweeps_} cold')
+    assert warm.rebinned_ and warm.converged_ and cold.converged_
+    assert np.abs(warm.e_hat_ - cold.e_hat_).max() < 1e-6 * np.abs(cold.e_hat_).max()

@@ -115,3 +115,124 @@ def test_warm_start_reaches_the_convex_minimiser():
     en = OQ.newton_solve(G2, w2)
     assert warm[3] and np.abs(warm[0] - en).max() < 1e-7 * np.abs(en).max()
     assert warm[2] < cold[2], (warm[2], cold[2])
+
+
+# ------------------------------------------------------------------------------------------ complex / rank K
+def _hist1d(total, seed):
+    """The 40-bin two-bump histogram of the reference's parity cells (results/binned_parity.py:21-26)."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(40, dtype=float)
+    p = np.exp(-0.5 * ((x - 14) / 3) ** 2) + 0.5 * np.exp(-0.5 * ((x - 27) / 2) ** 2) + 0.02
+    return rng.multinomial(total, p / p.sum()).astype(float)
+
+
+def _grams(fcs, Z):
+    return [OP.dense_pair_gram([('offset', 'sinc', [f], [0.0], [0])], np.ones(1), Z, Z) for f in fcs]
+
+
+def _stat_resid(Gs, E, w):
+    """max_b |(S e)_b |e_b|^2 / (n w_b) - e_b| / |e_b| under the dense Grams, written from the stationarity
+    condition (S_k e_k)_b = n w_b e_bk / |e_b|^2 (results/binned_parity.py:64-70), not from the oracle's code."""
+    n = w.sum()
+    U = np.stack([(Gs[0] if len(Gs) == 1 else Gs[k]) @ E[:, k] for k in range(E.shape[1])], 1)
+    nr = (np.abs(E) ** 2).sum(1)
+    return float(np.max(np.sqrt((np.abs(U * (nr / (n * w))[:, None] - E) ** 2).sum(1)) / np.sqrt(nr)))
+
+
+@pytest.mark.parametrize('shared', [True, False])
+def test_psd_row_update_is_row_stationary(shared):
+    """The new row solves its own stationarity condition R_bk + s_k e_bk = n w_b e_bk / |e_b|^2 (R = S e - s e
+    with the row's own term removed): the closed form for one kernel, the secular root for per-mode kernels."""
+    rng = np.random.default_rng(4)
+    m, K = 300, 3
+    R = rng.normal(size=(m, K)) + 1j * rng.normal(size=(m, K))
+    EI = rng.normal(size=(m, K)) + 1j * rng.normal(size=(m, K))
+    cc = rng.uniform(0.5, 80.0, m)
+    s = np.full(K, 0.7) if shared else np.array([0.7, 0.31, 0.52])
+    e = OQ.psd_row_update(R, EI, cc, s, shared)
+    lam = cc / (np.abs(e) ** 2).sum(1)
+    assert np.abs(R + s[None] * e - lam[:, None] * e).max() < 1e-9 * np.abs(R).max()
+    # a row with R = 0 keeps its direction (shared kernel)
+    if shared:
+        e0 = OQ.psd_row_update(np.zeros((1, K), complex), EI[:1], cc[:1], s, True)
+        assert np.allclose(e0 / np.linalg.norm(e0), EI[:1] / np.linalg.norm(EI[:1]))
+        assert abs(0.7 * np.linalg.norm(e0) ** 2 - cc[0]) < 1e-9 * cc[0]
+
+
+PSD_CASES = [
+    ('d1 sinc complex', [0.35], 1, True),
+    ('d1 sinc rank2 shared complex', [0.35], 2, True),
+    ('d1 sinc rank2 shared real', [0.35], 2, False),
+    ('d1 sinc rank2 per-mode complex', [0.35, 0.12], 2, True),
+]
+
+
+@pytest.mark.parametrize('name,fcs,K,cplx', PSD_CASES, ids=[c[0] for c in PSD_CASES])
+@pytest.mark.parametrize('solver', ['cd', 'lbfgs'])
+def test_psd_fit_is_stationary(name, fcs, K, cplx, solver):
+    w = _hist1d(3000, 1)
+    keep = w > 0
+    Z = np.flatnonzero(keep)[:, None]
+    w = w[keep]
+    Gs = _grams(fcs, Z)
+    out = OQ.psd_fit(Gs, w, tol=1e-6, block_size=4096, seed=0, solver=solver, K=K, complex_g=cplx)
+    E, SE, conv = out[0], out[1], out[3]
+    n = w.sum()
+    assert conv, (name, solver, out[2], out[4])
+    assert E.shape == (len(w), K) and np.iscomplexobj(E) == cplx
+    SEx = OQ.psd_apply(Gs, E)
+    assert np.abs(SE - SEx).max() < 1e-9 * np.abs(SEx).max()
+    assert abs(np.real(np.vdot(E, SEx)) - n * n) < 1e-6 * n * n     # stationarity: e'Se = sum n w = n^2
+    if solver == 'cd':
+        assert _stat_resid(Gs, E, w) < 1e-5
+    else:            # the likelihood stop: the near-null directions of S can keep the residual above tol
+        assert _stat_resid(Gs, E, w) < 1e-2
+
+
+@pytest.mark.parametrize('name,fcs,K,cplx', PSD_CASES, ids=[c[0] for c in PSD_CASES])
+def test_psd_cd_and_lbfgs_reach_the_same_likelihood(name, fcs, K, cplx):
+    w = _hist1d(3000, 1)
+    keep = w > 0
+    Z = np.flatnonzero(keep)[:, None]
+    w = w[keep]
+    Gs = _grams(fcs, Z)
+    a = OQ.psd_fit(Gs, w, tol=1e-8, seed=0, solver='cd', K=K, complex_g=cplx)
+    b = OQ.psd_fit(Gs, w, tol=1e-8, seed=0, solver='lbfgs', K=K, complex_g=cplx)
+    la = OQ.psd_loglik(a[0], a[1], w, w.sum())
+    lb = OQ.psd_loglik(b[0], b[1], w, w.sum())
+    assert abs(la - lb) < 1e-7, (name, la, lb)
+
+
+def test_complex_fit_not_below_the_real_fit():
+    """f = |g|^2 over complex g contains the real fits: the complex optimum's likelihood is at least the real
+    rank-1 fit's."""
+    w = _hist1d(3000, 1)
+    keep = w > 0
+    Z = np.flatnonzero(keep)[:, None]
+    w = w[keep]
+    Gs = _grams([0.35], Z)
+    e, Se = OQ.quick_fit(Gs[0], w, tol=1e-10)[:2]
+    n = w.sum()
+    l_real = float(w @ np.log(Se ** 2) - n * np.log(e @ Se)) / n
+    E, SE = OQ.psd_fit(Gs, w, tol=1e-9, complex_g=True)[:2]
+    l_c = OQ.psd_loglik(E, SE, w, n)
+    assert l_c >= l_real - 1e-9
+
+
+@pytest.mark.parametrize('name,fcs,K,cplx', [c for c in PSD_CASES if c[3]], ids=[c[0] for c in PSD_CASES if c[3]])
+def test_psd_counts_fit_matches_the_binned_oracle(name, fcs, K, cplx):
+    """The reference's own parity check (results/binned_parity.py:52-57): BLMLBinned on the full lattice and the
+    counts fit on its occupied bins (sinc: the lattice tables are exact, so both have the same Gram) reach the same
+    likelihood and density at the bins."""
+    from oracle import blml_oracle as O
+    N = _hist1d(3000, 1)
+    keep = N > 0
+    Z = np.flatnonzero(keep)[:, None]
+    w = N[keep]
+    modes = O.make_modes([[f] for f in fcs] if len(fcs) > 1 else fcs, ['sinc'], rank=K if len(fcs) == 1 else 1)
+    mb = O.BLMLBinnedOracle(tol=1e-7).fit(N, modes, complex_g=cplx)
+    E, SE = OQ.psd_fit(_grams(fcs, Z), w, tol=1e-7, K=K, complex_g=cplx)[:2]
+    n = w.sum()
+    fq = (np.abs(SE) ** 2).sum(1) / n ** 2
+    assert abs(OQ.psd_loglik(E, SE, w, n) - mb.loglik_) < 1e-6        # both sum w log(F / Q) / n
+    assert np.abs(fq / np.asarray(mb.density_)[keep] - 1).max() < 1e-4
