@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <limits>
 #include <mutex>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -240,6 +241,10 @@ int build_sep(Plan &pl, const double *fdata, const int32_t *freqs, int max_lds) 
     return BLML_OK;
 }
 
+// the dynamic LDS of the batched launches: the line buffers, or with resident intermediates their offset + size
+static int lds_mv_launch(const Plan &pl) { return pl.res_mv ? pl.res_off + pl.res_mv : pl.lds_bytes; }
+static int lds_fit_launch(const Plan &pl) { return pl.res_fit ? pl.res_off + pl.res_fit : pl.lds_bytes; }
+
 template <typename T>
 int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freqs, int64_t n_freqs) {
     std::vector<int> fac_m;
@@ -321,10 +326,32 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
     HIPCHECK(hipFree(d_fac_m));
     HIPCHECK(hipFree(d_tmp_off));
     HIPCHECK(hipFree(d_tmp));
+    // LDS-resident intermediates of the batched kernels: after the line buffers, as much as the workgroup's LDS
+    // leaves; the launch reserves the largest need among the fits that fit (the others stay in the workspace).
+    // BLML_LDS_RESIDENT (environment) names the kernels that use it: 'm' matvec_kernel, 'f' fit_kernel (default
+    // both; "0" neither).
+    {
+        const char *env = getenv("BLML_LDS_RESIDENT");
+        const std::string use = env ? env : "mf";
+        int st_mvk = 0, st_fitk = 0;
+        rc = static_lds((const void *)matvec_kernel<T>, st_mvk);
+        if (!rc) rc = static_lds((const void *)fit_kernel<T>, st_fitk);
+        if (rc) return rc;
+        pl.res_off = (pl.lds_bytes + 15) / 16 * 16;
+        const long cap_mv = (long)max_lds - st_mvk - pl.res_off, cap_fit = (long)max_lds - st_fitk - pl.res_off;
+        long need_mv = 0, need_fit = 0;
+        for (const blml_fit_t &f : pl.hfits) {
+            const long need = res_need(f, pl.hfacs.data(), (int)sizeof(T));
+            if (need <= cap_mv) need_mv = std::max(need_mv, need);
+            if (need <= cap_fit) need_fit = std::max(need_fit, need);
+        }
+        pl.res_mv = use.find('m') != std::string::npos ? (int)need_mv : 0;
+        pl.res_fit = use.find('f') != std::string::npos ? (int)need_fit : 0;
+    }
     // persistent grid: as many workgroups as fit on the device at once (capped by the number of fits)
     int ncu = 0, occ = 0;
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fit_kernel<T>, BLML_NT, pl.lds_bytes));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fit_kernel<T>, BLML_NT, lds_fit_launch(pl)));
     int khz = 0;
     HIPCHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     if (khz > 0) pl.tick_hz = 1e3 * khz;
@@ -474,12 +501,17 @@ int blml_matvec(void *plan, const void *E, void *U, void *stream) {
                                      : (const void *)matvec_grid_kernel<double>, args,
                            pipe ? pl->lds_pipe : pl->lds_mv, st);
     }
-    if (pl->dtype == BLML_F32)
-        matvec_kernel<float><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<float>(*pl), (const float *)E,
-                                                                          (float *)U);
-    else
-        matvec_kernel<double><<<pl->nfits, BLML_NT, pl->lds_bytes, st>>>(dev_plan<double>(*pl), (const double *)E,
-                                                                           (double *)U);
+    if (pl->dtype == BLML_F32) {
+        DevPlan<float> dp = dev_plan<float>(*pl);
+        dp.res_off = pl->res_off;
+        dp.res_cap = pl->res_mv;
+        matvec_kernel<float><<<pl->nfits, BLML_NT, lds_mv_launch(*pl), st>>>(dp, (const float *)E, (float *)U);
+    } else {
+        DevPlan<double> dp = dev_plan<double>(*pl);
+        dp.res_off = pl->res_off;
+        dp.res_cap = pl->res_mv;
+        matvec_kernel<double><<<pl->nfits, BLML_NT, lds_mv_launch(*pl), st>>>(dp, (const double *)E, (double *)U);
+    }
     HIPCHECK(hipGetLastError());
     return BLML_OK;
 }
@@ -514,15 +546,21 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
     }
     HIPCHECK(hipMemsetAsync(pl->d_queue, 0, sizeof(int), st));
     if (pl->dtype == BLML_F32) {
-        fit_kernel<float><<<pl->grid, BLML_NT, pl->lds_bytes, st>>>(
-            dev_plan<float>(*pl), pl->d_order, pl->d_queue, counts, draws, init, signs, (float *)e_out,
+        DevPlan<float> dp = dev_plan<float>(*pl);
+        dp.res_off = pl->res_off;
+        dp.res_cap = pl->res_fit;
+        fit_kernel<float><<<pl->grid, BLML_NT, lds_fit_launch(*pl), st>>>(
+            dp, pl->d_order, pl->d_queue, counts, draws, init, signs, (float *)e_out,
             (float *)se_out, density, results, merit_hist, rho_hist, pl->tick_hz);
         HIPCHECK(hipGetLastError());
         finish64_kernel<<<pl->nfits, BLML_NT, pl->lds64_bytes, st>>>(dev_plan64(*pl), (float *)se_out, counts,
                                                                      density, results);
     } else {
-        fit_kernel<double><<<pl->grid, BLML_NT, pl->lds_bytes, st>>>(
-            dev_plan<double>(*pl), pl->d_order, pl->d_queue, counts, draws, init, signs, (double *)e_out,
+        DevPlan<double> dp = dev_plan<double>(*pl);
+        dp.res_off = pl->res_off;
+        dp.res_cap = pl->res_fit;
+        fit_kernel<double><<<pl->grid, BLML_NT, lds_fit_launch(*pl), st>>>(
+            dp, pl->d_order, pl->d_queue, counts, draws, init, signs, (double *)e_out,
             (double *)se_out, density, results, merit_hist, rho_hist, pl->tick_hz);
     }
     HIPCHECK(hipGetLastError());

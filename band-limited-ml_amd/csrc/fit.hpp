@@ -23,6 +23,7 @@ struct DevPlan {
     const SepPlan *sep;    // spread plans of separable modes: [0] for the mat-vec kernel, [1] for the fit kernel
     const T *sep_tab;
     const long *sep_off;
+    int res_off, res_cap;  // LDS-resident intermediates: byte offset in the dynamic LDS and its size (0: none)
 };
 
 // A fit's workspace (bytes from ws_off): [X, Y: scratch_len complex values of the transform type]
@@ -36,6 +37,14 @@ __host__ __device__ inline int n_slots(const blml_params_t &P, const blml_fit_t 
 }
 __host__ __device__ inline long pool_bytes_off(const blml_fit_t &f, int esize) { return align256(4L * f.scratch_len * esize); }
 __host__ __device__ inline long vmax_of(const blml_fit_t &f) { return (long)f.K * f.B * (f.complex_g ? 2 : 1); }
+// LDS bytes a fit's transform intermediates need to stay in LDS: X (scratch_len complex values), and Y as well when
+// a mode has a multi-axis (GROUP) factor
+__host__ __device__ inline long res_need(const blml_fit_t &f, const blml_factor_t *facs, int esize) {
+    bool grp = false;
+    for (int k = 0; k < f.K; ++k)
+        for (int q = 0; q < f.fac_count[k]; ++q) grp = grp || facs[f.fac_begin[k] + q].kind != BLML_FAC_LINE;
+    return 2L * esize * f.scratch_len * (grp ? 2 : 1);
+}
 __host__ __device__ inline long fin_bytes_off(const blml_params_t &P, const blml_fit_t &f, int esize) {
     return align256(pool_bytes_off(f, esize) + 8L * n_slots(P, f) * vmax_of(f));
 }
@@ -116,6 +125,12 @@ __device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, Matvec
     mv.Lmax = p.Lmax;
     mv.X = (cpx<T> *)(p.ws + f.ws_off);
     mv.Y = mv.X + f.scratch_len;
+    // a fit run by one workgroup whose intermediates fit the launch's resident LDS keeps them there: every pass
+    // then stages its lines LDS to LDS, and only E is read and U written in HBM
+    if (p.res_cap > 0 && !sep && tm.nranks == 1 && res_need(f, p.facs, (int)sizeof(T)) <= p.res_cap) {
+        mv.X = reinterpret_cast<cpx<T> *>(g_dyn_lds + p.res_off);
+        mv.Y = mv.X + f.scratch_len;
+    }
 }
 
 template <typename T>

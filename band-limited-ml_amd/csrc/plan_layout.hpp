@@ -52,6 +52,8 @@ struct Plan {
     long *d_sep_off = nullptr;
     int lds_mv = 0, lds_fit = 0;   // dynamic LDS of the spread mat-vec and fit launches
     int lds_pipe = 0;              // dynamic LDS of matvec_pipe_kernel (largest tile, then every axis's table)
+    int res_off = 0;               // LDS-resident intermediates of the batched kernels: byte offset (after the lines)
+    int res_mv = 0, res_fit = 0;   // ... and the resident bytes matvec_kernel / fit_kernel launch with (0: none)
     int64_t out_elems = 0, dens_elems = 0;
     int64_t ws_elems = 0;
     size_t esize = 4;
@@ -79,6 +81,7 @@ DevPlan<T> dev_plan(const Plan &pl) {
     p.sep = pl.sep ? pl.d_sep : nullptr;
     p.sep_tab = (const T *)pl.d_sep_tab;
     p.sep_off = pl.d_sep_off;
+    p.res_off = p.res_cap = 0;
     return p;
 }
 
@@ -98,6 +101,7 @@ DevPlan<double> dev_plan64(const Plan &pl) {
     p.sep = nullptr;
     p.sep_tab = nullptr;
     p.sep_off = nullptr;
+    p.res_off = p.res_cap = 0;
     return p;
 }
 
