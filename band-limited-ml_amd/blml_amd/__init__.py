@@ -3,5 +3,6 @@ from .binned import BLMLBinned, FitBatch, fit_many
 from .modes import make_modes, parse_modes
 from .quick import BLMLQuick
 from .select import fc_sweep, select_fc
+from .streaming import BLMLBinnedStreaming
 
-__all__ = ['BLMLBinned', 'BLMLQuick', 'FitBatch', 'fc_sweep', 'fit_many', 'make_modes', 'parse_modes', 'select_fc']
+__all__ = ['BLMLBinned', 'BLMLBinnedStreaming', 'BLMLQuick', 'FitBatch', 'fc_sweep', 'fit_many', 'make_modes', 'parse_modes', 'select_fc']

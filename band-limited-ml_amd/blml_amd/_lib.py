@@ -55,7 +55,7 @@ EXPORTS = ('blml_plan_create', 'blml_plan_destroy', 'blml_matvec', 'blml_fit', '
            'blml_quick_sweep', 'blml_quick_exact', 'blml_quick_stats', 'blml_plan_matvec_kernel',
            'blml_pair_rows_noalloc', 'blml_point_rows', 'blml_grad_dots', 'blml_quick_psd_sweep',
            'blml_quick_psd_exact', 'blml_quick_psd_stats', 'blml_quick_psd_grad', 'blml_quick_psd_wscale',
-           'blml_vec_gram', 'blml_vec_lincomb', 'blml_lbfgs_twoloop', 'blml_last_error')
+           'blml_vec_gram', 'blml_vec_lincomb', 'blml_lbfgs_twoloop', 'blml_ift_elem', 'blml_last_error')
 
 _lib = None
 
@@ -107,6 +107,7 @@ def load(path=None):
     lib.blml_vec_gram.argtypes = [i32, vp, i32, vp, i64, vp, vp]
     lib.blml_vec_lincomb.argtypes = [i32, vp, vp, i32, i64, vp, vp]
     lib.blml_lbfgs_twoloop.argtypes = [i32, vp, vp, vp, f64, i64, vp, vp, vp]
+    lib.blml_ift_elem.argtypes = [i32, i32, i32, i64, i32, f64, f64, f64, f64, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.blml_last_error.restype = ctypes.c_char_p
     for name in EXPORTS[:-1]:
         getattr(lib, name).restype = ctypes.c_int

@@ -446,7 +446,7 @@ class BLMLBinned:
 
     def _adopt(self, fs, batch, r, index):
         self._batch, self._index = batch, index
-        self._gplans = None
+        self._gplans = self._iplan = None
         self.h_, self.origin_, self.pad_, self.shape_, self.d_ = fs.h, fs.origin, fs.pad, fs.shape, fs.d
         self.specs_, self.modes_, self.K_, self.shared_ = fs.specs, fs.modes, fs.K, fs.shared
         self.s_ = fs.s
@@ -584,6 +584,135 @@ class BLMLBinned:
             for j in range(len(emode)):
                 vals.append(2.0 * o[4 + 2 * j] / M - fz * o[5 + 2 * j] / Z)
         return [np.array([np.nan if e is None else vals[e] for e in row]) for row in where]
+
+    # ------------------------------------------------------- the refit response (blml_binned.py:566-628, 1485-1496)
+    def _ift_ops(self):
+        """The fit's FP64 device state and operators for implicit_grad: (E, U, w, S, dot, comb, elem, mask, K, B, nc),
+        S applying _Sall through a one-fit plan of this fit's kernels (built once)."""
+        torch = _torch()
+        lib = _lib.load()
+        b, i = self._batch, self._index
+        F = b._Fits[i]
+        fs = b.fits[i]
+        K, B, nc = fs.K, fs.B, fs.nc
+        V = K * B * nc
+        if getattr(self, '_iplan', None) is None:
+            ds = _FitSpec.__new__(_FitSpec)
+            ds.__dict__.update(fs.__dict__)
+            ds.init, ds.init_K, ds.signs, ds.draws = None, 0, None, np.zeros(1)
+            self._iplan = FitBatch([ds], self._params(), self.dtype, hist_cap=0).build()
+        plan = self._iplan
+        tdt = b.d_e.dtype
+        E = b.d_e[F.out_off:F.out_off + V].to(torch.float64)
+        U = b.d_se[F.out_off:F.out_off + V].to(torch.float64)
+        w = torch.from_numpy(np.ascontiguousarray(fs.counts.reshape(-1), np.float64)).to('cuda')
+        st = b.stream()
+        P = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+
+        def S(x):
+            return plan.matvec(x.to(tdt).contiguous()).to(torch.float64)
+
+        def dot(x, y):
+            out = torch.empty(4, dtype=torch.float64, device='cuda')
+            A = np.eye(2)
+            X = (ctypes.c_void_p * 2)(x.data_ptr(), y.data_ptr())
+            _lib.check(lib.blml_vec_gram(2, X, 2, A.ctypes.data_as(ctypes.c_void_p), x.numel(), P(out), st))
+            return float(out[1].item())
+
+        def comb(xs, c):
+            out = torch.empty_like(xs[0])
+            X = (ctypes.c_void_p * len(xs))(*[x.data_ptr() for x in xs])
+            _lib.check(lib.blml_vec_lincomb(len(xs), X, (ctypes.c_double * len(c))(*c), 0, xs[0].numel(), P(out), st))
+            return out
+
+        def elem(op, Kk=K, k=0, n=0.0, eps=0.0, Q=0.0, M=1.0, U_=None, dU=None, X=None, m=None, two=False):
+            out = torch.empty(Kk * B * nc, dtype=torch.float64, device='cuda')
+            out2 = torch.empty(B * nc, dtype=torch.float64, device='cuda') if two else None
+            _lib.check(lib.blml_ift_elem(op, Kk, nc, B, k, n, eps, Q, M, P(U_), P(dU), P(X), P(w), P(m), P(out),
+                                         P(out2), st))
+            return (out, out2) if two else out
+
+        mask = lambda x: elem(4, X=x)
+        return E, U, w, S, dot, comb, elem, mask, K, B, nc
+
+    def implicit_grad(self, counts, eps=0.0, tol=1e-6, maxit=300):
+        """d/dlog fc_u of param_grad's score with the coefficients' refit response (blml_binned.py:566-581):
+        G_u - <v, d grad / dlog fc_u>, H v = dP/de by CG (blml_binned.py:1485-1496) over the occupied bins, H the
+        Hessian of the fit's objective at its stationary point.  Every vector is FP64 on the device; the
+        Hessian-vector products are two mat-vecs each through the fit's own kernels (blml_matvec) plus their
+        elementwise terms (blml_ift_elem); ift_iters_ and ift_res_ hold the CG's iterations and relative residual."""
+        torch = _torch()
+        G = self.param_grad(counts, eps)
+        E, U, w, S, dot, comb, elem, mask, K, B, nc = self._ift_ops()
+        n = float(self.n_)
+        m = np.asarray(counts, float).reshape(-1)
+        if m.size != B:
+            raise ValueError("counts: on the fit's lattice")
+        dm = torch.from_numpy(np.ascontiguousarray(m)).to('cuda')
+        M = float(m.sum())
+        Q = dot(E, U)
+        src = elem(3, eps=eps, Q=Q, M=M, U_=U, m=dm)
+        fz = dot(src, U)
+        b = mask(comb([S(src), U], [2.0, -2.0 * fz / Q]))
+        psd = bool(self.psd_)
+
+        def A(x):
+            dU = S(x)
+            if not psd:
+                return mask(elem(1, n=n, dU=dU, X=E, m=x))
+            inner = elem(0, U_=U, dU=dU)
+            dQ = 2.0 * dot(U, x)
+            return mask(comb([S(inner), dU, U], [-2.0, 2.0 * n / Q, -2.0 * n * dQ / Q ** 2]))
+
+        # CG for the symmetric positive semidefinite H on the real view of the coefficients (_icg)
+        x = torch.zeros_like(b)
+        r = b.clone()
+        p_ = r.clone()
+        rs = dot(r, r)
+        bn = np.sqrt(rs) or 1.0
+        its, res = maxit, np.sqrt(rs) / bn
+        for it in range(1, maxit + 1):
+            Ap = A(p_)
+            pAp = dot(p_, Ap)
+            if not pAp > 0:
+                its, res = it, np.sqrt(rs) / bn
+                break
+            a = rs / pAp
+            x = comb([x, p_], [1.0, a])
+            r = comb([r, Ap], [1.0, -a])
+            rs_new = dot(r, r)
+            if np.sqrt(rs_new) / bn < tol:
+                its, res = it, np.sqrt(rs_new) / bn
+                break
+            p_ = comb([r, p_], [1.0, rs_new / rs])
+            rs = rs_new
+            its, res = it, np.sqrt(rs) / bn
+        self.ift_iters_, self.ift_res_ = its, res
+        v = x
+        plans, where = self._grad_plans()
+        step = B * nc
+        mode = lambda X, k: X[k * step:(k + 1) * step]
+        tdt = self._batch.d_e.dtype
+        mixed = {}
+        for pi, (plan, emode) in enumerate(plans):
+            Ed = torch.cat([mode(E, k) for k in emode]).to(tdt)
+            dUe = plan.matvec(Ed).to(torch.float64)
+            if not psd:
+                for j, k in enumerate(emode):
+                    mixed[pi * _lib.MAXK + j] = dot(v, mask(mode(dUe, j)))
+                continue
+            parts = [elem(2, k=k, U_=U, dU=mode(dUe, j), two=True) for j, k in enumerate(emode)]
+            wsU = torch.cat([q[1] for q in parts]).to(tdt)
+            app = plan.matvec(wsU).to(torch.float64)             # apply(dKf_e, ws U_k(e)) for every entry
+            for j, k in enumerate(emode):
+                dUk = mode(dUe, j)
+                dQ = dot(mode(E, k), dUk)
+                g = comb([S(parts[j][0]), U], [-2.0, -2.0 * n * dQ / Q ** 2])
+                gk = comb([mode(g, k), mode(app, j), dUk], [1.0, -2.0, 2.0 * n / Q])
+                g = torch.cat([gk if q == k else mode(g, q) for q in range(K)])
+                mixed[pi * _lib.MAXK + j] = dot(v, mask(g))
+        return [np.array([np.nan if e is None else G[k][u] - mixed[e] for u, e in enumerate(row)])
+                for k, row in enumerate(where)]
 
     def fc_grad(self):
         """d loglik_ / d log fc of every (mode, unit) at the fitted coefficients (blml_binned.py:520-530): the

@@ -969,6 +969,26 @@ static int grad_dots_t(const GradArgs &g, const T *E, const T *U, const T *dU, c
 }
 extern "C" {
 
+int blml_ift_elem(int32_t op, int32_t K, int32_t nc, int64_t B, int32_t k, double n, double eps, double Q, double M,
+                  const double *U, const double *dU, const double *X, const double *w, const double *m, double *out,
+                  double *out2, void *stream) {
+    if (op < 0 || op > 4) return fail(BLML_EVALUE, "ift: unknown op");
+    if (K < 1 || K > BLML_MAXK || (nc != 1 && nc != 2) || B < 1 || !w || !out)
+        return fail(BLML_EVALUE, "ift: K, nc, B, w, out");
+    if ((op == BLML_IFT_INNER || op == BLML_IFT_MIXED || op == BLML_IFT_DPDE) && !U) return fail(BLML_EVALUE, "ift: U");
+    if ((op == BLML_IFT_INNER || op == BLML_IFT_MIXED || op == BLML_IFT_REAL) && !dU) return fail(BLML_EVALUE, "ift: dU");
+    if ((op == BLML_IFT_REAL || op == BLML_IFT_MASK) && !X) return fail(BLML_EVALUE, "ift: X");
+    if ((op == BLML_IFT_REAL || op == BLML_IFT_DPDE) && !m) return fail(BLML_EVALUE, "ift: m");
+    if (op == BLML_IFT_MIXED && (k < 0 || k >= K || !out2)) return fail(BLML_EVALUE, "ift: mode k, out2");
+    if (op == BLML_IFT_REAL && K != 1) return fail(BLML_EVALUE, "ift: the real rank-1 product has K = 1");
+    IftArgs a{op, K, nc, k, B, n, eps, Q, M};
+    hipStream_t st = (hipStream_t)stream;
+    ift_elem_kernel<<<(unsigned)((B + BLML_GRAD_NT - 1) / BLML_GRAD_NT), BLML_GRAD_NT, 0, st>>>(a, U, dU, X, w, m, out,
+                                                                                               out2);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
 int blml_grad_dots(int32_t dtype, int32_t K, int32_t nc, int64_t B, const void *E, const void *U, const void *dU,
                    int32_t nE, const int32_t *emode, const double *m, double eps, double *out, void *stream) {
     if (dtype != BLML_F32 && dtype != BLML_F64) return fail(BLML_EVALUE, "dtype: BLML_F32 or BLML_F64");

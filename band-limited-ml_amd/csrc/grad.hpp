@@ -101,3 +101,73 @@ __global__ void __launch_bounds__(BLML_GRAD_NT) grad_pass2_kernel(GradArgs g, co
         if (threadIdx.x == 0) part[(long)blockIdx.x * W + i] = v;
     }
 }
+
+// ------------------------------------------------------------------------------- the refit response (implicit_grad)
+// Elementwise terms of the Hessian-vector product and the mixed derivative of the fit's objective at its stationary
+// point (blml_binned.py:583-628), FP64 over (K, B, nc) vectors laid out mode-major like the fit's outputs.  occ: w > 0
+// (the training counts); F = sum_k |U_k|^2, Fs = F on occupied bins, 1 elsewhere.
+#define BLML_IFT_INNER 0   // out = occ ? (w / Fs) dU - (w dF / Fs^2) U : 0, dF = 2 Re sum_k conj(U_k) dU_k
+#define BLML_IFT_REAL 1    // out = dU + (occ ? n w / e^2 : 0) D                    (real rank 1: X = E)
+#define BLML_IFT_MIXED 2   // out = -wd U_j + [j == k] ws dU_k, out2 = ws U_k; ws = occ ? w / Fs : 0,
+                           // wd = occ ? w dF_k / Fs^2 : 0, dF_k = 2 Re conj(U_k) dU_k (dU: mode k's B nc values)
+#define BLML_IFT_DPDE 3    // out = (m / max(F + eps Q, 1e-300) / M) U             (the score's gradient sources)
+#define BLML_IFT_MASK 4    // out = occ ? X : 0
+
+struct IftArgs {
+    int op, K, nc, k;
+    long B;
+    double n, eps, Q, M;
+};
+
+__global__ void __launch_bounds__(BLML_GRAD_NT) ift_elem_kernel(IftArgs a, const double *U, const double *dU,
+                                                                const double *X, const double *w, const double *m,
+                                                                double *out, double *out2) {
+    const long b = (long)blockIdx.x * BLML_GRAD_NT + threadIdx.x;
+    if (b >= a.B) return;
+    const int K = a.K, nc = a.nc;
+    const long B = a.B;
+    auto at = [&](int k, int c) { return ((long)k * B + b) * nc + c; };
+    const double wb = w[b];
+    const bool occ = wb > 0;
+    if (a.op == BLML_IFT_MASK) {
+        for (int k = 0; k < K; ++k)
+            for (int c = 0; c < nc; ++c) out[at(k, c)] = occ ? X[at(k, c)] : 0.0;
+        return;
+    }
+    if (a.op == BLML_IFT_REAL) {
+        const double e = X[at(0, 0)];
+        const double q = occ ? a.n * wb / (e * e) : 0.0;
+        for (int k = 0; k < K; ++k)
+            for (int c = 0; c < nc; ++c) out[at(k, c)] = dU[at(k, c)] + q * m[at(k, c)];
+        return;
+    }
+    double F = 0.0;
+    for (int k = 0; k < K; ++k)
+        for (int c = 0; c < nc; ++c) F += U[at(k, c)] * U[at(k, c)];
+    if (a.op == BLML_IFT_DPDE) {
+        const double wn = m[b] / fmax(F + a.eps * a.Q, 1e-300) / a.M;
+        for (int k = 0; k < K; ++k)
+            for (int c = 0; c < nc; ++c) out[at(k, c)] = wn * U[at(k, c)];
+        return;
+    }
+    const double Fs = occ ? F : 1.0;
+    if (a.op == BLML_IFT_INNER) {
+        double dF = 0.0;
+        for (int k = 0; k < K; ++k)
+            for (int c = 0; c < nc; ++c) dF += U[at(k, c)] * dU[at(k, c)];
+        dF *= 2.0;
+        const double p = wb / Fs, q = wb * dF / (Fs * Fs);
+        for (int k = 0; k < K; ++k)
+            for (int c = 0; c < nc; ++c) out[at(k, c)] = occ ? p * dU[at(k, c)] - q * U[at(k, c)] : 0.0;
+        return;
+    }
+    // BLML_IFT_MIXED
+    double dF = 0.0;
+    for (int c = 0; c < nc; ++c) dF += U[at(a.k, c)] * dU[b * nc + c];
+    dF *= 2.0;
+    const double ws = occ ? wb / Fs : 0.0, wd = occ ? wb * dF / (Fs * Fs) : 0.0;
+    for (int j = 0; j < K; ++j)
+        for (int c = 0; c < nc; ++c)
+            out[at(j, c)] = -wd * U[at(j, c)] + (j == a.k ? ws * dU[b * nc + c] : 0.0);
+    for (int c = 0; c < nc; ++c) out2[b * nc + c] = ws * U[at(a.k, c)];
+}

@@ -325,6 +325,19 @@ int blml_quick_psd_grad(const blml_pairtab_t *tabs, int32_t ntab, const int32_t 
 int blml_quick_psd_wscale(const double *SE, const double *w, int64_t B, int32_t K, int32_t nc, double coef,
                           double *out, void *stream);
 
+/* The elementwise terms of implicit_grad's Hessian-vector product and mixed derivative (blml_binned.py:583-628):
+ * FP64 (K, B, nc) vectors, mode-major like blml_fit's outputs; w the fit's counts (occupied: w > 0).
+ *   op 0  out = occ ? (w / Fs) dU - (w dF / Fs^2) U : 0, F = sum_k |U_k|^2, Fs = occ ? F : 1, dF = 2 Re sum conj(U) dU
+ *         (_hvp's inner term)
+ *   op 1  out = dU + (occ ? n w / e^2 : 0) m, e = X (K = 1: the real rank-1 Hessian, _hvp, m the direction D)
+ *   op 2  out = -wd U_j + [j == k] ws dU, out2 = ws U_k with ws = occ ? w / Fs : 0, wd = occ ? w dF_k / Fs^2 : 0,
+ *         dF_k = 2 Re conj(U_k) dU, dU mode k's B nc values (_mixed's sources)
+ *   op 3  out = (m / max(F + eps Q, 1e-300) / M) U (_dP_de's sources; m: B counts)
+ *   op 4  out = occ ? X : 0 (_free's mask) */
+int blml_ift_elem(int32_t op, int32_t K, int32_t nc, int64_t B, int32_t k, double n, double eps, double Q, double M,
+                  const double *U, const double *dU, const double *X, const double *w, const double *m, double *out,
+                  double *out2, void *stream);
+
 /* FP64 vector work of the Anderson mixing and L-BFGS above (the reference's CuPy array expressions,
  * blml_gpu.py:705-718, blml_trivial.py:2617-2641).  X: nv <= BLML_VEC_MAXV host-held device pointers of len values.
  * blml_vec_gram: out (ny x ny, device) = the Gram matrix of Y_i = sum_j A[i * nv + j] X_j (A host, ny <= 8).

@@ -666,6 +666,71 @@ class BLMLBinnedOracle:
             out.append(np.array(gk))
         return out
 
+    def implicit_grad(self, counts, eps=0.0, tol=1e-6, maxit=300):
+        """param_grad with the coefficients' refit response (blml_binned.py:566-628): G - <v, d grad / dlog fc>,
+        H v = dP/de by CG (_icg, blml_binned.py:1485-1496) over the occupied bins."""
+        G = self.param_grad(counts, eps)
+        E, U = np.asarray(self.e_hat_), np.asarray(self.Se_)
+        occ = np.asarray(self.occ_)
+        w, n = np.asarray(self.w_, float), float(self.n_)
+        F = (np.abs(U) ** 2).sum(0)
+        Q = float(np.real(np.sum(np.conj(E) * U)))
+        Fs = np.where(occ, F, 1.0)
+        m = np.asarray(counts, float); M = float(m.sum())
+        rdot = lambda a, b: float(np.real(np.sum(np.conj(a) * b)))
+        mask = occ[None].astype(float)
+        wn = m / np.maximum(F + eps * Q, 1e-300)
+        fz = float(np.sum(wn * F)) / M
+        b = (2 * self._Sall((wn / M)[None] * U) - fz * 2 * U / Q) * mask
+
+        def hvp(D):
+            dU = self._Sall(D)
+            if not self.psd_:
+                e = E[0]; c = n * w
+                return dU + np.where(occ, c / np.where(occ, e * e, 1.0), 0.0)[None] * D
+            dF = 2 * np.real(np.conj(U) * dU).sum(0); dQ = 2 * rdot(U, D)
+            inner = (w / Fs)[None] * dU - (w * dF / Fs ** 2)[None] * U
+            return -2 * self._Sall(np.where(occ[None], inner, 0)) + 2 * n * dU / Q - 2 * n * U * dQ / Q ** 2
+
+        x = np.zeros_like(b); r = b.copy(); p = r.copy(); rs = rdot(r, r); bn = np.sqrt(rs) or 1.0
+        self.ift_iters_, self.ift_res_ = maxit, np.sqrt(rs) / bn
+        for it in range(1, maxit + 1):
+            Ap = hvp(p) * mask; pAp = rdot(p, Ap)
+            if not pAp > 0:
+                self.ift_iters_, self.ift_res_ = it, np.sqrt(rs) / bn
+                break
+            a = rs / pAp; x = x + a * p; r = r - a * Ap; rs_new = rdot(r, r)
+            if np.sqrt(rs_new) / bn < tol:
+                self.ift_iters_, self.ift_res_ = it, np.sqrt(rs_new) / bn
+                break
+            p = r + (rs_new / rs) * p; rs = rs_new
+            self.ift_iters_, self.ift_res_ = it, np.sqrt(rs) / bn
+        v = x
+        ws = np.where(occ, w / Fs, 0.0)
+        for k, sp in enumerate(self.specs_):
+            for u in range(len(kernel_units(sp['kinds'], self.d_))):
+                dKf = self._dKf_fd(k, u)
+                if dKf is None:
+                    continue
+                dUk = self._apply(dKf, E[k])
+                if not self.psd_:
+                    mixed = dUk[None]
+                else:
+                    dF = 2 * np.real(np.conj(U[k]) * dUk); dQ = rdot(E[k], dUk)
+                    wd = np.where(occ, w * dF / Fs ** 2, 0.0)
+                    mixed = []
+                    for j in range(self.K_):
+                        t = -wd * U[j]
+                        if j == k:
+                            t = t + ws * dUk
+                        g = -2 * self._S(j, t) - 2 * n * U[j] * dQ / Q ** 2
+                        if j == k:
+                            g = g - 2 * self._apply(dKf, ws * U[k]) + 2 * n * dUk / Q
+                        mixed.append(g)
+                    mixed = np.stack(mixed)
+                G[k][u] -= rdot(v, mixed * mask)
+        return G
+
     def log_density(self, counts, eps=0.0):
         """Mean over counts m of log(F / Z + eps) (blml_binned.py:685-693)."""
         U = np.asarray(self.Se_); F = (np.abs(U) ** 2).sum(0)

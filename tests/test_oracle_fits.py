@@ -166,3 +166,35 @@ def test_param_grad_is_the_refit_derivative(kinds, fc, cplx):
     fd = (ll[0] - ll[1]) / (2 * d)
     assert abs(np.nansum(G) - fd) < 1e-5 * max(1.0, abs(fd)), (G, fd)
     assert abs(m.log_density(N) - m.loglik_) < 1e-12 * abs(m.loglik_)
+
+
+IGRAD_CASES = [('1d sinc real', [0.3], 'sinc', 1, False), ('1d sinc complex', [0.3], 'sinc', 1, True),
+               ('1d rank2 per-mode real', [[0.3], [0.15]], 'sinc', 1, False)]
+
+
+@pytest.mark.parametrize('name,fc,kind,rank,cplx', IGRAD_CASES, ids=[c[0] for c in IGRAD_CASES])
+def test_implicit_grad_is_the_refit_derivative(name, fc, kind, rank, cplx):
+    """implicit_grad (the held-out score's derivative with the coefficients' refit response) against the central
+    difference of the held-out score of warm-started refits at fc exp(+-delta) (the reference's own check,
+    results/binned_igrad_checks.py:20-35), and away from the fixed-coefficient param_grad."""
+    rng = np.random.default_rng(1)
+    x = np.arange(64.0)
+    p = np.exp(-0.5 * ((x - 20) / 4) ** 2) + 0.6 * np.exp(-0.5 * ((x - 42) / 3) ** 2) + 0.02
+    p /= p.sum()
+    tr, te = rng.multinomial(20000, p).astype(float), rng.multinomial(5000, p).astype(float)
+    m = O.BLMLBinnedOracle(tol=1e-11).fit(tr, O.make_modes(fc, [kind]), complex_g=cplx)
+    eps = 1e-3 * float(np.median(np.asarray(m.density_)[te > 0]))
+    Gi, Gf = m.implicit_grad(te, eps, tol=1e-10, maxit=2000), m.param_grad(te, eps)
+    F0 = np.array([sp['fc'] for sp in m.specs_], float)
+    d = 2e-3
+    for k in range(len(F0)):
+        ll = []
+        for sg in (1, -1):
+            F = F0.copy()
+            F[k] *= np.exp(sg * d)
+            ll.append(O.BLMLBinnedOracle(tol=1e-11).fit(tr, O.make_modes([list(f) for f in F] if len(F0) > 1 else
+                                                                         list(F[0]), [kind]),
+                                                        complex_g=cplx, init=m).log_density(te, eps))
+        fd = (ll[0] - ll[1]) / (2 * d)
+        assert abs(fd - Gi[k][0]) < 5e-3 * max(abs(fd), 1e-3), (name, k, fd, Gi[k][0])
+        assert abs(Gf[k][0] - Gi[k][0]) > 10 * abs(fd - Gi[k][0]), (name, k, Gf[k][0], Gi[k][0], fd)
