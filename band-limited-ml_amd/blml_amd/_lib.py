@@ -45,6 +45,15 @@ class Result(ctypes.Structure):
                 ('seconds', ctypes.c_double * 5)]
 
 
+class EmpParams(ctypes.Structure):
+    _fields_ = [('reg', ctypes.c_int32), ('normalise', ctypes.c_int32), ('anscombe', ctypes.c_int32),
+                ('retrieve', ctypes.c_int32), ('lam', ctypes.c_double), ('tol', ctypes.c_double),
+                ('outer_tol', ctypes.c_double), ('max_iter', ctypes.c_int32), ('max_outer', ctypes.c_int32),
+                ('anderson', ctypes.c_int32), ('reserved', ctypes.c_int32)]
+
+
+EMP_AUX_LEN, EMP_OUT_LEN = MAXK + 1, 4
+
 RESULT_DTYPE = np.dtype([(name, (np.float64, 5) if name == 'seconds' else
                           (np.float64 if t is ctypes.c_double else np.int32)) for name, t in Result._fields_])
 assert RESULT_DTYPE.itemsize == ctypes.sizeof(Result)
@@ -55,7 +64,8 @@ EXPORTS = ('blml_plan_create', 'blml_plan_destroy', 'blml_matvec', 'blml_fit', '
            'blml_quick_sweep', 'blml_quick_exact', 'blml_quick_stats', 'blml_plan_matvec_kernel',
            'blml_pair_rows_noalloc', 'blml_point_rows', 'blml_grad_dots', 'blml_quick_psd_sweep',
            'blml_quick_psd_exact', 'blml_quick_psd_stats', 'blml_quick_psd_grad', 'blml_quick_psd_wscale',
-           'blml_vec_gram', 'blml_vec_lincomb', 'blml_lbfgs_twoloop', 'blml_ift_elem', 'blml_last_error')
+           'blml_vec_gram', 'blml_vec_lincomb', 'blml_lbfgs_twoloop', 'blml_ift_elem', 'blml_empirical',
+           'blml_last_error')
 
 _lib = None
 
@@ -77,6 +87,7 @@ def load(path=None):
     lib.blml_matvec.argtypes = [vp, dp, dp, vp]
     lib.blml_fit.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, vp]
     lib.blml_certificate.argtypes = [vp, dp, dp, i32, ctypes.c_double, dp, dp, vp]
+    lib.blml_empirical.argtypes = [vp, ctypes.POINTER(EmpParams), dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, vp]
     lib.blml_plan_info.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i32),
                                    ctypes.POINTER(i32)]
     lib.blml_plan_matvec_kernel.argtypes = [vp, ctypes.POINTER(i32)]

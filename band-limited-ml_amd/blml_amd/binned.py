@@ -285,6 +285,24 @@ class FitBatch:
                                 ptr(self.d_mh), ptr(self.d_rh), self.stream()))
         return self
 
+    def run_empirical(self, ep, aux):
+        """One launch of BLMLEmpirical's fit (blml_empirical) over every fit of a batched plan: ep the knobs
+        (_lib.EmpParams), aux the per-fit max|Kf| of each mode and Delta ((nfits, EMP_AUX_LEN) doubles)."""
+        torch = _torch()
+        lib = _lib.load()
+        ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+        self.d_aux = torch.from_numpy(np.ascontiguousarray(aux, np.float64).ravel()).to('cuda')
+        self.d_emp = torch.zeros(len(self.fits) * _lib.EMP_OUT_LEN, dtype=torch.float64, device='cuda')
+        _lib.check(lib.blml_empirical(self.plan, ctypes.byref(ep), ptr(self.d_counts), ptr(self.d_draws),
+                                      ptr(self.d_init), ptr(self.d_aux), ptr(self.d_e), ptr(self.d_se),
+                                      ptr(self.d_dens), ptr(self.d_res), ptr(self.d_mh), ptr(self.d_emp),
+                                      self.stream()))
+        return self
+
+    def empirical_outputs(self):
+        """(nfits, EMP_OUT_LEN) host doubles of the last run_empirical: Zw, Zp, the outer criterion, scale."""
+        return self.d_emp.cpu().numpy().reshape(len(self.fits), _lib.EMP_OUT_LEN)
+
     def matvec(self, E):
         """U = _Sall(E) for every fit: E a device tensor laid out like the outputs (plan dtype)."""
         torch = _torch()
@@ -477,13 +495,16 @@ class BLMLBinned:
         """g_k(x) = (1/n) sum_b e_bk K_k(x - z_b) at points X (Nq, d) over the occupied bins (blml_binned.py:452-466):
         (Nq,) for rank 1, else (Nq, K); complex for complex_g fits.  The sums run on the device (blml_point_rows,
         the units' closed forms, FP64); chunk is accepted for compatibility."""
+        return self._g_at(X, np.asarray(self.occ_).ravel())
+
+    def _g_at(self, X, bins):
+        """g_k at points X from the coefficients of the bins selected by the mask `bins`."""
         from .points import point_rows, point_units
         X = np.atleast_2d(np.asarray(X, float))
         if X.shape[1] != self.d_:
             raise ValueError(f"X: points with {self.d_} coordinates")
-        occ = np.asarray(self.occ_).ravel()
-        Zb = self.centres()[occ]
-        E = np.asarray(self.e_hat_).reshape(self.K_, -1)[:, occ]
+        Zb = self.centres()[bins]
+        E = np.asarray(self.e_hat_).reshape(self.K_, -1)[:, bins]
         out = np.zeros((len(X), self.K_), dtype=complex if self.complex_g_ else float)
         for k, sp in enumerate(self.specs_):
             units, harm = point_units(sp)

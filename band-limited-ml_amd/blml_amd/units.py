@@ -285,6 +285,35 @@ def mode_factors(spec, shape, h, circ):
     return s, facs
 
 
+def mode_kf_max(spec, shape, h, pad):
+    """max |Kf| of one mode's kernel on the reference's padded lattice (the FISTA bound of BLMLEmpirical's l1 fit,
+    blml_binned.py:1179, with Kf = rfftn of the masked table, :79-90): the table is a product of units over disjoint
+    dims, so its spectrum is the outer product of the units' spectra and the maximum is the product of their
+    maxima.  Each unit's table is evaluated on its dims of pad (offsets wrapped as _offsets does, :63-72; masked to
+    |off| <= m - 1 on padded dims) and transformed in FP64."""
+    d = len(shape)
+    out = 1.0
+    for kind, dims in units_of(spec['kinds']):
+        name = split_kind(kind)[0]
+        fc = np.asarray(spec['fc'], float)
+        offs, masks = [], []
+        for j in dims:
+            L, m = int(pad[j]), int(shape[j])
+            i = np.arange(L)
+            off = np.where(i < (L + 1) // 2, i, i - L).astype(float)
+            offs.append(off * h[j])
+            masks.append(np.ones(L) if L == m else (np.abs(off) <= m - 1).astype(float))
+        if len(dims) == 1 and name not in ('ball', 'pball'):
+            T = one_column_values(kind, fc[dims[0]], spec['periods'][dims[0]], offs[0]) * masks[0]
+        else:
+            grids = np.meshgrid(*offs, indexing='ij')
+            T = multi_values(name, fc[list(dims)], [spec['periods'][j] for j in dims], grids)
+            for mk in np.meshgrid(*masks, indexing='ij'):
+                T = T * mk
+        out *= float(np.max(np.abs(np.fft.fftn(T))))
+    return out
+
+
 def mode_factor_dlogs(spec, shape, h, circ):
     """Per unit of mode_factors(spec, ...) (same order): the factor of dK/dlog fc_u (the unit's table replaced by its
     derivative: blml_binned.py:92-101's dKf, one table per unit), or None where fc moves in whole harmonics

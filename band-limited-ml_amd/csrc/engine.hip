@@ -27,6 +27,7 @@
 #include "quick.hpp"
 #include "quick_psd.hpp"
 #include "fit.hpp"
+#include "empirical.hpp"
 #include "plan_layout.hpp"
 
 namespace {
@@ -181,6 +182,26 @@ __global__ void __launch_bounds__(BLML_NT) fit_grid_kernel(DevPlan<T> p, unsigne
                                                            double *merit_hist, double *rho_hist, double tick_hz) {
     run_fit<T>(p, 0, Team{(int)blockIdx.x, (int)gridDim.x, bar, part, 0}, counts, draws, init, signs, e_out,
                se_out, density, results, merit_hist, rho_hist, tick_hz);
+}
+
+// BLMLEmpirical over a batched plan: the persistent queue of fit_kernel, each fit by run_empirical
+template <typename T>
+__global__ void __launch_bounds__(BLML_NT) emp_kernel(DevPlan<T> p, const int *order, int *queue,
+                                                      blml_emp_params_t ep, const double *counts,
+                                                      const double *draws, const double *init, const double *aux,
+                                                      T *e_out, T *se_out, double *density, blml_result_t *results,
+                                                      double *j_hist, double *emp_out, double tick_hz) {
+    __shared__ int s_fit;
+    for (;;) {
+        if (threadIdx.x == 0) s_fit = atomicAdd(queue, 1);
+        __syncthreads();
+        const int qi = __builtin_amdgcn_readfirstlane(s_fit);
+        __syncthreads();
+        if (qi >= p.nfits) break;
+        const int fi = __builtin_amdgcn_readfirstlane(order[qi]);
+        run_empirical<T>(p, fi, block_team(), ep, counts, draws, init, aux, e_out, se_out, density, results, j_hist,
+                         emp_out, tick_hz);
+    }
 }
 
 template <typename Te>
@@ -426,6 +447,30 @@ void free_plan(Plan *pl) {
 
 }  // namespace
 
+template <typename T>
+int empirical_launch(Plan &pl, const blml_emp_params_t &ep, const double *counts, const double *draws,
+                            const double *init, const double *aux, void *e_out, void *se_out, double *density,
+                            blml_result_t *results, double *j_hist, double *emp_out, hipStream_t st) {
+    const void *kern = (const void *)emp_kernel<T>;
+    int dev = 0, max_lds = 0, st_lds = 0, ncu = 0, occ = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    int rc = static_lds(kern, st_lds);
+    if (rc) return rc;
+    if (pl.lds_bytes + st_lds > max_lds)
+        return fail(BLML_EVALUE, "a transform line of this plan does not fit the LDS beside the empirical solver");
+    HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BLML_NT, pl.lds_bytes));
+    const int grid = std::max(1, std::min(pl.nfits, ncu * std::max(occ, 1)));
+    DevPlan<T> dp = dev_plan<T>(pl);
+    double tick = pl.tick_hz;
+    void *args[] = {&dp, &pl.d_order, &pl.d_queue, (void *)&ep, &counts, &draws, &init, &aux, &e_out, &se_out,
+                    &density, &results, &j_hist, &emp_out, &tick};
+    HIPCHECK(hipMemsetAsync(pl.d_queue, 0, sizeof(int), st));
+    HIPCHECK(hipLaunchKernel(kern, dim3(grid), dim3(BLML_NT), args, (size_t)pl.lds_bytes, st));
+    return BLML_OK;
+}
+
 // ------------------------------------------------------------------------------------------------ C-ABI
 extern "C" {
 
@@ -565,6 +610,36 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
     }
     HIPCHECK(hipGetLastError());
     return BLML_OK;
+}
+
+int blml_empirical(void *plan, const blml_emp_params_t *ep, const double *counts, const double *draws,
+                   const double *init, const double *aux, void *e_out, void *se_out, double *density,
+                   blml_result_t *results, double *j_hist, double *emp_out, void *stream) {
+    Plan *pl = (Plan *)plan;
+    if (!pl || !ep || !counts || !draws || !aux || !e_out || !se_out || !density || !results || !emp_out)
+        return fail(BLML_EVALUE, "null argument");
+    if (pl->spread) return fail(BLML_EVALUE, "BLMLEmpirical runs on batched plans (create the plan with spread = -1)");
+    if (ep->reg < 0 || ep->reg > 2) return fail(BLML_EVALUE, "reg: 'ridge' | 'l2' | 'l1'");
+    if (ep->normalise < 0 || ep->normalise > 1) return fail(BLML_EVALUE, "normalise: 'window' | 'plane'");
+    if (!(ep->lam >= 0) || !(ep->tol >= 0) || !(ep->outer_tol >= 0))
+        return fail(BLML_EVALUE, "lam, tol and outer_tol must be >= 0");
+    if (ep->max_iter < 1 || ep->max_outer < 0) return fail(BLML_EVALUE, "max_iter >= 1 and max_outer >= 0");
+    if (ep->anderson < 0 || ep->anderson > pl->params.anderson)
+        return fail(BLML_EVALUE, "anderson: 0.." + std::to_string(pl->params.anderson) +
+                                     " (the plan's anderson sizes the state pool)");
+    for (const blml_fit_t &f : pl->hfits) {
+        if ((f.complex_g || f.K > 1) && !ep->retrieve)
+            return fail(BLML_EVALUE, "BLMLEmpirical: rank K > 1 or complex_g need retrieve=True (the hidden phase)");
+        if (f.init_K != 0 && (!init || (ep->retrieve ? f.init_K != f.K : f.init_K != 1)))
+            return fail(BLML_EVALUE, "a warm-started empirical fit needs init with every mode (retrieve) or mode 0");
+        if (f.has_signs) return fail(BLML_EVALUE, "BLMLEmpirical takes no signs");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (pl->dtype == BLML_F32)
+        return empirical_launch<float>(*pl, *ep, counts, draws, init, aux, e_out, se_out, density, results, j_hist,
+                                       emp_out, st);
+    return empirical_launch<double>(*pl, *ep, counts, draws, init, aux, e_out, se_out, density, results, j_hist,
+                                    emp_out, st);
 }
 
 int blml_certificate(void *plan, const void *e_hat, const double *counts, int32_t max_iter, double tol,

@@ -137,6 +137,39 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
 int blml_certificate(void *plan, const void *e_hat, const double *counts, int32_t max_iter, double tol,
                      double *lam, int32_t *iters, void *stream);
 
+/* BLMLEmpirical (blml_binned.py:1121-1291): the fit on the observed lattice's normalisation, sum_b g_b^2 Delta = 1,
+ * by regularised linear fits of sqrt(h) (CG for reg ridge / l2, FISTA for l1) and, for complex or rank-K fits, the
+ * hidden-phase alternation with Anderson mixing.  Knobs of BLMLEmpirical.__init__ (blml_binned.py:1145-1153). */
+typedef struct {
+    int32_t reg;                     /* 0 'ridge', 1 'l2', 2 'l1' */
+    int32_t normalise;               /* 0 'window', 1 'plane' */
+    int32_t anscombe;                /* sqrt(w + 3/8) targets */
+    int32_t retrieve;                /* 1: the alternation (required for complex_g or K > 1) */
+    double lam;
+    double tol;                      /* CG / FISTA relative tolerance */
+    double outer_tol;
+    int32_t max_iter;                /* per linear fit */
+    int32_t max_outer;
+    int32_t anderson;                /* alternation history (<= the plan's anderson, which sizes its state pool) */
+    int32_t reserved;
+} blml_emp_params_t;
+
+#define BLML_EMP_AUX_LEN 9   /* aux doubles per fit: max|Kf| of modes 0..BLML_MAXK-1, then Delta = prod(h) */
+#define BLML_EMP_OUT_LEN 4   /* emp_out doubles per fit: window norm Zw, plane norm Zp, the outer criterion, scale */
+
+/* The BLMLEmpirical fit of every fit of a batched plan (blml_plan_create with spread = -1 and solver 'cd'), one
+ * persistent launch: replaces _fit_counts (blml_binned.py:1218-1291) with its _lin / _cg / _fista / _anderson_mix
+ * (:1168-1216, 1463-1526).  counts / draws as for blml_fit (draws: the start's phases and modes, in the order
+ * blml_binned.py:1244-1249 takes them); init (may be NULL): for a fit with init_K > 0 its start coefficients as
+ * (K, B, 2) doubles -- the raw c of a previous fit (retrieve: every mode, already scaled to this n; else mode 0).
+ * aux: BLML_EMP_AUX_LEN doubles per fit.  e_out = C sc, se_out = S C sc (plan dtype), density, results (loglik,
+ * residual = the largest linear-fit residual, n_sweeps = CG / FISTA iterations, converged, n_anderson,
+ * start_sweeps = alternations, n_hist = J entries); j_hist: hist_cap doubles per fit (J_history_, may be NULL);
+ * emp_out: BLML_EMP_OUT_LEN doubles per fit.  A fit whose Zw or Zp is not positive gets scale 0.               */
+int blml_empirical(void *plan, const blml_emp_params_t *ep, const double *counts, const double *draws,
+                   const double *init, const double *aux, void *e_out, void *se_out, double *density,
+                   blml_result_t *results, double *j_hist, double *emp_out, void *stream);
+
 /* ---------------------------------------------------------------------------------------------------------------
  * Pair accumulation over lattice bins (the reference's blml_local_gpu.rows / .cols, blml_local_gpu.py:42-105,
  * 320-361, and their CPU twins blml_local.py:203-282): the kernel value of a pair of bins is the product of its

@@ -14,6 +14,7 @@
 #define CPUHIP_DEFINE_SWITCH
 #include "hip/hip_runtime.h"
 #include "plan_layout.hpp"
+#include "empirical.hpp"
 
 #include <stdio.h>
 
@@ -38,6 +39,8 @@ struct In {
     std::vector<blml_factor_t> facs;
     std::vector<double> fdata, counts, draws, init, signs;
     std::vector<int32_t> freqs;
+    blml_emp_params_t ep{};          // flags & 2: blml_empirical's fit in place of blml_fit's
+    std::vector<double> aux;
 };
 
 template <typename V>
@@ -68,6 +71,10 @@ bool read_in(const char *path, In &in) {
              rd(f, in.fdata.data(), in.fdata.size()) && rd(f, in.freqs.data(), in.freqs.size()) &&
              rd(f, in.counts.data(), in.counts.size()) && rd(f, in.draws.data(), in.draws.size()) &&
              rd(f, in.init.data(), in.init.size()) && rd(f, in.signs.data(), in.signs.size());
+        if (ok && (in.flags & 2)) {
+            in.aux.resize((size_t)in.nfits * BLML_EMP_AUX_LEN);
+            ok = rd(f, &in.ep, 1) && rd(f, in.aux.data(), in.aux.size());
+        }
     }
     fclose(f);
     return ok;
@@ -219,7 +226,23 @@ int replay(In &in, const char *out_path) {
     double *mh = dev_alloc<double>((size_t)pl.nfits * nc_hist);
     double *rh = dev_alloc<double>((size_t)pl.nfits * nc_hist);
     const DevPlan<T> dp = dev_plan<T>(pl);
-    for (int fi = 0; fi < pl.nfits; ++fi) {
+    const bool emp = (in.flags & 2) != 0;
+    double *aux = emp ? dev_alloc<double>(in.aux.size(), in.aux.data()) : nullptr;
+    double *emp_out = emp ? dev_alloc<double>((size_t)pl.nfits * BLML_EMP_OUT_LEN) : nullptr;
+    for (int fi = 0; emp && fi < pl.nfits; ++fi) {
+        if (pl.spread) {
+            fprintf(stderr, "replay: blml_empirical runs batched plans only\n");
+            return 7;
+        }
+        launch(pl.lds_bytes, kStaticFit, 1, [&] {
+            run_empirical<T>(dp, fi, block_team(), in.ep, counts, draws, init, aux, e_out, se_out, dens, res, mh,
+                             emp_out, 1e8);
+        });
+        printf("replay: empirical fit %d: %d linear-fit iterations, %d alternations, loglik %.12f\n", fi,
+               res[fi].n_sweeps, res[fi].start_sweeps, res[fi].loglik);
+        fflush(stdout);
+    }
+    for (int fi = 0; !emp && fi < pl.nfits; ++fi) {
         if (pl.spread) memset(bar, 0, 4 * sizeof(unsigned));      // launch_coop clears the barrier words
         const long long nb = pl.spread
             ? launch(pl.lds_fit, kStaticFit, pl.coop, [&] {
@@ -237,7 +260,7 @@ int replay(In &in, const char *out_path) {
                res[fi].stop_reason, res[fi].loglik, nb);
         fflush(stdout);
     }
-    if (sizeof(T) == 4) {
+    if (sizeof(T) == 4 && !emp) {
         const DevPlan<double> d64 = dev_plan64(pl);
         for (int fi = 0; fi < pl.nfits; ++fi) {
             if (pl.spread) memset(bar, 0, 4 * sizeof(unsigned));
@@ -287,6 +310,7 @@ int replay(In &in, const char *out_path) {
     fwrite(its.data(), sizeof(int32_t), its.size(), fo);
     fwrite(mh, sizeof(double), (size_t)pl.nfits * nc_hist, fo);
     fwrite(rh, sizeof(double), (size_t)pl.nfits * nc_hist, fo);
+    if (emp) fwrite(emp_out, sizeof(double), (size_t)pl.nfits * BLML_EMP_OUT_LEN, fo);
     fclose(fo);
     for (const blml_fit_t &f : pl.hfits) unpoison(ws + f.ws_off, f.ws_len);
     free(pl.d_ws);

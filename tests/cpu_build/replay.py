@@ -19,7 +19,8 @@ CSRC = os.path.join(ROOT, 'band-limited-ml_amd', 'csrc')
 BUILD = os.path.join(HERE, '_build')
 EXE = os.path.join(BUILD, 'replay')
 CXX = os.environ.get('BLML_REPLAY_CXX', '/opt/rocm/llvm/bin/clang++')
-HEADERS = ('common.hpp', 'separable.hpp', 'transform.hpp', 'solver.hpp', 'fit.hpp', 'plan_layout.hpp')
+HEADERS = ('common.hpp', 'separable.hpp', 'transform.hpp', 'solver.hpp', 'fit.hpp', 'plan_layout.hpp',
+           'empirical.hpp')
 # every LDS declaration of the device headers -> a per-workgroup object of the CPU stand-in (one heap block each)
 LDS_REWRITES = {
     'common.hpp': [('extern __shared__ __align__(16) unsigned char g_dyn_lds[];',
@@ -66,9 +67,10 @@ def _bytes(x):
     return bytes(memoryview(x)) if x is not None else b''
 
 
-def write_plan(batch, path, cert_iters=0, cert_tol=1e-6, ranks=4):
+def write_plan(batch, path, cert_iters=0, cert_tol=1e-6, ranks=4, emp=None):
     """The plan of a FitBatch (host tables only: no device needed) in replay.cpp's input format; ranks: the
-    workgroups of a spread plan's cooperative grid."""
+    workgroups of a spread plan's cooperative grid; emp: (blml_emp_params_t, aux) to replay blml_empirical's fit
+    in place of blml_fit's."""
     from blml_amd import _lib
     prm = batch.params_struct()
     fits = batch._Fits
@@ -82,7 +84,8 @@ def write_plan(batch, path, cert_iters=0, cert_tol=1e-6, ranks=4):
     signs = None if batch.signs is None else np.ascontiguousarray(batch.signs, np.float64)
     with open(path, 'wb') as f:
         f.write(b'BLMLRPL1')
-        f.write(struct.pack('<iiiiiid', batch.code, len(batch.fits), nfacs, 1 if cert_iters else 0, int(cert_iters),
+        flags = (1 if cert_iters else 0) | (2 if emp is not None else 0)
+        f.write(struct.pack('<iiiiiid', batch.code, len(batch.fits), nfacs, flags, int(cert_iters),
                             int(ranks), float(cert_tol)))
         f.write(struct.pack('<qqqqqq', len(fdata), len(freqs), len(counts), len(draws),
                             0 if init is None else len(init), 0 if signs is None else len(signs)))
@@ -91,9 +94,12 @@ def write_plan(batch, path, cert_iters=0, cert_tol=1e-6, ranks=4):
         f.write(bytes(facs)[:nfacs * ctypes.sizeof(_lib.Factor)])
         for a in (fdata, freqs, counts, draws, init, signs):
             f.write(_bytes(a))
+        if emp is not None:
+            f.write(bytes(emp[0]))
+            f.write(_bytes(np.ascontiguousarray(emp[1], np.float64).ravel()))
 
 
-def run(batch, cert_iters=0, cert_tol=1e-6, timeout=1800, exe=None, ranks=4):
+def run(batch, cert_iters=0, cert_tol=1e-6, timeout=1800, exe=None, ranks=4, emp=None):
     """Replays the batch's fits on the CPU; returns dict(results (structured array), e, density, lam, iters, log).
     Raises RuntimeError with the replay's output when it fails (an AddressSanitizer report included)."""
     from blml_amd import _lib
@@ -101,7 +107,7 @@ def run(batch, cert_iters=0, cert_tol=1e-6, timeout=1800, exe=None, ranks=4):
     tmp = tempfile.mkdtemp(prefix='blml_replay_')
     try:
         pin, pout = os.path.join(tmp, 'plan.bin'), os.path.join(tmp, 'out.bin')
-        write_plan(batch, pin, cert_iters, cert_tol, ranks)
+        write_plan(batch, pin, cert_iters, cert_tol, ranks, emp)
         env = dict(os.environ, ASAN_OPTIONS='detect_leaks=0:abort_on_error=0:detect_stack_use_after_return=0')
         p = subprocess.run([exe, pin, pout], capture_output=True, text=True, timeout=timeout, env=env)
         log = p.stdout + p.stderr
@@ -122,7 +128,10 @@ def run(batch, cert_iters=0, cert_tol=1e-6, timeout=1800, exe=None, ranks=4):
     its = np.frombuffer(raw, np.int32, nf, o); o += nf * 4
     hc = batch.hist_cap
     mh = np.frombuffer(raw, np.float64, nf * hc, o).reshape(nf, hc); o += nf * hc * 8
-    rh = np.frombuffer(raw, np.float64, nf * hc, o).reshape(nf, hc)
+    rh = np.frombuffer(raw, np.float64, nf * hc, o).reshape(nf, hc); o += nf * hc * 8
     nh = [min(int(r['n_hist']), hc) for r in res]
-    return dict(results=res, e=e, density=dens, lam=lam, iters=its, log=log,
-                merit_history=[mh[i, :nh[i]] for i in range(nf)], rho_history=[rh[i, :nh[i]] for i in range(nf)])
+    out = dict(results=res, e=e, density=dens, lam=lam, iters=its, log=log,
+               merit_history=[mh[i, :nh[i]] for i in range(nf)], rho_history=[rh[i, :nh[i]] for i in range(nf)])
+    if emp is not None:
+        out['emp'] = np.frombuffer(raw, np.float64, nf * _lib.EMP_OUT_LEN, o).reshape(nf, _lib.EMP_OUT_LEN)
+    return out

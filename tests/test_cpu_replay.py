@@ -106,3 +106,55 @@ def test_fp32_plan_and_certificate(replay):
     D = E ** 2 / N.ravel()[occ]
     want = np.linalg.eigvalsh(np.sqrt(D)[:, None] * S * np.sqrt(D)[None])[-1] / N.sum()
     assert out['iters'][0] > 0 and abs(out['lam'][0] - want) < 1e-6 * want
+
+
+# ------------------------------------------------------------------------------------------------ BLMLEmpirical
+def _emp_replay(replay, est, counts, modes, **fk):
+    from blml_amd.binned import FitBatch
+    fs, params, ep, aux = est._prepare(counts, modes, 1.0, 0.0, fk.get('init'), fk.get('complex_g', False))
+    b = FitBatch([fs], params, est.dtype, hist_cap=int(est.max_outer) + 2, spread=-1)
+    return replay.run(b, emp=(ep, aux[None]))
+
+
+def _emp_counts(bins=64):
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.normal(-1.5, 0.5, 3000), rng.normal(1.8, 0.8, 2000)])
+    return np.histogram(x, bins=bins, range=(-6, 6))[0].astype(float)
+
+
+@pytest.mark.parametrize('reg,lam,max_iter', [('ridge', 1e-2, 20000), ('l2', 1e-3, 20000), ('l1', 1e-2, 400)])
+def test_empirical_linear_fits_match_oracle(replay, reg, lam, max_iter):
+    """blml_empirical's device code (CG / FISTA, targets, normalisation) on the CPU against the oracle: the same
+    iteration counts and the same fit to rounding (FP64 transforms), every access in bounds."""
+    import blml_amd
+    from blml_amd.empirical import BLMLEmpirical
+    from oracle import blml_empirical as OE
+    counts = _emp_counts()
+    kw = dict(reg=reg, lam=lam, tol=1e-10, max_iter=max_iter)
+    out = _emp_replay(replay, BLMLEmpirical(**kw), counts, blml_amd.make_modes([0.2], ['sinc']))
+    ref = OE.BLMLEmpiricalOracle(**kw).fit(counts, O.make_modes([0.2], ['sinc']))
+    r = out['results'][0]
+    assert int(r['n_sweeps']) == ref.n_iter_
+    assert abs(r['loglik'] - ref.loglik_) < 1e-12 * abs(ref.loglik_)
+    assert np.max(np.abs(out['density'][:counts.size] - ref.density_)) < 1e-10 * ref.density_.max()
+    Zw, Zp, _, sc = out['emp'][0]
+    assert Zw == pytest.approx(ref.window_norm_, rel=1e-12) and Zp == pytest.approx(ref.plane_norm_, rel=1e-12)
+
+
+def test_empirical_retrieve_matches_reference_golden(replay):
+    """The hidden-phase alternation with Anderson mixing (complex, ridge) on the CPU against the reference's own
+    fit (tests/golden/empirical_fits.npz): the same fixed point (loglik 1e-8); the trajectory's length differs with
+    the summation order of the dots (Anderson's accept test compares nearly equal J values)."""
+    import blml_amd
+    from blml_amd.empirical import BLMLEmpirical
+    from cases_empirical import empirical_cases
+    z = np.load(os.path.join(HERE, 'golden', 'empirical_fits.npz'))
+    name, counts, mm, est_kw, fit_kw = empirical_cases()[0]
+    modes = [md for kw in mm for md in blml_amd.make_modes(**kw)]
+    out = _emp_replay(replay, BLMLEmpirical(**est_kw), counts, modes, **fit_kw)
+    r = out['results'][0]
+    ref = float(z['c0_loglik'])
+    assert abs(r['loglik'] - ref) < 1e-8 * abs(ref)
+    assert np.max(np.abs(out['density'][:counts.size] - z['c0_density'])) < 1e-5 * z['c0_density'].max()
+    J = out['merit_history'][0]
+    assert len(J) == int(r['start_sweeps']) + 1 and np.all(np.diff(J) <= 1e-12 * np.abs(J[:-1]))
