@@ -53,6 +53,7 @@ class EmpParams(ctypes.Structure):
 
 
 EMP_AUX_LEN, EMP_OUT_LEN = MAXK + 1, 4
+TUNE_MAX = 4
 
 RESULT_DTYPE = np.dtype([(name, (np.float64, 5) if name == 'seconds' else
                           (np.float64 if t is ctypes.c_double else np.int32)) for name, t in Result._fields_])
@@ -65,7 +66,7 @@ EXPORTS = ('blml_plan_create', 'blml_plan_destroy', 'blml_matvec', 'blml_fit', '
            'blml_pair_rows_noalloc', 'blml_point_rows', 'blml_grad_dots', 'blml_quick_psd_sweep',
            'blml_quick_psd_exact', 'blml_quick_psd_stats', 'blml_quick_psd_grad', 'blml_quick_psd_wscale',
            'blml_vec_gram', 'blml_vec_lincomb', 'blml_lbfgs_twoloop', 'blml_ift_elem', 'blml_empirical',
-           'blml_last_error')
+           'blml_plan_autotune', 'blml_last_error')
 
 _lib = None
 
@@ -87,6 +88,7 @@ def load(path=None):
     lib.blml_matvec.argtypes = [vp, dp, dp, vp]
     lib.blml_fit.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, vp]
     lib.blml_certificate.argtypes = [vp, dp, dp, i32, ctypes.c_double, dp, dp, vp]
+    lib.blml_plan_autotune.argtypes = [vp, i32, dp, ctypes.POINTER(i32), vp]
     lib.blml_empirical.argtypes = [vp, ctypes.POINTER(EmpParams), dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, vp]
     lib.blml_plan_info.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i32),
                                    ctypes.POINTER(i32)]

@@ -252,6 +252,22 @@ class FitBatch:
         self.d_rh = torch.zeros(max(1, self.sizes['hist']), dtype=f64, device=dev)
         return self
 
+    def autotune(self, reps=3):
+        """The plan's autotuner (blml_plan_autotune): times the mat-vec under each candidate configuration (FFT
+        radix-stage order; LDS-resident intermediates for batched plans) and keeps the fastest.  Sets self.tuned =
+        dict(choice=(sched, resident), seconds={candidate: mean mat-vec seconds})."""
+        torch = _torch()
+        times = np.full(_lib.TUNE_MAX, np.nan)
+        choice = ctypes.c_int32()
+        _lib.check(_lib.load().blml_plan_autotune(self.plan, int(reps), times.ctypes.data, ctypes.byref(choice),
+                                                  self.stream()))
+        torch.cuda.synchronize()
+        code = choice.value
+        self.tuned = dict(choice=dict(fft_sched=code // 2, lds_resident=code % 2 == 0),
+                          seconds={f'sched{c // 2}_{"resident" if c % 2 == 0 else "global"}': float(t)
+                                   for c, t in enumerate(times) if np.isfinite(t)})
+        return self
+
     def refresh_init(self):
         """Copies the specs' current inits to the device (the same layout as at build(): every spec keeps its
         init_K, so only the values change; a warm-start refresh between launches)."""

@@ -366,6 +366,8 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
             if (need <= cap_mv) need_mv = std::max(need_mv, need);
             if (need <= cap_fit) need_fit = std::max(need_fit, need);
         }
+        pl.res_need_mv = (int)need_mv;
+        pl.res_need_fit = (int)need_fit;
         pl.res_mv = use.find('m') != std::string::npos ? (int)need_mv : 0;
         pl.res_fit = use.find('f') != std::string::npos ? (int)need_fit : 0;
     }
@@ -471,6 +473,13 @@ int empirical_launch(Plan &pl, const blml_emp_params_t &ep, const double *counts
     return BLML_OK;
 }
 
+// a deterministic, denormal-free test vector for the autotuner's timed mat-vecs
+template <typename T>
+__global__ void tune_fill_kernel(T *E, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        E[i] = (T)(0.5 + 0.25 * sin(0.37 * (double)(i % 9973)));
+}
+
 // ------------------------------------------------------------------------------------------------ C-ABI
 extern "C" {
 
@@ -512,6 +521,82 @@ int blml_plan_info(void *plan, int64_t *out_elems, int64_t *ws_bytes, int32_t *l
     if (ws_bytes) *ws_bytes = pl->ws_elems;
     if (lds_bytes) *lds_bytes = pl->lds_bytes;
     if (grid) *grid = pl->spread ? -pl->coop : pl->grid;   // negative: one fit spread over that many workgroups
+    return BLML_OK;
+}
+
+// The plan's autotuner: times the stand-alone mat-vec (the same passes the fit kernels run) under every candidate
+// configuration and keeps the fastest.  Candidates: the FFT passes' radix-stage order (fft_line's sched 0 / 1) and,
+// for batched plans, whether a fit's transform intermediates stay in LDS (the launch then reserves it).  A spread
+// plan on the separable path has no FFT passes: its one configuration is timed.  Every candidate computes the same
+// operator (the stage order changes the rounding of FP32 transforms only), so the choice is a speed choice.
+int blml_plan_autotune(void *plan, int32_t reps, double *times, int32_t *choice, void *stream) {
+    Plan *pl = (Plan *)plan;
+    if (!pl || !times || !choice) return fail(BLML_EVALUE, "null argument");
+    if (reps < 1) return fail(BLML_EVALUE, "reps must be >= 1");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t bytes = std::max<size_t>(16, (size_t)pl->out_elems * pl->esize);
+    StreamScratch sE(st), sU(st);
+    HIPCHECK(sE.alloc(bytes));
+    HIPCHECK(sU.alloc(bytes));
+    if (pl->dtype == BLML_F32)
+        tune_fill_kernel<float><<<256, 256, 0, st>>>((float *)sE.p, (long)pl->out_elems);
+    else
+        tune_fill_kernel<double><<<256, 256, 0, st>>>((double *)sE.p, (long)pl->out_elems);
+    HIPCHECK(hipGetLastError());
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    struct Cand { int sched; bool res; };
+    const bool batched = !pl->spread;
+    const bool fft = !pl->sep;
+    Cand cands[BLML_TUNE_MAX];
+    int nc = 0;
+    for (int sched = 0; sched < (fft ? 2 : 1); ++sched)
+        for (int res = 1; res >= 0; --res)
+            if (res == 0 || (batched && (pl->res_need_mv > 0 || pl->res_need_fit > 0))) cands[nc++] = {sched, res == 1};
+    for (int i = 0; i < BLML_TUNE_MAX; ++i) times[i] = NAN;
+    int best = 0;
+    int rc = BLML_OK;
+    for (int c = 0; c < nc && rc == BLML_OK; ++c) {
+        pl->fft_sched = cands[c].sched;
+        pl->res_mv = cands[c].res ? pl->res_need_mv : 0;
+        pl->res_fit = cands[c].res ? pl->res_need_fit : 0;
+        rc = blml_matvec(pl, sE.p, sU.p, st);                       // warm-up
+        if (rc == BLML_OK && hipEventRecord(e0, st) != hipSuccess) rc = fail(BLML_ERUNTIME, "hipEventRecord");
+        for (int r = 0; r < reps && rc == BLML_OK; ++r) rc = blml_matvec(pl, sE.p, sU.p, st);
+        if (rc == BLML_OK && hipEventRecord(e1, st) != hipSuccess) rc = fail(BLML_ERUNTIME, "hipEventRecord");
+        if (rc == BLML_OK && hipEventSynchronize(e1) != hipSuccess) rc = fail(BLML_ERUNTIME, "hipEventSynchronize");
+        float ms = 0.0f;
+        if (rc == BLML_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(BLML_ERUNTIME, "hipEventElapsedTime");
+        if (rc == BLML_OK) {
+            times[c] = (double)ms * 1e-3 / reps;
+            if (times[c] < times[best]) best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == BLML_OK && nc > 0) {
+        pl->fft_sched = cands[best].sched;
+        pl->res_mv = cands[best].res ? pl->res_need_mv : 0;
+        pl->res_fit = cands[best].res ? pl->res_need_fit : 0;
+        *choice = cands[best].sched * 2 + (cands[best].res ? 0 : 1);
+        if (batched) {      // the persistent fit grid follows the fit launch's LDS
+            int dev = 0, ncu = 0, occ = 0;
+            HIPCHECK(hipGetDevice(&dev));
+            HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            const void *fk = pl->dtype == BLML_F32 ? (const void *)fit_kernel<float> : (const void *)fit_kernel<double>;
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fk, BLML_NT, lds_fit_launch(*pl)));
+            pl->grid = std::max(1, std::min(pl->nfits, ncu * std::max(occ, 1)));
+        }
+        // the times in candidate-code order: code = sched * 2 + (resident ? 0 : 1)
+        double byc[BLML_TUNE_MAX];
+        for (int i = 0; i < BLML_TUNE_MAX; ++i) byc[i] = NAN;
+        for (int c = 0; c < nc; ++c) byc[cands[c].sched * 2 + (cands[c].res ? 0 : 1)] = times[c];
+        for (int i = 0; i < BLML_TUNE_MAX; ++i) times[i] = byc[i];
+    }
+    if (rc) return rc;
+    HIPCHECK(sE.release());
+    HIPCHECK(sU.release());
     return BLML_OK;
 }
 
@@ -676,6 +761,7 @@ static int pair_check(const blml_pairtab_t *t) {
     if (t->G < 0 || t->G > BLML_PAIR_MAXG)
         return fail(BLML_EVALUE, "pair tables: at most " + std::to_string(BLML_PAIR_MAXG) + " radial groups");
     if (!t->T || t->nT < 1) return fail(BLML_EVALUE, "pair tables: empty T");
+    if (t->nT >= (1L << 31) - 1) return fail(BLML_EVALUE, "pair tables: at most 2^31 - 2 values in T");
     if (t->G > 0 && (!t->RT || t->LR < 2)) return fail(BLML_EVALUE, "pair tables: radial rows missing");
     for (int k = 0; k < t->d; ++k) {
         const int kd = t->kind[k];
@@ -709,7 +795,8 @@ static int pair_rows_launch(const blml_pairtab_t &t, const int32_t *ZA, const in
     pair_rows_kernel<R_, TL, NG><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part)
 #define BLML_ROWS_BY_G(R_, TL) \
     do { \
-        if (t.G == 0) BLML_ROWS_LAUNCH(R_, TL, 0); \
+        if (t.G == 0) pair_rows_u_kernel<R_, TL, 4><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, \
+                                                                                   (int)slice, part); \
         else if (t.G == 1) BLML_ROWS_LAUNCH(R_, TL, 1); \
         else BLML_ROWS_LAUNCH(R_, TL, BLML_PAIR_MAXG); \
     } while (0)

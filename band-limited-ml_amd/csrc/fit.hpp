@@ -24,6 +24,7 @@ struct DevPlan {
     const T *sep_tab;
     const long *sep_off;
     int res_off, res_cap;  // LDS-resident intermediates: byte offset in the dynamic LDS and its size (0: none)
+    int fft_sched;         // the radix-stage order of the FFT passes (the plan's autotuned choice)
 };
 
 // A fit's workspace (bytes from ws_off): [X, Y: scratch_len complex values of the transform type]
@@ -99,6 +100,7 @@ __device__ void spec_factor(const DevPlan<T> &p, int fi, const int *fac_m, cpx<T
         ps.pairs = false;
         ps.rank = 0;
         ps.nranks = 1;
+        ps.sched = p.fft_sched;
         line_geom<T>(ext, fa.naxes, a, ps.outer, ps.inner);
         run_pass<T>(ps, cur, oth, p.tw, p.Lmax);
         cpx<T> *t = cur; cur = oth; oth = t;
@@ -125,6 +127,7 @@ __device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, Matvec
     mv.Lmax = p.Lmax;
     mv.X = (cpx<T> *)(p.ws + f.ws_off);
     mv.Y = mv.X + f.scratch_len;
+    mv.fft_sched = p.fft_sched;
     // a fit run by one workgroup whose intermediates fit the launch's resident LDS keeps them there: every pass
     // then stages its lines LDS to LDS, and only E is read and U written in HBM
     if (p.res_cap > 0 && !sep && tm.nranks == 1 && res_need(f, p.facs, (int)sizeof(T)) <= p.res_cap) {
@@ -426,6 +429,7 @@ __device__ void run_finish64(const DevPlan<double> &p, int fi, Team tm, float *s
     mv.Lmax = p.Lmax;
     mv.X = (cpx<double> *)(p.ws + f.ws_off + fin_bytes_off(p.params, f, 4));
     mv.Y = mv.X + f.scratch_len;
+    mv.fft_sched = p.fft_sched;
     const int nc = f.complex_g ? 2 : 1;
     const long V = (long)f.K * f.B * nc;
     // the fit's FP64 state: E in pool slot 0; U64 overwrites slot 1

@@ -97,6 +97,93 @@ __device__ __forceinline__ void pair_value(const blml_pairtab_t &t, const R_ *T,
     vdep = vd;
 }
 
+// The table index of a column factor (kinds 0, 1, 3) in 32-bit arithmetic (the binding keeps every table below 2^31
+// entries and every coordinate below 2^30), brought into [0, nT) by one unsigned minimum: a coordinate outside the
+// table's span (a stale table, which the binding rejects before the launch) reads a table entry, never past it.
+__device__ __forceinline__ int pair_col_index32(int kd, int za, int zb, int baseJ, int base, int mk, int nb,
+                                                unsigned nT1) {
+    int i;
+    if (kd == 0) {
+        i = baseJ + (za - zb);
+    } else if (kd == 3) {
+        i = base + za * mk + zb;
+    } else {
+        int m = za - zb;
+        if (m < 0) m += nb;
+        else if (m >= nb) m -= nb;
+        if (m > nb - m) m = nb - m;
+        i = base + m;
+    }
+    return (int)min((unsigned)i, nT1);
+}
+
+// pair_rows_kernel for tables without radial or torus groups (every column of kind 0, 1 or 3), U sources at a time:
+// per column one uniform branch on its kind, then the U sources' coordinates, indexes and table reads issue
+// together (independent LDS reads in flight instead of one dependent chain per source).  Each pair's product runs
+// in column order and the FP64 partial adds the sources in order, skipping zero weights: the same sums as
+// pair_rows_kernel<R_, TL, 0>, to the bit.
+template <typename R_, bool TL, int U>
+__global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_u_kernel(blml_pairtab_t t, const int *ZA, const int *ZB,
+                                                                   const double *w, int R, int M, int slice,
+                                                                   double *part) {
+    unsigned char *lds = g_dyn_lds;
+    const long tbytes = TL ? ((t.nT * (long)sizeof(R_) + 15) / 16) * 16 : 0;
+    R_ *lT = reinterpret_cast<R_ *>(lds);
+    int *lz = reinterpret_cast<int *>(lds + tbytes);
+    double *lw = reinterpret_cast<double *>(lds + tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16);
+    const R_ *T = reinterpret_cast<const R_ *>(t.T);
+    if (TL) {
+        for (long i = threadIdx.x; i < t.nT; i += BLML_PAIR_NT) lT[i] = T[i];
+        T = lT;
+    }
+    const int d = t.d;
+    const unsigned nT1 = (unsigned)(t.nT - 1);
+    const int r = blockIdx.x * BLML_PAIR_NT + threadIdx.x;
+    const bool has = r < R;
+    int za[BLML_PAIR_MAXD];
+#pragma unroll
+    for (int k = 0; k < BLML_PAIR_MAXD; ++k) za[k] = (has && k < d) ? ZA[(long)r * d + k] : 0;
+    const long j0 = (long)blockIdx.y * slice;
+    const long j1 = j0 + slice < M ? j0 + slice : M;
+    double acc = 0.0;
+    for (long c0 = j0; c0 < j1; c0 += BLML_PAIR_CH) {
+        const int n = (int)(j1 - c0 < BLML_PAIR_CH ? j1 - c0 : BLML_PAIR_CH);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n * d; i += BLML_PAIR_NT) lz[i] = ZB[c0 * d + i];
+        for (int i = threadIdx.x; i < n; i += BLML_PAIR_NT) lw[i] = w[c0 + i];
+        __syncthreads();
+        for (int jj = 0; jj < n; jj += U) {
+            R_ v[U];
+            double wv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                v[u] = (R_)1;
+                wv[u] = jj + u < n ? lw[jj + u] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < BLML_PAIR_MAXD; ++k) {
+                if (k < d) {
+                    const int kd = t.kind[k];
+                    const int base = (int)t.base[k], baseJ = (int)(t.base[k] + t.J[k]);
+                    const int mk = (int)t.mk[k], nb = (int)t.nbins[k];
+                    int idx[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        idx[u] = jj + u < n ? pair_col_index32(kd, za[k], lz[(jj + u) * d + k], baseJ, base, mk, nb,
+                                                               nT1)
+                                            : 0;
+#pragma unroll
+                    for (int u = 0; u < U; ++u) v[u] *= T[idx[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (wv[u] != 0.0) acc += (double)v[u] * wv[u];
+        }
+    }
+    if (has) part[(long)blockIdx.y * R + r] = acc;
+}
+
 // part[slice][r] = sum over the slice's sources j (w_j != 0) of S(ZA_r, ZB_j) w_j.  LDS: [T if TL][ZB chunk][w chunk].
 template <typename R_, bool TL, int NG>
 __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_kernel(blml_pairtab_t t, const int *ZA, const int *ZB,

@@ -54,6 +54,8 @@ struct Plan {
     int lds_pipe = 0;              // dynamic LDS of matvec_pipe_kernel (largest tile, then every axis's table)
     int res_off = 0;               // LDS-resident intermediates of the batched kernels: byte offset (after the lines)
     int res_mv = 0, res_fit = 0;   // ... and the resident bytes matvec_kernel / fit_kernel launch with (0: none)
+    int res_need_mv = 0, res_need_fit = 0;   // the resident bytes the plan's fits allow (the autotuner's options)
+    int fft_sched = 0;             // the radix-stage order of the FFT passes (fft_line's sched; autotuned)
     int64_t out_elems = 0, dens_elems = 0;
     int64_t ws_elems = 0;
     size_t esize = 4;
@@ -82,6 +84,7 @@ DevPlan<T> dev_plan(const Plan &pl) {
     p.sep_tab = (const T *)pl.d_sep_tab;
     p.sep_off = pl.d_sep_off;
     p.res_off = p.res_cap = 0;
+    p.fft_sched = pl.fft_sched;
     return p;
 }
 
@@ -102,6 +105,7 @@ DevPlan<double> dev_plan64(const Plan &pl) {
     p.sep_tab = nullptr;
     p.sep_off = nullptr;
     p.res_off = p.res_cap = 0;
+    p.fft_sched = pl.fft_sched;
     return p;
 }
 

@@ -189,8 +189,11 @@ __device__ __noinline__ void stockham_stage(int xoff, int L, int Ns, int j0, int
 
 // Whole FFT of a line of length L (<= 16 BLML_NT) in LDS, all threads participating (lines of a round share the
 // schedule).  dir = -1 forward, +1 inverse (unscaled).  mul (optional): spectrum applied on the first stage's reads.
+// sched: the order of the radix stages -- 0: radix 16 while 16 divides what is left, the remainder (8, 4 or 2)
+// last; 1: the remainder first, then radix 16 (the same stages, another register / LDS pattern; the plan's
+// autotuner picks by timing).
 template <typename T>
-__device__ void fft_line(int xoff, int L, int j0, int team, int toff, int dir, const T *mul) {
+__device__ void fft_line(int xoff, int L, int j0, int team, int toff, int dir, const T *mul, int sched = 0) {
     if (L < 16) {                       // one thread per line, one stage of radix L
         switch (L) {
             case 2: stockham_stage<T, 2, 1>(xoff, L, 1, 0, 1, toff, dir, mul); break;
@@ -202,10 +205,18 @@ __device__ void fft_line(int xoff, int L, int j0, int team, int toff, int dir, c
     }
     int Ns = 1;
     bool first = true;
+    // the remainder radix of L = 16^k r (r = 1, 2, 4 or 8)
+    int r0 = L;
+    while (r0 >= 16) r0 >>= 4;
     while (Ns < L) {
         const int rem = L / Ns;
         const T *m_ = first ? mul : nullptr;
-        if (rem >= 16) stockham_stage<T, 16, 1>(xoff, L, Ns, j0, team, toff, dir, m_), Ns *= 16;
+        if (sched == 1 && first && r0 > 1) {
+            if (r0 == 8) stockham_stage<T, 8, 2>(xoff, L, Ns, j0, team, toff, dir, m_);
+            else if (r0 == 4) stockham_stage<T, 4, 4>(xoff, L, Ns, j0, team, toff, dir, m_);
+            else stockham_stage<T, 2, 8>(xoff, L, Ns, j0, team, toff, dir, m_);
+            Ns *= r0;
+        } else if (rem >= 16) stockham_stage<T, 16, 1>(xoff, L, Ns, j0, team, toff, dir, m_), Ns *= 16;
         else if (rem == 8) stockham_stage<T, 8, 2>(xoff, L, Ns, j0, team, toff, dir, m_), Ns *= 8;
         else if (rem == 4) stockham_stage<T, 4, 4>(xoff, L, Ns, j0, team, toff, dir, m_), Ns *= 4;
         else stockham_stage<T, 2, 8>(xoff, L, Ns, j0, team, toff, dir, m_), Ns *= 2;
@@ -224,6 +235,7 @@ struct Pass {
     int rank, nranks;     // the team member running this call: rounds rank, rank + nranks, ... of the lines
     bool pairs;           // CONV ops on real data: two lines per transform, as real and imaginary part (exact: the
                           // operator is real, so S (x + i y) = S x + i S y)
+    int sched;            // FFT ops: the radix-stage order of fft_line
 };
 
 // Moves a round of lines between global memory and LDS (line l of the round at lbuf[l * stride_l + off + i]).
@@ -349,10 +361,10 @@ __device__ __noinline__ void run_pass(const Pass<T> &p, const cpx<T> *src, cpx<T
             const int j0 = has_line ? tid : 0;
             const int boff = (int)(buf - lds), roff = 0;
             if (p.op == OP_FFT_INV) {
-                fft_line<T>(boff, p.L, j0, team, roff, +1, nullptr);
+                fft_line<T>(boff, p.L, j0, team, roff, +1, nullptr, p.sched);
             } else {
-                fft_line<T>(boff, p.L, j0, team, roff, -1, nullptr);
-                if (p.op == OP_CONV_PAD) fft_line<T>(boff, p.L, j0, team, roff, +1, p.spec);
+                fft_line<T>(boff, p.L, j0, team, roff, -1, nullptr, p.sched);
+                if (p.op == OP_CONV_PAD) fft_line<T>(boff, p.L, j0, team, roff, +1, p.spec, p.sched);
             }
             const int nout = (p.op == OP_FFT_FWD || p.op == OP_FFT_FULL) ? p.L : p.m;
             round_io<T>(false, p.inner, nl, nout, nout, 0, lines, sh.buf, lofs_out, lo2, nullptr, dst, true);
@@ -416,6 +428,7 @@ struct MatvecCtx {
     const cpx<T> *tw;          // full-circle twiddles of the longest FFT
     int Lmax;
     cpx<T> *X, *Y;             // scratch (scratch_len complex each)
+    int fft_sched;             // the radix-stage order of every FFT pass (fft_line's sched)
     Team tm;                   // who runs the mat-vec (passes split their lines over the team)
     const SepPlan *sep;        // separable modes: dense line operators on LDS tiles (null: transform passes)
     bool sep_last_sync;        // team barrier after a separable mat-vec's last pass (false: the kernel ends there)
@@ -503,6 +516,7 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa, boo
         Pass<T> p;
         p.rank = c.tm.rank;
         p.nranks = c.tm.nranks;
+        p.sched = c.fft_sched;
         p.pairs = real;
         p.m = c.shape[a];
         p.n_in = p.n_out = p.m;
@@ -531,6 +545,7 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa, boo
         Pass<T> p;
         p.rank = c.tm.rank;
         p.nranks = c.tm.nranks;
+        p.sched = c.fft_sched;
         p.m = c.shape[a];
         p.n_in = ext[a];
         line_geom<T>(ext, c.d, a, p.outer, p.inner);
@@ -554,6 +569,7 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa, boo
         Pass<T> p;
         p.rank = c.tm.rank;
         p.nranks = c.tm.nranks;
+        p.sched = c.fft_sched;
         p.m = c.shape[a];
         p.n_in = ext[a];
         p.n_out = p.m;

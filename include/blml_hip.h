@@ -129,6 +129,15 @@ int blml_fit(void *plan, const double *counts, const double *draws, const double
              void *e_out, void *se_out, double *density, blml_result_t *results, double *merit_hist,
              double *rho_hist, void *stream);
 
+/* The plan's autotuner (no counterpart in the reference, whose transforms are library FFTs): times blml_matvec
+ * `reps` times under each candidate configuration of the plan -- the radix-stage order of its FFT passes and, for a
+ * batched plan, whether a fit's transform intermediates stay in LDS -- and keeps the fastest for every later launch
+ * (mat-vec, fit, certificate).  times: BLML_TUNE_MAX doubles, the mean seconds per mat-vec of candidate code
+ * (sched * 2 + (resident ? 0 : 1)), NaN for a candidate the plan does not have; choice: the code kept.  Allocates
+ * two mat-vec buffers for the duration of the call.                                                           */
+#define BLML_TUNE_MAX 4
+int blml_plan_autotune(void *plan, int32_t reps, double *times, int32_t *choice, void *stream);
+
 /* certificate(): lambda_max(D^1/2 S_0 D^1/2) / n per fit (blml_binned.py:501-518, which calls eigsh(k=1, tol=tol)) by
  * power iteration on the device, FP64 transforms (an FP32 plan uses its FP64 spectra) and an FP64 Rayleigh
  * quotient; a fit stops when its eigen-residual ||A x - lam x|| <= tol * lam for the unit iterate x (ARPACK's test

@@ -17,6 +17,7 @@
 #include "empirical.hpp"
 
 #include <stdio.h>
+#include <stdlib.h>
 
 #include <vector>
 
@@ -130,6 +131,8 @@ int replay(In &in, const char *out_path) {
     pl.hfits = in.fits;
     pl.hfacs = in.facs;
     pl.esize = sizeof(T);
+    // the FFT passes' radix-stage order (the autotuner's knob; fft_line's sched)
+    if (const char *sc = getenv("BLML_REPLAY_FFT_SCHED")) pl.fft_sched = atoi(sc);
     std::vector<int> fac_m;
     std::vector<long> tmp_off;
     long tmp_total = 0;

@@ -99,7 +99,7 @@ def write_plan(batch, path, cert_iters=0, cert_tol=1e-6, ranks=4, emp=None):
             f.write(_bytes(np.ascontiguousarray(emp[1], np.float64).ravel()))
 
 
-def run(batch, cert_iters=0, cert_tol=1e-6, timeout=1800, exe=None, ranks=4, emp=None):
+def run(batch, cert_iters=0, cert_tol=1e-6, timeout=1800, exe=None, ranks=4, emp=None, fft_sched=0):
     """Replays the batch's fits on the CPU; returns dict(results (structured array), e, density, lam, iters, log).
     Raises RuntimeError with the replay's output when it fails (an AddressSanitizer report included)."""
     from blml_amd import _lib
@@ -108,7 +108,8 @@ def run(batch, cert_iters=0, cert_tol=1e-6, timeout=1800, exe=None, ranks=4, emp
     try:
         pin, pout = os.path.join(tmp, 'plan.bin'), os.path.join(tmp, 'out.bin')
         write_plan(batch, pin, cert_iters, cert_tol, ranks, emp)
-        env = dict(os.environ, ASAN_OPTIONS='detect_leaks=0:abort_on_error=0:detect_stack_use_after_return=0')
+        env = dict(os.environ, ASAN_OPTIONS='detect_leaks=0:abort_on_error=0:detect_stack_use_after_return=0',
+                   BLML_REPLAY_FFT_SCHED=str(int(fft_sched)))
         p = subprocess.run([exe, pin, pout], capture_output=True, text=True, timeout=timeout, env=env)
         log = p.stdout + p.stderr
         if p.returncode != 0:

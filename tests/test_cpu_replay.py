@@ -73,6 +73,24 @@ def test_complex_slits_lbfgs_one_workgroup_in_bounds(replay):
     assert np.all(np.isfinite(out['density'][:counts.size]))
 
 
+@pytest.mark.parametrize('dtype', [np.float64, np.float32], ids=['f64', 'f32'])
+def test_fft_stage_order_gives_the_same_fit(replay, dtype):
+    """The autotuner's FFT knob (the remainder radix stage first instead of last) on padded FFT lines of 128 and 64
+    values, band DFTs and a group factor: every access in bounds and the same fit to the transforms' rounding."""
+    import blml_amd
+    rng = np.random.default_rng(5)
+    counts = _field_counts(rng, (40, 30), 20000)
+    modes = (blml_amd.make_modes([0.35, 0.3], ['slits:0.5', 'slits:0.4'])
+             + blml_amd.make_modes([0.3, 0.3], ['pball', 'pball'], periods=[40.0, 30.0]))
+    params = dict(PARAMS, solver='lbfgs', max_sweeps=12, ll_tol=1e-10)
+    a = replay.run(_batch(counts, modes, True, dtype, -1, params))
+    b = replay.run(_batch(counts, modes, True, dtype, -1, params), fft_sched=1)
+    tol = 1e-12 if dtype == np.float64 else 1e-5
+    la, lb = a['results']['loglik'][0], b['results']['loglik'][0]
+    assert abs(la - lb) < tol * abs(la)
+    assert np.max(np.abs(a['density'] - b['density'])) < tol * np.max(a['density']) * 10
+
+
 def test_spread_separable_fit_equals_one_workgroup(replay):
     """A 4-D sinc fit spread over a grid of 4 workgroups (FP64 separable tile passes on emulated matrix cores, grid
     barriers, rank-ordered team sums) against the same fit on one workgroup (transform passes)."""
