@@ -800,6 +800,31 @@ static int pair_rows_launch(const blml_pairtab_t &t, const int32_t *ZA, const in
         else if (t.G == 1) BLML_ROWS_LAUNCH(R_, TL, 1); \
         else BLML_ROWS_LAUNCH(R_, TL, BLML_PAIR_MAXG); \
     } while (0)
+    // tables in LDS whose columns are all offset rows: pair_rows_off_kernel, by column count
+    bool all_off = tl && t.G == 0 && t.d <= 5;
+    for (int k = 0; k < t.d && all_off; ++k) all_off = t.kind[k] == 0;
+    if (all_off) {
+#define BLML_OFF_LAUNCH(R_, D) \
+        pair_rows_off_kernel<R_, D, 4><<<grid, BLML_PAIR_NT, lds, st>>>(t, ZA, ZB, w, (int)R, (int)M, (int)slice, part)
+#define BLML_OFF_BY_D(R_) \
+        do { \
+            switch (t.d) { \
+                case 1: BLML_OFF_LAUNCH(R_, 1); break; \
+                case 2: BLML_OFF_LAUNCH(R_, 2); break; \
+                case 3: BLML_OFF_LAUNCH(R_, 3); break; \
+                case 4: BLML_OFF_LAUNCH(R_, 4); break; \
+                default: BLML_OFF_LAUNCH(R_, 5); break; \
+            } \
+        } while (0)
+        if (t.f32) BLML_OFF_BY_D(float);
+        else BLML_OFF_BY_D(double);
+#undef BLML_OFF_BY_D
+#undef BLML_OFF_LAUNCH
+        HIPCHECK(hipGetLastError());
+        pair_rows_sum_kernel<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(part, (int)R, (int)nsl, s);
+        HIPCHECK(hipGetLastError());
+        return BLML_OK;
+    }
     // pair_check: a column of kind 2 or 4 names a group < G, so G = 0 means no such column
     if (t.f32) {
         if (tl) BLML_ROWS_BY_G(float, true);

@@ -184,6 +184,69 @@ __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_u_kernel(blml_pairtab_
     if (has) part[(long)blockIdx.y * R + r] = acc;
 }
 
+// The rows of a table in LDS whose D columns are all offset rows (kind 0: sinc / slits lattices, the commonest
+// table): each thread folds its row's coordinates and the columns' bases into one constant per column, so a pair
+// costs per column one subtraction, one LDS address, one table read and one product -- no per-column kind branch,
+// no clamp (an index outside the table, which only a stale table gives and the binding rejects, reads other LDS of
+// this workgroup or, past its allocation, zero: never memory outside it).  U sources at a time; the same products
+// (column order) and the same ordered FP64 sums as pair_rows_kernel, to the bit (with finite tables).
+template <typename R_, int D, int U>
+__global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_off_kernel(blml_pairtab_t t, const int *ZA, const int *ZB,
+                                                                     const double *w, int R, int M, int slice,
+                                                                     double *part) {
+    unsigned char *lds = g_dyn_lds;
+    const long tbytes = ((t.nT * (long)sizeof(R_) + 15) / 16) * 16;
+    R_ *lT = reinterpret_cast<R_ *>(lds);
+    int *lz = reinterpret_cast<int *>(lds + tbytes);
+    double *lw = reinterpret_cast<double *>(lds + tbytes + ((BLML_PAIR_CH * BLML_PAIR_MAXD * 4 + 15) / 16) * 16);
+    const R_ *T = reinterpret_cast<const R_ *>(t.T);
+    for (long i = threadIdx.x; i < t.nT; i += BLML_PAIR_NT) lT[i] = T[i];
+    const int r = blockIdx.x * BLML_PAIR_NT + threadIdx.x;
+    const bool has = r < R;
+    // byte offsets: the sources' coordinates staged as zb sizeof(R_), so a pair's table entry sits zb sizeof(R_)
+    // bytes before the row's entry point (base + J + za) sizeof(R_) of each column
+    constexpr int ES = (int)sizeof(R_);
+    const unsigned char *pk[D];       // the row's entry point into each column's table (LDS)
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+        pk[k] = reinterpret_cast<const unsigned char *>(lT) +
+                ((int)(t.base[k] + t.J[k]) + (has ? ZA[(long)r * D + k] : 0)) * ES;
+    const long j0 = (long)blockIdx.y * slice;
+    const long j1 = j0 + slice < M ? j0 + slice : M;
+    double acc = 0.0;
+    for (long c0 = j0; c0 < j1; c0 += BLML_PAIR_CH) {
+        const int n = (int)(j1 - c0 < BLML_PAIR_CH ? j1 - c0 : BLML_PAIR_CH);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n * D; i += BLML_PAIR_NT) lz[i] = ZB[c0 * D + i] * ES;
+        for (int i = threadIdx.x; i < n; i += BLML_PAIR_NT) lw[i] = w[c0 + i];
+        // past the chunk's end (up to U - 1 sources): the chunk's first source again, so that every table read of
+        // the last group of U stays inside the table (their weights are taken as zero and skipped below)
+        for (int i = n * D + threadIdx.x; i < (n + U) * D && i < BLML_PAIR_CH * BLML_PAIR_MAXD; i += BLML_PAIR_NT)
+            lz[i] = ZB[c0 * D + i % D] * ES;
+        __syncthreads();
+        for (int jj = 0; jj < n; jj += U) {
+            R_ v[U];
+            double wv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                v[u] = (R_)1;
+                wv[u] = jj + u < n ? lw[jj + u] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    v[u] *= *reinterpret_cast<const R_ *>(pk[k] - lz[(jj + u) * D + k]);
+            }
+            // a zero weight adds a zero product, which leaves the sum as it is (the tables are finite): the same
+            // sum as skipping it
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc += (double)v[u] * wv[u];
+        }
+    }
+    if (has) part[(long)blockIdx.y * R + r] = acc;
+}
+
 // part[slice][r] = sum over the slice's sources j (w_j != 0) of S(ZA_r, ZB_j) w_j.  LDS: [T if TL][ZB chunk][w chunk].
 template <typename R_, bool TL, int NG>
 __global__ void __launch_bounds__(BLML_PAIR_NT) pair_rows_kernel(blml_pairtab_t t, const int *ZA, const int *ZB,

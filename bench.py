@@ -194,6 +194,23 @@ def measured_traffic(kernel, workload='32x5_sinc'):
     return None
 
 
+def measured_counter(stem, key):
+    """A per-launch counter figure from a rocprofv3 summary in profiles/<stem>.txt (a line 'key: value'), or None."""
+    try:
+        with open(os.path.join(PROFILES, f'{stem}.txt')) as f:
+            for line in f:
+                if line.startswith(key + ':'):
+                    return float(line.split(':', 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+# VALU issue peak of the MI355X: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4 cycles (16 lanes per
+# cycle), 2.4 GHz
+VALU_PEAK_WAVE_INST = 256 * 4 / 4 * 2.4e9
+
+
 def large_matvec(torch, blml_amd, _FitSpec, FitBatch, params, event_time, reps=20):
     """The 32^5 single fit's plan and its mat-vec timed with HIP events on the launch stream.  Returns
     (spec, fit params, description, batch, roofline dict, seconds per mat-vec)."""
@@ -264,6 +281,17 @@ def pair_leg(torch, st, event_time):
     pairs = dict(bins=int(len(Zp)), rows=int(n_rows), sources=int(len(Zp)), draws=2_000_000, lattice=[PAIR_BINS] * 3,
                  kernel='slits fill 0.5, fc 0.2 cycles/bin per column (offset rows)', precision='f32 tables, f64 sums',
                  seconds=t_p, pairs_per_s=n_rows * len(Zp) / t_p)
+    # the rows kernel's roofline: VALU issue (the kernel is bound by its per-pair index arithmetic, LDS lookups and
+    # products; HBM carries only the sources).  VALU wave instructions per launch from rocprofv3 --pmc SQ_INSTS_VALU
+    # (profiles/pair_rows_off_kernel_128cubed_slits_counters.txt), over this run's launch time
+    vi = measured_counter('pair_rows_off_kernel_128cubed_slits_counters', 'valu_insts_per_launch')
+    li = measured_counter('pair_rows_off_kernel_128cubed_slits_counters', 'lds_insts_per_launch')
+    if vi:
+        pairs['roofline'] = dict(bound='valu', kernel='pair_rows_off_kernel<float, 3, 4>',
+                                 valu_insts_per_launch=vi, lds_insts_per_launch=li,
+                                 valu_insts_per_pair=vi * 64 / (n_rows * len(Zp)),
+                                 achieved=vi / t_p / 1e9, peak=VALU_PEAK_WAVE_INST / 1e9,
+                                 unit='G wave64 VALU instructions/s', frac=vi / t_p / VALU_PEAK_WAVE_INST)
     del keep
     print(f'bench: pair rows {n_rows} x {len(Zp)} in {t_p * 1e3:.1f} ms', file=sys.stderr, flush=True)
     # derivative rows on the same bins (grad=True): dS/dlog fc of each column, three FP64 accumulators per row
@@ -339,8 +367,9 @@ def main():
             fits = uneven_fits(args.fits)
             specs = [_FitSpec(c, modes_of(g, blml_amd), 1.0, 0.0, kw['complex_g'], None, None, 0)
                      for c, g, kw, _ in fits]
-            bb = FitBatch(specs, fit_params(), np.float32, hist_cap=0).build()
+            bb = FitBatch(specs, fit_params(), np.float32, hist_cap=0).build().autotune()
             roof = batched_matvec(torch, bb, specs, ev_time, args.steps)
+            roof['autotune'] = bb.tuned
             bb.close()
         copy_known_size(torch)
         print(json.dumps(dict(roofline=roof)), flush=True)
@@ -361,9 +390,10 @@ def main():
     W.apply(net, specs)
     torch.cuda.synchronize()
     tb0 = time.perf_counter()
-    bw = FitBatch(specs, params, np.float32, hist_cap=0).build()
+    bw = FitBatch(specs, params, np.float32, hist_cap=0).build().autotune()
     torch.cuda.synchronize()
     t_build = time.perf_counter() - tb0
+    tuned = dict(warm_fp32=bw.tuned)
     pol = Polish(bw, params)
     torch.cuda.synchronize()
     t_build_judge = time.perf_counter() - tb0 - t_build
@@ -411,7 +441,8 @@ def main():
         fs.init, fs.init_K, fs.warm = None, 0, False
 
     def timed(dtype):
-        b = FitBatch(specs, params, dtype, hist_cap=0).build()
+        b = FitBatch(specs, params, dtype, hist_cap=0).build().autotune()
+        tuned['cold_' + np.dtype(dtype).name] = b.tuned
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         b.run()
@@ -429,7 +460,7 @@ def main():
                  warm_start_seconds=dtw / args.steps - t_launch - t_judge,
                  cold_fallbacks=int(judged['fallbacks']),
                  judge_pass_fraction=float(np.mean(np.abs(judged['polish_rel']) < 1e-6)),
-                 plan_build_seconds=dict(warm_fp32=t_build, judge_fp64=t_build_judge),
+                 plan_build_seconds=dict(warm_fp32=t_build, judge_fp64=t_build_judge), autotune=tuned,
                  warm_iterations=float(np.mean([r['n_sweeps'] + r['start_sweeps'] for r in rw])),
                  cold_iterations=float(np.mean([r['n_sweeps'] + r['start_sweeps'] for r in r32])),
                  warm_vs_cold_better=int((relw > 1e-6).sum()), warm_vs_cold_worse=int((relw < -1e-6).sum()),
