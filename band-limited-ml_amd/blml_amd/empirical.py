@@ -16,7 +16,7 @@ from . import _lib
 from .binned import BLMLBinned, FitBatch, _FitSpec
 from .units import mode_kf_max
 
-__all__ = ['BLMLEmpirical']
+__all__ = ['BLMLEmpirical', 'empirical_many']
 
 _REG = {'ridge': 0, 'l2': 1, 'l1': 2}
 
@@ -43,7 +43,7 @@ class BLMLEmpirical(BLMLBinned):
         fs, params, ep, aux = self._prepare(counts, modes, h, origin, init, complex_g)
         batch = FitBatch([fs], params, self.dtype, hist_cap=int(self.max_outer) + 2, spread=-1).build()
         batch.run_empirical(ep, aux[None])
-        return self._adopt_empirical(fs, batch, batch.results()[0], batch.empirical_outputs()[0], aux)
+        return self._adopt_empirical(fs, batch, batch.results()[0], batch.empirical_outputs()[0], aux, 0)
 
     def _prepare(self, counts, modes, h, origin, init, complex_g):
         """Host work of a fit: validation, the fit spec (tables, the start's draws), the warm start, the engine's
@@ -85,11 +85,11 @@ class BLMLEmpirical(BLMLBinned):
         ep.max_iter, ep.max_outer, ep.anderson = int(self.max_iter), int(self.max_outer), mA
         return fs, params, ep, aux
 
-    def _adopt_empirical(self, fs, batch, r, emp_out, aux):
-        """The fitted attributes from the engine's outputs (blml_binned.py:1277-1299)."""
+    def _adopt_empirical(self, fs, batch, r, emp_out, aux, index):
+        """The fitted attributes from the engine's outputs (blml_binned.py:1277-1299); index: the fit in batch."""
         n = fs.n
         Zw, Zp, crit, sc = (float(v) for v in emp_out)
-        self._adopt(fs, batch, r, 0)
+        self._adopt(fs, batch, r, index)
         self.Delta_ = float(aux[_lib.MAXK])
         self.window_norm_, self.plane_norm_ = Zw, Zp
         if not (Zw > 0 and Zp > 0):
@@ -145,3 +145,34 @@ class BLMLEmpirical(BLMLBinned):
                 best = (s, m)
         best[1].path_ = sorted(rows)
         return best[1]
+
+
+def empirical_many(jobs, dtype=np.float64, **kw):
+    """Many BLMLEmpirical fits (frames, windows, a grid of lattices) in one launch of blml_empirical per kind of fit:
+    jobs = [(counts, modes, fit keywords of BLMLEmpirical.fit)], kw the estimator's knobs (BLMLEmpirical.__init__).
+    Linear fits and hidden-phase (retrieve) fits run as two launches when both occur.  Returns one fitted
+    BLMLEmpirical per job, in order; each equals the same fit made alone."""
+    prepared = []
+    for counts, modes, fk in jobs:
+        fk = dict(fk)
+        e = BLMLEmpirical(dtype=dtype, **kw)
+        fs, params, ep, aux = e._prepare(counts, modes, fk.pop('h', 1.0), fk.pop('origin', 0.0), fk.pop('init', None),
+                                         fk.pop('complex_g', False))
+        if fk:
+            raise TypeError(f"unknown fit keywords {sorted(fk)}")
+        prepared.append((e, fs, params, ep, aux))
+    out = [None] * len(prepared)
+    for rt in (False, True):
+        idx = [i for i, p in enumerate(prepared) if p[0].retrieve_ == rt]
+        if not idx:
+            continue
+        e0, _, params, ep, _ = prepared[idx[0]]
+        fss = [prepared[i][1] for i in idx]
+        aux = np.stack([prepared[i][4] for i in idx])
+        batch = FitBatch(fss, params, dtype, hist_cap=int(e0.max_outer) + 2, spread=-1).build()
+        batch.run_empirical(ep, aux)
+        res, emp = batch.results(), batch.empirical_outputs()
+        for j, i in enumerate(idx):
+            e, fs = prepared[i][0], prepared[i][1]
+            out[i] = e._adopt_empirical(fs, batch, res[j], emp[j], prepared[i][4], j)
+    return out

@@ -122,3 +122,22 @@ def test_warm_start_select_and_outputs():
     og = OE.BLMLEmpiricalOracle(lam=best.lam, tol=1e-10).fit(train, O.make_modes([0.2], ['sinc']))
     g_hip, g_ref = best.predict_grid(3), og.predict_grid(3)
     assert np.max(np.abs(g_hip - g_ref)) < 1e-8 * g_ref.max()
+
+
+def test_many_fits_in_one_launch_equal_single_fits():
+    """empirical_many: linear and retrieve fits of uneven lattices in one launch each equal the same fits made
+    alone (the same device code per fit; FP64 sums in a fixed order)."""
+    import blml_amd
+    jobs = []
+    for i, (bins, cplx, kind) in enumerate([(48, False, 'sinc'), (64, True, 'sinc'), (40, False, 'slits:0.5'),
+                                            (56, True, 'slits:0.4')]):
+        jobs.append((_counts1(bins=bins, seed=10 + i), blml_amd.make_modes([0.25], [kind]), dict(complex_g=cplx)))
+    jobs.append((_counts2((18, 16), seed=4), blml_amd.make_modes([0.4, 0.35], ['sinc', 'sinc']), {}))
+    kw = dict(tol=1e-9, outer_tol=1e-5, max_outer=200)
+    _note('empirical_many')
+    many = blml_amd.empirical_many(jobs, **kw)
+    for (c, modes, fk), m in zip(jobs, many):
+        one = blml_amd.BLMLEmpirical(**kw).fit(c, modes, **fk)
+        assert m.retrieve_ == one.retrieve_
+        assert m.loglik_ == one.loglik_ and np.array_equal(m.density_, one.density_)
+        assert m.n_iter_ == one.n_iter_ and m.n_outer_ == one.n_outer_
