@@ -159,21 +159,24 @@ def modes_of(groups, lib):
 
 # ------------------------------------------------------------------------------------------------ bench
 def cpu_baseline(fits, seconds=20.0):
-    """The FP64 oracle (numpy + scipy.fft, workers=-1) on the first fits of the workload until ~seconds of work."""
+    """The FP64 oracle (numpy + scipy.fft, workers=-1) on a random sample of the workload's fits (a fixed
+    permutation, so the sample's sizes and kinds follow the workload's) until ~seconds of work."""
     from oracle import blml_oracle as O
     cores = len(os.sched_getaffinity(0))
     done, t0 = 0, time.perf_counter()
     bins = 0
-    for counts, groups, kw, _ in fits:
+    for i in np.random.default_rng(0).permutation(len(fits)):
+        counts, groups, kw, _ = fits[i]
         O.BLMLBinnedOracle(**SOLVER).fit(counts, modes_of(groups, O), **kw)
         done += 1
         bins += counts.size
         if time.perf_counter() - t0 > seconds:
             break
     dt = time.perf_counter() - t0
+    mean_all = sum(c.size for c, *_ in fits) / len(fits)
     return dict(value=done / dt, unit='fits/s', cores=cores, kind='port',
-                sample=f'first {done} of the {len(fits)} fits ({bins} bins), FP64 numpy/scipy.fft workers=-1, '
-                       f'{dt:.1f} s')
+                sample=f'{done} of the {len(fits)} fits drawn at random ({bins / done:.0f} bins per fit; the '
+                       f'workload averages {mean_all:.0f}), FP64 numpy/scipy.fft workers=-1, {dt:.1f} s')
 
 
 PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles')
@@ -548,11 +551,16 @@ def main():
           flush=True)
     pairs, Zp, wp, tabp = pair_leg(torch, st, event_time)
     cpu = cpu_baseline(fits, args.cpu_seconds)
+    # the pair path's CPU baseline: the C restatement of the rows (oracle/pairs_cpu.c, OpenMP over every host core;
+    # FP32 table products and FP64 sums as the device's) on the same 65,536 rows x 303,017 sources
     from oracle import blml_pairs as OP
+    n_cpu = min(len(Zp), 1 << 16)
+    cores = len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
-    OP.pair_rows(Zp[:64], Zp, wp, tabp)
-    pairs['cpu_pairs_per_s'] = 64 * len(Zp) / (time.perf_counter() - t0)
-    pairs['cpu_sample'] = f'64 rows x {len(Zp)} sources, the FP64 numpy restatement, one process'
+    OP.pair_rows_c(Zp[:n_cpu], Zp, wp, tabp, f32=True, threads=cores)
+    pairs['cpu_pairs_per_s'] = n_cpu * len(Zp) / (time.perf_counter() - t0)
+    pairs['cpu_sample'] = (f'{n_cpu} rows x {len(Zp)} sources (the timed launch), the C/OpenMP restatement '
+                           f'(oracle/pairs_cpu.c), {cores} threads')
     ms = dtw / args.steps * 1e3
     line = dict(metric='fits/sec at loglik tol 1e-6 (warm-start FP32 vs plain FP64) + mat-vec HBM GB/s',
                 value=len(fits) / (dtw / args.steps), unit='fits/s', n_gpus=1, steps=args.steps,

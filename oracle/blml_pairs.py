@@ -106,6 +106,37 @@ def pair_rows(ZA, ZB, w, tab, chunk=512):
     return s
 
 
+def pair_rows_c(ZA, ZB, w, tab, f32=False, threads=None):
+    """pair_rows by the C restatement (oracle/pairs_cpu.c, OpenMP over rows) for tables of offset, circular and rank
+    columns (no groups): the FP32 products of an f32 table (REAL = float, as the reference's GPU rows), FP64 sums.
+    Builds oracle/libpairs_cpu.so with make when it is missing."""
+    import ctypes
+    import os
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    path = os.path.join(here, 'libpairs_cpu.so')
+    if not os.path.exists(path):
+        subprocess.run(['make', '-s', '-C', here], check=True)
+    lib = ctypes.CDLL(path)
+    A = np.ascontiguousarray(rank_coords(ZA, tab), np.int32)
+    B = np.ascontiguousarray(rank_coords(ZB, tab), np.int32)
+    d = A.shape[1]
+    i64 = lambda k: np.ascontiguousarray(np.asarray(tab[k], np.int64))
+    kinds = np.ascontiguousarray(np.asarray(tab['kinds'], np.int32))
+    base, Jk, mk, nb = i64('base'), i64('Jk'), i64('mk'), i64('nbins')
+    T = np.ascontiguousarray(np.asarray(tab['T'], np.float32 if f32 else np.float64))
+    w = np.ascontiguousarray(np.asarray(w, np.float64))
+    s = np.zeros(len(A))
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    nt = int(threads or len(os.sched_getaffinity(0)))
+    rc = lib.pair_rows_cpu(ctypes.c_int(d), ptr(kinds), ptr(base), ptr(Jk), ptr(mk), ptr(nb),
+                           ctypes.c_int64(len(T)), ptr(T), ctypes.c_int(int(f32)), ptr(A), ptr(B), ptr(w),
+                           ctypes.c_int64(len(A)), ctypes.c_int64(len(B)), ptr(s), ctypes.c_int(nt))
+    if rc:
+        raise ValueError("pair_rows_c: radial / torus columns are not restated in C")
+    return s
+
+
 def _group_r2(A, B, tab, g):
     """The group's r^2 (radial) or row-major circular offset (torus) of every pair, and whether it is a torus."""
     kinds, nbins, grp = (np.asarray(tab[k]) for k in ('kinds', 'nbins', 'grp'))

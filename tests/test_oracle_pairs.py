@@ -108,6 +108,31 @@ def test_product_tables_match_oracle_tables():
     assert np.abs(OP.pair_rows(Zp, Zp, wp, tabp) - Gp @ wp).max() < 1e-12
 
 
+def test_c_rows_equal_the_dense_gram():
+    """The C restatement (oracle/pairs_cpu.c, the pair path's CPU baseline) over offset, circular and rank columns
+    equals the dense Gram (FP64 tables) and the numpy restatement; with FP32 tables its FP32 products stay within
+    FP32 rounding of them; a planted shift of the offset rows (J + 1) fails the check."""
+    from blml_amd import local
+    rng = np.random.default_rng(9)
+    Z = _bins(rng, (30, 12, 16), 300)
+    kinds = ['slits:0.5', 'sinc', 'dirichlet']
+    fc, per, h = [0.3, 0.45, 0.4], [None, None, 16.0], [1.0, 1.0, 1.0]
+    tab = local.lattice_tables(kinds, fc, per, h, Z)
+    units = [('offset', 'slits:0.5', [0.3], [0.0], [0]), ('offset', 'sinc', [0.45], [0.0], [1]),
+             ('circular', 'dirichlet', [0.4], [16.0], [2])]
+    w = rng.standard_normal(len(Z))
+    w[::7] = 0.0
+    G = OP.dense_pair_gram(units, np.asarray(h), Z, Z)
+    scale = np.abs(G).sum(1).max()
+    s = OP.pair_rows_c(Z, Z, w, tab, threads=4)
+    assert np.abs(s - G @ w).max() < 1e-12 * scale
+    assert np.abs(s - OP.pair_rows(Z, Z, w, tab)).max() < 1e-13 * scale
+    s32 = OP.pair_rows_c(Z, Z, w, tab, f32=True, threads=4)
+    assert np.abs(s32 - G @ w).max() < 1e-6 * scale
+    bad = dict(tab, Jk=np.asarray(tab['Jk']) + 1)
+    assert np.abs(OP.pair_rows_c(Z, Z, w, bad, threads=4) - G @ w).max() > 1e-4 * scale
+
+
 def test_ball_tables_radial_rows():
     """A ball over columns (one radial group) next to a sinc column: lattice_tables' radial row is the oracle's ball
     (kernel_eval, the reference's closed form) on the grid r = j / 4 lattice steps to 1e-14, so rows through it
