@@ -92,61 +92,18 @@ __global__ void __launch_bounds__(BLML_NT) matvec_grid_kernel(DevPlan<T> p, unsi
 
 // The stand-alone mat-vec of a separable spread plan whose every pass is a pipelined whole-tile pass (the host
 // checks it): only that path is compiled in, so the kernel's registers serve it alone and nothing is saved to
-// scratch.  Passes of a mode are separated by grid barriers, modes too.
+// scratch.  FP32 plans take the centrosymmetric split of the line operators (half the matrix-core work).
 template <typename T>
 __global__ void __launch_bounds__(BLML_NT) matvec_pipe_kernel(DevPlan<T> p, unsigned *bar, double *part, const T *E,
                                                              T *U) {
     const Team tm{(int)blockIdx.x, (int)gridDim.x, bar, part, 0};
-    const blml_fit_t *fp = p.fits;                     // read in place: a local copy indexed by k would be scratch
-    const int nc = fp->complex_g ? 2 : 1;
-    const int K = fp->K;
-    const long B = fp->B;
-    const SepPlan *sp = p.sep;
-    T *Z = reinterpret_cast<T *>(p.ws + fp->ws_off);
-    int shape[BLML_MAXD];
-#pragma unroll
-    for (int a = 0; a < BLML_MAXD; ++a) shape[a] = fp->shape[a];
-    for (int k = 0; k < K; ++k) {
-        const long off = fp->out_off + (long)k * B * nc;
-        const int fb = fp->fac_begin[k], fcnt = fp->fac_count[k];
-        int fac_of[BLML_MAXD];                          // indexed by constants only
-#pragma unroll
-        for (int a = 0; a < BLML_MAXD; ++a) {
-            int fa = -1;
-            for (int j = 0; j < fcnt; ++j)
-                if (p.facs[fb + j].axes[0] == a) fa = fb + j;
-            fac_of[a] = fa;
-        }
-        // every axis's table is staged once per mode, after the largest tile of any pass (the host sized the LDS)
-        const int np = sp->npass;
-        int o = 0;
-        for (int q = 0; q < np; ++q) {
-            const int t = sp->pass[q].OB * sp->pass[q].R * sp->pass[q].w;
-            o = max(o, sep_lds(t) + 1 + 64);
-        }
-        constexpr int BLK = SepMfma<T>::BLK;
-        int aoff[BLML_MAXD];
-#pragma unroll
-        for (int a = 0; a < BLML_MAXD; ++a) {
-            aoff[a] = -1;
-            if (fac_of[a] >= 0 && shape[a] > 1) {
-                aoff[a] = o;
-                o += BLK * (BLK + 1);
-            }
-        }
-        constexpr int NV = sizeof(T) == 4 ? 64 : 32;          // 128 KB tiles
-        for (int q = 0; q < np; ++q) {
-            const SepPass ps = sp->pass[q];
-            const T *src = q == 0 ? E + off : Z;
-            T *dst = q == np - 1 ? U + off : Z;
-            if (q == 0)
-                sep_pass_pipe<T, NV>(ps, shape, fac_of, aoff, src, dst, tm.rank, tm.nranks,
-                                     SepStageAll<T>{fac_of, p.sep_tab, p.sep_off, aoff});
-            else
-                sep_pass_pipe<T, NV>(ps, shape, fac_of, aoff, src, dst, tm.rank, tm.nranks, SepNoStage{});
-            if (q < np - 1 || k < K - 1) tm.sync();
+    if constexpr (sizeof(T) == 4) {
+        if (p.sep_tab_c) {
+            run_matvec_pipe<T, true>(p, tm, E, U);
+            return;
         }
     }
+    run_matvec_pipe<T, false>(p, tm, E, U);
 }
 
 // The batched path: a persistent grid whose workgroups take fits from an atomic queue, longest first; one

@@ -49,6 +49,7 @@ struct Plan {
     SepPlan *d_sep = nullptr;   // [0] mat-vec kernel, [1] fit kernel
     bool pipe_mv = false;       // the stand-alone mat-vec takes matvec_pipe_kernel (every pass pipelined)
     void *d_sep_tab = nullptr;
+    long sep_centro_off = -1;   // FP32 pipelined plans: where sep_centro_table's tables start in d_sep_tab (-1: none)
     long *d_sep_off = nullptr;
     int lds_mv = 0, lds_fit = 0;   // dynamic LDS of the spread mat-vec and fit launches
     int lds_pipe = 0;              // dynamic LDS of matvec_pipe_kernel (largest tile, then every axis's table)
@@ -82,6 +83,7 @@ DevPlan<T> dev_plan(const Plan &pl) {
     p.params = pl.params;
     p.sep = pl.sep ? pl.d_sep : nullptr;
     p.sep_tab = (const T *)pl.d_sep_tab;
+    p.sep_tab_c = pl.sep_centro_off >= 0 ? (const T *)pl.d_sep_tab + pl.sep_centro_off : nullptr;
     p.sep_off = pl.d_sep_off;
     p.res_off = p.res_cap = 0;
     p.fft_sched = pl.fft_sched;
@@ -103,6 +105,7 @@ DevPlan<double> dev_plan64(const Plan &pl) {
     p.params = pl.params;
     p.sep = nullptr;
     p.sep_tab = nullptr;
+    p.sep_tab_c = nullptr;
     p.sep_off = nullptr;
     p.res_off = p.res_cap = 0;
     p.fft_sched = pl.fft_sched;
@@ -296,6 +299,28 @@ void sep_tables(const blml_factor_t &fa, int m, int blk, const double *fdata, co
         for (int j = 0; j < P; ++j) out.push_back(i < m && j < m ? M[(size_t)i * m + j] : 0.0);
 }
 
+// The centrosymmetric split of one factor's 32 x 32 table M (sep_axis_centro): rows i < 16 hold
+// [C_i0 .. C_i15, D_i0 .. D_i15], C_ij = (M_ij + M_{i,31-j}) / 2 and D_ij = (M_ij - M_{i,31-j}) / 2 from the FP64
+// operator, the other rows zero.  false (nothing appended) unless the table is 32 x 32 and M[31-i][31-j] == M[i][j]
+// to the bit.
+bool sep_centro_table(const double *M, int P, std::vector<double> &out) {
+    if (P != 32) return false;
+    for (int i = 0; i < 32; ++i)
+        for (int j = 0; j < 32; ++j)
+            if (M[(size_t)(31 - i) * 32 + (31 - j)] != M[(size_t)i * 32 + j]) return false;
+    for (int i = 0; i < 32; ++i)
+        for (int j = 0; j < 32; ++j) {
+            double v = 0.0;
+            if (i < 16) {
+                const int jj = j & 15;
+                const double a = M[(size_t)i * 32 + jj], b = M[(size_t)i * 32 + 31 - jj];
+                v = 0.5 * (j < 16 ? a + b : a - b);
+            }
+            out.push_back(v);
+        }
+    return true;
+}
+
 // One tile pass over axes [lo, hi) in `avail` LDS values, which hold the tile (with its bank padding and dump
 // slots) and the staged tables of the pass's axes (stage[a] values each); rows of at least 16 values, or the whole
 // inner extent.  false if the group does not fit.
@@ -449,6 +474,25 @@ bool sep_plan_host(Plan &pl, const double *fdata, const int32_t *freqs, int max_
     const long lds_pipe = (pipe_vals + all_tabs + 2) * (long)es;
     pl.lds_pipe = (int)std::min<long>(lds_pipe, 1L << 30);
     pl.pipe_mv = st_pipe <= st_mv && lds_pipe + st_pipe <= max_lds && sep_pipe_host<T>(sp[0], f.shape);
+    // FP32 pipelined plans: the centrosymmetric tables after the dense ones, at the same offsets (every axis the
+    // pipelined passes act on has 32 points; tables of other factors are never staged there and stay zero)
+    pl.sep_centro_off = -1;
+    if (sizeof(T) == 4 && pl.pipe_mv) {
+        std::vector<double> ct;
+        bool ok = true;
+        for (int i = 0; i < pl.nfacs && ok; ++i) {
+            const int m = f.shape[pl.hfacs[i].axes[0]];
+            const int P = sep_table_dim(m, SepMfma<T>::BLK);
+            if (m == SepMfma<T>::BLK)
+                ok = sep_centro_table(tab.data() + off[i], P, ct);
+            else
+                ct.insert(ct.end(), (size_t)P * P, 0.0);
+        }
+        if (ok) {
+            pl.sep_centro_off = (long)out.tab.size();
+            out.tab.insert(out.tab.end(), ct.begin(), ct.end());
+        }
+    }
     return true;
 }
 

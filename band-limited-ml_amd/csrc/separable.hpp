@@ -195,6 +195,80 @@ __device__ __forceinline__ void sep_axis_mfma_full(int nvals, int st, int aoff) 
     }
 }
 
+// The full-block product of sep_axis_mfma_full (FP32, lines of 32 values) for a centrosymmetric operator
+// (M[31-i][31-j] = M[i][j]: every LINE operator is, being symmetric Toeplitz or symmetric circulant) at half the
+// matrix-core work.  With s_j = x_j + x_{31-j} and d_j = x_j - x_{31-j} (j < 16),
+//     y_i = (C s)_i + (D d)_i,   y_{31-i} = (C s)_i - (D d)_i   (i < 16),
+//     C_ij = (M_ij + M_{i,31-j}) / 2,   D_ij = (M_ij - M_{i,31-j}) / 2,
+// two 16 x 16 products per line instead of one 32 x 32.  The staged table (32 rows of 33, sep_centro_table) holds
+// [C | D] in its first 16 rows.  On v_mfma_f32_16x16x4f32 a block is 16 lines; K-step k's A operand in lane l is
+// C[l % 16][4 k + l / 16] (D likewise), its B operand s (d) at position 4 k + l / 16 of the block's line l % 16, and
+// accumulator register r of lane l holds row 4 (l / 16) + r of line l % 16.  A wave takes NH blocks at a time:
+// 16 NH lines, block h holding lines (h & 1) + 2 c + 32 (h >> 1) of the group (c = l % 16), so that the 16 lines
+// of a block sit on different LDS banks along the contiguous axis.
+template <int NH>
+__device__ __forceinline__ void sep_axis_centro(int nvals, int st, int aoff) {
+    static_assert(NH % 2 == 0, "blocks pair up on odd and even lines");
+    typedef float acc4_t __attribute__((ext_vector_type(4)));
+    st = __builtin_amdgcn_readfirstlane(st);
+    nvals = __builtin_amdgcn_readfirstlane(nvals);
+    float *tile = reinterpret_cast<float *>(g_dyn_lds);
+    const float *At = reinterpret_cast<const float *>(g_dyn_lds) + aoff;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, q = lane >> 4;
+    float ca[4], da[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        ca[k] = At[c * 33 + 4 * k + q];
+        da[k] = At[c * 33 + 16 + 4 * k + q];
+    }
+    const int ngrp = nvals / (32 * 16 * NH);
+    const int span = 32 * st;
+    const int stp = st == 1 ? 1 : st + (st >> 5);
+    const float inv_st = 1.0f / (float)st;
+    for (int g = wave; g < ngrp; g += BLML_NWAVE) {
+        int lb[NH];
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+            int lo;
+            const int hi = fdivmod(g * 16 * NH + (h & 1) + 2 * c + 32 * (h >> 1), st, inv_st, lo);
+            lb[h] = sep_lds(hi * span + lo);
+        }
+        float xa[NH][4], xb[NH][4];
+#pragma unroll
+        for (int h = 0; h < NH; ++h)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                xa[h][k] = tile[lb[h] + (4 * k + q) * stp];
+                xb[h][k] = tile[lb[h] + (31 - 4 * k - q) * stp];
+            }
+        __builtin_amdgcn_sched_barrier(0);        // every operand read before the first product
+        acc4_t u[NH], v[NH];
+#pragma unroll
+        for (int h = 0; h < NH; ++h)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) u[h][r] = v[h][r] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                u[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[k], xa[h][k] + xb[h][k], u[h], 0, 0, 0);
+                v[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(da[k], xa[h][k] - xb[h][k], v[h], 0, 0, 0);
+            }
+#pragma unroll
+        for (int h = 0; h < NH; ++h)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 4 * q + r;
+                tile[lb[h] + i * stp] = u[h][r] + v[h][r];
+                tile[lb[h] + (31 - i) * stp] = u[h][r] - v[h][r];
+            }
+    }
+}
+#ifndef BLML_SEP_CENTRO_NH
+#define BLML_SEP_CENTRO_NH 4
+#endif
+
 template <typename T>
 __device__ __forceinline__ void sep_axis(int toff, int nvals, int m, int st, int aoff) {
     constexpr int BLK = SepMfma<T>::BLK;
@@ -341,9 +415,11 @@ __device__ __forceinline__ void sep_move_fast(const SepPass &ps, long gbase, con
 //   w divides NT, VW divides w (rows): start = gbase + (VW t / w) rs + (VW t mod w),      step = (VW NT / w) rs
 // The caller guarantees src and dst 16-byte aligned and every tile's first value at an offset that VW divides.
 // stage() runs once, after the first tile's loads are issued: it may write LDS outside the tile (operator tables).
-template <typename T, int NV, typename Stage>
+// CENTRO (FP32): the staged tables are sep_centro_table's [C | D] and the axes take sep_axis_centro.
+template <typename T, int NV, bool CENTRO, typename Stage>
 __device__ __forceinline__ void sep_pass_pipe(const SepPass &ps, const int *shape, const int *fac_of, const int *aoff,
                                               const T *src, T *dst, int rank, int nranks, const Stage &stage) {
+    static_assert(!CENTRO || sizeof(T) == 4, "the centrosymmetric product runs on the FP32 matrix cores");
     constexpr int VW = 16 / (int)sizeof(T);
     constexpr int NQ = NV / VW;
     typedef T vec_t __attribute__((ext_vector_type(VW)));
@@ -411,7 +487,10 @@ __device__ __forceinline__ void sep_pass_pipe(const SepPass &ps, const int *shap
 #pragma unroll
         for (int a = BLML_MAXD - 1; a >= 0; --a) {          // innermost axis first
             if (act[a]) {
-                sep_axis_mfma_full<T>(tsize, ast[a], aao[a]);
+                if constexpr (CENTRO)
+                    sep_axis_centro<BLML_SEP_CENTRO_NH>(tsize, ast[a], aao[a]);
+                else
+                    sep_axis_mfma_full<T>(tsize, ast[a], aao[a]);
                 __syncthreads();
             }
         }
@@ -562,7 +641,7 @@ __device__ __forceinline__ void sep_pass(const SepPass &ps, int d, int nc, const
         constexpr int NV = sizeof(T) == 4 ? 64 : 32;          // 128 KB tiles
         const bool al16 = ((reinterpret_cast<unsigned long>(src) | reinterpret_cast<unsigned long>(dst)) & 15) == 0;
         if (al16 && sep_pipe_ok<T>(ps, NV, shape, fac_of)) {
-            sep_pass_pipe<T, NV>(ps, shape, fac_of, aoff, reinterpret_cast<const T *>(src), reinterpret_cast<T *>(dst),
+            sep_pass_pipe<T, NV, false>(ps, shape, fac_of, aoff, reinterpret_cast<const T *>(src), reinterpret_cast<T *>(dst),
                                  rank, nranks, SepNoStage{});
             return;
         }

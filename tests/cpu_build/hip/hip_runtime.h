@@ -53,6 +53,7 @@
 #define __builtin_amdgcn_sched_barrier(x) ((void)0)
 #define __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, x, y, z) cpuhip::mfma_f32_32x32x2(a, b, c)
 #define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) cpuhip::mfma_f64_16x16x4(a, b, c)
+#define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) cpuhip::mfma_f32_16x16x4(a, b, c)
 
 struct cpuhip_dim3 {
     unsigned x = 0, y = 0, z = 0;
@@ -214,6 +215,26 @@ inline cpuhip_f16 mfma_f32_32x32x2(float a, float b, cpuhip_f16 c) {
         const int i = (r & 3) + 8 * (r >> 2) + 4 * (l >> 5), j = l & 31;
         float acc = d[r];
         for (int k = 0; k < 2; ++k) acc += (float)B.xa[w0 + i + 32 * k] * (float)B.xb[w0 + j + 32 * k];
+        d[r] = acc;
+    }
+    wave_sync();
+    return d;
+}
+
+// v_mfma_f32_16x16x4f32 on one wave: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k; accumulator register r of
+// lane l holds D[4 (l >> 4) + r][l & 15]
+typedef float cpuhip_f4 __attribute__((ext_vector_type(4)));
+inline cpuhip_f4 mfma_f32_16x16x4(float a, float b, cpuhip_f4 c) {
+    Block &B = *g_blk;
+    const int t = B.cur, w0 = t & ~63, l = t & 63;
+    B.xa[t] = a;
+    B.xb[t] = b;
+    wave_sync();
+    cpuhip_f4 d = c;
+    for (int r = 0; r < 4; ++r) {
+        const int i = 4 * (l >> 4) + r, j = l & 15;
+        float acc = d[r];
+        for (int k = 0; k < 4; ++k) acc += (float)B.xa[w0 + i + 16 * k] * (float)B.xb[w0 + j + 16 * k];
         d[r] = acc;
     }
     wave_sync();

@@ -121,6 +121,12 @@ long long launch(int lds, int st, int blocks, std::function<void()> body) {
     return cpuhip::run_grid(blocks, BLML_NT, (size_t)lds, std::move(body));
 }
 
+// the pipelined mat-vec replay's input: a fixed sequence in [-1, 1) (Weyl sequence of the golden ratio)
+double pipe_value(long i) {
+    const double x = (double)i * 0.6180339887498949;
+    return 2.0 * (x - floor(x)) - 1.0;
+}
+
 template <typename T>
 int replay(In &in, const char *out_path) {
     Plan pl;
@@ -203,6 +209,41 @@ int replay(In &in, const char *out_path) {
         printf("replay: spread over %d workgroups, %s, LDS of the fit launch %d bytes\n", pl.coop,
                pl.sep ? "separable tile passes" : "transform passes", pl.lds_fit);
         fflush(stdout);
+    }
+    // BLML_REPLAY_PIPE: matvec_pipe_kernel's work on an FP32 pipelined plan instead of the fit, on the input
+    // E_i = pipe_value(i), once with the dense tables and once with the centrosymmetric ones; writes both U
+    if (getenv("BLML_REPLAY_PIPE")) {
+        if constexpr (sizeof(T) == 4) {
+            if (!pl.spread || !pl.sep || !pl.pipe_mv || pl.sep_centro_off < 0) {
+                fprintf(stderr, "replay: not an FP32 plan of the pipelined mat-vec with centrosymmetric tables\n");
+                return 8;
+            }
+            const blml_fit_t &f = pl.hfits[0];
+            const long n = (long)f.K * f.B * (f.complex_g ? 2 : 1);
+            T *E = dev_alloc<T>(n);
+            for (long i = 0; i < n; ++i) E[i] = (T)pipe_value(i);
+            T *U[2] = {dev_alloc<T>(n), dev_alloc<T>(n)};
+            const DevPlan<T> dp = dev_plan<T>(pl);
+            for (int c = 0; c < 2; ++c) {
+                memset(bar, 0, 4 * sizeof(unsigned));
+                launch(pl.lds_pipe, kStaticPipe, pl.coop, [&] {
+                    const Team tm{(int)blockIdx.x, (int)gridDim.x, bar, part, 0};
+                    if (c) run_matvec_pipe<T, true>(dp, tm, E, U[1]);
+                    else run_matvec_pipe<T, false>(dp, tm, E, U[0]);
+                });
+            }
+            FILE *fo = fopen(out_path, "wb");
+            if (!fo) return 5;
+            fwrite(U[0], sizeof(T), (size_t)n, fo);
+            fwrite(U[1], sizeof(T), (size_t)n, fo);
+            fclose(fo);
+            printf("replay: pipelined mat-vec of %ld values, dense and centrosymmetric\n", n);
+            printf("replay: ok\n");
+            return 0;
+        } else {
+            fprintf(stderr, "replay: the pipelined mat-vec replay takes FP32 plans\n");
+            return 8;
+        }
     }
     // the alignment gaps of every workspace: [X Y | gap | pool | gap | (FP32: X64 Y64) | gap]
     unsigned char *ws = (unsigned char *)pl.d_ws;

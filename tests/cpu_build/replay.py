@@ -136,3 +136,31 @@ def run(batch, cert_iters=0, cert_tol=1e-6, timeout=1800, exe=None, ranks=4, emp
     if emp is not None:
         out['emp'] = np.frombuffer(raw, np.float64, nf * _lib.EMP_OUT_LEN, o).reshape(nf, _lib.EMP_OUT_LEN)
     return out
+
+
+def pipe_input(n):
+    """replay.cpp's pipe_value: the input E of the pipelined mat-vec replay."""
+    x = np.arange(n, dtype=np.float64) * 0.6180339887498949
+    return (2.0 * (x - np.floor(x)) - 1.0).astype(np.float32)
+
+
+def run_pipe(batch, ranks=4, timeout=1800, exe=None):
+    """Replays matvec_pipe_kernel's work for an FP32 spread plan of one fit on pipe_input(n): returns (E, U with
+    the dense line operators, U with their centrosymmetric split)."""
+    exe = exe or build()
+    fs = batch.fits[0]
+    n = fs.K * fs.B * fs.nc
+    tmp = tempfile.mkdtemp(prefix='blml_replay_')
+    try:
+        pin, pout = os.path.join(tmp, 'plan.bin'), os.path.join(tmp, 'out.bin')
+        write_plan(batch, pin, 0, 1e-6, ranks, None)
+        env = dict(os.environ, ASAN_OPTIONS='detect_leaks=0:abort_on_error=0:detect_stack_use_after_return=0',
+                   BLML_REPLAY_PIPE='1')
+        p = subprocess.run([exe, pin, pout], capture_output=True, text=True, timeout=timeout, env=env)
+        log = p.stdout + p.stderr
+        if p.returncode != 0:
+            raise RuntimeError(f"replay exited with {p.returncode}:\n{log[-6000:]}")
+        raw = np.fromfile(pout, np.float32)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return pipe_input(n), raw[:n], raw[n:2 * n]

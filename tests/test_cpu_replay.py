@@ -105,6 +105,25 @@ def test_spread_separable_fit_equals_one_workgroup(replay):
     assert abs(la - lb) < 1e-9 * abs(lb)
 
 
+def test_pipelined_matvec_centrosymmetric_split(replay):
+    """The stand-alone mat-vec of an FP32 pipelined plan (matvec_pipe_kernel's work on emulated matrix cores, one
+    32^3 tile: line strides 1, 32 and 1024): the centrosymmetric split of the line operators (two 16 x 16 products
+    per line on v_mfma_f32_16x16x4f32) and the dense 32 x 32 products both equal the oracle's FFT mat-vec to FP32
+    rounding (2e-6 of max |S e| times log2 B), and each other to the same bound."""
+    import blml_amd
+    shape, fc = (32, 32, 32), [0.3, 0.2, 0.35]
+    counts = np.ones(shape)
+    E, Ud, Uc = replay.run_pipe(_batch(counts, blml_amd.make_modes(fc, ['sinc'] * 3), False, np.float32, 1, PARAMS),
+                                ranks=2)
+    om = O.BLMLBinnedOracle()
+    om._setup(counts, O.make_modes(fc, ['sinc'] * 3), 1.0, 0.0)
+    want = om._S(0, E.astype(np.float64).reshape(shape)).ravel()
+    bound = 2e-6 * np.abs(want).max() * np.log2(want.size)
+    assert np.abs(Ud - want).max() < bound
+    assert np.abs(Uc - want).max() < bound
+    assert np.abs(Uc - Ud).max() < bound
+
+
 def test_fp32_plan_and_certificate(replay):
     """An FP32 plan (FP32 transforms, the FP64 finishing pass) and the certificate of an FP64 fit (power iteration
     with the eigen-residual stop) against dense eigvalsh."""

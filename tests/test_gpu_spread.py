@@ -69,7 +69,8 @@ def test_spread_matvec_32x5_fp32_matches_fp64():
 
 def test_spread_matvec_unaligned_input_takes_the_general_kernel():
     """The pipelined 32^5 kernel moves 16-byte vectors, so blml_matvec hands an input that is not 16-byte aligned
-    to the general separable kernel: the same products of the same tiles, so the same U."""
+    to the general separable kernel: the same tiles, its dense 32 x 32 line products against the pipelined kernel's
+    centrosymmetric split, so the same U to FP32 rounding."""
     import torch
     import blml_amd
     shape = (32,) * 5
