@@ -46,12 +46,13 @@ def units_of(kinds):
     return one + [(name, dims) for name, dims in groups.values()]
 
 
-def parse_modes(modes, d):
-    """(specs, dep) as the reference returns them: specs = K dicts {fc (d,), kinds (d,), periods (d,)}; dep is
-    always None here (conditional modes are rejected by the binned fit)."""
+def parse_modes(modes, d, allow_dep=False):
+    """(specs, dep) as the reference returns them (blml_trivial.py:1247-1323): specs = K dicts {fc (d,), kinds (d,),
+    periods (d,)}; dep, the sorted columns marked 'dep' (the same in every mode; a unit's flag covers its columns),
+    or None.  The binned fit rejects dep (allow_dep False); the counts path fits conditionals."""
     if isinstance(modes, dict) or not isinstance(modes, (list, tuple)) or not modes:
         raise TypeError("modes: a non-empty list of mode dicts {column: {...}}; wrap one mode as [mode]")
-    specs = []
+    specs, deps = [], []
     for k, m in enumerate(modes):
         cols = {}
         for key, c in m.items():
@@ -79,12 +80,13 @@ def parse_modes(modes, d):
                 cols[key] = c
         if set(cols) != set(range(d)):
             raise ValueError(f"mode {k}: columns {sorted(cols)}, expected every column 0..{d - 1}")
-        fc, kinds, periods = [], [], []
+        fc, kinds, periods, dep = [], [], [], []
         for j in range(d):
             c = cols[j]
             kind = c.get('kind', 'sinc')
-            if c.get('dep'):
+            if c.get('dep') and not allow_dep:
                 raise ValueError("BLMLBinned: conditional (dep) modes are not supported; fit the joint")
+            dep.append(bool(c.get('dep', False)))
             v = _value(c.get('fc'))
             if v is None:
                 raise ValueError(f"mode {k} column {j}: fc missing")
@@ -116,15 +118,19 @@ def parse_modes(modes, d):
                 raise ValueError(f"radial group {name} over dims {dims} uses one fc for every direction; "
                                  f"got {np.array(fc)[dims]}")
         specs.append(dict(fc=np.array(fc), kinds=kinds, periods=periods))
-    return specs, None
+        deps.append(dep)
+    if any(dp != deps[0] for dp in deps):
+        raise ValueError("dep must mark the same columns in every mode")
+    dep = [j for j in range(d) if deps[0][j]]
+    return specs, (dep or None)
 
 
 def make_modes(fc, kinds=None, periods=None, rank=1, dep=None, d=None):
     """Modes from per-dim settings (blml_trivial.py:1467-1505): fc scalar, (d,) or (K, d); kinds per dim
     ('sinc', 'dirichlet[:fill]', 'slits:fill', 'ball[:tag]', 'pball[:tag]'); periods per dim (dirichlet, pball);
-    rank: K when fc has one row."""
-    if dep is not None:
-        raise ValueError("BLMLBinned: conditional (dep) modes are not supported")
+    rank: K when fc has one row; dep: the dependent column(s) of a conditional counts fit (marked on a unit's
+    first column, which carries the unit)."""
+    deps = set() if dep is None else {int(j) for j in np.atleast_1d(dep)}
     F = np.asarray(fc, dtype=float)
     if d is None:
         d = len(kinds) if kinds is not None else (F.shape[-1] if F.ndim else None)
@@ -152,6 +158,8 @@ def make_modes(fc, kinds=None, periods=None, rank=1, dep=None, d=None):
         if kind == 'pball':
             key = next(u for u in units if j in u)
             c['period'] = [periods[i] for i in key]
+        if j in deps:
+            c['dep'] = True
         return c
 
     out = []
@@ -162,4 +170,28 @@ def make_modes(fc, kinds=None, periods=None, rank=1, dep=None, d=None):
             if key == j or j == key[0]:
                 m[key] = col(k, j)
         out.append(m)
+    return out
+
+
+def strip_dep(modes):
+    """The modes without their dep flags (the joint of a conditional fit)."""
+    return [{key: {a: v for a, v in c.items() if a != 'dep'} for key, c in m.items()} for m in modes]
+
+
+def marginal_modes(modes, keep):
+    """The modes restricted to the columns in keep, renumbered 0.. in that order, dep flags dropped (the marginal of a
+    conditional fit, blml_trivial.py:1585-1598); a multi-column unit is kept whole or not at all."""
+    pos = {j: i for i, j in enumerate(keep)}
+    out = []
+    for m in modes:
+        mm = {}
+        for key, c in m.items():
+            cols = key if isinstance(key, tuple) else (key,)
+            inside = [j in pos for j in cols]
+            if all(inside):
+                nk = tuple(pos[j] for j in cols) if isinstance(key, tuple) else pos[key]
+                mm[nk] = {a: v for a, v in c.items() if a != 'dep'}
+            elif any(inside):
+                raise ValueError(f"columns {key}: a unit straddles kept and dropped columns")
+        out.append(dict(sorted(mm.items(), key=lambda t: t[0][0] if isinstance(t[0], tuple) else t[0])))
     return out

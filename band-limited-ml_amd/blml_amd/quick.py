@@ -24,19 +24,29 @@ Each block's S[:, I] d is one pair-rows pass of libblml_hip.so (the kernel table
 ProductKernel.tables layout in LDS, FP64 sums, no B x |I| block in memory).  e and Se are FP64 on the device;
 dtype picks the tables' precision (float32: the reference's f32 policy, products in FP32).
 
+Conditional fits (columns marked dep, blml_gpu.py:67-72, 105-107, 899-953): the joint fit on every column and the
+marginal fit of the other columns, predict_marginal and predict_ratio = joint / marginal.  Real rank 1: the
+reference fits the marginal on the joint's bins with the kernel of the units off the dep side; bins that differ
+only in dep columns then share one marginal kernel row, so at its optimum a bin's coefficient is its merged bin's
+split in proportion to the counts (e_b = e_m w_b / W_m).  This engine fits the marginal on the merged counts and
+returns that split as e_marg_ (the same fixed point, with fewer bins).  Complex / rank K: two separate fits, as the
+reference does.
+
 init= warm-starts a real rank-1 fit from a fit on the same lattice (the previous e at shared bins, each new bin at
 its 1-D minimiser) or on another lattice (re-binned: e_b = w_b / g_prev(z_b) where g_prev is not ~0,
-blml_gpu.py:213-219).  sqrt_density / predict evaluate the exact continuous kernel at any points
+blml_gpu.py:213-219); a conditional fit warm-starts its joint and, from a conditional fit over the same dep
+columns, its marginal.  sqrt_density / predict evaluate the exact continuous kernel at any points
 (blml_point_rows).  Not built on this engine (they raise NotImplementedError): the complex start search
 (n_starts other than (0, 0) for a complex fit with no periodic column, blml_trivial.py:1605-1676), warm starts of
-complex / rank-K fits, conditional dep columns, and points not already binned (modes without h)."""
+complex / rank-K fits (the reference's branch for them, blml_gpu.py:196-219, builds FP64 arrays from the complex
+(B, K) rows of e_hat_ and cannot run either), and points not already binned (modes without h)."""
 import ctypes
 import os
 
 import numpy as np
 
 from . import _lib, local
-from .modes import parse_modes
+from .modes import marginal_modes, parse_modes, strip_dep
 
 __all__ = ['BLMLQuick']
 
@@ -98,9 +108,9 @@ class BLMLQuick:
         w = np.ones(len(Z)) if sample_weight is None else np.asarray(sample_weight, float)
         if w.shape != (len(Z),) or not np.all(w >= 0):
             raise ValueError("sample_weight: (n,) non-negative counts")
-        specs, dep = parse_modes(strip_lattice(modes), d)
+        specs, dep = parse_modes(strip_lattice(modes), d, allow_dep=True)
         if dep is not None:
-            raise NotImplementedError("BLMLQuick(engine='hip'): conditional (dep) fits are not built")
+            return self._fit_conditional(Z, w, modes, specs, dep, d, init, signs, complex_g)
         psd = bool(complex_g) or len(specs) > 1
         if psd:
             if signs is not None:
@@ -156,10 +166,69 @@ class BLMLQuick:
         self.safeguard_used_ = safeguard
         self.rho_history_, self.obj_history_ = rho_hist, obj_hist
         self.complex_g_, self.rank_, self.dep_ = bool(complex_g), K, None
+        self._joint = self._marg = self.e_marg_ = None
         self.kinds_, self.periods_, self._fc_arr = sp['kinds'], sp['periods'], sp['fc']
         self._specs = specs
         self._h_vec, self._Zi, self._tab = h, Zi, tab
         return self
+
+    def _fit_conditional(self, Z, w, modes, specs, dep, d, init, signs, complex_g):
+        """The joint fit on every column and the marginal fit of the others (blml_gpu.py:105-107, 197-217, 899-921):
+        the marginal's counts are the joint's summed over the dep columns; its modes the joint's restricted to the
+        other columns.  Sets the joint's attributes on self, dep_, _joint, _marg and, for real rank 1, e_marg_ on
+        the joint's bins (the merged bin's coefficient split by the counts) with marg_converged_, marg_sweeps_ and
+        marg_residual_."""
+        from .modes import units_of
+        if len(dep) == d:
+            raise ValueError(f"dep {dep} must name some, but not all, of the {d} dims")
+        for _, dims in units_of(specs[0]['kinds']):
+            if 0 < sum(j in dep for j in dims) < len(dims):
+                raise ValueError(f"a radial group over columns {dims} straddles dep and conditioning columns")
+        if signs is not None:
+            raise ValueError("signs with dep=... (the marginal's g has its own signs)")
+        rest = [j for j in range(d) if j not in dep]
+        kw = dict(tol=self.tol, block_size=self.block_size, seed=self.seed, verbose=self.verbose, dtype=self.dtype,
+                  bin_scale=self.bin_scale, max_sweeps=self.max_sweeps, solver=self.solver, lbfgs_m=self.lbfgs_m)
+
+        def make():
+            q = BLMLQuick(**kw)
+            q.phase_noise, q.anderson, q.n_starts = self.phase_noise, self.anderson, self.n_starts
+            return q
+
+        ij = init if getattr(init, '_joint', None) is None else init._joint
+        im = init._marg if init is not None and getattr(init, 'dep_', None) == list(dep) else None
+        joint = make().fit(Z, strip_dep(modes), init=ij, complex_g=complex_g, sample_weight=w)
+        Zm, inv = np.unique(np.ascontiguousarray(Z[:, rest]), axis=0, return_inverse=True)
+        inv = np.ravel(inv)
+        wm = np.bincount(inv, weights=w, minlength=len(Zm))
+        marg = make().fit(Zm, marginal_modes(modes, rest), init=im, complex_g=complex_g, sample_weight=wm)
+        self.__dict__.update(joint.__dict__)
+        self._joint, self._marg, self.dep_ = joint, marg, list(dep)
+        if np.ndim(joint.e_hat_) == 1 and not np.iscomplexobj(joint.e_hat_):
+            keep = w > 0                                 # the joint's bins, in its order
+            mrow = inv[keep]
+            occ = np.flatnonzero(wm > 0)                 # the marginal's bins, in its order
+            pos = np.full(len(Zm), -1)
+            pos[occ] = np.arange(len(occ))
+            em = np.asarray(marg.e_hat_, np.float64)[pos[mrow]]
+            self.e_marg_ = em * w[keep] / wm[mrow]
+            self.marg_converged_, self.marg_sweeps_ = marg.converged_, marg.n_sweeps_
+            self.marg_residual_ = marg.residual_
+        return self
+
+    def predict_marginal(self, Xrest):
+        """The marginal density of the non-dep columns at points given in those columns (blml_gpu.py:923-932)."""
+        if getattr(self, 'dep_', None) is None:
+            raise RuntimeError("fit with dep=... to get the marginal")
+        return self._marg.predict(Xrest)
+
+    def predict_ratio(self, Xq):
+        """The conditional density of the dep columns given the others, joint / marginal (blml_gpu.py:939-943)."""
+        if getattr(self, 'dep_', None) is None:
+            raise RuntimeError("fit with dep=... to get the marginal")
+        Xq = np.atleast_2d(np.asarray(Xq, float))
+        rest = [k for k in range(self.dim_) if k not in self.dep_]
+        return self._joint.predict(Xq) / np.maximum(self.predict_marginal(Xq[:, rest]), 1e-300)
 
     def _warm(self, init, h, Zi, Z, w, sg):
         """init= for a real rank-1 fit.  On the same lattice (blml_gpu.py:145-153, 200-208): the previous fit's e at

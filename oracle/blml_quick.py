@@ -13,8 +13,14 @@ Restated (numpy, float64, a dense Gram matrix G over the occupied bins in place 
   warm_start(e_prev, new, G, w)   init= on the same lattice (blml_gpu.py:200-208, 304-320): the previous e at the
                           shared bins, each new bin at root(Se_b, n w_b, S_bb) from e with the new bins at 0
   newton_solve(G, w)      the unique minimiser of the same convex Phi by damped Newton (an independent solver)
+  quick_fit_dep(G, Gm, w, ...)  the conditional fit (dep columns, blml_gpu.py:226-441 with nfit = 2): the joint
+                          (Gram G) and the marginal (Gm: the kernel of the units off the dep side, on the joint's
+                          bins) in one loop, each sweep's permutation shared by the fits still running, a fit's
+                          redo sweep drawing its own, each fit with its own safeguard, stop and stall guard
 
-Pinning (tests/test_oracle_quick.py): root by its quadratic's residual and the 1-D objective's derivative; the
+Pinning (tests/test_oracle_quick.py): quick_fit_dep's joint by newton_solve, its marginal by newton_solve on the
+merged bins (bins that differ only in dep columns merged, counts summed) split back in proportion to the counts --
+the identity the reference's docstring states (blml_gpu.py:67-72); root by its quadratic's residual and the 1-D objective's derivative; the
 fixed point of quick_fit by newton_solve (the problem is convex, so both must reach the same e); each safeguarded
 block step against a direct evaluation of Phi; Se = G e at the end; a warm-started fit by the same Newton fixed
 point, and each new bin's start by the 1-D objective's derivative.  The trajectory itself (sweep counts, when the
@@ -484,3 +490,69 @@ def psd_fit(Gs, w, tol=1e-4, block_size=4096, seed=0, max_sweeps=None, solver='c
         return lbfgs_psd(Gs, w, e0, tol, max_sweeps, lbfgs_m)
     return psd_cd(Gs, w, e0, Bs, rng, tol, max_sweeps, anderson)
 
+
+
+def quick_fit_dep(G, Gm, w, tol=1e-4, block_size=4096, seed=0, max_sweeps=None):
+    """The conditional counts fit of the reference (blml_gpu.py:226-441, nfit = 2): fit 0 the joint (Gram G), fit 1
+    the marginal on the same bins (Gram Gm).  Returns [(e, Se, n_sweeps, converged, max_rho), ...] per fit."""
+    Gs = [np.asarray(G, float), np.asarray(Gm, float)]
+    w = np.asarray(w, float)
+    B = len(w)
+    n = float(w.sum())
+    rng = np.random.default_rng(seed)
+    Bs = max(1, min(int(block_size), -(-B // 8)))
+    es = [np.ones(B), np.ones(B)]
+    Ses = [Gs[f] @ es[f] for f in range(2)]
+    sbbs = [float(Gs[f][0, 0]) for f in range(2)]
+
+    def resid(e, Se):
+        rho = np.abs(Se * e / (n * w) - 1.0)
+        return float(np.max(np.where((e > 0) & (w > 0), rho, 0.0)))
+
+    def obj(e, Se):
+        return float(0.5 * e @ Se - n * np.sum(w * np.log(np.maximum(e, 1e-300)))) / n
+
+    def run(fits, safe):
+        perm = rng.permutation(B)
+        for f in fits:
+            sweep(Gs[f], w, n, sbbs[f], es[f], Ses[f], perm, Bs, safe[f])
+
+    safe, done, conv = [False, False], [False, False], [False, False]
+    sweeps, rhos, hists = [0, 0], [np.inf, np.inf], [[], []]
+    prev = [obj(es[f], Ses[f]) for f in range(2)]
+    total = 0
+    while not all(done):
+        if max_sweeps is not None and total >= max_sweeps:
+            break
+        total += 1
+        active = [f for f in range(2) if not done[f]]
+        saves = {f: (es[f].copy(), Ses[f].copy()) for f in active}
+        run(active, safe)
+        now = {f: obj(es[f], Ses[f]) for f in active}
+        redo = [f for f in active if not safe[f] and now[f] > prev[f] + 1e-10 * abs(prev[f])]
+        for f in redo:
+            es[f][:], Ses[f][:] = saves[f]
+            safe[f] = True
+        if redo:
+            run(redo, safe)
+            for f in redo:
+                now[f] = obj(es[f], Ses[f])
+        for f in active:
+            sweeps[f] += 1
+            prev[f] = now[f]
+            if not np.all(np.isfinite(es[f])):
+                rhos[f] = np.inf
+                hists[f].append(np.inf)
+                done[f] = True
+                continue
+            rho = resid(es[f], Ses[f])
+            if rho < tol:
+                Ses[f][:] = Gs[f] @ es[f]
+                rho = resid(es[f], Ses[f])
+            rhos[f] = rho
+            hists[f].append(rho)
+            if rho < tol:
+                conv[f] = done[f] = True
+            elif len(hists[f]) > 50 and hists[f][-1] > 0.99 * min(hists[f][:-50]):
+                done[f] = True
+    return [(es[f], Ses[f], sweeps[f], conv[f], rhos[f]) for f in range(2)]

@@ -4,7 +4,10 @@ the CPU restatement (oracle/blml_quick.py) with the dense Gram matrix of the sam
   history to 1e-6 relative (same permutations from default_rng(seed), so the same trajectory);
 - a safeguarded sweep with one block holding every bin, through the C-ABI, equals the oracle's sweep to 1e-10;
 - FP32 tables (the reference's default policy): e to 1e-4 of the FP64 oracle's converged e;
-- a capped fit is scaled to e'Se = n^2; sqrt_density at the bins is Se / n."""
+- a capped fit is scaled to e'Se = n^2; sqrt_density at the bins is Se / n;
+- a conditional fit (dep column 1): the joint's e and the marginal's e on the joint's bins (e_marg_) against the
+  oracle's fused fit (quick_fit_dep, the reference's loop) to 1e-7 of their scale; predict_marginal and
+  predict_ratio at the bins against the oracle's S e / n; a warm refit from it keeps both fixed points."""
 import ctypes
 import sys
 
@@ -165,3 +168,38 @@ def test_warm_start_matches_oracle():
     assert np.abs(q2.e_hat_ - e).max() < 1e-9 * np.abs(e).max()
     cold = BLMLQuick(**kw).fit(Z2 * hv, _modes(cols, hv), sample_weight=w2)
     assert np.abs(q2.e_hat_ - cold.e_hat_).max() < 1e-6 * np.abs(cold.e_hat_).max()
+
+
+def test_conditional_fit_matches_oracle():
+    from blml_amd import make_modes
+    from blml_amd.quick import BLMLQuick
+    units = [('offset', 'sinc', [0.35], [0.0], [0]), ('offset', 'sinc', [0.3], [0.0], [1])]
+    off = [u for u in units if 1 not in u[4]]
+    Z, w = _problem((14, 12), 6000, 3)
+    modes = [{j: dict(c, h=1.0) for j, c in m.items()} for m in make_modes([0.35, 0.3], ['sinc', 'sinc'], dep=[1])]
+    _note('quick: conditional fit, dep column 1')
+    q = BLMLQuick(tol=1e-10, block_size=37, seed=2, dtype=np.float64).fit(Z.astype(float), modes, sample_weight=w)
+    keep = w > 0
+    Zk, wk = Z[keep], w[keep]
+    G = OP.dense_pair_gram(units, np.ones(2), Zk, Zk)
+    Gm = OP.dense_pair_gram(off, np.ones(2), Zk, Zk)
+    (ej, Sej, _, cj, _), (em, Sem, _, cm, _) = OQ.quick_fit_dep(G, Gm, wk, tol=1e-10, block_size=37, seed=2)
+    assert q.dep_ == [1] and q.converged_ and q.marg_converged_ and cj and cm
+    assert np.abs(q.e_hat_ - ej).max() < 1e-7 * np.abs(ej).max()
+    assert np.abs(q.e_marg_ - em).max() < 1e-7 * np.abs(em).max()
+    n = wk.sum()
+    pm = q.predict_marginal(Zk[:, [0]].astype(float))
+    assert np.abs(pm - (Sem / n) ** 2).max() < 1e-7 * ((Sem / n) ** 2).max()
+    r = q.predict_ratio(Zk.astype(float))
+    want = (Sej / n) ** 2 / (Sem / n) ** 2
+    assert np.abs(r - want).max() < 1e-6 * want.max()
+    again = BLMLQuick(tol=1e-10, block_size=37, seed=2, dtype=np.float64).fit(Z.astype(float), modes,
+                                                                             sample_weight=w, init=q)
+    assert again.converged_ and again.n_sweeps_ <= 2 and again.marg_sweeps_ <= 2
+    assert np.abs(again.e_marg_ - em).max() < 1e-7 * np.abs(em).max()
+    with pytest.raises(ValueError, match='signs'):
+        BLMLQuick(dtype=np.float64).fit(Z.astype(float), modes, sample_weight=w, signs=np.ones(len(w)))
+    with pytest.raises(RuntimeError, match='dep'):
+        BLMLQuick(tol=1e-6, dtype=np.float64).fit(Z.astype(float), [{j: dict(c, h=1.0) for j, c in m.items()}
+                                                                    for m in make_modes([0.35, 0.3])],
+                                                  sample_weight=w).predict_ratio(Zk[:3].astype(float))

@@ -14,7 +14,9 @@ with dense Gram matrices of the same kernels (oracle/blml_pairs.py):
 - sqrt_density at the bins is Se / n (the exact kernel at lattice points); a complex fit with the start search
   asked for raises NotImplementedError;
 - a real fit warm-started from a fit on another lattice (re-binned through the exact kernel) reaches the cold
-  fit's fixed point."""
+  fit's fixed point;
+- a complex conditional fit (dep column 1): the joint and the marginal (the counts summed over column 1) as two
+  separate fits, each against the oracle's psd_fit on its own Gram matrix (likelihood to 1e-8)."""
 import sys
 
 import numpy as np
@@ -174,3 +176,33 @@ def test_rebinned_warm_start_reaches_the_cold_fit():
           f'against {cold.n_sweeps_} cold')
     assert warm.rebinned_ and warm.converged_ and cold.converged_
     assert np.abs(warm.e_hat_ - cold.e_hat_).max() < 1e-6 * np.abs(cold.e_hat_).max()
+
+
+def test_complex_conditional_fit_matches_oracle():
+    from blml_amd import make_modes
+    from blml_amd.quick import BLMLQuick
+    Z, w = _hist((24, 20), 8000, c2)
+    modes = [{j: dict(c, h=1.0) for j, c in m.items()}
+             for m in make_modes([0.4, 0.3], ['sinc', 'dirichlet'], periods=[None, 20.0], dep=[1])]
+    _note('psd quick: complex conditional fit, dep column 1')
+    q = BLMLQuick(tol=1e-7, dtype=np.float64, seed=0)
+    q.n_starts = (0, 0)
+    q.fit(Z.astype(float), modes, sample_weight=w, complex_g=True)
+    keep = w > 0
+    Gj = [OP.dense_pair_gram([('offset', 'sinc', [0.4], [0.0], [0]), ('circular', 'dirichlet', [0.3], [20.0], [1])],
+                             np.ones(2), Z[keep], Z[keep])]
+    Zm, inv = np.unique(Z[:, [0]], axis=0, return_inverse=True)
+    wm = np.bincount(np.ravel(inv), weights=w)
+    km = wm > 0
+    Gm = [OP.dense_pair_gram([('offset', 'sinc', [0.4], [0.0], [0])], np.ones(1), Zm[km], Zm[km])]
+    oj = OQ.psd_fit(Gj, w[keep], tol=1e-7, seed=0, K=1, complex_g=True)
+    om = OQ.psd_fit(Gm, wm[km], tol=1e-7, seed=0, K=1, complex_g=True)
+    assert q.dep_ == [1] and q.converged_ and q._marg.converged_
+    for est, o, Gs, ww in ((q._joint, oj, Gj, w[keep]), (q._marg, om, Gm, wm[km])):
+        n = ww.sum()
+        E = est.e_hat_[:, None]
+        l_q = OQ.psd_loglik(E, OQ.psd_apply(Gs, E), ww, n)
+        l_o = OQ.psd_loglik(o[0], OQ.psd_apply(Gs, o[0]), ww, n)
+        assert abs(l_q - l_o) < 1e-8 * abs(l_o), (l_q, l_o)
+    r = q.predict_ratio(Z[keep][:5].astype(float) + 0.25)
+    assert r.shape == (5,) and np.all(r > 0)

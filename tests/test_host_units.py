@@ -209,7 +209,29 @@ def test_fit_input_errors_mirror_reference():
     with pytest.raises(ValueError, match='modes'):
         spec(ok, modes=M.make_modes([[0.3]] * 9, ['sinc']))
     with pytest.raises(ValueError, match='dep'):
-        M.make_modes([0.3], ['sinc'], dep=[0])
+        M.parse_modes(M.make_modes([0.3, 0.2], ['sinc', 'sinc'], dep=[1]), 2)
+
+
+def test_conditional_modes_parse_and_restrict():
+    """dep columns (blml_trivial.py:1247-1323, 1467-1505, 1585-1602): make_modes marks them (a radial unit through
+    its first column), parse_modes returns them for the counts path and rejects modes that disagree; the joint's
+    modes drop the flags and the marginal's keep the other columns, renumbered, a unit whole or not at all."""
+    from blml_amd import modes as M
+    md = M.make_modes([0.3, 0.2, 0.25], ['ball:a', 'ball:a', 'sinc'], dep=[2])
+    specs, dep = M.parse_modes(md, 3, allow_dep=True)
+    assert dep == [2] and specs[0]['kinds'] == ['ball:0-1', 'ball:0-1', 'sinc']
+    md2 = M.make_modes([0.3, 0.3, 0.25], ['ball:a', 'ball:a', 'sinc'], dep=[0])
+    assert M.parse_modes(md2, 3, allow_dep=True)[1] == [0, 1]
+    assert M.parse_modes(M.strip_dep(md), 3)[1] is None
+    mg = M.marginal_modes(md, [0, 1])
+    assert list(mg[0]) == [(0, 1)] and 'dep' not in mg[0][(0, 1)]
+    assert M.marginal_modes(M.make_modes([0.3, 0.2, 0.25], dep=[0]), [1, 2]) == [{0: dict(kind='sinc', fc=0.2),
+                                                                               1: dict(kind='sinc', fc=0.25)}]
+    with pytest.raises(ValueError, match='straddles'):
+        M.marginal_modes(md, [0, 2])
+    bad = [dict(md[0]), {k: {a: v for a, v in c.items() if a != 'dep'} for k, c in md[0].items()}]
+    with pytest.raises(ValueError, match='same columns'):
+        M.parse_modes(bad, 3, allow_dep=True)
 
 
 def test_pair_spans_reject_ranks_outside_the_table():

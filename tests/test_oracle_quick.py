@@ -3,7 +3,9 @@
 - quick_fit's converged e equals the unique minimiser of the convex objective found by an independent damped Newton
   solve, and satisfies (G e)_b e_b = n w_b;
 - every safeguarded block step does not raise the objective (checked by evaluating it directly);
-- the returned Se is G e; a capped fit is scaled to e'Se = n^2."""
+- the returned Se is G e; a capped fit is scaled to e'Se = n^2;
+- the conditional fit's marginal is the merged-bin minimiser split by the counts (the reference's own statement of
+  what its fused fit computes, blml_gpu.py:67-72)."""
 import numpy as np
 import pytest
 
@@ -33,6 +35,32 @@ CASES = [
     ('2d sinc x dirichlet circular', (12, 16), [('offset', 'sinc', [0.4], [0.0], [0]),
                                                 ('circular', 'dirichlet', [0.3], [16.0], [1])], (1.0, 1.0), 5000),
 ]
+
+
+def test_conditional_fit_marginal_is_the_merged_fit_split_by_counts():
+    """quick_fit_dep (the joint and the marginal of dep column 1 in one loop): the joint is the convex minimiser of
+    the joint problem (newton_solve); the marginal, fitted on the joint's bins with the kernel of the units off the
+    dep side, is the minimiser on the merged bins (bins that differ only in column 1 merged, counts summed) split
+    back in proportion to the counts.  Leaving out the split (every bin of a merged bin given its whole
+    coefficient) fails the same check."""
+    units = [('offset', 'sinc', [0.35], [0.0], [0]), ('offset', 'sinc', [0.3], [0.0], [1])]
+    off = [u for u in units if 1 not in u[4]]
+    Z, w, G = problem((14, 12), units, (1.0, 1.0), 6000, 3)
+    Gm = OP.dense_pair_gram(off, np.ones(2), Z, Z)
+    (ej, Sej, nsj, cj, rj), (em, Sem, nsm, cm, rm) = OQ.quick_fit_dep(G, Gm, w, tol=1e-10, block_size=37, seed=2)
+    assert cj and cm, (nsj, rj, nsm, rm)
+    en = OQ.newton_solve(G, w)
+    assert np.abs(ej - en).max() < 1e-7 * np.abs(en).max()
+    Zr, first, inv = np.unique(Z[:, [0]], axis=0, return_index=True, return_inverse=True)
+    inv = np.ravel(inv)
+    W = np.bincount(inv, weights=w)
+    Gmm = OP.dense_pair_gram(off, np.ones(2), Z[first], Z[first])
+    emm = OQ.newton_solve(Gmm, W)
+    want = emm[inv] * w / W[inv]
+    assert np.abs(em - want).max() < 1e-7 * np.abs(want).max()
+    assert np.abs(emm[inv] - want).max() > 1e-2 * np.abs(want).max()
+    n = w.sum()
+    assert np.abs(Sem * em / (n * w) - 1).max() < 1e-10
 
 
 def test_root_solves_its_quadratic():
