@@ -52,6 +52,13 @@ class EmpParams(ctypes.Structure):
                 ('anderson', ctypes.c_int32), ('reserved', ctypes.c_int32)]
 
 
+class HioParams(ctypes.Structure):
+    _fields_ = [('d', ctypes.c_int32), ('shape', ctypes.c_int32 * MAXD), ('box', ctypes.c_int32 * MAXD),
+                ('n_hio', ctypes.c_int32), ('n_er', ctypes.c_int32), ('n_rand', ctypes.c_int32),
+                ('n_kick', ctypes.c_int32), ('beta', ctypes.c_double), ('ab2', ctypes.c_double),
+                ('nocc', ctypes.c_int64)]
+
+
 EMP_AUX_LEN, EMP_OUT_LEN = MAXK + 1, 4
 TUNE_MAX = 4
 
@@ -66,7 +73,7 @@ EXPORTS = ('blml_plan_create', 'blml_plan_destroy', 'blml_matvec', 'blml_fit', '
            'blml_pair_rows_noalloc', 'blml_point_rows', 'blml_grad_dots', 'blml_quick_psd_sweep',
            'blml_quick_psd_exact', 'blml_quick_psd_stats', 'blml_quick_psd_grad', 'blml_quick_psd_wscale',
            'blml_vec_gram', 'blml_vec_lincomb', 'blml_lbfgs_twoloop', 'blml_ift_elem', 'blml_empirical',
-           'blml_plan_autotune', 'blml_last_error')
+           'blml_plan_autotune', 'blml_complex_starts', 'blml_band_eval', 'blml_last_error')
 
 _lib = None
 
@@ -121,6 +128,8 @@ def load(path=None):
     lib.blml_vec_lincomb.argtypes = [i32, vp, vp, i32, i64, vp, vp]
     lib.blml_lbfgs_twoloop.argtypes = [i32, vp, vp, vp, f64, i64, vp, vp, vp]
     lib.blml_ift_elem.argtypes = [i32, i32, i32, i64, i32, f64, f64, f64, f64, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.blml_complex_starts.argtypes = [ctypes.POINTER(HioParams), vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.blml_band_eval.argtypes = [i32, vp, i64, i64, vp, vp, vp, vp, vp]
     lib.blml_last_error.restype = ctypes.c_char_p
     for name in EXPORTS[:-1]:
         getattr(lib, name).restype = ctypes.c_int

@@ -36,12 +36,15 @@ init= warm-starts a real rank-1 fit from a fit on the same lattice (the previous
 its 1-D minimiser) or on another lattice (re-binned: e_b = w_b / g_prev(z_b) where g_prev is not ~0,
 blml_gpu.py:213-219); a conditional fit warm-starts its joint and, from a conditional fit over the same dep
 columns, its marginal.  sqrt_density / predict evaluate the exact continuous kernel at any points
-(blml_point_rows).  Not built on this engine (they raise NotImplementedError): the complex start search
-(n_starts other than (0, 0) for a complex fit with no periodic column, blml_trivial.py:1605-1676), warm starts of
-complex / rank-K fits (the reference's branch for them, blml_gpu.py:196-219, builds FP64 arrays from the complex
-(B, K) rows of e_hat_ and cannot run either), and points not already binned (modes without h)."""
+(blml_point_rows).  A complex fit with n_starts = (n_hio, n_sb) other than (0, 0) and no periodic column searches
+its starts first (complex_starts: hybrid input-output runs on the FFT grid of the bin box, every run one workgroup
+of blml_complex_starts, blml_trivial.py:1605-1676) and keeps the fit of the largest likelihood (starts_).  Not
+built on this engine (they raise NotImplementedError): warm starts of complex / rank-K fits (the reference's branch
+for them, blml_gpu.py:196-219, builds FP64 arrays from the complex (B, K) rows of e_hat_ and cannot run either),
+and points not already binned (modes without h)."""
 import ctypes
 import os
+import warnings
 
 import numpy as np
 
@@ -120,12 +123,6 @@ class BLMLQuick:
                                           "built")
             if len(specs) > _lib.MAXK:
                 raise ValueError(f"at most {_lib.MAXK} modes on this engine")
-            nh, ns = self.n_starts
-            periodic = any(str(k).partition(':')[0] in ('dirichlet', 'pball') for k in specs[0]['kinds'])
-            if complex_g and nh + ns > 0 and not periodic:
-                raise NotImplementedError("BLMLQuick(engine='hip'): the complex start search (n_starts = "
-                                          f"{tuple(self.n_starts)}) is not built; set n_starts = (0, 0) for the "
-                                          "one-start fit (the real root with phases)")
         keep = w > 0
         sg = None
         if signs is not None:
@@ -149,11 +146,23 @@ class BLMLQuick:
         f32 = self.dtype == np.float32
         rng = np.random.default_rng(self.seed)
         out = self._solve(Zi, w, n, sg, tab, f32, e0, new, rng)
-        self.mode_specs_, self.aa_accepted_ = None, None
+        self.mode_specs_, self.aa_accepted_, self.starts_ = None, None, None
         K = len(specs)
         if psd:
             shared = all(_same_spec(q, specs[0]) for q in specs)
-            out = self._psd(Zi, w, n, specs if not shared else specs[:1], h, f32, out[0], K, bool(complex_g), rng)
+            starts = None
+            nh, ns = self.n_starts
+            if complex_g and nh + ns > 0:
+                # the start search on mode 0's band (blml_gpu.py:470-486)
+                if any(str(k).partition(':')[0] in ('dirichlet', 'pball') for k in specs[0]['kinds']):
+                    warnings.warn("complex_g: start search not implemented for periodic dims; using one start")
+                else:
+                    starts = complex_starts(Zi, w, h, specs[0]['fc'], specs[0]['kinds'], nh, ns, seed=self.seed)
+                    if starts is None:
+                        warnings.warn("complex_g: FFT grid of the bin box too large for the start search; using "
+                                      "one start")
+            out = self._psd(Zi, w, n, specs if not shared else specs[:1], h, f32, out[0], K, bool(complex_g), rng,
+                            starts)
             self.mode_specs_ = None if shared else [dict(q) for q in specs]
         e, Se, n_sweeps, converged, max_rho, rho_hist, obj_hist, safeguard = out
         if sg is not None:
@@ -358,9 +367,12 @@ class BLMLQuick:
                 obj_hist, safeguard)
 
     # ------------------------------------------------------------------------------------- complex / rank K
-    def _psd(self, Zi, w, n, tspecs, h, f32, e_real, K, complex_g, rng):
+    def _psd(self, Zi, w, n, tspecs, h, f32, e_real, K, complex_g, rng, starts=None):
         """The rank-K / complex fit from the real solution e_real (blml_gpu.py:565-744): tspecs holds one spec
-        (a kernel shared by every mode) or one per mode.  Returns _solve's tuple with E (B, K) (B, for K = 1)."""
+        (a kernel shared by every mode) or one per mode.  Returns _solve's tuple with E (B, K) (B, for K = 1).
+        starts: [(family, score, e0)] of the start search; each replaces the real start (blml_gpu.py:476-484), the
+        fits run in turn on the same generator, and the one of the largest likelihood is returned (self.starts_
+        records each)."""
         torch = _torch()
         lib = _lib.load()
         B = len(Zi)
@@ -390,10 +402,6 @@ class BLMLQuick:
         sarr = (ctypes.c_double * K)(*sbbs)
         dw = torch.from_numpy(np.ascontiguousarray(w, np.float64)).cuda()
         stats = torch.zeros(4, dtype=torch.float64, device='cuda')
-        e0 = _psd_start(np.asarray(e_real, np.float64), K, complex_g, rng, float(self.phase_noise))
-        host = np.stack([e0.real, e0.imag], -1) if complex_g else e0[..., None]
-        E = torch.from_numpy(np.ascontiguousarray(host, np.float64)).cuda().reshape(-1)
-        SE = torch.empty_like(E)
         Bs = max(1, min(int(self.block_size), -(-B // 8)))
 
         def ptrs(xs):
@@ -418,16 +426,35 @@ class BLMLQuick:
                                          P(out), st))
             return out.cpu().numpy().reshape(A.shape[0], A.shape[0])
 
-        exact(E, SE)
-        tr = measure(E, SE)[0]
-        sc = n / np.sqrt(max(tr, 1e-300))
-        comb([E], [sc], E)
-        comb([SE], [sc], SE)
-        if self.solver == 'lbfgs':
-            out = self._lbfgs_psd(E, SE, n, dw, tarr, ntab, zarr, B, K, nc, st, exact, measure, comb, gram)
+        def solve(ws):
+            e0 = _psd_start(np.asarray(e_real, np.float64), K, complex_g, rng, float(self.phase_noise), ws)
+            host = np.stack([e0.real, e0.imag], -1) if complex_g else e0[..., None]
+            E = torch.from_numpy(np.ascontiguousarray(host, np.float64)).cuda().reshape(-1)
+            SE = torch.empty_like(E)
+            exact(E, SE)
+            tr = measure(E, SE)[0]
+            sc = n / np.sqrt(max(tr, 1e-300))
+            comb([E], [sc], E)
+            comb([SE], [sc], SE)
+            if self.solver == 'lbfgs':
+                return self._lbfgs_psd(E, SE, n, dw, tarr, ntab, zarr, B, K, nc, st, exact, measure, comb, gram)
+            return self._psd_cd(E, SE, n, dw, tarr, ntab, zarr, B, K, nc, Bs, sarr, st, rng, exact, measure, comb,
+                                gram)
+
+        if starts:
+            best = None
+            self.starts_ = []
+            for family, score, e0 in starts:
+                out = solve(np.asarray(e0)[:, None])
+                oh = out[6]
+                ll = -n * oh[-1] if oh and np.isfinite(oh[-1]) else -np.inf
+                self.starts_.append(dict(family=family, start_score=float(score), loglik=float(ll),
+                                         converged=bool(out[3]), n_sweeps=int(out[2])))
+                if best is None or ll > best[0]:
+                    best = (ll, out)
+            out = best[1]
         else:
-            out = self._psd_cd(E, SE, n, dw, tarr, ntab, zarr, B, K, nc, Bs, sarr, st, rng, exact, measure, comb,
-                               gram)
+            out = solve(None)
         Ef, SEf = out[0], out[1]
         del keep
         to_np = lambda X: _to_host(X, B, K, nc)
@@ -663,22 +690,142 @@ class BLMLQuick:
         return (np.abs(g) ** 2).sum(1) if g.ndim == 2 else np.abs(g) ** 2
 
 
+def _radial_dims(kinds):
+    groups = {}
+    for j, k in enumerate(kinds):
+        name, _, tag = str(k).partition(':')
+        if name in ('ball', 'pball'):
+            groups.setdefault(tag, []).append(j)
+    return list(groups.values())
+
+
+def complex_starts(Zi, w, h, fc, kinds, n_hio=3, n_sb=2, hio_pool=30, sb_delta=1.0, n_hio_iter=400, n_er_iter=100,
+                   beta=0.9, seed=0, max_cells=1 << 22):
+    """Starts of a complex counts fit (blml_trivial.py:1605-1676), every run on the device (blml_complex_starts):
+    [(family, score, e0)] -- the n_hio best of hio_pool random-phase runs (n_hio_iter HIO + n_er_iter ER steps) and
+    the n_sb best symmetry-breaking runs (the band-limited sqrt(counts) kicked by exp(+-i sb_delta p) along each
+    even phase direction, n_er_iter ER steps), by the Poisson score of |g|^2 normalised on the box, each as
+    e0 = w / conj(g) at the bins; None when the search grid has more than max_cells cells or a side above 8192.
+    The host lays out the grid, the band, the draws (default_rng(seed).random per random run) and the kicks as the
+    reference does; the runs' arithmetic is FP64 on the device."""
+    torch = _torch()
+    lib = _lib.load()
+    Zi0 = np.asarray(Zi, np.int64)
+    w0 = np.asarray(w, np.float64)
+    h0 = np.asarray(h, float)
+    fc = np.asarray(fc, float)
+    d = Zi0.shape[1]
+    q = np.maximum(1, np.floor(0.5 / (fc * h0))).astype(np.int64)
+    Zc = np.floor_divide(Zi0 - Zi0.min(0), q)
+    if np.all(q == 1):
+        Zg, wg = Zc, w0
+    else:
+        Zg, inv = np.unique(Zc, axis=0, return_inverse=True)
+        wg = np.bincount(np.ravel(inv), weights=w0)
+    hc = h0 * q
+    lo = Zg.min(0)
+    L = Zg.max(0) - lo + 1
+    shape = tuple(1 << int(np.ceil(np.log2(2 * int(l)))) for l in L)
+    M = int(np.prod(shape, dtype=np.int64))
+    if M > max_cells or max(shape) > 8192:
+        return None
+    # the band: frequencies over fc / 2, |u_j| <= 1 per axis, ||u_G|| <= 1 per radial group
+    U = np.stack(np.meshgrid(*[np.fft.fftfreq(m, hj) / (0.5 * f) for m, hj, f in zip(shape, hc, fc)],
+                             indexing='ij'), -1)
+    band = np.ones(shape, bool)
+    rad = set()
+    for dims in _radial_dims(kinds):
+        band &= (U[..., dims] ** 2).sum(-1) <= 1
+        rad |= set(dims)
+    for j in range(d):
+        if j not in rad:
+            band &= np.abs(U[..., j]) <= 1
+    kicks = []
+    if n_sb > 0:
+        r2 = (U ** 2).sum(-1)
+        dirs = [2 * r2 - 1] + [U[..., 0] ** 2 - U[..., j] ** 2 for j in range(1, d)]
+        dirs += [2 * U[..., i] * U[..., j] for i in range(d) for j in range(i + 1, d)] + [6 * r2 ** 2 - 6 * r2 + 1]
+        kicks = [sg * sb_delta * p for p in dirs for sg in (1, -1)]
+    rng = np.random.default_rng(seed)
+    n_rand = hio_pool if n_hio > 0 else 0
+    phase = np.stack([rng.random(shape).ravel() for _ in range(n_rand)]) if n_rand else np.zeros((0, M))
+    runs = n_rand + len(kicks)
+    if runs == 0:
+        return []
+    amp = np.zeros(shape)
+    idx = tuple((Zg - lo).T)
+    amp[idx] = np.sqrt(wg)
+    occ = np.ravel_multi_index(idx, shape).astype(np.int64)
+    dev = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to('cuda', dt)
+    d_band = dev(band.ravel().astype(np.uint8), torch.uint8)
+    d_amp, d_occ, d_w = dev(amp.ravel()), dev(occ, torch.int64), dev(wg)
+    d_phase = dev(phase.reshape(-1)) if n_rand else None
+    d_kick = dev(np.stack([k.ravel() for k in kicks]).reshape(-1)) if kicks else None
+    dP = torch.empty(runs * M * 2, dtype=torch.float64, device='cuda')
+    dg = torch.empty(runs * len(occ) * 2, dtype=torch.float64, device='cuda')
+    dscore = torch.empty(runs, dtype=torch.float64, device='cuda')
+    prm = _lib.HioParams()
+    prm.d = d
+    for j in range(d):
+        prm.shape[j], prm.box[j] = shape[j], int(L[j])
+    prm.n_hio, prm.n_er, prm.n_rand, prm.n_kick = int(n_hio_iter), int(n_er_iter), n_rand, len(kicks)
+    prm.beta, prm.ab2, prm.nocc = float(beta), float(wg.sum()), len(occ)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    V = lambda x: None if x is None else ctypes.c_void_p(x.data_ptr())
+    _lib.check(lib.blml_complex_starts(ctypes.byref(prm), V(d_band), V(d_amp), V(d_occ), V(d_w), V(d_phase),
+                                       V(d_kick), V(dP), V(dg), V(dscore), st))
+    score = dscore.cpu().numpy()
+    fam = ['hio'] * n_rand + ['symbreak'] * len(kicks)
+    out = []
+    for family, k in (('hio', n_hio), ('symbreak', n_sb)):
+        cand = [r for r in range(runs) if fam[r] == family]
+        for r in sorted(cand, key=lambda r: -score[r])[:k]:
+            if np.all(q == 1):
+                gv = dg[r * len(occ) * 2:(r + 1) * len(occ) * 2].cpu().numpy()
+                g = gv[0::2] + 1j * gv[1::2]
+            else:
+                nz = np.flatnonzero(band.ravel())
+                Pr = dP[r * M * 2:(r + 1) * M * 2].cpu().numpy()
+                c = (Pr[0::2] + 1j * Pr[1::2])[nz] / M
+                kf = np.stack(np.unravel_index(nz, shape), 1)
+                kf = np.where(kf < np.array(shape) // 2, kf, kf - np.array(shape)).astype(np.int32)
+                dz = (Zi0 - Zi0.min(0)).astype(np.int32)
+                mq = (np.array(shape) * q).astype(np.int32)
+                dc = dev(np.stack([c.real, c.imag], 1).ravel())
+                ddz, dkf = dev(dz, torch.int32), dev(kf, torch.int32)
+                dgb = torch.empty(len(Zi0) * 2, dtype=torch.float64, device='cuda')
+                _lib.check(lib.blml_band_eval(d, mq.ctypes.data_as(ctypes.c_void_p), len(Zi0), len(nz), V(ddz),
+                                              V(dkf), V(dc), V(dgb), st))
+                gv = dgb.cpu().numpy()
+                g = gv[0::2] + 1j * gv[1::2]
+            out.append((family, float(score[r]), w0 / np.conj(np.where(np.abs(g) == 0, 1e-12, g))))
+    return out
+
+
 def _same_spec(a, b):
     """One kernel: the same kinds, periods and fc in every column (the reference's _shared_kernel)."""
     return (list(a['kinds']) == list(b['kinds']) and list(a['periods']) == list(b['periods'])
             and np.allclose(np.asarray(a['fc'], float), np.asarray(b['fc'], float)))
 
 
-def _psd_start(e_real, K, complex_g, rng, pn):
-    """E0 (B, K) of the rank-K / complex fit (blml_gpu.py:582-590): column 0 the real solution, with phases
-    exp(i pn N(0, 1)) when complex; the other columns pn |e| times CN(0, 1) (real: N(0, 1)), drawn in that order."""
+def _psd_start(e_real, K, complex_g, rng, pn, ws=None):
+    """E0 (B, K) of the rank-K / complex fit (blml_gpu.py:576-590): column 0 the real solution, with phases
+    exp(i pn N(0, 1)) when complex; the other columns pn |e| times CN(0, 1) (real: N(0, 1)), drawn in that order.
+    ws (B, K'): a start that replaces the first K' columns, the rest drawn as above around |ws[:, 0]|."""
     B = len(e_real)
     e0 = np.zeros((B, K), dtype=complex if complex_g else np.float64)
-    e0[:, 0] = e_real * (np.exp(1j * pn * rng.standard_normal(B)) if complex_g else 1.0)
-    for k in range(1, K):
+    if ws is not None:
+        ws = np.asarray(ws, dtype=complex if complex_g else np.float64)
+        k0 = min(K, ws.shape[1])
+        e0[:, :k0] = ws[:, :k0]
+        er = np.abs(ws[:, 0])
+    else:
+        k0, er = 1, e_real
+        e0[:, 0] = e_real * (np.exp(1j * pn * rng.standard_normal(B)) if complex_g else 1.0)
+    for k in range(k0, K):
         noise = ((rng.standard_normal(B) + 1j * rng.standard_normal(B)) / np.sqrt(2) if complex_g
                  else rng.standard_normal(B))
-        e0[:, k] = pn * e_real * noise
+        e0[:, k] = pn * er * noise
     return e0
 
 

@@ -28,6 +28,7 @@
 #include "quick_psd.hpp"
 #include "fit.hpp"
 #include "empirical.hpp"
+#include "starts.hpp"
 #include "plan_layout.hpp"
 
 namespace {
@@ -435,6 +436,13 @@ template <typename T>
 __global__ void tune_fill_kernel(T *E, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         E[i] = (T)(0.5 + 0.25 * sin(0.37 * (double)(i % 9973)));
+}
+
+// The start search: every run one workgroup, dealt round robin (runs are few: tens).
+__global__ void __launch_bounds__(BLML_NT) hio_kernel(HioArgs a) {
+    __shared__ double red[BLML_NWAVE];
+    const int runs = a.n_rand + a.n_kick;
+    for (int r = blockIdx.x; r < runs; r += gridDim.x) hio_run(a, r, red);
 }
 
 // ------------------------------------------------------------------------------------------------ C-ABI
@@ -1632,6 +1640,102 @@ int blml_net_adam(float *p, const float *g, float *m, float *v, void *bf, int64_
 int blml_net_to_bf16(const float *x, void *y, int64_t n, void *stream) {
     if (!x || !y || n < 0) return fail(BLML_EVALUE, "bad argument");
     if (n) net_to_bf16_kernel<<<NET_GRID(n), 256, 0, (hipStream_t)stream>>>(x, (bf16_t *)y, n);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+// ------------------------------------------------------------------------------------------------ start search
+int blml_complex_starts(const blml_hio_params_t *prm, const uint8_t *band, const double *amp, const int64_t *occ,
+                        const double *wocc, const double *phase, const double *kick, double *P, double *gocc,
+                        double *score, void *stream) {
+    if (!prm) return fail(BLML_EVALUE, "null parameters");
+    const blml_hio_params_t q = *prm;
+    if (q.d < 1 || q.d > BLML_MAXD) return fail(BLML_EVALUE, "d: 1 .. 5 axes");
+    long ncell = 1;
+    int Lmax = 2;
+    for (int j = 0; j < q.d; ++j) {
+        const int m = q.shape[j];
+        if (m < 2 || m > 16 * BLML_NT || (m & (m - 1)))
+            return fail(BLML_EVALUE, "grid sides: powers of two from 2 to 8192");
+        if (q.box[j] < 1 || q.box[j] > m) return fail(BLML_EVALUE, "box: 1 .. the grid side per axis");
+        ncell *= m;
+        Lmax = std::max(Lmax, m);
+    }
+    if (ncell >= (1L << 31)) return fail(BLML_EVALUE, "grid: below 2^31 cells");
+    if (q.n_rand < 0 || q.n_kick < 0 || q.n_rand + q.n_kick < 1) return fail(BLML_EVALUE, "runs: at least one");
+    if (q.n_hio < 0 || q.n_er < 0) return fail(BLML_EVALUE, "steps must be >= 0");
+    if (q.nocc < 1 || !(q.ab2 > 0)) return fail(BLML_EVALUE, "no counts on the box");
+    if (!band || !amp || !occ || !wocc || !P || !gocc || !score || (q.n_rand && !phase) || (q.n_kick && !kick))
+        return fail(BLML_EVALUE, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    int dev = 0, max_lds = 0, st_lds = 0, ncu = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    int rc = static_lds((const void *)hio_kernel, st_lds);
+    if (rc) return rc;
+    long lds = 0;      // the line passes' dynamic LDS, as run_pass lays it out
+    for (int j = 0; j < q.d; ++j) {
+        const PassShape sh = pass_shape(OP_FFT_FULL, q.shape[j], q.shape[j], 0, ncell / q.shape[j]);
+        lds = std::max(lds, (long)(sh.roots + (long)sh.lines * sh.buf) * (long)sizeof(cpx<double>));
+    }
+    if (lds + st_lds > max_lds) return fail(BLML_EVALUE, "grid side too long for the line passes' LDS");
+    const int runs = q.n_rand + q.n_kick;
+    StreamScratch sc(st);
+    HIPCHECK(sc.alloc(sizeof(cpx<double>) * ((size_t)runs * ncell + Lmax)));
+    cpx<double> *X = (cpx<double> *)sc.p, *tw = X + (size_t)runs * ncell;
+    twiddle_kernel<double><<<(Lmax + 255) / 256, 256, 0, st>>>(tw, Lmax);
+    HIPCHECK(hipGetLastError());
+    HioArgs a;
+    a.d = q.d;
+    for (int j = 0; j < BLML_MAXD; ++j) {
+        a.shape[j] = j < q.d ? q.shape[j] : 1;
+        a.box[j] = j < q.d ? q.box[j] : 1;
+    }
+    a.ncell = ncell;
+    a.n_hio = q.n_hio;
+    a.n_er = q.n_er;
+    a.n_rand = q.n_rand;
+    a.n_kick = q.n_kick;
+    a.beta = q.beta;
+    a.ab2 = q.ab2;
+    a.nocc = q.nocc;
+    a.band = band;
+    a.amp = amp;
+    a.occ = (const long *)occ;
+    a.wocc = wocc;
+    a.phase = phase;
+    a.kick = kick;
+    a.P = (cpx<double> *)P;
+    a.X = X;
+    a.gocc = (cpx<double> *)gocc;
+    a.score = score;
+    a.tw = tw;
+    a.Lmax = Lmax;
+    hio_kernel<<<std::min(runs, std::max(ncu, 1)), BLML_NT, (unsigned)lds, st>>>(a);
+    HIPCHECK(hipGetLastError());
+    return BLML_OK;
+}
+
+int blml_band_eval(int32_t d, const int32_t *mq, int64_t nb, int64_t nband, const int32_t *dz, const int32_t *kf,
+                   const double *c, double *g, void *stream) {
+    if (d < 1 || d > BLML_MAXD || !mq) return fail(BLML_EVALUE, "d: 1 .. 5 axes");
+    if (nb < 0 || nband < 0) return fail(BLML_EVALUE, "counts must be >= 0");
+    if (nb == 0) return BLML_OK;
+    if (!dz || !g || (nband && (!kf || !c))) return fail(BLML_EVALUE, "null argument");
+    BandEvalArgs a;
+    a.d = d;
+    for (int j = 0; j < BLML_MAXD; ++j) {
+        a.mq[j] = j < d ? mq[j] : 1;
+        if (j < d && mq[j] < 1) return fail(BLML_EVALUE, "mq: positive");
+    }
+    a.nb = nb;
+    a.nband = nband;
+    a.dz = dz;
+    a.kf = kf;
+    a.c = (const cpx<double> *)c;
+    a.g = (cpx<double> *)g;
+    band_eval_kernel<<<(unsigned)((nb + 255) / 256), 256, 0, (hipStream_t)stream>>>(a);
     HIPCHECK(hipGetLastError());
     return BLML_OK;
 }

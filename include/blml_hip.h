@@ -301,6 +301,37 @@ int blml_point_rows(const blml_pointunit_t *units, int32_t U, int32_t d, const i
                     const double *Z, const double *W, int32_t nw, int64_t Nq, int64_t M, double *S, void *stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * The start search of a complex counts fit (blml_trivial.py:1605-1676 complex_starts, which _BLMLQuickCUDA.fit
+ * calls at blml_gpu.py:470-486): hybrid input-output and error-reduction runs on the FFT grid of the bin box.  The
+ * host builds the grid, the band, the draws and the kicks as the reference does; every run is one workgroup. */
+typedef struct {
+    int32_t d;
+    int32_t shape[BLML_MAXD];   /* grid sides: powers of two, 2 .. 8192 */
+    int32_t box[BLML_MAXD];     /* the data box: cells [0, box_j) of each axis */
+    int32_t n_hio, n_er;        /* steps of a random-phase run (HIO, then ER); a symmetry-breaking run: n_er ER */
+    int32_t n_rand, n_kick;     /* runs: n_rand random-phase runs first, then n_kick symmetry-breaking runs */
+    double beta;                /* the HIO feedback */
+    double ab2;                 /* sum of the counts on the box */
+    int64_t nocc;               /* occupied cells */
+} blml_hio_params_t;
+
+/* Runs every start (all device pointers, cells row-major with the last axis fastest): band (ncell) uint8, amp
+ * (ncell) sqrt(counts) on the box, occ (nocc) int64 cells with counts wocc (nocc), phase (n_rand x ncell) uniform
+ * draws, kick (n_kick x ncell) the phase kicks; out: P (runs x ncell complex, interleaved doubles) the final
+ * band-limited spectra (numpy's fftn convention), gocc (runs x nocc complex) the fields ifftn(P) at the occupied
+ * cells, score (runs) sum_occ w log(|g|^2 / sum_box |g|^2). */
+int blml_complex_starts(const blml_hio_params_t *prm, const uint8_t *band, const double *amp, const int64_t *occ,
+                        const double *wocc, const double *phase, const double *kick, double *P, double *gocc,
+                        double *score, void *stream);
+
+/* The exact band-limited field of a coarse-grid start at the original bins (blml_trivial.py:1660-1664):
+ * g_b = sum_n c_n exp(2 pi i sum_j dz_bj kf_nj / mq_j) with dz (nb x d) int32 bin offsets from the box corner, kf
+ * (nband x d) int32 signed frequency indices, c (nband complex) and g (nb complex) device arrays; mq (d) host
+ * int32, the grid side times the coarsening of each axis. */
+int blml_band_eval(int32_t d, const int32_t *mq, int64_t nb, int64_t nband, const int32_t *dz, const int32_t *kf,
+                   const double *c, double *g, void *stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * The warm-start network (no counterpart in the reference: the learned start of the fit, entering through
  * BLMLBinned.fit(init=...), blml_binned.py:403-408).  NCHW images of width 64, even height; bf16 = 16-bit brain
  * floats (uint16 storage); FP32 accumulation.  All pointers are device pointers. */

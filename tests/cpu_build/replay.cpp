@@ -15,6 +15,7 @@
 #include "hip/hip_runtime.h"
 #include "plan_layout.hpp"
 #include "empirical.hpp"
+#include "starts.hpp"
 
 #include <stdio.h>
 #include <stdlib.h>
@@ -364,7 +365,93 @@ int replay(In &in, const char *out_path) {
 
 }  // namespace
 
+// --hio <in> <out>: the start search's runs (hio_run, hio_kernel's work for each run) on one workgroup each, every
+// array an allocation of its exact size.  Input: int32 d, shape[5], box[5], n_hio, n_er, n_rand, n_kick; double
+// beta, ab2; int64 nocc; uint8 band (ncell); double amp (ncell); int64 occ (nocc); double wocc (nocc), phase
+// (n_rand ncell), kick (n_kick ncell).  Output: P (runs ncell complex), gocc (runs nocc complex), score (runs).
+int replay_hio(const char *in_path, const char *out_path) {
+    FILE *f = fopen(in_path, "rb");
+    if (!f) return 2;
+    int32_t hdr[15];
+    double bd[2];
+    int64_t nocc = 0;
+    bool ok = rd(f, hdr, 15) && rd(f, bd, 2) && rd(f, &nocc, 1);
+    HioArgs a{};
+    a.d = hdr[0];
+    long M = 1;
+    int Lmax = 2;
+    for (int j = 0; j < BLML_MAXD; ++j) {
+        a.shape[j] = j < a.d ? hdr[1 + j] : 1;
+        a.box[j] = j < a.d ? hdr[6 + j] : 1;
+        if (j < a.d) {
+            M *= a.shape[j];
+            Lmax = std::max(Lmax, a.shape[j]);
+        }
+    }
+    a.n_hio = hdr[11];
+    a.n_er = hdr[12];
+    a.n_rand = hdr[13];
+    a.n_kick = hdr[14];
+    a.beta = bd[0];
+    a.ab2 = bd[1];
+    a.ncell = M;
+    a.nocc = nocc;
+    const int runs = a.n_rand + a.n_kick;
+    unsigned char *band = dev_alloc<unsigned char>(M);
+    double *amp = dev_alloc<double>(M);
+    long *occ = dev_alloc<long>(nocc);
+    double *wocc = dev_alloc<double>(nocc);
+    double *phase = dev_alloc<double>((size_t)a.n_rand * M);
+    double *kick = dev_alloc<double>((size_t)a.n_kick * M);
+    ok = ok && rd(f, band, M) && rd(f, amp, M) && rd(f, occ, nocc) && rd(f, wocc, nocc) &&
+         rd(f, phase, (size_t)a.n_rand * M) && rd(f, kick, (size_t)a.n_kick * M);
+    fclose(f);
+    if (!ok) return 2;
+    cpx<double> *P = dev_alloc<cpx<double>>((size_t)runs * M);
+    cpx<double> *X = dev_alloc<cpx<double>>((size_t)runs * M);
+    cpx<double> *g = dev_alloc<cpx<double>>((size_t)runs * nocc);
+    double *score = dev_alloc<double>(runs);
+    cpx<double> *tw = dev_alloc<cpx<double>>(Lmax);
+    for (int i = 0; i < Lmax; ++i) {
+        double sn, cs;
+        sincospi(-2.0 * (double)i / (double)Lmax, &sn, &cs);
+        tw[i] = {cs, sn};
+    }
+    a.band = band;
+    a.amp = amp;
+    a.occ = occ;
+    a.wocc = wocc;
+    a.phase = phase;
+    a.kick = kick;
+    a.P = P;
+    a.X = X;
+    a.gocc = g;
+    a.score = score;
+    a.tw = tw;
+    a.Lmax = Lmax;
+    long lds = 0;
+    for (int j = 0; j < a.d; ++j) {
+        const PassShape sh = pass_shape(OP_FFT_FULL, a.shape[j], a.shape[j], 0, M / a.shape[j]);
+        lds = std::max(lds, (long)(sh.roots + (long)sh.lines * sh.buf) * (long)sizeof(cpx<double>));
+    }
+    for (int r = 0; r < runs; ++r)
+        launch((int)lds, 8192 + 64, 1, [&] {
+            double *red = cpuhip::lds_obj<double>(6, BLML_NWAVE);
+            hio_run(a, r, red);
+        });
+    FILE *fo = fopen(out_path, "wb");
+    if (!fo) return 5;
+    fwrite(P, sizeof(cpx<double>), (size_t)runs * M, fo);
+    fwrite(g, sizeof(cpx<double>), (size_t)runs * nocc, fo);
+    fwrite(score, sizeof(double), runs, fo);
+    fclose(fo);
+    printf("replay: %d start-search runs on a grid of %ld cells\n", runs, M);
+    printf("replay: ok\n");
+    return 0;
+}
+
 int main(int argc, char **argv) {
+    if (argc == 4 && strcmp(argv[1], "--hio") == 0) return replay_hio(argv[2], argv[3]);
     if (argc != 3) {
         fprintf(stderr, "usage: replay <plan.bin> <out.bin>\n");
         return 2;

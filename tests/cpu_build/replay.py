@@ -20,7 +20,7 @@ BUILD = os.path.join(HERE, '_build')
 EXE = os.path.join(BUILD, 'replay')
 CXX = os.environ.get('BLML_REPLAY_CXX', '/opt/rocm/llvm/bin/clang++')
 HEADERS = ('common.hpp', 'separable.hpp', 'transform.hpp', 'solver.hpp', 'fit.hpp', 'plan_layout.hpp',
-           'empirical.hpp')
+           'empirical.hpp', 'starts.hpp')
 # every LDS declaration of the device headers -> a per-workgroup object of the CPU stand-in (one heap block each)
 LDS_REWRITES = {
     'common.hpp': [('extern __shared__ __align__(16) unsigned char g_dyn_lds[];',
@@ -164,3 +164,42 @@ def run_pipe(batch, ranks=4, timeout=1800, exe=None):
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return pipe_input(n), raw[:n], raw[n:2 * n]
+
+
+def run_hio(shape, box, n_hio, n_er, beta, ab2, band, amp, occ, wocc, phase, kick, timeout=1800, exe=None):
+    """Replays the start search's runs (hio_run) on the CPU: returns (P (runs, ncell) complex, g (runs, nocc)
+    complex, score (runs,)).  phase (n_rand, ncell), kick (n_kick, ncell)."""
+    exe = exe or build()
+    d = len(shape)
+    M = int(np.prod(shape))
+    phase = np.asarray(phase, np.float64).reshape(-1, M)
+    kick = np.asarray(kick, np.float64).reshape(-1, M)
+    runs = len(phase) + len(kick)
+    hdr = np.zeros(15, np.int32)
+    hdr[0] = d
+    hdr[1:1 + d] = shape
+    hdr[6:6 + d] = box
+    hdr[11:15] = n_hio, n_er, len(phase), len(kick)
+    tmp = tempfile.mkdtemp(prefix='blml_replay_')
+    try:
+        pin, pout = os.path.join(tmp, 'hio.bin'), os.path.join(tmp, 'out.bin')
+        with open(pin, 'wb') as f:
+            f.write(hdr.tobytes())
+            f.write(np.array([beta, ab2], np.float64).tobytes())
+            f.write(np.array([len(occ)], np.int64).tobytes())
+            f.write(np.ascontiguousarray(band, np.uint8).tobytes())
+            for a, dt in ((amp, np.float64), (occ, np.int64), (wocc, np.float64), (phase, np.float64),
+                          (kick, np.float64)):
+                f.write(np.ascontiguousarray(a, dt).tobytes())
+        env = dict(os.environ, ASAN_OPTIONS='detect_leaks=0:abort_on_error=0:detect_stack_use_after_return=0')
+        p = subprocess.run([exe, '--hio', pin, pout], capture_output=True, text=True, timeout=timeout, env=env)
+        if p.returncode != 0:
+            raise RuntimeError(f"replay exited with {p.returncode}:\n{(p.stdout + p.stderr)[-6000:]}")
+        raw = np.fromfile(pout, np.float64)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    nocc = len(occ)
+    P = raw[:runs * M * 2].reshape(runs, M, 2)
+    g = raw[runs * M * 2:runs * (M + nocc) * 2].reshape(runs, nocc, 2)
+    score = raw[runs * (M + nocc) * 2:]
+    return P[..., 0] + 1j * P[..., 1], g[..., 0] + 1j * g[..., 1], score

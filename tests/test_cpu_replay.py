@@ -195,3 +195,28 @@ def test_empirical_retrieve_matches_reference_golden(replay):
     assert np.max(np.abs(out['density'][:counts.size] - z['c0_density'])) < 1e-5 * z['c0_density'].max()
     J = out['merit_history'][0]
     assert len(J) == int(r['start_sweeps']) + 1 and np.all(np.diff(J) <= 1e-12 * np.abs(J[:-1]))
+
+
+def test_start_search_runs_match_oracle(replay):
+    """The start search's device code (hio_run: FFT line passes, the data and band constraints, the score) on the
+    CPU under AddressSanitizer, on the 2-D 24 x 20 case's 64 x 64 grid: three random runs and the eight symmetry-
+    breaking runs of 3 HIO + 2 ER steps (few enough that rounding has not grown: the iteration is chaotic over
+    hundreds) equal the oracle's runs -- spectra to 1e-9 of their scale, scores to 1e-10 relative."""
+    sys.path.insert(0, os.path.join(HERE, 'golden'))
+    from cases_starts import starts_cases
+    from oracle import blml_starts as OS
+    name, Z, w, h, fc, kinds = starts_cases()[1]
+    G, band, runs = OS.start_runs(Z, w, h, fc, kinds, hio_pool=3, n_hio_iter=3, n_er_iter=2, seed=0)
+    M = int(np.prod(G['shape']))
+    rng = np.random.default_rng(0)
+    phase = np.stack([rng.random(G['shape']).ravel() for _ in range(3)])
+    U, _ = OS.band_grid(G['shape'], G['hc'], fc, kinds)
+    kick = np.stack([(sg * 1.0 * p).ravel() for p in OS.even_kicks(U) for sg in (1, -1)])
+    occ = np.ravel_multi_index(G['idx'], G['shape'])
+    P, g, score = replay.run_hio(G['shape'], G['L'], 3, 2, 0.9, G['Ab2'], band.ravel(), G['A'].ravel(), occ,
+                                 G['wg'], phase, kick)
+    assert len(runs) == len(P) == 11
+    for r, (fam, _, Po, so) in enumerate(runs):
+        assert np.abs(P[r] - Po.ravel()).max() < 1e-9 * np.abs(Po).max(), (r, fam)
+        assert score[r] == pytest.approx(so, rel=1e-10)
+        assert np.abs(g[r] - np.fft.ifftn(Po)[G['idx']]).max() < 1e-9 * np.abs(np.fft.ifftn(Po)).max()

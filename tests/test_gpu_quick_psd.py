@@ -11,12 +11,14 @@ with dense Gram matrices of the same kernels (oracle/blml_pairs.py):
   (converged CD) or 1e-8, and the density at the bins to 1e-3 (E's components in the near-null space of S stay
   where each path left them, as lbfgs_psd's docstring says);
 - FP32 tables: the likelihood within 1e-6 of the FP64 oracle's;
-- sqrt_density at the bins is Se / n (the exact kernel at lattice points); a complex fit with the start search
-  asked for raises NotImplementedError;
+- sqrt_density at the bins is Se / n (the exact kernel at lattice points);
+- the start search (blml_complex_starts) step for step against the oracle on short runs, and a default complex fit
+  with its five starts against the oracle's fixed point;
 - a real fit warm-started from a fit on another lattice (re-binned through the exact kernel) reaches the cold
   fit's fixed point;
 - a complex conditional fit (dep column 1): the joint and the marginal (the counts summed over column 1) as two
   separate fits, each against the oracle's psd_fit on its own Gram matrix (likelihood to 1e-8)."""
+import os
 import sys
 
 import numpy as np
@@ -151,8 +153,59 @@ def test_psd_queries_and_start_search():
     assert np.abs(g - q.Se_ / q.n_samples_).max() < 1e-10 * np.abs(g).max()
     f = q.predict(q.Z_ + 0.37)
     assert f.shape == (len(q.Z_),) and np.all(f >= 0)
-    with pytest.raises(NotImplementedError):
-        BLMLQuick(tol=1e-6).fit(Z.astype(float), _modes(case[3]), sample_weight=w, complex_g=True)
+
+
+@pytest.mark.parametrize('i', range(4), ids=['1d', '2d', '1d coarse q=5', '2d ball'])
+def test_start_search_short_runs_match_oracle(i):
+    """blml_complex_starts through the product's complex_starts, 4 random runs and every symmetry-breaking run of
+    3 HIO + 2 ER steps (the iteration is chaotic over hundreds of steps, so two FFT implementations agree over a
+    few only), against the oracle's search with the same draws: the same families in the same order (the same
+    runs selected), scores to 1e-9 relative, starts e0 to 1e-7 of their scale (the coarse case through
+    blml_band_eval's exact band sum)."""
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), 'golden'))
+    from cases_starts import starts_cases
+    from oracle import blml_starts as OS
+    from blml_amd.quick import complex_starts
+    name, Z, w, h, fc, kinds = starts_cases()[i]
+    kw = dict(hio_pool=4, n_hio_iter=3, n_er_iter=2, seed=0)
+    _note(f'start search, short runs: {name}')
+    got = complex_starts(Z, w, h, fc, kinds, **kw)
+    want = OS.complex_starts(Z, w, h, fc, kinds, **kw)
+    assert [s[0] for s in got] == [s[0] for s in want]
+    for (_, sg, eg), (_, sw, ew) in zip(got, want):
+        assert sg == pytest.approx(sw, rel=1e-9)
+        assert np.abs(eg - ew).max() < 1e-7 * np.abs(ew).max()
+
+
+def test_complex_fit_with_the_start_search():
+    """The default complex counts fit (n_starts = (3, 2): 30 random runs of 400 HIO + 100 ER steps and the
+    symmetry-breaking runs, on the device): five starts, each fitted; the best start's fit equals the oracle's
+    fixed point on this case (every start of the reference's own search reaches it too) to 1e-9; the search's best
+    random score is within 5e-4 of the reference's (tests/golden/complex_starts.npz: the same draws, a chaotic
+    iteration, so the same distribution rather than the same runs)."""
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), 'golden'))
+    from cases_starts import starts_cases
+    from blml_amd.quick import BLMLQuick, complex_starts
+    name, Z, w, h, fc, kinds = starts_cases()[0]
+    _note('complex fit with the start search: 1d sinc 40 bins')
+    q = BLMLQuick(tol=1e-7, dtype=np.float64, seed=0).fit(Z.astype(float), [{0: dict(fc=0.35, h=1.0)}],
+                                                           sample_weight=w, complex_g=True)
+    assert len(q.starts_) == 5 and [s['family'] for s in q.starts_] == ['hio'] * 3 + ['symbreak'] * 2
+    assert all(s['converged'] for s in q.starts_) and q.converged_
+    G0 = [OP.dense_pair_gram([('offset', 'sinc', [0.35], [0.0], [0])], np.ones(1), Z, Z)]
+    one = OQ.psd_fit(G0, w, tol=1e-7, seed=0, K=1, complex_g=True)
+    n = w.sum()
+    l_o = OQ.psd_loglik(one[0], OQ.psd_apply(G0, one[0]), w, n)
+    E = q.e_hat_[:, None]
+    l_q = OQ.psd_loglik(E, OQ.psd_apply(G0, E), w, n)
+    assert abs(l_q - l_o) < 1e-9 * abs(l_o)
+    assert max(s['loglik'] for s in q.starts_) == pytest.approx(l_q * n, rel=1e-9)
+    z = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'complex_starts.npz'))
+    ref_best = float(z['c0_score'][0])
+    got_best = max(s['start_score'] for s in q.starts_ if s['family'] == 'hio')
+    assert abs(got_best - ref_best) < 5e-4 * abs(ref_best), (got_best, ref_best)
+    s2 = complex_starts(Z, w, np.ones(1), np.array([0.35]), ['sinc'])
+    assert [s[1] for s in s2] == [s['start_score'] for s in q.starts_]
 
 
 def test_rebinned_warm_start_reaches_the_cold_fit():

@@ -166,7 +166,8 @@ def test_struct_layouts_match_header(tmp_path):
     from blml_amd import local, points
     fields = {'blml_factor_t': _lib.Factor, 'blml_fit_t': _lib.Fit, 'blml_params_t': _lib.Params,
               'blml_result_t': _lib.Result, 'blml_pairtab_t': local.PairTab, 'blml_pairgrad_t': local.PairGrad,
-              'blml_pointunit_t': points.PointUnit, 'blml_emp_params_t': _lib.EmpParams}
+              'blml_pointunit_t': points.PointUnit, 'blml_emp_params_t': _lib.EmpParams,
+              'blml_hio_params_t': _lib.HioParams}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "blml_hip.h"', 'int main(void) {']
     for cname, py in fields.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
