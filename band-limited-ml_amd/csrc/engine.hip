@@ -91,20 +91,19 @@ __global__ void __launch_bounds__(BLML_NT) matvec_grid_kernel(DevPlan<T> p, unsi
     run_matvec<T>(p, 0, Team{(int)blockIdx.x, (int)gridDim.x, bar, part, 0}, E, U, p.sep);
 }
 
-// The stand-alone mat-vec of a separable spread plan whose every pass is a pipelined whole-tile pass (the host
-// checks it): only that path is compiled in, so the kernel's registers serve it alone and nothing is saved to
-// scratch.  FP32 plans take the centrosymmetric split of the line operators (half the matrix-core work).
+// One pass of one mode of the stand-alone mat-vec of a separable spread plan whose every pass is a pipelined
+// whole-tile pass (the host checks it): only that path is compiled in, so the kernel's registers serve it alone
+// and nothing is saved to scratch.  The passes are separate launches on one stream (the stream orders them; no
+// grid barrier, no cooperative launch).  FP32 plans take the centrosymmetric split of the line operators.
 template <typename T>
-__global__ void __launch_bounds__(BLML_NT) matvec_pipe_kernel(DevPlan<T> p, unsigned *bar, double *part, const T *E,
-                                                             T *U) {
-    const Team tm{(int)blockIdx.x, (int)gridDim.x, bar, part, 0};
+__global__ void __launch_bounds__(BLML_NT) matvec_pipe_kernel(DevPlan<T> p, int k, int q, const T *E, T *U) {
     if constexpr (sizeof(T) == 4) {
         if (p.sep_tab_c) {
-            run_matvec_pipe<T, true>(p, tm, E, U);
+            run_matvec_pipe_pass<T, true>(p, (int)blockIdx.x, (int)gridDim.x, k, q, E, U);
             return;
         }
     }
-    run_matvec_pipe<T, false>(p, tm, E, U);
+    run_matvec_pipe_pass<T, false>(p, (int)blockIdx.x, (int)gridDim.x, k, q, E, U);
 }
 
 // The batched path: a persistent grid whose workgroups take fits from an atomic queue, longest first; one
@@ -583,18 +582,27 @@ int blml_matvec(void *plan, const void *E, void *U, void *stream) {
         const bool al16 = (((unsigned long)E | (unsigned long)U) & 15) == 0 && (f.out_off * pl->esize) % 16 == 0 &&
                           (f.K == 1 || mb % 16 == 0);
         const bool pipe = pl->pipe_mv && al16;
+        if (pipe) {           // every pass of every mode a launch of its own, in order on the stream
+            for (int k = 0; k < f.K; ++k)
+                for (int q = 0; q < pl->sep_npass; ++q) {
+                    if (pl->dtype == BLML_F32)
+                        matvec_pipe_kernel<float><<<pl->coop, BLML_NT, pl->lds_pipe, st>>>(
+                            dev_plan<float>(*pl), k, q, (const float *)E, (float *)U);
+                    else
+                        matvec_pipe_kernel<double><<<pl->coop, BLML_NT, pl->lds_pipe, st>>>(
+                            dev_plan<double>(*pl), k, q, (const double *)E, (double *)U);
+                    HIPCHECK(hipGetLastError());
+                }
+            return BLML_OK;
+        }
         if (pl->dtype == BLML_F32) {
             DevPlan<float> dp = dev_plan<float>(*pl);
             void *args[] = {&dp, &pl->d_bar, &pl->d_part, &E, &U};
-            return launch_coop(*pl, pipe ? (const void *)matvec_pipe_kernel<float>
-                                         : (const void *)matvec_grid_kernel<float>, args,
-                               pipe ? pl->lds_pipe : pl->lds_mv, st);
+            return launch_coop(*pl, (const void *)matvec_grid_kernel<float>, args, pl->lds_mv, st);
         }
         DevPlan<double> dp = dev_plan<double>(*pl);
         void *args[] = {&dp, &pl->d_bar, &pl->d_part, &E, &U};
-        return launch_coop(*pl, pipe ? (const void *)matvec_pipe_kernel<double>
-                                     : (const void *)matvec_grid_kernel<double>, args,
-                           pipe ? pl->lds_pipe : pl->lds_mv, st);
+        return launch_coop(*pl, (const void *)matvec_grid_kernel<double>, args, pl->lds_mv, st);
     }
     if (pl->dtype == BLML_F32) {
         DevPlan<float> dp = dev_plan<float>(*pl);

@@ -152,13 +152,13 @@ __device__ void run_matvec(const DevPlan<T> &p, int fi, const Team &tm, const T 
 
 // One whole fit (start, solve, outputs), run by a team.
 // The stand-alone mat-vec of a separable spread plan whose every pass is a pipelined whole-tile pass (the host
-// checks it): passes of a mode separated by grid barriers, modes too.  FP32 plans with the centrosymmetric tables
-// run every axis as sep_axis_centro.
+// checks it), one pass at a time: pass q of mode k over the tiles rank, rank + nranks, ...  Each pass stages every
+// axis's table once (after the largest tile of any pass; the host sized the LDS), while its first tile loads.  FP32
+// plans with the centrosymmetric tables run every axis as sep_axis_centro.
 template <typename T, bool CENTRO>
-__device__ void run_matvec_pipe(const DevPlan<T> &p, const Team &tm, const T *E, T *U) {
+__device__ void run_matvec_pipe_pass(const DevPlan<T> &p, int rank, int nranks, int k, int q, const T *E, T *U) {
     const blml_fit_t *fp = p.fits;                     // read in place: a local copy indexed by k would be scratch
     const int nc = fp->complex_g ? 2 : 1;
-    const int K = fp->K;
     const long B = fp->B;
     const SepPlan *sp = p.sep;
     const T *tabs = CENTRO ? p.sep_tab_c : p.sep_tab;
@@ -166,47 +166,49 @@ __device__ void run_matvec_pipe(const DevPlan<T> &p, const Team &tm, const T *E,
     int shape[BLML_MAXD];
 #pragma unroll
     for (int a = 0; a < BLML_MAXD; ++a) shape[a] = fp->shape[a];
-    for (int k = 0; k < K; ++k) {
-        const long off = fp->out_off + (long)k * B * nc;
-        const int fb = fp->fac_begin[k], fcnt = fp->fac_count[k];
-        int fac_of[BLML_MAXD];                          // indexed by constants only
+    const long off = fp->out_off + (long)k * B * nc;
+    const int fb = fp->fac_begin[k], fcnt = fp->fac_count[k];
+    int fac_of[BLML_MAXD];                              // indexed by constants only
 #pragma unroll
-        for (int a = 0; a < BLML_MAXD; ++a) {
-            int fa = -1;
-            for (int j = 0; j < fcnt; ++j)
-                if (p.facs[fb + j].axes[0] == a) fa = fb + j;
-            fac_of[a] = fa;
-        }
-        // every axis's table is staged once per mode, after the largest tile of any pass (the host sized the LDS)
-        const int np = sp->npass;
-        int o = 0;
-        for (int q = 0; q < np; ++q) {
-            const int t = sp->pass[q].OB * sp->pass[q].R * sp->pass[q].w;
-            o = max(o, sep_lds(t) + 1 + 64);
-        }
-        constexpr int BLK = SepMfma<T>::BLK;
-        int aoff[BLML_MAXD];
+    for (int a = 0; a < BLML_MAXD; ++a) {
+        int fa = -1;
+        for (int j = 0; j < fcnt; ++j)
+            if (p.facs[fb + j].axes[0] == a) fa = fb + j;
+        fac_of[a] = fa;
+    }
+    const int np = sp->npass;
+    int o = 0;
+    for (int r = 0; r < np; ++r) {
+        const int t = sp->pass[r].OB * sp->pass[r].R * sp->pass[r].w;
+        o = max(o, sep_lds(t) + 1 + 64);
+    }
+    constexpr int BLK = SepMfma<T>::BLK;
+    int aoff[BLML_MAXD];
 #pragma unroll
-        for (int a = 0; a < BLML_MAXD; ++a) {
-            aoff[a] = -1;
-            if (fac_of[a] >= 0 && shape[a] > 1) {
-                aoff[a] = o;
-                o += BLK * (BLK + 1);
-            }
-        }
-        constexpr int NV = sizeof(T) == 4 ? 64 : 32;          // 128 KB tiles
-        for (int q = 0; q < np; ++q) {
-            const SepPass ps = sp->pass[q];
-            const T *src = q == 0 ? E + off : Z;
-            T *dst = q == np - 1 ? U + off : Z;
-            if (q == 0)
-                sep_pass_pipe<T, NV, CENTRO>(ps, shape, fac_of, aoff, src, dst, tm.rank, tm.nranks,
-                                             SepStageAll<T>{fac_of, tabs, p.sep_off, aoff});
-            else
-                sep_pass_pipe<T, NV, CENTRO>(ps, shape, fac_of, aoff, src, dst, tm.rank, tm.nranks, SepNoStage{});
-            if (q < np - 1 || k < K - 1) tm.sync();
+    for (int a = 0; a < BLML_MAXD; ++a) {
+        aoff[a] = -1;
+        if (fac_of[a] >= 0 && shape[a] > 1) {
+            aoff[a] = o;
+            o += BLK * (BLK + 1);
         }
     }
+    constexpr int NV = sizeof(T) == 4 ? 64 : 32;          // 128 KB tiles
+    const SepPass ps = sp->pass[q];
+    const T *src = q == 0 ? E + off : Z;
+    T *dst = q == np - 1 ? U + off : Z;
+    sep_pass_pipe<T, NV, CENTRO>(ps, shape, fac_of, aoff, src, dst, rank, nranks,
+                                 SepStageAll<T>{fac_of, tabs, p.sep_off, aoff});
+}
+
+// The same mat-vec by one team: every pass of every mode in order, the team synchronised between them.
+template <typename T, bool CENTRO>
+__device__ void run_matvec_pipe(const DevPlan<T> &p, const Team &tm, const T *E, T *U) {
+    const int K = p.fits->K, np = p.sep->npass;
+    for (int k = 0; k < K; ++k)
+        for (int q = 0; q < np; ++q) {
+            run_matvec_pipe_pass<T, CENTRO>(p, tm.rank, tm.nranks, k, q, E, U);
+            if (q < np - 1 || k < K - 1) tm.sync();
+        }
 }
 
 template <typename T>

@@ -53,6 +53,7 @@ struct Plan {
     long *d_sep_off = nullptr;
     int lds_mv = 0, lds_fit = 0;   // dynamic LDS of the spread mat-vec and fit launches
     int lds_pipe = 0;              // dynamic LDS of matvec_pipe_kernel (largest tile, then every axis's table)
+    int sep_npass = 0;             // passes of the stand-alone mat-vec's separable plan (one launch each)
     int res_off = 0;               // LDS-resident intermediates of the batched kernels: byte offset (after the lines)
     int res_mv = 0, res_fit = 0;   // ... and the resident bytes matvec_kernel / fit_kernel launch with (0: none)
     int res_need_mv = 0, res_need_fit = 0;   // the resident bytes the plan's fits allow (the autotuner's options)
@@ -474,6 +475,7 @@ bool sep_plan_host(Plan &pl, const double *fdata, const int32_t *freqs, int max_
     const long lds_pipe = (pipe_vals + all_tabs + 2) * (long)es;
     pl.lds_pipe = (int)std::min<long>(lds_pipe, 1L << 30);
     pl.pipe_mv = st_pipe <= st_mv && lds_pipe + st_pipe <= max_lds && sep_pipe_host<T>(sp[0], f.shape);
+    pl.sep_npass = sp[0].npass;
     // FP32 pipelined plans: the centrosymmetric tables after the dense ones, at the same offsets (every axis the
     // pipelined passes act on has 32 points; tables of other factors are never staged there and stay zero)
     pl.sep_centro_off = -1;
