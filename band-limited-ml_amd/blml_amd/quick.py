@@ -40,8 +40,11 @@ columns, its marginal.  sqrt_density / predict evaluate the exact continuous ker
 its starts first (complex_starts: hybrid input-output runs on the FFT grid of the bin box, every run one workgroup
 of blml_complex_starts, blml_trivial.py:1605-1676) and keeps the fit of the largest likelihood (starts_).  Not
 built on this engine (they raise NotImplementedError): warm starts of complex / rank-K fits (the reference's branch
-for them, blml_gpu.py:196-219, builds FP64 arrays from the complex (B, K) rows of e_hat_ and cannot run either),
-and points not already binned (modes without h)."""
+for them, blml_gpu.py:196-219, builds FP64 arrays from the complex (B, K) rows of e_hat_ and cannot run either).
+
+Points (modes without h, blml_gpu.py:128-176): binned on the fit's own lattice on the host (bin_points: target
+widths bin_scale n^-1/4 / fc, a folded period tiled by whole bins, the lattice kept for a refit whose targets moved
+less than (1 + rebin_tol)^(1/4), the signs resolved per bin by majority), then fitted as counts on it."""
 import ctypes
 import os
 import warnings
@@ -84,6 +87,142 @@ def strip_lattice(modes):
     return [{key: {a: v for a, v in c.items() if a != 'h'} for key, c in m.items()} for m in modes]
 
 
+def with_lattice(modes, h):
+    """The modes on the fixed lattice h (one step per column; a multi-column unit takes one step for all of its
+    columns, blml_trivial.py:1377-1389)."""
+    out = []
+    for m in modes:
+        mm = {}
+        for key, c in m.items():
+            cols = key if isinstance(key, tuple) else (key,)
+            hv = np.broadcast_to(np.asarray(h, float), (max(max(cols) + 1, np.size(h)),))
+            if isinstance(key, tuple) and len({float(hv[j]) for j in cols}) > 1:
+                raise ValueError(f"columns {key}: one unit, one lattice step")
+            mm[key] = dict(c, h=float(hv[cols[0]]))
+        out.append(mm)
+    return out
+
+
+def _kind_name(kind):
+    return str(kind).partition(':')[0]
+
+
+def lattice_fold(specs):
+    """Per column, whether the shared lattice folds it onto its period (blml_trivial.py:1326-1358): a periodic column
+    (dirichlet, or a periodic ball's) whose every mode is periodic there with one period and one kind; any other
+    column is binned raw."""
+    d = len(specs[0]['kinds'])
+    out = []
+    for j in range(d):
+        per = [_kind_name(sp['kinds'][j]) in ('dirichlet', 'pball') for sp in specs]
+        if not any(per):
+            out.append(True)
+            continue
+        out.append(all(per) and len({float(sp['periods'][j]) for sp in specs}) == 1
+                   and len({_kind_name(sp['kinds'][j]) for sp in specs}) == 1)
+    return out
+
+
+def lattice_steps(kinds, periods, fold, h_target):
+    """The lattice step of each column for the target widths (ProductKernel.steps, blml_kernels.py:55-66): a
+    folded periodic column (dirichlet, or a periodic ball's column) tiles its period P with P / max(8,
+    ceil(P / h - 1e-9)); every other column takes the target itself."""
+    h_t = np.asarray(h_target, float)
+    if not np.all(h_t > 0):
+        raise ValueError("bin width h must be positive")
+    h = np.empty(len(kinds))
+    for k, kind in enumerate(kinds):
+        if _kind_name(kind) in ('dirichlet', 'pball') and fold[k]:
+            P = float(periods[k])
+            h[k] = P / max(8, int(np.ceil(P / h_t[k] - 1e-9)))
+        else:
+            h[k] = h_t[k]
+    return h
+
+
+def quantize(X, kinds, periods, fold, h):
+    """Every point snapped to its bin centre (ProductKernel.quantize, blml_kernels.py:74-82): round(x / h) h, and on
+    a folded periodic column mod(round(mod(x, P) / h) h, P); -0.0 becomes +0.0 (bins are unique by their bytes)."""
+    X = np.atleast_2d(np.asarray(X, float))
+    Xq = np.empty_like(X)
+    for k, kind in enumerate(kinds):
+        hk = float(h[k])
+        if _kind_name(kind) in ('dirichlet', 'pball') and fold[k]:
+            P = float(periods[k])
+            Xq[:, k] = np.mod(np.round(np.mod(X[:, k], P) / hk) * hk, P)
+        else:
+            Xq[:, k] = np.round(X[:, k] / hk) * hk
+    return Xq + 0.0
+
+
+def resolve_signs(signs, Zc, inverse=None, n_samples=None):
+    """+-1 per bin centre, or None (blml_trivial.py:1137-1164): a callable of the centres, one value per centre, or,
+    with inverse (sample -> centre), one value per sample reduced to the majority sign of each centre."""
+    if signs is None:
+        return None
+    B = len(Zc)
+    if callable(signs):
+        v = np.asarray(signs(Zc), float).reshape(-1)
+        if len(v) != B:
+            raise ValueError("signs(Z) must return one value per centre")
+    else:
+        v = np.asarray(signs, float).reshape(-1)
+        if inverse is not None and n_samples is not None and len(v) == n_samples and len(v) != B:
+            sv = np.where(v < 0, -1.0, 1.0)
+            tot = np.bincount(inverse, weights=sv, minlength=B)
+            cnt = np.bincount(inverse, minlength=B)
+            mixed = int((np.abs(tot) < cnt).sum())
+            if mixed:
+                warnings.warn(f"signs disagree within {mixed} of {B} bins; using the majority sign")
+            v = tot
+        elif len(v) != B:
+            raise ValueError(f"signs has length {len(v)}; expected {B} (centres)"
+                             + (f" or {n_samples} (samples)" if n_samples is not None else ""))
+    return np.where(v < 0, -1.0, 1.0)
+
+
+def bin_points(X, specs, sample_weight=None, signs=None, init=None, rebin_tol=1.0, bin_scale=0.1):
+    """Points to counts on the fit's lattice (blml_gpu.py:128-176): target widths bin_scale n^-1/4 / fc (the finest
+    fc over the modes per column; a radial unit's columns share their finest width, blml_trivial.py:1333-1341), the
+    lattice kept from init while every target stays within (1 + rebin_tol)^(1/4) of init's (blml_trivial.py:1403-1410),
+    the columns' steps and folds, the points snapped to bin centres and merged (unique centres by their bytes, counts
+    or summed weights), the signs resolved per centre.  Returns dict(Z, w, h, signs, h_target, n_lattice, rebinned,
+    fc_lattice)."""
+    from .modes import units_of
+    X = np.atleast_2d(np.asarray(X, np.float64))
+    n_rows, d = X.shape
+    sw = None if sample_weight is None else np.asarray(sample_weight, np.float64)
+    if sw is not None and (sw.shape != (n_rows,) or not np.all(sw > 0)):
+        raise ValueError("sample_weight: (n,) positive weights")
+    n = float(n_rows) if sw is None else float(sw.sum())
+    shared = all(_same_spec(q, specs[0]) for q in specs)
+    fc_lat = np.asarray(specs[0]['fc'], float) if shared else np.max([np.asarray(q['fc'], float) for q in specs], 0)
+    h_t = bin_scale * n ** -0.25 / fc_lat
+    for q in specs:
+        for name, dims in units_of(q['kinds']):
+            if len(dims) > 1:
+                h_t[list(dims)] = h_t[list(dims)].min()
+    h0 = getattr(init, 'h_target_', None) if init is not None else None
+    keep = (h0 is not None and np.shape(h0) == np.shape(h_t)
+            and bool(np.all(np.abs(np.log(h_t / np.asarray(h0))) <= 0.25 * np.log1p(rebin_tol) + 1e-12)))
+    rebin = init is not None and not keep
+    h_target, n_lattice, h_use = h_t, n, h_t
+    if init is not None and not rebin:
+        h_target, n_lattice, h_use = np.asarray(init.h_target_), init.n_lattice_, np.asarray(init._h_vec, float)
+    kinds, periods = specs[0]['kinds'], specs[0]['periods']
+    fold = lattice_fold(specs)
+    h = lattice_steps(kinds, periods, fold, h_use)
+    Xq = np.ascontiguousarray(quantize(X, kinds, periods, fold, h))
+    vdt = np.dtype((np.void, Xq.dtype.itemsize * d))
+    _, idx, inv, cnt = np.unique(Xq.view(vdt), return_index=True, return_inverse=True, return_counts=True)
+    inv = np.ravel(inv)
+    Z = Xq[idx]
+    w = cnt.astype(float) if sw is None else np.bincount(inv, weights=sw, minlength=len(idx))
+    sg = resolve_signs(signs, Z, inverse=inv, n_samples=n_rows)
+    return dict(Z=Z, w=w, h=h, signs=sg, h_target=np.asarray(h_target, float), n_lattice=n_lattice,
+                rebinned=rebin, fc_lattice=fc_lat)
+
+
 class BLMLQuick:
     """The reference's BLMLQuick(engine='gpu') constructor (blml_gpu.py:41-58) and fit signature (:61)."""
 
@@ -106,14 +245,25 @@ class BLMLQuick:
         d = Z.shape[1]
         h = modes_lattice(modes, d)
         if h is None:
-            raise NotImplementedError("BLMLQuick(engine='hip'): points are fitted on counts on a fixed lattice "
-                                      "only (give h in the modes and the counts as sample_weight)")
+            # points: binned on the fit's lattice first (blml_gpu.py:128-176), then the counts fit on it
+            specs, _ = parse_modes(modes, d, allow_dep=True)
+            bp = bin_points(Z, specs, sample_weight, signs, init, rebin_tol, self.bin_scale)
+            self._fit_counts(bp['Z'], bp['w'], bp['h'], modes, init, bp['signs'], complex_g)
+            self.h_target_, self.n_lattice_, self.fc_lattice_ = bp['h_target'], bp['n_lattice'], bp['fc_lattice']
+            self.rebinned_ = bp['rebinned']
+            return self
         w = np.ones(len(Z)) if sample_weight is None else np.asarray(sample_weight, float)
         if w.shape != (len(Z),) or not np.all(w >= 0):
             raise ValueError("sample_weight: (n,) non-negative counts")
-        specs, dep = parse_modes(strip_lattice(modes), d, allow_dep=True)
+        return self._fit_counts(Z, w, h, strip_lattice(modes), init, signs, complex_g)
+
+    def _fit_counts(self, Z, w, h, modes, init, signs, complex_g):
+        """The fit of counts w at bin centres Z on the lattice of steps h (d,); modes without h."""
+        d = Z.shape[1]
+        h = np.asarray(h, float)
+        specs, dep = parse_modes(modes, d, allow_dep=True)
         if dep is not None:
-            return self._fit_conditional(Z, w, modes, specs, dep, d, init, signs, complex_g)
+            return self._fit_conditional(Z, w, h, modes, specs, dep, d, init, signs, complex_g)
         psd = bool(complex_g) or len(specs) > 1
         if psd:
             if signs is not None:
@@ -179,9 +329,15 @@ class BLMLQuick:
         self.kinds_, self.periods_, self._fc_arr = sp['kinds'], sp['periods'], sp['fc']
         self._specs = specs
         self._h_vec, self._Zi, self._tab = h, Zi, tab
+        # the lattice a later refit of points keeps or leaves (blml_gpu.py:158-168): a fixed lattice is its own target
+        same = init is not None and getattr(init, 'h_target_', None) is not None and not self.rebinned_
+        self.h_target_ = np.asarray(init.h_target_, float) if same else np.asarray(h, float).copy()
+        self.n_lattice_ = init.n_lattice_ if same else n
+        self.fc_lattice_ = np.asarray(sp['fc'], float) if shared_spec(specs) else np.max(
+            [np.asarray(q['fc'], float) for q in specs], 0)
         return self
 
-    def _fit_conditional(self, Z, w, modes, specs, dep, d, init, signs, complex_g):
+    def _fit_conditional(self, Z, w, h, modes, specs, dep, d, init, signs, complex_g):
         """The joint fit on every column and the marginal fit of the others (blml_gpu.py:105-107, 197-217, 899-921):
         the marginal's counts are the joint's summed over the dep columns; its modes the joint's restricted to the
         other columns.  Sets the joint's attributes on self, dep_, _joint, _marg and, for real rank 1, e_marg_ on
@@ -206,11 +362,11 @@ class BLMLQuick:
 
         ij = init if getattr(init, '_joint', None) is None else init._joint
         im = init._marg if init is not None and getattr(init, 'dep_', None) == list(dep) else None
-        joint = make().fit(Z, strip_dep(modes), init=ij, complex_g=complex_g, sample_weight=w)
+        joint = make()._fit_counts(Z, w, h, strip_dep(modes), ij, None, complex_g)
         Zm, inv = np.unique(np.ascontiguousarray(Z[:, rest]), axis=0, return_inverse=True)
         inv = np.ravel(inv)
         wm = np.bincount(inv, weights=w, minlength=len(Zm))
-        marg = make().fit(Zm, marginal_modes(modes, rest), init=im, complex_g=complex_g, sample_weight=wm)
+        marg = make()._fit_counts(Zm, wm, h[rest], marginal_modes(modes, rest), im, None, complex_g)
         self.__dict__.update(joint.__dict__)
         self._joint, self._marg, self.dep_ = joint, marg, list(dep)
         if np.ndim(joint.e_hat_) == 1 and not np.iscomplexobj(joint.e_hat_):
@@ -800,6 +956,10 @@ def complex_starts(Zi, w, h, fc, kinds, n_hio=3, n_sb=2, hio_pool=30, sb_delta=1
                 g = gv[0::2] + 1j * gv[1::2]
             out.append((family, float(score[r]), w0 / np.conj(np.where(np.abs(g) == 0, 1e-12, g))))
     return out
+
+
+def shared_spec(specs):
+    return all(_same_spec(q, specs[0]) for q in specs)
 
 
 def _same_spec(a, b):

@@ -203,3 +203,41 @@ def test_conditional_fit_matches_oracle():
         BLMLQuick(tol=1e-6, dtype=np.float64).fit(Z.astype(float), [{j: dict(c, h=1.0) for j, c in m.items()}
                                                                     for m in make_modes([0.35, 0.3])],
                                                   sample_weight=w).predict_ratio(Zk[:3].astype(float))
+
+
+@pytest.mark.parametrize('case', [1, 4], ids=['2d sinc x dirichlet', '2d periodic ball'])
+def test_points_fit_matches_oracle_on_its_bins(case):
+    """BLMLQuick on points (no h in the modes): binned on the fit's own lattice as the reference bins them (pinned
+    on the CPU by test_host_units), then the counts fit.  Its e and loglik's stationarity against the oracle's
+    quick_fit / Newton solve with the dense Gram of the same bins (FP64 tables, tol 1e-8): e to 1e-6 of its scale;
+    a refit with 10% more points keeps the lattice, starts from the previous e and ends at the cold fit's solution."""
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), 'golden'))
+    from cases_binning import binning_cases
+    from blml_amd import make_modes
+    from blml_amd.modes import parse_modes
+    from blml_amd.quick import BLMLQuick, bin_points
+    name, X, modes, bs = binning_cases(make_modes)[case]
+    _note(f'quick on points: {name}')
+    q = BLMLQuick(tol=1e-8, bin_scale=bs, dtype=np.float64).fit(X, modes)
+    specs, _ = parse_modes(modes, 2)
+    bp = bin_points(X, specs, bin_scale=bs)
+    assert np.array_equal(q.Z_, bp['Z']) and np.array_equal(q.w_, bp['w']) and np.allclose(q._h_vec, bp['h'])
+    Zi = np.round(bp['Z'] / bp['h']).astype(np.int64)
+    sp = specs[0]
+    if case == 1:
+        units = [('offset', 'sinc', [sp['fc'][0]], [0.0], [0]), ('circular', 'dirichlet', [sp['fc'][1]], [10.0], [1])]
+    else:
+        units = [('torus', 'pball', [sp['fc'][0], sp['fc'][1]], [12.0, 9.0], [0, 1])]
+    G = OP.dense_pair_gram(units, bp['h'], Zi, Zi)
+    en = OQ.newton_solve(G, bp['w'])
+    assert q.converged_
+    assert np.abs(q.e_hat_ - en).max() < 1e-6 * np.abs(en).max()
+    more = np.concatenate([X, X[:len(X) // 10] + 0.3 * bp['h']])
+    warm = BLMLQuick(tol=1e-8, bin_scale=bs, dtype=np.float64).fit(more, modes, init=q)
+    # the cold fit of the same counts on the kept lattice (a fresh fit of the points would pick its own lattice)
+    cold = BLMLQuick(tol=1e-8, bin_scale=bs, dtype=np.float64)._fit_counts(warm.Z_, warm.w_, warm._h_vec, modes,
+                                                                            None, None, False)
+    _note(f'   refit with 10% more points: {warm.n_sweeps_} sweeps warm, {cold.n_sweeps_} cold')
+    assert not warm.rebinned_ and np.allclose(warm._h_vec, q._h_vec) and warm.converged_
+    assert np.abs(warm.e_hat_ - cold.e_hat_).max() < 1e-6 * np.abs(cold.e_hat_).max()

@@ -250,3 +250,57 @@ def test_pair_spans_reject_ranks_outside_the_table():
         B[0, 0] = bad
         with pytest.raises(RuntimeError, match='rank outside'):
             local._check_spans(A, B, tab)
+
+
+def test_points_are_binned_as_the_reference_bins_them():
+    """BLMLQuick on points bins them on the fit's own lattice first (blml_amd.quick.bin_points, blml_gpu.py:128-176):
+    the target widths, the steps (a folded period tiled by whole bins), the bin centres and their counts equal the
+    reference's own binning (tests/golden/binning.npz, written by make_golden_binning.py from the reference's
+    lattice_target, lattice_fold_specs and ProductKernel.quantize), every point lands in its nearest bin (circular
+    distance on a folded column), and a refit whose targets moved less than (1 + rebin_tol)^(1/4) keeps the lattice."""
+    import json
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), 'golden'))
+    from cases_binning import binning_cases
+    from blml_amd import make_modes
+    from blml_amd.modes import parse_modes
+    from blml_amd.quick import bin_points, quantize, lattice_fold
+    z = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'binning.npz'))
+    meta = json.loads(str(z['meta']))
+    for i, (name, X, modes, bs) in enumerate(binning_cases(make_modes)):
+        assert meta[i]['name'] == name
+        specs, _ = parse_modes(modes, X.shape[1])
+        bp = bin_points(X, specs, bin_scale=bs)
+        p = f'c{i}_'
+        assert np.allclose(bp['h'], z[p + 'h'], rtol=1e-14, atol=0), name
+        assert np.allclose(bp['h_target'], z[p + 'h_target'], rtol=1e-14, atol=0), name
+        assert np.array_equal(bp['Z'], z[p + 'Z']) and np.array_equal(bp['w'], z[p + 'w']), name
+        assert bp['w'].sum() == len(X)
+        Xq = quantize(X, specs[0]['kinds'], specs[0]['periods'], lattice_fold(specs), bp['h'])
+        for k in range(X.shape[1]):
+            dist = np.abs(X[:, k] - Xq[:, k])
+            if str(specs[0]['kinds'][k]).partition(':')[0] in ('dirichlet', 'pball'):
+                P = specs[0]['periods'][k]
+                assert abs(P / bp['h'][k] - round(P / bp['h'][k])) < 1e-9 and P / bp['h'][k] >= 8
+                dist = np.minimum(np.mod(dist, P), P - np.mod(dist, P))
+            assert np.all(dist <= 0.5 * bp['h'][k] * (1 + 1e-9)), (name, k)
+
+    class Prev:
+        pass
+    name, X, modes, bs = binning_cases(make_modes)[1]
+    specs, _ = parse_modes(modes, 2)
+    first = bin_points(X, specs, bin_scale=bs)
+    prev = Prev()
+    prev.h_target_, prev.n_lattice_, prev._h_vec = first['h_target'], first['n_lattice'], first['h']
+    more = np.concatenate([X, X[:600] + 0.01])                       # 15% more points: targets move 3.6%
+    kept = bin_points(more, specs, init=prev, bin_scale=bs)
+    assert not kept['rebinned'] and np.array_equal(kept['h'], first['h'])
+    many = np.concatenate([X] * 3)                                    # 3x the points: 24% finer targets
+    moved = bin_points(many, specs, init=prev, bin_scale=bs)
+    assert moved['rebinned'] and np.all(moved['h'] < first['h'])
+    signs = np.where(X[:, 0] < 0, -1.0, 1.0)
+    sg = bin_points(X, specs, signs=signs, bin_scale=bs)['signs']
+    assert set(np.unique(sg)) <= {-1.0, 1.0}
+    with pytest.raises(ValueError, match='sample_weight'):
+        bin_points(X, specs, sample_weight=np.zeros(len(X)), bin_scale=bs)
