@@ -304,6 +304,12 @@ int create_t(Plan &pl, const double *fdata, int64_t n_fdata, const int32_t *freq
     HIPCHECK(hipFree(d_fac_m));
     HIPCHECK(hipFree(d_tmp_off));
     HIPCHECK(hipFree(d_tmp));
+    // FP32 plans sum the padded line factors of up to BLML_TOEP_MAX_DEFAULT bins directly (OP_TOEP) instead of by
+    // FFT; BLML_TOEP_MAX (environment) sets another bound (0: every padded line by FFT)
+    if (sizeof(T) == 4) {
+        const char *env = getenv("BLML_TOEP_MAX");
+        pl.toep_max = env ? atoi(env) : BLML_TOEP_MAX_DEFAULT;
+    }
     // LDS-resident intermediates of the batched kernels: after the line buffers, as much as the workgroup's LDS
     // leaves; the launch reserves the largest need among the fits that fit (the others stay in the workspace).
     // BLML_LDS_RESIDENT (environment) names the kernels that use it: 'm' matvec_kernel, 'f' fit_kernel (default

@@ -26,6 +26,7 @@ struct DevPlan {
     const long *sep_off;
     int res_off, res_cap;  // LDS-resident intermediates: byte offset in the dynamic LDS and its size (0: none)
     int fft_sched;         // the radix-stage order of the FFT passes (the plan's autotuned choice)
+    int toep_max;          // padded line factors up to this length are summed directly (OP_TOEP; 0: none)
 };
 
 // A fit's workspace (bytes from ws_off): [X, Y: scratch_len complex values of the transform type]
@@ -124,6 +125,8 @@ __device__ void make_matvec_ctx(const DevPlan<T> &p, const blml_fit_t &f, Matvec
     mv.facs = p.facs;
     mv.spec = p.spec;
     mv.freqs = p.freqs;
+    mv.fdata = p.fdata;
+    mv.toep_max = p.toep_max;
     mv.tw = p.tw;
     mv.Lmax = p.Lmax;
     mv.X = (cpx<T> *)(p.ws + f.ws_off);

@@ -58,6 +58,7 @@ struct Plan {
     int res_mv = 0, res_fit = 0;   // ... and the resident bytes matvec_kernel / fit_kernel launch with (0: none)
     int res_need_mv = 0, res_need_fit = 0;   // the resident bytes the plan's fits allow (the autotuner's options)
     int fft_sched = 0;             // the radix-stage order of the FFT passes (fft_line's sched; autotuned)
+    int toep_max = 0;              // FP32 plans: padded line factors up to this length summed directly (OP_TOEP)
     int64_t out_elems = 0, dens_elems = 0;
     int64_t ws_elems = 0;
     size_t esize = 4;
@@ -88,6 +89,7 @@ DevPlan<T> dev_plan(const Plan &pl) {
     p.sep_off = pl.d_sep_off;
     p.res_off = p.res_cap = 0;
     p.fft_sched = pl.fft_sched;
+    p.toep_max = sizeof(T) == 4 ? pl.toep_max : 0;
     return p;
 }
 
@@ -110,6 +112,7 @@ DevPlan<double> dev_plan64(const Plan &pl) {
     p.sep_off = nullptr;
     p.res_off = p.res_cap = 0;
     p.fft_sched = pl.fft_sched;
+    p.toep_max = 0;
     return p;
 }
 

@@ -26,9 +26,17 @@ enum {
     OP_DFT_FWD = 4,    // m -> nF
     OP_DFT_INV = 5,    // nF -> m (unscaled)
     OP_FFT_FULL = 6,   // L -> L forward (spectra of padded tables at plan creation)
+    OP_TOEP = 7,       // m -> m: the padded op's linear convolution summed directly (y_i = sum_j t_|i-j| x_j)
 };
 
 #define BLML_DFT_TEAM 16   // threads per line of the band-restricted DFT passes
+#define BLML_TOEP_R 8      // outputs per thread and inputs per step of the direct convolution (OP_TOEP)
+#ifndef BLML_TOEP_MAX_DEFAULT
+#define BLML_TOEP_MAX_DEFAULT 128   // FP32 plans: the longest padded line summed directly
+#endif
+
+// OP_TOEP's padded line: m rounded up to whole steps of BLML_TOEP_R
+__host__ __device__ inline int toep_len(int m) { return (m + BLML_TOEP_R - 1) / BLML_TOEP_R * BLML_TOEP_R; }
 
 // LDS layout of an FFT line: element i at i ^ ((i >> 4) & 15), a permutation within each block of 16.  A radix-16
 // stage writes with stride 16 (one LDS bank for the whole wavefront in plain order); the swizzle spreads those
@@ -52,10 +60,21 @@ __host__ __device__ inline bool is_fft_op(int op) {
 //   FFT ops: L / 16 threads per line (each holds 16 values: one radix-16 butterfly, or 16 / R radix-R ones), so up
 //            to BLML_NT / (L / 16) lines per round (L <= 16 BLML_NT); a line shorter than 16 is one thread's.
 //   DFT ops: 16 threads per line, up to BLML_NT / 16 lines per round.
+//   OP_TOEP: the rounds of OP_CONV_PAD at the same L (its lines per round), each line toep_len(m) values at an odd
+//            stride (b64 reads of consecutive lines spread over the banks); the shared table is the even row over
+//            offsets -(P - 1) .. P - 1 (P = toep_len(m)), 2P - 1 values of T in P complex elements.  Both fit in
+//            OP_CONV_PAD's LDS when m > BLML_TOEP_R (then P + 1 <= 2m - 1 <= L).
 __host__ __device__ inline PassShape pass_shape(int op, int m, int L, int nF, long nlines) {
     PassShape s{1, 1, 0, 0};
     int cap;
-    if (is_fft_op(op)) {
+    if (op == OP_TOEP) {
+        int t = L / 16;
+        if (t < 1) t = 1;
+        s.team = t;
+        cap = BLML_NT / t;
+        s.buf = toep_len(m) | 1;
+        s.roots = toep_len(m);
+    } else if (is_fft_op(op)) {
         int t = L / 16;
         if (t < 1) t = 1;
         s.team = t;
@@ -232,6 +251,7 @@ struct Pass {
     int outer, inner;     // lines = outer * inner; element (o, i, r) at (o * n * inner + i * inner + r)
     const int *F;         // circular: frequencies (0 <= F < m)
     const T *spec;        // CONV ops: spectrum values (real)
+    const double *row;    // OP_TOEP: the even row t_0 .. t_{m-1}
     int rank, nranks;     // the team member running this call: rounds rank, rank + nranks, ... of the lines
     bool pairs;           // CONV ops on real data: two lines per transform, as real and imaginary part (exact: the
                           // operator is real, so S (x + i y) = S x + i S y)
@@ -308,6 +328,68 @@ __device__ __forceinline__ void round_io(bool load, int inner, int nl, int nvali
     }
 }
 
+// OP_TOEP on one round of nl lines staged in LDS (line l at lines + l * stride, P = toep_len(m) values, zero past
+// m): y_i = sum_j t_{i-j} x_j for i < m, written straight to dst at lofs[l] + i * inner (with lofs2: the real and
+// imaginary parts as two real lines, like round_io's store).  t2[k] = t_{k-(P-1)}, the even row over offsets
+// -(P - 1) .. P - 1 (zero where |offset| >= m).  A thread takes BLML_TOEP_R consecutive outputs of one line and
+// steps through the inputs BLML_TOEP_R at a time: 2R - 1 table values and R inputs per R^2 products.  Consecutive
+// threads take consecutive lines at the same outputs, so the table reads are the same address across a wavefront.
+// The sums run in FP64 (the products of two FP32 values are exact there, and FP64 FMAs issue at the rate of
+// unpacked FP32 ones), so an FP32 plan's result is its FP32 inputs' convolution rounded once.  Every output of the
+// round is computed from LDS before this returns, so dst may be the round's own source.
+template <typename T>
+__device__ __noinline__ void toep_round(const cpx<T> *lines, int stride, const T *t2, int m, int nl, int inner,
+                                        const int *lofs, const int *lofs2, cpx<T> *dst) {
+    constexpr int R = BLML_TOEP_R;
+    const int P = toep_len(m);
+    const int nb = P / R;
+    for (int it = threadIdx.x; it < nl * nb; it += BLML_NT) {
+        const int b = it / nl, l = it - b * nl;
+        const int i0 = b * R;
+        const cpx<T> *x = lines + l * stride;
+        double ar[R], ai[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) ar[r] = ai[r] = 0.0;
+        for (int j0 = 0; j0 < P; j0 += R) {
+            const T *tp = t2 + (i0 - j0) + (P - 1) - (R - 1);    // tv[s] = t_{i0 - j0 - (R - 1) + s}
+            double tv[2 * R - 1];
+#pragma unroll
+            for (int s = 0; s < 2 * R - 1; ++s) tv[s] = (double)tp[s];
+            double xr[R], xi[R];
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const cpx<T> v = x[j0 + u];
+                xr[u] = (double)v.x;
+                xi[u] = (double)v.y;
+            }
+#pragma unroll
+            for (int u = 0; u < R; ++u)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const double c = tv[r - u + R - 1];             // t_{(i0 + r) - (j0 + u)}
+                    ar[r] = fma(c, xr[u], ar[r]);
+                    ai[r] = fma(c, xi[u], ai[r]);
+                }
+        }
+        const int lo = lofs[l];
+        if (lo >= 0) {
+            const int lo2 = lofs2 ? lofs2[l] : -1;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = i0 + r;
+                if (i < m) {
+                    if (lofs2) {
+                        dst[lo + (long)i * inner] = {(T)ar[r], (T)0};
+                        if (lo2 >= 0) dst[lo2 + (long)i * inner] = {(T)ai[r], (T)0};
+                    } else {
+                        dst[lo + (long)i * inner] = {(T)ar[r], (T)ai[r]};
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Applies one pass to every line.  src and dst may alias when n_in == n_out (a round's lines are staged in LDS
 // before any is written back, and lines are disjoint).  All threads execute the same sequence of barriers.
 template <typename T>
@@ -322,7 +404,14 @@ __device__ __noinline__ void run_pass(const Pass<T> &p, const cpx<T> *src, cpx<T
     cpx<T> *roots = lds;
     cpx<T> *lines = lds + sh.roots;
     cpx<T> *buf = lines + (has_line ? tm : 0) * sh.buf;
-    if (is_fft_op(p.op)) {
+    if (p.op == OP_TOEP) {
+        T *t2 = reinterpret_cast<T *>(roots);
+        const int P = toep_len(p.m);
+        for (int k = threadIdx.x; k < 2 * P - 1; k += BLML_NT) {
+            const int o = k < P - 1 ? P - 1 - k : k - (P - 1);
+            t2[k] = o < p.m ? (T)p.row[o] : (T)0;
+        }
+    } else if (is_fft_op(p.op)) {
         load_twiddles<T>(roots, p.L, tw, Lmax);
     } else {
         for (int j = threadIdx.x; j < p.m; j += BLML_NT) {
@@ -354,7 +443,13 @@ __device__ __noinline__ void run_pass(const Pass<T> &p, const cpx<T> *src, cpx<T
             }
         }
         __syncthreads();
-        if (is_fft_op(p.op)) {
+        if (p.op == OP_TOEP) {
+            const int P = toep_len(p.m);
+            round_io<T>(true, p.inner, nl, p.m, P, 0, lines, sh.buf, lofs_in, li2, src, nullptr);
+            __syncthreads();
+            toep_round<T>(lines, sh.buf, reinterpret_cast<const T *>(roots), p.m, nl, p.inner, lofs_out, lo2, dst);
+            __syncthreads();
+        } else if (is_fft_op(p.op)) {
             const int nin = (p.op == OP_FFT_FULL || p.op == OP_FFT_INV) ? p.L : p.m;
             round_io<T>(true, p.inner, nl, nin, p.L, 0, lines, sh.buf, lofs_in, li2, src, nullptr, true);
             __syncthreads();
@@ -434,6 +529,8 @@ struct MatvecCtx {
     bool sep_last_sync;        // team barrier after a separable mat-vec's last pass (false: the kernel ends there)
     const T *sep_tab;          // per factor: P, Q, c_mid, middle row
     const long *sep_off;
+    const double *fdata = nullptr;   // the plan's factor data (OP_TOEP reads a padded factor's row there)
+    int toep_max = 0;          // padded line factors with BLML_TOEP_R < m <= toep_max take OP_TOEP (0: none)
 };
 
 // U_k = S_k E_k of a separable mode by tile passes (see separable.hpp); the intermediate lives in c.X.
@@ -528,10 +625,11 @@ __device__ void apply_factor(const MatvecCtx<T> &c, const blml_factor_t &fa, boo
             p.F = c.freqs + fa.freq_off[0];
             p.L = 0;
         } else {
-            p.op = OP_CONV_PAD;
+            p.op = p.m > BLML_TOEP_R && p.m <= c.toep_max ? OP_TOEP : OP_CONV_PAD;
             p.L = fa.tlen[0];
             p.nF = 0;
             p.F = nullptr;
+            p.row = c.fdata + fa.data_off;
         }
         run_pass<T>(p, c.X, c.X, c.tw, c.Lmax);
         c.tm.sync();

@@ -140,6 +140,7 @@ int replay(In &in, const char *out_path) {
     pl.esize = sizeof(T);
     // the FFT passes' radix-stage order (the autotuner's knob; fft_line's sched)
     if (const char *sc = getenv("BLML_REPLAY_FFT_SCHED")) pl.fft_sched = atoi(sc);
+    if (const char *tm = getenv("BLML_REPLAY_TOEP")) pl.toep_max = atoi(tm);
     std::vector<int> fac_m;
     std::vector<long> tmp_off;
     long tmp_total = 0;
@@ -245,6 +246,28 @@ int replay(In &in, const char *out_path) {
             fprintf(stderr, "replay: the pipelined mat-vec replay takes FP32 plans\n");
             return 8;
         }
+    }
+    // BLML_REPLAY_MATVEC: matvec_kernel's work (each fit of a batched plan on one workgroup) on the input
+    // E_i = pipe_value(i) over the plan's outputs; writes U
+    if (getenv("BLML_REPLAY_MATVEC")) {
+        if (pl.spread) {
+            fprintf(stderr, "replay: the batched mat-vec replay takes batched plans\n");
+            return 8;
+        }
+        const long n = pl.out_elems;
+        T *E = dev_alloc<T>(n);
+        for (long i = 0; i < n; ++i) E[i] = (T)pipe_value(i);
+        T *U = dev_alloc<T>(n);
+        const DevPlan<T> dp = dev_plan<T>(pl);
+        for (int fi = 0; fi < pl.nfits; ++fi)
+            launch(pl.lds_bytes, kStaticMv, 1, [&] { run_matvec<T>(dp, fi, block_team(), E, U, nullptr); });
+        FILE *fo = fopen(out_path, "wb");
+        if (!fo) return 5;
+        fwrite(U, sizeof(T), (size_t)n, fo);
+        fclose(fo);
+        printf("replay: batched mat-vec of %ld values\n", n);
+        printf("replay: ok\n");
+        return 0;
     }
     // the alignment gaps of every workspace: [X Y | gap | pool | gap | (FP32: X64 Y64) | gap]
     unsigned char *ws = (unsigned char *)pl.d_ws;

@@ -99,7 +99,7 @@ def write_plan(batch, path, cert_iters=0, cert_tol=1e-6, ranks=4, emp=None):
             f.write(_bytes(np.ascontiguousarray(emp[1], np.float64).ravel()))
 
 
-def run(batch, cert_iters=0, cert_tol=1e-6, timeout=1800, exe=None, ranks=4, emp=None, fft_sched=0):
+def run(batch, cert_iters=0, cert_tol=1e-6, timeout=1800, exe=None, ranks=4, emp=None, fft_sched=0, toep_max=0):
     """Replays the batch's fits on the CPU; returns dict(results (structured array), e, density, lam, iters, log).
     Raises RuntimeError with the replay's output when it fails (an AddressSanitizer report included)."""
     from blml_amd import _lib
@@ -109,7 +109,7 @@ def run(batch, cert_iters=0, cert_tol=1e-6, timeout=1800, exe=None, ranks=4, emp
         pin, pout = os.path.join(tmp, 'plan.bin'), os.path.join(tmp, 'out.bin')
         write_plan(batch, pin, cert_iters, cert_tol, ranks, emp)
         env = dict(os.environ, ASAN_OPTIONS='detect_leaks=0:abort_on_error=0:detect_stack_use_after_return=0',
-                   BLML_REPLAY_FFT_SCHED=str(int(fft_sched)))
+                   BLML_REPLAY_FFT_SCHED=str(int(fft_sched)), BLML_REPLAY_TOEP=str(int(toep_max)))
         p = subprocess.run([exe, pin, pout], capture_output=True, text=True, timeout=timeout, env=env)
         log = p.stdout + p.stderr
         if p.returncode != 0:
@@ -164,6 +164,28 @@ def run_pipe(batch, ranks=4, timeout=1800, exe=None):
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return pipe_input(n), raw[:n], raw[n:2 * n]
+
+
+def run_matvec(batch, toep_max=0, timeout=1800, exe=None):
+    """Replays matvec_kernel's work (every fit on one workgroup) on pipe_input(n) over the plan's outputs: returns
+    (E, U).  toep_max: the plan's bound for the directly summed padded lines (FP32 plans)."""
+    exe = exe or build()
+    n = max(int(F.out_off) + fs.K * fs.B * fs.nc for F, fs in zip(batch._Fits, batch.fits))
+    es = np.float32 if batch.code == 0 else np.float64
+    tmp = tempfile.mkdtemp(prefix='blml_replay_')
+    try:
+        pin, pout = os.path.join(tmp, 'plan.bin'), os.path.join(tmp, 'out.bin')
+        write_plan(batch, pin, 0, 1e-6, 1, None)
+        env = dict(os.environ, ASAN_OPTIONS='detect_leaks=0:abort_on_error=0:detect_stack_use_after_return=0',
+                   BLML_REPLAY_MATVEC='1', BLML_REPLAY_TOEP=str(int(toep_max)))
+        p = subprocess.run([exe, pin, pout], capture_output=True, text=True, timeout=timeout, env=env)
+        log = p.stdout + p.stderr
+        if p.returncode != 0:
+            raise RuntimeError(f"replay exited with {p.returncode}:\n{log[-6000:]}")
+        raw = np.fromfile(pout, es)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return pipe_input(n).astype(es), raw[:n]
 
 
 def run_hio(shape, box, n_hio, n_er, beta, ab2, band, amp, occ, wocc, phase, kick, timeout=1800, exe=None):

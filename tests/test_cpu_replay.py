@@ -220,3 +220,49 @@ def test_start_search_runs_match_oracle(replay):
         assert np.abs(P[r] - Po.ravel()).max() < 1e-9 * np.abs(Po).max(), (r, fam)
         assert score[r] == pytest.approx(so, rel=1e-10)
         assert np.abs(g[r] - np.fft.ifftn(Po)[G['idx']]).max() < 1e-9 * np.abs(np.fft.ifftn(Po)).max()
+
+
+@pytest.mark.parametrize('cplx', [True, False], ids=['complex', 'real'])
+def test_direct_line_sums_match_the_oracle_matvec(replay, cplx):
+    """The batched mat-vec of an FP32 plan whose padded line factors are summed directly (OP_TOEP: lines of 37 and
+    9 values, strided and contiguous, real lines paired into complex ones, steps past the line's end; the 70-value
+    axis left to the FFT by the length bound) and of the same plan by FFT only: both equal the oracle's FP64
+    mat-vec of every mode to FP32 rounding (1e-5 of max |S e|), with every access in bounds."""
+    import blml_amd
+    shape = (37, 9, 70)
+    counts = _field_counts(np.random.default_rng(8), shape, 30000)
+    fc1, kinds1 = [0.35, 0.3, 0.3], ['slits:0.5', 'sinc', 'sinc']
+    fc2, per = [3 / 37, 3 / 9, 3 / 70], [37.0, 9.0, 70.0]
+    modes = blml_amd.make_modes(fc1, kinds1) + blml_amd.make_modes(fc2, ['dirichlet'] * 3, periods=per)
+    batch = _batch(counts, modes, cplx, np.float32, -1, PARAMS)
+    E, Uf = replay.run_matvec(batch, toep_max=0)
+    _, Ud = replay.run_matvec(batch, toep_max=64)
+    om = O.BLMLBinnedOracle()
+    om._setup(counts, O.make_modes(fc1, kinds1) + O.make_modes(fc2, ['dirichlet'] * 3, periods=per), 1.0, 0.0)
+    B, nc = counts.size, 2 if cplx else 1
+    for k in range(2):
+        e = E[k * B * nc:(k + 1) * B * nc].astype(np.float64).reshape(B, nc)
+        want = np.stack([om._S(k, e[:, c].reshape(shape)).ravel() for c in range(nc)], 1).ravel()
+        scale = np.max(np.abs(want))
+        for U in (Uf, Ud):
+            got = U[k * B * nc:(k + 1) * B * nc].astype(np.float64)
+            assert np.max(np.abs(got - want)) < 1e-5 * scale
+
+
+@pytest.mark.parametrize('cplx', [True, False], ids=['complex', 'real'])
+def test_direct_line_sums_in_bounds_through_a_fit(replay, cplx):
+    """A whole FP32 fit (start solve, 12 iterations, outputs, the FP64 finishing pass) with the padded lines of 37
+    and 9 values summed directly: every access in bounds, finite results, and the loglik of the FFT run's fit to
+    the FP32 iteration's own spread (the two differ in rounding only; 12 sweeps of FP32 CD amplify a change of one
+    unit in the last place to about 1e-4, as FP32 against FP64 does; the mat-vecs themselves are pinned above)."""
+    import blml_amd
+    rng = np.random.default_rng(8)
+    counts = _field_counts(rng, (37, 9, 70), 30000)
+    modes = (blml_amd.make_modes([0.35, 0.3, 0.3], ['slits:0.5', 'sinc', 'sinc'])
+             + blml_amd.make_modes([3 / 37, 3 / 9, 3 / 70], ['dirichlet'] * 3, periods=[37.0, 9.0, 70.0]))
+    params = dict(PARAMS, solver='lbfgs', max_sweeps=12, ll_tol=1e-10)
+    a = replay.run(_batch(counts, modes, cplx, np.float32, -1, params))
+    b = replay.run(_batch(counts, modes, cplx, np.float32, -1, params), toep_max=64)
+    la, lb = a['results']['loglik'][0], b['results']['loglik'][0]
+    assert np.isfinite(lb) and np.all(np.isfinite(b['density']))
+    assert abs(la - lb) < 1e-3 * abs(la)
