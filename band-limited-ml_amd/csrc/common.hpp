@@ -12,6 +12,11 @@ struct cpx {
     T x, y;
 };
 
+// two consecutive FP32 values read or written as one 8-byte access (the address must be 8-byte aligned)
+struct alignas(8) f32x2 {
+    float x, y;
+};
+
 // The launch's dynamic LDS.  Functions that are not inlined reach LDS only through this declaration (and offsets
 // into it), never through pointer arguments: a pointer argument is generic, which turns every LDS access into a
 // FLAT instruction, while a pointer derived from a __shared__ object in the same function compiles to ds_ ops.

@@ -876,13 +876,32 @@ struct FitSolver {
                 Sp[i] = hvec(9 + sl_);
                 Yp[i] = hvec(9 + ph + sl_);
             }
-            for (long e = tm.first(); e < n_; e += tm.stride()) {
-                const double ge = gn[e];
+            // two consecutive elements per thread and step: 8-byte loads of the FP32 history (every history
+            // vector starts 8-byte aligned), so twice the bytes are in flight per load
+            const long n2 = (n_ + 1) / 2;
+            for (long e2 = tm.first(); e2 < n2; e2 += tm.stride()) {
+                const long e = 2 * e2;
+                if (e + 1 < n_) {
+                    const double g0 = gn[e], g1 = gn[e + 1];
 #pragma unroll
-                for (int i = 0; i < BLML_CCHUNK; ++i) {
-                    if (i < nc_) {
-                        acc[i] += (double)Sp[i][e] * ge;
-                        acc[BLML_CCHUNK + i] += (double)Yp[i][e] * ge;
+                    for (int i = 0; i < BLML_CCHUNK; ++i) {
+                        if (i < nc_) {
+                            const f32x2 sv = *reinterpret_cast<const f32x2 *>(Sp[i] + e);
+                            const f32x2 yv = *reinterpret_cast<const f32x2 *>(Yp[i] + e);
+                            acc[i] += (double)sv.x * g0;
+                            acc[i] += (double)sv.y * g1;
+                            acc[BLML_CCHUNK + i] += (double)yv.x * g0;
+                            acc[BLML_CCHUNK + i] += (double)yv.y * g1;
+                        }
+                    }
+                } else {
+                    const double ge = gn[e];
+#pragma unroll
+                    for (int i = 0; i < BLML_CCHUNK; ++i) {
+                        if (i < nc_) {
+                            acc[i] += (double)Sp[i][e] * ge;
+                            acc[BLML_CCHUNK + i] += (double)Yp[i][e] * ge;
+                        }
                     }
                 }
             }
@@ -946,17 +965,19 @@ struct FitSolver {
                     for (int i = 0; i < cnt; ++i) g_sl.cp2[i] = -gam * u[i];
                 }
                 __syncthreads();
-                // q = gam g + sum_i (p1_i s_i + p2_i y_i), slope = g . q: UE elements per thread and the pairs in
-                // chunks of BLML_CCHUNK, so 2 BLML_CCHUNK UE loads are in flight (the additions in pair order)
+                // q = gam g + sum_i (p1_i s_i + p2_i y_i), slope = g . q: UE consecutive elements per thread (one
+                // 8-byte load per history vector) and the pairs in chunks of BLML_CCHUNK, so 2 BLML_CCHUNK 8-byte
+                // loads are in flight (the additions in pair order)
                 constexpr int UE = 2;
                 double acc = 0.0;
                 const double *g = vec(ig);
                 const long stp = tm.stride();
-                for (long e0 = tm.first(); e0 < n_; e0 += UE * stp) {
+                for (long e0 = UE * tm.first(); e0 < n_; e0 += UE * stp) {
+                    const bool whole = e0 + 1 < n_;
                     double v[UE], gv[UE];
 #pragma unroll
                     for (int u = 0; u < UE; ++u) {
-                        const long e = e0 + u * stp;
+                        const long e = e0 + u;
                         gv[u] = e < n_ ? g[e] : 0.0;
                         v[u] = gam * gv[u];
                     }
@@ -973,19 +994,24 @@ struct FitSolver {
                             a1[i] = g_sl.cp1[c0 + (i < nc_ ? i : 0)];
                             a2[i] = g_sl.cp2[c0 + (i < nc_ ? i : 0)];
                         }
+                        if (whole) {
 #pragma unroll
-                        for (int u = 0; u < UE; ++u) {
-                            const long e = e0 + u * stp;
-                            if (e < n_) {
+                            for (int i = 0; i < BLML_CCHUNK; ++i)
+                                if (i < nc_) {
+                                    const f32x2 sv = *reinterpret_cast<const f32x2 *>(Sp[i] + e0);
+                                    const f32x2 yv = *reinterpret_cast<const f32x2 *>(Yp[i] + e0);
+                                    v[0] += a1[i] * (double)sv.x + a2[i] * (double)yv.x;
+                                    v[1] += a1[i] * (double)sv.y + a2[i] * (double)yv.y;
+                                }
+                        } else {
 #pragma unroll
-                                for (int i = 0; i < BLML_CCHUNK; ++i)
-                                    if (i < nc_) v[u] += a1[i] * (double)Sp[i][e] + a2[i] * (double)Yp[i][e];
-                            }
+                            for (int i = 0; i < BLML_CCHUNK; ++i)
+                                if (i < nc_) v[0] += a1[i] * (double)Sp[i][e0] + a2[i] * (double)Yp[i][e0];
                         }
                     }
 #pragma unroll
                     for (int u = 0; u < UE; ++u) {
-                        const long e = e0 + u * stp;
+                        const long e = e0 + u;
                         if (e < n_) {
                             q[e] = v[u];
                             acc += gv[u] * v[u];
