@@ -12,10 +12,19 @@ struct cpx {
     T x, y;
 };
 
-// two consecutive FP32 values read or written as one 8-byte access (the address must be 8-byte aligned)
+// two / four consecutive FP32 values read as one 8- / 16-byte access (the address aligned to the access's size)
 struct alignas(8) f32x2 {
-    float x, y;
+    float v[2];
 };
+struct alignas(16) f32x4 {
+    float v[4];
+};
+template <int W>
+struct f32xw;
+template <>
+struct f32xw<2> { typedef f32x2 type; };
+template <>
+struct f32xw<4> { typedef f32x4 type; };
 
 // The launch's dynamic LDS.  Functions that are not inlined reach LDS only through this declaration (and offsets
 // into it), never through pointer arguments: a pointer argument is generic, which turns every LDS access into a

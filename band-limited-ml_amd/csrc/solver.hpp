@@ -860,6 +860,105 @@ struct FitSolver {
     //   u = R^-1 a,  p1 = R^-T ((D + gamma Y^T Y) u - gamma b),  p2 = -u,  H g = gamma g + S p1 + gamma Y p2.
     // One pass over the history per iteration produces S^T g_new and Y^T g_new, from which the new column of R
     // (s_i . y = s_i . g - s_i . g_new) and of Y^T Y follow; a second pass forms the direction.
+    template <int W>
+    __device__ __forceinline__ void dots_elems(const float *const *Sp, const float *const *Yp, int nc_,
+                                               const double *gn, double *acc) {
+        typedef typename f32xw<W>::type vw;
+        const long n_ = V();
+        const long nw = (n_ + W - 1) / W;
+        for (long ew = tm.first(); ew < nw; ew += tm.stride()) {
+            const long e = W * ew;
+            if (e + W <= n_) {
+                double gv[W];
+#pragma unroll
+                for (int u = 0; u < W; ++u) gv[u] = gn[e + u];
+#pragma unroll
+                for (int i = 0; i < BLML_CCHUNK; ++i) {
+                    if (i < nc_) {
+                        const vw sv = *reinterpret_cast<const vw *>(Sp[i] + e);
+                        const vw yv = *reinterpret_cast<const vw *>(Yp[i] + e);
+#pragma unroll
+                        for (int u = 0; u < W; ++u) {
+                            acc[i] += (double)sv.v[u] * gv[u];
+                            acc[BLML_CCHUNK + i] += (double)yv.v[u] * gv[u];
+                        }
+                    }
+                }
+            } else {
+                for (long ee = e; ee < n_; ++ee) {
+                    const double ge = gn[ee];
+#pragma unroll
+                    for (int i = 0; i < BLML_CCHUNK; ++i) {
+                        if (i < nc_) {
+                            acc[i] += (double)Sp[i][ee] * ge;
+                            acc[BLML_CCHUNK + i] += (double)Yp[i][ee] * ge;
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // this thread's share of the compact direction: q[e] = gam g[e] + sum_i (p1_i s_i[e] + p2_i y_i[e]) for W
+    // consecutive elements per step (W-value loads of the FP32 history, its pairs in chunks of BLML_CCHUNK);
+    // returns this thread's part of g . q
+    template <int W>
+    __device__ __forceinline__ double dir_elems(int head, int ph, int cnt, double gam, const double *g, double *q) {
+        typedef typename f32xw<W>::type vw;
+        const long n_ = V();
+        const long stp = tm.stride();
+        double acc = 0.0;
+        for (long e0 = W * tm.first(); e0 < n_; e0 += W * stp) {
+            const bool whole = e0 + W <= n_;
+            double v[W], gv[W];
+#pragma unroll
+            for (int u = 0; u < W; ++u) {
+                gv[u] = e0 + u < n_ ? g[e0 + u] : 0.0;
+                v[u] = gam * gv[u];
+            }
+            for (int c0 = 0; c0 < cnt; c0 += BLML_CCHUNK) {
+                const int nc_ = cnt - c0 < BLML_CCHUNK ? cnt - c0 : BLML_CCHUNK;
+                const float *Sp[BLML_CCHUNK], *Yp[BLML_CCHUNK];
+                double a1[BLML_CCHUNK], a2[BLML_CCHUNK];
+#pragma unroll
+                for (int i = 0; i < BLML_CCHUNK; ++i) {
+                    int sl_ = head + c0 + (i < nc_ ? i : 0);
+                    if (sl_ >= ph) sl_ -= ph;
+                    Sp[i] = hvec(9 + sl_);
+                    Yp[i] = hvec(9 + ph + sl_);
+                    a1[i] = g_sl.cp1[c0 + (i < nc_ ? i : 0)];
+                    a2[i] = g_sl.cp2[c0 + (i < nc_ ? i : 0)];
+                }
+                if (whole) {
+#pragma unroll
+                    for (int i = 0; i < BLML_CCHUNK; ++i)
+                        if (i < nc_) {
+                            const vw sv = *reinterpret_cast<const vw *>(Sp[i] + e0);
+                            const vw yv = *reinterpret_cast<const vw *>(Yp[i] + e0);
+#pragma unroll
+                            for (int u = 0; u < W; ++u) v[u] += a1[i] * (double)sv.v[u] + a2[i] * (double)yv.v[u];
+                        }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < W; ++u)
+                        if (e0 + u < n_) {
+#pragma unroll
+                            for (int i = 0; i < BLML_CCHUNK; ++i)
+                                if (i < nc_) v[u] += a1[i] * (double)Sp[i][e0 + u] + a2[i] * (double)Yp[i][e0 + u];
+                        }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < W; ++u) {
+                if (e0 + u < n_) {
+                    q[e0 + u] = v[u];
+                    acc += gv[u] * v[u];
+                }
+            }
+        }
+        return acc;
+    }
+
     __device__ void dots_history(int head, int ph, int cnt, const double *gn) {
         // csg[i] = s_i . gn, cyg[i] = y_i . gn for the cnt pairs (logical order), in chunks of BLML_CCHUNK pairs
         const long n_ = V();
@@ -876,35 +975,10 @@ struct FitSolver {
                 Sp[i] = hvec(9 + sl_);
                 Yp[i] = hvec(9 + ph + sl_);
             }
-            // two consecutive elements per thread and step: 8-byte loads of the FP32 history (every history
-            // vector starts 8-byte aligned), so twice the bytes are in flight per load
-            const long n2 = (n_ + 1) / 2;
-            for (long e2 = tm.first(); e2 < n2; e2 += tm.stride()) {
-                const long e = 2 * e2;
-                if (e + 1 < n_) {
-                    const double g0 = gn[e], g1 = gn[e + 1];
-#pragma unroll
-                    for (int i = 0; i < BLML_CCHUNK; ++i) {
-                        if (i < nc_) {
-                            const f32x2 sv = *reinterpret_cast<const f32x2 *>(Sp[i] + e);
-                            const f32x2 yv = *reinterpret_cast<const f32x2 *>(Yp[i] + e);
-                            acc[i] += (double)sv.x * g0;
-                            acc[i] += (double)sv.y * g1;
-                            acc[BLML_CCHUNK + i] += (double)yv.x * g0;
-                            acc[BLML_CCHUNK + i] += (double)yv.y * g1;
-                        }
-                    }
-                } else {
-                    const double ge = gn[e];
-#pragma unroll
-                    for (int i = 0; i < BLML_CCHUNK; ++i) {
-                        if (i < nc_) {
-                            acc[i] += (double)Sp[i][e] * ge;
-                            acc[BLML_CCHUNK + i] += (double)Yp[i][e] * ge;
-                        }
-                    }
-                }
-            }
+            // W consecutive elements per thread and step, each history vector read by W-value loads: 16-byte
+            // loads when every history vector starts 16-byte aligned (an even slot length), else 8-byte ones
+            if ((Vmax & 1) == 0) dots_elems<4>(Sp, Yp, nc_, gn, acc);
+            else dots_elems<2>(Sp, Yp, nc_, gn, acc);
             tm.sum<2 * BLML_CCHUNK>(acc, g_sl.red);
             if (threadIdx.x == 0)
                 for (int i = 0; i < nc_; ++i) { g_sl.csg[c0 + i] = acc[i]; g_sl.cyg[c0 + i] = acc[BLML_CCHUNK + i]; }
@@ -965,59 +1039,10 @@ struct FitSolver {
                     for (int i = 0; i < cnt; ++i) g_sl.cp2[i] = -gam * u[i];
                 }
                 __syncthreads();
-                // q = gam g + sum_i (p1_i s_i + p2_i y_i), slope = g . q: UE consecutive elements per thread (one
-                // 8-byte load per history vector) and the pairs in chunks of BLML_CCHUNK, so 2 BLML_CCHUNK 8-byte
-                // loads are in flight (the additions in pair order)
-                constexpr int UE = 2;
-                double acc = 0.0;
-                const double *g = vec(ig);
-                const long stp = tm.stride();
-                for (long e0 = UE * tm.first(); e0 < n_; e0 += UE * stp) {
-                    const bool whole = e0 + 1 < n_;
-                    double v[UE], gv[UE];
-#pragma unroll
-                    for (int u = 0; u < UE; ++u) {
-                        const long e = e0 + u;
-                        gv[u] = e < n_ ? g[e] : 0.0;
-                        v[u] = gam * gv[u];
-                    }
-                    for (int c0 = 0; c0 < cnt; c0 += BLML_CCHUNK) {
-                        const int nc_ = cnt - c0 < BLML_CCHUNK ? cnt - c0 : BLML_CCHUNK;
-                        const float *Sp[BLML_CCHUNK], *Yp[BLML_CCHUNK];
-                        double a1[BLML_CCHUNK], a2[BLML_CCHUNK];
-#pragma unroll
-                        for (int i = 0; i < BLML_CCHUNK; ++i) {
-                            int sl_ = head + c0 + (i < nc_ ? i : 0);
-                            if (sl_ >= ph) sl_ -= ph;
-                            Sp[i] = hvec(9 + sl_);
-                            Yp[i] = hvec(9 + ph + sl_);
-                            a1[i] = g_sl.cp1[c0 + (i < nc_ ? i : 0)];
-                            a2[i] = g_sl.cp2[c0 + (i < nc_ ? i : 0)];
-                        }
-                        if (whole) {
-#pragma unroll
-                            for (int i = 0; i < BLML_CCHUNK; ++i)
-                                if (i < nc_) {
-                                    const f32x2 sv = *reinterpret_cast<const f32x2 *>(Sp[i] + e0);
-                                    const f32x2 yv = *reinterpret_cast<const f32x2 *>(Yp[i] + e0);
-                                    v[0] += a1[i] * (double)sv.x + a2[i] * (double)yv.x;
-                                    v[1] += a1[i] * (double)sv.y + a2[i] * (double)yv.y;
-                                }
-                        } else {
-#pragma unroll
-                            for (int i = 0; i < BLML_CCHUNK; ++i)
-                                if (i < nc_) v[0] += a1[i] * (double)Sp[i][e0] + a2[i] * (double)Yp[i][e0];
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < UE; ++u) {
-                        const long e = e0 + u;
-                        if (e < n_) {
-                            q[e] = v[u];
-                            acc += gv[u] * v[u];
-                        }
-                    }
-                }
+                // q = gam g + sum_i (p1_i s_i + p2_i y_i), slope = g . q (the additions in pair order), with the
+                // history read as in dots_history
+                const double acc = (Vmax & 1) == 0 ? dir_elems<4>(head, ph, cnt, gam, vec(ig), q)
+                                                   : dir_elems<2>(head, ph, cnt, gam, vec(ig), q);
                 slope = tm.sum1(acc, g_sl.red);
             }
             if (!(slope > 0)) {

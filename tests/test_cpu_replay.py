@@ -266,3 +266,27 @@ def test_direct_line_sums_in_bounds_through_a_fit(replay, cplx):
     la, lb = a['results']['loglik'][0], b['results']['loglik'][0]
     assert np.isfinite(lb) and np.all(np.isfinite(b['density']))
     assert abs(la - lb) < 1e-3 * abs(la)
+
+
+@pytest.mark.parametrize('case', ['complex_1mode_odd_bins', 'real_3modes_odd_bins'])
+def test_lbfgs_history_passes_in_bounds_with_ragged_lengths(replay, case):
+    """The compact L-BFGS form's history passes read W consecutive FP32 values per load (16-byte loads when the
+    slot length K B nc is even, else 8-byte ones) with a scalar tail: a complex one-mode fit of 37 x 9 bins
+    (K B nc = 666, two values past the last whole group of 4) and a real three-mode fit of 37 x 9 bins (K B nc =
+    999, odd: 8-byte loads and a one-value tail), FP32 and 15 L-BFGS iterations each: every access in bounds and
+    finite results."""
+    import blml_amd
+    rng = np.random.default_rng(21)
+    counts = _field_counts(rng, (37, 9), 20000)
+    if case.startswith('complex'):
+        modes, cplx = blml_amd.make_modes([0.35, 0.3], ['sinc', 'sinc']), True
+    else:
+        modes = (blml_amd.make_modes([0.35, 0.3], ['sinc', 'sinc'])
+                 + blml_amd.make_modes([3 / 37, 3 / 9], ['dirichlet'] * 2, periods=[37.0, 9.0])
+                 + blml_amd.make_modes([0.2, 0.2], ['slits:0.5', 'slits:0.5']))
+        cplx = False
+    params = dict(PARAMS, solver='lbfgs', max_sweeps=15, ll_tol=1e-12, tol=1e-12)
+    out = replay.run(_batch(counts, modes, cplx, np.float32, -1, params))
+    r = out['results'][0]
+    assert np.isfinite(r['loglik']) and np.all(np.isfinite(out['density'][:counts.size]))
+    assert int(r['n_sweeps']) >= 5
