@@ -353,34 +353,64 @@ struct FitSolver {
         return a[0] - n * log(a[1]);
     }
     // A line-search trial in one pass over the bins: En = E + t q, Un = U + t Sq, then lval(En, Un, Vt, Qout) (the
-    // same values in the same order: every value of bin b is this thread's own, written before it is read back).
-    __device__ double lerp_lval(double *En, double *Un, const double *E, const double *U, const double *q,
-                                const double *Sq, double t, double *Vt, double *Qout) {
+    // same values in the same order: every value of bin b is this thread's own).  Complex fits take the NC = 2
+    // body: a bin's (re, im) pair of every vector moves as one 16-byte access (every state slot of a complex fit
+    // starts 16-byte aligned: its length K B 2 is even and the pool 256-byte aligned).
+    template <int NC>
+    __device__ __forceinline__ double lerp_lval_nc(double *En, double *Un, const double *E, const double *U,
+                                                   const double *q, const double *Sq, double t, double *Vt,
+                                                   double *Qout) {
+        if (NC == 2) {
+            En = (double *)__builtin_assume_aligned(En, 16);
+            Un = (double *)__builtin_assume_aligned(Un, 16);
+            Vt = (double *)__builtin_assume_aligned(Vt, 16);
+            E = (const double *)__builtin_assume_aligned(E, 16);
+            U = (const double *)__builtin_assume_aligned(U, 16);
+            q = (const double *)__builtin_assume_aligned(q, 16);
+            Sq = (const double *)__builtin_assume_aligned(Sq, 16);
+        }
         double a[2] = {0.0, 0.0};
         for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
+            double F = 0.0;
             for (int k = 0; k < K; ++k) {
-                const long i = ((long)k * B + b) * nc;
-                for (int c = 0; c < nc; ++c) {
-                    En[i + c] = E[i + c] + t * q[i + c];
-                    Un[i + c] = U[i + c] + t * Sq[i + c];
+                const long i = ((long)k * B + b) * NC;
+                double en[NC], un[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    en[c] = E[i + c] + t * q[i + c];
+                    un[c] = U[i + c] + t * Sq[i + c];
                 }
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    En[i + c] = en[c];
+                    Un[i + c] = un[c];
+                }
+                F += NC == 2 ? un[0] * un[0] + un[1] * un[1] : un[0] * un[0];
             }
             const bool o = occ(b);
-            const double F = Fb(Un, b);
             a[0] += w[b] * log(o ? F : 1.0);
             const double fac = o ? w[b] / F : 0.0;
             for (int k = 0; k < K; ++k) {
-                const long i = ((long)k * B + b) * nc;
-                for (int c = 0; c < nc; ++c) {
+                const long i = ((long)k * B + b) * NC;
+                double vt[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
                     a[1] += En[i + c] * Un[i + c];
-                    Vt[i + c] = fac * Un[i + c];
+                    vt[c] = fac * Un[i + c];
                 }
+#pragma unroll
+                for (int c = 0; c < NC; ++c) Vt[i + c] = vt[c];
             }
         }
         tm.sum<2>(a, g_sl.red);
         *Qout = a[1];
         if (!(a[1] > 0)) return -INFINITY;
         return a[0] - n * log(a[1]);
+    }
+    __device__ double lerp_lval(double *En, double *Un, const double *E, const double *U, const double *q,
+                                const double *Sq, double t, double *Vt, double *Qout) {
+        return nc == 2 ? lerp_lval_nc<2>(En, Un, E, U, q, Sq, t, Vt, Qout)
+                       : lerp_lval_nc<1>(En, Un, E, U, q, Sq, t, Vt, Qout);
     }
 
     // g = 2 (S Vt - n U / Q) on occupied bins (blml_binned.py:285-288); SVt: scratch
