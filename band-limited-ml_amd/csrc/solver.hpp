@@ -429,27 +429,43 @@ struct FitSolver {
 
     // lgrad_from(U, Q, Vt, SVt, g), and resid(E, U, Q > 0 ? n / sqrt(Q) : 1) of the same point summed in its pass
     // over the bins (resid()'s values; a maximum does not depend on the order, so the residual is the same to the bit)
-    __device__ double lgrad_resid(const double *E, const double *U, double Q, const double *Vt, double *SVt,
-                                  double *g) {
-        Sall(Vt, SVt);
+    // Complex fits take the NC = 2 body: a bin's (re, im) pairs move as 16-byte accesses (the slots of a complex
+    // fit start 16-byte aligned), with the same values in the same order.
+    template <int NC>
+    __device__ __forceinline__ double lgrad_resid_nc(const double *E, const double *U, double Q, double *SVt,
+                                                     double *g) {
+        if (NC == 2) {
+            E = (const double *)__builtin_assume_aligned(E, 16);
+            U = (const double *)__builtin_assume_aligned(U, 16);
+            SVt = (double *)__builtin_assume_aligned(SVt, 16);
+            g = (double *)__builtin_assume_aligned(g, 16);
+        }
         const double nq = n / Q;
         const double sc = Q > 0 ? n / sqrt(Q) : 1.0;
         double r = 0.0;
         for (int b = (int)tm.first(); b < B; b += (int)tm.stride()) {
             const bool o = occ(b);
             for (int k = 0; k < K; ++k) {
-                const long i = ((long)k * B + b) * nc;
-                for (int q = 0; q < nc; ++q) g[i + q] = o ? 2.0 * (SVt[i + q] - nq * U[i + q]) : 0.0;
+                const long i = ((long)k * B + b) * NC;
+                double gv[NC];
+#pragma unroll
+                for (int q = 0; q < NC; ++q) gv[q] = o ? 2.0 * (SVt[i + q] - nq * U[i + q]) : 0.0;
+#pragma unroll
+                for (int q = 0; q < NC; ++q) g[i + q] = gv[q];
             }
             if (!o) continue;
             double nrm2 = 0.0;
-            for (int k = 0; k < K; ++k) nrm2 += absq(E, k, b);
+            for (int k = 0; k < K; ++k) {
+                const long i = ((long)k * B + b) * NC;
+                nrm2 += NC == 2 ? E[i] * E[i] + E[i + 1] * E[i + 1] : E[i] * E[i];
+            }
             nrm2 *= sc * sc;
             const double fac = nrm2 / cb(b);
             double acc = 0.0;
             for (int k = 0; k < K; ++k) {
-                const long i = ((long)k * B + b) * nc;
-                for (int q = 0; q < nc; ++q) {
+                const long i = ((long)k * B + b) * NC;
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
                     const double v = sc * U[i + q] * fac - sc * E[i + q];
                     acc += v * v;
                 }
@@ -457,6 +473,11 @@ struct FitSolver {
             r = fmax(r, sqrt(acc) / sqrt(fmax(nrm2, 1e-300)));
         }
         return tm.max(r, g_sl.red);
+    }
+    __device__ double lgrad_resid(const double *E, const double *U, double Q, const double *Vt, double *SVt,
+                                  double *g) {
+        Sall(Vt, SVt);
+        return nc == 2 ? lgrad_resid_nc<2>(E, U, Q, SVt, g) : lgrad_resid_nc<1>(E, U, Q, SVt, g);
     }
 
     // ------------------------------------------------------------------ histories
